@@ -1,0 +1,1616 @@
+// ============================================================================
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// A CPU, FP64, single-threaded restatement of the Flow123d 1.7 algorithm for
+// the steady mixed-hybrid Darcy solve and the explicit upwind transport step.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker (or as the timed CPU baseline).
+// The product library never links or calls it.
+//
+// Every routine cites the reference file:line it follows (paths relative to
+// the Flow123d source tree, src/...).  The restatement follows the reference
+// step by step, including its two-stage sparse Schur elimination (it does NOT
+// use the per-element closed form the GPU path uses), so that the GPU path is
+// checked against an independent derivation.
+//
+// Parity pins (see tests/): simplest_cube topology counts
+// (test_units/mesh/mesh_test.cpp:53-81), tests/01 raw_output.txt goldens,
+// tests/06 analytic coupling values, the MH-RT0 linear patch test, local
+// matrix closed forms, dense brute-force solves of the full MH system, and
+// tests/13 transport/mass-balance goldens.
+//
+// Not pinned bit-for-bit: the summation order inside Armadillo 3.2.4
+// (arma::norm/dot) and PETSc 3.4 (MatMatMult, KSP), neither of which is
+// vendored with the reference; agreement there is to round-off.
+// ============================================================================
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace orc {
+
+static thread_local std::string g_err;
+
+// ---------------------------------------------------------------------------
+// Regions: src/mesh/region.cc:105-171 (boundary by '.' prefix, separate
+// boundary / bulk counters, "region_<id>" for unnamed, IMPLICIT BOUNDARY
+// id = undefined-2 added on first use, region.cc:85-86).
+// ---------------------------------------------------------------------------
+struct Region {
+    int id;
+    std::string label;
+    bool boundary;
+    int sub_idx;  // boundary_idx or bulk_idx
+};
+
+struct RegionDB {
+    std::vector<Region> regs;
+    std::map<int, int> by_id;
+    int n_boundary = 0, n_bulk = 0;
+    int add(int id, const std::string &label, bool boundary) {
+        auto it = by_id.find(id);
+        if (it != by_id.end()) return it->second;
+        Region r{id, label, boundary, boundary ? n_boundary++ : n_bulk++};
+        regs.push_back(r);
+        by_id[id] = (int)regs.size() - 1;
+        return (int)regs.size() - 1;
+    }
+    int add_named(int id, const std::string &label) {
+        bool b = !label.empty() && label[0] == '.';
+        return add(id, label, b);
+    }
+    int add_unnamed(int id) {
+        auto it = by_id.find(id);
+        if (it != by_id.end()) return it->second;
+        return add(id, "region_" + std::to_string(id), false);
+    }
+    int implicit_boundary() { return add(-3, "IMPLICIT BOUNDARY", true); }
+};
+
+// ---------------------------------------------------------------------------
+// Mesh: src/mesh/mesh.h, elements.h, edges.h, boundaries.h, neighbours.h
+// ---------------------------------------------------------------------------
+struct Element {
+    int id = 0, dim = 0, region = -1;
+    int node[4] = {-1, -1, -1, -1};
+    int edge_idx[4] = {-1, -1, -1, -1};
+    int boundary_idx[4] = {-1, -1, -1, -1};
+    std::vector<int> neigh_vb;  // indices into Mesh::vb
+    int n_nodes() const { return dim + 1; }
+    int n_sides() const { return dim + 1; }
+};
+struct Edge {
+    std::vector<std::pair<int, int>> sides;  // (element index, local side)
+};
+struct Boundary {
+    int edge_idx, bc_ele_idx;
+};
+struct Neighbour {
+    int element;  // lower-dimensional element
+    int edge_idx;
+};
+
+// RefElement<dim>::side_nodes, src/mesh/ref_element.cc:51-74
+static const int SIDE_NODES_1[2][1] = {{0}, {1}};
+static const int SIDE_NODES_2[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+static const int SIDE_NODES_3[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+static inline int side_node(int dim, int s, int k) {
+    if (dim == 1) return SIDE_NODES_1[s][k];
+    if (dim == 2) return SIDE_NODES_2[s][k];
+    return SIDE_NODES_3[s][k];
+}
+
+struct Mesh {
+    std::vector<double> X;  // 3 * n_nodes
+    std::vector<int> node_ids;
+    std::unordered_map<int, int> node_idx;
+    std::vector<Element> element, bc_elements;
+    int n_file_bc = 0;
+    std::vector<Edge> edges;
+    std::vector<Boundary> boundary;
+    std::vector<Neighbour> vb;
+    RegionDB rdb;
+    std::vector<std::vector<int>> node_elements;
+    std::vector<int> side_offset;  // MH_DofHandler: first global side of element
+    int n_sides = 0;
+
+    const double *P(int node) const { return &X[3 * node]; }
+};
+
+// ----- small vector helpers (system/math_fce.cc:46-98) -----
+static inline void sub3(const double *a, const double *b, double *r) {
+    r[0] = a[0] - b[0]; r[1] = a[1] - b[1]; r[2] = a[2] - b[2];
+}
+static inline void cross3(const double *u, const double *v, double *x) {
+    x[0] = u[1] * v[2] - u[2] * v[1];
+    x[1] = u[2] * v[0] - u[0] * v[2];
+    x[2] = u[0] * v[1] - u[1] * v[0];
+}
+static inline double dot3(const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+static inline double norm3(const double *u) { return std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]); }
+static inline void normalize3(double *u) {
+    double l = norm3(u);
+    u[0] /= l; u[1] /= l; u[2] /= l;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry: Element::measure/centre src/mesh/elements.cc:108-151;
+// Side::measure/normal/centre src/mesh/sides.cc:42-146.
+// ---------------------------------------------------------------------------
+static double elem_measure(const Mesh &m, const Element &e) {
+    switch (e.dim) {
+        case 0: return 1.0;
+        case 1: { double d[3]; sub3(m.P(e.node[1]), m.P(e.node[0]), d); return norm3(d); }
+        case 2: {
+            double a[3], b[3], c[3];
+            sub3(m.P(e.node[1]), m.P(e.node[0]), a);
+            sub3(m.P(e.node[2]), m.P(e.node[0]), b);
+            cross3(a, b, c);
+            return norm3(c) / 2.0;
+        }
+        case 3: {
+            double a[3], b[3], c[3], d[3];
+            sub3(m.P(e.node[1]), m.P(e.node[0]), a);
+            sub3(m.P(e.node[2]), m.P(e.node[0]), b);
+            cross3(a, b, c);
+            sub3(m.P(e.node[3]), m.P(e.node[0]), d);
+            return std::fabs(dot3(c, d)) / 6.0;
+        }
+    }
+    return 1.0;
+}
+static void elem_centre(const Mesh &m, const Element &e, double *c) {
+    c[0] = c[1] = c[2] = 0.0;
+    for (int i = 0; i < e.n_nodes(); i++) {
+        const double *p = m.P(e.node[i]);
+        c[0] += p[0]; c[1] += p[1]; c[2] += p[2];
+    }
+    double n = (double)e.n_nodes();
+    c[0] /= n; c[1] /= n; c[2] /= n;
+}
+static inline int snode(const Mesh &m, const Element &e, int s, int k) {
+    (void)m;
+    return e.node[side_node(e.dim, s, k)];
+}
+static double side_measure(const Mesh &m, const Element &e, int s) {
+    int sd = e.dim - 1;
+    if (sd == 0) return 1.0;
+    if (sd == 1) { double d[3]; sub3(m.P(snode(m, e, s, 1)), m.P(snode(m, e, s, 0)), d); return norm3(d); }
+    double a[3], b[3], c[3];
+    sub3(m.P(snode(m, e, s, 1)), m.P(snode(m, e, s, 0)), a);
+    sub3(m.P(snode(m, e, s, 2)), m.P(snode(m, e, s, 0)), b);
+    cross3(a, b, c);
+    return 0.5 * norm3(c);
+}
+static void side_centre(const Mesh &m, const Element &e, int s, double *c) {
+    c[0] = c[1] = c[2] = 0.0;
+    int nn = e.dim;  // side has dim nodes
+    for (int i = 0; i < nn; i++) {
+        const double *p = m.P(snode(m, e, s, i));
+        c[0] += p[0]; c[1] += p[1]; c[2] += p[2];
+    }
+    c[0] /= (double)nn; c[1] /= (double)nn; c[2] /= (double)nn;
+}
+static void side_normal(const Mesh &m, const Element &e, int s, double *nrm) {
+    int sd = e.dim - 1;
+    double ec[3];
+    elem_centre(m, e, ec);
+    if (sd == 0) {  // sides.cc:77-89
+        sub3(m.P(e.node[1]), m.P(e.node[0]), nrm);
+        double l = norm3(nrm);
+        nrm[0] /= l; nrm[1] /= l; nrm[2] /= l;
+        if (snode(m, e, s, 0) == e.node[0]) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2]; }
+        return;
+    }
+    if (sd == 1) {  // sides.cc:94-111
+        double a[3], b[3], en[3];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), a);
+        sub3(m.P(e.node[2]), m.P(e.node[0]), b);
+        cross3(a, b, en);
+        double l = norm3(en);
+        en[0] /= l; en[1] /= l; en[2] /= l;
+        double t[3];
+        sub3(m.P(snode(m, e, s, 1)), m.P(snode(m, e, s, 0)), t);
+        cross3(t, en, nrm);
+        l = norm3(nrm);
+        nrm[0] /= l; nrm[1] /= l; nrm[2] /= l;
+        double d[3];
+        sub3(ec, m.P(snode(m, e, s, 0)), d);
+        if (dot3(nrm, d) > 0.0) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2]; }
+        return;
+    }
+    // sides.cc:116-131
+    double a[3], b[3];
+    sub3(m.P(snode(m, e, s, 1)), m.P(snode(m, e, s, 0)), a);
+    sub3(m.P(snode(m, e, s, 2)), m.P(snode(m, e, s, 0)), b);
+    cross3(a, b, nrm);
+    double l = norm3(nrm);
+    nrm[0] /= l; nrm[1] /= l; nrm[2] /= l;
+    double d[3];
+    sub3(ec, m.P(snode(m, e, s, 0)), d);
+    if (dot3(nrm, d) > 0.0) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2]; }
+}
+
+// ---------------------------------------------------------------------------
+// GMSH 2.x reader: src/mesh/msh_gmshreader.cc:76-246
+// ---------------------------------------------------------------------------
+static bool read_gmsh(const char *path, Mesh &m) {
+    std::ifstream in(path);
+    if (!in) { g_err = std::string("cannot open ") + path; return false; }
+    std::string line;
+    // $PhysicalNames (optional, before $Nodes): msh_gmshreader.cc:224-246
+    bool have_nodes = false;
+    while (std::getline(in, line)) {
+        if (line.compare(0, 14, "$PhysicalNames") == 0) {
+            std::getline(in, line);
+            int n = std::stoi(line);
+            for (int i = 0; i < n; i++) {
+                std::getline(in, line);
+                std::istringstream ss(line);
+                int dim, id;
+                ss >> dim >> id;
+                std::string rest;
+                std::getline(ss, rest);
+                size_t a = rest.find('"'), b = rest.rfind('"');
+                std::string name = (a != std::string::npos && b > a) ? rest.substr(a + 1, b - a - 1) : rest;
+                m.rdb.add_named(id, name);
+            }
+        } else if (line.compare(0, 6, "$Nodes") == 0) {
+            have_nodes = true;
+            break;
+        }
+    }
+    if (!have_nodes) { g_err = "missing $Nodes"; return false; }
+    // $Nodes: msh_gmshreader.cc:89-118
+    std::getline(in, line);
+    int nn = std::stoi(line);
+    m.X.resize(3 * (size_t)nn);
+    m.node_ids.resize(nn);
+    for (int i = 0; i < nn; i++) {
+        int id;
+        double x, y, z;
+        if (!(in >> id >> x >> y >> z)) { g_err = "bad node line"; return false; }
+        m.node_ids[i] = id;
+        m.node_idx[id] = i;
+        m.X[3 * i] = x; m.X[3 * i + 1] = y; m.X[3 * i + 2] = z;
+    }
+    // $Elements: msh_gmshreader.cc:121-217
+    while (std::getline(in, line))
+        if (line.compare(0, 9, "$Elements") == 0) break;
+    std::getline(in, line);
+    int ne = std::stoi(line);
+    for (int i = 0; i < ne; i++) {
+        int id, type, ntags;
+        in >> id >> type >> ntags;
+        int dim;
+        switch (type) {
+            case 1: dim = 1; break;
+            case 2: dim = 2; break;
+            case 4: dim = 3; break;
+            case 15: dim = 0; break;
+            default: g_err = "unsupported element type " + std::to_string(type); return false;
+        }
+        std::vector<int> tags(ntags);
+        for (int t = 0; t < ntags; t++) in >> tags[t];
+        int region_id = tags.size() > 0 ? tags[0] : 0;
+        int ridx = m.rdb.add_unnamed(region_id);
+        Element e;
+        e.id = id;
+        e.dim = dim;
+        e.region = ridx;
+        for (int k = 0; k < dim + 1; k++) {
+            int nid;
+            in >> nid;
+            auto it = m.node_idx.find(nid);
+            if (it == m.node_idx.end()) { g_err = "unknown node id"; return false; }
+            e.node[k] = it->second;
+        }
+        if (m.rdb.regs[ridx].boundary) m.bc_elements.push_back(e);
+        else if (dim > 0) m.element.push_back(e);
+    }
+    m.n_file_bc = (int)m.bc_elements.size();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Topology: Mesh::make_neighbours_and_edges src/mesh/mesh.cc:343-500,
+// create_node_element_lists :264-274, intersect_element_lists :277-302,
+// find_lower_dim_element :305-323, same_sides :326-332,
+// element_to_neigh_vb :660-685.
+// ---------------------------------------------------------------------------
+static void intersect(const Mesh &m, const std::vector<int> &nodes, std::vector<int> &out) {
+    out.clear();
+    if (nodes.empty()) return;
+    out = m.node_elements[nodes[0]];
+    for (size_t k = 1; k < nodes.size(); k++) {
+        const std::vector<int> &b = m.node_elements[nodes[k]];
+        std::vector<int> tmp;
+        std::set_intersection(out.begin(), out.end(), b.begin(), b.end(), std::back_inserter(tmp));
+        out.swap(tmp);
+    }
+}
+static bool find_lower_dim(const Mesh &m, std::vector<int> &list, int dim, int &lower) {
+    bool is_nb = false;
+    std::vector<int> keep;
+    for (int e : list) {
+        if (m.element[e].dim == dim) keep.push_back(e);
+        else if (m.element[e].dim == dim - 1) {
+            if (is_nb) { g_err = "too many matching lower-dim elements"; }
+            is_nb = true;
+            lower = e;
+        }
+    }
+    list.swap(keep);
+    return is_nb;
+}
+static bool same_side(const Mesh &m, const Element &e, int s, const std::vector<int> &nodes) {
+    for (int k = 0; k < e.dim; k++)
+        if (std::find(nodes.begin(), nodes.end(), snode(m, e, s, k)) == nodes.end()) return false;
+    return true;
+}
+
+static bool setup_topology(Mesh &m) {
+    int nn = (int)m.node_ids.size();
+    m.node_elements.assign(nn, {});
+    for (int e = 0; e < (int)m.element.size(); e++)
+        for (int k = 0; k < m.element[e].n_nodes(); k++) m.node_elements[m.element[e].node[k]].push_back(e);
+    std::vector<int> side_nodes, isect;
+    // (1) boundary elements from the file, in file order: mesh.cc:358-409
+    for (int bi = 0; bi < (int)m.bc_elements.size(); bi++) {
+        const Element &bc = m.bc_elements[bi];
+        side_nodes.assign(bc.node, bc.node + bc.n_nodes());
+        intersect(m, side_nodes, isect);
+        int lower = -1;
+        bool nb = find_lower_dim(m, isect, bc.dim + 1, lower);
+        if (nb) { g_err = "boundary element matches a lower-dimensional bulk element"; return false; }
+        if (isect.empty()) continue;  // lonely boundary element (warning in the reference)
+        int ei = (int)m.edges.size();
+        m.edges.push_back(Edge());
+        int bdr = (int)m.boundary.size();
+        m.boundary.push_back(Boundary{ei, bi});
+        for (int el : isect) {
+            Element &E = m.element[el];
+            for (int s = 0; s < E.n_sides(); s++) {
+                if (same_side(m, E, s, side_nodes)) {
+                    m.edges[ei].sides.push_back({el, s});
+                    E.edge_idx[s] = ei;
+                    E.boundary_idx[s] = bdr;
+                    break;
+                }
+            }
+        }
+    }
+    // (2) all element sides in element order: mesh.cc:412-497
+    for (int e = 0; e < (int)m.element.size(); e++) {
+        for (int s = 0; s < m.element[e].n_sides(); s++) {
+            if (m.element[e].edge_idx[s] != -1) continue;
+            const Element &Ee = m.element[e];
+            side_nodes.clear();
+            for (int k = 0; k < Ee.dim; k++) side_nodes.push_back(snode(m, Ee, s, k));
+            intersect(m, side_nodes, isect);
+            int lower = -1;
+            bool nb = find_lower_dim(m, isect, Ee.dim, lower);
+            int ei = -1;
+            if (!nb) {
+                ei = (int)m.edges.size();
+                m.edges.push_back(Edge());
+                if (isect.size() == 1) {
+                    m.edges[ei].sides.push_back({e, s});
+                    m.element[e].edge_idx[s] = ei;
+                    int bdr = (int)m.boundary.size();
+                    m.element[e].boundary_idx[s] = bdr;
+                    Element bc;
+                    bc.id = -bdr;
+                    bc.dim = Ee.dim - 1;
+                    bc.region = m.rdb.implicit_boundary();
+                    for (size_t k = 0; k < side_nodes.size(); k++) bc.node[k] = side_nodes[k];
+                    m.bc_elements.push_back(bc);
+                    m.boundary.push_back(Boundary{ei, (int)m.bc_elements.size() - 1});
+                    continue;
+                }
+            }
+            for (int el : isect) {
+                Element &E = m.element[el];
+                for (int es = 0; es < E.n_sides(); es++) {
+                    if (E.edge_idx[es] != -1) continue;
+                    if (same_side(m, E, es, side_nodes)) {
+                        if (nb) {
+                            int ne = (int)m.edges.size();
+                            m.edges.push_back(Edge());
+                            m.edges[ne].sides.push_back({el, es});
+                            E.edge_idx[es] = ne;
+                            m.vb.push_back(Neighbour{lower, ne});
+                        } else {
+                            m.edges[ei].sides.push_back({el, es});
+                            E.edge_idx[es] = ei;
+                        }
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    // element_to_neigh_vb: mesh.cc:660-685
+    for (int n = 0; n < (int)m.vb.size(); n++) m.element[m.vb[n].element].neigh_vb.push_back(n);
+    // MH_DofHandler::reinit: src/flow/mh_dofhandler.cc:12-21
+    m.side_offset.resize(m.element.size() + 1);
+    int off = 0;
+    for (size_t e = 0; e < m.element.size(); e++) {
+        m.side_offset[e] = off;
+        off += m.element[e].n_sides();
+    }
+    m.side_offset[m.element.size()] = off;
+    m.n_sides = off;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Small inverses: src/system/math_fce.cc:124-230 (SUBDET2 math_fce.h:69,
+// NUM_ZERO math_fce.h:51).
+// ---------------------------------------------------------------------------
+static const double NUM_ZERO = std::numeric_limits<double>::min() / std::numeric_limits<double>::epsilon();
+#define A_(i, j) a[(i) * n + (j)]
+#define SD2(i, j, k, l) (A_(i, k) * A_(j, l) - A_(i, l) * A_(j, k))
+static double matrix_inverse(const double *a, double *b, int n) {
+#define B_(i, j) b[(i) * n + (j)]
+    if (n == 1) { b[0] = 1 / a[0]; return a[0]; }
+    if (n == 2) {
+        double det = SD2(0, 1, 0, 1);
+        if (std::fabs(det) < NUM_ZERO) return det;
+        B_(0, 0) = A_(1, 1) / det; B_(1, 0) = -A_(1, 0) / det;
+        B_(0, 1) = -A_(0, 1) / det; B_(1, 1) = A_(0, 0) / det;
+        return det;
+    }
+    if (n == 3) {
+        B_(0, 0) = SD2(1, 2, 1, 2); B_(1, 0) = -SD2(1, 2, 0, 2); B_(2, 0) = SD2(1, 2, 0, 1);
+        double det = A_(0, 0) * B_(0, 0) + A_(0, 1) * B_(1, 0) + A_(0, 2) * B_(2, 0);
+        if (std::fabs(det) < NUM_ZERO) return det;
+        B_(0, 0) /= det; B_(1, 0) /= det; B_(2, 0) /= det;
+        B_(0, 1) = -SD2(0, 2, 1, 2) / det; B_(1, 1) = SD2(0, 2, 0, 2) / det; B_(2, 1) = -SD2(0, 2, 0, 1) / det;
+        B_(0, 2) = SD2(0, 1, 1, 2) / det; B_(1, 2) = -SD2(0, 1, 0, 2) / det; B_(2, 2) = SD2(0, 1, 0, 1) / det;
+        return det;
+    }
+    double u[6], l[6];
+    u[0] = SD2(0, 1, 0, 1); u[1] = SD2(0, 1, 0, 2); u[2] = SD2(0, 1, 0, 3);
+    u[3] = SD2(0, 1, 1, 2); u[4] = SD2(0, 1, 1, 3); u[5] = SD2(0, 1, 2, 3);
+    l[0] = SD2(2, 3, 0, 1); l[1] = SD2(2, 3, 0, 2); l[2] = SD2(2, 3, 0, 3);
+    l[3] = SD2(2, 3, 1, 2); l[4] = SD2(2, 3, 1, 3); l[5] = SD2(2, 3, 2, 3);
+    B_(0, 0) = +A_(1, 1) * l[5] - A_(1, 2) * l[4] + A_(1, 3) * l[3];
+    B_(1, 0) = -A_(1, 0) * l[5] + A_(1, 2) * l[2] - A_(1, 3) * l[1];
+    B_(2, 0) = +A_(1, 0) * l[4] - A_(1, 1) * l[2] + A_(1, 3) * l[0];
+    B_(3, 0) = -A_(1, 0) * l[3] + A_(1, 1) * l[1] - A_(1, 2) * l[0];
+    double det = A_(0, 0) * B_(0, 0) + A_(0, 1) * B_(1, 0) + A_(0, 2) * B_(2, 0) + A_(0, 3) * B_(3, 0);
+    if (std::fabs(det) < NUM_ZERO) return det;
+    B_(0, 0) /= det; B_(1, 0) /= det; B_(2, 0) /= det; B_(3, 0) /= det;
+    B_(0, 1) = (-A_(0, 1) * l[5] + A_(0, 2) * l[4] - A_(0, 3) * l[3]) / det;
+    B_(1, 1) = (+A_(0, 0) * l[5] - A_(0, 2) * l[2] + A_(0, 3) * l[1]) / det;
+    B_(2, 1) = (-A_(0, 0) * l[4] + A_(0, 1) * l[2] - A_(0, 3) * l[0]) / det;
+    B_(3, 1) = (+A_(0, 0) * l[3] - A_(0, 1) * l[1] + A_(0, 2) * l[0]) / det;
+    B_(0, 2) = (+A_(3, 1) * u[5] - A_(3, 2) * u[4] + A_(3, 3) * u[3]) / det;
+    B_(1, 2) = (-A_(3, 0) * u[5] + A_(3, 2) * u[2] - A_(3, 3) * u[1]) / det;
+    B_(2, 2) = (+A_(3, 0) * u[4] - A_(3, 1) * u[2] + A_(3, 3) * u[0]) / det;
+    B_(3, 2) = (-A_(3, 0) * u[3] + A_(3, 1) * u[1] - A_(3, 2) * u[0]) / det;
+    B_(0, 3) = (-A_(2, 1) * u[5] + A_(2, 2) * u[4] - A_(2, 3) * u[3]) / det;
+    B_(1, 3) = (+A_(2, 0) * u[5] - A_(2, 2) * u[2] + A_(2, 3) * u[1]) / det;
+    B_(2, 3) = (-A_(2, 0) * u[4] + A_(2, 1) * u[2] - A_(2, 3) * u[0]) / det;
+    B_(3, 3) = (+A_(2, 0) * u[3] - A_(2, 1) * u[1] + A_(2, 2) * u[0]) / det;
+    return det;
+#undef B_
+}
+#undef SD2
+#undef A_
+
+// inverse of a general 3x3 (the anisotropy tensor); cofactor form.
+static void inv3x3(const double *a, double *r) {
+    double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    r[0] = c00 / det; r[3] = c01 / det; r[6] = c02 / det;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+// ---------------------------------------------------------------------------
+// RT0 local matrices: MHFEValues src/flow/mh_fe_values.cc:34-514.
+// R = inverse anisotropy (3x3 row-major); identity when aniso == nullptr.
+// Returns det of the local matrix (mh_fe_values.cc:54).
+// ---------------------------------------------------------------------------
+static double local_matrix(const Mesh &m, const Element &e, double K, double cs, const double *aniso,
+                           double *loc, double *inv) {
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (aniso) inv3x3(aniso, R);
+    double scale = 1 / K / cs;  // mh_fe_values.cc:38
+    const int d = e.dim;
+    if (d == 1) {  // local_matrix_line :129-153
+        double t[3];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), t);
+        double l = norm3(t);
+        t[0] /= l; t[1] /= l; t[2] /= l;
+        double Rt[3] = {R[0] * t[0] + R[1] * t[1] + R[2] * t[2], R[3] * t[0] + R[4] * t[1] + R[5] * t[2],
+                        R[6] * t[0] + R[7] * t[1] + R[8] * t[2]};
+        double val = scale * dot3(t, Rt) * elem_measure(m, e) / 3.0;
+        loc[0] = val; loc[3] = val; loc[1] = -val / 2.0; loc[2] = -val / 2.0;
+    } else if (d == 2) {  // local_matrix_triangle :159-211
+        double ex[3], ac[3], ez[3], ey[3];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), ex);
+        normalize3(ex);
+        sub3(m.P(e.node[2]), m.P(e.node[0]), ac);
+        cross3(ex, ac, ez);
+        normalize3(ez);
+        cross3(ez, ex, ey);
+        normalize3(ey);
+        // resistance_tensor = r^T (R r), r = [ex ey] (3x2)
+        double Rr[3][2];
+        for (int i = 0; i < 3; i++) {
+            Rr[i][0] = R[3 * i] * ex[0] + R[3 * i + 1] * ex[1] + R[3 * i + 2] * ex[2];
+            Rr[i][1] = R[3 * i] * ey[0] + R[3 * i + 1] * ey[1] + R[3 * i + 2] * ey[2];
+        }
+        double rt[2][2];
+        for (int j = 0; j < 2; j++) {
+            rt[0][j] = ex[0] * Rr[0][j] + ex[1] * Rr[1][j] + ex[2] * Rr[2][j];
+            rt[1][j] = ey[0] * Rr[0][j] + ey[1] * Rr[1][j] + ey[2] * Rr[2][j];
+        }
+        // node_coordinates_triangle :218-247
+        double u[3], v[3], n[3], tt[3], b[3], nod[3][2];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), u);
+        sub3(m.P(e.node[2]), m.P(e.node[0]), v);
+        cross3(u, v, n);
+        tt[0] = u[0]; tt[1] = u[1]; tt[2] = u[2];
+        normalize3(tt);
+        normalize3(n);
+        cross3(n, u, b);
+        normalize3(b);
+        nod[0][0] = 0.0; nod[0][1] = 0.0;
+        nod[1][0] = dot3(tt, u); nod[1][1] = dot3(b, u);
+        nod[2][0] = dot3(tt, v); nod[2][1] = dot3(b, v);
+        // side_midpoint_triangle :253-261
+        double mid[3][2];
+        mid[0][0] = (nod[0][0] + nod[1][0]) / 2.0; mid[0][1] = (nod[0][1] + nod[1][1]) / 2.0;
+        mid[1][0] = (nod[1][0] + nod[2][0]) / 2.0; mid[1][1] = (nod[1][1] + nod[2][1]) / 2.0;
+        mid[2][0] = (nod[2][0] + nod[0][0]) / 2.0; mid[2][1] = (nod[2][1] + nod[0][1]) / 2.0;
+        // basis_functions_triangle :268-295
+        double al[3], be[3], ga[3];
+        for (int i = 0; i < 3; i++) {
+            int i0 = i, i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            double x0 = nod[i0][0], y0 = nod[i0][1], x1 = nod[i1][0], y1 = nod[i1][1], x2 = nod[i2][0], y2 = nod[i2][1];
+            al[i] = x2; be[i] = y2;
+            ga[i] = 1.0 / std::fabs((x0 - x2) * (y1 - y0) + (y0 - y2) * (x0 - x1));
+        }
+        double meas = elem_measure(m, e);
+        for (int i = 0; i < 3; i++)
+            for (int j = i; j < 3; j++) {
+                // calc_polynom_triangle :299-319
+                double poly[6];
+                poly[0] = rt[0][0] * al[i] * al[j] + rt[0][1] * be[i] * al[j] + rt[1][0] * al[i] * be[j] +
+                          rt[1][1] * be[i] * be[j];
+                poly[1] = (rt[0][0] * al[i] + rt[0][0] * al[j] + rt[0][1] * be[i] + rt[1][0] * be[j]) * -1.0;
+                poly[2] = (rt[1][1] * be[i] + rt[1][1] * be[j] + rt[1][0] * al[i] + rt[0][1] * al[j]) * -1.0;
+                poly[3] = rt[0][0];
+                poly[4] = rt[0][1] + rt[1][0];
+                poly[5] = rt[1][1];
+                double p[3];
+                for (int q = 0; q < 3; q++)  // polynom_value_triangle :323-334
+                    p[q] = poly[0] + poly[1] * mid[q][0] + poly[2] * mid[q][1] + poly[3] * mid[q][0] * mid[q][0] +
+                           poly[4] * mid[q][0] * mid[q][1] + poly[5] * mid[q][1] * mid[q][1];
+                loc[3 * i + j] = scale * ga[i] * ga[j] * meas * (p[0] + p[1] + p[2]) / 3.0;
+                loc[3 * j + i] = loc[3 * i + j];
+            }
+    } else {  // local_matrix_tetrahedron :339-375
+        double al[4], be[4], ga[4], de[4];
+        for (int li = 0; li < 4; li++) {  // basis_functions_tetrahedron :379-397
+            const double *p = m.P(e.node[li]);
+            al[li] = p[0]; be[li] = p[1]; ga[li] = p[2];
+            double nrm[3], sc[3];
+            side_normal(m, e, li, nrm);
+            side_centre(m, e, li, sc);
+            de[li] = 1.0 / (side_measure(m, e, li) *
+                            (nrm[0] * sc[0] + nrm[1] * sc[1] + nrm[2] * sc[2] - nrm[0] * al[li] - nrm[1] * be[li] -
+                             nrm[2] * ga[li]));
+        }
+        double v = elem_measure(m, e), t[3];
+        elem_centre(m, e, t);
+        const double *n0 = m.P(e.node[0]), *n1 = m.P(e.node[1]), *n2 = m.P(e.node[2]), *n3 = m.P(e.node[3]);
+        auto mom = [&](int a, int b) {  // second moments used by polynom_integral_tetrahedron :452-514
+            return t[a] * t[b] + 1.0 / 20.0 *
+                                     ((n0[a] - t[a]) * (n0[b] - t[b]) + (n1[a] - t[a]) * (n1[b] - t[b]) +
+                                      (n2[a] - t[a]) * (n2[b] - t[b]) + (n3[a] - t[a]) * (n3[b] - t[b]));
+        };
+        double Mxy = mom(0, 1), Mxz = mom(0, 2), Myz = mom(1, 2), Mxx = mom(0, 0), Myy = mom(1, 1), Mzz = mom(2, 2);
+        const double *a = R;
+        auto A = [&](int i, int j) { return a[3 * i + j]; };
+        for (int i = 0; i < 4; i++)
+            for (int j = i; j < 4; j++) {
+                double poly[10];  // calc_polynom_tetrahedron :401-448
+                poly[0] = A(0, 0) * al[i] * al[j] + A(0, 1) * be[i] * al[j] + A(0, 2) * ga[i] * al[j] +
+                          A(1, 0) * al[i] * be[j] + A(1, 1) * be[i] * be[j] + A(1, 2) * ga[i] * be[j] +
+                          A(2, 0) * al[i] * ga[j] + A(2, 1) * be[i] * ga[j] + A(2, 2) * ga[i] * ga[j];
+                poly[1] = (A(0, 0) * al[i] + A(0, 1) * be[i] + A(0, 2) * ga[i] + A(0, 0) * al[j] + A(1, 0) * be[j] +
+                           A(2, 0) * ga[j]) * -1.0;
+                poly[2] = (A(1, 0) * al[i] + A(1, 1) * be[i] + A(1, 2) * ga[i] + A(0, 1) * al[j] + A(1, 1) * be[j] +
+                           A(2, 1) * ga[j]) * -1.0;
+                poly[3] = (A(2, 0) * al[i] + A(2, 1) * be[i] + A(2, 2) * ga[i] + A(0, 2) * al[j] + A(1, 2) * be[j] +
+                           A(2, 2) * ga[j]) * -1.0;
+                poly[4] = A(0, 1) + A(1, 0);
+                poly[5] = A(0, 2) + A(2, 0);
+                poly[6] = A(1, 2) + A(2, 1);
+                poly[7] = A(0, 0);
+                poly[8] = A(1, 1);
+                poly[9] = A(2, 2);
+                double rc = 0.0;
+                rc += poly[0] * v;
+                rc += poly[1] * v * t[0];
+                rc += poly[2] * v * t[1];
+                rc += poly[3] * v * t[2];
+                rc += poly[4] * v * Mxy;
+                rc += poly[5] * v * Mxz;
+                rc += poly[6] * v * Myz;
+                rc += poly[7] * v * Mxx;
+                rc += poly[8] * v * Myy;
+                rc += poly[9] * v * Mzz;
+                loc[4 * i + j] = scale * de[i] * de[j] * rc;
+                loc[4 * j + i] = loc[4 * i + j];
+            }
+    }
+    return matrix_inverse(loc, inv, d + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse matrix in triplet/CSR form (a stand-in for PETSc SeqAIJ, ADD mode).
+// ---------------------------------------------------------------------------
+struct CSR {
+    int nr = 0, nc = 0;
+    std::vector<int> ptr, col;
+    std::vector<double> val;
+    double get(int r, int c) const {
+        auto b = col.begin() + ptr[r], e = col.begin() + ptr[r + 1];
+        auto it = std::lower_bound(b, e, c);
+        if (it != e && *it == c) return val[it - col.begin()];
+        return 0.0;
+    }
+};
+struct Triplets {
+    std::vector<int> r, c;
+    std::vector<double> v;
+    void add(int i, int j, double x) { r.push_back(i); c.push_back(j); v.push_back(x); }
+    // combine duplicates in insertion order (ADD_VALUES semantics)
+    CSR to_csr(int nr, int nc) const {
+        CSR A;
+        A.nr = nr; A.nc = nc;
+        std::vector<size_t> idx(r.size());
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+            return r[a] != r[b] ? r[a] < r[b] : c[a] < c[b];
+        });
+        A.ptr.assign(nr + 1, 0);
+        for (size_t k = 0; k < idx.size(); k++) {
+            size_t i = idx[k];
+            if (!A.col.empty() && k > 0 && r[idx[k - 1]] == r[i] && c[idx[k - 1]] == c[i]) {
+                A.val.back() += v[i];
+            } else {
+                A.col.push_back(c[i]);
+                A.val.push_back(v[i]);
+                A.ptr[r[i] + 1]++;
+            }
+        }
+        for (int i = 0; i < nr; i++) A.ptr[i + 1] += A.ptr[i];
+        return A;
+    }
+};
+// submatrix rows [r0,r1) x cols [c0,c1) (MatGetSubMatrix)
+static CSR submat(const CSR &A, int r0, int r1, int c0, int c1) {
+    CSR B;
+    B.nr = r1 - r0; B.nc = c1 - c0;
+    B.ptr.assign(B.nr + 1, 0);
+    for (int i = r0; i < r1; i++) {
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+            if (A.col[k] >= c0 && A.col[k] < c1) { B.col.push_back(A.col[k] - c0); B.val.push_back(A.val[k]); }
+        B.ptr[i - r0 + 1] = (int)B.col.size();
+    }
+    return B;
+}
+// C = A*B, row-by-row with a dense accumulator in A's column order.
+static CSR spgemm(const CSR &A, const CSR &B) {
+    CSR C;
+    C.nr = A.nr; C.nc = B.nc;
+    C.ptr.assign(A.nr + 1, 0);
+    std::vector<double> acc(B.nc, 0.0);
+    std::vector<int> mark(B.nc, -1), cols;
+    for (int i = 0; i < A.nr; i++) {
+        cols.clear();
+        for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+            int k = A.col[ka];
+            double a = A.val[ka];
+            for (int kb = B.ptr[k]; kb < B.ptr[k + 1]; kb++) {
+                int j = B.col[kb];
+                if (mark[j] != i) { mark[j] = i; acc[j] = 0.0; cols.push_back(j); }
+                acc[j] += a * B.val[kb];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) { C.col.push_back(j); C.val.push_back(acc[j]); }
+        C.ptr[i + 1] = (int)C.col.size();
+    }
+    return C;
+}
+// y = A x (+ z)
+static void spmv(const CSR &A, const double *x, double *y, const double *z = nullptr) {
+    for (int i = 0; i < A.nr; i++) {
+        double s = z ? z[i] : 0.0;
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) s += A.val[k] * x[A.col[k]];
+        y[i] = s;
+    }
+}
+static void spmv_t(const CSR &A, const double *x, double *y) {  // y = A^T x
+    std::fill(y, y + A.nc, 0.0);
+    for (int i = 0; i < A.nr; i++)
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) y[A.col[k]] += A.val[k] * x[i];
+}
+// C := alpha*C + beta*X  with union pattern (MatScale + MatAXPY)
+static CSR axpy_union(const CSR &C, double alpha, const CSR &X, double beta) {
+    CSR R;
+    R.nr = C.nr; R.nc = C.nc;
+    R.ptr.assign(C.nr + 1, 0);
+    for (int i = 0; i < C.nr; i++) {
+        int a = C.ptr[i], ae = C.ptr[i + 1], b = X.ptr[i], be = X.ptr[i + 1];
+        while (a < ae || b < be) {
+            int ca = a < ae ? C.col[a] : INT32_MAX, cb = b < be ? X.col[b] : INT32_MAX;
+            if (ca == cb) { R.col.push_back(ca); R.val.push_back(alpha * C.val[a] + beta * X.val[b]); a++; b++; }
+            else if (ca < cb) { R.col.push_back(ca); R.val.push_back(alpha * C.val[a]); a++; }
+            else { R.col.push_back(cb); R.val.push_back(beta * X.val[b]); b++; }
+        }
+        R.ptr[i + 1] = (int)R.col.size();
+    }
+    return R;
+}
+
+// ---------------------------------------------------------------------------
+// Darcy MH (steady): src/flow/darcy_flow_mh.cc
+// ---------------------------------------------------------------------------
+struct Darcy {
+    Mesh *m;
+    int E, S, F, N;
+    std::vector<double> K, cs, sigma, f, aniso;  // per element
+    std::vector<int> bc_type;                     // per bc element (0 none,1 dir,2 neu,3 robin)
+    std::vector<double> bc_p, bc_flux, bc_sigma;  // per bc element
+    CSR M;                                        // MH matrix (S+E+F)
+    std::vector<double> rhs;
+    // Schur stage 1 / 2 (schur.cc)
+    CSR IA1, IAB1, C1;  // C1: complement (E+F)
+    std::vector<double> rhs1, rhsC1;
+    CSR IA2, IAB2, C2;  // C2: final edge system (F)
+    std::vector<double> rhsC1_el, rhsC2;
+    std::vector<double> sol;  // [q | p | lambda]
+    int iters = 0;
+    double resid = 0, bnorm1 = 0;
+    double t_assembly = 0, t_schur = 0, t_solve = 0, t_resolve = 0;
+};
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// assembly_steady_mh_matrix: darcy_flow_mh.cc:525-694 (n_schurs == 2, no mortar)
+static void assemble(Darcy &D) {
+    Mesh &m = *D.m;
+    Triplets T;
+    D.rhs.assign(D.N, 0.0);
+    const int S = D.S, E = D.E;
+    double loc[16], inv[16];
+    for (int e = 0; e < E; e++) {
+        const Element &el = m.element[e];
+        int ns = el.n_sides();
+        int el_row = S + e;
+        local_matrix(m, el, D.K[e], D.cs[e], D.aniso.empty() ? nullptr : &D.aniso[9 * e], loc, inv);
+        double csec = D.cs[e];
+        double ec[3];
+        elem_centre(m, el, ec);
+        int side_rows[4], edge_rows[4];
+        double loc_rhs[4];
+        for (int i = 0; i < ns; i++) {
+            side_rows[i] = m.side_offset[e] + i;
+            edge_rows[i] = S + E + el.edge_idx[i];
+            double sc[3];
+            side_centre(m, el, i, sc);
+            loc_rhs[i] = ec[2] - sc[2];  // gravity term :565
+            double c_val = 1.0;
+            int b = el.boundary_idx[i];
+            if (b >= 0) {
+                int bce = m.boundary[b].bc_ele_idx;
+                int type = D.bc_type[bce];
+                double bmeas = elem_measure(m, m.bc_elements[bce]);
+                if (type == 1) {
+                    c_val = 0.0;
+                    loc_rhs[i] -= D.bc_p[bce];
+                    D.rhs[edge_rows[i]] += -D.bc_p[bce];
+                    T.add(edge_rows[i], edge_rows[i], -1.0);
+                } else if (type == 2) {
+                    D.rhs[edge_rows[i]] += D.bc_flux[bce] * bmeas * csec;
+                } else if (type == 3) {
+                    D.rhs[edge_rows[i]] += -bmeas * D.bc_sigma[bce] * D.bc_p[bce] * csec;
+                    T.add(edge_rows[i], edge_rows[i], -bmeas * D.bc_sigma[bce] * csec);
+                }
+            }
+            T.add(side_rows[i], edge_rows[i], c_val);
+            T.add(edge_rows[i], side_rows[i], c_val);
+        }
+        for (int i = 0; i < ns; i++) D.rhs[side_rows[i]] += loc_rhs[i];
+        for (int i = 0; i < ns; i++)
+            for (int j = 0; j < ns; j++) T.add(side_rows[i], side_rows[j], loc[ns * i + j]);
+        for (int i = 0; i < ns; i++) T.add(el_row, side_rows[i], -1.0);
+        for (int i = 0; i < ns; i++) T.add(side_rows[i], el_row, -1.0);
+        D.rhs[el_row] += -1.0 * elem_measure(m, el) * D.cs[e] * D.f[e];
+        T.add(el_row, el_row, 0.0);
+        std::vector<int> vb_rows;
+        for (int n : el.neigh_vb) {
+            const Neighbour &ngh = m.vb[n];
+            int edge_row = S + E + ngh.edge_idx;
+            const auto &hs = m.edges[ngh.edge_idx].sides[0];  // Neighbour::side(), neighbours_impl.hh:16-20
+            const Element &hel = m.element[hs.first];
+            double value = D.sigma[e] * side_measure(m, hel, hs.second) * D.cs[hs.first];
+            T.add(el_row, el_row, -value);
+            T.add(el_row, edge_row, value);
+            T.add(edge_row, el_row, value);
+            T.add(edge_row, edge_row, -value);
+            for (int i = 0; i < ns; i++) { T.add(edge_rows[i], edge_row, 0.0); T.add(edge_row, edge_rows[i], 0.0); }
+            vb_rows.push_back(edge_row);
+        }
+        for (int a : vb_rows)
+            for (int b : vb_rows) T.add(a, b, 0.0);
+        for (int i = 0; i < ns; i++) { T.add(el_row, edge_rows[i], 0.0); T.add(edge_rows[i], el_row, 0.0); }
+        for (int i = 0; i < ns; i++)
+            for (int j = 0; j < ns; j++) T.add(edge_rows[i], edge_rows[j], 0.0);
+    }
+    D.M = T.to_csr(D.N, D.N);
+}
+
+// SchurComplement::form_schur + form_rhs: src/la/schur.cc:520-696
+static void form_schur(const CSR &Orig, const std::vector<double> &rhs, int nA, const CSR &IA, CSR &IAB,
+                       CSR &Compl, std::vector<double> &crhs) {
+    int n = Orig.nr;
+    CSR B = submat(Orig, 0, nA, nA, n);
+    IAB = spgemm(IA, B);
+    CSR Bt = submat(Orig, nA, n, 0, nA);
+    CSR xA = spgemm(Bt, IAB);
+    CSR C = submat(Orig, nA, n, nA, n);
+    Compl = axpy_union(C, -1.0, xA, 1.0);  // complement = Bt*IA*B - C
+    crhs.assign(n - nA, 0.0);
+    spmv_t(IAB, rhs.data(), crhs.data());  // MatMultTranspose(IAB, RHS1)
+    for (int i = 0; i < n - nA; i++) crhs[i] += -1 * rhs[nA + i];
+}
+// SchurComplement::resolve: schur.cc:714-773;  x1 = IA*RHS1 - IAB*x2
+static void resolve(const CSR &IA, const CSR &IAB, const double *rhs1, const double *x2, double *x1) {
+    int nA = IA.nr;
+    std::vector<double> t(nA);
+    spmv(IAB, x2, t.data());
+    for (int i = 0; i < nA; i++) t[i] *= -1;
+    spmv(IA, rhs1, x1, t.data());
+}
+
+// Jacobi-preconditioned CG on an SPD CSR system: the KSP CG loop of
+// src/la/solve.cc:316-463 with a diagonal preconditioner.
+static int cg(const CSR &A, const double *b, double *x, double rtol, double atol, int max_it, double &res) {
+    int n = A.nr;
+    std::vector<double> r(n), z(n), p(n), q(n), dinv(n);
+    for (int i = 0; i < n; i++) {
+        double d = A.get(i, i);
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    }
+    spmv(A, x, q.data());
+    double bn = 0, rr = 0;
+    for (int i = 0; i < n; i++) { r[i] = b[i] - q[i]; bn += b[i] * b[i]; rr += r[i] * r[i]; }
+    bn = std::sqrt(bn);
+    double tol = std::max(rtol * bn, atol);
+    for (int i = 0; i < n; i++) { z[i] = dinv[i] * r[i]; p[i] = z[i]; }
+    double rz = 0;
+    for (int i = 0; i < n; i++) rz += r[i] * z[i];
+    int it = 0;
+    res = std::sqrt(rr);
+    while (res > tol && it < max_it) {
+        spmv(A, p.data(), q.data());
+        double pq = 0;
+        for (int i = 0; i < n; i++) pq += p[i] * q[i];
+        double alpha = rz / pq;
+        rr = 0;
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; rr += r[i] * r[i]; }
+        double rz_new = 0;
+        for (int i = 0; i < n; i++) { z[i] = dinv[i] * r[i]; rz_new += r[i] * z[i]; }
+        double beta = rz_new / rz;
+        rz = rz_new;
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        res = std::sqrt(rr);
+        it++;
+    }
+    return it;
+}
+
+// Dense LU with partial pivoting (brute-force check on tiny systems).
+static bool dense_solve(std::vector<double> A, std::vector<double> b, int n, std::vector<double> &x) {
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)p * n + k])) p = i;
+        if (A[(size_t)p * n + k] == 0.0) return false;
+        if (p != k) {
+            for (int j = 0; j < n; j++) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+            std::swap(b[k], b[p]);
+        }
+        for (int i = k + 1; i < n; i++) {
+            double l = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            if (l == 0.0) continue;
+            for (int j = k; j < n; j++) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+            b[i] -= l * b[k];
+        }
+    }
+    x.assign(n, 0.0);
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= A[(size_t)i * n + j] * x[j];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+    return true;
+}
+
+// make_schur1 / make_schur2: darcy_flow_mh.cc:1009-1131
+static void make_schurs(Darcy &D) {
+    Mesh &m = *D.m;
+    const int S = D.S, E = D.E;
+    Triplets T;
+    double loc[16], inv[16];
+    for (int e = 0; e < E; e++) {
+        const Element &el = m.element[e];
+        int ns = el.n_sides();
+        local_matrix(m, el, D.K[e], D.cs[e], D.aniso.empty() ? nullptr : &D.aniso[9 * e], loc, inv);
+        for (int i = 0; i < ns; i++)
+            for (int j = 0; j < ns; j++) T.add(m.side_offset[e] + i, m.side_offset[e] + j, inv[ns * i + j]);
+    }
+    D.IA1 = T.to_csr(S, S);
+    form_schur(D.M, D.rhs, S, D.IA1, D.IAB1, D.C1, D.rhsC1);
+    // schur1 rhs norm for solution_precision (darcy_flow_mh.cc:463-482)
+    double bn = 0;
+    for (double v : D.rhsC1) bn += v * v;
+    D.bnorm1 = std::sqrt(bn);
+    // stage 2: IA2 = diag(C1)^-1 on the element block
+    Triplets T2;
+    for (int e = 0; e < E; e++) {
+        double d = D.C1.get(e, e);
+        T2.add(e, e, 1.0 / d);
+    }
+    D.IA2 = T2.to_csr(E, E);
+    form_schur(D.C1, D.rhsC1, E, D.IA2, D.IAB2, D.C2, D.rhsC2);
+    // scale(-1): schur.cc:702-707
+    for (double &v : D.C2.val) v *= -1.0;
+    for (double &v : D.rhsC2) v *= -1.0;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ============================================================================
+// C ABI used by tests (ctypes) — test infrastructure only.
+// ============================================================================
+extern "C" {
+
+const char *orc_last_error() { return g_err.c_str(); }
+
+void *orc_mesh_load(const char *path) {
+    Mesh *m = new Mesh();
+    if (!read_gmsh(path, *m) || !setup_topology(*m)) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
+void orc_mesh_free(void *h) { delete (Mesh *)h; }
+
+// counts: n_nodes, n_elements, n_bc_elements, n_sides, n_edges, n_boundaries,
+//         n_vb, n_regions, n_file_bc
+void orc_mesh_counts(void *h, int *out) {
+    Mesh &m = *(Mesh *)h;
+    out[0] = (int)m.node_ids.size();
+    out[1] = (int)m.element.size();
+    out[2] = (int)m.bc_elements.size();
+    out[3] = m.n_sides;
+    out[4] = (int)m.edges.size();
+    out[5] = (int)m.boundary.size();
+    out[6] = (int)m.vb.size();
+    out[7] = (int)m.rdb.regs.size();
+    out[8] = m.n_file_bc;
+}
+
+// Topology arrays (all int32):
+//  elem_dim[E], elem_id[E], elem_region[E] (region id), elem_nodes[4E] (node index, -1 pad),
+//  elem_edge[4E], elem_bnd[4E], edge_ptr[F+1], edge_side[S] (global side dof),
+//  vb_elem[V], vb_edge[V], bnd_edge[B], bnd_bcele[B], bc_region[NBC] (region id)
+void orc_mesh_topology(void *h, int *elem_dim, int *elem_id, int *elem_region, int *elem_nodes, int *elem_edge,
+                       int *elem_bnd, int *edge_ptr, int *edge_side, int *vb_elem, int *vb_edge, int *bnd_edge,
+                       int *bnd_bcele, int *bc_region) {
+    Mesh &m = *(Mesh *)h;
+    for (size_t e = 0; e < m.element.size(); e++) {
+        const Element &el = m.element[e];
+        elem_dim[e] = el.dim;
+        elem_id[e] = el.id;
+        elem_region[e] = m.rdb.regs[el.region].id;
+        for (int k = 0; k < 4; k++) {
+            elem_nodes[4 * e + k] = el.node[k];
+            elem_edge[4 * e + k] = el.edge_idx[k];
+            elem_bnd[4 * e + k] = el.boundary_idx[k];
+        }
+    }
+    edge_ptr[0] = 0;
+    int k = 0;
+    for (size_t f = 0; f < m.edges.size(); f++) {
+        for (auto &s : m.edges[f].sides) edge_side[k++] = m.side_offset[s.first] + s.second;
+        edge_ptr[f + 1] = k;
+    }
+    for (size_t n = 0; n < m.vb.size(); n++) { vb_elem[n] = m.vb[n].element; vb_edge[n] = m.vb[n].edge_idx; }
+    for (size_t b = 0; b < m.boundary.size(); b++) { bnd_edge[b] = m.boundary[b].edge_idx; bnd_bcele[b] = m.boundary[b].bc_ele_idx; }
+    for (size_t b = 0; b < m.bc_elements.size(); b++) bc_region[b] = m.rdb.regs[m.bc_elements[b].region].id;
+}
+
+void orc_mesh_nodes(void *h, double *xyz) {
+    Mesh &m = *(Mesh *)h;
+    std::memcpy(xyz, m.X.data(), m.X.size() * sizeof(double));
+}
+
+// Per-element geometry: measure[E], centre[3E]; per bc element: centre[3NBC], measure[NBC]
+void orc_mesh_geometry(void *h, double *measure, double *centre, double *bc_centre, double *bc_measure) {
+    Mesh &m = *(Mesh *)h;
+    for (size_t e = 0; e < m.element.size(); e++) {
+        measure[e] = elem_measure(m, m.element[e]);
+        elem_centre(m, m.element[e], centre + 3 * e);
+    }
+    for (size_t b = 0; b < m.bc_elements.size(); b++) {
+        elem_centre(m, m.bc_elements[b], bc_centre + 3 * b);
+        bc_measure[b] = elem_measure(m, m.bc_elements[b]);
+    }
+}
+
+// Region table: ids[R], is_boundary[R], sub_idx[R]; labels joined by '\n' into buf.
+void orc_mesh_regions(void *h, int *ids, int *is_boundary, int *sub_idx, char *buf, int buflen) {
+    Mesh &m = *(Mesh *)h;
+    std::string all;
+    for (size_t r = 0; r < m.rdb.regs.size(); r++) {
+        ids[r] = m.rdb.regs[r].id;
+        is_boundary[r] = m.rdb.regs[r].boundary;
+        sub_idx[r] = m.rdb.regs[r].sub_idx;
+        all += m.rdb.regs[r].label + "\n";
+    }
+    std::snprintf(buf, buflen, "%s", all.c_str());
+}
+
+// Local RT0 matrix and its inverse for element e; returns det.
+double orc_local_matrix(void *h, int e, double K, double cs, const double *aniso, double *loc, double *inv) {
+    Mesh &m = *(Mesh *)h;
+    return local_matrix(m, m.element[e], K, cs, aniso, loc, inv);
+}
+
+void *orc_darcy_create(void *mesh, const double *K, const double *cs, const double *aniso, const double *sigma,
+                       const double *f, const int *bc_type, const double *bc_p, const double *bc_flux,
+                       const double *bc_sigma) {
+    Mesh &m = *(Mesh *)mesh;
+    Darcy *D = new Darcy();
+    D->m = &m;
+    D->E = (int)m.element.size();
+    D->S = m.n_sides;
+    D->F = (int)m.edges.size();
+    D->N = D->S + D->E + D->F;
+    int E = D->E, NB = (int)m.bc_elements.size();
+    D->K.assign(K, K + E);
+    D->cs.assign(cs, cs + E);
+    D->sigma.assign(sigma, sigma + E);
+    D->f.assign(f, f + E);
+    if (aniso) D->aniso.assign(aniso, aniso + 9 * (size_t)E);
+    D->bc_type.assign(bc_type, bc_type + NB);
+    D->bc_p.assign(bc_p, bc_p + NB);
+    D->bc_flux.assign(bc_flux, bc_flux + NB);
+    D->bc_sigma.assign(bc_sigma, bc_sigma + NB);
+    return D;
+}
+void orc_darcy_free(void *h) { delete (Darcy *)h; }
+
+// Full steady solve (update_solution, darcy_flow_mh.cc:375-433, n_schurs=2).
+// method 0: Jacobi-CG on the edge system; method 1: dense LU of the edge system
+// (brute force, small meshes); method 2: dense LU of the FULL MH system.
+int orc_darcy_solve(void *h, int method, double rtol, double atol, int max_it) {
+    Darcy &D = *(Darcy *)h;
+    double t0 = now_s();
+    assemble(D);
+    double t1 = now_s();
+    D.t_assembly = t1 - t0;
+    D.sol.assign(D.N, 0.0);
+    if (method == 2) {
+        if ((size_t)D.N * D.N > 400000000ull) { g_err = "system too large for dense solve"; return -1; }
+        std::vector<double> A((size_t)D.N * D.N, 0.0), x;
+        for (int i = 0; i < D.N; i++)
+            for (int k = D.M.ptr[i]; k < D.M.ptr[i + 1]; k++) A[(size_t)i * D.N + D.M.col[k]] = D.M.val[k];
+        if (!dense_solve(A, D.rhs, D.N, x)) { g_err = "singular"; return -1; }
+        D.sol = x;
+        return 0;
+    }
+    make_schurs(D);
+    double t2 = now_s();
+    D.t_schur = t2 - t1;
+    int F = D.F, E = D.E, S = D.S;
+    std::vector<double> lam(F, 0.0);
+    if (method == 1) {
+        std::vector<double> A((size_t)F * F, 0.0);
+        for (int i = 0; i < F; i++)
+            for (int k = D.C2.ptr[i]; k < D.C2.ptr[i + 1]; k++) A[(size_t)i * F + D.C2.col[k]] = D.C2.val[k];
+        if (!dense_solve(A, D.rhsC2, F, lam)) { g_err = "singular"; return -1; }
+        D.iters = 0;
+    } else {
+        D.iters = cg(D.C2, D.rhsC2.data(), lam.data(), rtol, atol, max_it, D.resid);
+    }
+    double t3 = now_s();
+    D.t_solve = t3 - t2;
+    // schur2->resolve (elements from edges), schur1->resolve (sides)
+    std::vector<double> x2(E + F);
+    resolve(D.IA2, D.IAB2, D.rhsC1.data(), lam.data(), x2.data());
+    for (int f = 0; f < F; f++) x2[E + f] = lam[f];
+    std::vector<double> q(S);
+    resolve(D.IA1, D.IAB1, D.rhs.data(), x2.data(), q.data());
+    std::copy(q.begin(), q.end(), D.sol.begin());
+    std::copy(x2.begin(), x2.end(), D.sol.begin() + S);
+    D.t_resolve = now_s() - t3;
+    return D.iters;
+}
+
+int orc_darcy_size(void *h, int *out) {
+    Darcy &D = *(Darcy *)h;
+    out[0] = D.S; out[1] = D.E; out[2] = D.F; out[3] = D.N;
+    out[4] = (int)D.C2.col.size();
+    return D.N;
+}
+void orc_darcy_solution(void *h, double *out) {
+    Darcy &D = *(Darcy *)h;
+    std::memcpy(out, D.sol.data(), D.sol.size() * sizeof(double));
+}
+// stats: iters, residual, ||schur1 rhs||, t_assembly, t_schur, t_solve, t_resolve
+void orc_darcy_stats(void *h, double *out) {
+    Darcy &D = *(Darcy *)h;
+    out[0] = D.iters; out[1] = D.resid; out[2] = D.bnorm1;
+    out[3] = D.t_assembly; out[4] = D.t_schur; out[5] = D.t_solve; out[6] = D.t_resolve;
+}
+// Final SPD edge system (after both Schur stages and scale(-1)) as CSR.
+void orc_darcy_edge_system(void *h, int *ptr, int *col, double *val, double *rhs) {
+    Darcy &D = *(Darcy *)h;
+    std::memcpy(ptr, D.C2.ptr.data(), D.C2.ptr.size() * sizeof(int));
+    std::memcpy(col, D.C2.col.data(), D.C2.col.size() * sizeof(int));
+    std::memcpy(val, D.C2.val.data(), D.C2.val.size() * sizeof(double));
+    std::memcpy(rhs, D.rhsC2.data(), D.rhsC2.size() * sizeof(double));
+}
+// MH residual ||M x - b|| / ||b|| of the current solution (full system).
+double orc_darcy_mh_residual(void *h) {
+    Darcy &D = *(Darcy *)h;
+    std::vector<double> r(D.N);
+    spmv(D.M, D.sol.data(), r.data());
+    double rn = 0, bn = 0;
+    for (int i = 0; i < D.N; i++) { double d = r[i] - D.rhs[i]; rn += d * d; bn += D.rhs[i] * D.rhs[i]; }
+    return std::sqrt(rn) / std::max(std::sqrt(bn), 1e-300);
+}
+
+// Water balance per boundary region: DarcyFlowMHOutput::water_balance,
+// darcy_flow_mh_output.cc:671-773.  out arrays indexed by boundary region
+// sub-index: balance, plus (outflow), minus (inflow).
+void orc_darcy_water_balance(void *h, double *bal, double *plus, double *minus) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    int nbr = m.rdb.n_boundary;
+    std::fill(bal, bal + nbr, 0.0);
+    std::fill(plus, plus + nbr, 0.0);
+    std::fill(minus, minus + nbr, 0.0);
+    for (auto &b : m.boundary) {
+        auto s = m.edges[b.edge_idx].sides[0];
+        double flux = D.sol[m.side_offset[s.first] + s.second];
+        int r = m.rdb.regs[m.bc_elements[b.bc_ele_idx].region].sub_idx;
+        bal[r] += flux;
+        if (flux > 0) plus[r] += flux;
+        else minus[r] += flux;
+    }
+}
+
+// Per-element balance of the element row of the MH system
+// (darcy_flow_mh.cc:603-640): sum_i q_i + sum_vb v (p_e - lambda_n) - |e| cs f.
+void orc_darcy_element_balance(void *h, double *out) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    for (int e = 0; e < D.E; e++) {
+        const Element &el = m.element[e];
+        double s = 0;
+        for (int i = 0; i < el.n_sides(); i++) s += D.sol[m.side_offset[e] + i];
+        double pe = D.sol[D.S + e];
+        for (int n : el.neigh_vb) {
+            const auto &hs = m.edges[m.vb[n].edge_idx].sides[0];
+            double v = D.sigma[e] * side_measure(m, m.element[hs.first], hs.second) * D.cs[hs.first];
+            s += v * (pe - D.sol[D.S + D.E + m.vb[n].edge_idx]);
+        }
+        out[e] = s - elem_measure(m, el) * D.cs[e] * D.f[e];
+    }
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Explicit transport: src/transport/transport.cc, TimeGovernor
+// src/coupling/time_governor.cc:261-355, MassBalance mass_balance.cc:79-239.
+// ============================================================================
+namespace orc {
+
+static const double INF = std::numeric_limits<double>::infinity();
+static const double EPS = std::numeric_limits<double>::epsilon();
+
+struct TimeGov {  // the non-steady TimeGovernor restated for one equation
+    double time = 0, end_time = 0, time_step = 0, last_time_step = 0, fixed_dt = 0, end_fixed = 0;
+    double upper = INF, lower = EPS, max_ts = INF, min_ts = EPS;
+    bool dt_fixed_now = false, dt_changed = true;
+    int level = 0;
+    std::vector<double> marks;  // fixed-time marks, sorted
+    bool lt(double o) const { double b = o - 16 * EPS * std::max(std::fabs(time), std::fabs(o)); return !(time >= b); }
+    bool ge(double o) const { double b = o - 16 * EPS * std::max(std::fabs(time), std::fabs(o)); return time >= b; }
+    bool is_end() const { return ge(end_time) || time == INF; }
+    void add_mark(double t) {
+        auto it = std::lower_bound(marks.begin(), marks.end(), t);
+        if (it != marks.end() && *it == t) return;
+        marks.insert(it, t);
+    }
+    double next_mark() const {  // TimeMarks::next, time_marks.cc:125-132
+        auto it = std::lower_bound(marks.begin(), marks.end(), time);
+        while (it != marks.end() && !lt(*it)) ++it;
+        return it == marks.end() ? INF : *it;
+    }
+    double estimate_dt() const {  // time_governor.cc:261-307
+        if (time == INF || is_end()) return 0.0;
+        if (lt(end_fixed)) return fixed_dt;
+        double full = next_mark() - time;
+        double est = std::min(full, upper);
+        est = std::max(est, lower);
+        int n = (int)std::ceil(full / est);
+        int nf = (int)std::floor(full / est);
+        if (std::fabs(full / est - nf) < 1e-14) n = nf;
+        return full / n;
+    }
+    int set_upper_constraint(double u) {  // time_governor.cc:215-237
+        if (upper < u) return -1;
+        if (lower <= u) { upper = u; return 0; }
+        return 1;
+    }
+    double fix_dt_until_mark() {  // time_governor.hh:205-211
+        end_fixed = -INF;
+        fixed_dt = estimate_dt();
+        dt_fixed_now = true;
+        return end_fixed = next_mark();
+    }
+    void next_time() {  // time_governor.cc:314-355
+        if (time == INF || is_end()) return;
+        if (lt(end_fixed)) {
+            fixed_dt = (end_fixed - time) / std::round((end_fixed - time) / fixed_dt);
+            last_time_step = time_step;
+            time_step = fixed_dt;
+            if (dt_fixed_now) { dt_fixed_now = false; dt_changed = (last_time_step != time_step); }
+            else dt_changed = false;
+        } else {
+            last_time_step = time_step;
+            time_step = estimate_dt();
+            dt_changed = (last_time_step != time_step);
+        }
+        time += time_step;
+        level++;
+        upper = std::min(end_time - time, max_ts);
+        lower = min_ts;
+    }
+};
+
+struct Transport {
+    Mesh *m;
+    int E, nsub;
+    std::vector<double> flux;  // side flux [S]
+    std::vector<double> cs, por;
+    std::vector<double> bc_conc;    // [nsub][NBC]
+    std::vector<double> conc, pconc;  // [nsub][E]
+    std::vector<double> bcvcorr;    // [nsub][E]
+    CSR tm;
+    std::vector<int> diag_pos;
+    double cfl_max_step = 0;
+    bool matrix_built = false, scaled = false, need_rescale = true;
+    TimeGov tg;
+    // mass balance
+    int nbr = 0, nblk = 0;
+    std::vector<double> bcd_bal, bcd_plus, bcd_minus, mass, src;  // [nsub][nreg]
+    std::vector<double> initial_mass, integrated_fluxes, integrated_sources, last_total_flux;
+    double last_time = 0;
+    bool mb_initial = true;
+    int steps = 0;
+    double t_steps = 0;
+};
+
+// create_transport_matrix_mpi: transport.cc:788-943
+static void build_tm(Transport &T) {
+    Mesh &m = *T.m;
+    int E = T.E;
+    std::vector<double> edge_flow(m.edges.size(), 0.0);
+    for (size_t i = 0; i < m.edges.size(); i++)
+        for (auto &s : m.edges[i].sides) {
+            double fl = T.flux[m.side_offset[s.first] + s.second];
+            if (fl > 0) edge_flow[i] += fl;
+        }
+    // INSERT_VALUES: later insertions of the same entry overwrite
+    std::vector<std::map<int, double>> rows(E);
+    double max_sum = 0.0;
+    for (int e = 0; e < E; e++) {
+        const Element &el = m.element[e];
+        double aii = 0.0;
+        double vol = elem_measure(m, el) * T.cs[e] * T.por[e];
+        for (int si = 0; si < el.n_sides(); si++) {
+            double fl = T.flux[m.side_offset[e] + si];
+            if (el.boundary_idx[si] < 0) {
+                const Edge &edg = m.edges[el.edge_idx[si]];
+                double edg_flux = edge_flow[el.edge_idx[si]];
+                for (auto &s : edg.sides) {
+                    if (s.first == e && s.second == si) continue;
+                    double fl2 = T.flux[m.side_offset[s.first] + s.second];
+                    double aij;
+                    if (fl2 > 0.0 && fl < 0.0) aij = -(fl * fl2 / (edg_flux * elem_measure(m, el) * T.cs[e] * T.por[e]));
+                    else aij = 0;
+                    rows[e][s.first] = aij;
+                }
+                if (fl > 0.0) aii -= (fl / (elem_measure(m, el) * T.cs[e] * T.por[e]));
+            } else {
+                if (fl > 0.0) aii -= (fl / (elem_measure(m, el) * T.cs[e] * T.por[e]));
+            }
+        }
+        (void)vol;
+        for (int n : el.neigh_vb) {
+            const auto &hs = m.edges[m.vb[n].edge_idx].sides[0];
+            int e2 = hs.first;
+            double fl = T.flux[m.side_offset[e2] + hs.second];
+            double aij = fl > 0.0 ? fl / (elem_measure(m, el) * T.cs[e] * T.por[e]) : 0;
+            rows[e][e2] = aij;
+            if (fl < 0.0) {
+                aii -= (-fl) / (elem_measure(m, el) * T.cs[e] * T.por[e]);
+                aij = (-fl) / (elem_measure(m, m.element[e2]) * T.cs[e2] * T.por[e2]);
+            } else aij = 0;
+            rows[e2][e] = aij;
+        }
+        rows[e][e] = aii;
+        if (std::fabs(aii) > max_sum) max_sum = std::fabs(aii);
+    }
+    T.cfl_max_step = 1 / max_sum;
+    T.tm = CSR();
+    T.tm.nr = T.tm.nc = E;
+    T.tm.ptr.assign(E + 1, 0);
+    T.diag_pos.assign(E, -1);
+    for (int e = 0; e < E; e++) {
+        for (auto &kv : rows[e]) {
+            if (kv.first == e) T.diag_pos[e] = (int)T.tm.col.size();
+            T.tm.col.push_back(kv.first);
+            T.tm.val.push_back(kv.second);
+        }
+        T.tm.ptr[e + 1] = (int)T.tm.col.size();
+    }
+    T.matrix_built = true;
+    T.scaled = false;
+}
+
+// set_boundary_conditions: transport.cc:455-495
+static void set_bc(Transport &T) {
+    Mesh &m = *T.m;
+    std::fill(T.bcvcorr.begin(), T.bcvcorr.end(), 0.0);
+    int NBC = (int)m.bc_elements.size();
+    for (int e = 0; e < T.E; e++) {
+        const Element &el = m.element[e];
+        for (int si = 0; si < el.n_sides(); si++) {
+            int b = el.boundary_idx[si];
+            if (b < 0) continue;
+            double fl = T.flux[m.side_offset[e] + si];
+            if (fl < 0.0) {
+                double aij = -(fl / (elem_measure(m, el) * T.cs[e] * T.por[e]));
+                int bce = m.boundary[b].bc_ele_idx;
+                for (int s = 0; s < T.nsub; s++) T.bcvcorr[(size_t)s * T.E + e] += T.bc_conc[(size_t)s * NBC + bce] * aij;
+            }
+        }
+    }
+}
+
+// MassBalance::calculate: mass_balance.cc:79-239 with calc_fluxes /
+// calc_elem_sources transport.cc:1299-1366 (no sources in these runs).
+static void mb_calculate(Transport &T, double time) {
+    if (!T.mb_initial && T.last_time == time) return;
+    Mesh &m = *T.m;
+    int ns = T.nsub, NBC = (int)m.bc_elements.size();
+    std::fill(T.bcd_bal.begin(), T.bcd_bal.end(), 0.0);
+    std::fill(T.bcd_plus.begin(), T.bcd_plus.end(), 0.0);
+    std::fill(T.bcd_minus.begin(), T.bcd_minus.end(), 0.0);
+    std::fill(T.mass.begin(), T.mass.end(), 0.0);
+    std::fill(T.src.begin(), T.src.end(), 0.0);
+    for (auto &b : m.boundary) {
+        auto s = m.edges[b.edge_idx].sides[0];
+        double wf = T.flux[m.side_offset[s.first] + s.second];
+        int r = m.rdb.regs[m.bc_elements[b.bc_ele_idx].region].sub_idx;
+        for (int k = 0; k < ns; k++) {
+            double mf = wf < 0 ? wf * T.bc_conc[(size_t)k * NBC + b.bc_ele_idx] : wf * T.pconc[(size_t)k * T.E + s.first];
+            T.bcd_bal[k * T.nbr + r] += mf;
+            if (mf > 0) T.bcd_plus[k * T.nbr + r] += mf;
+            else T.bcd_minus[k * T.nbr + r] += mf;
+        }
+    }
+    for (int k = 0; k < ns; k++)
+        for (int e = 0; e < T.E; e++) {
+            const Element &el = m.element[e];
+            int r = m.rdb.regs[el.region].sub_idx;
+            T.mass[k * T.nblk + r] += T.por[e] * T.cs[e] * T.conc[(size_t)k * T.E + e] * elem_measure(m, el);
+        }
+    std::vector<double> tot_bal(ns, 0.0), tot_src(ns, 0.0), mass_tot(ns, 0.0);
+    for (int k = 0; k < ns; k++) {
+        for (int r = 0; r < T.nbr; r++) tot_bal[k] += T.bcd_bal[k * T.nbr + r];
+        for (int r = 0; r < T.nblk; r++) { mass_tot[k] += T.mass[k * T.nblk + r]; tot_src[k] += T.src[k * T.nblk + r]; }
+    }
+    if (T.mb_initial) {
+        T.last_time = time;
+        for (int k = 0; k < ns; k++) T.initial_mass[k] = mass_tot[k];
+        T.mb_initial = false;
+    }
+    for (int k = 0; k < ns; k++) {
+        T.integrated_sources[k] += tot_src[k] * (time - T.last_time);
+        T.integrated_fluxes[k] += tot_bal[k] * (time - T.last_time);
+    }
+    T.last_time = time;
+}
+
+// compute_one_step: transport.cc:593-715
+static void one_step(Transport &T) {
+    if (!T.matrix_built) {
+        build_tm(T);
+        T.tg.set_upper_constraint(T.cfl_max_step);
+        T.tg.fix_dt_until_mark();
+        set_bc(T);
+        double est = T.tg.estimate_dt();
+        for (double &v : T.bcvcorr) v *= est;
+        T.need_rescale = true;
+    }
+    if (T.need_rescale) {
+        if (T.scaled) {
+            double fac = T.tg.estimate_dt() / T.tg.time_step;
+            for (int e = 0; e < T.E; e++) T.tm.val[T.diag_pos[e]] += -1.0;
+            for (double &v : T.tm.val) v *= fac;
+            for (int e = 0; e < T.E; e++) T.tm.val[T.diag_pos[e]] += 1.0;
+            for (double &v : T.bcvcorr) v *= fac;
+        } else {
+            double est = T.tg.estimate_dt();
+            for (double &v : T.tm.val) v *= est;
+            for (int e = 0; e < T.E; e++) T.tm.val[T.diag_pos[e]] += 1.0;
+            T.scaled = true;
+        }
+        T.need_rescale = false;
+    }
+    T.tg.next_time();
+    T.pconc = T.conc;
+    for (int k = 0; k < T.nsub; k++) {
+        // cumulative_corr = bcvcorr + sources_corr (sources are zero here)
+        const double *cp = &T.pconc[(size_t)k * T.E];
+        double *c = &T.conc[(size_t)k * T.E];
+        const double *bc = &T.bcvcorr[(size_t)k * T.E];
+        for (int i = 0; i < T.E; i++) {
+            double s = bc[i] + 0.0;
+            for (int p = T.tm.ptr[i]; p < T.tm.ptr[i + 1]; p++) s += T.tm.val[p] * cp[T.tm.col[p]];
+            c[i] = s;
+        }
+    }
+    T.steps++;
+}
+
+// set_target_time: transport.cc:718-733
+static void set_target(Transport &T, double t) {
+    T.tg.add_mark(t);
+    T.tg.set_upper_constraint(T.cfl_max_step);
+    T.tg.fix_dt_until_mark();
+    T.need_rescale = true;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Transport over a steady flow field.  side_flux[S]; cs, por[E];
+// bc_conc[nsub][NBC] (per bc element); init_conc[nsub][E].
+void *orc_transport_create(void *mesh, const double *side_flux, const double *cs, const double *por, int nsub,
+                           const double *bc_conc, const double *init_conc, double end_time, double save_step) {
+    Mesh &m = *(Mesh *)mesh;
+    Transport *T = new Transport();
+    T->m = &m;
+    T->E = (int)m.element.size();
+    T->nsub = nsub;
+    T->flux.assign(side_flux, side_flux + m.n_sides);
+    T->cs.assign(cs, cs + T->E);
+    T->por.assign(por, por + T->E);
+    int NBC = (int)m.bc_elements.size();
+    T->bc_conc.assign(bc_conc, bc_conc + (size_t)nsub * NBC);
+    T->conc.assign(init_conc, init_conc + (size_t)nsub * T->E);
+    T->pconc.assign((size_t)nsub * T->E, 0.0);
+    T->bcvcorr.assign((size_t)nsub * T->E, 0.0);
+    // TimeGovernor(input) constructor: time_governor.cc:75-117
+    T->tg.time = 0.0;
+    T->tg.end_time = end_time;
+    T->tg.end_fixed = 0.0;
+    T->tg.max_ts = end_time;
+    T->tg.time_step = end_time;
+    T->tg.last_time_step = 0.0;
+    T->tg.add_mark(0.0);
+    T->tg.add_mark(end_time);
+    for (double t = 0.0; t <= end_time * (1 + 1e-12); t += save_step) T->tg.add_mark(t);
+    T->cfl_max_step = end_time;
+    T->nbr = m.rdb.n_boundary;
+    T->nblk = m.rdb.n_bulk;
+    T->bcd_bal.assign(nsub * T->nbr, 0); T->bcd_plus = T->bcd_bal; T->bcd_minus = T->bcd_bal;
+    T->mass.assign(nsub * T->nblk, 0); T->src = T->mass;
+    T->initial_mass.assign(nsub, 0); T->integrated_fluxes = T->initial_mass; T->integrated_sources = T->initial_mass;
+    return T;
+}
+void orc_transport_free(void *h) { delete (Transport *)h; }
+
+// Advance to target time t (TransportOperatorSplitting::update_solution,
+// transport_operator_splitting.cc:244-287); returns steps taken.
+int orc_transport_update(void *h, double t) {
+    Transport &T = *(Transport *)h;
+    if (T.mb_initial) mb_calculate(T, T.tg.time);  // first_time_call output at t=0
+    set_target(T, t);
+    int steps = 0;
+    double t0 = now_s();
+    while (T.tg.lt(t)) {
+        one_step(T);
+        mb_calculate(T, T.tg.time);
+        steps++;
+    }
+    T.t_steps += now_s() - t0;
+    return steps;
+}
+double orc_transport_time(void *h) { return ((Transport *)h)->tg.time; }
+double orc_transport_dt(void *h) { return ((Transport *)h)->tg.time_step; }
+double orc_transport_cfl(void *h) { return ((Transport *)h)->cfl_max_step; }
+void orc_transport_conc(void *h, double *out) {
+    Transport &T = *(Transport *)h;
+    std::memcpy(out, T.conc.data(), T.conc.size() * sizeof(double));
+}
+// Scaled transport matrix tm = I + dt*A (current): ptr[E+1], col, val
+int orc_transport_matrix(void *h, int *ptr, int *col, double *val) {
+    Transport &T = *(Transport *)h;
+    if (ptr) {
+        std::memcpy(ptr, T.tm.ptr.data(), T.tm.ptr.size() * sizeof(int));
+        std::memcpy(col, T.tm.col.data(), T.tm.col.size() * sizeof(int));
+        std::memcpy(val, T.tm.val.data(), T.tm.val.size() * sizeof(double));
+    }
+    return (int)T.tm.col.size();
+}
+// Mass balance (last calculate): bcd_bal/plus/minus [nsub][nbr], mass [nsub][nblk],
+// totals[4*nsub]: initial mass, integrated fluxes, integrated sources, current mass
+void orc_transport_mass_balance(void *h, double *bal, double *plus, double *minus, double *mass, double *totals) {
+    Transport &T = *(Transport *)h;
+    std::memcpy(bal, T.bcd_bal.data(), T.bcd_bal.size() * sizeof(double));
+    std::memcpy(plus, T.bcd_plus.data(), T.bcd_plus.size() * sizeof(double));
+    std::memcpy(minus, T.bcd_minus.data(), T.bcd_minus.size() * sizeof(double));
+    std::memcpy(mass, T.mass.data(), T.mass.size() * sizeof(double));
+    for (int k = 0; k < T.nsub; k++) {
+        double mt = 0;
+        for (int r = 0; r < T.nblk; r++) mt += T.mass[k * T.nblk + r];
+        totals[4 * k] = T.initial_mass[k];
+        totals[4 * k + 1] = T.integrated_fluxes[k];
+        totals[4 * k + 2] = T.integrated_sources[k];
+        totals[4 * k + 3] = mt;
+    }
+}
+double orc_transport_step_seconds(void *h) { return ((Transport *)h)->t_steps; }
+
+}  // extern "C"
