@@ -1,0 +1,181 @@
+/*
+ * C ABI of the MI355X steady mixed-hybrid Darcy solve and explicit upwind
+ * transport step.  Plain pointers and sizes only; every call returns 0 on
+ * success or a negative status, with the message in fl_last_error().
+ * Handles are not thread-safe; each handle owns one HIP stream.
+ *
+ * Each entry point names the Flow123d 1.7 interface it replaces
+ * (paths relative to the Flow123d source tree).
+ *
+ * Arrays follow the reference's numbering exactly:
+ *   E bulk elements in mesh-file order; S sides numbered element by element
+ *   (MH_DofHandler::reinit, src/flow/mh_dofhandler.cc:12-21); F edges in the
+ *   order Mesh::make_neighbours_and_edges creates them (src/mesh/mesh.cc:343-500);
+ *   NBC boundary elements = file boundary elements then implicit ones;
+ *   the MH solution is [side fluxes (S) | element pressures (E) | edge traces (F)]
+ *   (src/flow/darcy_flow_mh.cc:330-366).
+ */
+#ifndef FLOW123D_GPU_H
+#define FLOW123D_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fl_mesh fl_mesh;
+typedef struct fl_darcy fl_darcy;
+typedef struct fl_transport fl_transport;
+
+/* Message of the last failed call on this thread. */
+const char *fl_last_error(void);
+
+/* ---- mesh ---------------------------------------------------------------- */
+
+/* Mesh::Mesh + GmshMeshReader::read_mesh + Mesh::setup_topology
+ * (src/mesh/msh_gmshreader.cc:76-85, src/mesh/mesh.cc:227-245). */
+int fl_mesh_read_gmsh(const char *path, fl_mesh **out);
+
+/* Same, from arrays already in memory: the element list in file order with
+ * gmsh types (1 line, 2 triangle, 4 tetrahedron, 15 point), region ids and
+ * node indices (0-based, 4 per element, -1 padded); physical names optional
+ * (labels starting with '.' are boundary regions, msh_gmshreader.cc:236-239). */
+int fl_mesh_from_arrays(int n_nodes, const double *xyz, int n_elem, const int *elem_id, const int *gmsh_type,
+                        const int *region_id, const int *nodes4, int n_phys, const int *phys_id,
+                        const char *const *phys_name, fl_mesh **out);
+
+void fl_mesh_free(fl_mesh *m);
+
+/* counts[9] = n_nodes, E, NBC, S, F, n_boundaries, n_vb, n_regions, n_file_bc
+ * (Mesh::n_elements/n_sides/n_edges/n_vb_neighbours, mesh.h:108-332). */
+int fl_mesh_counts(const fl_mesh *m, int *counts);
+
+/* Topology arrays (the bit-exact numbering contract):
+ * elem_dim[E], elem_id[E], elem_region[E] (region id), elem_nodes[4E],
+ * elem_edge[4E] (Element::edge_idx_), elem_bnd[4E] (Element::boundary_idx_),
+ * edge_ptr[F+1] + edge_side[S] (Edge::side_, as global side dofs),
+ * vb_elem[V] + vb_edge[V] (Neighbour::element_, Neighbour::edge_idx_),
+ * bnd_edge[B] + bnd_bcele[B] (Boundary::edge_idx_, Boundary::bc_ele_idx_),
+ * bc_region[NBC] (region id of each boundary element). */
+int fl_mesh_topology(const fl_mesh *m, int *elem_dim, int *elem_id, int *elem_region, int *elem_nodes,
+                     int *elem_edge, int *elem_bnd, int *edge_ptr, int *edge_side, int *vb_elem, int *vb_edge,
+                     int *bnd_edge, int *bnd_bcele, int *bc_region);
+
+/* Region table (RegionDB, src/mesh/region.cc:105-171): ids, boundary flag,
+ * index among boundary or bulk regions; labels '\n'-separated into buf. */
+int fl_mesh_regions(const fl_mesh *m, int *ids, int *is_boundary, int *sub_idx, char *buf, int buflen);
+
+/* ---- steady MH Darcy flow -------------------------------------------------- */
+
+/* DarcyFlowMH_Steady::DarcyFlowMH_Steady (src/flow/darcy_flow_mh.cc:249-367):
+ * builds the device mesh, the condensed edge-system pattern and the
+ * precomputed scatter lists. */
+int fl_darcy_create(fl_mesh *m, fl_darcy **out);
+void fl_darcy_free(fl_darcy *d);
+
+/* Field values the reference reads in assembly_steady_mh_matrix
+ * (darcy_flow_mh.cc:525-694, EqData fields :152-183):
+ * per bulk element: conductivity K, cross_section cs, sigma, water_source_density f,
+ * anisotropy (9 per element, row-major; NULL = identity);
+ * per boundary element: bc_type (0 none, 1 dirichlet, 2 neumann, 3 robin),
+ * bc_pressure, bc_flux, bc_robin_sigma. */
+int fl_darcy_set_data(fl_darcy *d, const double *K, const double *cs, const double *aniso, const double *sigma,
+                      const double *f, const int *bc_type, const double *bc_pressure, const double *bc_flux,
+                      const double *bc_robin_sigma);
+
+/* Replace the conductivity only (new Monte Carlo realization). */
+int fl_darcy_set_conductivity(fl_darcy *d, const double *K);
+
+/* Solver options (Solver record, src/la/solve.cc:69-84): r_tol, a_tol, max_it. */
+typedef struct {
+    double r_tol;
+    double a_tol;
+    int max_it;
+} fl_solver_opts;
+
+/* Result of a solve: CG iterations, final residual 2-norm, rhs 2-norm,
+ * converged flag, device times of the phases in ms. */
+typedef struct {
+    int iterations;
+    int converged;
+    double residual;
+    double rhs_norm;
+    double ms_assembly;
+    double ms_solve;
+    double ms_resolve;
+    double ms_total;
+} fl_solve_info;
+
+/* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433). */
+int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *opts, fl_solve_info *info);
+
+/* DarcyFlowMH_Steady::get_solution_vector (darcy_flow_mh.cc:485-500):
+ * copies the [q | p | lambda] vector (S+E+F doubles) to out. */
+int fl_darcy_get_solution(fl_darcy *d, double *out);
+
+/* Device pointer of the same vector (valid until the next update_solution);
+ * the analogue of get_parallel_solution_vector (darcy_flow_mh.cc:502-505). */
+const double *fl_darcy_device_solution(fl_darcy *d);
+
+/* DarcyFlowMH_Steady::solution_precision (darcy_flow_mh.cc:463-482). */
+double fl_darcy_solution_precision(fl_darcy *d, double r_tol, double a_tol);
+
+/* Per-element mass balance of the element row of the MH system
+ * (darcy_flow_mh.cc:603-640): sum_i q_i + sum_vb v (p_e - lambda) - |e| cs f. */
+int fl_darcy_element_balance(fl_darcy *d, double *out);
+
+/* DarcyFlowMHOutput::water_balance (darcy_flow_mh_output.cc:671-773):
+ * per boundary region (by boundary sub-index) total, outflow, inflow. */
+int fl_darcy_water_balance(fl_darcy *d, double *balance, double *outflow, double *inflow);
+
+/* The condensed SPD edge system (the reference's schur2 system after
+ * SchurComplement::scale(-1), schur.cc:702-707): CSR with F+1 row pointers,
+ * nnz columns / values, and the rhs.  nnz via fl_darcy_edge_nnz. */
+int fl_darcy_edge_nnz(fl_darcy *d);
+int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs);
+
+/* ---- explicit upwind transport -------------------------------------------- */
+
+/* ConvectionTransport::ConvectionTransport (src/transport/transport.cc:105-188)
+ * with TransportOperatorSplitting's time governor: n_subst substances (1..4),
+ * mobile porosity por[E], cross_section cs[E], bc_conc[n_subst][NBC],
+ * init_conc[n_subst][E], end_time, save_step (output marks). */
+int fl_transport_create(fl_mesh *m, int n_subst, const double *por, const double *cs, const double *bc_conc,
+                        const double *init_conc, double end_time, double save_step, fl_transport **out);
+void fl_transport_free(fl_transport *t);
+
+/* TransportBase::set_velocity_field (transport_operator_splitting.hh:90-92):
+ * side fluxes [S] from host memory, or straight from a Darcy handle's device
+ * solution. */
+int fl_transport_set_velocity(fl_transport *t, const double *side_flux);
+int fl_transport_set_velocity_from_darcy(fl_transport *t, fl_darcy *d);
+
+/* TransportOperatorSplitting::update_solution (transport_operator_splitting.cc:244-287):
+ * ConvectionTransport::set_target_time + compute_one_step until target_time.
+ * steps (optional) receives the number of steps taken. */
+int fl_transport_update_solution(fl_transport *t, double target_time, int *steps);
+
+/* ConvectionTransport::get_concentration_matrix (transport.h:107-173): out[n_subst][E]. */
+int fl_transport_get_conc(fl_transport *t, double *out);
+
+/* time(), dt(), cfl_max_step */
+int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl);
+
+/* MassBalance::calculate (mass_balance.cc:79-239) state after the last step:
+ * bal/plus/minus[n_subst][n_boundary_regions], mass[n_subst][n_bulk_regions],
+ * totals[n_subst][4] = initial mass, integrated boundary flux, integrated
+ * sources, current mass. */
+int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass,
+                              double *totals);
+
+/* Scaled transport matrix tm = I + dt*A (create_transport_matrix_mpi,
+ * transport.cc:788-943, after the scaling at :628-648). */
+int fl_transport_nnz(fl_transport *t);
+int fl_transport_matrix(fl_transport *t, int *ptr, int *col, double *val);
+
+/* Device time (ms) of the transport steps so far and their count. */
+int fl_transport_stats(fl_transport *t, double *ms_steps, int *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

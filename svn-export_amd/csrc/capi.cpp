@@ -1,0 +1,303 @@
+// C ABI over the host mesh module and the HIP solvers (include/flow123d_gpu.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/flow123d_gpu.h"
+#include "common.hh"
+#include "darcy.hh"
+#include "mesh.hh"
+#include "transport.hh"
+
+struct fl_mesh {
+    fl::HostMesh m;
+};
+struct fl_darcy {
+    std::unique_ptr<fl::DarcyGPU> g;
+    fl_mesh *mesh;
+};
+struct fl_transport {
+    std::unique_ptr<fl::TransportGPU> g;
+    fl_mesh *mesh;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F &&f) {
+    try {
+        return f();
+    } catch (const fl::Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+static int check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) throw fl::Error(-6, "no HIP device available (the GPU path has no CPU fallback)");
+    return 0;
+}
+
+extern "C" {
+
+const char *fl_last_error(void) { return g_err.c_str(); }
+
+int fl_mesh_read_gmsh(const char *path, fl_mesh **out) {
+    return guard([&] {
+        auto *mh = new fl_mesh();
+        std::string err = fl::read_gmsh(path, mh->m);
+        if (err.empty()) err = fl::build_topology(mh->m);
+        if (!err.empty()) {
+            delete mh;
+            throw fl::Error(-1, err);
+        }
+        *out = mh;
+        return 0;
+    });
+}
+
+int fl_mesh_from_arrays(int n_nodes, const double *xyz, int n_elem, const int *elem_id, const int *gmsh_type,
+                        const int *region_id, const int *nodes4, int n_phys, const int *phys_id,
+                        const char *const *phys_name, fl_mesh **out) {
+    return guard([&] {
+        auto *mh = new fl_mesh();
+        std::string err = fl::mesh_from_arrays(mh->m, n_nodes, xyz, n_elem, elem_id, gmsh_type, region_id, nodes4,
+                                               n_phys, phys_id, phys_name);
+        if (err.empty()) err = fl::build_topology(mh->m);
+        if (!err.empty()) {
+            delete mh;
+            throw fl::Error(-1, err);
+        }
+        *out = mh;
+        return 0;
+    });
+}
+
+void fl_mesh_free(fl_mesh *m) { delete m; }
+
+int fl_mesh_counts(const fl_mesh *mh, int *c) {
+    const fl::HostMesh &m = mh->m;
+    c[0] = m.n_nodes(); c[1] = m.n_elements(); c[2] = m.n_bc_elements(); c[3] = m.n_sides(); c[4] = m.n_edges();
+    c[5] = m.n_boundaries(); c[6] = m.n_vb(); c[7] = (int)m.regions.size(); c[8] = m.n_file_bc;
+    return 0;
+}
+
+int fl_mesh_topology(const fl_mesh *mh, int *elem_dim, int *elem_id, int *elem_region, int *elem_nodes,
+                     int *elem_edge, int *elem_bnd, int *edge_ptr, int *edge_side, int *vb_elem, int *vb_edge,
+                     int *bnd_edge, int *bnd_bcele, int *bc_region) {
+    const fl::HostMesh &m = mh->m;
+    const int E = m.n_elements();
+    for (int e = 0; e < E; e++) {
+        elem_dim[e] = m.el_dim[e];
+        elem_id[e] = m.el_id[e];
+        elem_region[e] = m.regions[m.el_region[e]].id;
+        for (int k = 0; k < 4; k++) {
+            elem_nodes[4 * e + k] = m.el_nodes[4 * e + k];
+            bool has = k <= m.el_dim[e];
+            elem_edge[4 * e + k] = has ? m.side_edge[m.side_off[e] + k] : -1;
+            elem_bnd[4 * e + k] = has ? m.side_bnd[m.side_off[e] + k] : -1;
+        }
+    }
+    std::memcpy(edge_ptr, m.edge_ptr.data(), m.edge_ptr.size() * sizeof(int));
+    std::memcpy(edge_side, m.edge_side.data(), m.edge_side.size() * sizeof(int));
+    std::memcpy(vb_elem, m.vb_elem.data(), m.vb_elem.size() * sizeof(int));
+    std::memcpy(vb_edge, m.vb_edge.data(), m.vb_edge.size() * sizeof(int));
+    std::memcpy(bnd_edge, m.bnd_edge.data(), m.bnd_edge.size() * sizeof(int));
+    std::memcpy(bnd_bcele, m.bnd_bcele.data(), m.bnd_bcele.size() * sizeof(int));
+    for (int b = 0; b < m.n_bc_elements(); b++) bc_region[b] = m.regions[m.bc_region[b]].id;
+    return 0;
+}
+
+int fl_mesh_regions(const fl_mesh *mh, int *ids, int *is_boundary, int *sub_idx, char *buf, int buflen) {
+    const fl::HostMesh &m = mh->m;
+    std::string all;
+    for (size_t r = 0; r < m.regions.size(); r++) {
+        ids[r] = m.regions[r].id;
+        is_boundary[r] = m.regions[r].boundary;
+        sub_idx[r] = m.regions[r].sub_idx;
+        all += m.regions[r].label + "\n";
+    }
+    std::snprintf(buf, buflen, "%s", all.c_str());
+    return 0;
+}
+
+int fl_darcy_create(fl_mesh *m, fl_darcy **out) {
+    return guard([&] {
+        check_device();
+        auto *d = new fl_darcy();
+        d->mesh = m;
+        try {
+            d->g.reset(new fl::DarcyGPU(m->m));
+        } catch (...) {
+            delete d;
+            throw;
+        }
+        *out = d;
+        return 0;
+    });
+}
+
+void fl_darcy_free(fl_darcy *d) { delete d; }
+
+int fl_darcy_set_data(fl_darcy *d, const double *K, const double *cs, const double *aniso, const double *sigma,
+                      const double *f, const int *bc_type, const double *bc_pressure, const double *bc_flux,
+                      const double *bc_robin_sigma) {
+    return guard([&] {
+        d->g->set_data(K, cs, aniso, sigma, f, bc_type, bc_pressure, bc_flux, bc_robin_sigma);
+        return 0;
+    });
+}
+
+int fl_darcy_set_conductivity(fl_darcy *d, const double *K) {
+    return guard([&] {
+        d->g->set_conductivity(K);
+        return 0;
+    });
+}
+
+int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info *info) {
+    return guard([&] {
+        double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
+        int max_it = o ? o->max_it : 10000;  // solve.cc:69-84 defaults
+        fl::SolveStats st = d->g->solve(rtol, atol, max_it);
+        if (info) {
+            info->iterations = st.iterations;
+            info->converged = st.converged;
+            info->residual = st.residual;
+            info->rhs_norm = st.bnorm;
+            info->ms_assembly = st.ms_assembly;
+            info->ms_solve = st.ms_cg;
+            info->ms_resolve = st.ms_resolve;
+            info->ms_total = st.ms_total;
+        }
+        return 0;
+    });
+}
+
+int fl_darcy_get_solution(fl_darcy *d, double *out) {
+    return guard([&] {
+        d->g->get_solution(out);
+        return 0;
+    });
+}
+
+const double *fl_darcy_device_solution(fl_darcy *d) { return d->g->device_solution(); }
+
+double fl_darcy_solution_precision(fl_darcy *d, double r_tol, double a_tol) {
+    double v = -1.0;
+    guard([&] {
+        v = d->g->solution_precision(r_tol, a_tol);
+        return 0;
+    });
+    return v;
+}
+
+int fl_darcy_element_balance(fl_darcy *d, double *out) {
+    return guard([&] {
+        d->g->element_balance(out);
+        return 0;
+    });
+}
+
+int fl_darcy_water_balance(fl_darcy *d, double *b, double *p, double *m) {
+    return guard([&] {
+        d->g->water_balance(b, p, m);
+        return 0;
+    });
+}
+
+int fl_darcy_edge_nnz(fl_darcy *d) { return d->g->nnz(); }
+
+int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs) {
+    return guard([&] {
+        d->g->get_edge_system(ptr, col, val, rhs);
+        return 0;
+    });
+}
+
+int fl_transport_create(fl_mesh *m, int n_subst, const double *por, const double *cs, const double *bc_conc,
+                        const double *init_conc, double end_time, double save_step, fl_transport **out) {
+    return guard([&] {
+        check_device();
+        if (n_subst < 1 || n_subst > 4) throw fl::Error(-5, "n_subst must be in 1..4");
+        auto *t = new fl_transport();
+        t->mesh = m;
+        try {
+            t->g.reset(new fl::TransportGPU(m->m, n_subst, por, cs, bc_conc, init_conc, end_time, save_step));
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+        return 0;
+    });
+}
+
+void fl_transport_free(fl_transport *t) { delete t; }
+
+int fl_transport_set_velocity(fl_transport *t, const double *side_flux) {
+    return guard([&] {
+        t->g->set_velocity_host(side_flux);
+        return 0;
+    });
+}
+
+int fl_transport_set_velocity_from_darcy(fl_transport *t, fl_darcy *d) {
+    return guard([&] {
+        FL_HIP(hipStreamSynchronize(d->g->stream()));
+        t->g->set_velocity_device(d->g->device_solution());
+        return 0;
+    });
+}
+
+int fl_transport_update_solution(fl_transport *t, double target, int *steps) {
+    return guard([&] {
+        int s = t->g->update_solution(target);
+        if (steps) *steps = s;
+        return 0;
+    });
+}
+
+int fl_transport_get_conc(fl_transport *t, double *out) {
+    return guard([&] {
+        t->g->get_conc(out);
+        return 0;
+    });
+}
+
+int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl) {
+    if (time) *time = t->g->time();
+    if (dt) *dt = t->g->dt();
+    if (cfl) *cfl = t->g->cfl();
+    return 0;
+}
+
+int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass,
+                              double *totals) {
+    return guard([&] {
+        t->g->mass_balance(bal, plus, minus, mass, totals);
+        return 0;
+    });
+}
+
+int fl_transport_nnz(fl_transport *t) { return t->g->nnz(); }
+
+int fl_transport_matrix(fl_transport *t, int *ptr, int *col, double *val) {
+    return guard([&] {
+        t->g->get_matrix(ptr, col, val);
+        return 0;
+    });
+}
+
+int fl_transport_stats(fl_transport *t, double *ms, int *steps) {
+    if (ms) *ms = t->g->stats().ms_steps;
+    if (steps) *steps = t->g->stats().steps;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// GPU steady mixed-hybrid Darcy solver (DarcyFlowMH_Steady::update_solution,
+// src/flow/darcy_flow_mh.cc:375-433, n_schurs = 2).
+//
+// Per solve:
+//  1. k_assemble<D>: one thread per element of dimension D builds the RT0
+//     local matrix, inverts it and condenses the element onto its connected
+//     edges (own sides + vb neighbour edges), writing a dense local block and
+//     a local right-hand side.  This is the per-element closed form of the
+//     reference's two Schur stages (schur.cc:520-696 with
+//     darcy_flow_mh.cc:1009-1131): for a = A^-1 1, s = 1^T a, V = sum of vb
+//     coefficients, w = [c.*a ; v], D = s + V, h = |e| cs f - 1^T A^-1 g:
+//        S_e = [C A^-1 C, 0; 0, diag(v)] + diag(Dirichlet/Robin terms) - w w^T / D
+//        r_e = [C A^-1 g ; 0] - rhs_edge + w h / D
+//  2. k_gather_csr / k_gather_rhs: each CSR entry / rhs entry sums its
+//     precomputed list of block contributions in element order
+//     (deterministic, no atomics).
+//  3. Jacobi-preconditioned CG on the SPD edge system, two fused kernels per
+//     iteration with last-block reductions and a device-side stop flag.
+//  4. k_resolve<D>: element pressure and side fluxes from the edge traces
+//     (SchurComplement::resolve, schur.cc:714-773), written in the
+//     reference's [q | p | lambda] layout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mesh.hh"
+
+namespace fl {
+
+struct CGState {
+    double rz, rz_old, pq, rr, tol2, bnorm2;
+    int it, max_it, done, pad;
+};
+
+struct SolveStats {
+    int iterations = 0;
+    int converged = 0;
+    double residual = 0, bnorm = 0;
+    double ms_assembly = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
+};
+
+// Device view of the mesh + pattern + fields, passed to kernels by value.
+struct DarcyDev {
+    int E, S, F, NBC, nnz;
+    int dim_start[4];
+    const double *xyz;
+    const int *el_nodes, *perm, *side_off, *side_edge, *side_bnd, *bnd_bcele;
+    const double *bc_measure;
+    const int *el_vb_ptr, *el_vb, *vb_edge, *vb_hel;
+    const double *vb_hmeas;
+    const int *conn_off, *conn, *blk_off;
+    const int *csr_ptr, *csr_col, *diag_pos, *cptr, *cidx, *rptr, *ridx;
+    // fields
+    const double *K, *cs, *sigma, *f, *aniso;
+    const int *bc_type;
+    const double *bc_p, *bc_flux, *bc_sigma;
+    // work
+    double *blk, *rhsc, *val, *rhs, *dinv, *sol;
+};
+
+class DarcyGPU {
+   public:
+    explicit DarcyGPU(const HostMesh &m);
+    ~DarcyGPU();
+    // per-element fields [E] (aniso: 9*E or null), per-bc-element BC data [NBC]
+    void set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
+                  const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma);
+    void set_conductivity(const double *K);  // host array [E]
+    SolveStats solve(double rtol, double atol, int max_it);
+    void get_solution(double *out) const;                   // [S+E+F]
+    void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
+    void element_balance(double *out) const;                // [E]
+    void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region
+    double solution_precision(double rtol, double atol) const;
+    int nnz() const { return nnz_; }
+    const HostMesh &mesh() const { return m_; }
+    hipStream_t stream() const { return stream_; }
+    const double *device_solution() const { return d_sol_; }
+    // CG workspace access for mixed precision / batched solvers
+    const DarcyDev &dev() const { return dv_; }
+
+   private:
+    void build_pattern();
+    void assemble();
+    int cg(double rtol, double atol, int max_it, SolveStats &st);
+    void resolve();
+
+    const HostMesh &m_;
+    hipStream_t stream_ = nullptr;
+    DarcyDev dv_{};
+    int nnz_ = 0;
+    long blk_total_ = 0;
+    std::vector<void *> owned_;
+    double *d_K_ = nullptr, *d_cs_ = nullptr, *d_sigma_ = nullptr, *d_f_ = nullptr, *d_aniso_ = nullptr;
+    int *d_bc_type_ = nullptr;
+    double *d_bc_p_ = nullptr, *d_bc_flux_ = nullptr, *d_bc_sigma_ = nullptr;
+    double *d_x_ = nullptr, *d_r_ = nullptr, *d_z_ = nullptr, *d_p0_ = nullptr, *d_p1_ = nullptr, *d_q_ = nullptr;
+    double *d_partial_ = nullptr, *d_sol_ = nullptr;
+    CGState *d_state_ = nullptr;
+    unsigned *d_count_ = nullptr;
+    bool have_data_ = false;
+    std::vector<double> h_bc_measure_;
+    std::vector<double> h_cs_, h_f_, h_sigma_;
+};
+
+}  // namespace fl

@@ -1,0 +1,683 @@
+// GPU steady MH Darcy solve.  See darcy.hh for the method and the reference
+// functions each phase replaces.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "common.hh"
+#include "darcy.hh"
+#include "rt0_local.hh"
+
+namespace fl {
+
+// ---------------------------------------------------------------------------
+// Element condensation (assembly_steady_mh_matrix darcy_flow_mh.cc:525-694 +
+// make_schur1/2 :1009-1131 in closed form)
+// ---------------------------------------------------------------------------
+template <int D>
+struct Local {
+    double inv[(D + 1) * (D + 1)];
+    double a[D + 1], Ag[D + 1], c[D + 1], dlt[D + 1], rf[D + 1];
+    double s, h, meas;
+};
+
+template <int D>
+__device__ __forceinline__ void element_local(const DarcyDev &dv, int e, Local<D> &L) {
+    Elem<D> el;
+    for (int i = 0; i <= D; i++) {
+        int n = dv.el_nodes[4 * e + i];
+        el.P[i] = V3{dv.xyz[3 * n], dv.xyz[3 * n + 1], dv.xyz[3 * n + 2]};
+    }
+    double K = dv.K[e], cs = dv.cs[e];
+    double scale = 1 / K / cs;  // mh_fe_values.cc:38
+    double R[9];
+    const double *Rp = nullptr;
+    if (dv.aniso) {
+        inv3x3(dv.aniso + 9 * (size_t)e, R);
+        Rp = R;
+    }
+    double loc[(D + 1) * (D + 1)];
+    local_matrix<D>(el, scale, Rp, loc);
+    small_inverse<D + 1>(loc, L.inv);
+    V3 ec = elem_centre<D>(el);
+    int so = dv.side_off[e];
+    double g[D + 1];
+    for (int i = 0; i <= D; i++) {
+        V3 sc = side_centre<D>(el, i);
+        g[i] = ec.z - sc.z;  // gravity term, darcy_flow_mh.cc:565
+        L.c[i] = 1.0;
+        L.dlt[i] = 0.0;
+        L.rf[i] = 0.0;
+        int b = dv.side_bnd[so + i];
+        if (b >= 0) {
+            int bce = dv.bnd_bcele[b];
+            int type = dv.bc_type[bce];
+            double bm = dv.bc_measure[bce];
+            if (type == 1) {  // dirichlet :575-581
+                L.c[i] = 0.0;
+                g[i] -= dv.bc_p[bce];
+                L.dlt[i] = 1.0;
+                L.rf[i] = dv.bc_p[bce];
+            } else if (type == 2) {  // neumann :582-584
+                L.rf[i] = -(dv.bc_flux[bce] * bm * cs);
+            } else if (type == 3) {  // robin :586-591
+                L.rf[i] = -(-bm * dv.bc_sigma[bce] * dv.bc_p[bce] * cs);
+                L.dlt[i] = -(-bm * dv.bc_sigma[bce] * cs);
+            }
+        }
+    }
+    double s = 0.0;
+    for (int i = 0; i <= D; i++) {
+        double ai = 0.0, agi = 0.0;
+        for (int j = 0; j <= D; j++) {
+            ai += L.inv[i * (D + 1) + j];
+            agi += L.inv[i * (D + 1) + j] * g[j];
+        }
+        L.a[i] = ai;
+        L.Ag[i] = agi;
+        s += ai;
+    }
+    L.s = s;
+    L.meas = elem_measure<D>(el);
+    double sumAg = 0.0;
+    for (int i = 0; i <= D; i++) sumAg += L.Ag[i];
+    L.h = L.meas * cs * dv.f[e] - sumAg;
+}
+
+__device__ __forceinline__ double vb_coef(const DarcyDev &dv, int e, int n) {
+    // darcy_flow_mh.cc:630-632: sigma(lower) * |higher side| * cs(higher)
+    return dv.sigma[e] * dv.vb_hmeas[n] * dv.cs[dv.vb_hel[n]];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_assemble(const DarcyDev dv) {
+    int idx = dv.dim_start[D - 1] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= dv.dim_start[D]) return;
+    int e = dv.perm[idx];
+    Local<D> L;
+    element_local<D>(dv, e, L);
+    const int n0 = D + 1;
+    int vb0 = dv.el_vb_ptr[e], nvb = dv.el_vb_ptr[e + 1] - vb0;
+    double V = 0.0;
+    for (int k = 0; k < nvb; k++) V += vb_coef(dv, e, dv.el_vb[vb0 + k]);
+    double Dd = L.s + V;
+    double w[n0];
+    for (int i = 0; i < n0; i++) w[i] = L.c[i] * L.a[i];
+    const int m = n0 + nvb;
+    double *B = dv.blk + dv.blk_off[e];
+    double *r = dv.rhsc + dv.conn_off[e];
+    double hD = L.h / Dd;
+    for (int i = 0; i < n0; i++) {
+        for (int j = 0; j < n0; j++) {
+            double v = L.c[i] * L.c[j] * L.inv[i * n0 + j] - w[i] * w[j] / Dd;
+            if (i == j) v += L.dlt[i];
+            B[i * m + j] = v;
+        }
+        for (int k = 0; k < nvb; k++) {
+            double vk = vb_coef(dv, e, dv.el_vb[vb0 + k]);
+            double v = -w[i] * vk / Dd;
+            B[i * m + n0 + k] = v;
+            B[(n0 + k) * m + i] = v;
+        }
+        r[i] = L.c[i] * L.Ag[i] + L.rf[i] + w[i] * hD;
+    }
+    for (int k = 0; k < nvb; k++) {
+        double vk = vb_coef(dv, e, dv.el_vb[vb0 + k]);
+        for (int l = 0; l < nvb; l++) {
+            double vl = vb_coef(dv, e, dv.el_vb[vb0 + l]);
+            double v = -vk * vl / Dd;
+            if (k == l) v += vk;
+            B[(n0 + k) * m + n0 + l] = v;
+        }
+        r[n0 + k] = vk * hD;
+    }
+}
+
+__global__ void k_gather_csr(const DarcyDev dv) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < dv.nnz; k += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = dv.cptr[k]; j < dv.cptr[k + 1]; j++) s += dv.blk[dv.cidx[j]];
+        dv.val[k] = s;
+    }
+}
+
+__global__ void k_gather_rhs(const DarcyDev dv) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = dv.rptr[f]; j < dv.rptr[f + 1]; j++) s += dv.rhsc[dv.ridx[j]];
+        dv.rhs[f] = s;
+        double d = dv.val[dv.diag_pos[f]];
+        dv.dinv[f] = d != 0.0 ? 1.0 / d : 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned CG (KSPSolve, solve.cc:453) with fused kernels.
+// ---------------------------------------------------------------------------
+struct CGArgs {
+    int n;
+    const int *ptr, *col;
+    const double *val, *b, *dinv;
+    double *x, *r, *z, *q, *partial;
+    CGState *st;
+    unsigned *count;
+};
+
+// Last block of the grid folds the per-block partials (fixed order).
+__device__ __forceinline__ bool last_block(unsigned *count) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(count, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    return is_last;
+}
+
+__global__ void __launch_bounds__(256) k_cg_init(CGArgs a, double rtol2, double atol2, int max_it) {
+    __shared__ double sh[8];
+    double rz = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double b = a.b[i];
+        a.x[i] = 0.0;
+        a.r[i] = b;
+        double z = a.dinv[i] * b;
+        a.z[i] = z;
+        a.q[i] = 0.0;
+        rz += b * z;
+        rr += b * b;
+    }
+    double s1 = block_sum(rz, sh), s2 = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        a.partial[2 * blockIdx.x] = s1;
+        a.partial[2 * blockIdx.x + 1] = s2;
+    }
+    if (last_block(a.count) && threadIdx.x == 0) {
+        double t1 = 0.0, t2 = 0.0;
+        for (unsigned g = 0; g < gridDim.x; g++) { t1 += a.partial[2 * g]; t2 += a.partial[2 * g + 1]; }
+        CGState *st = a.st;
+        st->rz = t1;
+        st->rz_old = 1.0;
+        st->rr = t2;
+        st->bnorm2 = t2;
+        st->tol2 = fmax(rtol2 * t2, atol2);
+        st->it = 0;
+        st->max_it = max_it;
+        st->done = (t2 <= st->tol2) ? 1 : 0;
+        *a.count = 0;
+    }
+}
+
+// p_out = z + beta p_in ; q = A p_out (gathered as z + beta p_in) ; pq partials
+__global__ void __launch_bounds__(256) k_cg_spmv(CGArgs a, const double *__restrict__ p_in, double *__restrict__ p_out) {
+    __shared__ double sh[8];
+    const CGState *st = a.st;
+    if (st->done) return;
+    double beta = st->it == 0 ? 0.0 : st->rz / st->rz_old;
+    double pq = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = a.ptr[i]; k < a.ptr[i + 1]; k++) {
+            int c = a.col[k];
+            s += a.val[k] * (a.z[c] + beta * p_in[c]);
+        }
+        double p = a.z[i] + beta * p_in[i];
+        p_out[i] = p;
+        a.q[i] = s;
+        pq += p * s;
+    }
+    double t = block_sum(pq, sh);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = t;
+    if (last_block(a.count) && threadIdx.x == 0) {
+        double tot = 0.0;
+        for (unsigned g = 0; g < gridDim.x; g++) tot += a.partial[g];
+        a.st->pq = tot;
+        *a.count = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cg_update(CGArgs a, const double *__restrict__ p) {
+    __shared__ double sh[8];
+    CGState *st = a.st;
+    if (st->done) return;
+    double alpha = st->rz / st->pq;
+    double rz = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        a.x[i] += alpha * p[i];
+        double r = a.r[i] - alpha * a.q[i];
+        a.r[i] = r;
+        double z = a.dinv[i] * r;
+        a.z[i] = z;
+        rz += r * z;
+        rr += r * r;
+    }
+    double s1 = block_sum(rz, sh), s2 = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        a.partial[2 * blockIdx.x] = s1;
+        a.partial[2 * blockIdx.x + 1] = s2;
+    }
+    if (last_block(a.count) && threadIdx.x == 0) {
+        double t1 = 0.0, t2 = 0.0;
+        for (unsigned g = 0; g < gridDim.x; g++) { t1 += a.partial[2 * g]; t2 += a.partial[2 * g + 1]; }
+        st->rz_old = st->rz;
+        st->rz = t1;
+        st->rr = t2;
+        st->it += 1;
+        if (t2 <= st->tol2 || st->it >= st->max_it || !(t2 == t2)) st->done = 1;
+        *a.count = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (SchurComplement::resolve, schur.cc:714-773)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double *__restrict__ lam) {
+    int idx = dv.dim_start[D - 1] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= dv.dim_start[D]) return;
+    int e = dv.perm[idx];
+    Local<D> L;
+    element_local<D>(dv, e, L);
+    const int n0 = D + 1;
+    int so = dv.side_off[e];
+    int vb0 = dv.el_vb_ptr[e], nvb = dv.el_vb_ptr[e + 1] - vb0;
+    double V = 0.0, wl = 0.0;
+    for (int k = 0; k < nvb; k++) {
+        int n = dv.el_vb[vb0 + k];
+        double vk = vb_coef(dv, e, n);
+        V += vk;
+        wl += vk * lam[dv.vb_edge[n]];
+    }
+    double Dd = L.s + V;
+    double cl[n0];
+    double own = 0.0;
+    for (int i = 0; i < n0; i++) {
+        cl[i] = L.c[i] * lam[dv.side_edge[so + i]];
+        own += L.c[i] * L.a[i] * lam[dv.side_edge[so + i]];
+    }
+    double pe = (L.h + (own + wl)) / Dd;
+    for (int i = 0; i < n0; i++) {
+        double t = 0.0;
+        for (int j = 0; j < n0; j++) t += L.inv[i * n0 + j] * cl[j];
+        dv.sol[so + i] = L.Ag[i] + pe * L.a[i] - t;
+    }
+    dv.sol[dv.S + e] = pe;
+}
+
+__global__ void k_copy_lambda(const DarcyDev dv, const double *__restrict__ lam) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x)
+        dv.sol[dv.S + dv.E + f] = lam[f];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <class T>
+static T *up(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s) {
+    T *p = dev_upload(v.data(), v.size(), s);
+    own.push_back(p);
+    return p;
+}
+
+DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
+    FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    build_pattern();
+}
+
+DarcyGPU::~DarcyGPU() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *p : owned_) dev_free(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DarcyGPU::build_pattern() {
+    const HostMesh &m = m_;
+    const int E = m.n_elements(), S = m.n_sides(), F = m.n_edges(), V = m.n_vb(), NBC = m.n_bc_elements();
+    // elements grouped by dimension
+    std::vector<int> perm;
+    perm.reserve(E);
+    int ds[4] = {0, 0, 0, 0};
+    for (int d = 1; d <= 3; d++) {
+        for (int e = 0; e < E; e++)
+            if (m.el_dim[e] == d) perm.push_back(e);
+        ds[d] = (int)perm.size();
+    }
+    // connected edges per element: own sides then vb edges
+    std::vector<int> conn_off(E + 1, 0), blk_off(E + 1, 0);
+    long blk_total = 0;
+    for (int e = 0; e < E; e++) {
+        int mm = m.el_dim[e] + 1 + (m.el_vb_ptr[e + 1] - m.el_vb_ptr[e]);
+        conn_off[e + 1] = conn_off[e] + mm;
+        blk_off[e] = (int)blk_total;
+        blk_total += (long)mm * mm;
+    }
+    if (blk_total > 0x7fffffffL) throw Error(-3, "element block storage exceeds 2^31 entries");
+    blk_off[E] = (int)blk_total;
+    blk_total_ = blk_total;
+    std::vector<int> conn(conn_off[E]);
+    for (int e = 0; e < E; e++) {
+        int o = conn_off[e];
+        for (int i = 0; i <= m.el_dim[e]; i++) conn[o++] = m.side_edge[m.side_off[e] + i];
+        for (int k = m.el_vb_ptr[e]; k < m.el_vb_ptr[e + 1]; k++) conn[o++] = m.vb_edge[m.el_vb[k]];
+    }
+    // edge -> elements containing it in conn
+    std::vector<int> ee_ptr(F + 1, 0);
+    for (int i = 0; i < conn_off[E]; i++) ee_ptr[conn[i] + 1]++;
+    for (int f = 0; f < F; f++) ee_ptr[f + 1] += ee_ptr[f];
+    std::vector<int> ee(conn_off[E]), fillp(ee_ptr.begin(), ee_ptr.end() - 1);
+    for (int e = 0; e < E; e++)
+        for (int i = conn_off[e]; i < conn_off[e + 1]; i++) ee[fillp[conn[i]]++] = e;
+    // CSR rows: union of connected edges of the elements touching the row
+    std::vector<int> csr_ptr(F + 1, 0), csr_col;
+    csr_col.reserve((size_t)conn_off[E] * 4);
+    std::vector<int> tmp;
+    for (int f = 0; f < F; f++) {
+        tmp.clear();
+        for (int k = ee_ptr[f]; k < ee_ptr[f + 1]; k++) {
+            int e = ee[k];
+            for (int i = conn_off[e]; i < conn_off[e + 1]; i++) tmp.push_back(conn[i]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        csr_col.insert(csr_col.end(), tmp.begin(), tmp.end());
+        csr_ptr[f + 1] = (int)csr_col.size();
+    }
+    nnz_ = (int)csr_col.size();
+    std::vector<int> diag(F);
+    for (int f = 0; f < F; f++) {
+        auto b = csr_col.begin() + csr_ptr[f], en = csr_col.begin() + csr_ptr[f + 1];
+        diag[f] = (int)(std::lower_bound(b, en, f) - csr_col.begin());
+    }
+    // block contributions per CSR entry (counting sort keeps element order)
+    std::vector<int> ckey(blk_total);
+    for (int e = 0; e < E; e++) {
+        int mm = conn_off[e + 1] - conn_off[e];
+        for (int a = 0; a < mm; a++) {
+            int row = conn[conn_off[e] + a];
+            auto b = csr_col.begin() + csr_ptr[row], en = csr_col.begin() + csr_ptr[row + 1];
+            for (int bb = 0; bb < mm; bb++) {
+                int colv = conn[conn_off[e] + bb];
+                ckey[blk_off[e] + a * mm + bb] = (int)(std::lower_bound(b, en, colv) - csr_col.begin());
+            }
+        }
+    }
+    std::vector<int> cptr(nnz_ + 1, 0), cidx(blk_total);
+    for (long i = 0; i < blk_total; i++) cptr[ckey[i] + 1]++;
+    for (int k = 0; k < nnz_; k++) cptr[k + 1] += cptr[k];
+    {
+        std::vector<int> fp(cptr.begin(), cptr.end() - 1);
+        for (long i = 0; i < blk_total; i++) cidx[fp[ckey[i]]++] = (int)i;
+    }
+    std::vector<int>().swap(ckey);
+    // rhs contributions per edge
+    std::vector<int> rptr(F + 1, 0), ridx(conn_off[E]);
+    for (int i = 0; i < conn_off[E]; i++) rptr[conn[i] + 1]++;
+    for (int f = 0; f < F; f++) rptr[f + 1] += rptr[f];
+    {
+        std::vector<int> fp(rptr.begin(), rptr.end() - 1);
+        for (int i = 0; i < conn_off[E]; i++) ridx[fp[conn[i]]++] = i;
+    }
+    // vb: higher-dim element and measure of its side on the vb edge
+    std::vector<int> vb_hel(std::max(V, 1), 0);
+    std::vector<double> vb_hmeas(std::max(V, 1), 0.0);
+    for (int n = 0; n < V; n++) {
+        int sd = m.edge_side[m.edge_ptr[m.vb_edge[n]]];  // Neighbour::side(), neighbours_impl.hh:16-20
+        int he = m.side_elem[sd];
+        vb_hel[n] = he;
+        vb_hmeas[n] = side_measure(m, he, sd - m.side_off[he]);
+    }
+    h_bc_measure_.assign(std::max(NBC, 1), 0.0);
+    for (int b = 0; b < NBC; b++) h_bc_measure_[b] = element_measure(m, m.bc_dim[b], &m.bc_nodes[4 * b]);
+
+    hipStream_t s = stream_;
+    DarcyDev &d = dv_;
+    d.E = E; d.S = S; d.F = F; d.NBC = NBC; d.nnz = nnz_;
+    for (int i = 0; i < 4; i++) d.dim_start[i] = ds[i];
+    d.xyz = up(owned_, m.xyz, s);
+    d.el_nodes = up(owned_, m.el_nodes, s);
+    d.perm = up(owned_, perm, s);
+    d.side_off = up(owned_, m.side_off, s);
+    d.side_edge = up(owned_, m.side_edge, s);
+    d.side_bnd = up(owned_, m.side_bnd, s);
+    std::vector<int> bb = m.bnd_bcele;
+    if (bb.empty()) bb.push_back(0);
+    d.bnd_bcele = up(owned_, bb, s);
+    d.bc_measure = up(owned_, h_bc_measure_, s);
+    d.el_vb_ptr = up(owned_, m.el_vb_ptr, s);
+    std::vector<int> elvb = m.el_vb, vbe = m.vb_edge;
+    if (elvb.empty()) { elvb.push_back(0); vbe.push_back(0); }
+    d.el_vb = up(owned_, elvb, s);
+    d.vb_edge = up(owned_, vbe, s);
+    d.vb_hel = up(owned_, vb_hel, s);
+    d.vb_hmeas = up(owned_, vb_hmeas, s);
+    d.conn_off = up(owned_, conn_off, s);
+    d.conn = up(owned_, conn, s);
+    d.blk_off = up(owned_, blk_off, s);
+    d.csr_ptr = up(owned_, csr_ptr, s);
+    d.csr_col = up(owned_, csr_col, s);
+    d.diag_pos = up(owned_, diag, s);
+    d.cptr = up(owned_, cptr, s);
+    d.cidx = up(owned_, cidx, s);
+    d.rptr = up(owned_, rptr, s);
+    d.ridx = up(owned_, ridx, s);
+    auto alloc_d = [&](size_t n) {
+        double *p = dev_alloc<double>(n);
+        owned_.push_back(p);
+        return p;
+    };
+    d.blk = alloc_d(blk_total);
+    d.rhsc = alloc_d(conn_off[E]);
+    d.val = alloc_d(nnz_);
+    d.rhs = alloc_d(F);
+    d.dinv = alloc_d(F);
+    d_sol_ = alloc_d((size_t)S + E + F);
+    d.sol = d_sol_;
+    d_x_ = alloc_d(F); d_r_ = alloc_d(F); d_z_ = alloc_d(F);
+    d_p0_ = alloc_d(F); d_p1_ = alloc_d(F); d_q_ = alloc_d(F);
+    d_partial_ = alloc_d(2 * 4096);
+    d_state_ = dev_alloc<CGState>(1);
+    owned_.push_back(d_state_);
+    d_count_ = dev_alloc<unsigned>(1);
+    owned_.push_back(d_count_);
+    FL_HIP(hipMemsetAsync(d_count_, 0, sizeof(unsigned), s));
+    d_K_ = alloc_d(E); d_cs_ = alloc_d(E); d_sigma_ = alloc_d(E); d_f_ = alloc_d(E);
+    d_bc_type_ = dev_alloc<int>(std::max(NBC, 1));
+    owned_.push_back(d_bc_type_);
+    d_bc_p_ = alloc_d(NBC); d_bc_flux_ = alloc_d(NBC); d_bc_sigma_ = alloc_d(NBC);
+    d.K = d_K_; d.cs = d_cs_; d.sigma = d_sigma_; d.f = d_f_; d.aniso = nullptr;
+    d.bc_type = d_bc_type_; d.bc_p = d_bc_p_; d.bc_flux = d_bc_flux_; d.bc_sigma = d_bc_sigma_;
+    FL_HIP(hipStreamSynchronize(s));
+}
+
+void DarcyGPU::set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
+                        const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma) {
+    const int E = m_.n_elements(), NBC = m_.n_bc_elements();
+    hipStream_t s = stream_;
+    FL_HIP(hipMemcpyAsync(d_K_, K, E * sizeof(double), hipMemcpyHostToDevice, s));
+    FL_HIP(hipMemcpyAsync(d_cs_, cs, E * sizeof(double), hipMemcpyHostToDevice, s));
+    FL_HIP(hipMemcpyAsync(d_sigma_, sigma, E * sizeof(double), hipMemcpyHostToDevice, s));
+    FL_HIP(hipMemcpyAsync(d_f_, f, E * sizeof(double), hipMemcpyHostToDevice, s));
+    h_cs_.assign(cs, cs + E);
+    h_f_.assign(f, f + E);
+    h_sigma_.assign(sigma, sigma + E);
+    if (aniso) {
+        if (!d_aniso_) {
+            d_aniso_ = dev_alloc<double>(9 * (size_t)E);
+            owned_.push_back(d_aniso_);
+        }
+        FL_HIP(hipMemcpyAsync(d_aniso_, aniso, 9 * (size_t)E * sizeof(double), hipMemcpyHostToDevice, s));
+        dv_.aniso = d_aniso_;
+    } else {
+        dv_.aniso = nullptr;
+    }
+    if (NBC) {
+        FL_HIP(hipMemcpyAsync(d_bc_type_, bc_type, NBC * sizeof(int), hipMemcpyHostToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_p_, bc_p, NBC * sizeof(double), hipMemcpyHostToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_flux_, bc_flux, NBC * sizeof(double), hipMemcpyHostToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_sigma_, bc_sigma, NBC * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    FL_HIP(hipStreamSynchronize(s));
+    have_data_ = true;
+}
+
+void DarcyGPU::set_conductivity(const double *K) {
+    FL_HIP(hipMemcpyAsync(d_K_, K, m_.n_elements() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void DarcyGPU::assemble() {
+    const DarcyDev &d = dv_;
+    for (int D = 1; D <= 3; D++) {
+        int n = d.dim_start[D] - d.dim_start[D - 1];
+        if (n <= 0) continue;
+        dim3 g(grid_for(n)), b(kBlock);
+        if (D == 1) hipLaunchKernelGGL(k_assemble<1>, g, b, 0, stream_, d);
+        if (D == 2) hipLaunchKernelGGL(k_assemble<2>, g, b, 0, stream_, d);
+        if (D == 3) hipLaunchKernelGGL(k_assemble<3>, g, b, 0, stream_, d);
+    }
+    int g1 = std::min(grid_for(d.nnz), 8192);
+    hipLaunchKernelGGL(k_gather_csr, dim3(g1), dim3(kBlock), 0, stream_, d);
+    int g2 = std::min(grid_for(d.F), 8192);
+    hipLaunchKernelGGL(k_gather_rhs, dim3(g2), dim3(kBlock), 0, stream_, d);
+    FL_HIP(hipGetLastError());
+}
+
+int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
+    CGArgs a;
+    a.n = dv_.F;
+    a.ptr = dv_.csr_ptr; a.col = dv_.csr_col; a.val = dv_.val; a.b = dv_.rhs; a.dinv = dv_.dinv;
+    a.x = d_x_; a.r = d_r_; a.z = d_z_; a.q = d_q_; a.partial = d_partial_; a.st = d_state_; a.count = d_count_;
+    int grid = std::min(grid_for(a.n), 4096);
+    FL_HIP(hipMemsetAsync(d_p0_, 0, a.n * sizeof(double), stream_));
+    hipLaunchKernelGGL(k_cg_init, dim3(grid), dim3(kBlock), 0, stream_, a, rtol * rtol, atol * atol, max_it);
+    double *pin = d_p0_, *pout = d_p1_;
+    CGState h{};
+    int launched = 0;
+    int chunk = 32;
+    while (true) {
+        for (int k = 0; k < chunk; k++) {
+            hipLaunchKernelGGL(k_cg_spmv, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pin, pout);
+            hipLaunchKernelGGL(k_cg_update, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pout);
+            std::swap(pin, pout);
+        }
+        launched += chunk;
+        FL_HIP(hipGetLastError());
+        FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+        if (h.done || launched >= max_it + chunk) break;
+        if (chunk < 256) chunk *= 2;
+    }
+    st.iterations = h.it;
+    st.residual = std::sqrt(h.rr);
+    st.bnorm = std::sqrt(h.bnorm2);
+    st.converged = (h.rr <= h.tol2) ? 1 : 0;
+    return h.it;
+}
+
+void DarcyGPU::resolve() {
+    const DarcyDev &d = dv_;
+    for (int D = 1; D <= 3; D++) {
+        int n = d.dim_start[D] - d.dim_start[D - 1];
+        if (n <= 0) continue;
+        dim3 g(grid_for(n)), b(kBlock);
+        if (D == 1) hipLaunchKernelGGL(k_resolve<1>, g, b, 0, stream_, d, (const double *)d_x_);
+        if (D == 2) hipLaunchKernelGGL(k_resolve<2>, g, b, 0, stream_, d, (const double *)d_x_);
+        if (D == 3) hipLaunchKernelGGL(k_resolve<3>, g, b, 0, stream_, d, (const double *)d_x_);
+    }
+    hipLaunchKernelGGL(k_copy_lambda, dim3(std::min(grid_for(d.F), 8192)), dim3(kBlock), 0, stream_, d,
+                       (const double *)d_x_);
+    FL_HIP(hipGetLastError());
+}
+
+SolveStats DarcyGPU::solve(double rtol, double atol, int max_it) {
+    if (!have_data_) throw Error(-4, "Darcy data not set");
+    SolveStats st;
+    hipEvent_t e0, e1, e2, e3;
+    FL_HIP(hipEventCreate(&e0)); FL_HIP(hipEventCreate(&e1)); FL_HIP(hipEventCreate(&e2)); FL_HIP(hipEventCreate(&e3));
+    FL_HIP(hipEventRecord(e0, stream_));
+    assemble();
+    FL_HIP(hipEventRecord(e1, stream_));
+    cg(rtol, atol, max_it, st);
+    FL_HIP(hipEventRecord(e2, stream_));
+    resolve();
+    FL_HIP(hipEventRecord(e3, stream_));
+    FL_HIP(hipEventSynchronize(e3));
+    float t01, t12, t23;
+    FL_HIP(hipEventElapsedTime(&t01, e0, e1));
+    FL_HIP(hipEventElapsedTime(&t12, e1, e2));
+    FL_HIP(hipEventElapsedTime(&t23, e2, e3));
+    st.ms_assembly = t01; st.ms_cg = t12; st.ms_resolve = t23; st.ms_total = t01 + t12 + t23;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+    return st;
+}
+
+void DarcyGPU::get_solution(double *out) const {
+    size_t n = (size_t)m_.n_sides() + m_.n_elements() + m_.n_edges();
+    FL_HIP(hipMemcpyAsync(out, d_sol_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void DarcyGPU::get_edge_system(int *ptr, int *col, double *val, double *rhs) const {
+    const int F = m_.n_edges();
+    FL_HIP(hipMemcpyAsync(ptr, dv_.csr_ptr, (F + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(col, dv_.csr_col, nnz_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(val, dv_.val, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(rhs, dv_.rhs, F * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void DarcyGPU::element_balance(double *out) const {
+    const HostMesh &m = m_;
+    const int E = m.n_elements(), S = m.n_sides();
+    std::vector<double> sol((size_t)S + E + m.n_edges());
+    get_solution(sol.data());
+    std::vector<double> vbm(m.n_vb());
+    for (int n = 0; n < m.n_vb(); n++) {
+        int sd = m.edge_side[m.edge_ptr[m.vb_edge[n]]];
+        int he = m.side_elem[sd];
+        vbm[n] = side_measure(m, he, sd - m.side_off[he]) * h_cs_[he];
+    }
+    for (int e = 0; e < E; e++) {
+        double s = 0.0;
+        for (int i = m.side_off[e]; i < m.side_off[e + 1]; i++) s += sol[i];
+        double pe = sol[S + e];
+        for (int k = m.el_vb_ptr[e]; k < m.el_vb_ptr[e + 1]; k++) {
+            int n = m.el_vb[k];
+            s += h_sigma_[e] * vbm[n] * (pe - sol[S + E + m.vb_edge[n]]);
+        }
+        out[e] = s - element_measure(m, m.el_dim[e], &m.el_nodes[4 * e]) * h_cs_[e] * h_f_[e];
+    }
+}
+
+void DarcyGPU::water_balance(double *bal, double *plus, double *minus) const {
+    const HostMesh &m = m_;
+    std::vector<double> sol((size_t)m.n_sides() + m.n_elements() + m.n_edges());
+    get_solution(sol.data());
+    int nbr = m.n_boundary_regions;
+    std::fill(bal, bal + nbr, 0.0);
+    std::fill(plus, plus + nbr, 0.0);
+    std::fill(minus, minus + nbr, 0.0);
+    for (int b = 0; b < m.n_boundaries(); b++) {
+        double flux = sol[m.edge_side[m.edge_ptr[m.bnd_edge[b]]]];
+        int r = m.regions[m.bc_region[m.bnd_bcele[b]]].sub_idx;
+        bal[r] += flux;
+        if (flux > 0) plus[r] += flux;
+        else minus[r] += flux;
+    }
+}
+
+double DarcyGPU::solution_precision(double rtol, double atol) const {
+    // darcy_flow_mh.cc:463-482 uses the norm of the first Schur complement's
+    // rhs; the condensed system's rhs norm is reported here.
+    std::vector<double> rhs(m_.n_edges());
+    FL_HIP(hipMemcpyAsync(rhs.data(), dv_.rhs, rhs.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    double bn = 0.0;
+    for (double v : rhs) bn += v * v;
+    return std::max(atol, rtol * std::sqrt(bn));
+}
+
+}  // namespace fl

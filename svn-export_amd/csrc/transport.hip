@@ -1,0 +1,584 @@
+// GPU explicit upwind transport.  See transport.hh.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+
+#include "common.hh"
+#include "transport.hh"
+
+namespace fl {
+
+// ---------------------------------------------------------------------------
+// TimeGovernor restated (time_governor.cc:75-117, 215-237, 261-355,
+// time_governor.hh:205-211, 347-355; TimeMarks::next time_marks.cc:125-132)
+// ---------------------------------------------------------------------------
+static const double kInf = std::numeric_limits<double>::infinity();
+static const double kEps = std::numeric_limits<double>::epsilon();
+
+void TimeGovernorR::init(double end, double save_step) {
+    time = 0.0;
+    end_time = end;
+    end_fixed = 0.0;
+    max_ts = end;
+    time_step = end;
+    last_time_step = 0.0;
+    upper = kInf;
+    lower = kEps;
+    min_ts = kEps;
+    marks.clear();
+    add_mark(0.0);
+    add_mark(end);
+    for (double t = 0.0; t <= end * (1 + 1e-12); t += save_step) add_mark(t);
+}
+bool TimeGovernorR::lt(double o) const {
+    double b = o - 16 * kEps * std::max(std::fabs(time), std::fabs(o));
+    return !(time >= b);
+}
+bool TimeGovernorR::ge(double o) const {
+    double b = o - 16 * kEps * std::max(std::fabs(time), std::fabs(o));
+    return time >= b;
+}
+bool TimeGovernorR::is_end() const { return ge(end_time) || time == kInf; }
+void TimeGovernorR::add_mark(double t) {
+    auto it = std::lower_bound(marks.begin(), marks.end(), t);
+    if (it != marks.end() && *it == t) return;
+    marks.insert(it, t);
+}
+double TimeGovernorR::next_mark() const {
+    auto it = std::lower_bound(marks.begin(), marks.end(), time);
+    while (it != marks.end() && !lt(*it)) ++it;
+    return it == marks.end() ? kInf : *it;
+}
+double TimeGovernorR::estimate_dt() const {
+    if (time == kInf || is_end()) return 0.0;
+    if (lt(end_fixed)) return fixed_dt;
+    double full = next_mark() - time;
+    double est = std::min(full, upper);
+    est = std::max(est, lower);
+    int n = (int)std::ceil(full / est);
+    int nf = (int)std::floor(full / est);
+    if (std::fabs(full / est - nf) < 1e-14) n = nf;
+    return full / n;
+}
+int TimeGovernorR::set_upper_constraint(double u) {
+    if (upper < u) return -1;
+    if (lower <= u) { upper = u; return 0; }
+    return 1;
+}
+double TimeGovernorR::fix_dt_until_mark() {
+    end_fixed = -kInf;
+    fixed_dt = estimate_dt();
+    dt_fixed_now = true;
+    return end_fixed = next_mark();
+}
+void TimeGovernorR::next_time() {
+    if (time == kInf || is_end()) return;
+    if (lt(end_fixed)) {
+        fixed_dt = (end_fixed - time) / std::round((end_fixed - time) / fixed_dt);
+        last_time_step = time_step;
+        time_step = fixed_dt;
+        if (dt_fixed_now) { dt_fixed_now = false; dt_changed = (last_time_step != time_step); }
+        else dt_changed = false;
+    } else {
+        last_time_step = time_step;
+        time_step = estimate_dt();
+        dt_changed = (last_time_step != time_step);
+    }
+    time += time_step;
+    level++;
+    upper = std::min(end_time - time, max_ts);
+    lower = min_ts;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+enum : int { kDiag = 0, kSame = 1, kVbLow = 2, kVbHigh = 3 };
+
+struct TmDev {
+    int E, nsub, NBC;
+    const int *ptr, *col, *kind, *s1, *s2;
+    const int *side_off, *side_edge, *side_bnd, *bnd_bcele, *edge_ptr, *edge_side;
+    const int *el_vb_ptr, *el_vb, *vb_hside, *diag, *bnd_rep;
+    const double *meas, *cs, *por, *flux, *edge_flow, *bc_conc;
+    double *val, *bcv, *partial, *mb;
+    unsigned *count;
+};
+
+__device__ __forceinline__ bool tm_last_block(unsigned *count) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(count, 1u) == gridDim.x - 1);
+    __syncthreads();
+    return is_last;
+}
+
+__global__ void k_edge_flow(const int F, const int *__restrict__ edge_ptr, const int *__restrict__ edge_side,
+                            const double *__restrict__ flux, double *__restrict__ edge_flow) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = edge_ptr[f]; k < edge_ptr[f + 1]; k++) {
+            double q = flux[edge_side[k]];
+            if (q > 0) s += q;
+        }
+        edge_flow[f] = s;
+    }
+}
+
+// transport.cc:841-909 for one row; diagonal accumulation in side order, then vb
+__device__ double row_diag(const TmDev &t, int i) {
+    double aii = 0.0;
+    double mcp = t.meas[i] * t.cs[i] * t.por[i];
+    for (int s = t.side_off[i]; s < t.side_off[i + 1]; s++) {
+        double q = t.flux[s];
+        if (q > 0.0) aii -= (q / mcp);
+    }
+    for (int k = t.el_vb_ptr[i]; k < t.el_vb_ptr[i + 1]; k++) {
+        double q = t.flux[t.vb_hside[t.el_vb[k]]];
+        if (q < 0.0) aii -= (-q) / mcp;
+    }
+    return aii;
+}
+
+__global__ void __launch_bounds__(256) k_tm_build(const TmDev t) {
+    __shared__ double sh[4];
+    double mx = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
+            int kind = t.kind[k];
+            double a = 0.0;
+            if (kind == kDiag) {
+                a = row_diag(t, i);
+                mx = fmax(mx, fabs(a));
+            } else if (kind == kSame) {
+                int si = t.s1[k], sj = t.s2[k];
+                double q = t.flux[si], q2 = t.flux[sj];
+                double ef = t.edge_flow[t.side_edge[si]];
+                if (q2 > 0.0 && q < 0.0) a = -(q * q2 / (ef * t.meas[i] * t.cs[i] * t.por[i]));
+                else a = 0;
+            } else if (kind == kVbLow) {
+                double q = t.flux[t.s1[k]];
+                a = q > 0.0 ? q / (t.meas[i] * t.cs[i] * t.por[i]) : 0;
+            } else {  // kVbHigh: row i is the higher-dimensional element
+                double q = t.flux[t.s1[k]];
+                a = q < 0.0 ? (-q) / (t.meas[i] * t.cs[i] * t.por[i]) : 0;
+            }
+            t.val[k] = a;
+        }
+    }
+    // max |a_ii| over the grid (CFL, transport.cc:911-921)
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
+        t.partial[blockIdx.x] = m;
+    }
+    if (tm_last_block(t.count) && threadIdx.x == 0) {
+        double m = 0.0;
+        for (unsigned g = 0; g < gridDim.x; g++) m = fmax(m, ((volatile double *)t.partial)[g]);
+        t.mb[0] = m;
+        *t.count = 0;
+    }
+}
+
+// set_boundary_conditions (transport.cc:455-495) followed by VecScale(est)
+__global__ void k_set_bc(const TmDev t, double est) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        for (int s = 0; s < t.nsub; s++) {
+            double acc = 0.0;
+            for (int sd = t.side_off[i]; sd < t.side_off[i + 1]; sd++) {
+                int b = t.side_bnd[sd];
+                if (b < 0) continue;
+                double q = t.flux[sd];
+                if (q < 0.0) {
+                    double aij = -(q / (t.meas[i] * t.cs[i] * t.por[i]));
+                    acc += t.bc_conc[(size_t)s * t.NBC + t.bnd_bcele[b]] * aij;
+                }
+            }
+            t.bcv[(size_t)i * t.nsub + s] = acc * est;
+        }
+    }
+}
+
+// first: tm = est*A + I ; else tm = (tm - I)*fac + I, bcv *= fac
+__global__ void k_tm_scale(const TmDev t, int first, double fac) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        int dk = t.diag[i];
+        for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
+            double v = t.val[k];
+            if (first) v = v * fac;
+            else v = (k == dk ? (v + -1.0) : v) * fac;
+            if (k == dk) v = v + 1.0;
+            t.val[k] = v;
+        }
+        if (!first)
+            for (int s = 0; s < t.nsub; s++) t.bcv[(size_t)i * t.nsub + s] *= fac;
+    }
+}
+
+// c_new = tm c_old + bcv (all substances), boundary mass flux reduction
+template <int M>
+__global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__restrict__ c_old,
+                                             double *__restrict__ c_new, double tdiff) {
+    __shared__ double sh[4][M > 0 ? M : 1];
+    double bf[M];
+    for (int s = 0; s < M; s++) bf[s] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        double acc[M];
+        for (int s = 0; s < M; s++) acc[s] = t.bcv[(size_t)i * M + s] + 0.0;
+        for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
+            double a = t.val[k];
+            const double *cj = c_old + (size_t)t.col[k] * M;
+            for (int s = 0; s < M; s++) acc[s] += a * cj[s];
+        }
+        for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
+        // boundary mass flux with the previous concentration (transport.cc:1299-1337)
+        for (int sd = t.side_off[i]; sd < t.side_off[i + 1]; sd++) {
+            int b = t.bnd_rep[sd];
+            if (b < 0) continue;
+            double wf = t.flux[sd];
+            int bce = t.bnd_bcele[b];
+            for (int s = 0; s < M; s++)
+                bf[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * c_old[(size_t)i * M + s];
+        }
+    }
+    for (int s = 0; s < M; s++) {
+        double v = bf[s];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][s] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int s = 0; s < M; s++) {
+            double v = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += sh[w][s];
+            t.partial[(size_t)blockIdx.x * M + s] = v;
+        }
+    if (tm_last_block(t.count) && threadIdx.x == 0) {
+        for (int s = 0; s < M; s++) {
+            double tot = 0.0;
+            for (unsigned g = 0; g < gridDim.x; g++) tot += ((volatile double *)t.partial)[(size_t)g * M + s];
+            t.mb[1 + s] += tot * tdiff;  // integrated_fluxes
+        }
+        *t.count = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+template <class T>
+static T *upv(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s) {
+    T *p = dev_upload(v.data(), v.size(), s);
+    own.push_back(p);
+    return p;
+}
+
+TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const double *cs, const double *bc_conc,
+                           const double *init_conc, double end_time, double save_step)
+    : m_(m), E_(m.n_elements()), nsub_(nsub), NBC_(m.n_bc_elements()) {
+    FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    const int E = E_, S = m.n_sides(), F = m.n_edges(), V = m.n_vb();
+    // matrix pattern: rows in the reference's insertion order, later entries
+    // overwrite earlier ones (INSERT_VALUES), then columns sorted per row.
+    struct Ent { int kind, s1, s2; };
+    std::vector<std::map<int, Ent>> rows(E);
+    std::vector<int> vb_hside(std::max(V, 1), 0);
+    for (int n = 0; n < V; n++) vb_hside[n] = m.edge_side[m.edge_ptr[m.vb_edge[n]]];
+    for (int i = 0; i < E; i++) {
+        for (int sd = m.side_off[i]; sd < m.side_off[i + 1]; sd++) {
+            if (m.side_bnd[sd] >= 0) continue;
+            int f = m.side_edge[sd];
+            for (int k = m.edge_ptr[f]; k < m.edge_ptr[f + 1]; k++) {
+                int sj = m.edge_side[k];
+                if (sj == sd) continue;
+                rows[i][m.side_elem[sj]] = Ent{kSame, sd, sj};
+            }
+        }
+        for (int k = m.el_vb_ptr[i]; k < m.el_vb_ptr[i + 1]; k++) {
+            int n = m.el_vb[k];
+            int hs = vb_hside[n];
+            int e2 = m.side_elem[hs];
+            rows[i][e2] = Ent{kVbLow, hs, -1};
+            rows[e2][i] = Ent{kVbHigh, hs, -1};
+        }
+        rows[i][i] = Ent{kDiag, -1, -1};
+    }
+    std::vector<int> ptr(E + 1, 0), col, kind, s1, s2, diag(E);
+    for (int i = 0; i < E; i++) {
+        for (auto &kv : rows[i]) {
+            if (kv.first == i) diag[i] = (int)col.size();
+            col.push_back(kv.first);
+            kind.push_back(kv.second.kind);
+            s1.push_back(kv.second.s1);
+            s2.push_back(kv.second.s2);
+        }
+        ptr[i + 1] = (int)col.size();
+    }
+    std::vector<std::map<int, Ent>>().swap(rows);
+    nnz_ = (int)col.size();
+    // boundary representative sides: Boundary::side() is side 0 of its edge
+    h_bnd_rep_.assign(S, -1);
+    for (int b = 0; b < m.n_boundaries(); b++) h_bnd_rep_[m.edge_side[m.edge_ptr[m.bnd_edge[b]]]] = b;
+    h_meas_.resize(E);
+    for (int i = 0; i < E; i++) h_meas_[i] = element_measure(m, m.el_dim[i], &m.el_nodes[4 * i]);
+    h_cs_.assign(cs, cs + E);
+    h_por_.assign(por, por + E);
+    h_bc_conc_.assign(bc_conc, bc_conc + (size_t)nsub * NBC_);
+    hipStream_t s = stream_;
+    d_ptr_ = upv(owned_, ptr, s); d_col_ = upv(owned_, col, s); d_kind_ = upv(owned_, kind, s);
+    d_s1_ = upv(owned_, s1, s); d_s2_ = upv(owned_, s2, s); d_diag_ = upv(owned_, diag, s);
+    d_side_off_ = upv(owned_, m.side_off, s); d_side_edge_ = upv(owned_, m.side_edge, s);
+    d_side_bnd_ = upv(owned_, m.side_bnd, s);
+    std::vector<int> bb = m.bnd_bcele;
+    if (bb.empty()) bb.push_back(0);
+    d_bnd_bcele_ = upv(owned_, bb, s);
+    d_edge_ptr_ = upv(owned_, m.edge_ptr, s); d_edge_side_ = upv(owned_, m.edge_side, s);
+    d_el_vb_ptr_ = upv(owned_, m.el_vb_ptr, s);
+    std::vector<int> elvb = m.el_vb;
+    if (elvb.empty()) elvb.push_back(0);
+    d_el_vb_ = upv(owned_, elvb, s);
+    d_vb_hside_ = upv(owned_, vb_hside, s);
+    d_bnd_rep_ = upv(owned_, h_bnd_rep_, s);
+    d_meas_ = upv(owned_, h_meas_, s); d_cs_ = upv(owned_, h_cs_, s); d_por_ = upv(owned_, h_por_, s);
+    std::vector<double> bcc = h_bc_conc_;
+    if (bcc.empty()) bcc.push_back(0);
+    d_bc_conc_ = upv(owned_, bcc, s);
+    auto alloc_d = [&](size_t n) { double *p = dev_alloc<double>(n); owned_.push_back(p); return p; };
+    d_flux_ = alloc_d(S);
+    d_edge_flow_ = alloc_d(F);
+    d_val_ = alloc_d(nnz_);
+    d_bcv_ = alloc_d((size_t)E * nsub);
+    d_c0_ = alloc_d((size_t)E * nsub);
+    d_c1_ = alloc_d((size_t)E * nsub);
+    d_partial_ = alloc_d(4096 * 8);
+    d_mb_ = alloc_d(1 + nsub);
+    d_count_ = dev_alloc<unsigned>(1);
+    owned_.push_back(d_count_);
+    FL_HIP(hipMemsetAsync(d_count_, 0, sizeof(unsigned), s));
+    FL_HIP(hipMemsetAsync(d_mb_, 0, (1 + nsub) * sizeof(double), s));
+    FL_HIP(hipMemsetAsync(d_c1_, 0, (size_t)E * nsub * sizeof(double), s));
+    // init_conc [nsub][E] -> interleaved [E][nsub]
+    std::vector<double> c0((size_t)E * nsub);
+    for (int k = 0; k < nsub; k++)
+        for (int i = 0; i < E; i++) c0[(size_t)i * nsub + k] = init_conc[(size_t)k * E + i];
+    FL_HIP(hipMemcpyAsync(d_c0_, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    FL_HIP(hipStreamSynchronize(s));
+    tg_.init(end_time, save_step);
+    cfl_max_step_ = end_time;
+    h_el_region_sub_.resize(E);
+    for (int i = 0; i < E; i++) h_el_region_sub_[i] = m.regions[m.el_region[i]].sub_idx;
+}
+
+TransportGPU::~TransportGPU() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *p : owned_) dev_free(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+static TmDev make_tm(const TransportGPU *, int E, int nsub, int NBC, int *ptr, int *col, int *kind, int *s1, int *s2,
+                     int *side_off, int *side_edge, int *side_bnd, int *bnd_bcele, int *edge_ptr, int *edge_side,
+                     int *el_vb_ptr, int *el_vb, int *vb_hside, int *diag, int *bnd_rep, double *meas, double *cs,
+                     double *por, double *flux, double *edge_flow, double *bc_conc, double *val, double *bcv,
+                     double *partial, double *mb, unsigned *count) {
+    TmDev t;
+    t.E = E; t.nsub = nsub; t.NBC = NBC;
+    t.ptr = ptr; t.col = col; t.kind = kind; t.s1 = s1; t.s2 = s2;
+    t.side_off = side_off; t.side_edge = side_edge; t.side_bnd = side_bnd; t.bnd_bcele = bnd_bcele;
+    t.edge_ptr = edge_ptr; t.edge_side = edge_side; t.el_vb_ptr = el_vb_ptr; t.el_vb = el_vb;
+    t.vb_hside = vb_hside; t.diag = diag; t.bnd_rep = bnd_rep;
+    t.meas = meas; t.cs = cs; t.por = por; t.flux = flux; t.edge_flow = edge_flow; t.bc_conc = bc_conc;
+    t.val = val; t.bcv = bcv; t.partial = partial; t.mb = mb; t.count = count;
+    return t;
+}
+
+#define TMDEV                                                                                                    \
+    make_tm(this, E_, nsub_, NBC_, d_ptr_, d_col_, d_kind_, d_s1_, d_s2_, d_side_off_, d_side_edge_, d_side_bnd_, \
+            d_bnd_bcele_, d_edge_ptr_, d_edge_side_, d_el_vb_ptr_, d_el_vb_, d_vb_hside_, d_diag_, d_bnd_rep_,   \
+            d_meas_, d_cs_, d_por_, d_flux_, d_edge_flow_, d_bc_conc_, d_val_, d_bcv_, d_partial_, d_mb_, d_count_)
+
+void TransportGPU::set_velocity_host(const double *side_flux) {
+    FL_HIP(hipMemcpyAsync(d_flux_, side_flux, m_.n_sides() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    flux_set_ = true;
+    matrix_built_ = false;  // mh_dh->time_changed() > transport_matrix_time
+}
+
+void TransportGPU::set_velocity_device(const double *d_side_flux) {
+    FL_HIP(hipMemcpyAsync(d_flux_, d_side_flux, m_.n_sides() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    flux_set_ = true;
+    matrix_built_ = false;
+}
+
+void TransportGPU::build_matrix() {
+    TmDev t = TMDEV;
+    int F = m_.n_edges();
+    hipEvent_t a, b;
+    FL_HIP(hipEventCreate(&a)); FL_HIP(hipEventCreate(&b));
+    FL_HIP(hipEventRecord(a, stream_));
+    hipLaunchKernelGGL(k_edge_flow, dim3(std::min(grid_for(F), 4096)), dim3(kBlock), 0, stream_, F,
+                       (const int *)d_edge_ptr_, (const int *)d_edge_side_, (const double *)d_flux_, d_edge_flow_);
+    hipLaunchKernelGGL(k_tm_build, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t);
+    FL_HIP(hipGetLastError());
+    double mx = 0.0;
+    FL_HIP(hipMemcpyAsync(&mx, d_mb_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipEventRecord(b, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    float ms = 0;
+    FL_HIP(hipEventElapsedTime(&ms, a, b));
+    stats_.ms_build += ms;
+    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    cfl_max_step_ = 1 / mx;
+    matrix_built_ = true;
+    scaled_ = false;
+}
+
+void TransportGPU::set_bc() {
+    TmDev t = TMDEV;
+    double est = tg_.estimate_dt();
+    hipLaunchKernelGGL(k_set_bc, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t, est);
+    FL_HIP(hipGetLastError());
+}
+
+void TransportGPU::mb_initial() {
+    // MassBalance::calculate at the first call fixes the initial mass and time
+    std::vector<double> c((size_t)E_ * nsub_);
+    FL_HIP(hipMemcpyAsync(c.data(), cur_is_c0_ ? d_c0_ : d_c1_, c.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    initial_mass_.assign(nsub_, 0.0);
+    for (int k = 0; k < nsub_; k++)
+        for (int i = 0; i < E_; i++) initial_mass_[k] += h_por_[i] * h_cs_[i] * c[(size_t)i * nsub_ + k] * h_meas_[i];
+    last_time_ = tg_.time;
+    mb_init_ = false;
+}
+
+void TransportGPU::one_step() {
+    TmDev t = TMDEV;
+    int grid = std::min(grid_for(E_), 4096);
+    if (!matrix_built_) {
+        build_matrix();
+        tg_.set_upper_constraint(cfl_max_step_);
+        tg_.fix_dt_until_mark();
+        set_bc();
+        need_rescale_ = true;
+    }
+    if (need_rescale_) {
+        if (scaled_) {
+            double fac = tg_.estimate_dt() / tg_.time_step;
+            hipLaunchKernelGGL(k_tm_scale, dim3(grid), dim3(kBlock), 0, stream_, t, 0, fac);
+        } else {
+            hipLaunchKernelGGL(k_tm_scale, dim3(grid), dim3(kBlock), 0, stream_, t, 1, tg_.estimate_dt());
+            scaled_ = true;
+        }
+        need_rescale_ = false;
+    }
+    tg_.next_time();
+    double tdiff = tg_.time - last_time_;
+    last_time_ = tg_.time;
+    const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
+    double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
+    switch (nsub_) {
+        case 1: hipLaunchKernelGGL(k_step<1>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
+        case 2: hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
+        case 3: hipLaunchKernelGGL(k_step<3>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
+        case 4: hipLaunchKernelGGL(k_step<4>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
+        default: throw Error(-5, "transport supports 1..4 substances per handle");
+    }
+    FL_HIP(hipGetLastError());
+    cur_is_c0_ = !cur_is_c0_;
+    stats_.steps++;
+}
+
+int TransportGPU::update_solution(double target) {
+    if (!flux_set_) throw Error(-4, "velocity field not set");
+    if (mb_init_) mb_initial();
+    // set_target_time (transport.cc:718-733)
+    tg_.add_mark(target);
+    tg_.set_upper_constraint(cfl_max_step_);
+    tg_.fix_dt_until_mark();
+    need_rescale_ = true;
+    hipEvent_t a, b;
+    FL_HIP(hipEventCreate(&a)); FL_HIP(hipEventCreate(&b));
+    FL_HIP(hipEventRecord(a, stream_));
+    int steps = 0;
+    while (tg_.lt(target)) {
+        double t_before = tg_.time;
+        one_step();
+        steps++;
+        if (!(tg_.time > t_before) || !std::isfinite(tg_.time))
+            throw Error(-7, "transport time step does not advance (dt = " + std::to_string(tg_.time_step) + ")");
+    }
+    FL_HIP(hipEventRecord(b, stream_));
+    FL_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    FL_HIP(hipEventElapsedTime(&ms, a, b));
+    stats_.ms_steps += ms;
+    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    return steps;
+}
+
+void TransportGPU::get_conc(double *out) const {
+    std::vector<double> c((size_t)E_ * nsub_);
+    FL_HIP(hipMemcpyAsync(c.data(), cur_is_c0_ ? d_c0_ : d_c1_, c.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    for (int k = 0; k < nsub_; k++)
+        for (int i = 0; i < E_; i++) out[(size_t)k * E_ + i] = c[(size_t)i * nsub_ + k];
+}
+
+void TransportGPU::get_matrix(int *ptr, int *col, double *val) const {
+    FL_HIP(hipMemcpyAsync(ptr, d_ptr_, (E_ + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(col, d_col_, nnz_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(val, d_val_, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+// Per-region report of the last step (MassBalance::calculate): boundary
+// fluxes with the previous concentrations, mass with the current ones.
+void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double *mass, double *totals) {
+    const HostMesh &m = m_;
+    int nbr = m.n_boundary_regions, nblk = m.n_bulk_regions, ns = nsub_;
+    std::vector<double> cur((size_t)E_ * ns), prev((size_t)E_ * ns), mb(1 + ns);
+    FL_HIP(hipMemcpyAsync(cur.data(), cur_is_c0_ ? d_c0_ : d_c1_, cur.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(prev.data(), cur_is_c0_ ? d_c1_ : d_c0_, prev.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(mb.data(), d_mb_, mb.size() * 8, hipMemcpyDeviceToHost, stream_));
+    std::vector<double> flux(m.n_sides());
+    FL_HIP(hipMemcpyAsync(flux.data(), d_flux_, flux.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    if (stats_.steps == 0) std::fill(prev.begin(), prev.end(), 0.0);
+    std::fill(bal, bal + ns * nbr, 0.0);
+    std::fill(plus, plus + ns * nbr, 0.0);
+    std::fill(minus, minus + ns * nbr, 0.0);
+    std::fill(mass, mass + ns * nblk, 0.0);
+    for (int b = 0; b < m.n_boundaries(); b++) {
+        int sd = m.edge_side[m.edge_ptr[m.bnd_edge[b]]];
+        int i = m.side_elem[sd];
+        double wf = flux[sd];
+        int bce = m.bnd_bcele[b];
+        int r = m.regions[m.bc_region[bce]].sub_idx;
+        for (int k = 0; k < ns; k++) {
+            double mf = wf < 0 ? wf * h_bc_conc_[(size_t)k * NBC_ + bce] : wf * prev[(size_t)i * ns + k];
+            bal[k * nbr + r] += mf;
+            if (mf > 0) plus[k * nbr + r] += mf;
+            else minus[k * nbr + r] += mf;
+        }
+    }
+    for (int k = 0; k < ns; k++)
+        for (int i = 0; i < E_; i++)
+            mass[k * nblk + h_el_region_sub_[i]] += h_por_[i] * h_cs_[i] * cur[(size_t)i * ns + k] * h_meas_[i];
+    for (int k = 0; k < ns; k++) {
+        double mt = 0.0;
+        for (int r = 0; r < nblk; r++) mt += mass[k * nblk + r];
+        totals[4 * k + 0] = k < (int)initial_mass_.size() ? initial_mass_[k] : 0.0;
+        totals[4 * k + 1] = mb[1 + k];
+        totals[4 * k + 2] = 0.0;
+        totals[4 * k + 3] = mt;
+    }
+}
+
+}  // namespace fl

@@ -1,0 +1,294 @@
+"""Python binding of the MI355X Darcy / transport C ABI (include/flow123d_gpu.h).
+
+Mirrors the reference's equation interface: ``DarcyFlowMH_Steady``
+(update_solution, get_solution_vector, solution_precision, water balance)
+and ``ConvectionTransport`` driven as TransportOperatorSplitting
+(set_velocity_field, update_solution, concentrations, mass balance).
+Every call goes through libflow123d_gpu.so; there is no CPU fallback: a
+missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflow123d_gpu.so")
+_LIB = None
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class FlowError(RuntimeError):
+    pass
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("r_tol", C.c_double), ("a_tol", C.c_double), ("max_it", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
+                ("rhs_norm", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
+                ("ms_resolve", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# symbol -> (restype, argtypes)
+SIGNATURES = {
+    "fl_last_error": (C.c_char_p, []),
+    "fl_mesh_read_gmsh": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "fl_mesh_from_arrays": (C.c_int, [C.c_int, _f64p, C.c_int, C.c_void_p, _i32p, _i32p, _i32p, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "fl_mesh_free": (None, [C.c_void_p]),
+    "fl_mesh_counts": (C.c_int, [C.c_void_p, _i32p]),
+    "fl_mesh_topology": (C.c_int, [C.c_void_p] + [_i32p] * 13),
+    "fl_mesh_regions": (C.c_int, [C.c_void_p, _i32p, _i32p, _i32p, C.c_char_p, C.c_int]),
+    "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "fl_darcy_free": (None, [C.c_void_p]),
+    "fl_darcy_set_data": (C.c_int, [C.c_void_p, _f64p, _f64p, C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p,
+                                    _f64p]),
+    "fl_darcy_set_conductivity": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_darcy_update_solution": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
+    "fl_darcy_get_solution": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_darcy_device_solution": (C.c_void_p, [C.c_void_p]),
+    "fl_darcy_solution_precision": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
+    "fl_darcy_element_balance": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_darcy_water_balance": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
+    "fl_darcy_edge_nnz": (C.c_int, [C.c_void_p]),
+    "fl_darcy_edge_system": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p, _f64p]),
+    "fl_transport_create": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                      C.POINTER(C.c_void_p)]),
+    "fl_transport_free": (None, [C.c_void_p]),
+    "fl_transport_set_velocity": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_transport_set_velocity_from_darcy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_transport_update_solution": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
+    "fl_transport_get_conc": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_transport_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+    "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 5),
+    "fl_transport_nnz": (C.c_int, [C.c_void_p]),
+    "fl_transport_matrix": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p]),
+    "fl_transport_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlowError("libflow123d_gpu.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise FlowError("flow123d_gpu error %d: %s" % (rc, lib().fl_last_error().decode()))
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+class Mesh:
+    """Mesh + topology (Mesh, msh_gmshreader.cc / mesh.cc) built by the library's host module."""
+
+    def __init__(self, path=None, arrays=None):
+        L = lib()
+        h = C.c_void_p()
+        if path is not None:
+            _check(L.fl_mesh_read_gmsh(os.fsencode(path), C.byref(h)))
+        else:
+            a = arrays
+            xyz = _f64(a["nodes"])
+            en = np.ascontiguousarray(a["elem_nodes"], np.int32).reshape(-1)
+            et = np.ascontiguousarray(a["elem_type"], np.int32)
+            er = np.ascontiguousarray(a["elem_region"], np.int32)
+            phys = a.get("physical", [])
+            pid = (C.c_int * max(len(phys), 1))(*[p[1] for p in phys])
+            pnm = (C.c_char_p * max(len(phys), 1))(*[p[2].encode() for p in phys])
+            _check(L.fl_mesh_from_arrays(len(xyz) // 3, xyz, len(et), None, et, er, en, len(phys),
+                                         C.cast(pid, C.c_void_p), C.cast(pnm, C.c_void_p), C.byref(h)))
+        self.h = h
+        c = np.zeros(9, np.int32)
+        L.fl_mesh_counts(h, c)
+        (self.n_nodes, self.n_elements, self.n_bc_elements, self.n_sides, self.n_edges, self.n_boundaries,
+         self.n_vb, self.n_regions, self.n_file_bc) = [int(v) for v in c]
+        E, F, S, V, B, NB = self.n_elements, self.n_edges, self.n_sides, self.n_vb, self.n_boundaries, \
+            self.n_bc_elements
+        self.elem_dim = np.zeros(E, np.int32)
+        self.elem_id = np.zeros(E, np.int32)
+        self.elem_region = np.zeros(E, np.int32)
+        self.elem_nodes = np.zeros(4 * E, np.int32)
+        self.elem_edge = np.zeros(4 * E, np.int32)
+        self.elem_bnd = np.zeros(4 * E, np.int32)
+        self.edge_ptr = np.zeros(F + 1, np.int32)
+        self.edge_side = np.zeros(max(S, 1), np.int32)
+        vb_elem = np.zeros(max(V, 1), np.int32)
+        vb_edge = np.zeros(max(V, 1), np.int32)
+        bnd_edge = np.zeros(max(B, 1), np.int32)
+        bnd_bcele = np.zeros(max(B, 1), np.int32)
+        bc_region = np.zeros(max(NB, 1), np.int32)
+        _check(L.fl_mesh_topology(h, self.elem_dim, self.elem_id, self.elem_region, self.elem_nodes,
+                                  self.elem_edge, self.elem_bnd, self.edge_ptr, self.edge_side, vb_elem, vb_edge,
+                                  bnd_edge, bnd_bcele, bc_region))
+        self.vb_elem, self.vb_edge = vb_elem[:V], vb_edge[:V]
+        self.bnd_edge, self.bnd_bcele, self.bc_region = bnd_edge[:B], bnd_bcele[:B], bc_region[:NB]
+        R = self.n_regions
+        ids, isb, sub = np.zeros(R, np.int32), np.zeros(R, np.int32), np.zeros(R, np.int32)
+        buf = C.create_string_buffer(1 << 16)
+        _check(L.fl_mesh_regions(h, ids, isb, sub, buf, len(buf)))
+        labels = buf.value.decode().split("\n")[:R]
+        self.regions = [dict(id=int(i), boundary=bool(b), sub_idx=int(s), label=l)
+                        for i, b, s, l in zip(ids, isb, sub, labels)]
+        self.side_off = np.concatenate([[0], np.cumsum(self.elem_dim + 1)]).astype(np.int64)
+
+    @property
+    def n_boundary_regions(self):
+        return sum(1 for r in self.regions if r["boundary"])
+
+    @property
+    def n_bulk_regions(self):
+        return sum(1 for r in self.regions if not r["boundary"])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_mesh_free(self.h)
+            self.h = None
+
+
+class DarcyFlowMH_Steady:
+    """GPU replacement of DarcyFlowMH_Steady (darcy_flow_mh.cc:249-505)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib().fl_darcy_create(mesh.h, C.byref(h)))
+        self.h = h
+        self.S, self.E, self.F = mesh.n_sides, mesh.n_elements, mesh.n_edges
+
+    def set_data(self, K, cs, sigma, f, bc_type, bc_pressure, bc_flux, bc_sigma, aniso=None):
+        E, NB = self.E, self.mesh.n_bc_elements
+        self._keep = [_f64(K, E), _f64(cs, E), _f64(sigma, E), _f64(f, E), _f64(bc_pressure, NB),
+                      _f64(bc_flux, NB), _f64(bc_sigma, NB)]
+        bct = np.ascontiguousarray(bc_type, np.int32)
+        an = None if aniso is None else _f64(aniso, 9 * E)
+        self._an = an
+        K, cs, sigma, f, bp, bf, bs = self._keep
+        _check(lib().fl_darcy_set_data(self.h, K, cs, None if an is None else an.ctypes.data, sigma, f, bct, bp,
+                                       bf, bs))
+
+    def set_conductivity(self, K):
+        _check(lib().fl_darcy_set_conductivity(self.h, _f64(K, self.E)))
+
+    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000):
+        o = SolverOpts(r_tol, a_tol, max_it)
+        info = SolveInfo()
+        _check(lib().fl_darcy_update_solution(self.h, C.byref(o), C.byref(info)))
+        self.info = info.as_dict()
+        return self.info
+
+    def get_solution_vector(self):
+        out = np.zeros(self.S + self.E + self.F)
+        _check(lib().fl_darcy_get_solution(self.h, out))
+        return out
+
+    def solution_precision(self, r_tol, a_tol):
+        return lib().fl_darcy_solution_precision(self.h, r_tol, a_tol)
+
+    def element_balance(self):
+        out = np.zeros(self.E)
+        _check(lib().fl_darcy_element_balance(self.h, out))
+        return out
+
+    def water_balance(self):
+        n = max(self.mesh.n_boundary_regions, 1)
+        b, p, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().fl_darcy_water_balance(self.h, b, p, m))
+        k = self.mesh.n_boundary_regions
+        return b[:k], p[:k], m[:k]
+
+    def edge_system(self):
+        nnz = lib().fl_darcy_edge_nnz(self.h)
+        ptr, col, val, rhs = np.zeros(self.F + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz), \
+            np.zeros(self.F)
+        _check(lib().fl_darcy_edge_system(self.h, ptr, col, val, rhs))
+        return ptr, col, val, rhs
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_darcy_free(self.h)
+            self.h = None
+
+
+class ConvectionTransport:
+    """GPU replacement of ConvectionTransport + TransportOperatorSplitting (transport.cc)."""
+
+    def __init__(self, mesh, por, cs, bc_conc, init_conc, end_time, save_step):
+        self.mesh = mesh
+        NB, E = mesh.n_bc_elements, mesh.n_elements
+        bcc = np.ascontiguousarray(bc_conc, np.float64).reshape(-1, NB)
+        self.nsub = bcc.shape[0]
+        self._keep = [_f64(por, E), _f64(cs, E), _f64(bcc), _f64(init_conc, self.nsub * E)]
+        h = C.c_void_p()
+        _check(lib().fl_transport_create(mesh.h, self.nsub, self._keep[0], self._keep[1], self._keep[2],
+                                         self._keep[3], end_time, save_step, C.byref(h)))
+        self.h = h
+
+    def set_velocity_field(self, side_flux=None, darcy=None):
+        if darcy is not None:
+            _check(lib().fl_transport_set_velocity_from_darcy(self.h, darcy.h))
+        else:
+            _check(lib().fl_transport_set_velocity(self.h, _f64(side_flux, self.mesh.n_sides)))
+
+    def update_solution(self, t):
+        s = C.c_int()
+        _check(lib().fl_transport_update_solution(self.h, t, C.byref(s)))
+        return s.value
+
+    def times(self):
+        t, dt, cfl = C.c_double(), C.c_double(), C.c_double()
+        lib().fl_transport_time(self.h, C.byref(t), C.byref(dt), C.byref(cfl))
+        return t.value, dt.value, cfl.value
+
+    def conc(self):
+        out = np.zeros(self.nsub * self.mesh.n_elements)
+        _check(lib().fl_transport_get_conc(self.h, out))
+        return out.reshape(self.nsub, -1)
+
+    def matrix(self):
+        nnz = lib().fl_transport_nnz(self.h)
+        ptr, col, val = np.zeros(self.mesh.n_elements + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz)
+        _check(lib().fl_transport_matrix(self.h, ptr, col, val))
+        return ptr, col, val
+
+    def mass_balance(self):
+        ns, nbr, nblk = self.nsub, self.mesh.n_boundary_regions, self.mesh.n_bulk_regions
+        bal, plus, minus = np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1))
+        mass, tot = np.zeros(ns * max(nblk, 1)), np.zeros(4 * ns)
+        _check(lib().fl_transport_mass_balance(self.h, bal, plus, minus, mass, tot))
+        return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
+                    minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
+                    totals=tot.reshape(ns, 4))
+
+    def stats(self):
+        ms, st = C.c_double(), C.c_int()
+        lib().fl_transport_stats(self.h, C.byref(ms), C.byref(st))
+        return ms.value, st.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_transport_free(self.h)
+            self.h = None
