@@ -93,7 +93,9 @@ typedef struct {
 } fl_solver_opts;
 
 /* Result of a solve: CG iterations, final residual 2-norm, rhs 2-norm,
- * converged flag, device times of the phases in ms. */
+ * converged flag, device times of the phases in ms, and the average device
+ * durations of the two CG kernels (SpMV + direction update, and the vector
+ * update), sampled with HIP events on the solver stream every 8th iteration. */
 typedef struct {
     int iterations;
     int converged;
@@ -103,6 +105,9 @@ typedef struct {
     double ms_solve;
     double ms_resolve;
     double ms_total;
+    double ms_spmv_avg;
+    double ms_update_avg;
+    int kernel_samples;
 } fl_solve_info;
 
 /* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433). */

@@ -1015,6 +1015,33 @@ void *orc_mesh_load(const char *path) {
     return m;
 }
 void orc_mesh_free(void *h) { delete (Mesh *)h; }
+// Build from arrays (same element list semantics as the gmsh reader):
+// gmsh types, region ids, 0-based node indices (4 per element, -1 padded),
+// physical names (labels starting with '.' are boundary regions).
+void *orc_mesh_from_arrays(int n_nodes, const double *xyz, int n_elem, const int *gmsh_type, const int *region_id,
+                           const int *nodes4, int n_phys, const int *phys_id, const char *const *phys_name) {
+    Mesh *m = new Mesh();
+    for (int i = 0; i < n_phys; i++) m->rdb.add_named(phys_id[i], phys_name[i] ? phys_name[i] : "");
+    m->X.assign(xyz, xyz + 3 * (size_t)n_nodes);
+    m->node_ids.resize(n_nodes);
+    for (int i = 0; i < n_nodes; i++) { m->node_ids[i] = i + 1; m->node_idx[i + 1] = i; }
+    for (int i = 0; i < n_elem; i++) {
+        int t = gmsh_type[i];
+        int dim = t == 1 ? 1 : t == 2 ? 2 : t == 4 ? 3 : t == 15 ? 0 : -1;
+        if (dim < 0) { g_err = "unsupported element type"; delete m; return nullptr; }
+        Element e;
+        e.id = i + 1;
+        e.dim = dim;
+        e.region = m->rdb.add_unnamed(region_id[i]);
+        for (int k = 0; k <= dim; k++) e.node[k] = nodes4[4 * (size_t)i + k];
+        if (m->rdb.regs[e.region].boundary) m->bc_elements.push_back(e);
+        else if (dim > 0) m->element.push_back(e);
+    }
+    m->n_file_bc = (int)m->bc_elements.size();
+    if (!setup_topology(*m)) { delete m; return nullptr; }
+    return m;
+}
+
 
 // counts: n_nodes, n_elements, n_bc_elements, n_sides, n_edges, n_boundaries,
 //         n_vb, n_regions, n_file_bc

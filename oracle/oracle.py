@@ -27,6 +27,9 @@ def lib():
         L.orc_mesh_load.restype = C.c_void_p
         L.orc_mesh_load.argtypes = [C.c_char_p]
         L.orc_mesh_free.argtypes = [C.c_void_p]
+        L.orc_mesh_from_arrays.restype = C.c_void_p
+        L.orc_mesh_from_arrays.argtypes = [C.c_int, _f64p, C.c_int, _i32p, _i32p, _i32p, C.c_int, C.c_void_p,
+                                           C.c_void_p]
         L.orc_mesh_counts.argtypes = [C.c_void_p, _i32p]
         L.orc_mesh_topology.argtypes = [C.c_void_p] + [_i32p] * 13
         L.orc_mesh_nodes.argtypes = [C.c_void_p, _f64p]
@@ -66,9 +69,21 @@ def lib():
 class Mesh:
     """Mesh read and topologised by the oracle (msh_gmshreader.cc, mesh.cc)."""
 
-    def __init__(self, path):
+    def __init__(self, path=None, arrays=None):
         L = lib()
-        self.h = L.orc_mesh_load(os.fsencode(path))
+        if path is not None:
+            self.h = L.orc_mesh_load(os.fsencode(path))
+        else:
+            a = arrays
+            xyz = np.ascontiguousarray(a["nodes"], np.float64).reshape(-1)
+            et = np.ascontiguousarray(a["elem_type"], np.int32)
+            er = np.ascontiguousarray(a["elem_region"], np.int32)
+            en = np.ascontiguousarray(a["elem_nodes"], np.int32).reshape(-1)
+            phys = a.get("physical", [])
+            pid = (C.c_int * max(len(phys), 1))(*[p[1] for p in phys])
+            pnm = (C.c_char_p * max(len(phys), 1))(*[p[2].encode() for p in phys])
+            self.h = L.orc_mesh_from_arrays(len(xyz) // 3, xyz, len(et), et, er, en, len(phys),
+                                            C.cast(pid, C.c_void_p), C.cast(pnm, C.c_void_p))
         if not self.h:
             raise RuntimeError("oracle mesh load failed: %s" % L.orc_last_error().decode())
         c = np.zeros(9, np.int32)

@@ -174,6 +174,9 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
             info->ms_solve = st.ms_cg;
             info->ms_resolve = st.ms_resolve;
             info->ms_total = st.ms_total;
+            info->ms_spmv_avg = st.ms_spmv_avg;
+            info->ms_update_avg = st.ms_update_avg;
+            info->kernel_samples = st.kernel_samples;
         }
         return 0;
     });

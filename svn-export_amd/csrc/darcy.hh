@@ -38,6 +38,9 @@ struct SolveStats {
     int converged = 0;
     double residual = 0, bnorm = 0;
     double ms_assembly = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
+    // live per-kernel durations sampled with HIP events on the solver stream
+    double ms_spmv_avg = 0, ms_update_avg = 0;
+    int kernel_samples = 0;
 };
 
 // Device view of the mesh + pattern + fields, passed to kernels by value.
@@ -100,6 +103,7 @@ class DarcyGPU {
     CGState *d_state_ = nullptr;
     unsigned *d_count_ = nullptr;
     bool have_data_ = false;
+    std::vector<hipEvent_t> ev_pool_;
     std::vector<double> h_bc_measure_;
     std::vector<double> h_cs_, h_f_, h_sigma_;
 };

@@ -329,6 +329,7 @@ DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
 
 DarcyGPU::~DarcyGPU() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     for (void *p : owned_) dev_free(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -557,10 +558,35 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
     CGState h{};
     int launched = 0;
     int chunk = 32;
+    // every kSampleEvery-th iteration is bracketed by events (spmv, update)
+    const int kSampleEvery = 8;
+    struct Sample { int ev, it; };
+    std::vector<Sample> samples;
+    int ev_used = 0;
+    auto next_ev = [&]() {
+        if (ev_used == (int)ev_pool_.size()) {
+            hipEvent_t e;
+            FL_HIP(hipEventCreate(&e));
+            ev_pool_.push_back(e);
+        }
+        return ev_used++;
+    };
     while (true) {
         for (int k = 0; k < chunk; k++) {
+            int itx = launched + k;
+            bool samp = (itx % kSampleEvery) == 1;
+            int e0 = 0;
+            if (samp) {
+                e0 = next_ev(); next_ev(); next_ev();
+                FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
+            }
             hipLaunchKernelGGL(k_cg_spmv, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pin, pout);
+            if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pout);
+            if (samp) {
+                FL_HIP(hipEventRecord(ev_pool_[e0 + 2], stream_));
+                samples.push_back(Sample{e0, itx});
+            }
             std::swap(pin, pout);
         }
         launched += chunk;
@@ -570,6 +596,18 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
         if (h.done || launched >= max_it + chunk) break;
         if (chunk < 256) chunk *= 2;
     }
+    double s1 = 0.0, s2 = 0.0;
+    int ns = 0;
+    for (const Sample &sm : samples) {
+        if (sm.it >= h.it) continue;  // launches after convergence exit early
+        float t1 = 0, t2 = 0;
+        FL_HIP(hipEventElapsedTime(&t1, ev_pool_[sm.ev], ev_pool_[sm.ev + 1]));
+        FL_HIP(hipEventElapsedTime(&t2, ev_pool_[sm.ev + 1], ev_pool_[sm.ev + 2]));
+        s1 += t1; s2 += t2; ns++;
+    }
+    st.kernel_samples = ns;
+    st.ms_spmv_avg = ns ? s1 / ns : 0.0;
+    st.ms_update_avg = ns ? s2 / ns : 0.0;
     st.iterations = h.it;
     st.residual = std::sqrt(h.rr);
     st.bnorm = std::sqrt(h.bnorm2);

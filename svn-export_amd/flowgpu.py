@@ -31,7 +31,8 @@ class SolverOpts(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("rhs_norm", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
-                ("ms_resolve", C.c_double), ("ms_total", C.c_double)]
+                ("ms_resolve", C.c_double), ("ms_total", C.c_double), ("ms_spmv_avg", C.c_double),
+                ("ms_update_avg", C.c_double), ("kernel_samples", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

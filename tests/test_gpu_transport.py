@@ -10,7 +10,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test13_flow(om):
+def _flow13(om):
     d = cases.darcy_case("test13", om)
     D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"],
                  aniso=d["aniso"])
@@ -33,7 +33,7 @@ def test_transport_bitexact_vs_oracle():
     """Same fluxes in -> identical transport matrix and concentrations (bitwise)."""
     path = cases.ref_path("test13.msh")
     om, gm = O.Mesh(path), G.Mesh(path)
-    d, D0 = test13_flow(om)
+    d, D0 = _flow13(om)
     flux = D0.solution[:D0.S]
     E = om.n_elements
     bcc = cases.test13_bc_conc(om)
