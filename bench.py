@@ -143,8 +143,7 @@ def main():
         por = np.ones(E)
         probe = G.ConvectionTransport(gm, por, cs, bcc, np.zeros((nsub, E)), 1.0, 1.0)
         probe.set_velocity_field(darcy=darcy)
-        probe.update_solution(1e-300)  # builds the matrix: CFL limit
-        cfl = probe.times()[2]
+        cfl = probe.prepare()  # transport matrix of this flux field: CFL limit
         del probe
         T_end = args.transport_steps * cfl
         tr = G.ConvectionTransport(gm, por, cs, bcc, np.zeros((nsub, E)), T_end, T_end / 10)

@@ -154,6 +154,11 @@ void fl_transport_free(fl_transport *t);
 int fl_transport_set_velocity(fl_transport *t, const double *side_flux);
 int fl_transport_set_velocity_from_darcy(fl_transport *t, fl_darcy *d);
 
+/* ConvectionTransport::create_transport_matrix_mpi (transport.cc:788-943) for
+ * the current velocity field, done ahead of the first step; cfl receives
+ * cfl_max_step = 1 / max_i |a_ii| (transport.cc:911-921). */
+int fl_transport_prepare(fl_transport *t, double *cfl);
+
 /* TransportOperatorSplitting::update_solution (transport_operator_splitting.cc:244-287):
  * ConvectionTransport::set_target_time + compute_one_step until target_time.
  * steps (optional) receives the number of steps taken. */

@@ -258,6 +258,14 @@ int fl_transport_set_velocity_from_darcy(fl_transport *t, fl_darcy *d) {
     });
 }
 
+int fl_transport_prepare(fl_transport *t, double *cfl) {
+    return guard([&] {
+        double c = t->g->prepare();
+        if (cfl) *cfl = c;
+        return 0;
+    });
+}
+
 int fl_transport_update_solution(fl_transport *t, double target, int *steps) {
     return guard([&] {
         int s = t->g->update_solution(target);

@@ -58,4 +58,56 @@ __device__ inline double block_sum(double v, double *sh) {
     return r;
 }
 
+// Grid-wide reduction of NV doubles per thread without a second launch: every
+// block writes its NV block totals to `partial`, the last block to arrive
+// (atomic ticket) folds all partials with all of its threads in a fixed order
+// and a fixed tree, so the result is deterministic.  Returns true in the last
+// block only, where v[] then holds the grid totals in every thread.  The
+// ticket counter is reset for the next launch.  Requires blockDim.x == 256.
+struct SumOp {
+    __device__ static double id() { return 0.0; }
+    __device__ static double op(double a, double b) { return a + b; }
+};
+struct MaxOp {
+    __device__ static double id() { return 0.0; }  // used on non-negative values
+    __device__ static double op(double a, double b) { return fmax(a, b); }
+};
+
+template <int NV, class Op = SumOp>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partial, unsigned *count) {
+    __shared__ double sh[NV][4];
+    __shared__ bool is_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x = Op::op(x, __shfl_down(x, o, 64));
+        if (lane == 0) sh[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; k++)
+            partial[(size_t)NV * blockIdx.x + k] = Op::op(Op::op(sh[k][0], sh[k][1]), Op::op(sh[k][2], sh[k][3]));
+        __threadfence();
+        is_last = (atomicAdd(count, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    const volatile double *pv = partial;
+    double acc[NV];
+    for (int k = 0; k < NV; k++) acc[k] = Op::id();
+    for (unsigned g = threadIdx.x; g < gridDim.x; g += blockDim.x)
+        for (int k = 0; k < NV; k++) acc[k] = Op::op(acc[k], pv[(size_t)NV * g + k]);
+    __syncthreads();
+    for (int k = 0; k < NV; k++) {
+        double x = acc[k];
+        for (int o = 32; o > 0; o >>= 1) x = Op::op(x, __shfl_down(x, o, 64));
+        if (lane == 0) sh[k][wid] = x;
+    }
+    __syncthreads();
+    for (int k = 0; k < NV; k++) v[k] = Op::op(Op::op(sh[k][0], sh[k][1]), Op::op(sh[k][2], sh[k][3]));
+    if (threadIdx.x == 0) *count = 0;
+    return true;
+}
+
 }  // namespace fl

@@ -164,22 +164,8 @@ struct CGArgs {
     unsigned *count;
 };
 
-// Last block of the grid folds the per-block partials (fixed order).
-__device__ __forceinline__ bool last_block(unsigned *count) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(count, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    return is_last;
-}
-
 __global__ void __launch_bounds__(256) k_cg_init(CGArgs a, double rtol2, double atol2, int max_it) {
-    __shared__ double sh[8];
-    double rz = 0.0, rr = 0.0;
+    double v[2] = {0.0, 0.0};  // r.z, r.r
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
         double b = a.b[i];
         a.x[i] = 0.0;
@@ -187,37 +173,28 @@ __global__ void __launch_bounds__(256) k_cg_init(CGArgs a, double rtol2, double 
         double z = a.dinv[i] * b;
         a.z[i] = z;
         a.q[i] = 0.0;
-        rz += b * z;
-        rr += b * b;
+        v[0] += b * z;
+        v[1] += b * b;
     }
-    double s1 = block_sum(rz, sh), s2 = block_sum(rr, sh);
-    if (threadIdx.x == 0) {
-        a.partial[2 * blockIdx.x] = s1;
-        a.partial[2 * blockIdx.x + 1] = s2;
-    }
-    if (last_block(a.count) && threadIdx.x == 0) {
-        double t1 = 0.0, t2 = 0.0;
-        for (unsigned g = 0; g < gridDim.x; g++) { t1 += a.partial[2 * g]; t2 += a.partial[2 * g + 1]; }
+    if (grid_reduce<2>(v, a.partial, a.count) && threadIdx.x == 0) {
         CGState *st = a.st;
-        st->rz = t1;
+        st->rz = v[0];
         st->rz_old = 1.0;
-        st->rr = t2;
-        st->bnorm2 = t2;
-        st->tol2 = fmax(rtol2 * t2, atol2);
+        st->rr = v[1];
+        st->bnorm2 = v[1];
+        st->tol2 = fmax(rtol2 * v[1], atol2);
         st->it = 0;
         st->max_it = max_it;
-        st->done = (t2 <= st->tol2) ? 1 : 0;
-        *a.count = 0;
+        st->done = (v[1] <= st->tol2) ? 1 : 0;
     }
 }
 
-// p_out = z + beta p_in ; q = A p_out (gathered as z + beta p_in) ; pq partials
+// p_out = z + beta p_in ; q = A p_out (gathered as z + beta p_in) ; p.q
 __global__ void __launch_bounds__(256) k_cg_spmv(CGArgs a, const double *__restrict__ p_in, double *__restrict__ p_out) {
-    __shared__ double sh[8];
     const CGState *st = a.st;
     if (st->done) return;
     double beta = st->it == 0 ? 0.0 : st->rz / st->rz_old;
-    double pq = 0.0;
+    double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int k = a.ptr[i]; k < a.ptr[i + 1]; k++) {
@@ -227,47 +204,32 @@ __global__ void __launch_bounds__(256) k_cg_spmv(CGArgs a, const double *__restr
         double p = a.z[i] + beta * p_in[i];
         p_out[i] = p;
         a.q[i] = s;
-        pq += p * s;
+        v[0] += p * s;
     }
-    double t = block_sum(pq, sh);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = t;
-    if (last_block(a.count) && threadIdx.x == 0) {
-        double tot = 0.0;
-        for (unsigned g = 0; g < gridDim.x; g++) tot += a.partial[g];
-        a.st->pq = tot;
-        *a.count = 0;
-    }
+    if (grid_reduce<1>(v, a.partial, a.count) && threadIdx.x == 0) a.st->pq = v[0];
 }
 
+// x += alpha p ; r -= alpha q ; z = D^-1 r ; r.z, r.r ; stop test
 __global__ void __launch_bounds__(256) k_cg_update(CGArgs a, const double *__restrict__ p) {
-    __shared__ double sh[8];
     CGState *st = a.st;
     if (st->done) return;
     double alpha = st->rz / st->pq;
-    double rz = 0.0, rr = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
         a.x[i] += alpha * p[i];
         double r = a.r[i] - alpha * a.q[i];
         a.r[i] = r;
         double z = a.dinv[i] * r;
         a.z[i] = z;
-        rz += r * z;
-        rr += r * r;
+        v[0] += r * z;
+        v[1] += r * r;
     }
-    double s1 = block_sum(rz, sh), s2 = block_sum(rr, sh);
-    if (threadIdx.x == 0) {
-        a.partial[2 * blockIdx.x] = s1;
-        a.partial[2 * blockIdx.x + 1] = s2;
-    }
-    if (last_block(a.count) && threadIdx.x == 0) {
-        double t1 = 0.0, t2 = 0.0;
-        for (unsigned g = 0; g < gridDim.x; g++) { t1 += a.partial[2 * g]; t2 += a.partial[2 * g + 1]; }
+    if (grid_reduce<2>(v, a.partial, a.count) && threadIdx.x == 0) {
         st->rz_old = st->rz;
-        st->rz = t1;
-        st->rr = t2;
+        st->rz = v[0];
+        st->rr = v[1];
         st->it += 1;
-        if (t2 <= st->tol2 || st->it >= st->max_it || !(t2 == t2)) st->done = 1;
-        *a.count = 0;
+        if (v[1] <= st->tol2 || st->it >= st->max_it || !(v[1] == v[1])) st->done = 1;
     }
 }
 

@@ -55,6 +55,7 @@ class TransportGPU {
     void set_velocity_host(const double *side_flux);       // [S]
     void set_velocity_device(const double *d_side_flux);   // device [S] (e.g. the Darcy solution)
     int update_solution(double target_time);               // steps taken
+    double prepare();                                       // build the matrix now; returns the CFL step
     void get_conc(double *out) const;                      // [nsub][E]
     void mass_balance(double *bal, double *plus, double *minus, double *mass, double *totals);
     double time() const { return tg_.time; }
@@ -89,7 +90,7 @@ class TransportGPU {
     // state
     TimeGovernorR tg_;
     double cfl_max_step_ = 0;
-    bool matrix_built_ = false, scaled_ = false, need_rescale_ = true, flux_set_ = false;
+    bool matrix_built_ = false, setup_done_ = false, scaled_ = false, need_rescale_ = true, flux_set_ = false;
     bool mb_init_ = true;
     double last_time_ = -1;
     std::vector<double> h_meas_, h_cs_, h_por_, h_bc_conc_, initial_mass_;

@@ -106,15 +106,6 @@ struct TmDev {
     unsigned *count;
 };
 
-__device__ __forceinline__ bool tm_last_block(unsigned *count) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(count, 1u) == gridDim.x - 1);
-    __syncthreads();
-    return is_last;
-}
-
 __global__ void k_edge_flow(const int F, const int *__restrict__ edge_ptr, const int *__restrict__ edge_side,
                             const double *__restrict__ flux, double *__restrict__ edge_flow) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
@@ -143,7 +134,6 @@ __device__ double row_diag(const TmDev &t, int i) {
 }
 
 __global__ void __launch_bounds__(256) k_tm_build(const TmDev t) {
-    __shared__ double sh[4];
     double mx = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
         for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
@@ -169,20 +159,8 @@ __global__ void __launch_bounds__(256) k_tm_build(const TmDev t) {
         }
     }
     // max |a_ii| over the grid (CFL, transport.cc:911-921)
-    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
-        t.partial[blockIdx.x] = m;
-    }
-    if (tm_last_block(t.count) && threadIdx.x == 0) {
-        double m = 0.0;
-        for (unsigned g = 0; g < gridDim.x; g++) m = fmax(m, ((volatile double *)t.partial)[g]);
-        t.mb[0] = m;
-        *t.count = 0;
-    }
+    double v[1] = {mx};
+    if (grid_reduce<1, MaxOp>(v, t.partial, t.count) && threadIdx.x == 0) t.mb[0] = v[0];
 }
 
 // set_boundary_conditions (transport.cc:455-495) followed by VecScale(est)
@@ -224,7 +202,6 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
 template <int M>
 __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__restrict__ c_old,
                                              double *__restrict__ c_new, double tdiff) {
-    __shared__ double sh[4][M > 0 ? M : 1];
     double bf[M];
     for (int s = 0; s < M; s++) bf[s] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
@@ -246,26 +223,8 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
                 bf[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * c_old[(size_t)i * M + s];
         }
     }
-    for (int s = 0; s < M; s++) {
-        double v = bf[s];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][s] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int s = 0; s < M; s++) {
-            double v = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += sh[w][s];
-            t.partial[(size_t)blockIdx.x * M + s] = v;
-        }
-    if (tm_last_block(t.count) && threadIdx.x == 0) {
-        for (int s = 0; s < M; s++) {
-            double tot = 0.0;
-            for (unsigned g = 0; g < gridDim.x; g++) tot += ((volatile double *)t.partial)[(size_t)g * M + s];
-            t.mb[1 + s] += tot * tdiff;  // integrated_fluxes
-        }
-        *t.count = 0;
-    }
+    if (grid_reduce<M>(bf, t.partial, t.count) && threadIdx.x == 0)
+        for (int s = 0; s < M; s++) t.mb[1 + s] += bf[s] * tdiff;  // integrated_fluxes
 }
 
 // ---------------------------------------------------------------------------
@@ -406,6 +365,7 @@ void TransportGPU::set_velocity_host(const double *side_flux) {
     FL_HIP(hipStreamSynchronize(stream_));
     flux_set_ = true;
     matrix_built_ = false;  // mh_dh->time_changed() > transport_matrix_time
+    setup_done_ = false;
 }
 
 void TransportGPU::set_velocity_device(const double *d_side_flux) {
@@ -413,6 +373,13 @@ void TransportGPU::set_velocity_device(const double *d_side_flux) {
     FL_HIP(hipStreamSynchronize(stream_));
     flux_set_ = true;
     matrix_built_ = false;
+    setup_done_ = false;
+}
+
+double TransportGPU::prepare() {
+    if (!flux_set_) throw Error(-4, "velocity field not set");
+    if (!matrix_built_) build_matrix();
+    return cfl_max_step_;
 }
 
 void TransportGPU::build_matrix() {
@@ -460,8 +427,9 @@ void TransportGPU::mb_initial() {
 void TransportGPU::one_step() {
     TmDev t = TMDEV;
     int grid = std::min(grid_for(E_), 4096);
-    if (!matrix_built_) {
-        build_matrix();
+    if (!matrix_built_) build_matrix();
+    if (!setup_done_) {
+        setup_done_ = true;
         tg_.set_upper_constraint(cfl_max_step_);
         tg_.fix_dt_until_mark();
         set_bc();

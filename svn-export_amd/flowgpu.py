@@ -66,6 +66,7 @@ SIGNATURES = {
     "fl_transport_free": (None, [C.c_void_p]),
     "fl_transport_set_velocity": (C.c_int, [C.c_void_p, _f64p]),
     "fl_transport_set_velocity_from_darcy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_transport_prepare": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "fl_transport_update_solution": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
     "fl_transport_get_conc": (C.c_int, [C.c_void_p, _f64p]),
     "fl_transport_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -253,6 +254,12 @@ class ConvectionTransport:
             _check(lib().fl_transport_set_velocity_from_darcy(self.h, darcy.h))
         else:
             _check(lib().fl_transport_set_velocity(self.h, _f64(side_flux, self.mesh.n_sides)))
+
+    def prepare(self):
+        """Builds the transport matrix now and returns the CFL time step."""
+        c = C.c_double()
+        _check(lib().fl_transport_prepare(self.h, C.byref(c)))
+        return c.value
 
     def update_solution(self, t):
         s = C.c_int()
