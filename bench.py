@@ -215,7 +215,7 @@ def cpu_baseline(mesh, K, cs, bct, bcp, gpu_iters, cg_iters):
     om = O.Mesh(arrays=mesh)
     E, NB = om.n_elements, om.n_bc_elements
     z = np.zeros(NB)
-    D = O.Darcy(om, K, cs, np.ones(E), np.zeros(E), bct, bcp, z, z, z)
+    D = O.Darcy(om, K, cs, np.ones(E), np.zeros(E), bct, bcp, z, z)
     D.solve(0, rtol=1e-30, atol=0.0, max_it=cg_iters)
     st = D.stats
     per_it = st["t_solve"] / max(st["iters"], 1)

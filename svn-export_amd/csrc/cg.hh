@@ -1,0 +1,188 @@
+// Jacobi-preconditioned conjugate gradients on a SELL-64 matrix, the GPU
+// replacement of the KSPSolve CG loop (src/la/solve.cc:316-463) on the
+// condensed edge system.
+//
+// Layout: rows are grouped in slices of 64 (one wavefront); slice s holds
+// w_s = max row length entries per row, stored column-major
+// (val[slice_ptr[s] + k*64 + lane]), so every load of a wavefront is one
+// contiguous 512 B (FP64) segment.  Padding entries have value 0 and point
+// at their own row.
+//
+// Two kernels per iteration, no atomics and no device-scope fences: every
+// block writes its partial dot products to a per-block array, and every block
+// of the NEXT kernel folds the whole (small) partial array itself in a fixed
+// order.  All blocks therefore see bitwise identical scalars (alpha, beta, the
+// stop test), and the result does not depend on block scheduling.
+//   K_A(k): rz_k, rr_k, rz_{k-1} folded; stop test; beta; p = z + beta p_old;
+//           q = A p; partial p.q
+//   K_B(k): alpha = rz_k / p.q; x += alpha p; r -= alpha q; z = D^-1 r;
+//           partial r.z, r.r into the (k+1) parity buffers
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fl {
+
+constexpr int kCgBlock = 256;
+
+struct CGState {
+    double tol2, rr, bnorm2, pad;
+    int done, it, max_it, pad2;
+};
+
+template <class V>
+struct CGDev {
+    int n, nslices;
+    const int *slice_ptr;  // nslices+1, entry offsets (multiples of 64)
+    const int *col;
+    const V *val;
+    const V *dinv;
+    const V *b;
+    V *x, *r, *z, *q;
+    double *part_pq, *part_rz0, *part_rz1, *part_rr0, *part_rr1;
+    CGState *st;
+};
+
+// Sum NV per-block partial arrays (each of length G) in a fixed order; the
+// totals are returned to every thread of the block.
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double *const (&parts)[NV], int G, double (&out)[NV]) {
+    __shared__ double sh[NV][kCgBlock / 64];
+    double acc[NV];
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x)
+        for (int k = 0; k < NV; k++) acc[k] += parts[k][g];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < NV; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) sh[k][wid] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < NV; k++) {
+        double t = 0.0;
+        for (int w = 0; w < kCgBlock / 64; w++) t += sh[k][w];
+        out[k] = t;
+    }
+    __syncthreads();
+}
+
+// Block total of NV values, written by thread 0 to dst[k][blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *const (&dst)[NV]) {
+    __shared__ double sh[NV][kCgBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) sh[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; k++) {
+            double t = 0.0;
+            for (int w = 0; w < kCgBlock / 64; w++) t += sh[k][w];
+            dst[k][blockIdx.x] = t;
+        }
+}
+
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_init(CGDev<V> a, V *p0) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        V b = a.b[i];
+        a.x[i] = V(0);
+        a.r[i] = b;
+        V z = a.dinv[i] * b;
+        a.z[i] = z;
+        p0[i] = V(0);
+        v[0] += (double)b * (double)z;
+        v[1] += (double)b * (double)b;
+    }
+    double *const dst[2] = {a.part_rz0, a.part_rr0};
+    block_partials<2>(v, dst);
+}
+
+// G: number of blocks of the main kernels (length of the partial arrays)
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double rtol2, double atol2, int max_it) {
+    const double *const parts[1] = {a.part_rr0};
+    double t[1];
+    fold_partials<1>(parts, G, t);
+    if (threadIdx.x == 0) {
+        a.st->bnorm2 = t[0];
+        a.st->rr = t[0];
+        a.st->tol2 = fmax(rtol2 * t[0], atol2);
+        a.st->max_it = max_it;
+        a.st->it = 0;
+        a.st->done = 0;
+    }
+}
+
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
+                                                    V *__restrict__ p_out) {
+    CGState *st = a.st;
+    if (st->done) return;
+    const double *cur_rz = (k & 1) ? a.part_rz1 : a.part_rz0;
+    const double *cur_rr = (k & 1) ? a.part_rr1 : a.part_rr0;
+    const double *old_rz = (k & 1) ? a.part_rz0 : a.part_rz1;
+    const double *const parts[3] = {cur_rz, cur_rr, old_rz};
+    double s[3];
+    fold_partials<3>(parts, G, s);
+    double rz = s[0], rr = s[1];
+    if (rr <= st->tol2 || k >= st->max_it || !(rr == rr)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = 1;
+            st->it = k;
+            st->rr = rr;
+        }
+        return;
+    }
+    const V beta = (V)(k == 0 ? 0.0 : rz / s[2]);
+    double v[1] = {0.0};
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+        const int row = sl * 64 + lane;
+        const int e0 = a.slice_ptr[sl], e1 = a.slice_ptr[sl + 1];
+        V acc = V(0);
+        for (int e = e0 + lane; e < e1; e += 64) {
+            int c = a.col[e];
+            acc += a.val[e] * (a.z[c] + beta * p_in[c]);
+        }
+        if (row < a.n) {
+            V p = a.z[row] + beta * p_in[row];
+            p_out[row] = p;
+            a.q[row] = acc;
+            v[0] += (double)p * (double)acc;
+        }
+    }
+    double *const dst[1] = {a.part_pq};
+    block_partials<1>(v, dst);
+}
+
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_update(CGDev<V> a, int k, int G, const V *__restrict__ p) {
+    CGState *st = a.st;
+    if (st->done) return;
+    const double *cur_rz = (k & 1) ? a.part_rz1 : a.part_rz0;
+    const double *const parts[2] = {cur_rz, a.part_pq};
+    double s[2];
+    fold_partials<2>(parts, G, s);
+    const V alpha = (V)(s[0] / s[1]);
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        a.x[i] += alpha * p[i];
+        V r = a.r[i] - alpha * a.q[i];
+        a.r[i] = r;
+        V z = a.dinv[i] * r;
+        a.z[i] = z;
+        v[0] += (double)r * (double)z;
+        v[1] += (double)r * (double)r;
+    }
+    double *const dst[2] = {(k & 1) ? a.part_rz0 : a.part_rz1, (k & 1) ? a.part_rr0 : a.part_rr1};
+    block_partials<2>(v, dst);
+}
+
+}  // namespace fl

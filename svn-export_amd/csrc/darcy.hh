@@ -11,11 +11,12 @@
 //     coefficients, w = [c.*a ; v], D = s + V, h = |e| cs f - 1^T A^-1 g:
 //        S_e = [C A^-1 C, 0; 0, diag(v)] + diag(Dirichlet/Robin terms) - w w^T / D
 //        r_e = [C A^-1 g ; 0] - rhs_edge + w h / D
-//  2. k_gather_csr / k_gather_rhs: each CSR entry / rhs entry sums its
+//  2. k_gather_sell / k_gather_rhs: each matrix entry / rhs entry sums its
 //     precomputed list of block contributions in element order
-//     (deterministic, no atomics).
-//  3. Jacobi-preconditioned CG on the SPD edge system, two fused kernels per
-//     iteration with last-block reductions and a device-side stop flag.
+//     (deterministic, no atomics), written straight into the SELL-64 layout
+//     of the edge system (rows permuted by length within windows).
+//  3. Jacobi-preconditioned CG on the SPD edge system (cg.hh): two fused
+//     kernels per iteration, fence-free reductions, device-side stop test.
 //  4. k_resolve<D>: element pressure and side fluxes from the edge traces
 //     (SchurComplement::resolve, schur.cc:714-773), written in the
 //     reference's [q | p | lambda] layout.
@@ -24,14 +25,10 @@
 
 #include <vector>
 
+#include "cg.hh"
 #include "mesh.hh"
 
 namespace fl {
-
-struct CGState {
-    double rz, rz_old, pq, rr, tol2, bnorm2;
-    int it, max_it, done, pad;
-};
 
 struct SolveStats {
     int iterations = 0;
@@ -45,7 +42,7 @@ struct SolveStats {
 
 // Device view of the mesh + pattern + fields, passed to kernels by value.
 struct DarcyDev {
-    int E, S, F, NBC, nnz;
+    int E, S, F, NBC, nnz, nslots;
     int dim_start[4];
     const double *xyz;
     const int *el_nodes, *perm, *side_off, *side_edge, *side_bnd, *bnd_bcele;
@@ -53,7 +50,7 @@ struct DarcyDev {
     const int *el_vb_ptr, *el_vb, *vb_edge, *vb_hel;
     const double *vb_hmeas;
     const int *conn_off, *conn, *blk_off;
-    const int *csr_ptr, *csr_col, *diag_pos, *cptr, *cidx, *rptr, *ridx;
+    const int *rperm, *diag_slot, *cptr, *cidx, *rptr, *ridx;
     // fields
     const double *K, *cs, *sigma, *f, *aniso;
     const int *bc_type;
@@ -99,9 +96,14 @@ class DarcyGPU {
     int *d_bc_type_ = nullptr;
     double *d_bc_p_ = nullptr, *d_bc_flux_ = nullptr, *d_bc_sigma_ = nullptr;
     double *d_x_ = nullptr, *d_r_ = nullptr, *d_z_ = nullptr, *d_p0_ = nullptr, *d_p1_ = nullptr, *d_q_ = nullptr;
-    double *d_partial_ = nullptr, *d_sol_ = nullptr;
+    double *d_sol_ = nullptr;
+    double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
-    unsigned *d_count_ = nullptr;
+    int cg_grid_ = 1024;
+    // SELL-64 edge system
+    int nslices_ = 0;
+    int *d_slice_ptr_ = nullptr, *d_sell_col_ = nullptr;
+    std::vector<int> h_csr_ptr_, h_csr_col_, h_csr2slot_, h_rperm_;
     bool have_data_ = false;
     std::vector<hipEvent_t> ev_pool_;
     std::vector<double> h_bc_measure_;

@@ -134,102 +134,23 @@ __global__ void __launch_bounds__(256) k_assemble(const DarcyDev dv) {
     }
 }
 
-__global__ void k_gather_csr(const DarcyDev dv) {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < dv.nnz; k += gridDim.x * blockDim.x) {
+// matrix values straight into their SELL slots (padding slots sum nothing)
+__global__ void k_gather_sell(const DarcyDev dv) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < dv.nslots; k += gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int j = dv.cptr[k]; j < dv.cptr[k + 1]; j++) s += dv.blk[dv.cidx[j]];
         dv.val[k] = s;
     }
 }
 
+// rhs and Jacobi diagonal in the permuted (SELL) row order
 __global__ void k_gather_rhs(const DarcyDev dv) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int j = dv.rptr[f]; j < dv.rptr[f + 1]; j++) s += dv.rhsc[dv.ridx[j]];
         dv.rhs[f] = s;
-        double d = dv.val[dv.diag_pos[f]];
+        double d = dv.val[dv.diag_slot[f]];
         dv.dinv[f] = d != 0.0 ? 1.0 / d : 1.0;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Jacobi-preconditioned CG (KSPSolve, solve.cc:453) with fused kernels.
-// ---------------------------------------------------------------------------
-struct CGArgs {
-    int n;
-    const int *ptr, *col;
-    const double *val, *b, *dinv;
-    double *x, *r, *z, *q, *partial;
-    CGState *st;
-    unsigned *count;
-};
-
-__global__ void __launch_bounds__(256) k_cg_init(CGArgs a, double rtol2, double atol2, int max_it) {
-    double v[2] = {0.0, 0.0};  // r.z, r.r
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
-        double b = a.b[i];
-        a.x[i] = 0.0;
-        a.r[i] = b;
-        double z = a.dinv[i] * b;
-        a.z[i] = z;
-        a.q[i] = 0.0;
-        v[0] += b * z;
-        v[1] += b * b;
-    }
-    if (grid_reduce<2>(v, a.partial, a.count) && threadIdx.x == 0) {
-        CGState *st = a.st;
-        st->rz = v[0];
-        st->rz_old = 1.0;
-        st->rr = v[1];
-        st->bnorm2 = v[1];
-        st->tol2 = fmax(rtol2 * v[1], atol2);
-        st->it = 0;
-        st->max_it = max_it;
-        st->done = (v[1] <= st->tol2) ? 1 : 0;
-    }
-}
-
-// p_out = z + beta p_in ; q = A p_out (gathered as z + beta p_in) ; p.q
-__global__ void __launch_bounds__(256) k_cg_spmv(CGArgs a, const double *__restrict__ p_in, double *__restrict__ p_out) {
-    const CGState *st = a.st;
-    if (st->done) return;
-    double beta = st->it == 0 ? 0.0 : st->rz / st->rz_old;
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int k = a.ptr[i]; k < a.ptr[i + 1]; k++) {
-            int c = a.col[k];
-            s += a.val[k] * (a.z[c] + beta * p_in[c]);
-        }
-        double p = a.z[i] + beta * p_in[i];
-        p_out[i] = p;
-        a.q[i] = s;
-        v[0] += p * s;
-    }
-    if (grid_reduce<1>(v, a.partial, a.count) && threadIdx.x == 0) a.st->pq = v[0];
-}
-
-// x += alpha p ; r -= alpha q ; z = D^-1 r ; r.z, r.r ; stop test
-__global__ void __launch_bounds__(256) k_cg_update(CGArgs a, const double *__restrict__ p) {
-    CGState *st = a.st;
-    if (st->done) return;
-    double alpha = st->rz / st->pq;
-    double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
-        a.x[i] += alpha * p[i];
-        double r = a.r[i] - alpha * a.q[i];
-        a.r[i] = r;
-        double z = a.dinv[i] * r;
-        a.z[i] = z;
-        v[0] += r * z;
-        v[1] += r * r;
-    }
-    if (grid_reduce<2>(v, a.partial, a.count) && threadIdx.x == 0) {
-        st->rz_old = st->rz;
-        st->rz = v[0];
-        st->rr = v[1];
-        st->it += 1;
-        if (v[1] <= st->tol2 || st->it >= st->max_it || !(v[1] == v[1])) st->done = 1;
     }
 }
 
@@ -269,9 +190,10 @@ __global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double
     dv.sol[dv.S + e] = pe;
 }
 
-__global__ void k_copy_lambda(const DarcyDev dv, const double *__restrict__ lam) {
+// edge traces from the permuted CG solution into the [q | p | lambda] vector
+__global__ void k_scatter_lambda(const DarcyDev dv, const double *__restrict__ x) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x)
-        dv.sol[dv.S + dv.E + f] = lam[f];
+        dv.sol[dv.S + dv.E + dv.rperm[f]] = x[f];
 }
 
 // ---------------------------------------------------------------------------
@@ -375,14 +297,83 @@ void DarcyGPU::build_pattern() {
         for (long i = 0; i < blk_total; i++) cidx[fp[ckey[i]]++] = (int)i;
     }
     std::vector<int>().swap(ckey);
-    // rhs contributions per edge
-    std::vector<int> rptr(F + 1, 0), ridx(conn_off[E]);
-    for (int i = 0; i < conn_off[E]; i++) rptr[conn[i] + 1]++;
-    for (int f = 0; f < F; f++) rptr[f + 1] += rptr[f];
+    // SELL-64 layout: rows sorted by decreasing length inside windows of 4096
+    // rows (keeps the gather locality of the element ordering), 64-row slices
+    // stored column-major, padding entries point at their own row.
+    const int kWin = 4096;
+    std::vector<int> rperm(F), rinv(F);
     {
-        std::vector<int> fp(rptr.begin(), rptr.end() - 1);
-        for (int i = 0; i < conn_off[E]; i++) ridx[fp[conn[i]]++] = i;
+        std::vector<int> idx;
+        for (int w0 = 0; w0 < F; w0 += kWin) {
+            int w1 = std::min(F, w0 + kWin);
+            idx.resize(w1 - w0);
+            std::iota(idx.begin(), idx.end(), w0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+                return csr_ptr[a + 1] - csr_ptr[a] > csr_ptr[b + 1] - csr_ptr[b];
+            });
+            std::copy(idx.begin(), idx.end(), rperm.begin() + w0);
+        }
+        for (int i = 0; i < F; i++) rinv[rperm[i]] = i;
     }
+    nslices_ = (F + 63) / 64;
+    std::vector<int> slice_ptr(nslices_ + 1, 0);
+    for (int sl = 0; sl < nslices_; sl++) {
+        int w = 0;
+        for (int l = 0; l < 64 && sl * 64 + l < F; l++) {
+            int o = rperm[sl * 64 + l];
+            w = std::max(w, csr_ptr[o + 1] - csr_ptr[o]);
+        }
+        slice_ptr[sl + 1] = slice_ptr[sl] + 64 * w;
+    }
+    const int nslots = slice_ptr[nslices_];
+    std::vector<int> sell_col(nslots), csr2slot(nnz_), slot_csr(nslots, -1);
+    for (int sl = 0; sl < nslices_; sl++) {
+        int w = (slice_ptr[sl + 1] - slice_ptr[sl]) / 64;
+        for (int l = 0; l < 64; l++) {
+            int row = sl * 64 + l;
+            int o = row < F ? rperm[row] : -1;
+            int len = o >= 0 ? csr_ptr[o + 1] - csr_ptr[o] : 0;
+            for (int k = 0; k < w; k++) {
+                int slot = slice_ptr[sl] + k * 64 + l;
+                if (k < len) {
+                    int pos = csr_ptr[o] + k;
+                    sell_col[slot] = rinv[csr_col[pos]];
+                    csr2slot[pos] = slot;
+                    slot_csr[slot] = pos;
+                } else {
+                    sell_col[slot] = std::min(row, F - 1);
+                }
+            }
+        }
+    }
+    std::vector<int> scptr(nslots + 1, 0), scidx(blk_total);
+    for (int sl = 0; sl < nslots; sl++) {
+        int pos = slot_csr[sl];
+        scptr[sl + 1] = scptr[sl] + (pos >= 0 ? cptr[pos + 1] - cptr[pos] : 0);
+        if (pos >= 0) std::copy(cidx.begin() + cptr[pos], cidx.begin() + cptr[pos + 1], scidx.begin() + scptr[sl]);
+    }
+    std::vector<int>().swap(cidx);
+    std::vector<int>().swap(cptr);
+    std::vector<int> diag_slot(F);
+    for (int i = 0; i < F; i++) diag_slot[i] = csr2slot[diag[rperm[i]]];
+    // rhs contributions per permuted row
+    std::vector<int> ecount(F + 1, 0), rptr(F + 1, 0), ridx(conn_off[E]);
+    for (int i = 0; i < conn_off[E]; i++) ecount[conn[i] + 1]++;
+    for (int f = 0; f < F; f++) ecount[f + 1] += ecount[f];  // original-order offsets
+    {
+        std::vector<int> orig(conn_off[E]), fp(ecount.begin(), ecount.end() - 1);
+        for (int i = 0; i < conn_off[E]; i++) orig[fp[conn[i]]++] = i;
+        for (int nr = 0; nr < F; nr++) {
+            int o = rperm[nr];
+            int c = ecount[o + 1] - ecount[o];
+            rptr[nr + 1] = rptr[nr] + c;
+            std::copy(orig.begin() + ecount[o], orig.begin() + ecount[o + 1], ridx.begin() + rptr[nr]);
+        }
+    }
+    h_csr_ptr_ = csr_ptr;
+    h_csr_col_ = csr_col;
+    h_csr2slot_ = csr2slot;
+    h_rperm_ = rperm;
     // vb: higher-dim element and measure of its side on the vb edge
     std::vector<int> vb_hel(std::max(V, 1), 0);
     std::vector<double> vb_hmeas(std::max(V, 1), 0.0);
@@ -397,7 +388,7 @@ void DarcyGPU::build_pattern() {
 
     hipStream_t s = stream_;
     DarcyDev &d = dv_;
-    d.E = E; d.S = S; d.F = F; d.NBC = NBC; d.nnz = nnz_;
+    d.E = E; d.S = S; d.F = F; d.NBC = NBC; d.nnz = nnz_; d.nslots = nslots;
     for (int i = 0; i < 4; i++) d.dim_start[i] = ds[i];
     d.xyz = up(owned_, m.xyz, s);
     d.el_nodes = up(owned_, m.el_nodes, s);
@@ -419,11 +410,12 @@ void DarcyGPU::build_pattern() {
     d.conn_off = up(owned_, conn_off, s);
     d.conn = up(owned_, conn, s);
     d.blk_off = up(owned_, blk_off, s);
-    d.csr_ptr = up(owned_, csr_ptr, s);
-    d.csr_col = up(owned_, csr_col, s);
-    d.diag_pos = up(owned_, diag, s);
-    d.cptr = up(owned_, cptr, s);
-    d.cidx = up(owned_, cidx, s);
+    d.rperm = up(owned_, rperm, s);
+    d.diag_slot = up(owned_, diag_slot, s);
+    d.cptr = up(owned_, scptr, s);
+    d.cidx = up(owned_, scidx, s);
+    d_slice_ptr_ = up(owned_, slice_ptr, s);
+    d_sell_col_ = up(owned_, sell_col, s);
     d.rptr = up(owned_, rptr, s);
     d.ridx = up(owned_, ridx, s);
     auto alloc_d = [&](size_t n) {
@@ -433,19 +425,16 @@ void DarcyGPU::build_pattern() {
     };
     d.blk = alloc_d(blk_total);
     d.rhsc = alloc_d(conn_off[E]);
-    d.val = alloc_d(nnz_);
+    d.val = alloc_d(nslots);
     d.rhs = alloc_d(F);
     d.dinv = alloc_d(F);
     d_sol_ = alloc_d((size_t)S + E + F);
     d.sol = d_sol_;
     d_x_ = alloc_d(F); d_r_ = alloc_d(F); d_z_ = alloc_d(F);
     d_p0_ = alloc_d(F); d_p1_ = alloc_d(F); d_q_ = alloc_d(F);
-    d_partial_ = alloc_d(2 * 4096);
+    d_part_ = alloc_d(5 * (size_t)cg_grid_);
     d_state_ = dev_alloc<CGState>(1);
     owned_.push_back(d_state_);
-    d_count_ = dev_alloc<unsigned>(1);
-    owned_.push_back(d_count_);
-    FL_HIP(hipMemsetAsync(d_count_, 0, sizeof(unsigned), s));
     d_K_ = alloc_d(E); d_cs_ = alloc_d(E); d_sigma_ = alloc_d(E); d_f_ = alloc_d(E);
     d_bc_type_ = dev_alloc<int>(std::max(NBC, 1));
     owned_.push_back(d_bc_type_);
@@ -501,21 +490,30 @@ void DarcyGPU::assemble() {
         if (D == 2) hipLaunchKernelGGL(k_assemble<2>, g, b, 0, stream_, d);
         if (D == 3) hipLaunchKernelGGL(k_assemble<3>, g, b, 0, stream_, d);
     }
-    int g1 = std::min(grid_for(d.nnz), 8192);
-    hipLaunchKernelGGL(k_gather_csr, dim3(g1), dim3(kBlock), 0, stream_, d);
+    int g1 = std::min(grid_for(d.nslots), 8192);
+    hipLaunchKernelGGL(k_gather_sell, dim3(g1), dim3(kBlock), 0, stream_, d);
     int g2 = std::min(grid_for(d.F), 8192);
     hipLaunchKernelGGL(k_gather_rhs, dim3(g2), dim3(kBlock), 0, stream_, d);
     FL_HIP(hipGetLastError());
 }
 
 int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
-    CGArgs a;
+    CGDev<double> a;
     a.n = dv_.F;
-    a.ptr = dv_.csr_ptr; a.col = dv_.csr_col; a.val = dv_.val; a.b = dv_.rhs; a.dinv = dv_.dinv;
-    a.x = d_x_; a.r = d_r_; a.z = d_z_; a.q = d_q_; a.partial = d_partial_; a.st = d_state_; a.count = d_count_;
-    int grid = std::min(grid_for(a.n), 4096);
-    FL_HIP(hipMemsetAsync(d_p0_, 0, a.n * sizeof(double), stream_));
-    hipLaunchKernelGGL(k_cg_init, dim3(grid), dim3(kBlock), 0, stream_, a, rtol * rtol, atol * atol, max_it);
+    a.nslices = nslices_;
+    a.slice_ptr = d_slice_ptr_;
+    a.col = d_sell_col_;
+    a.val = dv_.val;
+    a.dinv = dv_.dinv;
+    a.b = dv_.rhs;
+    a.x = d_x_; a.r = d_r_; a.z = d_z_; a.q = d_q_;
+    const int G = cg_grid_;
+    a.part_pq = d_part_; a.part_rz0 = d_part_ + G; a.part_rz1 = d_part_ + 2 * G;
+    a.part_rr0 = d_part_ + 3 * G; a.part_rr1 = d_part_ + 4 * G;
+    a.st = d_state_;
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
+    hipLaunchKernelGGL(k_cg_setup<double>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
+                       max_it);
     double *pin = d_p0_, *pout = d_p1_;
     CGState h{};
     int launched = 0;
@@ -542,9 +540,11 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
                 e0 = next_ev(); next_ev(); next_ev();
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
-            hipLaunchKernelGGL(k_cg_spmv, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pin, pout);
+            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
+                               (const double *)pin, pout);
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
-            hipLaunchKernelGGL(k_cg_update, dim3(grid), dim3(kBlock), 0, stream_, a, (const double *)pout);
+            hipLaunchKernelGGL(k_cg_update<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
+                               (const double *)pout);
             if (samp) {
                 FL_HIP(hipEventRecord(ev_pool_[e0 + 2], stream_));
                 samples.push_back(Sample{e0, itx});
@@ -555,7 +555,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
         FL_HIP(hipGetLastError());
         FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
-        if (h.done || launched >= max_it + chunk) break;
+        if (h.done || launched > max_it + chunk) break;
         if (chunk < 256) chunk *= 2;
     }
     double s1 = 0.0, s2 = 0.0;
@@ -579,16 +579,17 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
 
 void DarcyGPU::resolve() {
     const DarcyDev &d = dv_;
+    hipLaunchKernelGGL(k_scatter_lambda, dim3(std::min(grid_for(d.F), 8192)), dim3(kBlock), 0, stream_, d,
+                       (const double *)d_x_);
+    const double *lam = d_sol_ + d.S + d.E;
     for (int D = 1; D <= 3; D++) {
         int n = d.dim_start[D] - d.dim_start[D - 1];
         if (n <= 0) continue;
         dim3 g(grid_for(n)), b(kBlock);
-        if (D == 1) hipLaunchKernelGGL(k_resolve<1>, g, b, 0, stream_, d, (const double *)d_x_);
-        if (D == 2) hipLaunchKernelGGL(k_resolve<2>, g, b, 0, stream_, d, (const double *)d_x_);
-        if (D == 3) hipLaunchKernelGGL(k_resolve<3>, g, b, 0, stream_, d, (const double *)d_x_);
+        if (D == 1) hipLaunchKernelGGL(k_resolve<1>, g, b, 0, stream_, d, lam);
+        if (D == 2) hipLaunchKernelGGL(k_resolve<2>, g, b, 0, stream_, d, lam);
+        if (D == 3) hipLaunchKernelGGL(k_resolve<3>, g, b, 0, stream_, d, lam);
     }
-    hipLaunchKernelGGL(k_copy_lambda, dim3(std::min(grid_for(d.F), 8192)), dim3(kBlock), 0, stream_, d,
-                       (const double *)d_x_);
     FL_HIP(hipGetLastError());
 }
 
@@ -622,11 +623,14 @@ void DarcyGPU::get_solution(double *out) const {
 
 void DarcyGPU::get_edge_system(int *ptr, int *col, double *val, double *rhs) const {
     const int F = m_.n_edges();
-    FL_HIP(hipMemcpyAsync(ptr, dv_.csr_ptr, (F + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(col, dv_.csr_col, nnz_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(val, dv_.val, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(rhs, dv_.rhs, F * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    std::vector<double> sv(dv_.nslots), rp(F);
+    FL_HIP(hipMemcpyAsync(sv.data(), dv_.val, sv.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(rp.data(), dv_.rhs, F * sizeof(double), hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
+    std::copy(h_csr_ptr_.begin(), h_csr_ptr_.end(), ptr);
+    std::copy(h_csr_col_.begin(), h_csr_col_.end(), col);
+    for (int k = 0; k < nnz_; k++) val[k] = sv[h_csr2slot_[k]];
+    for (int i = 0; i < F; i++) rhs[h_rperm_[i]] = rp[i];
 }
 
 void DarcyGPU::element_balance(double *out) const {
@@ -672,7 +676,7 @@ void DarcyGPU::water_balance(double *bal, double *plus, double *minus) const {
 double DarcyGPU::solution_precision(double rtol, double atol) const {
     // darcy_flow_mh.cc:463-482 uses the norm of the first Schur complement's
     // rhs; the condensed system's rhs norm is reported here.
-    std::vector<double> rhs(m_.n_edges());
+    std::vector<double> rhs(m_.n_edges());  // permuted order; only the norm is used
     FL_HIP(hipMemcpyAsync(rhs.data(), dv_.rhs, rhs.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
     double bn = 0.0;

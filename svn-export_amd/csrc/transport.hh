@@ -74,6 +74,11 @@ class TransportGPU {
     void set_bc();
     void one_step();
     void mb_initial();
+    void flush_flux();
+    int step_grid() const;
+    static constexpr int kMaxStepGrid = 1024;
+    bool pending_ = false, step_parity_ = false;
+    double pending_tdiff_ = 0.0;
 
     const HostMesh &m_;
     hipStream_t stream_ = nullptr;

@@ -198,10 +198,36 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
     }
 }
 
-// c_new = tm c_old + bcv (all substances), boundary mass flux reduction
+// Folds one step's per-block boundary-flux partials (G blocks x M) in a fixed
+// order and adds them, times that step's length, to the integrated flux
+// (MassBalance::calculate, mass_balance.cc:208-221).  Called by one block.
+template <int M>
+__device__ __forceinline__ void fold_flux(const TmDev &t, const double *__restrict__ buf, int G, double tdiff) {
+    __shared__ double sh[M][4];
+    double acc[M];
+    for (int s = 0; s < M; s++) acc[s] = 0.0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x)
+        for (int s = 0; s < M; s++) acc[s] += buf[(size_t)g * M + s];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int s = 0; s < M; s++) {
+        double v = acc[s];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) sh[s][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int s = 0; s < M; s++) t.mb[1 + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
+    __syncthreads();
+}
+
+// c_new = tm c_old + bcv for all substances; per-block boundary mass flux
+// partials of this step into out_buf; block 0 first folds the previous step's
+// partials (fold_buf) so no grid-wide synchronisation is needed.
 template <int M>
 __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__restrict__ c_old,
-                                             double *__restrict__ c_new, double tdiff) {
+                                             double *__restrict__ c_new, const double *__restrict__ fold_buf,
+                                             double fold_tdiff, int G, double *__restrict__ out_buf) {
+    if (fold_buf && blockIdx.x == 0) fold_flux<M>(t, fold_buf, G, fold_tdiff);
     double bf[M];
     for (int s = 0; s < M; s++) bf[s] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
@@ -223,8 +249,22 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
                 bf[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * c_old[(size_t)i * M + s];
         }
     }
-    if (grid_reduce<M>(bf, t.partial, t.count) && threadIdx.x == 0)
-        for (int s = 0; s < M; s++) t.mb[1 + s] += bf[s] * tdiff;  // integrated_fluxes
+    __shared__ double sh[M][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int s = 0; s < M; s++) {
+        double v = bf[s];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) sh[s][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int s = 0; s < M; s++) out_buf[(size_t)blockIdx.x * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
+}
+
+template <int M>
+__global__ void __launch_bounds__(256) k_flux_flush(const TmDev t, const double *__restrict__ buf, int G,
+                                                   double tdiff) {
+    fold_flux<M>(t, buf, G, tdiff);
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +464,23 @@ void TransportGPU::mb_initial() {
     mb_init_ = false;
 }
 
+int TransportGPU::step_grid() const { return std::min(grid_for(E_), kMaxStepGrid); }
+
+void TransportGPU::flush_flux() {
+    if (!pending_) return;
+    TmDev t = TMDEV;
+    const int G = step_grid();
+    const double *last = d_partial_ + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 4;
+    switch (nsub_) {
+        case 1: hipLaunchKernelGGL(k_flux_flush<1>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
+        case 2: hipLaunchKernelGGL(k_flux_flush<2>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
+        case 3: hipLaunchKernelGGL(k_flux_flush<3>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
+        case 4: hipLaunchKernelGGL(k_flux_flush<4>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
+    }
+    FL_HIP(hipGetLastError());
+    pending_ = false;
+}
+
 void TransportGPU::one_step() {
     TmDev t = TMDEV;
     int grid = std::min(grid_for(E_), 4096);
@@ -450,13 +507,20 @@ void TransportGPU::one_step() {
     last_time_ = tg_.time;
     const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
     double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
+    const int G = step_grid();
+    double *out = d_partial_ + (size_t)(step_parity_ ? 1 : 0) * kMaxStepGrid * 4;
+    const double *fold = pending_ ? d_partial_ + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 4 : nullptr;
+    double ftd = pending_tdiff_;
     switch (nsub_) {
-        case 1: hipLaunchKernelGGL(k_step<1>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
-        case 2: hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
-        case 3: hipLaunchKernelGGL(k_step<3>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
-        case 4: hipLaunchKernelGGL(k_step<4>, dim3(grid), dim3(kBlock), 0, stream_, t, cold, cnew, tdiff); break;
+        case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
+        case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
+        case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
+        case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
         default: throw Error(-5, "transport supports 1..4 substances per handle");
     }
+    pending_ = true;
+    pending_tdiff_ = tdiff;
+    step_parity_ = !step_parity_;
     FL_HIP(hipGetLastError());
     cur_is_c0_ = !cur_is_c0_;
     stats_.steps++;
@@ -481,6 +545,7 @@ int TransportGPU::update_solution(double target) {
         if (!(tg_.time > t_before) || !std::isfinite(tg_.time))
             throw Error(-7, "transport time step does not advance (dt = " + std::to_string(tg_.time_step) + ")");
     }
+    flush_flux();
     FL_HIP(hipEventRecord(b, stream_));
     FL_HIP(hipEventSynchronize(b));
     float ms = 0;
