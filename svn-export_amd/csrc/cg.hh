@@ -140,10 +140,16 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
     }
     const V beta = (V)(k == 0 ? 0.0 : rz / s[2]);
     double v[1] = {0.0};
+    // XCD-aware mapping: the hardware dispatches block b to XCD b % 8, so the
+    // blocks of one XCD get one contiguous eighth of the slices and the
+    // gathered z / p entries stay inside that XCD's L2.
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = gridDim.x >> 3;
+    const int per = (a.nslices + 7) >> 3;
+    const int s_lo = xcd * per, s_hi = min(a.nslices, s_lo + per);
+    const int wave = lb * (kCgBlock / 64) + (threadIdx.x >> 6);
+    const int nwaves = nlb * (kCgBlock / 64);
+    for (int sl = s_lo + wave; sl < s_hi; sl += nwaves) {
         const int row = sl * 64 + lane;
         const int e0 = a.slice_ptr[sl], e1 = a.slice_ptr[sl + 1];
         V acc = V(0);
