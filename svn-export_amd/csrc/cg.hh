@@ -140,22 +140,31 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
     }
     const V beta = (V)(k == 0 ? 0.0 : rz / s[2]);
     double v[1] = {0.0};
-    // XCD-aware mapping: the hardware dispatches block b to XCD b % 8, so the
-    // blocks of one XCD get one contiguous eighth of the slices and the
-    // gathered z / p entries stay inside that XCD's L2.
     const int lane = threadIdx.x & 63;
-    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = gridDim.x >> 3;
-    const int per = (a.nslices + 7) >> 3;
-    const int s_lo = xcd * per, s_hi = min(a.nslices, s_lo + per);
-    const int wave = lb * (kCgBlock / 64) + (threadIdx.x >> 6);
-    const int nwaves = nlb * (kCgBlock / 64);
-    for (int sl = s_lo + wave; sl < s_hi; sl += nwaves) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int sl = wave; sl < a.nslices; sl += nwaves) {
         const int row = sl * 64 + lane;
-        const int e0 = a.slice_ptr[sl], e1 = a.slice_ptr[sl + 1];
+        const int e0 = a.slice_ptr[sl] + lane;
+        const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
         V acc = V(0);
-        for (int e = e0 + lane; e < e1; e += 64) {
-            int c = a.col[e];
-            acc += a.val[e] * (a.z[c] + beta * p_in[c]);
+        int k = 0;
+        // 4 entries per step: all index / value loads issue before the gathers
+        for (; k + 4 <= w; k += 4) {
+            const int i0 = e0 + k * 64;
+            int c0 = a.col[i0], c1 = a.col[i0 + 64], c2 = a.col[i0 + 128], c3 = a.col[i0 + 192];
+            V v0 = a.val[i0], v1 = a.val[i0 + 64], v2 = a.val[i0 + 128], v3 = a.val[i0 + 192];
+            V g0 = a.z[c0] + beta * p_in[c0], g1 = a.z[c1] + beta * p_in[c1];
+            V g2 = a.z[c2] + beta * p_in[c2], g3 = a.z[c3] + beta * p_in[c3];
+            acc += v0 * g0;
+            acc += v1 * g1;
+            acc += v2 * g2;
+            acc += v3 * g3;
+        }
+        for (; k < w; k++) {
+            const int i0 = e0 + k * 64;
+            int c = a.col[i0];
+            acc += a.val[i0] * (a.z[c] + beta * p_in[c]);
         }
         if (row < a.n) {
             V p = a.z[row] + beta * p_in[row];
