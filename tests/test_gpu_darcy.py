@@ -157,3 +157,14 @@ def test_darcy_repeat_and_new_realization(tmp_path):
                  d2["bc_sigma"])
     D0.solve(0, rtol=1e-15)
     assert rel(c, D0.solution) < 1e-9
+
+
+def test_cpp_adapter_example():
+    """The C++ adapter (DarcyFlowMH_Steady / ConvectionTransport method set) over the C ABI."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(G.LIB_PATH), "examples", "steady_flow")
+    assert os.path.exists(exe), "build() compiles the example"
+    out = subprocess.run([exe, cases.ref_path("test01_mesh.msh")], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    assert "darcy:" in out.stdout and "transport:" in out.stdout
