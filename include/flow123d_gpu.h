@@ -172,10 +172,16 @@ int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl);
 
 /* MassBalance::calculate (mass_balance.cc:79-239) state after the last step:
  * bal/plus/minus[n_subst][n_boundary_regions], mass[n_subst][n_bulk_regions],
+ * src[n_subst][n_bulk_regions] (source term of the last step),
  * totals[n_subst][4] = initial mass, integrated boundary flux, integrated
  * sources, current mass. */
-int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass,
+int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass, double *src,
                               double *totals);
+
+/* Concentration sources (ConvectionTransport::compute_concentration_sources,
+ * transport.cc:501-544): sources_density, sources_sigma, sources_conc, each
+ * [n_subst][E]; the step adds (density + max(conc_src - c, 0) sigma) dt. */
+int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc);
 
 /* Scaled transport matrix tm = I + dt*A (create_transport_matrix_mpi,
  * transport.cc:788-943, after the scaling at :628-648). */

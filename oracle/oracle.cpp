@@ -1349,6 +1349,7 @@ struct Transport {
     std::vector<double> bc_conc;    // [nsub][NBC]
     std::vector<double> conc, pconc;  // [nsub][E]
     std::vector<double> bcvcorr;    // [nsub][E]
+    std::vector<double> src_density, src_sigma, src_conc;  // [nsub][E], empty = no sources
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -1479,6 +1480,13 @@ static void mb_calculate(Transport &T, double time) {
             const Element &el = m.element[e];
             int r = m.rdb.regs[el.region].sub_idx;
             T.mass[k * T.nblk + r] += T.por[e] * T.cs[e] * T.conc[(size_t)k * T.E + e] * elem_measure(m, el);
+            if (!T.src_density.empty()) {
+                // compute_concentration_sources_for_mass_balance (transport.cc:546-590) with pconc
+                size_t j = (size_t)k * T.E + e;
+                double diff = T.src_conc[j] - T.pconc[j];
+                double snd = diff > 0.0 ? T.src_density[j] + diff * T.src_sigma[j] : T.src_density[j];
+                T.src[k * T.nblk + r] += snd * elem_measure(m, el);
+            }
         }
     std::vector<double> tot_bal(ns, 0.0), tot_src(ns, 0.0), mass_tot(ns, 0.0);
     for (int k = 0; k < ns; k++) {
@@ -1526,12 +1534,21 @@ static void one_step(Transport &T) {
     T.tg.next_time();
     T.pconc = T.conc;
     for (int k = 0; k < T.nsub; k++) {
-        // cumulative_corr = bcvcorr + sources_corr (sources are zero here)
+        // compute_concentration_sources (transport.cc:501-544) with the dt of
+        // this step, then cumulative_corr = bcvcorr + sources_corr (:676)
         const double *cp = &T.pconc[(size_t)k * T.E];
         double *c = &T.conc[(size_t)k * T.E];
         const double *bc = &T.bcvcorr[(size_t)k * T.E];
+        double dt = T.tg.time_step;
         for (int i = 0; i < T.E; i++) {
-            double s = bc[i] + 0.0;
+            double src = 0.0;
+            if (!T.src_density.empty()) {
+                size_t j = (size_t)k * T.E + i;
+                double diff = T.src_conc[j] - cp[i];
+                if (diff > 0.0) src = (T.src_density[j] + diff * T.src_sigma[j]) * dt;
+                else src = T.src_density[j] * dt;
+            }
+            double s = bc[i] + src;
             for (int p = T.tm.ptr[i]; p < T.tm.ptr[i + 1]; p++) s += T.tm.val[p] * cp[T.tm.col[p]];
             c[i] = s;
         }
@@ -1588,6 +1605,14 @@ void *orc_transport_create(void *mesh, const double *side_flux, const double *cs
 }
 void orc_transport_free(void *h) { delete (Transport *)h; }
 
+void orc_transport_set_sources(void *h, const double *density, const double *sigma, const double *conc) {
+    Transport &T = *(Transport *)h;
+    size_t n = (size_t)T.nsub * T.E;
+    T.src_density.assign(density, density + n);
+    T.src_sigma.assign(sigma, sigma + n);
+    T.src_conc.assign(conc, conc + n);
+}
+
 // Advance to target time t (TransportOperatorSplitting::update_solution,
 // transport_operator_splitting.cc:244-287); returns steps taken.
 int orc_transport_update(void *h, double t) {
@@ -1623,8 +1648,10 @@ int orc_transport_matrix(void *h, int *ptr, int *col, double *val) {
 }
 // Mass balance (last calculate): bcd_bal/plus/minus [nsub][nbr], mass [nsub][nblk],
 // totals[4*nsub]: initial mass, integrated fluxes, integrated sources, current mass
-void orc_transport_mass_balance(void *h, double *bal, double *plus, double *minus, double *mass, double *totals) {
+void orc_transport_mass_balance(void *h, double *bal, double *plus, double *minus, double *mass, double *src,
+                                double *totals) {
     Transport &T = *(Transport *)h;
+    std::memcpy(src, T.src.data(), T.src.size() * sizeof(double));
     std::memcpy(bal, T.bcd_bal.data(), T.bcd_bal.size() * sizeof(double));
     std::memcpy(plus, T.bcd_plus.data(), T.bcd_plus.size() * sizeof(double));
     std::memcpy(minus, T.bcd_minus.data(), T.bcd_minus.size() * sizeof(double));

@@ -61,7 +61,8 @@ def lib():
         L.orc_transport_conc.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_matrix.restype = C.c_int
         L.orc_transport_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 5
+        L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
+        L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         _LIB = L
     return _LIB
 
@@ -236,11 +237,15 @@ class Transport:
         nblk = sum(1 for r in self.mesh.regions if not r["boundary"])
         ns = self.nsub
         bal, plus, minus = np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1))
-        mass, tot = np.zeros(ns * max(nblk, 1)), np.zeros(4 * ns)
-        lib().orc_transport_mass_balance(self.h, bal, plus, minus, mass, tot)
+        mass, src, tot = np.zeros(ns * max(nblk, 1)), np.zeros(ns * max(nblk, 1)), np.zeros(4 * ns)
+        lib().orc_transport_mass_balance(self.h, bal, plus, minus, mass, src, tot)
         return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
                     minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
-                    totals=tot.reshape(ns, 4))
+                    src=src[:ns * nblk].reshape(ns, nblk), totals=tot.reshape(ns, 4))
+
+    def set_sources(self, density, sigma, conc):
+        self._src = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (density, sigma, conc)]
+        lib().orc_transport_set_sources(self.h, *self._src)
 
     def step_seconds(self):
         return lib().orc_transport_step_seconds(self.h)

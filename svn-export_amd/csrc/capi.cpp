@@ -288,10 +288,17 @@ int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl) {
     return 0;
 }
 
-int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass,
+int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass, double *src,
                               double *totals) {
     return guard([&] {
-        t->g->mass_balance(bal, plus, minus, mass, totals);
+        t->g->mass_balance(bal, plus, minus, mass, src, totals);
+        return 0;
+    });
+}
+
+int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc) {
+    return guard([&] {
+        t->g->set_sources(density, sigma, conc);
         return 0;
     });
 }

@@ -7,13 +7,16 @@
 //    PETSc AIJ row) is built on the host once per mesh.
 //  * k_tm_scale: tm := dt*A + I, or (tm - I)*dt'/dt + I on a time-step change
 //    (transport.cc:628-648).
-//  * k_step: c_new = tm * c_old + bcvcorr for all substances at once;
-//    concentrations are substance-interleaved [E][m] so one gathered column
-//    feeds every substance, the matrix row is read once per step.  The sum
-//    runs in the row's column order with no fused multiply-add, which is the
-//    sequential MatMultAdd order (transport.cc:676-681).  The same kernel
-//    reduces the boundary mass flux used by the cumulative mass balance
-//    (MassBalance::calculate, mass_balance.cc:79-239).
+//  * k_step: c_new = tm * c_old + bcvcorr + sources for all substances at
+//    once; concentrations are substance-interleaved [E][m] so one gathered
+//    column feeds every substance and the matrix row is read once per step.
+//    The sum runs in the row's column order with no fused multiply-add, which
+//    is the sequential MatMultAdd order (transport.cc:668-683).  Each XCD
+//    works on one contiguous eighth of the rows so the gathered columns stay
+//    in its L2.  The same kernel reduces the boundary mass flux and the source
+//    term of the cumulative mass balance (MassBalance::calculate,
+//    mass_balance.cc:79-239) without grid-wide synchronisation: step j's
+//    per-block partials are folded by block 0 of step j+1.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -46,18 +49,33 @@ struct TransportStats {
     double ms_steps = 0, ms_build = 0;
 };
 
+// Device view of the transport problem, passed to kernels by value.
+struct TmDev {
+    int E, nsub, NBC, has_src;
+    const int *ptr, *col, *kind, *s1, *s2, *diag;
+    const int *side_off, *side_edge, *side_bnd, *bnd_bcele, *edge_ptr, *edge_side;
+    const int *el_vb_ptr, *el_vb, *vb_hside;
+    const int *bl_ptr, *bl_side, *bl_bcele;  // representative boundary sides per element
+    const double *meas, *cs, *por, *flux, *edge_flow, *bc_conc;
+    const double *src_density, *src_sigma, *src_conc;  // [E][m] interleaved (when has_src)
+    double *val, *bcv, *partial, *mb;
+    unsigned *count;
+};
+
 class TransportGPU {
    public:
     // por, cs [E]; bc_conc [nsub][NBC]; init_conc [nsub][E] (host arrays)
     TransportGPU(const HostMesh &m, int nsub, const double *por, const double *cs, const double *bc_conc,
                  const double *init_conc, double end_time, double save_step);
     ~TransportGPU();
-    void set_velocity_host(const double *side_flux);       // [S]
-    void set_velocity_device(const double *d_side_flux);   // device [S] (e.g. the Darcy solution)
+    // sources_density / sources_sigma / sources_conc, each [nsub][E] (transport.cc:501-544)
+    void set_sources(const double *density, const double *sigma, const double *conc);
+    void set_velocity_host(const double *side_flux);      // [S]
+    void set_velocity_device(const double *d_side_flux);  // device [S] (e.g. the Darcy solution)
+    double prepare();                                      // build the matrix now; returns the CFL step
     int update_solution(double target_time);               // steps taken
-    double prepare();                                       // build the matrix now; returns the CFL step
     void get_conc(double *out) const;                      // [nsub][E]
-    void mass_balance(double *bal, double *plus, double *minus, double *mass, double *totals);
+    void mass_balance(double *bal, double *plus, double *minus, double *mass, double *src, double *totals);
     double time() const { return tg_.time; }
     double dt() const { return tg_.time_step; }
     double cfl() const { return cfl_max_step_; }
@@ -66,8 +84,6 @@ class TransportGPU {
     void get_matrix(int *ptr, int *col, double *val) const;
     const TransportStats &stats() const { return stats_; }
     hipStream_t stream() const { return stream_; }
-    // Capture the fixed-dt step sequence into hipGraphs (launch-overhead free).
-    void set_use_graph(bool on) { use_graph_ = on; }
 
    private:
     void build_matrix();
@@ -77,21 +93,16 @@ class TransportGPU {
     void flush_flux();
     int step_grid() const;
     static constexpr int kMaxStepGrid = 1024;
-    bool pending_ = false, step_parity_ = false;
-    double pending_tdiff_ = 0.0;
 
     const HostMesh &m_;
     hipStream_t stream_ = nullptr;
     int E_, nsub_, nnz_ = 0, NBC_;
     std::vector<void *> owned_;
-    // device
-    int *d_ptr_, *d_col_, *d_kind_, *d_s1_, *d_s2_, *d_diag_;
-    int *d_side_off_, *d_side_edge_, *d_side_bnd_, *d_bnd_bcele_, *d_edge_ptr_, *d_edge_side_;
-    int *d_el_vb_ptr_, *d_el_vb_, *d_vb_hside_, *d_bnd_rep_;
-    double *d_meas_, *d_cs_, *d_por_, *d_flux_, *d_edge_flow_, *d_val_;
-    double *d_bc_conc_, *d_bcv_, *d_c0_, *d_c1_, *d_partial_, *d_mb_;
-    unsigned *d_count_;
+    TmDev t_{};
+    double *d_c0_ = nullptr, *d_c1_ = nullptr, *d_flux_ = nullptr;
     bool cur_is_c0_ = true;
+    bool pending_ = false, step_parity_ = false;
+    double pending_tdiff_ = 0.0;
     // state
     TimeGovernorR tg_;
     double cfl_max_step_ = 0;
@@ -99,9 +110,9 @@ class TransportGPU {
     bool mb_init_ = true;
     double last_time_ = -1;
     std::vector<double> h_meas_, h_cs_, h_por_, h_bc_conc_, initial_mass_;
-    std::vector<int> h_el_region_sub_, h_bnd_rep_;
+    std::vector<double> h_src_density_, h_src_sigma_, h_src_conc_;
+    std::vector<int> h_el_region_sub_;
     TransportStats stats_;
-    bool use_graph_ = false;
 };
 
 }  // namespace fl

@@ -96,16 +96,6 @@ void TimeGovernorR::next_time() {
 // ---------------------------------------------------------------------------
 enum : int { kDiag = 0, kSame = 1, kVbLow = 2, kVbHigh = 3 };
 
-struct TmDev {
-    int E, nsub, NBC;
-    const int *ptr, *col, *kind, *s1, *s2;
-    const int *side_off, *side_edge, *side_bnd, *bnd_bcele, *edge_ptr, *edge_side;
-    const int *el_vb_ptr, *el_vb, *vb_hside, *diag, *bnd_rep;
-    const double *meas, *cs, *por, *flux, *edge_flow, *bc_conc;
-    double *val, *bcv, *partial, *mb;
-    unsigned *count;
-};
-
 __global__ void k_edge_flow(const int F, const int *__restrict__ edge_ptr, const int *__restrict__ edge_side,
                             const double *__restrict__ flux, double *__restrict__ edge_flow) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
@@ -158,7 +148,7 @@ __global__ void __launch_bounds__(256) k_tm_build(const TmDev t) {
             t.val[k] = a;
         }
     }
-    // max |a_ii| over the grid (CFL, transport.cc:911-921)
+    // max |a_ii| over the grid (CFL, transport.cc:911-921); once per flux field
     double v[1] = {mx};
     if (grid_reduce<1, MaxOp>(v, t.partial, t.count) && threadIdx.x == 0) t.mb[0] = v[0];
 }
@@ -198,67 +188,84 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
     }
 }
 
-// Folds one step's per-block boundary-flux partials (G blocks x M) in a fixed
-// order and adds them, times that step's length, to the integrated flux
-// (MassBalance::calculate, mass_balance.cc:208-221).  Called by one block.
+// Folds one step's per-block partials (G blocks x 2M: boundary mass flux and
+// source term per substance) in a fixed order and adds them, times that
+// step's length, to the integrated quantities (mass_balance.cc:208-221).
+// Executed by one block.
 template <int M>
 __device__ __forceinline__ void fold_flux(const TmDev &t, const double *__restrict__ buf, int G, double tdiff) {
-    __shared__ double sh[M][4];
-    double acc[M];
-    for (int s = 0; s < M; s++) acc[s] = 0.0;
+    __shared__ double sh[2 * M][4];
+    double acc[2 * M];
+    for (int s = 0; s < 2 * M; s++) acc[s] = 0.0;
     for (int g = threadIdx.x; g < G; g += blockDim.x)
-        for (int s = 0; s < M; s++) acc[s] += buf[(size_t)g * M + s];
+        for (int s = 0; s < 2 * M; s++) acc[s] += buf[(size_t)g * 2 * M + s];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int s = 0; s < M; s++) {
+    for (int s = 0; s < 2 * M; s++) {
         double v = acc[s];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if (lane == 0) sh[s][wid] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int s = 0; s < M; s++) t.mb[1 + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
+        for (int s = 0; s < 2 * M; s++) t.mb[1 + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
     __syncthreads();
 }
 
-// c_new = tm c_old + bcv for all substances; per-block boundary mass flux
-// partials of this step into out_buf; block 0 first folds the previous step's
-// partials (fold_buf) so no grid-wide synchronisation is needed.
+// c_new = tm c_old + (bcvcorr + sources) for all M substances; per-block
+// partials of the boundary mass flux (transport.cc:1299-1337) and of the
+// source term (transport.cc:1339-1366) of this step into out_buf; block 0
+// first folds the previous step's partials (fold_buf).
 template <int M>
 __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__restrict__ c_old,
-                                             double *__restrict__ c_new, const double *__restrict__ fold_buf,
-                                             double fold_tdiff, int G, double *__restrict__ out_buf) {
+                                             double *__restrict__ c_new, double dt,
+                                             const double *__restrict__ fold_buf, double fold_tdiff, int G,
+                                             double *__restrict__ out_buf) {
     if (fold_buf && blockIdx.x == 0) fold_flux<M>(t, fold_buf, G, fold_tdiff);
-    double bf[M];
-    for (int s = 0; s < M; s++) bf[s] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+    double bf[2 * M];
+    for (int s = 0; s < 2 * M; s++) bf[s] = 0.0;
+    // XCD-aware rows: block b runs on XCD b % 8, which owns one eighth of the rows
+    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = gridDim.x >> 3;
+    const int per = (t.E + 7) >> 3;
+    const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
+    for (int i = r_lo + lb * blockDim.x + threadIdx.x; i < r_hi; i += nlb * blockDim.x) {
         double acc[M];
-        for (int s = 0; s < M; s++) acc[s] = t.bcv[(size_t)i * M + s] + 0.0;
+        for (int s = 0; s < M; s++) acc[s] = t.bcv[(size_t)i * M + s];
+        if (t.has_src) {
+            // compute_concentration_sources (transport.cc:501-544), dt of this step
+            for (int s = 0; s < M; s++) {
+                double dens = t.src_density[(size_t)i * M + s], sig = t.src_sigma[(size_t)i * M + s];
+                double diff = t.src_conc[(size_t)i * M + s] - c_old[(size_t)i * M + s];
+                double snd = diff > 0.0 ? (dens + diff * sig) : dens;
+                acc[s] = acc[s] + snd * dt;
+                bf[M + s] += snd * t.meas[i];
+            }
+        } else {
+            for (int s = 0; s < M; s++) acc[s] = acc[s] + 0.0;
+        }
         for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
             double a = t.val[k];
             const double *cj = c_old + (size_t)t.col[k] * M;
             for (int s = 0; s < M; s++) acc[s] += a * cj[s];
         }
         for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
-        // boundary mass flux with the previous concentration (transport.cc:1299-1337)
-        for (int sd = t.side_off[i]; sd < t.side_off[i + 1]; sd++) {
-            int b = t.bnd_rep[sd];
-            if (b < 0) continue;
-            double wf = t.flux[sd];
-            int bce = t.bnd_bcele[b];
+        for (int k = t.bl_ptr[i]; k < t.bl_ptr[i + 1]; k++) {
+            double wf = t.flux[t.bl_side[k]];
+            int bce = t.bl_bcele[k];
             for (int s = 0; s < M; s++)
                 bf[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * c_old[(size_t)i * M + s];
         }
     }
-    __shared__ double sh[M][4];
+    __shared__ double sh[2 * M][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int s = 0; s < M; s++) {
+    for (int s = 0; s < 2 * M; s++) {
         double v = bf[s];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if (lane == 0) sh[s][wid] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int s = 0; s < M; s++) out_buf[(size_t)blockIdx.x * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
+        for (int s = 0; s < 2 * M; s++)
+            out_buf[(size_t)blockIdx.x * 2 * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
 }
 
 template <int M>
@@ -320,46 +327,61 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     }
     std::vector<std::map<int, Ent>>().swap(rows);
     nnz_ = (int)col.size();
-    // boundary representative sides: Boundary::side() is side 0 of its edge
-    h_bnd_rep_.assign(S, -1);
-    for (int b = 0; b < m.n_boundaries(); b++) h_bnd_rep_[m.edge_side[m.edge_ptr[m.bnd_edge[b]]]] = b;
+    // representative boundary sides per element: Boundary::side() is side 0 of its edge
+    std::vector<int> rep(S, -1);
+    for (int b = 0; b < m.n_boundaries(); b++) rep[m.edge_side[m.edge_ptr[m.bnd_edge[b]]]] = b;
+    std::vector<int> bl_ptr(E + 1, 0), bl_side, bl_bcele;
+    for (int i = 0; i < E; i++) {
+        for (int sd = m.side_off[i]; sd < m.side_off[i + 1]; sd++)
+            if (rep[sd] >= 0) {
+                bl_side.push_back(sd);
+                bl_bcele.push_back(m.bnd_bcele[rep[sd]]);
+            }
+        bl_ptr[i + 1] = (int)bl_side.size();
+    }
+    if (bl_side.empty()) { bl_side.push_back(0); bl_bcele.push_back(0); }
     h_meas_.resize(E);
     for (int i = 0; i < E; i++) h_meas_[i] = element_measure(m, m.el_dim[i], &m.el_nodes[4 * i]);
     h_cs_.assign(cs, cs + E);
     h_por_.assign(por, por + E);
     h_bc_conc_.assign(bc_conc, bc_conc + (size_t)nsub * NBC_);
     hipStream_t s = stream_;
-    d_ptr_ = upv(owned_, ptr, s); d_col_ = upv(owned_, col, s); d_kind_ = upv(owned_, kind, s);
-    d_s1_ = upv(owned_, s1, s); d_s2_ = upv(owned_, s2, s); d_diag_ = upv(owned_, diag, s);
-    d_side_off_ = upv(owned_, m.side_off, s); d_side_edge_ = upv(owned_, m.side_edge, s);
-    d_side_bnd_ = upv(owned_, m.side_bnd, s);
+    TmDev &t = t_;
+    t.E = E; t.nsub = nsub; t.NBC = NBC_; t.has_src = 0;
+    t.ptr = upv(owned_, ptr, s); t.col = upv(owned_, col, s); t.kind = upv(owned_, kind, s);
+    t.s1 = upv(owned_, s1, s); t.s2 = upv(owned_, s2, s); t.diag = upv(owned_, diag, s);
+    t.side_off = upv(owned_, m.side_off, s); t.side_edge = upv(owned_, m.side_edge, s);
+    t.side_bnd = upv(owned_, m.side_bnd, s);
     std::vector<int> bb = m.bnd_bcele;
     if (bb.empty()) bb.push_back(0);
-    d_bnd_bcele_ = upv(owned_, bb, s);
-    d_edge_ptr_ = upv(owned_, m.edge_ptr, s); d_edge_side_ = upv(owned_, m.edge_side, s);
-    d_el_vb_ptr_ = upv(owned_, m.el_vb_ptr, s);
+    t.bnd_bcele = upv(owned_, bb, s);
+    t.edge_ptr = upv(owned_, m.edge_ptr, s); t.edge_side = upv(owned_, m.edge_side, s);
+    t.el_vb_ptr = upv(owned_, m.el_vb_ptr, s);
     std::vector<int> elvb = m.el_vb;
     if (elvb.empty()) elvb.push_back(0);
-    d_el_vb_ = upv(owned_, elvb, s);
-    d_vb_hside_ = upv(owned_, vb_hside, s);
-    d_bnd_rep_ = upv(owned_, h_bnd_rep_, s);
-    d_meas_ = upv(owned_, h_meas_, s); d_cs_ = upv(owned_, h_cs_, s); d_por_ = upv(owned_, h_por_, s);
+    t.el_vb = upv(owned_, elvb, s);
+    t.vb_hside = upv(owned_, vb_hside, s);
+    t.bl_ptr = upv(owned_, bl_ptr, s); t.bl_side = upv(owned_, bl_side, s); t.bl_bcele = upv(owned_, bl_bcele, s);
+    t.meas = upv(owned_, h_meas_, s); t.cs = upv(owned_, h_cs_, s); t.por = upv(owned_, h_por_, s);
     std::vector<double> bcc = h_bc_conc_;
     if (bcc.empty()) bcc.push_back(0);
-    d_bc_conc_ = upv(owned_, bcc, s);
+    t.bc_conc = upv(owned_, bcc, s);
+    t.src_density = t.src_sigma = t.src_conc = nullptr;
     auto alloc_d = [&](size_t n) { double *p = dev_alloc<double>(n); owned_.push_back(p); return p; };
     d_flux_ = alloc_d(S);
-    d_edge_flow_ = alloc_d(F);
-    d_val_ = alloc_d(nnz_);
-    d_bcv_ = alloc_d((size_t)E * nsub);
+    t.flux = d_flux_;
+    t.edge_flow = alloc_d(F);
+    t.val = alloc_d(nnz_);
+    t.bcv = alloc_d((size_t)E * nsub);
     d_c0_ = alloc_d((size_t)E * nsub);
     d_c1_ = alloc_d((size_t)E * nsub);
-    d_partial_ = alloc_d(4096 * 8);
-    d_mb_ = alloc_d(1 + nsub);
-    d_count_ = dev_alloc<unsigned>(1);
-    owned_.push_back(d_count_);
-    FL_HIP(hipMemsetAsync(d_count_, 0, sizeof(unsigned), s));
-    FL_HIP(hipMemsetAsync(d_mb_, 0, (1 + nsub) * sizeof(double), s));
+    t.partial = alloc_d((size_t)2 * kMaxStepGrid * 8 + 4096);
+    t.mb = alloc_d(1 + 2 * nsub);
+    unsigned *cnt = dev_alloc<unsigned>(1);
+    owned_.push_back(cnt);
+    t.count = cnt;
+    FL_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
+    FL_HIP(hipMemsetAsync(t.mb, 0, (1 + 2 * nsub) * sizeof(double), s));
     FL_HIP(hipMemsetAsync(d_c1_, 0, (size_t)E * nsub * sizeof(double), s));
     // init_conc [nsub][E] -> interleaved [E][nsub]
     std::vector<double> c0((size_t)E * nsub);
@@ -379,26 +401,24 @@ TransportGPU::~TransportGPU() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-static TmDev make_tm(const TransportGPU *, int E, int nsub, int NBC, int *ptr, int *col, int *kind, int *s1, int *s2,
-                     int *side_off, int *side_edge, int *side_bnd, int *bnd_bcele, int *edge_ptr, int *edge_side,
-                     int *el_vb_ptr, int *el_vb, int *vb_hside, int *diag, int *bnd_rep, double *meas, double *cs,
-                     double *por, double *flux, double *edge_flow, double *bc_conc, double *val, double *bcv,
-                     double *partial, double *mb, unsigned *count) {
-    TmDev t;
-    t.E = E; t.nsub = nsub; t.NBC = NBC;
-    t.ptr = ptr; t.col = col; t.kind = kind; t.s1 = s1; t.s2 = s2;
-    t.side_off = side_off; t.side_edge = side_edge; t.side_bnd = side_bnd; t.bnd_bcele = bnd_bcele;
-    t.edge_ptr = edge_ptr; t.edge_side = edge_side; t.el_vb_ptr = el_vb_ptr; t.el_vb = el_vb;
-    t.vb_hside = vb_hside; t.diag = diag; t.bnd_rep = bnd_rep;
-    t.meas = meas; t.cs = cs; t.por = por; t.flux = flux; t.edge_flow = edge_flow; t.bc_conc = bc_conc;
-    t.val = val; t.bcv = bcv; t.partial = partial; t.mb = mb; t.count = count;
-    return t;
+void TransportGPU::set_sources(const double *density, const double *sigma, const double *conc) {
+    const size_t n = (size_t)E_ * nsub_;
+    h_src_density_.assign(density, density + n);
+    h_src_sigma_.assign(sigma, sigma + n);
+    h_src_conc_.assign(conc, conc + n);
+    std::vector<double> d(n), g(n), c(n);
+    for (int k = 0; k < nsub_; k++)
+        for (int i = 0; i < E_; i++) {
+            d[(size_t)i * nsub_ + k] = density[(size_t)k * E_ + i];
+            g[(size_t)i * nsub_ + k] = sigma[(size_t)k * E_ + i];
+            c[(size_t)i * nsub_ + k] = conc[(size_t)k * E_ + i];
+        }
+    t_.src_density = upv(owned_, d, stream_);
+    t_.src_sigma = upv(owned_, g, stream_);
+    t_.src_conc = upv(owned_, c, stream_);
+    t_.has_src = 1;
+    FL_HIP(hipStreamSynchronize(stream_));
 }
-
-#define TMDEV                                                                                                    \
-    make_tm(this, E_, nsub_, NBC_, d_ptr_, d_col_, d_kind_, d_s1_, d_s2_, d_side_off_, d_side_edge_, d_side_bnd_, \
-            d_bnd_bcele_, d_edge_ptr_, d_edge_side_, d_el_vb_ptr_, d_el_vb_, d_vb_hside_, d_diag_, d_bnd_rep_,   \
-            d_meas_, d_cs_, d_por_, d_flux_, d_edge_flow_, d_bc_conc_, d_val_, d_bcv_, d_partial_, d_mb_, d_count_)
 
 void TransportGPU::set_velocity_host(const double *side_flux) {
     FL_HIP(hipMemcpyAsync(d_flux_, side_flux, m_.n_sides() * sizeof(double), hipMemcpyHostToDevice, stream_));
@@ -423,17 +443,16 @@ double TransportGPU::prepare() {
 }
 
 void TransportGPU::build_matrix() {
-    TmDev t = TMDEV;
     int F = m_.n_edges();
     hipEvent_t a, b;
     FL_HIP(hipEventCreate(&a)); FL_HIP(hipEventCreate(&b));
     FL_HIP(hipEventRecord(a, stream_));
-    hipLaunchKernelGGL(k_edge_flow, dim3(std::min(grid_for(F), 4096)), dim3(kBlock), 0, stream_, F,
-                       (const int *)d_edge_ptr_, (const int *)d_edge_side_, (const double *)d_flux_, d_edge_flow_);
-    hipLaunchKernelGGL(k_tm_build, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t);
+    hipLaunchKernelGGL(k_edge_flow, dim3(std::min(grid_for(F), 4096)), dim3(kBlock), 0, stream_, F, t_.edge_ptr,
+                       t_.edge_side, (const double *)d_flux_, (double *)t_.edge_flow);
+    hipLaunchKernelGGL(k_tm_build, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t_);
     FL_HIP(hipGetLastError());
     double mx = 0.0;
-    FL_HIP(hipMemcpyAsync(&mx, d_mb_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(&mx, t_.mb, sizeof(double), hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipEventRecord(b, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
     float ms = 0;
@@ -446,9 +465,8 @@ void TransportGPU::build_matrix() {
 }
 
 void TransportGPU::set_bc() {
-    TmDev t = TMDEV;
     double est = tg_.estimate_dt();
-    hipLaunchKernelGGL(k_set_bc, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t, est);
+    hipLaunchKernelGGL(k_set_bc, dim3(std::min(grid_for(E_), 4096)), dim3(kBlock), 0, stream_, t_, est);
     FL_HIP(hipGetLastError());
 }
 
@@ -464,26 +482,25 @@ void TransportGPU::mb_initial() {
     mb_init_ = false;
 }
 
-int TransportGPU::step_grid() const { return std::min(grid_for(E_), kMaxStepGrid); }
+int TransportGPU::step_grid() const { return std::max(8, std::min(grid_for(E_) / 8 * 8, kMaxStepGrid)); }
 
 void TransportGPU::flush_flux() {
     if (!pending_) return;
-    TmDev t = TMDEV;
     const int G = step_grid();
-    const double *last = d_partial_ + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 4;
+    const double *last = t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8;
     switch (nsub_) {
-        case 1: hipLaunchKernelGGL(k_flux_flush<1>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
-        case 2: hipLaunchKernelGGL(k_flux_flush<2>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
-        case 3: hipLaunchKernelGGL(k_flux_flush<3>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
-        case 4: hipLaunchKernelGGL(k_flux_flush<4>, dim3(1), dim3(kBlock), 0, stream_, t, last, G, pending_tdiff_); break;
+        case 1: hipLaunchKernelGGL(k_flux_flush<1>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+        case 2: hipLaunchKernelGGL(k_flux_flush<2>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+        case 3: hipLaunchKernelGGL(k_flux_flush<3>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+        case 4: hipLaunchKernelGGL(k_flux_flush<4>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
     }
     FL_HIP(hipGetLastError());
     pending_ = false;
 }
 
+// ConvectionTransport::compute_one_step (transport.cc:593-715)
 void TransportGPU::one_step() {
-    TmDev t = TMDEV;
-    int grid = std::min(grid_for(E_), 4096);
+    const int gridS = std::min(grid_for(E_), 4096);
     if (!matrix_built_) build_matrix();
     if (!setup_done_) {
         setup_done_ = true;
@@ -495,9 +512,9 @@ void TransportGPU::one_step() {
     if (need_rescale_) {
         if (scaled_) {
             double fac = tg_.estimate_dt() / tg_.time_step;
-            hipLaunchKernelGGL(k_tm_scale, dim3(grid), dim3(kBlock), 0, stream_, t, 0, fac);
+            hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 0, fac);
         } else {
-            hipLaunchKernelGGL(k_tm_scale, dim3(grid), dim3(kBlock), 0, stream_, t, 1, tg_.estimate_dt());
+            hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 1, tg_.estimate_dt());
             scaled_ = true;
         }
         need_rescale_ = false;
@@ -508,24 +525,25 @@ void TransportGPU::one_step() {
     const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
     double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
     const int G = step_grid();
-    double *out = d_partial_ + (size_t)(step_parity_ ? 1 : 0) * kMaxStepGrid * 4;
-    const double *fold = pending_ ? d_partial_ + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 4 : nullptr;
-    double ftd = pending_tdiff_;
+    double *out = t_.partial + (size_t)(step_parity_ ? 1 : 0) * kMaxStepGrid * 8;
+    const double *fold = pending_ ? t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8 : nullptr;
+    double ftd = pending_tdiff_, dt = tg_.time_step;
     switch (nsub_) {
-        case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
-        case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
-        case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
-        case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t, cold, cnew, fold, ftd, G, out); break;
+        case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+        case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+        case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+        case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
         default: throw Error(-5, "transport supports 1..4 substances per handle");
     }
+    FL_HIP(hipGetLastError());
     pending_ = true;
     pending_tdiff_ = tdiff;
     step_parity_ = !step_parity_;
-    FL_HIP(hipGetLastError());
     cur_is_c0_ = !cur_is_c0_;
     stats_.steps++;
 }
 
+// TransportOperatorSplitting::update_solution (transport_operator_splitting.cc:244-287)
 int TransportGPU::update_solution(double target) {
     if (!flux_set_) throw Error(-4, "velocity field not set");
     if (mb_init_) mb_initial();
@@ -565,21 +583,24 @@ void TransportGPU::get_conc(double *out) const {
 }
 
 void TransportGPU::get_matrix(int *ptr, int *col, double *val) const {
-    FL_HIP(hipMemcpyAsync(ptr, d_ptr_, (E_ + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(col, d_col_, nnz_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(val, d_val_, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(ptr, t_.ptr, (E_ + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(col, t_.col, nnz_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(val, t_.val, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
 }
 
-// Per-region report of the last step (MassBalance::calculate): boundary
-// fluxes with the previous concentrations, mass with the current ones.
-void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double *mass, double *totals) {
+// Per-region report of the last step (MassBalance::calculate/output at an
+// output time): boundary fluxes with the previous concentrations, mass with
+// the current ones; the cumulative totals come from the per-step reductions
+// on the device.
+void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double *mass, double *src,
+                                double *totals) {
     const HostMesh &m = m_;
     int nbr = m.n_boundary_regions, nblk = m.n_bulk_regions, ns = nsub_;
-    std::vector<double> cur((size_t)E_ * ns), prev((size_t)E_ * ns), mb(1 + ns);
+    std::vector<double> cur((size_t)E_ * ns), prev((size_t)E_ * ns), mb(1 + 2 * ns);
     FL_HIP(hipMemcpyAsync(cur.data(), cur_is_c0_ ? d_c0_ : d_c1_, cur.size() * 8, hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipMemcpyAsync(prev.data(), cur_is_c0_ ? d_c1_ : d_c0_, prev.size() * 8, hipMemcpyDeviceToHost, stream_));
-    FL_HIP(hipMemcpyAsync(mb.data(), d_mb_, mb.size() * 8, hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipMemcpyAsync(mb.data(), t_.mb, mb.size() * 8, hipMemcpyDeviceToHost, stream_));
     std::vector<double> flux(m.n_sides());
     FL_HIP(hipMemcpyAsync(flux.data(), d_flux_, flux.size() * 8, hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
@@ -588,6 +609,7 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
     std::fill(plus, plus + ns * nbr, 0.0);
     std::fill(minus, minus + ns * nbr, 0.0);
     std::fill(mass, mass + ns * nblk, 0.0);
+    std::fill(src, src + ns * nblk, 0.0);
     for (int b = 0; b < m.n_boundaries(); b++) {
         int sd = m.edge_side[m.edge_ptr[m.bnd_edge[b]]];
         int i = m.side_elem[sd];
@@ -602,14 +624,22 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
         }
     }
     for (int k = 0; k < ns; k++)
-        for (int i = 0; i < E_; i++)
-            mass[k * nblk + h_el_region_sub_[i]] += h_por_[i] * h_cs_[i] * cur[(size_t)i * ns + k] * h_meas_[i];
+        for (int i = 0; i < E_; i++) {
+            int r = h_el_region_sub_[i];
+            mass[k * nblk + r] += h_por_[i] * h_cs_[i] * cur[(size_t)i * ns + k] * h_meas_[i];
+            if (t_.has_src && stats_.steps > 0) {
+                size_t j = (size_t)k * E_ + i;
+                double diff = h_src_conc_[j] - prev[(size_t)i * ns + k];
+                double snd = diff > 0.0 ? (h_src_density_[j] + diff * h_src_sigma_[j]) : h_src_density_[j];
+                src[k * nblk + r] += snd * h_meas_[i];
+            }
+        }
     for (int k = 0; k < ns; k++) {
         double mt = 0.0;
         for (int r = 0; r < nblk; r++) mt += mass[k * nblk + r];
         totals[4 * k + 0] = k < (int)initial_mass_.size() ? initial_mass_[k] : 0.0;
         totals[4 * k + 1] = mb[1 + k];
-        totals[4 * k + 2] = 0.0;
+        totals[4 * k + 2] = mb[1 + ns + k];
         totals[4 * k + 3] = mt;
     }
 }

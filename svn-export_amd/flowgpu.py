@@ -71,7 +71,8 @@ SIGNATURES = {
     "fl_transport_get_conc": (C.c_int, [C.c_void_p, _f64p]),
     "fl_transport_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
-    "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 5),
+    "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 6),
+    "fl_transport_set_sources": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
     "fl_transport_nnz": (C.c_int, [C.c_void_p]),
     "fl_transport_matrix": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p]),
     "fl_transport_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
@@ -285,11 +286,17 @@ class ConvectionTransport:
     def mass_balance(self):
         ns, nbr, nblk = self.nsub, self.mesh.n_boundary_regions, self.mesh.n_bulk_regions
         bal, plus, minus = np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1)), np.zeros(ns * max(nbr, 1))
-        mass, tot = np.zeros(ns * max(nblk, 1)), np.zeros(4 * ns)
-        _check(lib().fl_transport_mass_balance(self.h, bal, plus, minus, mass, tot))
+        mass, src, tot = np.zeros(ns * max(nblk, 1)), np.zeros(ns * max(nblk, 1)), np.zeros(4 * ns)
+        _check(lib().fl_transport_mass_balance(self.h, bal, plus, minus, mass, src, tot))
         return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
                     minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
-                    totals=tot.reshape(ns, 4))
+                    src=src[:ns * nblk].reshape(ns, nblk), totals=tot.reshape(ns, 4))
+
+    def set_sources(self, density, sigma, conc):
+        """sources_density, sources_sigma, sources_conc, each [n_subst][E] (transport.cc:501-544)."""
+        n = self.nsub * self.mesh.n_elements
+        self._src = [_f64(density, n), _f64(sigma, n), _f64(conc, n)]
+        _check(lib().fl_transport_set_sources(self.h, *self._src))
 
     def stats(self):
         ms, st = C.c_double(), C.c_int()

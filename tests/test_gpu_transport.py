@@ -108,3 +108,36 @@ def test_transport_generated_multisubstance(tmp_path):
     for t in (0.25, 0.5, 0.75, 1.0):
         assert T0.update(t) == T1.update_solution(t)
         assert np.array_equal(T0.conc(), T1.conc())
+
+
+def test_transport_sources_bitexact(tmp_path):
+    """Concentration sources (transport.cc:501-544): GPU equals the oracle bitwise;
+    with por = cs = 1 the cumulative balance initial + sources - flux = mass closes."""
+    path = str(tmp_path / "sq12.msh")
+    cases.generated_mesh("square", 12, path)
+    om, gm = O.Mesh(path), G.Mesh(path)
+    d = cases.darcy_case("square", om, seed=8)
+    D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    D0.solve(0, rtol=1e-14)
+    flux = D0.solution[:D0.S]
+    E, NB = om.n_elements, om.n_bc_elements
+    rng = np.random.default_rng(1)
+    ns = 3
+    dens, sig, sc = rng.uniform(0, 0.2, (ns, E)), rng.uniform(0, 2.0, (ns, E)), rng.uniform(0, 1.5, (ns, E))
+    bcc = np.zeros((ns, NB))
+    ic = rng.uniform(0, 1, (ns, E))
+    one = np.ones(E)
+    T0 = O.Transport(om, flux, one, one, bcc, ic, 0.6, 0.2)
+    T0.set_sources(dens, sig, sc)
+    T1 = G.ConvectionTransport(gm, one, one, bcc, ic, 0.6, 0.2)
+    T1.set_velocity_field(side_flux=flux)
+    T1.set_sources(dens, sig, sc)
+    for t in (0.2, 0.4, 0.6):
+        assert T0.update(t) == T1.update_solution(t)
+        assert np.array_equal(T0.conc(), T1.conc())
+    mb0, mb1 = T0.mass_balance(), T1.mass_balance()
+    for k in range(ns):
+        a, flx, src, cur = mb1["totals"][k]
+        assert abs(a + src - flx - cur) < 1e-12 * max(abs(cur), 1.0)
+        assert np.allclose(mb1["totals"][k], mb0["totals"][k], rtol=1e-12, atol=1e-14)
+    assert np.allclose(mb1["src"], mb0["src"], rtol=1e-12, atol=1e-14)
