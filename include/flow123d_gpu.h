@@ -85,11 +85,14 @@ int fl_darcy_set_data(fl_darcy *d, const double *K, const double *cs, const doub
 /* Replace the conductivity only (new Monte Carlo realization). */
 int fl_darcy_set_conductivity(fl_darcy *d, const double *K);
 
-/* Solver options (Solver record, src/la/solve.cc:69-84): r_tol, a_tol, max_it. */
+/* Solver options (Solver record, src/la/solve.cc:69-84): r_tol, a_tol, max_it;
+ * precision: 0 = FP64 CG (the drop-in path), 1 = FP32 CG with FP64 reliable
+ * updates (the stop test is always on the true FP64 residual). */
 typedef struct {
     double r_tol;
     double a_tol;
     int max_it;
+    int precision;
 } fl_solver_opts;
 
 /* Result of a solve: CG iterations, final residual 2-norm, rhs 2-norm,
@@ -108,6 +111,7 @@ typedef struct {
     double ms_spmv_avg;
     double ms_update_avg;
     int kernel_samples;
+    int reliable_updates;
 } fl_solve_info;
 
 /* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433). */

@@ -164,7 +164,9 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
     return guard([&] {
         double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
         int max_it = o ? o->max_it : 10000;  // solve.cc:69-84 defaults
-        fl::SolveStats st = d->g->solve(rtol, atol, max_it);
+        int prec = o ? o->precision : 0;
+        if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
+        fl::SolveStats st = d->g->solve(rtol, atol, max_it, prec);
         if (info) {
             info->iterations = st.iterations;
             info->converged = st.converged;
@@ -177,6 +179,7 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
             info->ms_spmv_avg = st.ms_spmv_avg;
             info->ms_update_avg = st.ms_update_avg;
             info->kernel_samples = st.kernel_samples;
+            info->reliable_updates = st.reliable_updates;
         }
         return 0;
     });

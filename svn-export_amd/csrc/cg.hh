@@ -200,4 +200,58 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_update(CGDev<V> a, int k, int G
     block_partials<2>(v, dst);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 CG with FP64 reliable updates.  The FP32 iteration
+// runs the same two kernels with V = float (half the bytes); every few
+// iterations, and before accepting convergence, the correction accumulated in
+// x32 is added to the FP64 iterate, the true residual b - A x is recomputed in
+// FP64 from the FP64 matrix and replaces the recursive FP32 residual (the
+// search direction is kept, so the Krylov progress is not lost).  The stop test
+// of the step after an update therefore uses the true FP64 residual.
+// ---------------------------------------------------------------------------
+static __global__ void k_to_float(int n, const double *__restrict__ src, float *__restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = (float)src[i];
+}
+
+static __global__ void k_ru_accum(int n, double *__restrict__ x64, float *__restrict__ x32) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x64[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// r64 = b - A64 x64 ; r32 = r64 ; z32 = dinv32 r32 ; partials r.z, r.r (FP64)
+static __global__ void __launch_bounds__(kCgBlock) k_ru_resid(CGDev<float> a, const double *__restrict__ b64,
+                                                     const double *__restrict__ val64,
+                                                     const double *__restrict__ x64, double *part_rz,
+                                                     double *part_rr) {
+    double v[2] = {0.0, 0.0};
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+        const int row = sl * 64 + lane;
+        const int e0 = a.slice_ptr[sl] + lane;
+        const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
+        double acc = 0.0;
+        for (int k = 0; k < w; k++) {
+            const int i0 = e0 + k * 64;
+            acc += val64[i0] * x64[a.col[i0]];
+        }
+        if (row < a.n) {
+            double r = b64[row] - acc;
+            float r32 = (float)r;
+            float z32 = a.dinv[row] * r32;
+            a.r[row] = r32;
+            a.z[row] = z32;
+            v[0] += (double)r32 * (double)z32;
+            v[1] += r * r;
+        }
+    }
+    double *const dst[2] = {part_rz, part_rr};
+    block_partials<2>(v, dst);
+}
+
+static __global__ void k_cg_clear_done(CGState *st) { st->done = 0; }
+
 }  // namespace fl

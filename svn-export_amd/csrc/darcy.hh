@@ -38,6 +38,7 @@ struct SolveStats {
     // live per-kernel durations sampled with HIP events on the solver stream
     double ms_spmv_avg = 0, ms_update_avg = 0;
     int kernel_samples = 0;
+    int reliable_updates = 0;
 };
 
 // Device view of the mesh + pattern + fields, passed to kernels by value.
@@ -67,7 +68,8 @@ class DarcyGPU {
     void set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
                   const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma);
     void set_conductivity(const double *K);  // host array [E]
-    SolveStats solve(double rtol, double atol, int max_it);
+    // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates
+    SolveStats solve(double rtol, double atol, int max_it, int precision = 0);
     void get_solution(double *out) const;                   // [S+E+F]
     void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
     void element_balance(double *out) const;                // [E]
@@ -84,6 +86,7 @@ class DarcyGPU {
     void build_pattern();
     void assemble();
     int cg(double rtol, double atol, int max_it, SolveStats &st);
+    int cg_mixed(double rtol, double atol, int max_it, int ru_every, SolveStats &st);
     void resolve();
 
     const HostMesh &m_;
@@ -100,6 +103,9 @@ class DarcyGPU {
     double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
     int cg_grid_ = 1024;
+    int ru_every_ = 100;
+    float *d_val32_ = nullptr, *d_dinv32_ = nullptr, *d_x32_ = nullptr, *d_r32_ = nullptr, *d_z32_ = nullptr;
+    float *d_q32_ = nullptr, *d_p32a_ = nullptr, *d_p32b_ = nullptr;
     // SELL-64 edge system
     int nslices_ = 0;
     int *d_slice_ptr_ = nullptr, *d_sell_col_ = nullptr;

@@ -582,6 +582,118 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
     return h.it;
 }
 
+int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, SolveStats &st) {
+    const int F = dv_.F, G = cg_grid_;
+    if (!d_val32_) {
+        auto fa = [&](size_t n) { float *p = dev_alloc<float>(n); owned_.push_back(p); return p; };
+        d_val32_ = fa(dv_.nslots);
+        d_dinv32_ = fa(F);
+        d_x32_ = fa(F); d_r32_ = fa(F); d_z32_ = fa(F); d_q32_ = fa(F); d_p32a_ = fa(F); d_p32b_ = fa(F);
+    }
+    hipLaunchKernelGGL(k_to_float, dim3(std::min(grid_for(dv_.nslots), 8192)), dim3(kBlock), 0, stream_,
+                       dv_.nslots, (const double *)dv_.val, d_val32_);
+    hipLaunchKernelGGL(k_to_float, dim3(std::min(grid_for(F), 8192)), dim3(kBlock), 0, stream_, F,
+                       (const double *)dv_.dinv, d_dinv32_);
+    CGDev<float> a;
+    a.n = F;
+    a.nslices = nslices_;
+    a.slice_ptr = d_slice_ptr_;
+    a.col = d_sell_col_;
+    a.val = d_val32_;
+    a.dinv = d_dinv32_;
+    a.b = nullptr;
+    a.x = d_x32_; a.r = d_r32_; a.z = d_z32_; a.q = d_q32_;
+    a.part_pq = d_part_; a.part_rz0 = d_part_ + G; a.part_rz1 = d_part_ + 2 * G;
+    a.part_rr0 = d_part_ + 3 * G; a.part_rr1 = d_part_ + 4 * G;
+    a.st = d_state_;
+    const int gV = std::min(grid_for(F), 8192);
+    FL_HIP(hipMemsetAsync(d_x_, 0, F * sizeof(double), stream_));
+    FL_HIP(hipMemsetAsync(d_x32_, 0, F * sizeof(float), stream_));
+    FL_HIP(hipMemsetAsync(d_p32a_, 0, F * sizeof(float), stream_));
+    auto reliable_update = [&](int k) {
+        hipLaunchKernelGGL(k_ru_accum, dim3(gV), dim3(kBlock), 0, stream_, F, d_x_, d_x32_);
+        hipLaunchKernelGGL(k_ru_resid, dim3(G), dim3(kCgBlock), 0, stream_, a, (const double *)dv_.rhs,
+                           (const double *)dv_.val, (const double *)d_x_, (k & 1) ? a.part_rz1 : a.part_rz0,
+                           (k & 1) ? a.part_rr1 : a.part_rr0);
+    };
+    reliable_update(0);  // r = b, with the true FP64 norms
+    hipLaunchKernelGGL(k_cg_setup<float>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
+                       max_it);
+    CGState h{};
+    int k = 0, last_ru = 0, chunk = 32, n_ru = 1;
+    const int kSampleEvery = 8;
+    struct Sample { int ev, it; };
+    std::vector<Sample> samples;
+    int ev_used = 0;
+    auto next_ev = [&]() {
+        if (ev_used == (int)ev_pool_.size()) {
+            hipEvent_t e;
+            FL_HIP(hipEventCreate(&e));
+            ev_pool_.push_back(e);
+        }
+        return ev_used++;
+    };
+    while (true) {
+        for (int j = 0; j < chunk; j++, k++) {
+            if (k > 0 && k % ru_every == 0 && k != last_ru) {
+                reliable_update(k);
+                last_ru = k;
+                n_ru++;
+            }
+            float *pin = (k & 1) ? d_p32b_ : d_p32a_, *pout = (k & 1) ? d_p32a_ : d_p32b_;
+            bool samp = (k % kSampleEvery) == 1;
+            int e0 = 0;
+            if (samp) {
+                e0 = next_ev(); next_ev(); next_ev();
+                FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
+            }
+            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pin,
+                               pout);
+            if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
+            hipLaunchKernelGGL(k_cg_update<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pout);
+            if (samp) {
+                FL_HIP(hipEventRecord(ev_pool_[e0 + 2], stream_));
+                samples.push_back(Sample{e0, k});
+            }
+        }
+        FL_HIP(hipGetLastError());
+        FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+        if (h.done) {
+            if (h.it == last_ru || h.it >= max_it) break;  // stop test saw the true FP64 residual
+            // recursive FP32 residual converged: confirm with the true residual at that step
+            k = h.it;
+            reliable_update(k);
+            last_ru = k;
+            n_ru++;
+            hipLaunchKernelGGL(k_cg_clear_done, dim3(1), dim3(1), 0, stream_, d_state_);
+            chunk = 32;
+            continue;
+        }
+        if (k > max_it + chunk) break;
+        if (chunk < 256) chunk *= 2;
+    }
+    hipLaunchKernelGGL(k_ru_accum, dim3(gV), dim3(kBlock), 0, stream_, F, d_x_, d_x32_);
+    double s1 = 0.0, s2 = 0.0;
+    int ns = 0;
+    for (const Sample &sm : samples) {
+        if (sm.it >= h.it) continue;
+        float t1 = 0, t2 = 0;
+        FL_HIP(hipEventElapsedTime(&t1, ev_pool_[sm.ev], ev_pool_[sm.ev + 1]));
+        FL_HIP(hipEventElapsedTime(&t2, ev_pool_[sm.ev + 1], ev_pool_[sm.ev + 2]));
+        s1 += t1; s2 += t2; ns++;
+    }
+    st.kernel_samples = ns;
+    st.ms_spmv_avg = ns ? s1 / ns : 0.0;
+    st.ms_update_avg = ns ? s2 / ns : 0.0;
+    st.iterations = h.it;
+    st.residual = std::sqrt(h.rr);
+    st.bnorm = std::sqrt(h.bnorm2);
+    st.converged = (h.rr <= h.tol2) ? 1 : 0;
+    st.reliable_updates = n_ru;
+    return h.it;
+}
+
 void DarcyGPU::resolve() {
     const DarcyDev &d = dv_;
     hipLaunchKernelGGL(k_scatter_lambda, dim3(std::min(grid_for(d.F), 8192)), dim3(kBlock), 0, stream_, d,
@@ -598,7 +710,7 @@ void DarcyGPU::resolve() {
     FL_HIP(hipGetLastError());
 }
 
-SolveStats DarcyGPU::solve(double rtol, double atol, int max_it) {
+SolveStats DarcyGPU::solve(double rtol, double atol, int max_it, int precision) {
     if (!have_data_) throw Error(-4, "Darcy data not set");
     SolveStats st;
     hipEvent_t e0, e1, e2, e3;
@@ -606,7 +718,8 @@ SolveStats DarcyGPU::solve(double rtol, double atol, int max_it) {
     FL_HIP(hipEventRecord(e0, stream_));
     assemble();
     FL_HIP(hipEventRecord(e1, stream_));
-    cg(rtol, atol, max_it, st);
+    if (precision == 1) cg_mixed(rtol, atol, max_it, ru_every_, st);
+    else cg(rtol, atol, max_it, st);
     FL_HIP(hipEventRecord(e2, stream_));
     resolve();
     FL_HIP(hipEventRecord(e3, stream_));

@@ -61,7 +61,7 @@ struct DarcyData {
 
 class DarcyFlowMH_Steady {
    public:
-    DarcyFlowMH_Steady(GpuMesh &mesh, const DarcyData &data, fl_solver_opts solver = {1e-7, 1e-9, 10000})
+    DarcyFlowMH_Steady(GpuMesh &mesh, const DarcyData &data, fl_solver_opts solver = {1e-7, 1e-9, 10000, 0})
         : mesh_(mesh), solver_(solver) {
         check(fl_darcy_create(mesh.handle(), &d_), "darcy create");
         check(fl_darcy_set_data(d_, data.conductivity.data(), data.cross_section.data(),

@@ -25,14 +25,14 @@ class FlowError(RuntimeError):
 
 
 class SolverOpts(C.Structure):
-    _fields_ = [("r_tol", C.c_double), ("a_tol", C.c_double), ("max_it", C.c_int)]
+    _fields_ = [("r_tol", C.c_double), ("a_tol", C.c_double), ("max_it", C.c_int), ("precision", C.c_int)]
 
 
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("rhs_norm", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
                 ("ms_resolve", C.c_double), ("ms_total", C.c_double), ("ms_spmv_avg", C.c_double),
-                ("ms_update_avg", C.c_double), ("kernel_samples", C.c_int)]
+                ("ms_update_avg", C.c_double), ("kernel_samples", C.c_int), ("reliable_updates", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -196,8 +196,9 @@ class DarcyFlowMH_Steady:
     def set_conductivity(self, K):
         _check(lib().fl_darcy_set_conductivity(self.h, _f64(K, self.E)))
 
-    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000):
-        o = SolverOpts(r_tol, a_tol, max_it)
+    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
+        """precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates."""
+        o = SolverOpts(r_tol, a_tol, max_it, precision)
         info = SolveInfo()
         _check(lib().fl_darcy_update_solution(self.h, C.byref(o), C.byref(info)))
         self.info = info.as_dict()

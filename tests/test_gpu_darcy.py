@@ -168,3 +168,40 @@ def test_cpp_adapter_example():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "darcy:" in out.stdout and "transport:" in out.stdout
+
+
+@pytest.mark.parametrize("case,mesh", REF_CASES)
+def test_darcy_mixed_precision_reference_meshes(case, mesh):
+    """FP32 CG with FP64 reliable updates: the stop test is on the true FP64
+    residual, so the result matches the FP64 oracle like the FP64 path does."""
+    path = cases.ref_path(mesh)
+    om, gm = O.Mesh(path), G.Mesh(path)
+    d = cases.darcy_case(case, om)
+    D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"],
+                 aniso=d["aniso"])
+    D0.solve(0, rtol=1e-15)
+    g = G.DarcyFlowMH_Steady(gm)
+    g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"],
+               aniso=d["aniso"])
+    info = g.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=1)
+    print(case, "mixed iters", info["iterations"], "reliable updates", info["reliable_updates"])
+    assert info["converged"] == 1 and info["residual"] <= 1e-13 * info["rhs_norm"] * (1 + 1e-12)
+    check_solution(D0, g, tol=1e-9)
+
+
+@pytest.mark.parametrize("kind,n,kw", [("square", 100, {}),
+                                       ("cube", 8, dict(n_fractures=3, n_channels=10, channel_len=6, seed=4))])
+def test_darcy_mixed_precision_generated(tmp_path, kind, n, kw):
+    path = str(tmp_path / "m.msh")
+    cases.generated_mesh(kind, n, path, **kw)
+    om, gm = O.Mesh(path), G.Mesh(path)
+    d = cases.darcy_case(kind, om, seed=5)
+    D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    D0.solve(0, rtol=1e-15, max_it=200000)
+    g = G.DarcyFlowMH_Steady(gm)
+    g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    i64 = g.update_solution(r_tol=1e-12, a_tol=0.0, max_it=200000, precision=0)
+    i32 = g.update_solution(r_tol=1e-12, a_tol=0.0, max_it=200000, precision=1)
+    print(kind, "iterations fp64", i64["iterations"], "mixed", i32["iterations"], "ru", i32["reliable_updates"])
+    assert i32["converged"] == 1
+    check_solution(D0, g, tol=1e-8)
