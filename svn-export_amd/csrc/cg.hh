@@ -24,8 +24,9 @@ namespace fl {
 
 constexpr int kCgBlock = 256;
 
+// done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
 struct CGState {
-    double tol2, rr, bnorm2, pad;
+    double tol2, rr, bnorm2, rr_ru;
     int done, it, max_it, pad2;
 };
 
@@ -115,12 +116,16 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double
         a.st->max_it = max_it;
         a.st->it = 0;
         a.st->done = 0;
+        a.st->rr_ru = t[0];
     }
 }
 
+// is_ru: the residual of step k is a freshly recomputed true residual;
+// delta2 > 0: pause (done = 2) once the recursive residual has dropped by
+// sqrt(delta2) since the last true residual (reliable-update criterion).
 template <class V>
 __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
-                                                    V *__restrict__ p_out) {
+                                                    V *__restrict__ p_out, int is_ru, double delta2) {
     CGState *st = a.st;
     if (st->done) return;
     const double *cur_rz = (k & 1) ? a.part_rz1 : a.part_rz0;
@@ -133,6 +138,16 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
     if (rr <= st->tol2 || k >= st->max_it || !(rr == rr)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->done = 1;
+            st->it = k;
+            st->rr = rr;
+        }
+        return;
+    }
+    if (is_ru) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_ru = rr;
+    } else if (delta2 > 0.0 && rr < delta2 * st->rr_ru) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = 2;
             st->it = k;
             st->rr = rr;
         }

@@ -546,7 +546,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
             hipLaunchKernelGGL(k_cg_spmv<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
-                               (const double *)pin, pout);
+                               (const double *)pin, pout, itx == 0 ? 1 : 0, 0.0);
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
                                (const double *)pout);
@@ -621,6 +621,8 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
                        max_it);
     CGState h{};
     int k = 0, last_ru = 0, chunk = 32, n_ru = 1;
+    const double delta2 = 0.01;  // true residual recomputed after every 10x drop
+    const int kMaxUpdates = 400;
     const int kSampleEvery = 8;
     struct Sample { int ev, it; };
     std::vector<Sample> samples;
@@ -635,11 +637,6 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
     };
     while (true) {
         for (int j = 0; j < chunk; j++, k++) {
-            if (k > 0 && k % ru_every == 0 && k != last_ru) {
-                reliable_update(k);
-                last_ru = k;
-                n_ru++;
-            }
             float *pin = (k & 1) ? d_p32b_ : d_p32a_, *pout = (k & 1) ? d_p32a_ : d_p32b_;
             bool samp = (k % kSampleEvery) == 1;
             int e0 = 0;
@@ -648,7 +645,7 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
             hipLaunchKernelGGL(k_cg_spmv<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pin,
-                               pout);
+                               pout, k == last_ru ? 1 : 0, delta2);
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pout);
             if (samp) {
@@ -660,8 +657,10 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
         FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
         if (h.done) {
-            if (h.it == last_ru || h.it >= max_it) break;  // stop test saw the true FP64 residual
-            // recursive FP32 residual converged: confirm with the true residual at that step
+            // converged on a true FP64 residual, or out of iterations
+            if ((h.done == 1 && h.it == last_ru) || h.it >= max_it || n_ru >= kMaxUpdates) break;
+            // paused (residual dropped 10x) or converged on the recursive residual:
+            // fold x32 into x64 and restart the step from the true residual
             k = h.it;
             reliable_update(k);
             last_ru = k;
