@@ -104,15 +104,18 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_init(CGDev<V> a, V *p0) {
 }
 
 // G: number of blocks of the main kernels (length of the partial arrays)
+// bnorm2 > 0: norm^2 of b given (warm start, the initial residual is not b)
 template <class V>
-__global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double rtol2, double atol2, int max_it) {
+__global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double rtol2, double atol2, int max_it,
+                                                     double bnorm2) {
     const double *const parts[1] = {a.part_rr0};
     double t[1];
     fold_partials<1>(parts, G, t);
     if (threadIdx.x == 0) {
-        a.st->bnorm2 = t[0];
+        double bb = bnorm2 > 0.0 ? bnorm2 : t[0];
+        a.st->bnorm2 = bb;
         a.st->rr = t[0];
-        a.st->tol2 = fmax(rtol2 * t[0], atol2);
+        a.st->tol2 = fmax(rtol2 * bb, atol2);
         a.st->max_it = max_it;
         a.st->it = 0;
         a.st->done = 0;
@@ -268,5 +271,35 @@ static __global__ void __launch_bounds__(kCgBlock) k_ru_resid(CGDev<float> a, co
 }
 
 static __global__ void k_cg_clear_done(CGState *st) { st->done = 0; }
+
+// FP64 warm start: r = b - A x, z = D^-1 r, p0 = 0, partials r.z, r.r
+static __global__ void __launch_bounds__(kCgBlock) k_cg_init_warm(CGDev<double> a, double *p0) {
+    double v[2] = {0.0, 0.0};
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+        const int row = sl * 64 + lane;
+        const int e0 = a.slice_ptr[sl] + lane;
+        const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
+        double acc = 0.0;
+        for (int k = 0; k < w; k++) {
+            const int i0 = e0 + k * 64;
+            acc += a.val[i0] * a.x[a.col[i0]];
+        }
+        if (row < a.n) {
+            double r = a.b[row] - acc;
+            double z = a.dinv[row] * r;
+            a.r[row] = r;
+            a.z[row] = z;
+            a.q[row] = 0.0;
+            p0[row] = 0.0;
+            v[0] += r * z;
+            v[1] += r * r;
+        }
+    }
+    double *const dst[2] = {a.part_rz0, a.part_rr0};
+    block_partials<2>(v, dst);
+}
 
 }  // namespace fl

@@ -39,6 +39,7 @@ struct SolveStats {
     double ms_spmv_avg = 0, ms_update_avg = 0;
     int kernel_samples = 0;
     int reliable_updates = 0;
+    int fp64_iterations = 0;  // FP64 iterations after an FP32 stagnation
 };
 
 // Device view of the mesh + pattern + fields, passed to kernels by value.
@@ -85,7 +86,7 @@ class DarcyGPU {
    private:
     void build_pattern();
     void assemble();
-    int cg(double rtol, double atol, int max_it, SolveStats &st);
+    int cg(double rtol, double atol, int max_it, SolveStats &st, bool warm = false, double bnorm2 = 0.0);
     int cg_mixed(double rtol, double atol, int max_it, int ru_every, SolveStats &st);
     void resolve();
 

@@ -502,7 +502,7 @@ void DarcyGPU::assemble() {
     FL_HIP(hipGetLastError());
 }
 
-int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
+int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm, double bnorm2) {
     CGDev<double> a;
     a.n = dv_.F;
     a.nslices = nslices_;
@@ -516,9 +516,10 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st) {
     a.part_pq = d_part_; a.part_rz0 = d_part_ + G; a.part_rz1 = d_part_ + 2 * G;
     a.part_rr0 = d_part_ + 3 * G; a.part_rr1 = d_part_ + 4 * G;
     a.st = d_state_;
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
+    if (warm) hipLaunchKernelGGL(k_cg_init_warm, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
+    else hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
     hipLaunchKernelGGL(k_cg_setup<double>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
-                       max_it);
+                       max_it, warm ? bnorm2 : 0.0);
     double *pin = d_p0_, *pout = d_p1_;
     CGState h{};
     int launched = 0;
@@ -618,11 +619,13 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
     };
     reliable_update(0);  // r = b, with the true FP64 norms
     hipLaunchKernelGGL(k_cg_setup<float>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
-                       max_it);
+                       max_it, 0.0);
     CGState h{};
     int k = 0, last_ru = 0, chunk = 32, n_ru = 1;
     const double delta2 = 0.01;  // true residual recomputed after every 10x drop
     const int kMaxUpdates = 400;
+    std::vector<double> ru_hist;  // true residual norm^2 at each reliable update
+    bool stagnated = false;
     const int kSampleEvery = 8;
     struct Sample { int ev, it; };
     std::vector<Sample> samples;
@@ -660,7 +663,17 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
             // converged on a true FP64 residual, or out of iterations
             if ((h.done == 1 && h.it == last_ru) || h.it >= max_it || n_ru >= kMaxUpdates) break;
             // paused (residual dropped 10x) or converged on the recursive residual:
-            // fold x32 into x64 and restart the step from the true residual
+            // fold x32 into x64 and restart the step from the true residual.
+            // Stagnation of the true residual (FP32 attainable accuracy reached):
+            // finish in FP64 from the current iterate.
+            if (h.done == 1 || h.done == 2) {
+                ru_hist.push_back(h.rr_ru);
+                size_t nh = ru_hist.size();
+                if (nh >= 4 && ru_hist[nh - 1] > 0.01 * ru_hist[nh - 4]) {
+                    stagnated = true;
+                    break;
+                }
+            }
             k = h.it;
             reliable_update(k);
             last_ru = k;
@@ -690,7 +703,16 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
     st.bnorm = std::sqrt(h.bnorm2);
     st.converged = (h.rr <= h.tol2) ? 1 : 0;
     st.reliable_updates = n_ru;
-    return h.it;
+    if ((stagnated || !st.converged) && h.it < max_it) {
+        // FP64 CG warm-started from the FP64 iterate; the stop test keeps ||b||
+        SolveStats s64;
+        cg(rtol, atol, max_it - h.it, s64, true, h.bnorm2);
+        st.iterations = h.it + s64.iterations;
+        st.residual = s64.residual;
+        st.converged = s64.converged;
+        st.fp64_iterations = s64.iterations;
+    }
+    return st.iterations;
 }
 
 void DarcyGPU::resolve() {
