@@ -8,88 +8,10 @@
 
 #include "common.hh"
 #include "darcy.hh"
+#include "condense.hh"
 #include "rt0_local.hh"
 
 namespace fl {
-
-// ---------------------------------------------------------------------------
-// Element condensation (assembly_steady_mh_matrix darcy_flow_mh.cc:525-694 +
-// make_schur1/2 :1009-1131 in closed form)
-// ---------------------------------------------------------------------------
-template <int D>
-struct Local {
-    double inv[(D + 1) * (D + 1)];
-    double a[D + 1], Ag[D + 1], c[D + 1], dlt[D + 1], rf[D + 1];
-    double s, h, meas;
-};
-
-template <int D>
-__device__ __forceinline__ void element_local(const DarcyDev &dv, int e, Local<D> &L) {
-    Elem<D> el;
-    for (int i = 0; i <= D; i++) {
-        int n = dv.el_nodes[4 * e + i];
-        el.P[i] = V3{dv.xyz[3 * n], dv.xyz[3 * n + 1], dv.xyz[3 * n + 2]};
-    }
-    double K = dv.K[e], cs = dv.cs[e];
-    double scale = 1 / K / cs;  // mh_fe_values.cc:38
-    double R[9];
-    const double *Rp = nullptr;
-    if (dv.aniso) {
-        inv3x3(dv.aniso + 9 * (size_t)e, R);
-        Rp = R;
-    }
-    double loc[(D + 1) * (D + 1)];
-    local_matrix<D>(el, scale, Rp, loc);
-    small_inverse<D + 1>(loc, L.inv);
-    V3 ec = elem_centre<D>(el);
-    int so = dv.side_off[e];
-    double g[D + 1];
-    for (int i = 0; i <= D; i++) {
-        V3 sc = side_centre<D>(el, i);
-        g[i] = ec.z - sc.z;  // gravity term, darcy_flow_mh.cc:565
-        L.c[i] = 1.0;
-        L.dlt[i] = 0.0;
-        L.rf[i] = 0.0;
-        int b = dv.side_bnd[so + i];
-        if (b >= 0) {
-            int bce = dv.bnd_bcele[b];
-            int type = dv.bc_type[bce];
-            double bm = dv.bc_measure[bce];
-            if (type == 1) {  // dirichlet :575-581
-                L.c[i] = 0.0;
-                g[i] -= dv.bc_p[bce];
-                L.dlt[i] = 1.0;
-                L.rf[i] = dv.bc_p[bce];
-            } else if (type == 2) {  // neumann :582-584
-                L.rf[i] = -(dv.bc_flux[bce] * bm * cs);
-            } else if (type == 3) {  // robin :586-591
-                L.rf[i] = -(-bm * dv.bc_sigma[bce] * dv.bc_p[bce] * cs);
-                L.dlt[i] = -(-bm * dv.bc_sigma[bce] * cs);
-            }
-        }
-    }
-    double s = 0.0;
-    for (int i = 0; i <= D; i++) {
-        double ai = 0.0, agi = 0.0;
-        for (int j = 0; j <= D; j++) {
-            ai += L.inv[i * (D + 1) + j];
-            agi += L.inv[i * (D + 1) + j] * g[j];
-        }
-        L.a[i] = ai;
-        L.Ag[i] = agi;
-        s += ai;
-    }
-    L.s = s;
-    L.meas = elem_measure<D>(el);
-    double sumAg = 0.0;
-    for (int i = 0; i <= D; i++) sumAg += L.Ag[i];
-    L.h = L.meas * cs * dv.f[e] - sumAg;
-}
-
-__device__ __forceinline__ double vb_coef(const DarcyDev &dv, int e, int n) {
-    // darcy_flow_mh.cc:630-632: sigma(lower) * |higher side| * cs(higher)
-    return dv.sigma[e] * dv.vb_hmeas[n] * dv.cs[dv.vb_hel[n]];
-}
 
 template <int D>
 __global__ void __launch_bounds__(256) k_assemble(const DarcyDev dv) {
@@ -97,7 +19,7 @@ __global__ void __launch_bounds__(256) k_assemble(const DarcyDev dv) {
     if (idx >= dv.dim_start[D]) return;
     int e = dv.perm[idx];
     Local<D> L;
-    element_local<D>(dv, e, L);
+    element_local<D>(dv, e, dv.K[e], L);
     const int n0 = D + 1;
     int vb0 = dv.el_vb_ptr[e], nvb = dv.el_vb_ptr[e + 1] - vb0;
     double V = 0.0;
@@ -164,7 +86,7 @@ __global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double
     if (idx >= dv.dim_start[D]) return;
     int e = dv.perm[idx];
     Local<D> L;
-    element_local<D>(dv, e, L);
+    element_local<D>(dv, e, dv.K[e], L);
     const int n0 = D + 1;
     int so = dv.side_off[e];
     int vb0 = dv.el_vb_ptr[e], nvb = dv.el_vb_ptr[e + 1] - vb0;
