@@ -142,6 +142,49 @@ int fl_darcy_water_balance(fl_darcy *d, double *balance, double *outflow, double
 int fl_darcy_edge_nnz(fl_darcy *d);
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs);
 
+/* ---- batched Monte-Carlo solves ------------------------------------------- */
+
+/* n_real conductivity realizations of one Darcy problem solved together (no
+ * counterpart in the reference: a caller would otherwise loop
+ * DarcyFlowMH_Steady::update_solution, darcy_flow_mh.cc:375-433, over K
+ * fields).  The batch shares the mesh, cross-sections, sources and boundary
+ * conditions of `base` (fl_darcy_set_data must have been called); only K
+ * differs.  The base handle must outlive the batch. */
+typedef struct fl_darcy_batch fl_darcy_batch;
+int fl_darcy_batch_create(fl_darcy *base, int n_real, fl_darcy_batch **out);
+void fl_darcy_batch_free(fl_darcy_batch *b);
+/* K[n_real][E]; device_ptr != 0: K is a device pointer. */
+int fl_darcy_batch_set_conductivity(fl_darcy_batch *b, const double *K, int device_ptr);
+
+typedef struct {
+    int n_real, chunks, reliable_updates, fp64_finish, max_iterations, n_converged;
+    long long total_iterations; /* summed over realizations */
+    double ms_setup, ms_solve, ms_resolve, ms_total;
+    double ms_op_avg, ms_update_avg; /* live event samples of the operator / update kernels */
+    int grid, xcd_map, chunk;        /* launch shape used */
+} fl_batch_info;
+
+/* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as
+ * for fl_darcy_update_solution; precision 1 = FP32 CG with FP64 reliable
+ * updates).  Per-realization iterations / converged flags / final FP64
+ * relative residuals go to the optional arrays [n_real]. */
+int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *opts, fl_batch_info *info,
+                                   int *iterations, int *converged, double *residual);
+/* Per-realization outcome of the last batch solve: status (1 converged,
+ * 4 iteration limit, 5 breakdown) and the number of FP64 reliable updates
+ * (precision 1); arrays [n_real], either may be NULL. */
+int fl_darcy_batch_diagnostics(fl_darcy_batch *b, int *status, int *reliable_updates);
+/* [q | p | lambda] of realization r, as fl_darcy_get_solution. */
+int fl_darcy_batch_get_solution(fl_darcy_batch *b, int r, double *out);
+/* Device solutions, realization-interleaved: value i of realization r at
+ * [i * n_padded + r], n_padded = n_real rounded up to 64. */
+const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
+/* Launch shape (grid blocks per 64 realizations, multiple of 8 up to 2048;
+ * XCD-aware row mapping 0/1; iterations per captured graph chunk) and the
+ * autotuner that picks grid and mapping by timing the operator kernel. */
+int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
+int fl_darcy_batch_autotune(fl_darcy_batch *b);
+
 /* ---- explicit upwind transport -------------------------------------------- */
 
 /* ConvectionTransport::ConvectionTransport (src/transport/transport.cc:105-188)

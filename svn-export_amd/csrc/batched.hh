@@ -1,0 +1,122 @@
+// Batched Monte-Carlo Darcy solver: R conductivity realizations on one mesh
+// (same cross-sections, sources and boundary conditions), solved together.
+// There is no counterpart in the reference; each realization is the solve of
+// DarcyFlowMH_Steady::update_solution (src/flow/darcy_flow_mh.cc:375-433) with
+// its own conductivity field.
+//
+// Matrix-free condensed operator.  With A_e(K) = A_e(1)/K, every quantity of
+// the per-element condensation is a K-independent geometric factor times a
+// power of K:
+//    A^-1 = K inv0, a = K a0, s = K s0, A^-1 g = K Ag0,
+//    S_e(K) x = K [C inv0 C] x + dlt.x - u (u^T x) / D,   u = [K c.a0 ; v],
+//    D = K s0 + V (vb coefficients v do not depend on K).
+// The K-independent factors of all elements form one table, shared by the
+// realizations; per realization only K (one value per element) is read by the
+// operator, instead of the ~14 matrix values per edge row an assembled batch
+// of CSR matrices would stream.
+//
+// Layout: every per-edge vector is realization-interleaved, v[row * Rp + r]
+// with Rp = R rounded up to 64.  One wavefront handles one edge row for 64
+// realizations: the element table and the index lists are wave-uniform
+// (scalar loads) and every vector or K access is one contiguous segment.
+//
+// CG: per-realization alpha, beta, stop test and iteration count.  Four
+// kernels per iteration (operator + partial p.q, fold, update + partial r.z,
+// r.r, fold); converged realizations drop out of every memory access.  A chunk
+// of iterations (and, in FP32 mode, the FP64 reliable update that follows) is
+// captured once into a hipGraph and replayed until every realization stops.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "darcy.hh"
+
+namespace fl {
+
+struct BatchStats {
+    int n_real = 0, chunks = 0, reliable_updates = 0, fp64_finish = 0, max_iterations = 0;
+    long total_iterations = 0;  // summed over realizations
+    double ms_setup = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
+    double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels
+    int op_samples = 0;
+    int grid = 0, xcd_map = 0, chunk = 0;     // launch shape used (autotuned)
+    std::vector<int> iterations, converged;  // per realization
+    std::vector<double> residual;             // ||b - S x|| / ||b|| (FP64, true residual at the end)
+};
+
+// per-realization scalars of the batched CG
+struct BatchScal {
+    double *alpha, *beta, *rz, *rz_prev, *rr, *tol2, *bnorm2, *rr_hist, *rr_ru;
+    int *status, *it, *n_ru, *par, *it_ru, *active_count;  // par: buffer holding the current search direction
+};
+
+// Batched CG status per realization
+enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5 };
+
+class DarcyBatchGPU {
+   public:
+    // base: a configured single solver (set_data called); its mesh, fields and
+    // boundary data are shared.  R realizations.
+    DarcyBatchGPU(DarcyGPU &base, int R);
+    ~DarcyBatchGPU();
+    // K [R][E] (host or device pointer)
+    void set_conductivity(const double *K, bool device_ptr);
+    // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates (FP64 finish
+    // for realizations whose FP32 residual stagnates)
+    BatchStats solve(double rtol, double atol, int max_it, int precision);
+    void get_solution(int r, double *out) const;  // [S+E+F] of realization r
+    void diagnostics(int *status, int *n_ru) const;  // [R] each, either may be null
+    const double *device_solution() const { return d_sol_; }  // [(S+E+F)][Rp] interleaved
+    int n_real() const { return R_; }
+    int padded() const { return Rp_; }
+    int n_rows() const { return F_; }
+    long table_bytes() const { return tab_bytes_; }
+    long contributions() const { return ncontrib_; }
+    hipStream_t stream() const { return stream_; }
+    // launch shape: grid blocks per 64-realization chunk, XCD-aware row mapping,
+    // iterations per captured chunk.  autotune() times the operator kernel
+    // over candidate shapes and keeps the fastest.
+    void set_launch(int grid, int xcd_map, int chunk);
+    void autotune();
+
+   private:
+    void build();
+    void alloc32();
+    void tune();
+    void setup_system();  // table, rhs, Jacobi diagonal for the current K
+    void free_graphs();
+
+    DarcyGPU &base_;
+    const DarcyDev &dv_;
+    hipStream_t stream_ = nullptr;
+    int R_, Rp_, E_, S_, F_, nchunk_;
+    long tab_bytes_ = 0, ncontrib_ = 0;
+    std::vector<void *> owned_;
+    // shared structure
+    int *d_meta_ = nullptr, *d_op_off_ = nullptr, *d_rs_off_ = nullptr, *d_conn_off_ = nullptr, *d_conn_ = nullptr;
+    int *d_row_ptr_ = nullptr, *d_row_c_ = nullptr, *d_perm_ = nullptr, *d_eperm_ = nullptr;
+    double *d_tab_ = nullptr, *d_rsT_ = nullptr;  // operator table (FP64), resolve table
+    float *d_tab32_ = nullptr;
+    long op_total_ = 0, rs_total_ = 0;
+    int dim_start_[4] = {0, 0, 0, 0};
+    // per realization
+    double *d_K_ = nullptr, *d_b_ = nullptr, *d_dinv_ = nullptr;
+    float *d_K32_ = nullptr, *d_dinv32_ = nullptr;
+    double *d_x_ = nullptr, *d_r_ = nullptr, *d_z_ = nullptr, *d_p0_ = nullptr, *d_p1_ = nullptr, *d_q_ = nullptr;
+    float *d_x32_ = nullptr, *d_r32_ = nullptr, *d_z32_ = nullptr, *d_p32a_ = nullptr, *d_p32b_ = nullptr,
+          *d_q32_ = nullptr;
+    double *d_sol_ = nullptr, *d_ext_ = nullptr;
+    double *d_part_ = nullptr;  // 3 partial arrays of 2048 * Rp
+    BatchScal sc_{};
+    int *h_count_ = nullptr;  // pinned
+    // launch shape
+    int grid_ = 512, xcd_map_ = 0, chunk_ = 16;
+    bool tuned_ = false;
+    hipGraphExec_t g64_ = nullptr, g32_ = nullptr;
+    int g64_key_ = -1, g32_key_ = -1;
+    std::vector<hipEvent_t> ev_;
+    bool have_K_ = false;
+};
+
+}  // namespace fl

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/flow123d_gpu.h"
+#include "batched.hh"
 #include "common.hh"
 #include "darcy.hh"
 #include "mesh.hh"
@@ -15,6 +16,9 @@ struct fl_mesh {
 struct fl_darcy {
     std::unique_ptr<fl::DarcyGPU> g;
     fl_mesh *mesh;
+};
+struct fl_darcy_batch {
+    std::unique_ptr<fl::DarcyBatchGPU> g;
 };
 struct fl_transport {
     std::unique_ptr<fl::TransportGPU> g;
@@ -222,6 +226,102 @@ int fl_darcy_edge_nnz(fl_darcy *d) { return d->g->nnz(); }
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs) {
     return guard([&] {
         d->g->get_edge_system(ptr, col, val, rhs);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_create(fl_darcy *base, int n_real, fl_darcy_batch **out) {
+    return guard([&] {
+        check_device();
+        if (!base || !out) throw fl::Error(-1, "null handle");
+        auto *b = new fl_darcy_batch();
+        try {
+            b->g.reset(new fl::DarcyBatchGPU(*base->g, n_real));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+        return 0;
+    });
+}
+
+void fl_darcy_batch_free(fl_darcy_batch *b) { delete b; }
+
+int fl_darcy_batch_set_conductivity(fl_darcy_batch *b, const double *K, int device_ptr) {
+    return guard([&] {
+        b->g->set_conductivity(K, device_ptr != 0);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, fl_batch_info *info,
+                                   int *iterations, int *converged, double *residual) {
+    return guard([&] {
+        double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
+        int max_it = o ? o->max_it : 10000;
+        int prec = o ? o->precision : 0;
+        if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
+        fl::BatchStats st = b->g->solve(rtol, atol, max_it, prec);
+        const int R = b->g->n_real();
+        int nconv = 0;
+        for (int r = 0; r < R; r++) {
+            nconv += st.converged[r];
+            if (iterations) iterations[r] = st.iterations[r];
+            if (converged) converged[r] = st.converged[r];
+            if (residual) residual[r] = st.residual[r];
+        }
+        if (info) {
+            info->n_real = R;
+            info->chunks = st.chunks;
+            info->reliable_updates = st.reliable_updates;
+            info->fp64_finish = st.fp64_finish;
+            info->max_iterations = st.max_iterations;
+            info->n_converged = nconv;
+            info->total_iterations = st.total_iterations;
+            info->ms_setup = st.ms_setup;
+            info->ms_solve = st.ms_cg;
+            info->ms_resolve = st.ms_resolve;
+            info->ms_total = st.ms_total;
+            info->ms_op_avg = st.ms_op_avg;
+            info->ms_update_avg = st.ms_update_avg;
+            info->grid = st.grid;
+            info->xcd_map = st.xcd_map;
+            info->chunk = st.chunk;
+        }
+        return 0;
+    });
+}
+
+int fl_darcy_batch_diagnostics(fl_darcy_batch *b, int *status, int *reliable_updates) {
+    return guard([&] {
+        b->g->diagnostics(status, reliable_updates);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_get_solution(fl_darcy_batch *b, int r, double *out) {
+    return guard([&] {
+        b->g->get_solution(r, out);
+        return 0;
+    });
+}
+
+const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded) {
+    if (n_padded) *n_padded = b->g->padded();
+    return b->g->device_solution();
+}
+
+int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk) {
+    return guard([&] {
+        b->g->set_launch(grid, xcd_map, chunk);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_autotune(fl_darcy_batch *b) {
+    return guard([&] {
+        b->g->autotune();
         return 0;
     });
 }

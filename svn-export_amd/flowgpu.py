@@ -38,6 +38,17 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("n_real", C.c_int), ("chunks", C.c_int), ("reliable_updates", C.c_int), ("fp64_finish", C.c_int),
+                ("max_iterations", C.c_int), ("n_converged", C.c_int), ("total_iterations", C.c_longlong),
+                ("ms_setup", C.c_double), ("ms_solve", C.c_double), ("ms_resolve", C.c_double),
+                ("ms_total", C.c_double), ("ms_op_avg", C.c_double), ("ms_update_avg", C.c_double),
+                ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # symbol -> (restype, argtypes)
 SIGNATURES = {
     "fl_last_error": (C.c_char_p, []),
@@ -61,6 +72,16 @@ SIGNATURES = {
     "fl_darcy_water_balance": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
     "fl_darcy_edge_nnz": (C.c_int, [C.c_void_p]),
     "fl_darcy_edge_system": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p, _f64p]),
+    "fl_darcy_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fl_darcy_batch_free": (None, [C.c_void_p]),
+    "fl_darcy_batch_set_conductivity": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "fl_darcy_batch_update_solution": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(BatchInfo), _i32p,
+                                                 _i32p, _f64p]),
+    "fl_darcy_batch_get_solution": (C.c_int, [C.c_void_p, C.c_int, _f64p]),
+    "fl_darcy_batch_diagnostics": (C.c_int, [C.c_void_p, _i32p, _i32p]),
+    "fl_darcy_batch_device_solution": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int)]),
+    "fl_darcy_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fl_darcy_batch_autotune": (C.c_int, [C.c_void_p]),
     "fl_transport_create": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                       C.POINTER(C.c_void_p)]),
     "fl_transport_free": (None, [C.c_void_p]),
@@ -234,6 +255,65 @@ class DarcyFlowMH_Steady:
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.fl_darcy_free(self.h)
+            self.h = None
+
+
+class DarcyBatch:
+    """n_real conductivity realizations of one DarcyFlowMH_Steady problem solved together (Monte Carlo).
+
+    Shares mesh, cross-sections, sources and boundary data with `darcy` (set_data must have been called)."""
+
+    def __init__(self, darcy, n_real):
+        self.darcy = darcy
+        self.n_real = int(n_real)
+        h = C.c_void_p()
+        _check(lib().fl_darcy_batch_create(darcy.h, self.n_real, C.byref(h)))
+        self.h = h
+        self.S, self.E, self.F = darcy.S, darcy.E, darcy.F
+
+    def set_conductivity(self, K):
+        """K: [n_real, E] host array, or a device pointer (int) to n_real*E doubles."""
+        if isinstance(K, int):
+            _check(lib().fl_darcy_batch_set_conductivity(self.h, C.c_void_p(K), 1))
+        else:
+            Kh = _f64(K, self.n_real * self.E)
+            _check(lib().fl_darcy_batch_set_conductivity(self.h, Kh.ctypes.data, 0))
+
+    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
+        o = SolverOpts(r_tol, a_tol, max_it, precision)
+        info = BatchInfo()
+        it = np.zeros(self.n_real, np.int32)
+        conv = np.zeros(self.n_real, np.int32)
+        res = np.zeros(self.n_real)
+        _check(lib().fl_darcy_batch_update_solution(self.h, C.byref(o), C.byref(info), it, conv, res))
+        self.info = info.as_dict()
+        self.info.update(iterations=it, converged=conv, residual=res)
+        return self.info
+
+    def get_solution_vector(self, r):
+        out = np.zeros(self.S + self.E + self.F)
+        _check(lib().fl_darcy_batch_get_solution(self.h, int(r), out))
+        return out
+
+    def diagnostics(self):
+        st, ru = np.zeros(self.n_real, np.int32), np.zeros(self.n_real, np.int32)
+        _check(lib().fl_darcy_batch_diagnostics(self.h, st, ru))
+        return st, ru
+
+    def device_solution(self):
+        n = C.c_int()
+        p = lib().fl_darcy_batch_device_solution(self.h, C.byref(n))
+        return p, n.value
+
+    def set_launch(self, grid, xcd_map, chunk):
+        _check(lib().fl_darcy_batch_set_launch(self.h, grid, xcd_map, chunk))
+
+    def autotune(self):
+        _check(lib().fl_darcy_batch_autotune(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_darcy_batch_free(self.h)
             self.h = None
 
 

@@ -126,3 +126,21 @@ def generated_mesh(kind, n, path, **kw):
         m = meshgen.cube_mesh(n, **kw)
     meshgen.write_gmsh(m, path)
     return m
+
+
+def mc_conductivity(om, n_real, seed=100, sigma=(0.5, 3.0), corr=(0.05, 0.3), base=None):
+    """Monte-Carlo conductivity realizations [n_real, E]: lognormal fields with
+    sigma and correlation length drawn per realization, per-realization
+    seeds; fracture / channel elements scaled by 1e3 as in darcy_case."""
+    rng = np.random.default_rng(seed)
+    cen_e = om.centre.reshape(-1, 3)
+    lab = np.array([_region_label(om, r) for r in om.elem_region])
+    scale = np.where((lab == "fracture") | (lab == "channel"), 1e3, 1.0)
+    K = np.empty((n_real, om.n_elements))
+    for r in range(n_real):
+        s = rng.uniform(*sigma)
+        ell = rng.uniform(*corr)
+        K[r] = meshgen.lognormal_field(cen_e, sigma=s, corr_len=ell, seed=seed * 1000 + r, n_features=256) * scale
+        if base is not None:
+            K[r] *= base
+    return K

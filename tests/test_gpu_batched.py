@@ -1,0 +1,161 @@
+"""Batched Monte-Carlo Darcy solves (many conductivity realizations, one
+launch sequence) against the CPU oracle and the single-solve GPU path.  Every
+call goes through the C ABI."""
+import numpy as np
+import pytest
+
+import cases
+import flowgpu as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def parts(x, S, E):
+    return x[:S], x[S:S + E], x[S + E:]
+
+
+def setup(path, case, seed=0):
+    om, gm = O.Mesh(path), G.Mesh(path)
+    d = cases.darcy_case(case, om, seed=seed)
+    g = G.DarcyFlowMH_Steady(gm)
+    g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"],
+               aniso=d["aniso"])
+    return om, gm, d, g
+
+
+def oracle_solve(om, d, K, rtol=1e-15):
+    D0 = O.Darcy(om, K, d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"],
+                 aniso=d["aniso"])
+    D0.solve(0, rtol=rtol, max_it=200000)
+    return D0.solution
+
+
+@pytest.mark.parametrize("case,mesh", [("test01", "test01_mesh.msh"), ("test06_32", "test6_32d.msh"),
+                                       ("test06_21", "test6_21d.msh"), ("test13", "test13.msh")])
+def test_batch_reference_meshes(case, mesh):
+    """Dirichlet / Neumann / anisotropy / vb coupling of the reference's own
+    inputs, three K realizations (the case's K times a random factor field)."""
+    om, gm, d, g = setup(cases.ref_path(mesh), case)
+    R = 3
+    rng = np.random.default_rng(7)
+    K = d["K"][None, :] * np.exp(0.7 * rng.standard_normal((R, om.n_elements)))
+    b = G.DarcyBatch(g, R)
+    b.set_conductivity(K)
+    info = b.update_solution(r_tol=1e-15, a_tol=0.0, max_it=100000, precision=0)
+    print(case, "iterations", info["iterations"], "residual", info["residual"], flush=True)
+    S, E = om.n_sides, om.n_elements
+    for r in range(R):
+        x0 = oracle_solve(om, d, K[r])
+        x1 = b.get_solution_vector(r)
+        eq, ep, el = [rel(a, c) for a, c in zip(parts(x1, S, E), parts(x0, S, E))]
+        print("  r=%d rel err flux %.2e pressure %.2e trace %.2e" % (r, eq, ep, el), flush=True)
+        assert eq <= 1e-10 and ep <= 1e-10 and el <= 1e-10
+
+
+@pytest.fixture(scope="module")
+def cube6(tmp_path_factory):
+    path = str(tmp_path_factory.mktemp("mc") / "cube6.msh")
+    cases.generated_mesh("cube", 6, path, n_fractures=3, n_channels=8, channel_len=5, seed=3)
+    om, gm, d, g = setup(path, "cube", seed=1)
+    return om, gm, d, g
+
+
+def test_batch_matches_single_solves(cube6):
+    """70 realizations (padded to 128 lanes): FP64 batch vs the single FP64 GPU
+    solve of each sampled realization; uneven iteration counts."""
+    om, gm, d, g = cube6
+    R = 70
+    K = cases.mc_conductivity(om, R, seed=3)
+    b = G.DarcyBatch(g, R)
+    b.set_conductivity(K)
+    info = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=0)
+    it = info["iterations"]
+    print("iterations min %d max %d; grid %d xcd %d chunk %d" % (it.min(), it.max(), info["grid"],
+                                                                  info["xcd_map"], info["chunk"]), flush=True)
+    assert info["converged"].all() and info["n_converged"] == R
+    assert it.max() > it.min()
+    assert (info["residual"] <= 2e-13).all()
+    S, E = om.n_sides, om.n_elements
+    for r in (0, 31, 63, 64, 69):
+        g.set_conductivity(K[r])
+        g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=100000)
+        x0 = g.get_solution_vector()
+        x1 = b.get_solution_vector(r)
+        errs = [rel(a, c) for a, c in zip(parts(x1, S, E), parts(x0, S, E))]
+        print("  r=%d rel err flux %.2e pressure %.2e trace %.2e" % (r, *errs), flush=True)
+        assert max(errs) <= 1e-9
+    g.set_conductivity(d["K"])
+
+
+def test_batch_mixed_precision(cube6):
+    """FP32 CG with FP64 reliable updates: flux within 1e-8 of the FP64 oracle.
+    With sigma up to 3 the flux error is up to ~2e4 times the relative
+    residual, so the stop test is 1e-13."""
+    om, gm, d, g = cube6
+    R = 64
+    K = cases.mc_conductivity(om, R, seed=4)
+    b = G.DarcyBatch(g, R)
+    b.set_conductivity(K)
+    i64 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=0)
+    i32 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=1)
+    st, ru = b.diagnostics()
+    print("fp64 iterations total %d, mixed %d, reliable updates %d, fp64 finish %d" % (
+        i64["total_iterations"], i32["total_iterations"], ru.sum(), i32["fp64_finish"]), flush=True)
+    assert i32["converged"].all() and (st == 1).all()
+    assert (i32["residual"] <= 1e-13 * 1.01).all()
+    assert i32["total_iterations"] < 2 * i64["total_iterations"]
+    S = om.n_sides
+    for r in (0, 17, 63):
+        x0 = oracle_solve(om, d, K[r])
+        x1 = b.get_solution_vector(r)
+        e = rel(x1[:S], x0[:S])
+        print("  r=%d flux rel err vs oracle %.2e" % (r, e), flush=True)
+        assert e <= 1e-8
+
+
+def test_batch_launch_shapes_and_repeat(cube6):
+    """Repeated solves are bitwise identical; another launch shape (grid,
+    XCD mapping, chunk) changes only the reduction order; K from a device
+    pointer gives the same result as from the host."""
+    import torch
+    om, gm, d, g = cube6
+    R = 64
+    K = cases.mc_conductivity(om, R, seed=5)
+    b = G.DarcyBatch(g, R)
+    b.set_conductivity(K)
+    b.set_launch(512, 0, 16)
+    b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    a = np.stack([b.get_solution_vector(r) for r in (0, 40)])
+    b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    a2 = np.stack([b.get_solution_vector(r) for r in (0, 40)])
+    assert np.array_equal(a, a2)
+    b.set_launch(256, 1, 7)
+    info = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    assert info["grid"] == 256 and info["xcd_map"] == 1 and info["chunk"] == 7
+    c = np.stack([b.get_solution_vector(r) for r in (0, 40)])
+    assert rel(c, a) < 1e-11
+    Kd = torch.tensor(K, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    b.set_conductivity(Kd.data_ptr())
+    b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    e = np.stack([b.get_solution_vector(r) for r in (0, 40)])
+    assert np.array_equal(e, c)
+    with pytest.raises(G.FlowError):
+        b.get_solution_vector(R)
+
+
+def test_batch_autotune(cube6):
+    om, gm, d, g = cube6
+    R = 128
+    K = cases.mc_conductivity(om, R, seed=6)
+    b = G.DarcyBatch(g, R)
+    b.set_conductivity(K)
+    b.autotune()
+    info = b.update_solution(r_tol=1e-12, a_tol=0.0, max_it=100000, precision=1)
+    print("autotuned grid %d xcd %d" % (info["grid"], info["xcd_map"]), flush=True)
+    assert info["grid"] in (256, 512, 1024, 2048) and info["converged"].all()
