@@ -152,6 +152,10 @@ int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *r
  * differs.  The base handle must outlive the batch. */
 typedef struct fl_darcy_batch fl_darcy_batch;
 int fl_darcy_batch_create(fl_darcy *base, int n_real, fl_darcy_batch **out);
+/* The same with n_slots concurrent realization lanes (rounded up to 64;
+ * 0 = min(n_real, 256)): a slot whose realization has converged is refilled
+ * with the next one, hardest realizations (largest ln K variance) first. */
+int fl_darcy_batch_create_slots(fl_darcy *base, int n_real, int n_slots, fl_darcy_batch **out);
 void fl_darcy_batch_free(fl_darcy_batch *b);
 /* K[n_real][E]; device_ptr != 0: K is a device pointer. */
 int fl_darcy_batch_set_conductivity(fl_darcy_batch *b, const double *K, int device_ptr);
@@ -162,6 +166,9 @@ typedef struct {
     double ms_setup, ms_solve, ms_resolve, ms_total;
     double ms_op_avg, ms_update_avg; /* live event samples of the operator / update kernels */
     int grid, xcd_map, chunk;        /* launch shape used */
+    int compactions;                 /* times live realizations were packed into fewer 64-wide chunks */
+    int handovers;                   /* realizations finished by the single-solve FP64 path */
+    long long handover_iterations;   /* CG iterations they took there */
 } fl_batch_info;
 
 /* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as
@@ -180,10 +187,15 @@ int fl_darcy_batch_get_solution(fl_darcy_batch *b, int r, double *out);
  * [i * n_padded + r], n_padded = n_real rounded up to 64. */
 const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
 /* Launch shape (grid blocks per 64 realizations, multiple of 8 up to 2048;
- * XCD-aware row mapping 0/1; iterations per captured graph chunk) and the
+ * row mapping 0 strided, 1 XCD-contiguous, 2 XCD- and block-contiguous;
+ * iterations per captured graph chunk) and the
  * autotuner that picks grid and mapping by timing the operator kernel. */
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
+/* Once the queue is empty and at most n realizations are still iterating,
+ * finish them one by one with the single-solve FP64 CG (warm start from the
+ * batch iterate); 0 disables.  Default 16. */
+int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n);
 
 /* ---- explicit upwind transport -------------------------------------------- */
 

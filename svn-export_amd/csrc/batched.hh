@@ -35,7 +35,9 @@
 namespace fl {
 
 struct BatchStats {
-    int n_real = 0, chunks = 0, reliable_updates = 0, fp64_finish = 0, max_iterations = 0;
+    int n_real = 0, chunks = 0, reliable_updates = 0, fp64_finish = 0, max_iterations = 0, compactions = 0;
+    int handovers = 0;              // realizations finished by the single-solve path
+    long handover_iterations = 0;   // their iterations there
     long total_iterations = 0;  // summed over realizations
     double ms_setup = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
     double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels
@@ -52,13 +54,15 @@ struct BatchScal {
 };
 
 // Batched CG status per realization
-enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5 };
+enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6 };
 
 class DarcyBatchGPU {
    public:
     // base: a configured single solver (set_data called); its mesh, fields and
-    // boundary data are shared.  R realizations.
-    DarcyBatchGPU(DarcyGPU &base, int R);
+    // boundary data are shared.  R realizations solved in `slots` concurrent
+    // lanes (rounded up to 64; 0: min(R, 256)); a slot whose realization has
+    // converged is refilled with the next one.
+    DarcyBatchGPU(DarcyGPU &base, int R, int slots = 0);
     ~DarcyBatchGPU();
     // K [R][E] (host or device pointer)
     void set_conductivity(const double *K, bool device_ptr);
@@ -67,9 +71,10 @@ class DarcyBatchGPU {
     BatchStats solve(double rtol, double atol, int max_it, int precision);
     void get_solution(int r, double *out) const;  // [S+E+F] of realization r
     void diagnostics(int *status, int *n_ru) const;  // [R] each, either may be null
-    const double *device_solution() const { return d_sol_; }  // [(S+E+F)][Rp] interleaved
+    const double *device_solution() const { return d_sol_; }  // [(S+E+F)][Rout] interleaved
     int n_real() const { return R_; }
-    int padded() const { return Rp_; }
+    int padded() const { return Rout_; }
+    int slots() const { return Rp_; }
     int n_rows() const { return F_; }
     long table_bytes() const { return tab_bytes_; }
     long contributions() const { return ncontrib_; }
@@ -79,6 +84,8 @@ class DarcyBatchGPU {
     // over candidate shapes and keeps the fastest.
     void set_launch(int grid, int xcd_map, int chunk);
     void autotune();
+    // live-realization count at which the rest go to the single-solve path (0: never)
+    void set_handover(int n) { handover_ = n < 0 ? 0 : n; }
 
    private:
     void build();
@@ -86,16 +93,21 @@ class DarcyBatchGPU {
     void tune();
     void setup_system();  // table, rhs, Jacobi diagonal for the current K
     void free_graphs();
+    void alloc_scal(BatchScal &sc);
+    // move slot src[s] to slot s for every live array of the phase (0 FP64, 1 FP32)
+    void permute_slots(const std::vector<int> &src, int phase);
+    // (slot, realization) pairs to the device list and the slot flags
+    void set_pairs(const std::vector<int> &pairs);
 
     DarcyGPU &base_;
     const DarcyDev &dv_;
     hipStream_t stream_ = nullptr;
-    int R_, Rp_, E_, S_, F_, nchunk_;
+    int R_, Rp_, Rout_, E_, S_, F_, nchunk_;  // Rp_: slots (multiple of 64); Rout_: R rounded up to 64
     long tab_bytes_ = 0, ncontrib_ = 0;
     std::vector<void *> owned_;
     // shared structure
     int *d_meta_ = nullptr, *d_op_off_ = nullptr, *d_rs_off_ = nullptr, *d_conn_off_ = nullptr, *d_conn_ = nullptr;
-    int *d_row_ptr_ = nullptr, *d_row_c_ = nullptr, *d_perm_ = nullptr, *d_eperm_ = nullptr;
+    int *d_row_ptr_ = nullptr, *d_row_c_ = nullptr, *d_perm_ = nullptr, *d_eperm_ = nullptr, *d_rec_ = nullptr;
     double *d_tab_ = nullptr, *d_rsT_ = nullptr;  // operator table (FP64), resolve table
     float *d_tab32_ = nullptr;
     long op_total_ = 0, rs_total_ = 0;
@@ -108,13 +120,27 @@ class DarcyBatchGPU {
           *d_q32_ = nullptr;
     double *d_sol_ = nullptr, *d_ext_ = nullptr;
     double *d_part_ = nullptr;  // 3 partial arrays of 2048 * Rp
-    BatchScal sc_{};
+    BatchScal sc_{}, sc2_{};
     int *h_count_ = nullptr;  // pinned
+    // slot -> realization (-1: empty), live chunk count, spare buffers
+    std::vector<int> rid_;
+    double *d_Kall_ = nullptr, *d_var_ = nullptr;  // K [R][E] as given; difficulty proxy [R]
+    double *d_Ksave_ = nullptr;                    // the base solver's K while it is borrowed
+    int handover_ = 16;  // live realizations at or below which the single-solve path finishes them
+    int *d_pairs_ = nullptr, *d_lsel_ = nullptr;   // pair list [2 * slots], slot flags [slots]
+    int npairs_ = 0;
+    // per-realization results of the last solve
+    std::vector<int> h_it_, h_status_, h_ru_;
+    std::vector<double> h_rr_, h_bb_, h_tol_;
+    int nch_ = 0;
+    int *d_src_ = nullptr;
+    double *d_sp64_ = nullptr, *d_spK_ = nullptr;
+    float *d_sp32_ = nullptr, *d_spK32_ = nullptr;
     // launch shape
     int grid_ = 512, xcd_map_ = 0, chunk_ = 16;
     bool tuned_ = false;
     hipGraphExec_t g64_ = nullptr, g32_ = nullptr;
-    int g64_key_ = -1, g32_key_ = -1;
+    long long g64_key_ = -1, g32_key_ = -1;  // (iteration limit, live chunks) of the captured graphs
     std::vector<hipEvent_t> ev_;
     bool have_K_ = false;
 };

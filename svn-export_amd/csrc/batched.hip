@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <numeric>
 
@@ -43,6 +44,7 @@ template <class V>
 struct BArgs {
     int F, Rp, G, xcd;
     const int *row_ptr, *row_c, *meta, *op_off, *conn_off, *conn;
+    const int *rec;  // packed row records (16 ints per row)
     const V *opT, *K, *dinv;
     const double *opT64, *K64;  // FP64 table and K (elements with vb neighbours)
     V *x, *r, *z, *p0, *p1, *q;
@@ -50,13 +52,27 @@ struct BArgs {
     BatchScal sc;
 };
 
-// rows of this wave: start, stride, end
+// Rows of this wave: start, stride, end.
+//  xcd 0: rows strided over the whole grid
+//  xcd 1: block b runs on XCD b % 8 (round-robin dispatch); each XCD owns one
+//         contiguous eighth of the rows, strided over its blocks
+//  xcd 2: as 1, and each block owns one contiguous run of rows inside its
+//         XCD's eighth, its 4 waves stepping through it together, so the
+//         neighbour gathers of consecutive rows are reused from L2
 __device__ __forceinline__ void row_range(int F, int G, int xcd, int &start, int &step, int &end) {
-    const int wid = threadIdx.x >> 6;
-    if (!xcd) {
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: rows index scalar loads
+    if (xcd == 0) {
         start = blockIdx.x * kWaves + wid;
         step = G * kWaves;
         end = F;
+    } else if (xcd == 2) {
+        const int x = blockIdx.x & 7, lb = blockIdx.x >> 3, nb = G >> 3;
+        const int per = (F + 7) >> 3;
+        const int rb = (per + nb - 1) / nb;
+        const int lo = x * per + lb * rb;
+        start = lo + wid;
+        step = kWaves;
+        end = min(min(lo + rb, (x + 1) * per), F);
     } else {
         // block b runs on XCD b % 8 (round-robin dispatch); give each XCD one
         // contiguous eighth of the rows so its gathers stay in its own L2
@@ -142,13 +158,54 @@ __device__ __forceinline__ double elem_row_vb(const double *__restrict__ T, int 
     return vk * (xi - g);
 }
 
+// Packed row records: 16 ints per row, two element contributions of 8 ints
+// {e, i, n0, op_off, conn[4]}; e = -1 marks an absent second contribution,
+// e = -2 in the first marks a row that needs the general path (a contribution
+// from an element with vb neighbours, or more than two contributions).  The
+// record is read with wave-uniform scalar loads, so one row costs one scalar
+// load followed by independent gathers.
+constexpr int kRecInts = 16;
+
+template <class V, int N0, bool FLY>
+__device__ __forceinline__ V rec_contrib(const int *__restrict__ R, const V *__restrict__ opT,
+                                         const V *__restrict__ Kv, const V *__restrict__ za,
+                                         const V *__restrict__ pb, V beta, size_t Rp, int r) {
+    const int e = R[0], i = R[1];
+    const V *T = opT + R[3];
+    const V K = Kv[(size_t)e * Rp + r];
+    V t = V(0), xi = V(0);
+#pragma unroll
+    for (int j = 0; j < N0; j++) {
+        const V xj = gather(za, pb, beta, (size_t)R[4 + j] * Rp + r, FLY);
+        t += T[i * N0 + j] * xj;
+        if (j == i) xi = xj;
+    }
+    return K * t + T[N0 * N0 + i] * xi;
+}
+
 template <class V, bool FLY>
-__device__ __forceinline__ V op_row(const int *__restrict__ row_ptr, const int *__restrict__ row_c,
-                                    const int *__restrict__ meta, const int *__restrict__ op_off,
-                                    const int *__restrict__ conn_off, const int *__restrict__ conn,
-                                    const V *__restrict__ opT, const V *__restrict__ Kv,
-                                    const double *__restrict__ opT64, const double *__restrict__ K64, int row,
-                                    const V *__restrict__ za, const V *__restrict__ pb, V beta, size_t Rp, int r) {
+__device__ __forceinline__ V rec_any(const int *__restrict__ R, const V *__restrict__ opT, const V *__restrict__ Kv,
+                                     const V *__restrict__ za, const V *__restrict__ pb, V beta, size_t Rp, int r) {
+    const int n0 = R[2];
+    if (n0 == 4) return rec_contrib<V, 4, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
+    if (n0 == 3) return rec_contrib<V, 3, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
+    return rec_contrib<V, 2, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
+}
+
+template <class V, bool FLY>
+__device__ __forceinline__ V op_row(const int *__restrict__ rec, const int *__restrict__ row_ptr,
+                                    const int *__restrict__ row_c, const int *__restrict__ meta,
+                                    const int *__restrict__ op_off, const int *__restrict__ conn_off,
+                                    const int *__restrict__ conn, const V *__restrict__ opT,
+                                    const V *__restrict__ Kv, const double *__restrict__ opT64,
+                                    const double *__restrict__ K64, int row, const V *__restrict__ za,
+                                    const V *__restrict__ pb, V beta, size_t Rp, int r) {
+    const int *R = rec + (size_t)row * kRecInts;
+    if (R[0] != -2) {
+        V y = rec_any<V, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
+        if (R[8] >= 0) y += rec_any<V, FLY>(R + 8, opT, Kv, za, pb, beta, Rp, r);
+        return y;
+    }
     V y = V(0);
     for (int c = row_ptr[row]; c < row_ptr[row + 1]; c++) {
         const int code = row_c[c];
@@ -237,36 +294,79 @@ __global__ void k_b_to_float(size_t n, const double *__restrict__ src, float *__
         dst[i] = (float)src[i];
 }
 
-// K [R][E] -> K64 / K32 [E][Rp]; padding realizations repeat realization 0
-__global__ void k_b_transpose_K(const double *__restrict__ K, int R, int Rp, int E, double *__restrict__ K64,
-                                float *__restrict__ K32) {
+// Load realizations into slots: K64 / K32 [E][Sp] column slot <- K[real][E]
+// for the listed (slot, real) pairs, through an LDS transpose (64 elements x
+// 64 pairs per block).
+__global__ void k_b_slot_K(const double *__restrict__ K, int E, int Sp, const int *__restrict__ pairs, int npairs,
+                           double *__restrict__ K64, float *__restrict__ K32) {
     __shared__ double t[64][65];
-    const int e0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int e0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int j = wid; j < 64; j += 4) {
-        int r = r0 + j, e = e0 + lane;
-        if (r >= R) r = 0;
-        t[j][lane] = e < E ? K[(size_t)r * E + e] : 0.0;
+        const int pi = p0 + j, e = e0 + lane;
+        t[j][lane] = (pi < npairs && e < E) ? K[(size_t)pairs[2 * pi + 1] * E + e] : 0.0;
     }
     __syncthreads();
+    const int pi = p0 + lane;
+    if (pi >= npairs) return;
+    const int slot = pairs[2 * pi];
     for (int j = wid; j < 64; j += 4) {
-        int e = e0 + j;
+        const int e = e0 + j;
         if (e < E) {
-            double v = t[lane][j];
-            K64[(size_t)e * Rp + r0 + lane] = v;
-            K32[(size_t)e * Rp + r0 + lane] = (float)v;
+            const double v = t[lane][j];
+            K64[(size_t)e * Sp + slot] = v;
+            K32[(size_t)e * Sp + slot] = (float)v;
         }
     }
 }
 
-// right-hand side and Jacobi diagonal of every realization (FP64)
+// Difficulty proxy per realization: variance of ln K over the elements (the
+// CG iteration count grows with the log-conductivity spread).
+__global__ void k_b_logk_var(const double *__restrict__ K, int E, double *__restrict__ out) {
+    __shared__ double sh[2][4];
+    const double *k = K + (size_t)blockIdx.x * E;
+    double s1 = 0.0, s2 = 0.0;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        const double l = log(k[e]);
+        s1 += l;
+        s2 += l * l;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wid] = s1;
+        sh[1][wid] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < 4; w++) {
+            a += sh[0][w];
+            b += sh[1][w];
+        }
+        a /= E;
+        out[blockIdx.x] = b / E - a * a;
+    }
+}
+
+__global__ void k_b_status_fill(int *status, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) status[i] = v;
+}
+
+// right-hand side and Jacobi diagonal of the selected slots (FP64)
 __global__ void __launch_bounds__(256) k_b_rhs(int F, size_t Rp, const int *__restrict__ row_ptr,
                                                const int *__restrict__ row_c, const int *__restrict__ meta,
                                                const int *__restrict__ rs_off, const double *__restrict__ rsT,
                                                const double *__restrict__ Kv, double *__restrict__ b,
-                                               double *__restrict__ dinv, float *__restrict__ dinv32) {
+                                               double *__restrict__ dinv, float *__restrict__ dinv32,
+                                               const int *__restrict__ lsel) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.y * 64 + lane;
+    if (lsel && !lsel[r]) return;  // no barrier in this kernel
     for (int row = blockIdx.x * kWaves + (threadIdx.x >> 6); row < F; row += gridDim.x * kWaves) {
         double sb = 0.0, sd = 0.0;
         for (int c = row_ptr[row]; c < row_ptr[row + 1]; c++) {
@@ -304,14 +404,17 @@ __global__ void __launch_bounds__(256) k_b_rhs(int F, size_t Rp, const int *__re
     }
 }
 
-// cold start: x = 0, r = b, z = D^-1 b, p = 0 (both buffers); partials r.z, r.r
+// cold start of the selected slots: x = 0, r = b, z = D^-1 b, p = 0 (both
+// buffers); partials r.z, r.r
 template <class V>
-__global__ void __launch_bounds__(256) k_b_init(BArgs<V> a, const double *__restrict__ b64, double *x64) {
+__global__ void __launch_bounds__(256) k_b_init(BArgs<V> a, const double *__restrict__ b64, double *x64,
+                                                const int *__restrict__ lsel) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.y * 64 + lane;
     int s0, st, en;
     row_range(a.F, a.G, a.xcd, s0, st, en);
     double v[2] = {0.0, 0.0};
+    if (lsel && !lsel[r]) en = s0;  // unselected slot: partials 0, still joins the block reduction
     for (int row = s0; row < en; row += st) {
         const size_t o = (size_t)row * a.Rp + r;
         const double b = b64[o];
@@ -346,8 +449,8 @@ __global__ void __launch_bounds__(256) k_b_op(BArgs<V> a) {
         row_range(a.F, a.G, a.xcd, s0, st, en);
         for (int row = s0; row < en; row += st) {
             if (act) {
-                const V q = op_row<V, true>(a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
-                                            a.opT64, a.K64, row, a.z, pin, beta, a.Rp, r);
+                const V q = op_row<V, true>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT,
+                                            a.K, a.opT64, a.K64, row, a.z, pin, beta, a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
                 const V p = a.z[o] + beta * pin[o];
                 pout[o] = p;
@@ -399,12 +502,13 @@ __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
 }
 
 // mixed precision: x64 += x32, x32 = 0 for the selected realizations
-// (all: every realization, at the end of the FP32 phase)
-__global__ void __launch_bounds__(256) k_b_accum(int F, int G, int xcd, size_t Rp, BatchScal sc, int all,
-                                                 double *__restrict__ x64, float *__restrict__ x32) {
+// (lsel: the listed slots instead, when they leave the FP32 phase)
+__global__ void __launch_bounds__(256) k_b_accum(int F, int G, int xcd, size_t Rp, BatchScal sc,
+                                                 const int *__restrict__ lsel, double *__restrict__ x64,
+                                                 float *__restrict__ x32) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.y * 64 + lane;
-    const bool act = all || ru_pick(sc, r, sc.status[r]);
+    const bool act = lsel ? lsel[r] != 0 : ru_pick(sc, r, sc.status[r]);
     if (!__any(act)) return;
     int s0, st, en;
     row_range(F, G, xcd, s0, st, en);
@@ -418,25 +522,28 @@ __global__ void __launch_bounds__(256) k_b_accum(int F, int G, int xcd, size_t R
 }
 
 // True FP64 residual r64 = b - S64 x64 for the realizations whose status bit
-// is set in sel (sel < 0: the reliable-update selection).  MODE 0: store r and
+// is set in sel (sel = -1: the reliable-update selection; sel = -2: the slots
+// flagged in lsel).  MODE 0: store r and
 // z = D^-1 r in V; MODE 1: also zero both search-direction buffers (FP64 warm
 // start); MODE 2: partial r.r only.
 template <class V, int MODE>
-__global__ void __launch_bounds__(256) k_b_resid(BArgs<V> a, int sel, const int *__restrict__ op_off64,
+__global__ void __launch_bounds__(256) k_b_resid(BArgs<V> a, int sel, const int *__restrict__ lsel,
+                                                 const int *__restrict__ op_off64,
                                                  const double *__restrict__ opT64, const double *__restrict__ K64,
                                                  const double *__restrict__ b64, const double *__restrict__ x64) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.y * 64 + lane;
     const int st0 = a.sc.status[r];
-    const bool act = sel < 0 ? ru_pick(a.sc, r, st0) : ((sel >> st0) & 1) != 0;
+    const bool act = sel == -2 ? lsel[r] != 0 : (sel == -1 ? ru_pick(a.sc, r, st0) : ((sel >> st0) & 1) != 0);
     double v[2] = {0.0, 0.0};
     if (__any(act)) {
         int s0, st, en;
         row_range(a.F, a.G, a.xcd, s0, st, en);
         for (int row = s0; row < en; row += st) {
             if (act) {
-                const double ax = op_row<double, false>(a.row_ptr, a.row_c, a.meta, op_off64, a.conn_off, a.conn,
-                                                        opT64, K64, opT64, K64, row, x64, nullptr, 0.0, a.Rp, r);
+                const double ax = op_row<double, false>(a.rec, a.row_ptr, a.row_c, a.meta, op_off64, a.conn_off,
+                                                        a.conn, opT64, K64, opT64, K64, row, x64, nullptr, 0.0,
+                                                        a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
                 const double rr = b64[o] - ax;
                 v[1] += rr * rr;
@@ -466,7 +573,8 @@ template <int MODE>
 __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const double *__restrict__ part0,
                                                          const double *__restrict__ part1,
                                                          const double *__restrict__ part2, int G, int Rp, int R,
-                                                         double rtol2, double atol2, int max_it, int mixed) {
+                                                         double rtol2, double atol2, int max_it, int mixed,
+                                                         const int *__restrict__ lsel) {
     __shared__ double sh[3][kFoldThreads / 64][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int r = blockIdx.x * 64 + lane;
@@ -488,23 +596,26 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
         s[k] = t;
     }
     int st = sc.status[r];
+    const bool picked = !lsel || lsel[r] != 0;  // Init / Final touch only the selected slots
     if (MODE == kFoldInit) {
-        // s[1] = r.z, s[2] = r.r = b.b
-        sc.bnorm2[r] = s[2];
-        sc.tol2[r] = fmax(rtol2 * s[2], atol2);
-        sc.rz[r] = s[1];
-        sc.rz_prev[r] = 1.0;
-        sc.rr[r] = s[2];
-        sc.beta[r] = 0.0;
-        sc.alpha[r] = 0.0;
-        sc.it[r] = 0;
-        sc.n_ru[r] = 0;
-        sc.it_ru[r] = 0;
-        sc.par[r] = 0;
-        sc.rr_hist[r] = INFINITY;
-        sc.rr_ru[r] = s[2];
-        st = (r < R) ? (s[2] <= sc.tol2[r] ? kConverged : kRun) : kConverged;
-        sc.status[r] = st;
+        if (picked) {
+            // s[1] = r.z, s[2] = r.r = b.b
+            sc.bnorm2[r] = s[2];
+            sc.tol2[r] = fmax(rtol2 * s[2], atol2);
+            sc.rz[r] = s[1];
+            sc.rz_prev[r] = 1.0;
+            sc.rr[r] = s[2];
+            sc.beta[r] = 0.0;
+            sc.alpha[r] = 0.0;
+            sc.it[r] = 0;
+            sc.n_ru[r] = 0;
+            sc.it_ru[r] = 0;
+            sc.par[r] = 0;
+            sc.rr_hist[r] = INFINITY;
+            sc.rr_ru[r] = s[2];
+            st = s[2] <= sc.tol2[r] ? kConverged : kRun;
+            sc.status[r] = st;
+        }
     } else if (MODE == kFoldAlpha) {
         if (st == kRun) {
             const double pq = s[0];
@@ -563,7 +674,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
             sc.status[r] = st;
         }
     } else {  // final true residual
-        sc.rr[r] = s[2];
+        if (picked) sc.rr[r] = s[2];
     }
     const unsigned long long run = __ballot(st == kRun), f64 = __ballot(st == kFp64);
     if (lane == 0) {
@@ -573,18 +684,21 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
 }
 
 // Reconstruction of element pressure and side fluxes (schur.cc:714-773) for
-// every realization; output interleaved sol[idx * Rp + r].
+// the listed (slot, realization) pairs: edge traces lam and K read from the
+// slot, output written to the realization's column, sol[idx * Rout + real].
 template <int D>
 __global__ void __launch_bounds__(256) k_b_resolve(int e_begin, int e_end, const int *__restrict__ perm,
                                                    const int *__restrict__ side_off, int S, size_t Rp,
                                                    const int *__restrict__ meta, const int *__restrict__ rs_off,
                                                    const double *__restrict__ rsT, const int *__restrict__ conn_off,
                                                    const int *__restrict__ conn, const double *__restrict__ Kv,
-                                                   const double *__restrict__ lam, double *__restrict__ sol) {
+                                                   const double *__restrict__ lam, double *__restrict__ sol,
+                                                   const int *__restrict__ pairs, int npairs, size_t Rout) {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.y * 64 + lane;
+    const int pi = blockIdx.y * 64 + lane;
     const int idx = e_begin + blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (idx >= e_end) return;
+    if (idx >= e_end || pi >= npairs) return;
+    const int r = pairs[2 * pi], real = pairs[2 * pi + 1];
     constexpr int n0 = D + 1;
     const int e = perm[idx];
     const int mt = meta[e];
@@ -612,22 +726,80 @@ __global__ void __launch_bounds__(256) k_b_resolve(int e_begin, int e_end, const
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < n0; j++) t += Q[i * n0 + j] * cl[j];
-        sol[(size_t)(so + i) * Rp + r] = K * Q[n0 * n0 + n0 + i] + pe * (K * Q[n0 * n0 + i]) - K * t;
+        sol[(size_t)(so + i) * Rout + real] = K * Q[n0 * n0 + n0 + i] + pe * (K * Q[n0 * n0 + i]) - K * t;
     }
-    sol[(size_t)(S + e) * Rp + r] = pe;
+    sol[(size_t)(S + e) * Rout + real] = pe;
 }
 
 __global__ void k_b_scatter_lambda(int F, size_t Rp, const int *__restrict__ perm, const double *__restrict__ x,
-                                   double *__restrict__ lam_out) {
+                                   double *__restrict__ lam_out, const int *__restrict__ pairs, int npairs,
+                                   size_t Rout) {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.y * 64 + lane;
+    const int pi = blockIdx.y * 64 + lane;
+    if (pi >= npairs) return;
+    const int r = pairs[2 * pi], real = pairs[2 * pi + 1];
     for (int row = blockIdx.x * kWaves + (threadIdx.x >> 6); row < F; row += gridDim.x * kWaves)
-        lam_out[(size_t)perm[row] * Rp + r] = x[(size_t)row * Rp + r];
+        lam_out[(size_t)perm[row] * Rout + real] = x[(size_t)row * Rp + r];
+}
+
+// edge traces of one slot, rows -> edge order
+__global__ void k_b_extract_x(int F, size_t Sp, int slot, const int *__restrict__ perm, const double *__restrict__ x,
+                              double *__restrict__ out) {
+    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < F; row += gridDim.x * blockDim.x)
+        out[perm[row]] = x[(size_t)row * Sp + slot];
+}
+
+// a single-solve [q | p | lambda] vector into a realization's output column
+__global__ void k_b_put_col(size_t n, size_t Rout, int real, const double *__restrict__ src,
+                            double *__restrict__ sol) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        sol[i * Rout + real] = src[i];
 }
 
 __global__ void k_b_extract(size_t n, size_t Rp, int r, const double *__restrict__ sol, double *__restrict__ out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = sol[i * Rp + r];
+}
+
+// Slot permutation of a realization-interleaved array: out[n][s] = in[n][src[s]]
+template <class T>
+__global__ void k_b_perm(const T *__restrict__ in, T *__restrict__ out, size_t N, size_t Rp,
+                         const int *__restrict__ src) {
+    const int s = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int from = src[s];
+    for (size_t n = blockIdx.x * (size_t)kWaves + (threadIdx.x >> 6); n < N; n += (size_t)gridDim.x * kWaves)
+        out[n * Rp + s] = in[n * Rp + from];
+}
+
+// the same for the search direction, taken from the buffer each slot's parity names
+template <class T>
+__global__ void k_b_perm_p(const T *__restrict__ p0, const T *__restrict__ p1, T *__restrict__ out, size_t N,
+                           size_t Rp, const int *__restrict__ src, const int *__restrict__ par) {
+    const int s = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int from = src[s];
+    const T *in = par[from] ? p1 : p0;
+    for (size_t n = blockIdx.x * (size_t)kWaves + (threadIdx.x >> 6); n < N; n += (size_t)gridDim.x * kWaves)
+        out[n * Rp + s] = in[n * Rp + from];
+}
+
+__global__ void k_b_perm_scal(BatchScal in, BatchScal out, const int *__restrict__ src, int Rp) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Rp) return;
+    const int f = src[s];
+    out.alpha[s] = in.alpha[f];
+    out.beta[s] = in.beta[f];
+    out.rz[s] = in.rz[f];
+    out.rz_prev[s] = in.rz_prev[f];
+    out.rr[s] = in.rr[f];
+    out.tol2[s] = in.tol2[f];
+    out.bnorm2[s] = in.bnorm2[f];
+    out.rr_hist[s] = in.rr_hist[f];
+    out.rr_ru[s] = in.rr_ru[f];
+    out.status[s] = in.status[f];
+    out.it[s] = in.it[f];
+    out.n_ru[s] = in.n_ru[f];
+    out.it_ru[s] = in.it_ru[f];
+    out.par[s] = 0;  // directions are moved into buffer 0
 }
 
 }  // namespace
@@ -642,10 +814,12 @@ static T *b_up(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s)
     return p;
 }
 
-DarcyBatchGPU::DarcyBatchGPU(DarcyGPU &base, int R) : base_(base), dv_(base.dev()) {
+DarcyBatchGPU::DarcyBatchGPU(DarcyGPU &base, int R, int slots) : base_(base), dv_(base.dev()) {
     if (R < 1) throw Error(-1, "batch needs at least one realization");
+    if (slots < 0) throw Error(-1, "slot count must be >= 0");
     R_ = R;
-    Rp_ = (R + 63) / 64 * 64;
+    Rout_ = (R + 63) / 64 * 64;
+    Rp_ = std::min(Rout_, slots > 0 ? (slots + 63) / 64 * 64 : 256);
     nchunk_ = Rp_ / 64;
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     build();
@@ -688,7 +862,59 @@ void DarcyBatchGPU::build() {
         for (int i = 0; i <= m.el_dim[e]; i++) conn[o++] = m.side_edge[m.side_off[e] + i];
         for (int k = m.el_vb_ptr[e]; k < m.el_vb_ptr[e + 1]; k++) conn[o++] = m.vb_edge[m.el_vb[k]];
     }
-    // rows in edge order; contributions of each row in element order
+    // Rows in Morton (Z-curve) order of the edge positions: a row's neighbours
+    // are then close in memory, so the gathers of rows processed together hit
+    // the same L2 lines.  perm: row -> edge; conn is renumbered to rows.
+    std::vector<int> perm(F);
+    std::iota(perm.begin(), perm.end(), 0);
+    if (!std::getenv("FL_BATCH_EDGE_ORDER")) {
+        std::vector<double> ec(3 * (size_t)F, 0.0);
+        std::vector<int> cnt(F, 0);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int e = 0; e < E; e++) {
+            double c[3] = {0, 0, 0};
+            const int nn = m.el_dim[e] + 1;
+            for (int k = 0; k < nn; k++)
+                for (int d = 0; d < 3; d++) c[d] += m.xyz[3 * (size_t)m.el_nodes[4 * (size_t)e + k] + d] / nn;
+            for (int i = conn_off[e]; i < conn_off[e + 1]; i++) {
+                for (int d = 0; d < 3; d++) ec[3 * (size_t)conn[i] + d] += c[d];
+                cnt[conn[i]]++;
+            }
+        }
+        for (int f = 0; f < F; f++)
+            for (int d = 0; d < 3; d++) {
+                double v = ec[3 * (size_t)f + d] / std::max(cnt[f], 1);
+                ec[3 * (size_t)f + d] = v;
+                lo[d] = std::min(lo[d], v);
+                hi[d] = std::max(hi[d], v);
+            }
+        auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+            v &= 0x1fffff;
+            v = (v | v << 32) & 0x1f00000000ffffULL;
+            v = (v | v << 16) & 0x1f0000ff0000ffULL;
+            v = (v | v << 8) & 0x100f00f00f00f00fULL;
+            v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+            v = (v | v << 2) & 0x1249249249249249ULL;
+            return v;
+        };
+        std::vector<uint64_t> key(F);
+        for (int f = 0; f < F; f++) {
+            uint64_t k = 0;
+            for (int d = 0; d < 3; d++) {
+                double w = hi[d] > lo[d] ? (ec[3 * (size_t)f + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+                uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, w * 2097151.0));
+                k |= spread(q) << d;
+            }
+            key[f] = k;
+        }
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    {
+        std::vector<int> rinv(F);
+        for (int r = 0; r < F; r++) rinv[perm[r]] = r;
+        for (auto &c : conn) c = rinv[c];
+    }
+    // contributions of each row in element order
     std::vector<int> row_ptr(F + 1, 0), row_c(conn_off[E]);
     for (int i = 0; i < conn_off[E]; i++) row_ptr[conn[i] + 1]++;
     for (int f = 0; f < F; f++) row_ptr[f + 1] += row_ptr[f];
@@ -698,8 +924,31 @@ void DarcyBatchGPU::build() {
             for (int i = conn_off[e]; i < conn_off[e + 1]; i++) row_c[fp[conn[i]]++] = e * kMaxConn + (i - conn_off[e]);
     }
     ncontrib_ = conn_off[E];
-    std::vector<int> perm(F);
-    std::iota(perm.begin(), perm.end(), 0);
+    std::vector<int> rec((size_t)F * kRecInts, -1);
+    for (int f = 0; f < F; f++) {
+        int *Rr = &rec[(size_t)f * kRecInts];
+        const int nc = row_ptr[f + 1] - row_ptr[f];
+        bool simple = nc <= 2;
+        for (int c = row_ptr[f]; c < row_ptr[f + 1] && simple; c++) {
+            const int e = row_c[c] / kMaxConn;
+            simple = m.el_vb_ptr[e + 1] == m.el_vb_ptr[e];
+        }
+        if (!simple) {
+            Rr[0] = -2;
+            continue;
+        }
+        for (int k = 0; k < nc; k++) {
+            const int code = row_c[row_ptr[f] + k];
+            const int e = code / kMaxConn, i = code % kMaxConn, n0 = m.el_dim[e] + 1;
+            int *Q = Rr + 8 * k;
+            Q[0] = e;
+            Q[1] = i;
+            Q[2] = n0;
+            Q[3] = op_off[e];
+            for (int j = 0; j < 4; j++) Q[4 + j] = j < n0 ? conn[conn_off[e] + j] : 0;
+        }
+    }
+    d_rec_ = b_up(owned_, rec, stream_);
     std::vector<int> dperm;
     for (int d = 1; d <= 3; d++) {
         for (int e = 0; e < E; e++)
@@ -732,17 +981,20 @@ void DarcyBatchGPU::build() {
     d_dinv_ = ad(nv);
     d_dinv32_ = af(nv);
     d_x_ = ad(nv); d_r_ = ad(nv); d_z_ = ad(nv); d_p0_ = ad(nv); d_p1_ = ad(nv); d_q_ = ad(nv);
-    d_sol_ = ad(((size_t)S_ + E + F) * Rp_);
+    d_sol_ = ad(((size_t)S_ + E + F) * Rout_);
+    d_Kall_ = ad((size_t)R_ * E);
+    d_var_ = ad(R_);
+    d_pairs_ = ai(2 * (size_t)Rp_);
+    d_Ksave_ = ad(E);
+    if (const char *h = std::getenv("FL_BATCH_HANDOVER")) handover_ = std::max(0, std::atoi(h));
+    d_lsel_ = ai(Rp_);
     d_ext_ = ad((size_t)S_ + E + F);
     d_part_ = ad(3 * (size_t)kMaxGrid * Rp_);
-    double *scd = ad(9 * (size_t)Rp_);
-    sc_.alpha = scd; sc_.beta = scd + Rp_; sc_.rz = scd + 2 * Rp_; sc_.rz_prev = scd + 3 * Rp_;
-    sc_.rr = scd + 4 * Rp_; sc_.tol2 = scd + 5 * Rp_; sc_.bnorm2 = scd + 6 * Rp_; sc_.rr_hist = scd + 7 * Rp_;
-    sc_.rr_ru = scd + 8 * Rp_;
-    int *sci = ai(5 * (size_t)Rp_ + 2 * nchunk_);
-    sc_.status = sci; sc_.it = sci + Rp_; sc_.n_ru = sci + 2 * Rp_; sc_.par = sci + 3 * Rp_;
-    sc_.it_ru = sci + 4 * Rp_;
-    sc_.active_count = sci + 5 * Rp_;
+    alloc_scal(sc_);
+    alloc_scal(sc2_);
+    d_src_ = ai(Rp_);
+    d_sp64_ = ad(nv);
+    d_spK_ = ad((size_t)E * Rp_);
     FL_HIP(hipHostMalloc((void **)&h_count_, 2 * nchunk_ * sizeof(int), hipHostMallocDefault));
     if (const char *g = std::getenv("FL_BATCH_GRID")) {  // launch-shape override: grid[,xcd[,chunk]]
         int gv = 0, xv = 0, cv = 16;
@@ -759,31 +1011,95 @@ void DarcyBatchGPU::alloc32() {
     const size_t nv = (size_t)F_ * Rp_;
     auto af = [&](size_t n) { float *p = dev_alloc<float>(n); owned_.push_back(p); return p; };
     d_x32_ = af(nv); d_r32_ = af(nv); d_z32_ = af(nv); d_p32a_ = af(nv); d_p32b_ = af(nv); d_q32_ = af(nv);
+    d_sp32_ = af(nv);
+    d_spK32_ = af((size_t)E_ * Rp_);
+}
+
+void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
+    double *scd = dev_alloc<double>(9 * (size_t)Rp_);
+    int *sci = dev_alloc<int>(6 * (size_t)Rp_ + 2 * nchunk_);
+    owned_.push_back(scd);
+    owned_.push_back(sci);
+    sc.alpha = scd; sc.beta = scd + Rp_; sc.rz = scd + 2 * Rp_; sc.rz_prev = scd + 3 * Rp_;
+    sc.rr = scd + 4 * Rp_; sc.tol2 = scd + 5 * Rp_; sc.bnorm2 = scd + 6 * Rp_; sc.rr_hist = scd + 7 * Rp_;
+    sc.rr_ru = scd + 8 * Rp_;
+    sc.status = sci; sc.it = sci + Rp_; sc.n_ru = sci + 2 * Rp_; sc.par = sci + 3 * Rp_;
+    sc.it_ru = sci + 4 * Rp_;
+    sc.active_count = sci + 6 * Rp_;
+}
+
+void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
+    FL_HIP(hipMemcpyAsync(d_src_, src.data(), Rp_ * sizeof(int), hipMemcpyHostToDevice, stream_));
+    const dim3 blk(256);
+    const dim3 gF(std::min((F_ + kWaves - 1) / kWaves, 2048), nchunk_), gE(std::min((E_ + kWaves - 1) / kWaves, 2048), nchunk_);
+    auto perm64 = [&](double *&a) {
+        hipLaunchKernelGGL(k_b_perm<double>, gF, blk, 0, stream_, (const double *)a, d_sp64_, (size_t)F_,
+                           (size_t)Rp_, (const int *)d_src_);
+        std::swap(a, d_sp64_);
+    };
+    auto perm32 = [&](float *&a) {
+        hipLaunchKernelGGL(k_b_perm<float>, gF, blk, 0, stream_, (const float *)a, d_sp32_, (size_t)F_,
+                           (size_t)Rp_, (const int *)d_src_);
+        std::swap(a, d_sp32_);
+    };
+    perm64(d_x_);
+    perm64(d_b_);
+    perm64(d_dinv_);
+    hipLaunchKernelGGL(k_b_perm<double>, gE, blk, 0, stream_, (const double *)d_K_, d_spK_, (size_t)E_, (size_t)Rp_,
+                       (const int *)d_src_);
+    std::swap(d_K_, d_spK_);
+    if (phase == 0) {
+        perm64(d_r_);
+        perm64(d_z_);
+        hipLaunchKernelGGL(k_b_perm_p<double>, gF, blk, 0, stream_, (const double *)d_p0_, (const double *)d_p1_,
+                           d_sp64_, (size_t)F_, (size_t)Rp_, (const int *)d_src_, (const int *)sc_.par);
+        std::swap(d_p0_, d_sp64_);
+    } else {
+        perm32(d_x32_);
+        perm32(d_r32_);
+        perm32(d_z32_);
+        perm32(d_dinv32_);
+        hipLaunchKernelGGL(k_b_perm_p<float>, gF, blk, 0, stream_, (const float *)d_p32a_, (const float *)d_p32b_,
+                           d_sp32_, (size_t)F_, (size_t)Rp_, (const int *)d_src_, (const int *)sc_.par);
+        std::swap(d_p32a_, d_sp32_);
+        hipLaunchKernelGGL(k_b_perm<float>, gE, blk, 0, stream_, (const float *)d_K32_, d_spK32_, (size_t)E_,
+                           (size_t)Rp_, (const int *)d_src_);
+        std::swap(d_K32_, d_spK32_);
+    }
+    hipLaunchKernelGGL(k_b_perm_scal, dim3((Rp_ + 255) / 256), blk, 0, stream_, sc_, sc2_, (const int *)d_src_, Rp_);
+    std::swap(sc_, sc2_);
+    FL_HIP(hipGetLastError());
+    std::vector<int> nr(Rp_);
+    for (int s = 0; s < Rp_; s++) nr[s] = rid_[src[s]];
+    rid_.swap(nr);
+    free_graphs();
+    FL_HIP(hipStreamSynchronize(stream_));  // src staging buffer is reused by the next call
+}
+
+void DarcyBatchGPU::set_pairs(const std::vector<int> &pairs) {
+    npairs_ = (int)pairs.size() / 2;
+    std::vector<int> sel(Rp_, 0);
+    for (int k = 0; k < npairs_; k++) sel[pairs[2 * k]] = 1;
+    if (npairs_) FL_HIP(hipMemcpyAsync(d_pairs_, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipMemcpyAsync(d_lsel_, sel.data(), Rp_ * sizeof(int), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));  // host staging vectors go out of scope
 }
 
 void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
     if (grid < 8 || grid > kMaxGrid || grid % 8) throw Error(-1, "batch grid must be a multiple of 8 in [8, 2048]");
     if (chunk < 1 || chunk > 1024) throw Error(-1, "batch chunk must be in [1, 1024]");
-    if (grid != grid_ || (xcd_map != 0) != (xcd_map_ != 0) || chunk != chunk_) free_graphs();
+    if (grid != grid_ || xcd_map != xcd_map_ || chunk != chunk_) free_graphs();
+    if (xcd_map < 0 || xcd_map > 2) throw Error(-1, "batch row mapping must be 0, 1 or 2");
     grid_ = grid;
-    xcd_map_ = xcd_map ? 1 : 0;
+    xcd_map_ = xcd_map;
     chunk_ = chunk;
     tuned_ = true;
 }
 
 void DarcyBatchGPU::set_conductivity(const double *K, bool device_ptr) {
-    const double *src = K;
-    double *tmp = nullptr;
-    if (!device_ptr) {
-        tmp = dev_alloc<double>((size_t)R_ * E_);
-        FL_HIP(hipMemcpyAsync(tmp, K, (size_t)R_ * E_ * sizeof(double), hipMemcpyHostToDevice, stream_));
-        src = tmp;
-    }
-    dim3 g((E_ + 63) / 64, nchunk_);
-    hipLaunchKernelGGL(k_b_transpose_K, g, dim3(256), 0, stream_, src, R_, Rp_, E_, d_K_, d_K32_);
-    FL_HIP(hipGetLastError());
+    FL_HIP(hipMemcpyAsync(d_Kall_, K, (size_t)R_ * E_ * sizeof(double),
+                          device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
-    if (tmp) dev_free(tmp);
     have_K_ = true;
 }
 
@@ -799,20 +1115,19 @@ void DarcyBatchGPU::setup_system() {
     }
     hipLaunchKernelGGL(k_b_to_float, dim3(std::min(grid_for(op_total_), 4096)), dim3(kBlock), 0, stream_,
                        (size_t)op_total_, (const double *)d_tab_, d_tab32_);
-    dim3 g(std::min((F_ + kWaves - 1) / kWaves, 4096), nchunk_);
-    hipLaunchKernelGGL(k_b_rhs, g, dim3(256), 0, stream_, F_, (size_t)Rp_, d_row_ptr_, d_row_c_, d_meta_, d_rs_off_,
-                       d_rsT_, d_K_, d_b_, d_dinv_, d_dinv32_);
     FL_HIP(hipGetLastError());
 }
 
 template <class V>
-static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *row_ptr, const int *row_c, const int *meta,
+static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *rec, const int *row_ptr, const int *row_c,
+                          const int *meta,
                           const int *op_off, const int *conn_off, const int *conn, const V *opT, const V *K,
                           const double *opT64, const double *K64,
                           const V *dinv, V *x, V *r, V *z, V *p0, V *p1, V *q, double *part, const BatchScal &sc) {
     BArgs<V> a;
     a.F = F; a.Rp = Rp; a.G = G; a.xcd = xcd;
     a.row_ptr = row_ptr; a.row_c = row_c; a.meta = meta; a.op_off = op_off; a.conn_off = conn_off; a.conn = conn;
+    a.rec = rec;
     a.opT = opT; a.K = K; a.dinv = dinv;
     a.opT64 = opT64; a.K64 = K64;
     a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
@@ -824,30 +1139,41 @@ static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *row_ptr, con
 BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precision) {
     if (!have_K_) throw Error(-4, "batch conductivities not set");
     if (precision != 0 && precision != 1) throw Error(-1, "batch precision must be 0 (FP64) or 1 (FP32 + FP64)");
+    if (max_it < 1) throw Error(-1, "max_it must be positive");
     if (precision == 1) alloc32();
     BatchStats st;
     st.n_real = R_;
     auto t_start = std::chrono::steady_clock::now();
-    hipEvent_t e0, e1, e2, e3;
-    FL_HIP(hipEventCreate(&e0)); FL_HIP(hipEventCreate(&e1)); FL_HIP(hipEventCreate(&e2)); FL_HIP(hipEventCreate(&e3));
+    hipEvent_t e0, e1;
+    FL_HIP(hipEventCreate(&e0));
+    FL_HIP(hipEventCreate(&e1));
     FL_HIP(hipEventRecord(e0, stream_));
     setup_system();
-    if (!tuned_) tune();
     FL_HIP(hipEventRecord(e1, stream_));
-    const int G = grid_, X = xcd_map_;
-    const dim3 grow(G, nchunk_), blk(256), gfold(nchunk_), bfold(kFoldThreads);
+    const dim3 blk(256), bfold(kFoldThreads);
     const double rtol2 = rtol * rtol, atol2 = atol * atol;
-    BArgs<double> a64 = make_args<double>(F_, Rp_, G, X, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_,
-                                          d_conn_, d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_, d_p0_, d_p1_, d_q_,
-                                          d_part_, sc_);
-    BArgs<float> a32 = make_args<float>(F_, Rp_, G, X, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_,
-                                        d_conn_, d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_, d_x32_, d_r32_, d_z32_, d_p32a_,
-                                        d_p32b_, d_q32_, d_part_, sc_);
-    double *P0 = a64.part0, *P1 = a64.part1, *P2 = a64.part2;
-    auto fold = [&](int mode, int mixed) {
-        const double *q0 = P0, *q1 = P1, *q2 = P2;
-#define FL_FOLD(M) \
-    hipLaunchKernelGGL(k_b_fold<M>, gfold, bfold, 0, stream_, sc_, q0, q1, q2, G, Rp_, R_, rtol2, atol2, max_it, mixed)
+    nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
+    BArgs<double> a64;
+    BArgs<float> a32;
+    int G = grid_, X = xcd_map_;
+    auto rebuild = [&]() {
+        G = grid_;
+        X = xcd_map_;
+        a64 = make_args<double>(F_, Rp_, G, X, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_, d_conn_,
+                                d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_, d_p0_, d_p1_, d_q_, d_part_,
+                                sc_);
+        a32 = make_args<float>(F_, Rp_, G, X, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_, d_conn_,
+                               d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_, d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_,
+                               d_q32_, d_part_, sc_);
+    };
+    rebuild();
+    auto grow = [&]() { return dim3(G, nch_); };
+    auto fold = [&](int mode, int mixed, const int *lsel) {
+        const double *q0 = a64.part0, *q1 = a64.part1, *q2 = a64.part2;
+        const dim3 gf(nch_);
+#define FL_FOLD(M)                                                                                                 \
+    hipLaunchKernelGGL(k_b_fold<M>, gf, bfold, 0, stream_, sc_, q0, q1, q2, G, Rp_, R_, rtol2, atol2, max_it, mixed, \
+                       lsel)
         switch (mode) {
             case kFoldInit: FL_FOLD(kFoldInit); break;
             case kFoldAlpha: FL_FOLD(kFoldAlpha); break;
@@ -858,44 +1184,235 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         }
 #undef FL_FOLD
     };
+    const int *nosel = nullptr;
     // one CG iteration; ev: optional events around the operator and the update
     auto iter64 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        hipLaunchKernelGGL(k_b_op<double>, grow, blk, 0, stream_, a64);
+        hipLaunchKernelGGL(k_b_op<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
-        fold(kFoldAlpha, 0);
+        fold(kFoldAlpha, 0, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        hipLaunchKernelGGL(k_b_update<double>, grow, blk, 0, stream_, a64);
+        hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        fold(kFoldBeta, 0);
+        fold(kFoldBeta, 0, nosel);
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a32);
+        hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
-        fold(kFoldAlpha, 1);
+        fold(kFoldAlpha, 1, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        hipLaunchKernelGGL(k_b_update<float>, grow, blk, 0, stream_, a32);
+        hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        fold(kFoldBeta, 1);
+        fold(kFoldBeta, 1, nosel);
     };
     auto reliable_update = [&]() {
-        hipLaunchKernelGGL(k_b_accum, grow, blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, 0, d_x_, d_x32_);
-        hipLaunchKernelGGL((k_b_resid<float, 0>), grow, blk, 0, stream_, a32, -1,
+        hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_, d_x32_);
+        hipLaunchKernelGGL((k_b_resid<float, 0>), grow(), blk, 0, stream_, a32, -1, nosel, (const int *)d_op_off_,
+                           (const double *)d_tab_, (const double *)d_K_, (const double *)d_b_,
+                           (const double *)d_x_);
+        fold(kFoldRU, 1, nosel);
+    };
+    // graph of `chunk_` iterations (kind 0: FP64; kind 1: FP32 + reliable update),
+    // re-captured when the launch width or the iteration limit changes
+    auto graph = [&](int kind) -> hipGraphExec_t {
+        hipGraphExec_t &ex = kind ? g32_ : g64_;
+        long long &key = kind ? g32_key_ : g64_key_;
+        const long long want = (long long)max_it * 65536 + nch_;
+        if (ex && key == want) return ex;
+        if (ex) (void)hipGraphExecDestroy(ex);
+        ex = nullptr;
+        hipGraph_t gr;
+        FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < chunk_; k++) {
+            if (kind) iter32(nullptr);
+            else iter64(nullptr);
+        }
+        if (kind) reliable_update();
+        FL_HIP(hipStreamEndCapture(stream_, &gr));
+        FL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        key = want;
+        return ex;
+    };
+    bool ksaved = false;  // the base solver's K, borrowed by hand-overs, is restored at the end
+    // per-realization results
+    h_it_.assign(R_, 0);
+    h_status_.assign(R_, kIdle);
+    h_ru_.assign(R_, 0);
+    h_rr_.assign(R_, 0.0);
+    h_bb_.assign(R_, 0.0);
+    h_tol_.assign(R_, 0.0);
+    std::vector<int> stv(Rp_), itv(Rp_), ruv(Rp_);
+    std::vector<double> rrv(Rp_), bbv(Rp_), tolv(Rp_);
+    auto read_status = [&]() {
+        FL_HIP(hipMemcpyAsync(stv.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+    };
+    // Load realizations into empty slots: K, right-hand side, Jacobi diagonal,
+    // cold-start vectors and scalars of those slots only.
+    auto refill = [&](const std::vector<int> &pairs, int mixed) {
+        if (pairs.empty()) return;
+        set_pairs(pairs);
+        const int np = (int)pairs.size() / 2;
+        hipLaunchKernelGGL(k_b_slot_K, dim3((E_ + 63) / 64, (np + 63) / 64), blk, 0, stream_, (const double *)d_Kall_,
+                           E_, Rp_, (const int *)d_pairs_, np, d_K_, d_K32_);
+        hipLaunchKernelGGL(k_b_rhs, dim3(std::min((F_ + kWaves - 1) / kWaves, 4096), nch_), blk, 0, stream_, F_,
+                           (size_t)Rp_, (const int *)d_row_ptr_, (const int *)d_row_c_, (const int *)d_meta_,
+                           (const int *)d_rs_off_, (const double *)d_rsT_, (const double *)d_K_, d_b_, d_dinv_,
+                           d_dinv32_, (const int *)d_lsel_);
+        if (mixed)
+            hipLaunchKernelGGL(k_b_init<float>, grow(), blk, 0, stream_, a32, (const double *)d_b_, d_x_,
+                               (const int *)d_lsel_);
+        else
+            hipLaunchKernelGGL(k_b_init<double>, grow(), blk, 0, stream_, a64, (const double *)d_b_,
+                               (double *)nullptr, (const int *)d_lsel_);
+        fold(kFoldInit, mixed, d_lsel_);
+        FL_HIP(hipGetLastError());
+        for (int k = 0; k < np; k++) rid_[pairs[2 * k]] = pairs[2 * k + 1];
+    };
+    // Finished slots: final FP64 true residual, reconstruction into the
+    // realization's output column, per-realization statistics; slot emptied.
+    auto harvest = [&](const std::vector<int> &pairs, int mixed) {
+        if (pairs.empty()) return;
+        set_pairs(pairs);
+        const int np = (int)pairs.size() / 2;
+        if (mixed)
+            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
+                               d_x_, d_x32_);
+        hipLaunchKernelGGL((k_b_resid<double, 2>), grow(), blk, 0, stream_, a64, -2, (const int *)d_lsel_,
                            (const int *)d_op_off_, (const double *)d_tab_, (const double *)d_K_,
                            (const double *)d_b_, (const double *)d_x_);
-        fold(kFoldRU, 1);
-    };
-    auto counts = [&](int &run, int &f64) {
-        FL_HIP(hipMemcpyAsync(h_count_, sc_.active_count, 2 * nchunk_ * sizeof(int), hipMemcpyDeviceToHost,
-                              stream_));
+        fold(kFoldFinal, 0, d_lsel_);
+        double *lam = d_sol_ + ((size_t)S_ + E_) * Rout_;
+        const dim3 gl(std::min((F_ + kWaves - 1) / kWaves, 4096), (np + 63) / 64);
+        hipLaunchKernelGGL(k_b_scatter_lambda, gl, blk, 0, stream_, F_, (size_t)Rp_, (const int *)d_perm_,
+                           (const double *)d_x_, lam, (const int *)d_pairs_, np, (size_t)Rout_);
+        for (int D = 1; D <= 3; D++) {
+            int n = dim_start_[D] - dim_start_[D - 1];
+            if (n <= 0) continue;
+            dim3 g((n + kWaves - 1) / kWaves, (np + 63) / 64);
+#define FL_RESOLVE(DD)                                                                                        \
+    hipLaunchKernelGGL(k_b_resolve<DD>, g, blk, 0, stream_, dim_start_[D - 1], dim_start_[D],                 \
+                       (const int *)d_eperm_, dv_.side_off, S_, (size_t)Rp_, (const int *)d_meta_,            \
+                       (const int *)d_rs_off_, (const double *)d_rsT_, (const int *)d_conn_off_,              \
+                       (const int *)d_conn_, (const double *)d_K_, (const double *)d_x_, d_sol_,              \
+                       (const int *)d_pairs_, np, (size_t)Rout_)
+            if (D == 1) FL_RESOLVE(1);
+            else if (D == 2) FL_RESOLVE(2);
+            else FL_RESOLVE(3);
+#undef FL_RESOLVE
+        }
+        FL_HIP(hipGetLastError());
+        FL_HIP(hipMemcpyAsync(stv.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(itv.data(), sc_.it, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(ruv.data(), sc_.n_ru, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(rrv.data(), sc_.rr, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(bbv.data(), sc_.bnorm2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(tolv.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
-        run = f64 = 0;
-        for (int c = 0; c < nchunk_; c++) {
-            run += h_count_[2 * c];
-            f64 += h_count_[2 * c + 1];
+        for (int k = 0; k < np; k++) {
+            const int sl = pairs[2 * k], r = pairs[2 * k + 1];
+            h_status_[r] = stv[sl];
+            h_it_[r] = itv[sl];
+            h_ru_[r] = ruv[sl];
+            h_rr_[r] = rrv[sl];
+            h_bb_[r] = bbv[sl];
+            h_tol_[r] = tolv[sl];
+            rid_[sl] = -1;
         }
     };
+    // Hand the last live realizations to the single-solve FP64 path,
+    // warm-started from the batch iterate: a batched iteration costs a sweep
+    // over all rows whatever the number of live lanes, a single solve's
+    // iteration touches one realization only.
+    auto handover = [&](const std::vector<int> &pairs, int mixed_now) {
+        if (pairs.empty()) return;
+        set_pairs(pairs);
+        if (mixed_now)
+            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
+                               d_x_, d_x32_);
+        FL_HIP(hipMemcpyAsync(stv.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(itv.data(), sc_.it, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(ruv.data(), sc_.n_ru, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(bbv.data(), sc_.bnorm2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(tolv.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+        if (!ksaved) {
+            FL_HIP(hipMemcpyAsync(d_Ksave_, base_.device_K(), E_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            FL_HIP(hipStreamSynchronize(stream_));
+            ksaved = true;
+        }
+        const size_t n = (size_t)S_ + E_ + F_;
+        for (size_t k = 0; k < pairs.size(); k += 2) {
+            const int sl = pairs[k], r = pairs[k + 1];
+            hipLaunchKernelGGL(k_b_extract_x, dim3(std::min(grid_for(F_), 4096)), dim3(kBlock), 0, stream_, F_,
+                               (size_t)Rp_, sl, (const int *)d_perm_, (const double *)d_x_, d_ext_);
+            FL_HIP(hipStreamSynchronize(stream_));
+            base_.set_conductivity_device(d_Kall_ + (size_t)r * E_);
+            const int left = std::max(1, max_it - itv[sl]);
+            SolveStats ss = base_.solve_warm(d_ext_, rtol, atol, left, bbv[sl]);
+            hipLaunchKernelGGL(k_b_put_col, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0, stream_, n,
+                               (size_t)Rout_, r, base_.device_solution(), d_sol_);
+            FL_HIP(hipGetLastError());
+            h_status_[r] = ss.converged ? kConverged : kMaxIt;
+            h_it_[r] = itv[sl] + ss.iterations;
+            h_ru_[r] = ruv[sl];
+            h_rr_[r] = ss.residual * ss.residual;
+            h_bb_[r] = bbv[sl];
+            h_tol_[r] = tolv[sl];
+            rid_[sl] = -1;
+            hipLaunchKernelGGL(k_b_status_fill, dim3(1), dim3(64), 0, stream_, sc_.status + sl, 1, (int)kIdle);
+            st.handovers++;
+            st.handover_iterations += ss.iterations;
+        }
+        FL_HIP(hipStreamSynchronize(stream_));
+    };
+    // move live slots to the front when that frees whole 64-slot chunks
+    auto live = [&](int s) { return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64); };
+    auto maybe_compact = [&](int phase) {
+        if (nch_ <= 1) return;
+        std::vector<int> src;
+        src.reserve(Rp_);
+        for (int s = 0; s < Rp_; s++)
+            if (live(s)) src.push_back(s);
+        const int nn = std::max(1, ((int)src.size() + 63) / 64);
+        if (nn >= nch_) return;
+        for (int s = 0; s < Rp_; s++)
+            if (!live(s)) src.push_back(s);
+        permute_slots(src, phase);
+        nch_ = nn;
+        st.compactions++;
+        rebuild();
+        read_status();
+    };
+    // queue: hardest realizations first (largest ln K variance), so the
+    // longest solves start early and the final tail is short
+    std::vector<int> queue(R_);
+    {
+        std::vector<double> var(R_);
+        hipLaunchKernelGGL(k_b_logk_var, dim3(R_), blk, 0, stream_, (const double *)d_Kall_, E_, d_var_);
+        FL_HIP(hipMemcpyAsync(var.data(), d_var_, R_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+        std::iota(queue.begin(), queue.end(), 0);
+        std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return var[a] > var[b]; });
+    }
+    size_t next = 0;
+    rid_.assign(Rp_, -1);
+    hipLaunchKernelGGL(k_b_status_fill, dim3((Rp_ + 255) / 256), blk, 0, stream_, sc_.status, Rp_, (int)kIdle);
+    const int mixed = precision == 1 ? 1 : 0;
+    {
+        std::vector<int> pairs;
+        for (int sl = 0; sl < Rp_ && next < queue.size(); sl++) {
+            pairs.push_back(sl);
+            pairs.push_back(queue[next++]);
+        }
+        refill(pairs, mixed);
+    }
+    if (!tuned_) {
+        tune();  // times the operator on the filled slots; the CG state is unchanged
+        rebuild();
+    }
     // live kernel timing: the first iterations run eagerly between events
     const int kSample = 4;
     while ((int)ev_.size() < 4 * kSample) {
@@ -903,8 +1420,14 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         FL_HIP(hipEventCreate(&e));
         ev_.push_back(e);
     }
-    auto sample_times = [&]() {
+    for (int k = 0; k < kSample; k++) {
+        if (mixed) iter32(&ev_[4 * k]);
+        else iter64(&ev_[4 * k]);
+    }
+    if (mixed) reliable_update();
+    {
         double so = 0, su = 0;
+        FL_HIP(hipStreamSynchronize(stream_));
         for (int k = 0; k < kSample; k++) {
             float t1 = 0, t2 = 0;
             FL_HIP(hipEventElapsedTime(&t1, ev_[4 * k], ev_[4 * k + 1]));
@@ -915,145 +1438,125 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         st.ms_op_avg = so / kSample;
         st.ms_update_avg = su / kSample;
         st.op_samples = kSample;
-    };
-    int run = 0, f64 = 0;
-    long launched = 0;
-    if (precision == 0) {
-        hipLaunchKernelGGL(k_b_init<double>, grow, blk, 0, stream_, a64, (const double *)d_b_, (double *)nullptr);
-        fold(kFoldInit, 0);
-        for (int k = 0; k < kSample; k++) iter64(&ev_[4 * k]);
-        launched = kSample;
-        counts(run, f64);
-        sample_times();
-        if (g64_key_ != max_it) {  // max_it is baked into the captured folds
-            if (g64_) (void)hipGraphExecDestroy(g64_);
-            g64_ = nullptr;
-            hipGraph_t gr;
-            FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < chunk_; k++) iter64(nullptr);
-            FL_HIP(hipStreamEndCapture(stream_, &gr));
-            FL_HIP(hipGraphInstantiate(&g64_, gr, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(gr);
-            g64_key_ = max_it;
+    }
+    // main loop: graph chunks; harvest finished slots, refill them from the
+    // queue, compact once the queue is empty
+    auto terminal = [&](int v) { return v == kConverged || v == kMaxIt || v == kBreakdown; };
+    long guard = 0;
+    const long guard_max = ((long)max_it + chunk_) * ((R_ + Rp_ - 1) / Rp_ + 2) / chunk_ + 16;
+    while (true) {
+        read_status();
+        std::vector<int> done;
+        int nrun = 0, nf64 = 0;
+        for (int sl = 0; sl < Rp_; sl++) {
+            if (rid_[sl] < 0) continue;
+            if (terminal(stv[sl])) {
+                done.push_back(sl);
+                done.push_back(rid_[sl]);
+            } else if (stv[sl] == kFp64) {
+                nf64++;
+            } else {
+                nrun++;
+            }
         }
-        while (run > 0 && launched < (long)max_it + chunk_) {
-            FL_HIP(hipGraphLaunch(g64_, stream_));
-            launched += chunk_;
-            st.chunks++;
-            counts(run, f64);
+        if (!done.empty()) {
+            harvest(done, mixed);
+            std::vector<int> pairs;
+            for (size_t k = 0; k < done.size() && next < queue.size(); k += 2) {
+                pairs.push_back(done[k]);
+                pairs.push_back(queue[next++]);
+            }
+            if (!pairs.empty()) {
+                // refilled slots may lie beyond the live chunks after a compaction
+                int hi = 0;
+                for (size_t k = 0; k < pairs.size(); k += 2) hi = std::max(hi, pairs[k] / 64 + 1);
+                if (hi > nch_) {
+                    nch_ = hi;
+                    rebuild();
+                }
+                refill(pairs, mixed);
+                nrun += (int)pairs.size() / 2;
+            }
+            read_status();
         }
-    } else {
-        hipLaunchKernelGGL(k_b_init<float>, grow, blk, 0, stream_, a32, (const double *)d_b_, d_x_);
-        fold(kFoldInit, 1);
-        for (int k = 0; k < kSample; k++) iter32(&ev_[4 * k]);
-        launched = kSample;
-        reliable_update();
-        st.reliable_updates++;
-        counts(run, f64);
-        sample_times();
-        if (g32_key_ != max_it) {
-            if (g32_) (void)hipGraphExecDestroy(g32_);
-            g32_ = nullptr;
-            hipGraph_t gr;
-            FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < chunk_; k++) iter32(nullptr);
-            reliable_update();
-            FL_HIP(hipStreamEndCapture(stream_, &gr));
-            FL_HIP(hipGraphInstantiate(&g32_, gr, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(gr);
-            g32_key_ = max_it;
+        if (nrun == 0) break;
+        if (next >= queue.size() && nrun + nf64 <= handover_) {
+            std::vector<int> pairs;
+            for (int sl = 0; sl < Rp_; sl++)
+                if (rid_[sl] >= 0 && !terminal(stv[sl])) {
+                    pairs.push_back(sl);
+                    pairs.push_back(rid_[sl]);
+                }
+            handover(pairs, mixed);
+            break;
         }
-        while (run > 0 && launched < (long)max_it + chunk_) {
-            FL_HIP(hipGraphLaunch(g32_, stream_));
-            launched += chunk_;
-            st.chunks++;
-            st.reliable_updates++;
-            counts(run, f64);
-        }
-        // corrections of realizations stopped at the iteration limit
-        hipLaunchKernelGGL(k_b_accum, grow, blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, 1, d_x_, d_x32_);
-        if (f64 > 0) {
-            // FP64 finish for the realizations whose FP32 residual stagnated
-            st.fp64_finish = f64;
-            hipLaunchKernelGGL((k_b_resid<double, 1>), grow, blk, 0, stream_, a64, 1 << kFp64,
+        if (next >= queue.size() && nrun + nf64 <= 64 * (nch_ - 1)) maybe_compact(mixed);
+        if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
+        FL_HIP(hipGraphLaunch(graph(mixed), stream_));
+        st.chunks++;
+    }
+    if (mixed) {
+        // FP64 finish for the realizations whose FP32 residual stagnated
+        std::vector<int> f64slots;
+        for (int sl = 0; sl < Rp_; sl++)
+            if (rid_[sl] >= 0 && stv[sl] == kFp64) f64slots.push_back(sl);
+        if (!f64slots.empty()) {
+            st.fp64_finish = (int)f64slots.size();
+            std::vector<int> pairs;
+            for (int sl : f64slots) {
+                pairs.push_back(sl);
+                pairs.push_back(rid_[sl]);
+            }
+            set_pairs(pairs);
+            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
+                               d_x_, d_x32_);
+            read_status();
+            maybe_compact(0);
+            hipLaunchKernelGGL((k_b_resid<double, 1>), grow(), blk, 0, stream_, a64, 1 << kFp64, nosel,
                                (const int *)d_op_off_, (const double *)d_tab_, (const double *)d_K_,
                                (const double *)d_b_, (const double *)d_x_);
-            fold(kFoldWarm, 0);
-            counts(run, f64);
-            if (g64_key_ != max_it) {  // max_it is baked into the captured folds
-                if (g64_) (void)hipGraphExecDestroy(g64_);
-                g64_ = nullptr;
-                hipGraph_t gr;
-                FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < chunk_; k++) iter64(nullptr);
-                FL_HIP(hipStreamEndCapture(stream_, &gr));
-                FL_HIP(hipGraphInstantiate(&g64_, gr, nullptr, nullptr, 0));
-                (void)hipGraphDestroy(gr);
-                g64_key_ = max_it;
-            }
-            long l64 = 0;
-            while (run > 0 && l64 < (long)max_it + chunk_) {
-                FL_HIP(hipGraphLaunch(g64_, stream_));
-                l64 += chunk_;
+            fold(kFoldWarm, 0, nosel);
+            while (true) {
+                read_status();
+                std::vector<int> done;
+                int nrun = 0;
+                for (int sl = 0; sl < Rp_; sl++) {
+                    if (rid_[sl] < 0) continue;
+                    if (terminal(stv[sl])) {
+                        done.push_back(sl);
+                        done.push_back(rid_[sl]);
+                    } else {
+                        nrun++;
+                    }
+                }
+                if (!done.empty()) harvest(done, 0);
+                if (nrun == 0) break;
+                if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
+                FL_HIP(hipGraphLaunch(graph(0), stream_));
                 st.chunks++;
-                counts(run, f64);
             }
         }
     }
-    // final true FP64 residual of every realization
-    hipLaunchKernelGGL((k_b_resid<double, 2>), grow, blk, 0, stream_, a64, 0xff, (const int *)d_op_off_,
-                       (const double *)d_tab_, (const double *)d_K_, (const double *)d_b_, (const double *)d_x_);
-    fold(kFoldFinal, 0);
-    FL_HIP(hipEventRecord(e2, stream_));
-    // reconstruction
-    {
-        double *lam = d_sol_ + ((size_t)S_ + E_) * Rp_;
-        dim3 gl(std::min((F_ + kWaves - 1) / kWaves, 4096), nchunk_);
-        hipLaunchKernelGGL(k_b_scatter_lambda, gl, blk, 0, stream_, F_, (size_t)Rp_, (const int *)d_perm_,
-                           (const double *)d_x_, lam);
-        for (int D = 1; D <= 3; D++) {
-            int n = dim_start_[D] - dim_start_[D - 1];
-            if (n <= 0) continue;
-            dim3 g((n + kWaves - 1) / kWaves, nchunk_);
-#define FL_RESOLVE(DD)                                                                                         \
-    hipLaunchKernelGGL(k_b_resolve<DD>, g, blk, 0, stream_, dim_start_[D - 1], dim_start_[D],                  \
-                       (const int *)d_eperm_, dv_.side_off, S_, (size_t)Rp_, (const int *)d_meta_,             \
-                       (const int *)d_rs_off_, (const double *)d_rsT_, (const int *)d_conn_off_,               \
-                       (const int *)d_conn_, (const double *)d_K_, (const double *)lam, d_sol_)
-            if (D == 1) FL_RESOLVE(1);
-            else if (D == 2) FL_RESOLVE(2);
-            else FL_RESOLVE(3);
-#undef FL_RESOLVE
-        }
-    }
-    FL_HIP(hipGetLastError());
-    FL_HIP(hipEventRecord(e3, stream_));
-    FL_HIP(hipEventSynchronize(e3));
-    float t01, t12, t23;
+    if (ksaved) base_.set_conductivity_device(d_Ksave_);
+    FL_HIP(hipStreamSynchronize(stream_));
+    float t01 = 0;
     FL_HIP(hipEventElapsedTime(&t01, e0, e1));
-    FL_HIP(hipEventElapsedTime(&t12, e1, e2));
-    FL_HIP(hipEventElapsedTime(&t23, e2, e3));
-    st.ms_setup = t01; st.ms_cg = t12; st.ms_resolve = t23;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.ms_setup = t01;
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
-    // per-realization results
-    std::vector<int> h_status(Rp_), h_it(Rp_);
-    std::vector<double> h_rr(Rp_), h_bb(Rp_), h_tol(Rp_);
-    FL_HIP(hipMemcpy(h_status.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost));
-    FL_HIP(hipMemcpy(h_it.data(), sc_.it, Rp_ * sizeof(int), hipMemcpyDeviceToHost));
-    FL_HIP(hipMemcpy(h_rr.data(), sc_.rr, Rp_ * sizeof(double), hipMemcpyDeviceToHost));
-    FL_HIP(hipMemcpy(h_bb.data(), sc_.bnorm2, Rp_ * sizeof(double), hipMemcpyDeviceToHost));
-    FL_HIP(hipMemcpy(h_tol.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost));
+    st.ms_cg = st.ms_total - st.ms_setup;
     st.iterations.resize(R_);
     st.converged.resize(R_);
     st.residual.resize(R_);
     for (int r = 0; r < R_; r++) {
-        st.iterations[r] = h_it[r];
-        st.total_iterations += h_it[r];
-        st.max_iterations = std::max(st.max_iterations, h_it[r]);
+        st.iterations[r] = h_it_[r];
+        st.total_iterations += h_it_[r];
+        st.max_iterations = std::max(st.max_iterations, h_it_[r]);
+        if (precision == 1) st.reliable_updates += h_ru_[r];
         // the stop test is re-checked on the final FP64 true residual
-        st.converged[r] = (h_status[r] == kConverged && h_rr[r] <= 4.0 * h_tol[r]) ? 1 : 0;
-        st.residual[r] = h_bb[r] > 0 ? std::sqrt(h_rr[r] / h_bb[r]) : std::sqrt(h_rr[r]);
+        st.converged[r] = (h_status_[r] == kConverged && h_rr_[r] <= 4.0 * h_tol_[r]) ? 1 : 0;
+        st.residual[r] = h_bb_[r] > 0 ? std::sqrt(h_rr_[r] / h_bb_[r]) : std::sqrt(h_rr_[r]);
     }
     st.grid = grid_;
     st.xcd_map = xcd_map_;
@@ -1062,32 +1565,28 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
 }
 
 // Time the operator kernel (idempotent: it rewrites q and the other search
-// direction buffer from z, beta and the current direction) over launch shapes
-// and keep the fastest.  Runs on a cold-started FP32 state.
+// direction buffer from z, beta and the current direction of the filled
+// slots) over launch shapes and keep the fastest.
 void DarcyBatchGPU::autotune() {
     if (!have_K_) throw Error(-4, "batch conductivities not set");
-    setup_system();
-    tune();
+    tuned_ = false;
+    (void)solve(1e-30, 0.0, 1, 1);  // fills the slots, tunes, one iteration
 }
 
 void DarcyBatchGPU::tune() {
     alloc32();
-    const int cand_g[] = {256, 512, 1024, 2048};
+    const int cand_g[] = {512, 1024, 2048};
     float best = 1e30f;
     int bg = grid_, bx = xcd_map_;
     hipEvent_t t0, t1;
     FL_HIP(hipEventCreate(&t0));
     FL_HIP(hipEventCreate(&t1));
-    for (int x = 0; x <= 1; x++)
+    for (int x = 0; x <= 2; x++)
         for (int G : cand_g) {
-            BArgs<float> a = make_args<float>(F_, Rp_, G, x, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_,
-                                              d_conn_, d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_, d_x32_, d_r32_, d_z32_, d_p32a_,
-                                              d_p32b_, d_q32_, d_part_, sc_);
-            const dim3 grow(G, nchunk_), blk(256);
-            hipLaunchKernelGGL(k_b_init<float>, grow, blk, 0, stream_, a, (const double *)d_b_, d_x_);
-            hipLaunchKernelGGL(k_b_fold<kFoldInit>, dim3(nchunk_), dim3(kFoldThreads), 0, stream_, sc_,
-                               (const double *)a.part0, (const double *)a.part1, (const double *)a.part2, G, Rp_, R_,
-                               0.0, 0.0, 1, 1);
+            BArgs<float> a = make_args<float>(F_, Rp_, G, x, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_,
+                                              d_conn_off_, d_conn_, d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_,
+                                              d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_, d_q32_, d_part_, sc_);
+            const dim3 grow(G, nch_), blk(256);
             hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);  // warm
             FL_HIP(hipEventRecord(t0, stream_));
             for (int k = 0; k < 3; k++) hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);
@@ -1108,14 +1607,9 @@ void DarcyBatchGPU::tune() {
 }
 
 void DarcyBatchGPU::diagnostics(int *status, int *n_ru) const {
-    std::vector<int> h(Rp_);
-    if (status) {
-        FL_HIP(hipMemcpy(h.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost));
-        std::copy(h.begin(), h.begin() + R_, status);
-    }
-    if (n_ru) {
-        FL_HIP(hipMemcpy(h.data(), sc_.n_ru, Rp_ * sizeof(int), hipMemcpyDeviceToHost));
-        std::copy(h.begin(), h.begin() + R_, n_ru);
+    for (int r = 0; r < R_; r++) {
+        if (status) status[r] = r < (int)h_status_.size() ? h_status_[r] : kIdle;
+        if (n_ru) n_ru[r] = r < (int)h_ru_.size() ? h_ru_[r] : 0;
     }
 }
 
@@ -1123,7 +1617,7 @@ void DarcyBatchGPU::get_solution(int r, double *out) const {
     if (r < 0 || r >= R_) throw Error(-1, "realization index out of range");
     const size_t n = (size_t)S_ + E_ + F_;
     hipLaunchKernelGGL(k_b_extract, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0, stream_, n,
-                       (size_t)Rp_, r, (const double *)d_sol_, d_ext_);
+                       (size_t)Rout_, r, (const double *)d_sol_, d_ext_);
     FL_HIP(hipGetLastError());
     FL_HIP(hipMemcpyAsync(out, d_ext_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     FL_HIP(hipStreamSynchronize(stream_));

@@ -231,12 +231,16 @@ int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *r
 }
 
 int fl_darcy_batch_create(fl_darcy *base, int n_real, fl_darcy_batch **out) {
+    return fl_darcy_batch_create_slots(base, n_real, 0, out);
+}
+
+int fl_darcy_batch_create_slots(fl_darcy *base, int n_real, int n_slots, fl_darcy_batch **out) {
     return guard([&] {
         check_device();
         if (!base || !out) throw fl::Error(-1, "null handle");
         auto *b = new fl_darcy_batch();
         try {
-            b->g.reset(new fl::DarcyBatchGPU(*base->g, n_real));
+            b->g.reset(new fl::DarcyBatchGPU(*base->g, n_real, n_slots));
         } catch (...) {
             delete b;
             throw;
@@ -288,6 +292,9 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->grid = st.grid;
             info->xcd_map = st.xcd_map;
             info->chunk = st.chunk;
+            info->compactions = st.compactions;
+            info->handovers = st.handovers;
+            info->handover_iterations = st.handover_iterations;
         }
         return 0;
     });
@@ -315,6 +322,13 @@ const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded) {
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk) {
     return guard([&] {
         b->g->set_launch(grid, xcd_map, chunk);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n) {
+    return guard([&] {
+        b->g->set_handover(n);
         return 0;
     });
 }

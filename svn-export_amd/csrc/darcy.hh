@@ -69,6 +69,11 @@ class DarcyGPU {
     void set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
                   const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma);
     void set_conductivity(const double *K);  // host array [E]
+    void set_conductivity_device(const double *dK);  // device array [E]
+    const double *device_K() const { return d_K_; }
+    // FP64 CG warm-started from the edge traces x0 (device, edge order); bnorm2
+    // = ||b||^2 of the edge system (the stop test is relative to it)
+    SolveStats solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2);
     // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates
     SolveStats solve(double rtol, double atol, int max_it, int precision = 0);
     void get_solution(double *out) const;                   // [S+E+F]

@@ -1,6 +1,7 @@
 // GPU steady MH Darcy solve.  See darcy.hh for the method and the reference
 // functions each phase replaces.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -111,6 +112,11 @@ __global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double
         dv.sol[so + i] = L.Ag[i] + pe * L.a[i] - t;
     }
     dv.sol[dv.S + e] = pe;
+}
+
+// initial guess in edge order into the permuted CG vector
+__global__ void k_gather_x0(const DarcyDev dv, const double *__restrict__ x0, double *__restrict__ x) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x) x[f] = x0[dv.rperm[f]];
 }
 
 // edge traces from the permuted CG solution into the [q | p | lambda] vector
@@ -405,6 +411,25 @@ void DarcyGPU::set_data(const double *K, const double *cs, const double *aniso, 
 void DarcyGPU::set_conductivity(const double *K) {
     FL_HIP(hipMemcpyAsync(d_K_, K, m_.n_elements() * sizeof(double), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void DarcyGPU::set_conductivity_device(const double *dK) {
+    FL_HIP(hipMemcpyAsync(d_K_, dK, m_.n_elements() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+SolveStats DarcyGPU::solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2) {
+    if (!have_data_) throw Error(-4, "Darcy data not set");
+    SolveStats st;
+    auto t0 = std::chrono::steady_clock::now();
+    assemble();
+    hipLaunchKernelGGL(k_gather_x0, dim3(std::min(grid_for(dv_.F), 8192)), dim3(kBlock), 0, stream_, dv_, d_x0, d_x_);
+    FL_HIP(hipGetLastError());
+    cg(rtol, atol, max_it, st, true, bnorm2);
+    resolve();
+    FL_HIP(hipStreamSynchronize(stream_));
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
 }
 
 void DarcyGPU::assemble() {

@@ -43,7 +43,8 @@ class BatchInfo(C.Structure):
                 ("max_iterations", C.c_int), ("n_converged", C.c_int), ("total_iterations", C.c_longlong),
                 ("ms_setup", C.c_double), ("ms_solve", C.c_double), ("ms_resolve", C.c_double),
                 ("ms_total", C.c_double), ("ms_op_avg", C.c_double), ("ms_update_avg", C.c_double),
-                ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int)]
+                ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int), ("compactions", C.c_int),
+                ("handovers", C.c_int), ("handover_iterations", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +74,7 @@ SIGNATURES = {
     "fl_darcy_edge_nnz": (C.c_int, [C.c_void_p]),
     "fl_darcy_edge_system": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p, _f64p]),
     "fl_darcy_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fl_darcy_batch_create_slots": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "fl_darcy_batch_free": (None, [C.c_void_p]),
     "fl_darcy_batch_set_conductivity": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "fl_darcy_batch_update_solution": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(BatchInfo), _i32p,
@@ -82,6 +84,7 @@ SIGNATURES = {
     "fl_darcy_batch_device_solution": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int)]),
     "fl_darcy_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fl_darcy_batch_autotune": (C.c_int, [C.c_void_p]),
+    "fl_darcy_batch_set_handover": (C.c_int, [C.c_void_p, C.c_int]),
     "fl_transport_create": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                       C.POINTER(C.c_void_p)]),
     "fl_transport_free": (None, [C.c_void_p]),
@@ -263,11 +266,11 @@ class DarcyBatch:
 
     Shares mesh, cross-sections, sources and boundary data with `darcy` (set_data must have been called)."""
 
-    def __init__(self, darcy, n_real):
+    def __init__(self, darcy, n_real, slots=0):
         self.darcy = darcy
         self.n_real = int(n_real)
         h = C.c_void_p()
-        _check(lib().fl_darcy_batch_create(darcy.h, self.n_real, C.byref(h)))
+        _check(lib().fl_darcy_batch_create_slots(darcy.h, self.n_real, int(slots), C.byref(h)))
         self.h = h
         self.S, self.E, self.F = darcy.S, darcy.E, darcy.F
 
@@ -310,6 +313,9 @@ class DarcyBatch:
 
     def autotune(self):
         _check(lib().fl_darcy_batch_autotune(self.h))
+
+    def set_handover(self, n):
+        _check(lib().fl_darcy_batch_set_handover(self.h, int(n)))
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:

@@ -79,6 +79,7 @@ def test_batch_matches_single_solves(cube6):
                                                                   info["xcd_map"], info["chunk"]), flush=True)
     assert info["converged"].all() and info["n_converged"] == R
     assert it.max() > it.min()
+    assert info["compactions"] >= 1  # 70 realizations in two 64-slot chunks: the tail runs in one
     assert (info["residual"] <= 2e-13).all()
     S, E = om.n_sides, om.n_elements
     for r in (0, 31, 63, 64, 69):
@@ -89,6 +90,32 @@ def test_batch_matches_single_solves(cube6):
         errs = [rel(a, c) for a, c in zip(parts(x1, S, E), parts(x0, S, E))]
         print("  r=%d rel err flux %.2e pressure %.2e trace %.2e" % (r, *errs), flush=True)
         assert max(errs) <= 1e-9
+    g.set_conductivity(d["K"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_slot_refill(cube6, precision):
+    """100 realizations through 64 slots: converged slots are refilled from
+    the queue (hardest first); every realization matches its single solve."""
+    om, gm, d, g = cube6
+    R = 100
+    K = cases.mc_conductivity(om, R, seed=8)
+    b = G.DarcyBatch(g, R, slots=64)
+    b.set_conductivity(K)
+    info = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=precision)
+    it = info["iterations"]
+    print("refill precision %d: iterations min %d max %d total %d" % (precision, it.min(), it.max(),
+                                                                      info["total_iterations"]), flush=True)
+    assert info["converged"].all() and (it > 0).all()
+    S, E = om.n_sides, om.n_elements
+    for r in (0, 63, 64, 99):
+        g.set_conductivity(K[r])
+        g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=100000)
+        x0 = g.get_solution_vector()
+        x1 = b.get_solution_vector(r)
+        errs = [rel(a, c) for a, c in zip(parts(x1, S, E), parts(x0, S, E))]
+        print("  r=%d rel err flux %.2e pressure %.2e trace %.2e" % (r, *errs), flush=True)
+        assert max(errs) <= 1e-8
     g.set_conductivity(d["K"])
 
 
@@ -136,7 +163,11 @@ def test_batch_launch_shapes_and_repeat(cube6):
     assert np.array_equal(a, a2)
     b.set_launch(256, 1, 7)
     info = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    assert info["converged"].all()
     assert info["grid"] == 256 and info["xcd_map"] == 1 and info["chunk"] == 7
+    b.set_launch(1024, 2, 16)
+    c2 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    assert c2["xcd_map"] == 2 and c2["converged"].all()
     c = np.stack([b.get_solution_vector(r) for r in (0, 40)])
     assert rel(c, a) < 1e-11
     Kd = torch.tensor(K, dtype=torch.float64, device="cuda")
@@ -158,4 +189,34 @@ def test_batch_autotune(cube6):
     b.autotune()
     info = b.update_solution(r_tol=1e-12, a_tol=0.0, max_it=100000, precision=1)
     print("autotuned grid %d xcd %d" % (info["grid"], info["xcd_map"]), flush=True)
-    assert info["grid"] in (256, 512, 1024, 2048) and info["converged"].all()
+    assert info["grid"] in (512, 1024, 2048) and info["xcd_map"] in (0, 1, 2) and info["converged"].all()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_handover(cube6, precision):
+    """The last live realizations are finished by the single-solve FP64 path
+    (warm start from the batch iterate): same answers as the batch alone."""
+    om, gm, d, g = cube6
+    R = 64
+    K = cases.mc_conductivity(om, R, seed=9)
+    b = G.DarcyBatch(g, R, slots=64)
+    b.set_conductivity(K)
+    b.set_handover(0)
+    i0 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=precision)
+    x0 = [b.get_solution_vector(r) for r in (0, 30, 63)]
+    b.set_handover(48)
+    i1 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=precision)
+    x1 = [b.get_solution_vector(r) for r in (0, 30, 63)]
+    print("handovers %d (%d iterations) vs none; totals %d / %d" % (
+        i1["handovers"], i1["handover_iterations"], i1["total_iterations"], i0["total_iterations"]), flush=True)
+    assert i0["handovers"] == 0 and i1["handovers"] >= 1
+    assert i1["converged"].all() and i0["converged"].all()
+    S = om.n_sides
+    for a, c in zip(x1, x0):
+        assert rel(a[:S], c[:S]) <= 1e-8
+    # the base solver's own conductivity is restored
+    g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=100000)
+    xs = g.get_solution_vector()
+    D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    D0.solve(0, rtol=1e-15, max_it=200000)
+    assert rel(xs, D0.solution) <= 1e-9
