@@ -169,6 +169,7 @@ typedef struct {
     int compactions;                 /* times live realizations were packed into fewer 64-wide chunks */
     int handovers;                   /* realizations finished by the single-solve FP64 path */
     long long handover_iterations;   /* CG iterations they took there */
+    double ms_handover;              /* wall time of that phase */
 } fl_batch_info;
 
 /* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as
@@ -192,6 +193,51 @@ const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
  * autotuner that picks grid and mapping by timing the operator kernel. */
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
+/* Two-level preconditioner of the batched CG: ncol 0 = Jacobi only, 1 = plus
+ * one constant coarse column per aggregate, 2 = plus a second column phi per
+ * aggregate (fl_darcy_batch_set_coarse_basis; e.g. the learned basis of
+ * fl_gnn_*).  M^-1 = D^-1 + Z (Z^T S Z)^-1 Z^T, applied in flexible CG. */
+int fl_darcy_batch_set_coarse(fl_darcy_batch *b, int ncol);
+/* phi[n_real][F] (FP32, device pointer, edge order); normalised per aggregate. */
+int fl_darcy_batch_set_coarse_basis(fl_darcy_batch *b, const float *phi);
+/* Aggregate of every edge (edge order) into edge_agg[F] (may be NULL);
+ * returns the number of aggregates. */
+int fl_darcy_batch_aggregates(fl_darcy_batch *b, int *edge_agg);
+/* Edge traces lambda of every realization of the last solve into
+ * out[n_real][F] (device pointer, edge order). */
+int fl_darcy_batch_lambda(fl_darcy_batch *b, double *out);
+
+/* ---- learned coarse basis (graph network) --------------------------------- */
+
+/* A 3-layer message-passing network (hidden width 64) on the element-edge
+ * graph of the batch's mesh that maps a conductivity field to the second
+ * coarse column phi of fl_darcy_batch_set_coarse(b, 2).  No counterpart in
+ * the reference.  BF16 MFMA forward and backward kernels with FP32
+ * accumulation, FP32 master weights and Adam, all on the GPU.  Trained on
+ * (K, lambda) pairs from the solver: the loss is the share of lambda that the
+ * columns {1, phi} of each aggregate cannot represent. */
+typedef struct fl_gnn fl_gnn;
+int fl_gnn_create(fl_darcy_batch *b, unsigned seed, fl_gnn **out);
+void fl_gnn_free(fl_gnn *g);
+int fl_gnn_n_params(fl_gnn *g);
+int fl_gnn_get_params(fl_gnn *g, float *out); /* host arrays, n_params */
+int fl_gnn_set_params(fl_gnn *g, const float *in);
+/* one realization: K[E] (device FP64) -> phi[F] (device FP32, edge order; may be NULL) */
+int fl_gnn_forward(fl_gnn *g, const double *K, float *phi);
+/* loss of the last forward against lambda[F] (device FP64, edge order);
+ * the gradient is added to the accumulated gradient */
+int fl_gnn_loss_grad(fl_gnn *g, const double *lambda, double *loss);
+int fl_gnn_zero_grad(fl_gnn *g);
+int fl_gnn_get_grad(fl_gnn *g, float *out); /* host, n_params */
+int fl_gnn_adam_step(fl_gnn *g, double lr);
+/* Training on n pairs K[n][E], lambda[n][F] (device): `epochs` passes, an
+ * Adam step every `batch` samples; mean loss per epoch into epoch_loss. */
+int fl_gnn_train(fl_gnn *g, const double *K, const double *lambda, int n, int epochs, int batch, double lr,
+                 double *epoch_loss);
+/* phi[R][F] for K[R][E] (device arrays) */
+int fl_gnn_infer(fl_gnn *g, const double *K, int R, float *phi);
+/* inputs of the last forward (host FP32 [E][16], [F][16]; either may be NULL) */
+int fl_gnn_features(fl_gnn *g, float *xe, float *xf);
 /* Once the queue is empty and at most n realizations are still iterating,
  * finish them one by one with the single-solve FP64 CG (warm start from the
  * batch iterate); 0 disables.  Default 16. */

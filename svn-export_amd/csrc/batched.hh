@@ -38,6 +38,7 @@ struct BatchStats {
     int n_real = 0, chunks = 0, reliable_updates = 0, fp64_finish = 0, max_iterations = 0, compactions = 0;
     int handovers = 0;              // realizations finished by the single-solve path
     long handover_iterations = 0;   // their iterations there
+    double ms_handover = 0;         // wall time of the hand-over phase
     long total_iterations = 0;  // summed over realizations
     double ms_setup = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
     double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels
@@ -79,6 +80,9 @@ class DarcyBatchGPU {
     long table_bytes() const { return tab_bytes_; }
     long contributions() const { return ncontrib_; }
     hipStream_t stream() const { return stream_; }
+    DarcyGPU &base() { return base_; }
+    // edge traces lambda of every realization into out[R][F] (device, edge order)
+    void lambda_device(double *out) const;
     // launch shape: grid blocks per 64-realization chunk, XCD-aware row mapping,
     // iterations per captured chunk.  autotune() times the operator kernel
     // over candidate shapes and keeps the fastest.
@@ -86,6 +90,16 @@ class DarcyBatchGPU {
     void autotune();
     // live-realization count at which the rest go to the single-solve path (0: never)
     void set_handover(int n) { handover_ = n < 0 ? 0 : n; }
+    // Two-level preconditioner: ncol 0 off (Jacobi), 1 one constant column per
+    // aggregate, 2 constant + basis column phi per aggregate
+    void set_coarse(int ncol);
+    int coarse_columns() const { return ncol_; }
+    int aggregates() const { return nagg_; }
+    // phi [R][F] FP32 device array in edge order (normalised per aggregate on use)
+    void set_coarse_basis(const float *d_phi_edges);
+    // row order of the batched vectors (row -> edge), aggregate of each row
+    const std::vector<int> &row_edges() const { return h_perm_; }
+    const std::vector<int> &row_aggregate() const { return h_agg_; }
 
    private:
     void build();
@@ -127,6 +141,19 @@ class DarcyBatchGPU {
     double *d_Kall_ = nullptr, *d_var_ = nullptr;  // K [R][E] as given; difficulty proxy [R]
     double *d_Ksave_ = nullptr;                    // the base solver's K while it is borrowed
     int handover_ = 16;  // live realizations at or below which the single-solve path finishes them
+    // two-level preconditioner
+    void coarse_setup(const std::vector<int> &pairs);  // coarse inverses of the listed slots
+    int ncol_ = 0, nagg_ = 0, nc_ = 0, nbmax_ = 0;
+    bool coarse_ok_ = false;  // aggregation usable (neighbour counts within the kernel limit)
+    bool have_phi_ = false;
+    std::vector<int> h_perm_, h_agg_;
+    int *d_agg_ = nullptr, *d_aptr_ = nullptr, *d_nbptr_ = nullptr, *d_nb_ = nullptr, *d_lbmap_ = nullptr;
+    float *d_phiAll_ = nullptr, *d_phi_ = nullptr, *d_ainv_ = nullptr, *d_spphi_ = nullptr, *d_spainv_ = nullptr;
+    double *d_Asp_ = nullptr, *d_rc_ = nullptr, *d_qc_ = nullptr, *d_y_ = nullptr, *d_spy_ = nullptr;
+    double *d_part3_ = nullptr, *d_Apack_ = nullptr;
+    double **d_Aptrs_ = nullptr;
+    int *d_info_ = nullptr;
+    void *solver_ = nullptr;  // rocblas handle
     int *d_pairs_ = nullptr, *d_lsel_ = nullptr;   // pair list [2 * slots], slot flags [slots]
     int npairs_ = 0;
     // per-realization results of the last solve

@@ -10,6 +10,9 @@
 #include <cstdlib>
 #include <numeric>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "batched.hh"
 #include "common.hh"
 #include "condense.hh"
@@ -103,22 +106,56 @@ __device__ __forceinline__ void block_part(const double (&v)[NV], double *const 
     }
 }
 
+// Sources of the vector the operator is applied to, value at edge row j
+// (idx = j * Rp + slot):
+//   XPlain: x itself;
+//   XFly:   the new search direction z + beta p, formed on the fly;
+//   XTwoLevel: the same with z = D^-1 r + Z y formed on the fly from the
+//           residual, the Jacobi diagonal and the coarse correction y of the
+//           two-level preconditioner (row j's aggregate from a wave-uniform
+//           table; Z = [1, phi] per aggregate).
 template <class V>
-__device__ __forceinline__ V gather(const V *__restrict__ za, const V *__restrict__ pb, V beta, size_t idx,
-                                    bool fly) {
-    return fly ? za[idx] + beta * pb[idx] : za[idx];
-}
+struct XPlain {
+    const V *__restrict__ x;
+    __device__ __forceinline__ V operator()(int, size_t idx) const { return x[idx]; }
+};
+template <class V>
+struct XFly {
+    const V *__restrict__ z;
+    const V *__restrict__ p;
+    V beta;
+    __device__ __forceinline__ V operator()(int, size_t idx) const { return z[idx] + beta * p[idx]; }
+};
+template <class V>
+struct XTwoLevel {
+    const V *__restrict__ r;
+    const V *__restrict__ dinv;
+    const V *__restrict__ p;
+    const float *__restrict__ phi;  // [F][Rp] or null (constant column only)
+    const double *__restrict__ y;   // [nc][Rp]
+    const int *__restrict__ agg;    // row -> aggregate
+    V beta;
+    size_t Rp;
+    int slot;
+    __device__ __forceinline__ V z(int j, size_t idx) const {
+        const int a = agg[j];
+        if (phi) {
+            const double c = y[(size_t)(2 * a) * Rp + slot] + (double)phi[idx] * y[(size_t)(2 * a + 1) * Rp + slot];
+            return dinv[idx] * r[idx] + (V)c;
+        }
+        return dinv[idx] * r[idx] + (V)y[(size_t)a * Rp + slot];
+    }
+    __device__ __forceinline__ V operator()(int j, size_t idx) const { return z(j, idx) + beta * p[idx]; }
+};
 
-// Row i of S_e(K) x for an element without vb neighbours.  FLY: x = za + beta
-// pb (the new search direction, formed on the fly from z and the previous one).
-template <class V, int N0, bool FLY>
+// Row i of S_e(K) x for an element without vb neighbours.
+template <class V, int N0, class X>
 __device__ __forceinline__ V elem_row_plain(const V *__restrict__ T, V K, const int *__restrict__ cn, int i,
-                                            const V *__restrict__ za, const V *__restrict__ pb, V beta, size_t Rp,
-                                            int r) {
+                                            const X &xs, size_t Rp, int r) {
     V t = V(0), xi = V(0);
 #pragma unroll
     for (int j = 0; j < N0; j++) {
-        const V xj = gather(za, pb, beta, (size_t)cn[j] * Rp + r, FLY);
+        const V xj = xs(cn[j], (size_t)cn[j] * Rp + r);
         t += T[i * N0 + j] * xj;
         if (j == i) xi = xj;
     }
@@ -126,13 +163,12 @@ __device__ __forceinline__ V elem_row_plain(const V *__restrict__ T, V K, const 
 }
 
 // Row i of S_e(K) x for an element with vb neighbours, in FP64.
-template <class V, int N0, bool FLY>
+template <int N0, class X>
 __device__ __forceinline__ double elem_row_vb(const double *__restrict__ T, int nvb, double K,
-                                              const int *__restrict__ cn, int i, const V *__restrict__ za,
-                                              const V *__restrict__ pb, V beta, size_t Rp, int r) {
+                                              const int *__restrict__ cn, int i, const X &xs, size_t Rp, int r) {
     double xv[N0];
 #pragma unroll
-    for (int j = 0; j < N0; j++) xv[j] = (double)gather(za, pb, beta, (size_t)cn[j] * Rp + r, FLY);
+    for (int j = 0; j < N0; j++) xv[j] = (double)xs(cn[j], (size_t)cn[j] * Rp + r);
     const double *C0 = T, *w0 = T + N0 * N0, *dlt = w0 + N0, *sv = dlt + N0, *vv = sv + 2;
     double ux = 0.0;
 #pragma unroll
@@ -140,7 +176,7 @@ __device__ __forceinline__ double elem_row_vb(const double *__restrict__ T, int 
     ux *= K;
     double xi = 0.0;
     for (int k = 0; k < nvb; k++) {
-        const double xk = (double)gather(za, pb, beta, (size_t)cn[N0 + k] * Rp + r, FLY);
+        const double xk = (double)xs(cn[N0 + k], (size_t)cn[N0 + k] * Rp + r);
         ux += vv[k] * xk;
         if (N0 + k == i) xi = xk;
     }
@@ -166,44 +202,42 @@ __device__ __forceinline__ double elem_row_vb(const double *__restrict__ T, int 
 // load followed by independent gathers.
 constexpr int kRecInts = 16;
 
-template <class V, int N0, bool FLY>
+template <class V, int N0, class X>
 __device__ __forceinline__ V rec_contrib(const int *__restrict__ R, const V *__restrict__ opT,
-                                         const V *__restrict__ Kv, const V *__restrict__ za,
-                                         const V *__restrict__ pb, V beta, size_t Rp, int r) {
+                                         const V *__restrict__ Kv, const X &xs, size_t Rp, int r) {
     const int e = R[0], i = R[1];
     const V *T = opT + R[3];
     const V K = Kv[(size_t)e * Rp + r];
     V t = V(0), xi = V(0);
 #pragma unroll
     for (int j = 0; j < N0; j++) {
-        const V xj = gather(za, pb, beta, (size_t)R[4 + j] * Rp + r, FLY);
+        const V xj = xs(R[4 + j], (size_t)R[4 + j] * Rp + r);
         t += T[i * N0 + j] * xj;
         if (j == i) xi = xj;
     }
     return K * t + T[N0 * N0 + i] * xi;
 }
 
-template <class V, bool FLY>
+template <class V, class X>
 __device__ __forceinline__ V rec_any(const int *__restrict__ R, const V *__restrict__ opT, const V *__restrict__ Kv,
-                                     const V *__restrict__ za, const V *__restrict__ pb, V beta, size_t Rp, int r) {
+                                     const X &xs, size_t Rp, int r) {
     const int n0 = R[2];
-    if (n0 == 4) return rec_contrib<V, 4, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
-    if (n0 == 3) return rec_contrib<V, 3, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
-    return rec_contrib<V, 2, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
+    if (n0 == 4) return rec_contrib<V, 4>(R, opT, Kv, xs, Rp, r);
+    if (n0 == 3) return rec_contrib<V, 3>(R, opT, Kv, xs, Rp, r);
+    return rec_contrib<V, 2>(R, opT, Kv, xs, Rp, r);
 }
 
-template <class V, bool FLY>
+template <class V, class X>
 __device__ __forceinline__ V op_row(const int *__restrict__ rec, const int *__restrict__ row_ptr,
                                     const int *__restrict__ row_c, const int *__restrict__ meta,
                                     const int *__restrict__ op_off, const int *__restrict__ conn_off,
                                     const int *__restrict__ conn, const V *__restrict__ opT,
                                     const V *__restrict__ Kv, const double *__restrict__ opT64,
-                                    const double *__restrict__ K64, int row, const V *__restrict__ za,
-                                    const V *__restrict__ pb, V beta, size_t Rp, int r) {
+                                    const double *__restrict__ K64, int row, const X &xs, size_t Rp, int r) {
     const int *R = rec + (size_t)row * kRecInts;
     if (R[0] != -2) {
-        V y = rec_any<V, FLY>(R, opT, Kv, za, pb, beta, Rp, r);
-        if (R[8] >= 0) y += rec_any<V, FLY>(R + 8, opT, Kv, za, pb, beta, Rp, r);
+        V y = rec_any<V>(R, opT, Kv, xs, Rp, r);
+        if (R[8] >= 0) y += rec_any<V>(R + 8, opT, Kv, xs, Rp, r);
         return y;
     }
     V y = V(0);
@@ -216,16 +250,16 @@ __device__ __forceinline__ V op_row(const int *__restrict__ rec, const int *__re
         if (nvb == 0) {
             const V *T = opT + op_off[e];
             const V K = Kv[(size_t)e * Rp + r];
-            if (n0 == 4) y += elem_row_plain<V, 4, FLY>(T, K, cn, i, za, pb, beta, Rp, r);
-            else if (n0 == 3) y += elem_row_plain<V, 3, FLY>(T, K, cn, i, za, pb, beta, Rp, r);
-            else y += elem_row_plain<V, 2, FLY>(T, K, cn, i, za, pb, beta, Rp, r);
+            if (n0 == 4) y += elem_row_plain<V, 4>(T, K, cn, i, xs, Rp, r);
+            else if (n0 == 3) y += elem_row_plain<V, 3>(T, K, cn, i, xs, Rp, r);
+            else y += elem_row_plain<V, 2>(T, K, cn, i, xs, Rp, r);
         } else {
             const double *T = opT64 + op_off[e];
             const double K = K64[(size_t)e * Rp + r];
             double t;
-            if (n0 == 4) t = elem_row_vb<V, 4, FLY>(T, nvb, K, cn, i, za, pb, beta, Rp, r);
-            else if (n0 == 3) t = elem_row_vb<V, 3, FLY>(T, nvb, K, cn, i, za, pb, beta, Rp, r);
-            else t = elem_row_vb<V, 2, FLY>(T, nvb, K, cn, i, za, pb, beta, Rp, r);
+            if (n0 == 4) t = elem_row_vb<4>(T, nvb, K, cn, i, xs, Rp, r);
+            else if (n0 == 3) t = elem_row_vb<3>(T, nvb, K, cn, i, xs, Rp, r);
+            else t = elem_row_vb<2>(T, nvb, K, cn, i, xs, Rp, r);
             y += (V)t;
         }
     }
@@ -317,6 +351,27 @@ __global__ void k_b_slot_K(const double *__restrict__ K, int E, int Sp, const in
             K64[(size_t)e * Sp + slot] = v;
             K32[(size_t)e * Sp + slot] = (float)v;
         }
+    }
+}
+
+// Per-realization FP32 rows [real][n] into slot columns dst[n][slot] for the
+// listed (slot, real) pairs (LDS transpose, 64 entries x 64 pairs per block)
+__global__ void k_b_slot_col(const float *__restrict__ src, int n, int Sp, const int *__restrict__ pairs, int npairs,
+                             float *__restrict__ dst) {
+    __shared__ float t[64][65];
+    const int e0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = wid; j < 64; j += 4) {
+        const int pi = p0 + j, e = e0 + lane;
+        t[j][lane] = (pi < npairs && e < n) ? src[(size_t)pairs[2 * pi + 1] * n + e] : 0.0f;
+    }
+    __syncthreads();
+    const int pi = p0 + lane;
+    if (pi >= npairs) return;
+    const int slot = pairs[2 * pi];
+    for (int j = wid; j < 64; j += 4) {
+        const int e = e0 + j;
+        if (e < n) dst[(size_t)e * Sp + slot] = t[lane][j];
     }
 }
 
@@ -449,8 +504,9 @@ __global__ void __launch_bounds__(256) k_b_op(BArgs<V> a) {
         row_range(a.F, a.G, a.xcd, s0, st, en);
         for (int row = s0; row < en; row += st) {
             if (act) {
-                const V q = op_row<V, true>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT,
-                                            a.K, a.opT64, a.K64, row, a.z, pin, beta, a.Rp, r);
+                const XFly<V> xs{a.z, pin, beta};
+                const V q = op_row<V>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
+                                      a.opT64, a.K64, row, xs, a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
                 const V p = a.z[o] + beta * pin[o];
                 pout[o] = p;
@@ -541,9 +597,9 @@ __global__ void __launch_bounds__(256) k_b_resid(BArgs<V> a, int sel, const int 
         row_range(a.F, a.G, a.xcd, s0, st, en);
         for (int row = s0; row < en; row += st) {
             if (act) {
-                const double ax = op_row<double, false>(a.rec, a.row_ptr, a.row_c, a.meta, op_off64, a.conn_off,
-                                                        a.conn, opT64, K64, opT64, K64, row, x64, nullptr, 0.0,
-                                                        a.Rp, r);
+                const XPlain<double> xs{x64};
+                const double ax = op_row<double>(a.rec, a.row_ptr, a.row_c, a.meta, op_off64, a.conn_off, a.conn,
+                                                 opT64, K64, opT64, K64, row, xs, a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
                 const double rr = b64[o] - ax;
                 v[1] += rr * rr;
@@ -756,9 +812,417 @@ __global__ void k_b_put_col(size_t n, size_t Rout, int real, const double *__res
         sol[i * Rout + real] = src[i];
 }
 
+__global__ void k_b_lambda_out(int R, size_t Rout, int F, size_t off, const double *__restrict__ sol,
+                               double *__restrict__ out) {
+    const int r = blockIdx.y;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x)
+        out[(size_t)r * F + f] = sol[(off + f) * Rout + r];
+}
+
 __global__ void k_b_extract(size_t n, size_t Rp, int r, const double *__restrict__ sol, double *__restrict__ out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = sol[i * Rp + r];
+}
+
+// ---------------------------------------------------------------------------
+// Two-level preconditioner: M^-1 r = D^-1 r + Z (Z^T S Z)^-1 Z^T r with Z one
+// (constant) or two (constant, phi) columns per aggregate.  Aggregates are
+// contiguous row ranges (cells of the row ordering's Z-curve), so the
+// restriction Z^T r is a block reduction of the update kernel, and z is never
+// stored: the operator forms z = D^-1 r + Z y on the fly from r and the
+// coarse correction y.  The conjugate-gradient step uses the flexible
+// (Polak-Ribiere) beta = z_new . (r_new - r_old) / z_old . r_old, which stays
+// robust when the coarse inverse is applied in FP32.
+// ---------------------------------------------------------------------------
+constexpr int kNbMax = 32;  // neighbour aggregates per aggregate (27 on a regular Z-curve level)
+
+struct CArgs {
+    int ncol, nagg, nc;
+    const int *agg, *aptr, *nbptr, *nb, *lbmap;  // lbmap[a * nagg + b]: local index of b among a's neighbours
+    const float *phi;                            // [F][Rp] (ncol 2) or null
+    const float *ainv;                           // [Rp][nc][nc]
+    double *rc, *qc, *y;                         // [nc][Rp]
+    double *part3;                               // [nagg][Rp]
+};
+
+// entry (i, j) of the condensed element block S_e(K) (FP64 operator table)
+__device__ __forceinline__ double elem_entry(const double *__restrict__ T, int n0, int nvb, double K, int i, int j) {
+    if (nvb == 0) return K * T[i * n0 + j] + (i == j ? T[n0 * n0 + i] : 0.0);
+    const double *C0 = T, *w0 = T + n0 * n0, *dlt = w0 + n0, *sv = dlt + n0, *vv = sv + 2;
+    const double ui = i < n0 ? K * w0[i] : vv[i - n0];
+    const double uj = j < n0 ? K * w0[j] : vv[j - n0];
+    double v = 0.0;
+    if (i < n0 && j < n0) v = K * C0[i * n0 + j] + (i == j ? dlt[i] : 0.0);
+    else if (i == j) v = vv[i - n0];
+    return v - ui * uj / (K * sv[0] + sv[1]);
+}
+
+// Coarse operator Z^T S Z of the selected slots, as per-aggregate blocks
+// Asp[((a * kNbMax + lb) * ncol^2 + s * ncol + t) * Rp + slot] (lb: local
+// index of the neighbour aggregate).  One wave per (aggregate, 64 slots);
+// each wave owns its accumulators, so the sums are in a fixed order.
+template <int NCOL>
+__global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, const int *__restrict__ lsel,
+                                                   double *__restrict__ Asp) {
+    __shared__ double acc[kNbMax][NCOL * NCOL][64];
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.y * 64 + lane;
+    const int ag = blockIdx.x;
+    const bool act = lsel[slot] != 0;
+    const int nb0 = c.nbptr[ag], nnb = c.nbptr[ag + 1] - nb0;
+    for (int k = 0; k < nnb; k++)
+        for (int q = 0; q < NCOL * NCOL; q++) acc[k][q][lane] = 0.0;
+    if (__any(act)) {
+        for (int row = c.aptr[ag]; row < c.aptr[ag + 1]; row++) {
+            if (!act) continue;
+            const double zf1 = NCOL == 2 ? (double)c.phi[(size_t)row * a.Rp + slot] : 0.0;
+            for (int cc = a.row_ptr[row]; cc < a.row_ptr[row + 1]; cc++) {
+                const int code = a.row_c[cc];
+                const int e = code >> 4, i = code & 15;
+                const int mt = a.meta[e];
+                const int n0 = mt & 15, nvb = (mt >> 4) & 255;
+                const double *T = a.opT64 + a.op_off[e];
+                const double K = a.K64[(size_t)e * a.Rp + slot];
+                const int *cn = a.conn + a.conn_off[e];
+                for (int j = 0; j < n0 + nvb; j++) {
+                    const int g = cn[j];
+                    const int lb = c.lbmap[(size_t)ag * c.nagg + c.agg[g]];
+                    const double sij = elem_entry(T, n0, nvb, K, i, j);
+                    if constexpr (NCOL == 1) {
+                        acc[lb][0][lane] += sij;
+                    } else {
+                        const double zg1 = (double)c.phi[(size_t)g * a.Rp + slot];
+                        acc[lb][0][lane] += sij;
+                        acc[lb][1][lane] += sij * zg1;
+                        acc[lb][2][lane] += zf1 * sij;
+                        acc[lb][3][lane] += zf1 * sij * zg1;
+                    }
+                }
+            }
+        }
+    }
+    if (!act) return;
+    for (int k = 0; k < nnb; k++)
+        for (int q = 0; q < NCOL * NCOL; q++)
+            Asp[((size_t)(ag * kNbMax + k) * (NCOL * NCOL) + q) * a.Rp + slot] = acc[k][q][lane];
+}
+
+// Dense coarse matrices of the listed slots, packed [np][nc][nc] for the
+// batched Cholesky; an empty coarse column (phi constant on an aggregate)
+// becomes a decoupled unit diagonal, which Z maps to zero.
+template <int NCOL>
+__global__ void k_c_pack(CArgs c, const double *__restrict__ Asp, size_t Rp, const int *__restrict__ pairs,
+                         double *__restrict__ Apack) {
+    const int i = blockIdx.x, k = blockIdx.y;
+    const int slot = pairs[2 * k];
+    const int ai = i / NCOL, s = i % NCOL;
+    double *row = Apack + ((size_t)k * c.nc + i) * c.nc;
+    for (int j = threadIdx.x; j < c.nc; j += blockDim.x) {
+        const int bj = j / NCOL, t = j % NCOL;
+        const int lb = c.lbmap[(size_t)ai * c.nagg + bj];
+        double v = lb >= 0 ? Asp[((size_t)(ai * kNbMax + lb) * (NCOL * NCOL) + s * NCOL + t) * Rp + slot] : 0.0;
+        if (i == j && !(v > 0.0)) v = 1.0;
+        row[j] = v;
+    }
+}
+
+// Inverse from the batched Cholesky (column-major lower triangle) to the full
+// symmetric FP32 matrix of each listed slot; a failed factorization leaves
+// that slot with the one-level (Jacobi) preconditioner.
+__global__ void k_c_unpack(int nc, const double *__restrict__ Apack, const int *__restrict__ info,
+                           const int *__restrict__ info2, const int *__restrict__ pairs, float *__restrict__ ainv) {
+    const int i = blockIdx.x, k = blockIdx.y;
+    const int slot = pairs[2 * k];
+    const double *A = Apack + (size_t)k * nc * nc;
+    float *out = ainv + ((size_t)slot * nc + i) * nc;
+    const bool ok = info[k] == 0 && info2[k] == 0;
+    for (int j = threadIdx.x; j < nc; j += blockDim.x)
+        out[j] = ok ? (float)(i >= j ? A[(size_t)j * nc + i] : A[(size_t)i * nc + j]) : 0.0f;
+}
+
+// phi of the selected slots: per aggregate zero mean and unit mean square
+// (keeps the two coarse columns independent and scaled alike)
+__global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, float *__restrict__ phi) {
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.y * 64 + lane;
+    const int ag = blockIdx.x;
+    if (!lsel[slot]) return;
+    const int r0 = c.aptr[ag], r1 = c.aptr[ag + 1];
+    double s1 = 0.0, s2 = 0.0;
+    for (int row = r0; row < r1; row++) {
+        const double v = phi[(size_t)row * Rp + slot];
+        s1 += v;
+        s2 += v * v;
+    }
+    const double n = (double)(r1 - r0);
+    const double mean = s1 / n, var = s2 / n - mean * mean;
+    const double sc = var > 1e-24 * (s2 / n + 1e-300) ? 1.0 / sqrt(var) : 0.0;
+    for (int row = r0; row < r1; row++) {
+        const size_t o = (size_t)row * Rp + slot;
+        phi[o] = (float)(((double)phi[o] - mean) * sc);
+    }
+}
+
+// Shared tail of the aggregate-blocked kernels: block reduction of three
+// partials and the 2 x ncol restrictions, written per aggregate.
+template <int NCOL>
+__device__ __forceinline__ void agg_part(const double (&v)[3], const double (&rc)[2], const double (&qc)[2],
+                                         double *p1, double *p2, double *p3, CArgs &c, size_t Rp, int slot) {
+    __shared__ double sh[3 + 2 * NCOL][kWaves][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 3; k++) sh[k][wid][lane] = v[k];
+    for (int s = 0; s < NCOL; s++) {
+        sh[3 + s][wid][lane] = rc[s];
+        sh[3 + NCOL + s][wid][lane] = qc[s];
+    }
+    __syncthreads();
+    if (wid == 0) {
+        double t[3 + 2 * NCOL];
+        for (int k = 0; k < 3 + 2 * NCOL; k++) {
+            double x = sh[k][0][lane];
+            for (int w = 1; w < kWaves; w++) x += sh[k][w][lane];
+            t[k] = x;
+        }
+        const size_t o = (size_t)blockIdx.x * Rp + slot;
+        p1[o] = t[0];
+        p2[o] = t[1];
+        p3[o] = t[2];
+        for (int s = 0; s < NCOL; s++) {
+            c.rc[(size_t)(blockIdx.x * NCOL + s) * Rp + slot] = t[3 + s];
+            c.qc[(size_t)(blockIdx.x * NCOL + s) * Rp + slot] = t[3 + NCOL + s];
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; partials D^-1 r.r, r.r, D^-1 r.q and the
+// restrictions Z^T r, Z^T q, one block per (aggregate, 64 slots)
+template <class V, int NCOL>
+__global__ void __launch_bounds__(256) k_c_update(BArgs<V> a, CArgs c) {
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.y * 64 + lane;
+    const bool act = a.sc.status[slot] == kRun;
+    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    if (__any(act)) {
+        const V alpha = act ? (V)a.sc.alpha[slot] : V(0);
+        const V *p = a.sc.par[slot] ? a.p0 : a.p1;  // the buffer the operator just wrote
+        const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
+            if (!act) continue;
+            const size_t o = (size_t)row * a.Rp + slot;
+            a.x[o] += alpha * p[o];
+            const V q = a.q[o];
+            const V rv = a.r[o] - alpha * q;
+            a.r[o] = rv;
+            const V dr = a.dinv[o] * rv;
+            v[0] += (double)dr * (double)rv;
+            v[1] += (double)rv * (double)rv;
+            v[2] += (double)dr * (double)q;
+            rc[0] += (double)rv;
+            qc[0] += (double)q;
+            if constexpr (NCOL == 2) {
+                const double ph = (double)c.phi[o];
+                rc[1] += ph * (double)rv;
+                qc[1] += ph * (double)q;
+            }
+        }
+    }
+    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+}
+
+// cold start of the selected slots: x = 0, r = b, p = 0; partials and Z^T b
+template <class V, int NCOL>
+__global__ void __launch_bounds__(256) k_c_init(BArgs<V> a, CArgs c, const double *__restrict__ b64, double *x64,
+                                                const int *__restrict__ lsel) {
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.y * 64 + lane;
+    const bool act = lsel[slot] != 0;
+    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    if (__any(act)) {
+        const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
+            if (!act) continue;
+            const size_t o = (size_t)row * a.Rp + slot;
+            const double b = b64[o];
+            const V bv = (V)b;
+            a.x[o] = V(0);
+            if (x64) x64[o] = 0.0;
+            a.r[o] = bv;
+            a.p0[o] = V(0);
+            a.p1[o] = V(0);
+            v[0] += (double)(a.dinv[o] * bv) * (double)bv;
+            v[1] += b * b;
+            rc[0] += (double)bv;
+            if (NCOL == 2) rc[1] += (double)c.phi[o] * (double)bv;
+        }
+    }
+    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+}
+
+// True FP64 residual of the reliable-update selection (mixed precision):
+// r = b - S64 x64 stored in V, partials and Z^T r
+template <class V, int NCOL>
+__global__ void __launch_bounds__(256) k_c_resid(BArgs<V> a, CArgs c, const double *__restrict__ b64,
+                                                 const double *__restrict__ x64) {
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.y * 64 + lane;
+    const bool act = ru_pick(a.sc, slot, a.sc.status[slot]);
+    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    if (__any(act)) {
+        const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const XPlain<double> xs{x64};
+        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
+            if (!act) continue;
+            const double ax = op_row<double>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
+                                             a.opT64, a.K64, a.opT64, a.K64, row, xs, a.Rp, slot);
+            const size_t o = (size_t)row * a.Rp + slot;
+            const double rr = b64[o] - ax;
+            const V rv = (V)rr;
+            a.r[o] = rv;
+            v[0] += (double)(a.dinv[o] * rv) * (double)rv;
+            v[1] += rr * rr;
+            rc[0] += (double)rv;
+            if (NCOL == 2) rc[1] += (double)c.phi[o] * (double)rv;
+        }
+    }
+    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+}
+
+// q = S p with p = z + beta p_old, z = D^-1 r + Z y formed on the fly; partial p.q
+template <class V>
+__global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.y * 64 + lane;
+    const bool act = a.sc.status[r] == kRun;
+    double v[1] = {0.0};
+    if (__any(act)) {
+        const V beta = act ? (V)a.sc.beta[r] : V(0);
+        const int par = a.sc.par[r];
+        const V *pin = par ? a.p1 : a.p0;
+        V *pout = par ? a.p0 : a.p1;
+        const XTwoLevel<V> xs{a.r, a.dinv, pin, c.ncol == 2 ? c.phi : nullptr, c.y, c.agg, beta, (size_t)a.Rp, r};
+        int s0, st, en;
+        row_range(a.F, a.G, a.xcd, s0, st, en);
+        for (int row = s0; row < en; row += st) {
+            if (act) {
+                const V q = op_row<V>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
+                                      a.opT64, a.K64, row, xs, a.Rp, r);
+                const size_t o = (size_t)row * a.Rp + r;
+                const V p = xs(row, o);
+                pout[o] = p;
+                a.q[o] = q;
+                v[0] += (double)p * (double)q;
+            }
+        }
+    }
+    double *const dst[1] = {a.part0};
+    block_part<1>(v, dst, a.Rp);
+}
+
+// Per-slot fold of the aggregate partials, coarse solve y = (Z^T S Z)^-1 Z^T r
+// (FP32 inverse, FP64 accumulation), then the scalar step of CG for the slot.
+// One block per slot.  MODE: kFoldInit (selected slots), kFoldBeta (running
+// slots, flexible beta), kFoldRU (reliable-update selection).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const double *__restrict__ part1,
+                                                const double *__restrict__ part2, size_t Rp, double rtol2,
+                                                double atol2, int max_it, int mixed, const int *__restrict__ lsel) {
+    __shared__ double srx[2048], sqx[2048];
+    __shared__ double red[5][4];
+    const int slot = blockIdx.x;
+    const int st0 = sc.status[slot];
+    bool act;
+    if (MODE == kFoldInit) act = lsel[slot] != 0;
+    else if (MODE == kFoldBeta) act = st0 == kRun;
+    else act = ru_pick(sc, slot, st0);
+    if (!act) return;  // block-uniform
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int g = tid; g < c.nagg; g += blockDim.x) {
+        const size_t o = (size_t)g * Rp + slot;
+        s[0] += part1[o];
+        s[1] += part2[o];
+        if (MODE == kFoldBeta) s[2] += c.part3[o];
+    }
+    for (int i = tid; i < c.nc; i += blockDim.x) {
+        srx[i] = c.rc[(size_t)i * Rp + slot];
+        sqx[i] = MODE == kFoldBeta ? c.qc[(size_t)i * Rp + slot] : 0.0;
+    }
+    __syncthreads();
+    const float *Ai = c.ainv + (size_t)slot * c.nc * c.nc;
+    for (int i = tid; i < c.nc; i += blockDim.x) {
+        double yi = 0.0;
+        for (int j = 0; j < c.nc; j++) yi += (double)Ai[(size_t)j * c.nc + i] * srx[j];
+        c.y[(size_t)i * Rp + slot] = yi;
+        s[3] += yi * srx[i];
+        s[4] += yi * sqx[i];
+    }
+    for (int k = 0; k < 5; k++) {
+        double x = s[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double t[5];
+    for (int k = 0; k < 5; k++) t[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+    const double rz = t[0] + t[3], rr = t[1];
+    int st = st0;
+    if (MODE == kFoldInit) {
+        sc.bnorm2[slot] = rr;
+        sc.tol2[slot] = fmax(rtol2 * rr, atol2);
+        sc.rz[slot] = rz;
+        sc.rz_prev[slot] = 1.0;
+        sc.rr[slot] = rr;
+        sc.beta[slot] = 0.0;
+        sc.alpha[slot] = 0.0;
+        sc.it[slot] = 0;
+        sc.n_ru[slot] = 0;
+        sc.it_ru[slot] = 0;
+        sc.par[slot] = 0;
+        sc.rr_hist[slot] = INFINITY;
+        sc.rr_ru[slot] = rr;
+        st = rr <= sc.tol2[slot] ? kConverged : kRun;
+    } else if (MODE == kFoldBeta) {
+        const int it = sc.it[slot] + 1;
+        sc.it[slot] = it;
+        sc.par[slot] ^= 1;
+        const double zq = t[2] + t[4];
+        sc.rz_prev[slot] = sc.rz[slot];
+        sc.rz[slot] = rz;
+        // flexible CG: beta = z_new . (r_new - r_old) / z_old . r_old, r_new - r_old = -alpha q
+        sc.beta[slot] = -sc.alpha[slot] * zq / sc.rz_prev[slot];
+        sc.rr[slot] = rr;
+        if (!(rr == rr)) st = kBreakdown;
+        else if (rr <= sc.tol2[slot]) st = mixed ? kCandidate : kConverged;
+        else if (it >= max_it) st = kMaxIt;
+        else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
+    } else {  // reliable update
+        sc.n_ru[slot] += 1;
+        sc.rr_ru[slot] = rr;
+        sc.it_ru[slot] = sc.it[slot];
+        sc.rz[slot] = rz;
+        sc.beta[slot] = rz / sc.rz_prev[slot];
+        sc.rr[slot] = rr;
+        if (!(rr == rr)) st = kBreakdown;
+        else if (st0 == kCandidate) {
+            if (rr <= sc.tol2[slot]) st = kConverged;
+            else {
+                st = (rr > 0.01 * sc.rr_hist[slot]) ? kFp64 : kRun;
+                sc.rr_hist[slot] = rr;
+            }
+        } else if (sc.it[slot] >= max_it) st = kMaxIt;
+    }
+    sc.status[slot] = st;
+}
+
+// slot-major arrays ([slot][n]): out[s] = in[src[s]]
+template <class T>
+__global__ void k_b_perm_major(const T *__restrict__ in, T *__restrict__ out, size_t n, const int *__restrict__ src) {
+    const int s = blockIdx.y;
+    const T *a = in + (size_t)src[s] * n;
+    T *b = out + (size_t)s * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
 }
 
 // Slot permutation of a realization-interleaved array: out[n][s] = in[n][src[s]]
@@ -814,6 +1278,24 @@ static T *b_up(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s)
     return p;
 }
 
+template <class V>
+static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *rec, const int *row_ptr, const int *row_c,
+                          const int *meta,
+                          const int *op_off, const int *conn_off, const int *conn, const V *opT, const V *K,
+                          const double *opT64, const double *K64,
+                          const V *dinv, V *x, V *r, V *z, V *p0, V *p1, V *q, double *part, const BatchScal &sc) {
+    BArgs<V> a;
+    a.F = F; a.Rp = Rp; a.G = G; a.xcd = xcd;
+    a.row_ptr = row_ptr; a.row_c = row_c; a.meta = meta; a.op_off = op_off; a.conn_off = conn_off; a.conn = conn;
+    a.rec = rec;
+    a.opT = opT; a.K = K; a.dinv = dinv;
+    a.opT64 = opT64; a.K64 = K64;
+    a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
+    a.part0 = part; a.part1 = part + (size_t)kMaxGrid * Rp; a.part2 = part + 2 * (size_t)kMaxGrid * Rp;
+    a.sc = sc;
+    return a;
+}
+
 DarcyBatchGPU::DarcyBatchGPU(DarcyGPU &base, int R, int slots) : base_(base), dv_(base.dev()) {
     if (R < 1) throw Error(-1, "batch needs at least one realization");
     if (slots < 0) throw Error(-1, "slot count must be >= 0");
@@ -831,6 +1313,7 @@ DarcyBatchGPU::~DarcyBatchGPU() {
     for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
     for (void *p : owned_) dev_free(p);
     if (h_count_) (void)hipHostFree(h_count_);
+    if (solver_) (void)rocblas_destroy_handle((rocblas_handle)solver_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -867,6 +1350,7 @@ void DarcyBatchGPU::build() {
     // the same L2 lines.  perm: row -> edge; conn is renumbered to rows.
     std::vector<int> perm(F);
     std::iota(perm.begin(), perm.end(), 0);
+    std::vector<uint64_t> key;  // Z-curve key per edge (empty: edge order kept)
     if (!std::getenv("FL_BATCH_EDGE_ORDER")) {
         std::vector<double> ec(3 * (size_t)F, 0.0);
         std::vector<int> cnt(F, 0);
@@ -897,7 +1381,7 @@ void DarcyBatchGPU::build() {
             v = (v | v << 2) & 0x1249249249249249ULL;
             return v;
         };
-        std::vector<uint64_t> key(F);
+        key.assign(F, 0);
         for (int f = 0; f < F; f++) {
             uint64_t k = 0;
             for (int d = 0; d < 3; d++) {
@@ -913,6 +1397,68 @@ void DarcyBatchGPU::build() {
         std::vector<int> rinv(F);
         for (int r = 0; r < F; r++) rinv[perm[r]] = r;
         for (auto &c : conn) c = rinv[c];
+    }
+    h_perm_ = perm;
+    // Aggregates of the two-level preconditioner: the cells of one level of
+    // the Z-curve (about 800 rows each), i.e. contiguous row ranges; without
+    // the curve, runs of 800 consecutive rows.
+    {
+        std::vector<int> agg(F);
+        int n = 0;
+        if (!key.empty()) {
+            int L = (int)std::lround(std::log(std::max(1.0, F / 800.0)) / std::log(8.0));
+            L = std::max(1, std::min(L, 3));
+            const int sh = 63 - 3 * L;
+            uint64_t prev = ~0ULL;
+            for (int r = 0; r < F; r++) {
+                const uint64_t pfx = key[perm[r]] >> sh;
+                if (pfx != prev) {
+                    prev = pfx;
+                    n++;
+                }
+                agg[r] = n - 1;
+            }
+        } else {
+            for (int r = 0; r < F; r++) agg[r] = r / 800;
+            n = (F + 799) / 800;
+        }
+        nagg_ = n;
+        h_agg_ = agg;
+        std::vector<int> aptr(n + 1, 0);
+        for (int r = 0; r < F; r++) aptr[agg[r] + 1]++;
+        for (int a = 0; a < n; a++) aptr[a + 1] += aptr[a];
+        std::vector<std::vector<int>> nb(n);
+        for (int e = 0; e < E; e++) {
+            int as[kMaxConn];
+            int na = 0;
+            for (int i = conn_off[e]; i < conn_off[e + 1]; i++) as[na++] = agg[conn[i]];
+            for (int x = 0; x < na; x++)
+                for (int y = 0; y < na; y++) nb[as[x]].push_back(as[y]);
+        }
+        std::vector<int> nbptr(n + 1, 0), nbv;
+        nbmax_ = 0;
+        for (int a = 0; a < n; a++) {
+            nb[a].push_back(a);
+            std::sort(nb[a].begin(), nb[a].end());
+            nb[a].erase(std::unique(nb[a].begin(), nb[a].end()), nb[a].end());
+            nbmax_ = std::max(nbmax_, (int)nb[a].size());
+            nbv.insert(nbv.end(), nb[a].begin(), nb[a].end());
+            nbptr[a + 1] = (int)nbv.size();
+        }
+        coarse_ok_ = nbmax_ <= kNbMax && n <= 1024;
+        std::vector<int> lbmap;
+        if (coarse_ok_) {
+            lbmap.assign((size_t)n * n, -1);
+            for (int a = 0; a < n; a++)
+                for (int k = nbptr[a]; k < nbptr[a + 1]; k++) lbmap[(size_t)a * n + nbv[k]] = k - nbptr[a];
+        } else {
+            lbmap.assign(1, -1);
+        }
+        d_agg_ = b_up(owned_, agg, stream_);
+        d_aptr_ = b_up(owned_, aptr, stream_);
+        d_nbptr_ = b_up(owned_, nbptr, stream_);
+        d_nb_ = b_up(owned_, nbv, stream_);
+        d_lbmap_ = b_up(owned_, lbmap, stream_);
     }
     // contributions of each row in element order
     std::vector<int> row_ptr(F + 1, 0), row_c(conn_off[E]);
@@ -1066,6 +1612,19 @@ void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
                            (size_t)Rp_, (const int *)d_src_);
         std::swap(d_K32_, d_spK32_);
     }
+    if (ncol_) {
+        hipLaunchKernelGGL(k_b_perm<double>, dim3(std::min(nc_, 2048), nchunk_), blk, 0, stream_, (const double *)d_y_,
+                           d_spy_, (size_t)nc_, (size_t)Rp_, (const int *)d_src_);
+        std::swap(d_y_, d_spy_);
+        hipLaunchKernelGGL(k_b_perm_major<float>, dim3(1024, Rp_), blk, 0, stream_, (const float *)d_ainv_,
+                           d_spainv_, (size_t)nc_ * nc_, (const int *)d_src_);
+        std::swap(d_ainv_, d_spainv_);
+        if (ncol_ == 2) {
+            hipLaunchKernelGGL(k_b_perm<float>, gF, blk, 0, stream_, (const float *)d_phi_, d_spphi_, (size_t)F_,
+                               (size_t)Rp_, (const int *)d_src_);
+            std::swap(d_phi_, d_spphi_);
+        }
+    }
     hipLaunchKernelGGL(k_b_perm_scal, dim3((Rp_ + 255) / 256), blk, 0, stream_, sc_, sc2_, (const int *)d_src_, Rp_);
     std::swap(sc_, sc2_);
     FL_HIP(hipGetLastError());
@@ -1083,6 +1642,108 @@ void DarcyBatchGPU::set_pairs(const std::vector<int> &pairs) {
     if (npairs_) FL_HIP(hipMemcpyAsync(d_pairs_, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipMemcpyAsync(d_lsel_, sel.data(), Rp_ * sizeof(int), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));  // host staging vectors go out of scope
+}
+
+__global__ void k_c_phi_rows(const float *__restrict__ src, const int *__restrict__ perm, int F, int R,
+                             float *__restrict__ dst) {
+    const int r = blockIdx.y;
+    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < F; row += gridDim.x * blockDim.x)
+        dst[(size_t)r * F + row] = src[(size_t)r * F + perm[row]];
+}
+
+void DarcyBatchGPU::set_coarse(int ncol) {
+    if (ncol < 0 || ncol > 2) throw Error(-1, "coarse columns must be 0, 1 or 2");
+    if (ncol > 0 && !coarse_ok_)
+        throw Error(-3, "aggregation not usable on this mesh (more than 32 neighbour aggregates or 1024 aggregates)");
+    if (ncol == 2 && !have_phi_) throw Error(-4, "coarse basis phi not set");
+    if (ncol != ncol_) free_graphs();
+    ncol_ = ncol;
+    if (ncol == 0) return;
+    const size_t nv = (size_t)F_ * Rp_;
+    auto ad = [&](size_t n) { double *p = dev_alloc<double>(n); owned_.push_back(p); return p; };
+    auto af = [&](size_t n) { float *p = dev_alloc<float>(n); owned_.push_back(p); return p; };
+    const int ncmax = 2 * nagg_;
+    if (!d_rc_) {
+        d_rc_ = ad((size_t)ncmax * Rp_);
+        d_qc_ = ad((size_t)ncmax * Rp_);
+        d_y_ = ad((size_t)ncmax * Rp_);
+        d_spy_ = ad((size_t)ncmax * Rp_);
+        d_part3_ = ad((size_t)nagg_ * Rp_);
+        d_Asp_ = ad((size_t)nagg_ * kNbMax * 4 * Rp_);
+        d_ainv_ = af((size_t)Rp_ * ncmax * ncmax);
+        d_spainv_ = af((size_t)Rp_ * ncmax * ncmax);
+        d_Apack_ = ad((size_t)Rp_ * ncmax * ncmax);
+        double **pp = dev_alloc<double *>(Rp_);
+        owned_.push_back(pp);
+        d_Aptrs_ = pp;
+        int *inf = dev_alloc<int>(2 * (size_t)Rp_);  // potrf, potri status
+        owned_.push_back(inf);
+        d_info_ = inf;
+        rocblas_handle h;
+        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(-2, "rocblas_create_handle failed");
+        solver_ = h;
+        rocblas_set_stream(h, stream_);
+    }
+    if (ncol == 2 && !d_phi_) {
+        d_phi_ = af(nv);
+        d_spphi_ = af(nv);
+    }
+    nc_ = ncol * nagg_;
+}
+
+void DarcyBatchGPU::set_coarse_basis(const float *d_phi_edges) {
+    if (!coarse_ok_) throw Error(-3, "aggregation not usable on this mesh");
+    if (!d_phiAll_) {
+        d_phiAll_ = dev_alloc<float>((size_t)R_ * F_);
+        owned_.push_back(d_phiAll_);
+    }
+    hipLaunchKernelGGL(k_c_phi_rows, dim3(std::min(grid_for(F_), 1024), R_), dim3(kBlock), 0, stream_, d_phi_edges,
+                       (const int *)d_perm_, F_, R_, d_phiAll_);
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipStreamSynchronize(stream_));
+    have_phi_ = true;
+}
+
+// Coarse operators of the listed (slot, realization) pairs: phi columns,
+// Z^T S Z by aggregate blocks, batched Cholesky + inverse, FP32 inverse.
+void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
+    const int np = (int)pairs.size() / 2;
+    if (!np) return;
+    const dim3 gagg(nagg_, nch_), blk(256);
+    CArgs c{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_qc_, d_y_,
+            d_part3_};
+    if (ncol_ == 2) {
+        hipLaunchKernelGGL(k_b_slot_col, dim3((F_ + 63) / 64, (np + 63) / 64), blk, 0, stream_,
+                           (const float *)d_phiAll_, F_, Rp_, (const int *)d_pairs_, np, d_phi_);
+        hipLaunchKernelGGL(k_c_phi_norm, gagg, dim3(64), 0, stream_, c, (size_t)Rp_, (const int *)d_lsel_, d_phi_);
+    }
+    BArgs<double> a = make_args<double>(F_, Rp_, grid_, xcd_map_, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_,
+                                        d_conn_off_, d_conn_, d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_,
+                                        d_p0_, d_p1_, d_q_, d_part_, sc_);
+    if (ncol_ == 1) {
+        hipLaunchKernelGGL(k_c_assemble<1>, gagg, dim3(64), 0, stream_, a, c, (const int *)d_lsel_, d_Asp_);
+        hipLaunchKernelGGL(k_c_pack<1>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
+                           (const int *)d_pairs_, d_Apack_);
+    } else {
+        hipLaunchKernelGGL(k_c_assemble<2>, gagg, dim3(64), 0, stream_, a, c, (const int *)d_lsel_, d_Asp_);
+        hipLaunchKernelGGL(k_c_pack<2>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
+                           (const int *)d_pairs_, d_Apack_);
+    }
+    FL_HIP(hipGetLastError());
+    std::vector<double *> ptrs(np);
+    for (int k = 0; k < np; k++) ptrs[k] = d_Apack_ + (size_t)k * nc_ * nc_;
+    FL_HIP(hipMemcpyAsync(d_Aptrs_, ptrs.data(), np * sizeof(double *), hipMemcpyHostToDevice, stream_));
+    rocblas_handle h = (rocblas_handle)solver_;
+    if (rocsolver_dpotrf_batched(h, rocblas_fill_lower, nc_, d_Aptrs_, nc_, d_info_, np) != rocblas_status_success)
+        throw Error(-2, "rocsolver_dpotrf_batched failed");
+    // the inverse of a failed factorization is discarded by k_c_unpack
+    if (rocsolver_dpotri_batched(h, rocblas_fill_lower, nc_, d_Aptrs_, nc_, d_info_ + Rp_, np) !=
+        rocblas_status_success)
+        throw Error(-2, "rocsolver_dpotri_batched failed");
+    FL_HIP(hipStreamSynchronize(stream_));  // host pointer staging
+    hipLaunchKernelGGL(k_c_unpack, dim3(nc_, np), blk, 0, stream_, nc_, (const double *)d_Apack_,
+                       (const int *)d_info_, (const int *)(d_info_ + Rp_), (const int *)d_pairs_, d_ainv_);
+    FL_HIP(hipGetLastError());
 }
 
 void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
@@ -1118,23 +1779,6 @@ void DarcyBatchGPU::setup_system() {
     FL_HIP(hipGetLastError());
 }
 
-template <class V>
-static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *rec, const int *row_ptr, const int *row_c,
-                          const int *meta,
-                          const int *op_off, const int *conn_off, const int *conn, const V *opT, const V *K,
-                          const double *opT64, const double *K64,
-                          const V *dinv, V *x, V *r, V *z, V *p0, V *p1, V *q, double *part, const BatchScal &sc) {
-    BArgs<V> a;
-    a.F = F; a.Rp = Rp; a.G = G; a.xcd = xcd;
-    a.row_ptr = row_ptr; a.row_c = row_c; a.meta = meta; a.op_off = op_off; a.conn_off = conn_off; a.conn = conn;
-    a.rec = rec;
-    a.opT = opT; a.K = K; a.dinv = dinv;
-    a.opT64 = opT64; a.K64 = K64;
-    a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
-    a.part0 = part; a.part1 = part + (size_t)kMaxGrid * Rp; a.part2 = part + 2 * (size_t)kMaxGrid * Rp;
-    a.sc = sc;
-    return a;
-}
 
 BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precision) {
     if (!have_K_) throw Error(-4, "batch conductivities not set");
@@ -1155,10 +1799,14 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
     BArgs<double> a64;
     BArgs<float> a32;
+    CArgs ca{};
     int G = grid_, X = xcd_map_;
+    const int NC = ncol_;  // two-level preconditioner columns (0: Jacobi)
     auto rebuild = [&]() {
         G = grid_;
         X = xcd_map_;
+        ca = CArgs{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_qc_, d_y_,
+                   d_part3_};
         a64 = make_args<double>(F_, Rp_, G, X, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_, d_conn_,
                                 d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_, d_p0_, d_p1_, d_q_, d_part_,
                                 sc_);
@@ -1185,29 +1833,60 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
 #undef FL_FOLD
     };
     const int *nosel = nullptr;
+    auto gagg = [&]() { return dim3(nagg_, nch_); };
+    auto cfold = [&](int mode, int mixed, const int *lsel) {
+        const dim3 gs(64 * nch_);
+        if (mode == kFoldInit)
+            hipLaunchKernelGGL(k_c_fold<kFoldInit>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
+                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
+        else if (mode == kFoldBeta)
+            hipLaunchKernelGGL(k_c_fold<kFoldBeta>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
+                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
+        else
+            hipLaunchKernelGGL(k_c_fold<kFoldRU>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
+                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
+    };
     // one CG iteration; ev: optional events around the operator and the update
     auto iter64 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        hipLaunchKernelGGL(k_b_op<double>, grow(), blk, 0, stream_, a64);
+        if (NC) hipLaunchKernelGGL(k_c_op<double>, grow(), blk, 0, stream_, a64, ca);
+        else hipLaunchKernelGGL(k_b_op<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
         fold(kFoldAlpha, 0, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
+        if (NC == 1) hipLaunchKernelGGL((k_c_update<double, 1>), gagg(), blk, 0, stream_, a64, ca);
+        else if (NC == 2) hipLaunchKernelGGL((k_c_update<double, 2>), gagg(), blk, 0, stream_, a64, ca);
+        else hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        fold(kFoldBeta, 0, nosel);
+        if (NC) cfold(kFoldBeta, 0, nosel);
+        else fold(kFoldBeta, 0, nosel);
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
+        if (NC) hipLaunchKernelGGL(k_c_op<float>, grow(), blk, 0, stream_, a32, ca);
+        else hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
         fold(kFoldAlpha, 1, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
+        if (NC == 1) hipLaunchKernelGGL((k_c_update<float, 1>), gagg(), blk, 0, stream_, a32, ca);
+        else if (NC == 2) hipLaunchKernelGGL((k_c_update<float, 2>), gagg(), blk, 0, stream_, a32, ca);
+        else hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        fold(kFoldBeta, 1, nosel);
+        if (NC) cfold(kFoldBeta, 1, nosel);
+        else fold(kFoldBeta, 1, nosel);
     };
     auto reliable_update = [&]() {
         hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_, d_x32_);
+        if (NC) {
+            if (NC == 1)
+                hipLaunchKernelGGL((k_c_resid<float, 1>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                                   (const double *)d_x_);
+            else
+                hipLaunchKernelGGL((k_c_resid<float, 2>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                                   (const double *)d_x_);
+            cfold(kFoldRU, 1, nosel);
+            return;
+        }
         hipLaunchKernelGGL((k_b_resid<float, 0>), grow(), blk, 0, stream_, a32, -1, nosel, (const int *)d_op_off_,
                            (const double *)d_tab_, (const double *)d_K_, (const double *)d_b_,
                            (const double *)d_x_);
@@ -1261,13 +1940,30 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                            (size_t)Rp_, (const int *)d_row_ptr_, (const int *)d_row_c_, (const int *)d_meta_,
                            (const int *)d_rs_off_, (const double *)d_rsT_, (const double *)d_K_, d_b_, d_dinv_,
                            d_dinv32_, (const int *)d_lsel_);
-        if (mixed)
-            hipLaunchKernelGGL(k_b_init<float>, grow(), blk, 0, stream_, a32, (const double *)d_b_, d_x_,
-                               (const int *)d_lsel_);
-        else
-            hipLaunchKernelGGL(k_b_init<double>, grow(), blk, 0, stream_, a64, (const double *)d_b_,
-                               (double *)nullptr, (const int *)d_lsel_);
-        fold(kFoldInit, mixed, d_lsel_);
+        if (NC) {
+            coarse_setup(pairs);
+            if (mixed && NC == 1)
+                hipLaunchKernelGGL((k_c_init<float, 1>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                                   d_x_, (const int *)d_lsel_);
+            else if (mixed)
+                hipLaunchKernelGGL((k_c_init<float, 2>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                                   d_x_, (const int *)d_lsel_);
+            else if (NC == 1)
+                hipLaunchKernelGGL((k_c_init<double, 1>), gagg(), blk, 0, stream_, a64, ca, (const double *)d_b_,
+                                   (double *)nullptr, (const int *)d_lsel_);
+            else
+                hipLaunchKernelGGL((k_c_init<double, 2>), gagg(), blk, 0, stream_, a64, ca, (const double *)d_b_,
+                                   (double *)nullptr, (const int *)d_lsel_);
+            cfold(kFoldInit, mixed, d_lsel_);
+        } else {
+            if (mixed)
+                hipLaunchKernelGGL(k_b_init<float>, grow(), blk, 0, stream_, a32, (const double *)d_b_, d_x_,
+                                   (const int *)d_lsel_);
+            else
+                hipLaunchKernelGGL(k_b_init<double>, grow(), blk, 0, stream_, a64, (const double *)d_b_,
+                                   (double *)nullptr, (const int *)d_lsel_);
+            fold(kFoldInit, mixed, d_lsel_);
+        }
         FL_HIP(hipGetLastError());
         for (int k = 0; k < np; k++) rid_[pairs[2 * k]] = pairs[2 * k + 1];
     };
@@ -1328,6 +2024,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     // iteration touches one realization only.
     auto handover = [&](const std::vector<int> &pairs, int mixed_now) {
         if (pairs.empty()) return;
+        const auto th0 = std::chrono::steady_clock::now();
         set_pairs(pairs);
         if (mixed_now)
             hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
@@ -1367,6 +2064,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             st.handover_iterations += ss.iterations;
         }
         FL_HIP(hipStreamSynchronize(stream_));
+        st.ms_handover += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     };
     // move live slots to the front when that frees whole 64-slot chunks
     auto live = [&](int s) { return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64); };
@@ -1446,7 +2144,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const long guard_max = ((long)max_it + chunk_) * ((R_ + Rp_ - 1) / Rp_ + 2) / chunk_ + 16;
     while (true) {
         read_status();
-        std::vector<int> done;
+        std::vector<int> done, stalled;
         int nrun = 0, nf64 = 0;
         for (int sl = 0; sl < Rp_; sl++) {
             if (rid_[sl] < 0) continue;
@@ -1454,13 +2152,34 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                 done.push_back(sl);
                 done.push_back(rid_[sl]);
             } else if (stv[sl] == kFp64) {
+                stalled.push_back(sl);
+                stalled.push_back(rid_[sl]);
                 nf64++;
             } else {
                 nrun++;
             }
         }
+        if (!stalled.empty()) {
+            // the FP32 recursion of these stagnated: the single-solve FP64 path
+            // finishes them from their current iterate, their slots are refilled
+            st.fp64_finish += nf64;
+            handover(stalled, mixed);
+            for (size_t k = 0; k < stalled.size(); k += 2) {
+                done.push_back(stalled[k]);
+                done.push_back(-1);  // already harvested
+            }
+            nf64 = 0;
+        }
         if (!done.empty()) {
-            harvest(done, mixed);
+            {
+                std::vector<int> h;
+                for (size_t k = 0; k < done.size(); k += 2)
+                    if (done[k + 1] >= 0) {
+                        h.push_back(done[k]);
+                        h.push_back(done[k + 1]);
+                    }
+                harvest(h, mixed);
+            }
             std::vector<int> pairs;
             for (size_t k = 0; k < done.size() && next < queue.size(); k += 2) {
                 pairs.push_back(done[k]);
@@ -1494,48 +2213,6 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
         FL_HIP(hipGraphLaunch(graph(mixed), stream_));
         st.chunks++;
-    }
-    if (mixed) {
-        // FP64 finish for the realizations whose FP32 residual stagnated
-        std::vector<int> f64slots;
-        for (int sl = 0; sl < Rp_; sl++)
-            if (rid_[sl] >= 0 && stv[sl] == kFp64) f64slots.push_back(sl);
-        if (!f64slots.empty()) {
-            st.fp64_finish = (int)f64slots.size();
-            std::vector<int> pairs;
-            for (int sl : f64slots) {
-                pairs.push_back(sl);
-                pairs.push_back(rid_[sl]);
-            }
-            set_pairs(pairs);
-            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
-                               d_x_, d_x32_);
-            read_status();
-            maybe_compact(0);
-            hipLaunchKernelGGL((k_b_resid<double, 1>), grow(), blk, 0, stream_, a64, 1 << kFp64, nosel,
-                               (const int *)d_op_off_, (const double *)d_tab_, (const double *)d_K_,
-                               (const double *)d_b_, (const double *)d_x_);
-            fold(kFoldWarm, 0, nosel);
-            while (true) {
-                read_status();
-                std::vector<int> done;
-                int nrun = 0;
-                for (int sl = 0; sl < Rp_; sl++) {
-                    if (rid_[sl] < 0) continue;
-                    if (terminal(stv[sl])) {
-                        done.push_back(sl);
-                        done.push_back(rid_[sl]);
-                    } else {
-                        nrun++;
-                    }
-                }
-                if (!done.empty()) harvest(done, 0);
-                if (nrun == 0) break;
-                if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
-                FL_HIP(hipGraphLaunch(graph(0), stream_));
-                st.chunks++;
-            }
-        }
     }
     if (ksaved) base_.set_conductivity_device(d_Ksave_);
     FL_HIP(hipStreamSynchronize(stream_));
@@ -1611,6 +2288,13 @@ void DarcyBatchGPU::diagnostics(int *status, int *n_ru) const {
         if (status) status[r] = r < (int)h_status_.size() ? h_status_[r] : kIdle;
         if (n_ru) n_ru[r] = r < (int)h_ru_.size() ? h_ru_[r] : 0;
     }
+}
+
+void DarcyBatchGPU::lambda_device(double *out) const {
+    hipLaunchKernelGGL(k_b_lambda_out, dim3(std::min(grid_for(F_), 1024), R_), dim3(kBlock), 0, stream_, R_,
+                       (size_t)Rout_, F_, (size_t)S_ + E_, (const double *)d_sol_, out);
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipStreamSynchronize(stream_));
 }
 
 void DarcyBatchGPU::get_solution(int r, double *out) const {

@@ -7,6 +7,7 @@
 #include "batched.hh"
 #include "common.hh"
 #include "darcy.hh"
+#include "gnn.hh"
 #include "mesh.hh"
 #include "transport.hh"
 
@@ -19,6 +20,9 @@ struct fl_darcy {
 };
 struct fl_darcy_batch {
     std::unique_ptr<fl::DarcyBatchGPU> g;
+};
+struct fl_gnn {
+    std::unique_ptr<fl::GnnGPU> g;
 };
 struct fl_transport {
     std::unique_ptr<fl::TransportGPU> g;
@@ -295,6 +299,7 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->compactions = st.compactions;
             info->handovers = st.handovers;
             info->handover_iterations = st.handover_iterations;
+            info->ms_handover = st.ms_handover;
         }
         return 0;
     });
@@ -329,6 +334,145 @@ int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chun
 int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n) {
     return guard([&] {
         b->g->set_handover(n);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_set_coarse(fl_darcy_batch *b, int ncol) {
+    return guard([&] {
+        b->g->set_coarse(ncol);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_set_coarse_basis(fl_darcy_batch *b, const float *phi) {
+    return guard([&] {
+        if (!phi) throw fl::Error(-1, "null basis");
+        b->g->set_coarse_basis(phi);
+        return 0;
+    });
+}
+
+int fl_darcy_batch_aggregates(fl_darcy_batch *b, int *edge_agg) {
+    return guard([&] {
+        const auto &pe = b->g->row_edges();
+        const auto &ag = b->g->row_aggregate();
+        if (edge_agg)
+            for (size_t r = 0; r < pe.size(); r++) edge_agg[pe[r]] = ag[r];
+        return b->g->aggregates();
+    });
+}
+
+int fl_darcy_batch_lambda(fl_darcy_batch *b, double *out) {
+    return guard([&] {
+        if (!out) throw fl::Error(-1, "null output");
+        b->g->lambda_device(out);
+        return 0;
+    });
+}
+
+int fl_gnn_create(fl_darcy_batch *b, unsigned seed, fl_gnn **out) {
+    return guard([&] {
+        check_device();
+        if (!b || !out) throw fl::Error(-1, "null handle");
+        auto *g = new fl_gnn();
+        try {
+            g->g.reset(new fl::GnnGPU(*b->g, seed));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+        return 0;
+    });
+}
+
+void fl_gnn_free(fl_gnn *g) { delete g; }
+
+int fl_gnn_n_params(fl_gnn *g) { return g ? g->g->n_params() : -1; }
+
+int fl_gnn_get_params(fl_gnn *g, float *out) {
+    return guard([&] {
+        g->g->get_params(out);
+        return 0;
+    });
+}
+
+int fl_gnn_set_params(fl_gnn *g, const float *in) {
+    return guard([&] {
+        g->g->set_params(in);
+        return 0;
+    });
+}
+
+int fl_gnn_forward(fl_gnn *g, const double *K, float *phi) {
+    return guard([&] {
+        g->g->forward(K, phi);
+        return 0;
+    });
+}
+
+int fl_gnn_loss_grad(fl_gnn *g, const double *lambda, double *loss) {
+    return guard([&] {
+        double L = g->g->loss_backward(lambda);
+        if (loss) *loss = L;
+        return 0;
+    });
+}
+
+int fl_gnn_zero_grad(fl_gnn *g) {
+    return guard([&] {
+        g->g->zero_grad();
+        return 0;
+    });
+}
+
+int fl_gnn_get_grad(fl_gnn *g, float *out) {
+    return guard([&] {
+        g->g->get_grad(out);
+        return 0;
+    });
+}
+
+int fl_gnn_adam_step(fl_gnn *g, double lr) {
+    return guard([&] {
+        g->g->adam_step(lr);
+        return 0;
+    });
+}
+
+int fl_gnn_train(fl_gnn *g, const double *K, const double *lambda, int n, int epochs, int batch, double lr,
+                 double *epoch_loss) {
+    return guard([&] {
+        if (n < 1 || epochs < 1 || batch < 1) throw fl::Error(-1, "bad training sizes");
+        const size_t E = g->g->n_elements(), F = g->g->n_edges();
+        for (int ep = 0; ep < epochs; ep++) {
+            double sum = 0.0;
+            g->g->zero_grad();
+            for (int i = 0; i < n; i++) {
+                g->g->forward(K + (size_t)i * E, nullptr);
+                sum += g->g->loss_backward(lambda + (size_t)i * F);
+                if ((i + 1) % batch == 0 || i + 1 == n) {
+                    g->g->adam_step(lr);
+                    g->g->zero_grad();
+                }
+            }
+            if (epoch_loss) epoch_loss[ep] = sum / n;
+        }
+        return 0;
+    });
+}
+
+int fl_gnn_infer(fl_gnn *g, const double *K, int R, float *phi) {
+    return guard([&] {
+        g->g->infer(K, R, phi);
+        return 0;
+    });
+}
+
+int fl_gnn_features(fl_gnn *g, float *xe, float *xf) {
+    return guard([&] {
+        g->g->get_features(xe, xf);
         return 0;
     });
 }

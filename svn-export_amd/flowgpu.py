@@ -44,7 +44,7 @@ class BatchInfo(C.Structure):
                 ("ms_setup", C.c_double), ("ms_solve", C.c_double), ("ms_resolve", C.c_double),
                 ("ms_total", C.c_double), ("ms_op_avg", C.c_double), ("ms_update_avg", C.c_double),
                 ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int), ("compactions", C.c_int),
-                ("handovers", C.c_int), ("handover_iterations", C.c_longlong)]
+                ("handovers", C.c_int), ("handover_iterations", C.c_longlong), ("ms_handover", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,6 +85,24 @@ SIGNATURES = {
     "fl_darcy_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fl_darcy_batch_autotune": (C.c_int, [C.c_void_p]),
     "fl_darcy_batch_set_handover": (C.c_int, [C.c_void_p, C.c_int]),
+    "fl_darcy_batch_set_coarse": (C.c_int, [C.c_void_p, C.c_int]),
+    "fl_darcy_batch_set_coarse_basis": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_darcy_batch_aggregates": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_darcy_batch_lambda": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_gnn_create": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_void_p)]),
+    "fl_gnn_free": (None, [C.c_void_p]),
+    "fl_gnn_n_params": (C.c_int, [C.c_void_p]),
+    "fl_gnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_gnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_gnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fl_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    "fl_gnn_zero_grad": (C.c_int, [C.c_void_p]),
+    "fl_gnn_get_grad": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_gnn_adam_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "fl_gnn_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                               C.c_void_p]),
+    "fl_gnn_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "fl_gnn_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fl_transport_create": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                       C.POINTER(C.c_void_p)]),
     "fl_transport_free": (None, [C.c_void_p]),
@@ -317,9 +335,90 @@ class DarcyBatch:
     def set_handover(self, n):
         _check(lib().fl_darcy_batch_set_handover(self.h, int(n)))
 
+    def set_coarse(self, ncol):
+        """Two-level preconditioner: 0 Jacobi, 1 constant per aggregate, 2 constant + basis phi."""
+        _check(lib().fl_darcy_batch_set_coarse(self.h, int(ncol)))
+
+    def set_coarse_basis(self, phi_ptr):
+        """phi: device pointer (int) to n_real x F float32 values in edge order."""
+        _check(lib().fl_darcy_batch_set_coarse_basis(self.h, C.c_void_p(phi_ptr)))
+
+    def lambda_device(self, out_ptr):
+        """Edge traces of all realizations into a device array [n_real, F] (float64, edge order)."""
+        _check(lib().fl_darcy_batch_lambda(self.h, C.c_void_p(out_ptr)))
+
+    def aggregates(self):
+        out = np.zeros(self.F, np.int32)
+        n = lib().fl_darcy_batch_aggregates(self.h, out.ctypes.data)
+        if n < 0:
+            _check(n)
+        return n, out
+
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.fl_darcy_batch_free(self.h)
+            self.h = None
+
+
+class LearnedCoarseBasis:
+    """Graph network producing the second coarse column phi of a DarcyBatch's
+    two-level preconditioner (trained on the GPU on solver-generated pairs).
+    Device arrays are passed as integer pointers (e.g. torch tensor.data_ptr())."""
+
+    def __init__(self, batch, seed=0):
+        self.batch = batch
+        h = C.c_void_p()
+        _check(lib().fl_gnn_create(batch.h, int(seed), C.byref(h)))
+        self.h = h
+        self.n_params = lib().fl_gnn_n_params(h)
+        self.E, self.F = batch.E, batch.F
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().fl_gnn_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().fl_gnn_set_params(self.h, p.ctypes.data))
+
+    def forward(self, K_ptr, phi_ptr=0):
+        _check(lib().fl_gnn_forward(self.h, C.c_void_p(K_ptr), C.c_void_p(phi_ptr or None)))
+
+    def loss_grad(self, lam_ptr):
+        L = C.c_double()
+        _check(lib().fl_gnn_loss_grad(self.h, C.c_void_p(lam_ptr), C.byref(L)))
+        return L.value
+
+    def zero_grad(self):
+        _check(lib().fl_gnn_zero_grad(self.h))
+
+    def get_grad(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().fl_gnn_get_grad(self.h, out.ctypes.data))
+        return out
+
+    def adam_step(self, lr):
+        _check(lib().fl_gnn_adam_step(self.h, float(lr)))
+
+    def train(self, K_ptr, lam_ptr, n, epochs, batch, lr):
+        losses = np.zeros(epochs)
+        _check(lib().fl_gnn_train(self.h, C.c_void_p(K_ptr), C.c_void_p(lam_ptr), int(n), int(epochs), int(batch),
+                                  float(lr), losses.ctypes.data))
+        return losses
+
+    def infer(self, K_ptr, R, phi_ptr):
+        _check(lib().fl_gnn_infer(self.h, C.c_void_p(K_ptr), int(R), C.c_void_p(phi_ptr)))
+
+    def features(self):
+        xe = np.zeros((self.E, 16), np.float32)
+        xf = np.zeros((self.F, 16), np.float32)
+        _check(lib().fl_gnn_features(self.h, xe.ctypes.data, xf.ctypes.data))
+        return xe, xf
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_gnn_free(self.h)
             self.h = None
 
 

@@ -1,0 +1,106 @@
+"""Learned coarse basis: the GPU graph network (BF16 MFMA kernels) against the
+FP64 CPU oracle (forward, loss, gradients), training on solver-generated
+pairs, and what the trained network gains inside the two-level flexible CG."""
+import numpy as np
+import pytest
+
+import cases
+import flowgpu as G
+import gnn as N
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def graph_of(om):
+    E, F = om.n_elements, om.n_edges
+    ee = om.elem_edge.reshape(-1, 4)
+    vb = {}
+    for n in range(om.n_vb):
+        vb.setdefault(int(om.vb_elem[n]), []).append(int(om.vb_edge[n]))
+    ptr, lst = [0], []
+    for e in range(E):
+        lst += list(ee[e, :om.elem_dim[e] + 1]) + vb.get(e, [])
+        ptr.append(len(lst))
+    return N.Graph(ptr, lst, F)
+
+
+@pytest.fixture(scope="module")
+def setup(tmp_path_factory):
+    import torch
+    path = str(tmp_path_factory.mktemp("gnn") / "cube6.msh")
+    cases.generated_mesh("cube", 6, path, n_fractures=3, n_channels=8, channel_len=5, seed=3)
+    om, gm = O.Mesh(path), G.Mesh(path)
+    d = cases.darcy_case("cube", om, seed=1)
+    g = G.DarcyFlowMH_Steady(gm)
+    g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    R = 96
+    K = cases.mc_conductivity(om, R, seed=21)
+    b = G.DarcyBatch(g, R, slots=64)
+    b.set_conductivity(K)
+    b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    Kd = torch.tensor(K, dtype=torch.float64, device="cuda")
+    lam = torch.zeros((R, om.n_edges), dtype=torch.float64, device="cuda")
+    b.lambda_device(lam.data_ptr())
+    torch.cuda.synchronize()
+    return om, gm, g, b, K, Kd, lam
+
+
+def test_gnn_forward_and_gradient_match_oracle(setup):
+    import torch
+    om, gm, g, b, K, Kd, lam = setup
+    net = G.LearnedCoarseBasis(b, seed=3)
+    P = N.unflatten(net.get_params().astype(np.float64))
+    phi = torch.zeros(om.n_edges, dtype=torch.float32, device="cuda")
+    net.forward(Kd[5].data_ptr(), phi.data_ptr())
+    xe, xf = net.features()
+    graph = graph_of(om)
+    # element features follow the oracle's formula
+    xe_ref = N.element_features(np.log(K[5]), om.elem_dim, np.log(cases.darcy_case("cube", om, seed=1)["cs"]))
+    assert np.abs(xe - xe_ref).max() < 2e-2 * max(1.0, np.abs(xe_ref).max())
+    phi_ref, cache = N.forward(P, graph, xe.astype(np.float64), xf.astype(np.float64))
+    p = phi.cpu().numpy()
+    e = np.linalg.norm(p - phi_ref) / np.linalg.norm(phi_ref)
+    print("forward rel err %.2e" % e, flush=True)
+    assert e < 3e-2
+    net.zero_grad()
+    L = net.loss_grad(lam[5].data_ptr())
+    n_agg, eagg = b.aggregates()
+    L_ref, dphi = N.coarse_fit_loss(phi_ref, lam[5].cpu().numpy(), eagg, n_agg)
+    gr = net.get_grad().astype(np.float64)
+    gr_ref = N.flatten(N.backward(P, graph, xe.astype(np.float64), xf.astype(np.float64), cache, dphi))
+    cos = gr @ gr_ref / (np.linalg.norm(gr) * np.linalg.norm(gr_ref))
+    rel = np.linalg.norm(gr - gr_ref) / np.linalg.norm(gr_ref)
+    print("loss %.5f oracle %.5f; gradient cosine %.5f rel err %.2e" % (L, L_ref, cos, rel), flush=True)
+    assert abs(L - L_ref) < 2e-2 * abs(L_ref)
+    assert cos > 0.99 and rel < 0.15
+
+
+def test_gnn_training_and_gain(setup):
+    """Train on 64 solver-generated pairs, evaluate the learned coarse column
+    on 32 held-out realizations against the constant-only coarse space."""
+    import torch
+    om, gm, g, b, K, Kd, lam = setup
+    net = G.LearnedCoarseBasis(b, seed=7)
+    ntr = 64
+    losses = net.train(Kd.data_ptr(), lam.data_ptr(), ntr, 40, 8, 3e-3)
+    print("training loss: first %.4f last %.4f" % (losses[0], losses[-1]), flush=True)
+    assert losses[-1] < 0.8 * losses[0]
+    R = K.shape[0]
+    phi = torch.zeros((R, om.n_edges), dtype=torch.float32, device="cuda")
+    net.infer(Kd.data_ptr(), R, phi.data_ptr())
+    torch.cuda.synchronize()
+    held = np.arange(ntr, R)
+    b.set_handover(0)
+    b.set_coarse(0)
+    i0 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    b.set_coarse(1)
+    i1 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    b.set_coarse_basis(phi.data_ptr())
+    b.set_coarse(2)
+    i2 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000)
+    t0, t1, t2 = (int(i["iterations"][held].sum()) for i in (i0, i1, i2))
+    print("held-out iterations: jacobi %d, constant coarse %d, learned coarse %d" % (t0, t1, t2), flush=True)
+    assert i2["converged"].all()
+    assert t1 < t0 and t2 < t0
+    b.set_coarse(0)
