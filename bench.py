@@ -117,6 +117,12 @@ def main():
     ap.add_argument("--slots", type=int, default=128)
     ap.add_argument("--precision", type=int, default=1, help="batched CG: 0 FP64, 1 FP32 + FP64 reliable updates")
     ap.add_argument("--rtol", type=float, default=1e-12)
+    ap.add_argument("--coarse", type=int, default=2,
+                    help="two-level preconditioner columns per aggregate: 0 Jacobi, 1 constant, 2 constant + learned")
+    ap.add_argument("--train-realizations", type=int, default=128)
+    ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--handover", type=int, default=16)
+    ap.add_argument("--compare", type=int, default=1, help="also time the Jacobi and constant-coarse batch forms")
     ap.add_argument("--plain-sample", type=int, default=16, help="realizations timed on the plain path")
     ap.add_argument("--accuracy-sample", type=int, default=4)
     ap.add_argument("--c2", type=int, default=1, help="run the 1M-tet drop-in + transport section")
@@ -156,25 +162,78 @@ def main():
     nnz = G.lib().fl_darcy_edge_nnz(darcy.h)
     log("setup %.1fs: E=%d F=%d S=%d nnz=%d, %d realizations" % (time.time() - t0, E, F, S, nnz, R))
 
-    # ------------------------------------------------------- chosen form: batch
+    # --------------------------------------------- learned coarse basis (GNN)
     batch = G.DarcyBatch(darcy, R, slots=args.slots)
     batch.set_conductivity(K.data_ptr())
-    for _ in range(args.warmup):
-        batch.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=args.precision)
-    infos = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        infos.append(batch.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=args.precision))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    info = infos[-1]
+    batch.set_handover(args.handover)
+    training = None
+    if args.coarse == 2:
+        # training pairs from the solver itself: FP64 batch solves of other realizations
+        t_tr = time.perf_counter()
+        NT = args.train_realizations
+        Kt, _, _ = mc_fields(torch, cen, scale, NT, seed=99)
+        bt = G.DarcyBatch(darcy, NT, slots=args.slots)
+        bt.set_conductivity(Kt.data_ptr())
+        bt.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=0)
+        lam = torch.zeros((NT, F), dtype=torch.float64, device="cuda")
+        bt.lambda_device(lam.data_ptr())
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t_tr
+        del bt
+        net = G.LearnedCoarseBasis(batch, seed=1)
+        t1 = time.perf_counter()
+        losses = net.train(Kt.data_ptr(), lam.data_ptr(), NT, args.train_epochs, 8, 3e-3)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t1
+        training = {"pairs": NT, "epochs": args.train_epochs, "loss_first_epoch": float(losses[0]),
+                    "loss_last_epoch": float(losses[-1]), "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                    "params": net.n_params}
+        log("learned basis: %d solver pairs in %.1f s, %d epochs in %.1f s, loss %.4f -> %.4f"
+            % (NT, t_data, args.train_epochs, t_train, losses[0], losses[-1]))
+        del Kt, lam
+
+    def run_batch(coarse, precision, steps, warmup, timed_infer):
+        batch.set_coarse(0)
+        infos, t_inf = [], 0.0
+        for k in range(warmup + steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            if coarse == 2:
+                # the network's inference is part of every solve
+                phi = torch.empty((R, F), dtype=torch.float32, device="cuda")
+                net.infer(K.data_ptr(), R, phi.data_ptr())
+                batch.set_coarse_basis(phi.data_ptr())
+                del phi
+            batch.set_coarse(coarse)
+            torch.cuda.synchronize()
+            ti = time.perf_counter() - t
+            inf = batch.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=precision)
+            torch.cuda.synchronize()
+            if k >= warmup:
+                t_inf += ti
+                infos.append((time.perf_counter() - t, inf))
+        el = sum(t for t, _ in infos)
+        return el, infos[-1][1], t_inf
+
+    # ------------------------------------------------------- chosen form: batch
+    elapsed, info, t_infer = run_batch(args.coarse, args.precision, args.steps, args.warmup, True)
+    infos = [info]
     value = R * args.steps / elapsed
     it = info["iterations"]
-    log("batch: %.2f s/step -> %.2f solves/s; iterations mean %.0f min %d max %d; converged %d/%d; hand-overs %d; "
-        "op %.3f ms update %.3f ms; grid %d map %d" % (elapsed / args.steps, value, it.mean(), it.min(), it.max(),
-                                                      info["n_converged"], R, info["handovers"], info["ms_op_avg"],
-                                                      info["ms_update_avg"], info["grid"], info["xcd_map"]))
+    log("batch (coarse %d, precision %d): %.2f s/step -> %.2f solves/s; iterations mean %.0f min %d max %d; "
+        "converged %d/%d; hand-overs %d; op %.3f update %.3f coarse %.3f ms; grid %d map %d"
+        % (args.coarse, args.precision, elapsed / args.steps, value, it.mean(), it.min(), it.max(),
+           info["n_converged"], R, info["handovers"], info["ms_op_avg"], info["ms_update_avg"],
+           info["ms_coarse_avg"], info["grid"], info["xcd_map"]))
+    compare = {}
+    if args.compare:
+        for name, (cz, pr) in {"jacobi_fp64": (0, 0), "jacobi_mixed": (0, 1), "constant_coarse_mixed": (1, 1)}.items():
+            el, inf, _ = run_batch(cz, pr, 1, 0, False)
+            compare[name] = {"solves_per_s": round(R / el, 3), "mean_iterations": float(inf["iterations"].mean()),
+                             "max_iterations": int(inf["iterations"].max())}
+            log("  %s: %.2f solves/s, mean %.0f iterations" % (name, R / el, inf["iterations"].mean()))
+        # leave the chosen form's solutions in the batch for the acceptance check
+        run_batch(args.coarse, args.precision, 1, 0, False)
 
     # ------------------------------------------- acceptance: flux error sample
     Kh = K.cpu().numpy()
@@ -211,17 +270,21 @@ def main():
     # --------------------------------------------------------------- roofline
     w = 4.0 if args.precision == 1 else 8.0
     slots = min(((args.slots + 63) // 64) * 64, ((R + 63) // 64) * 64)
-    b_op = op_bytes(F, E, slots, w, 64.0 * F, 0.0)
+    # two-level path: the operator gathers one vector (p) instead of z and the previous direction
+    b_op = op_bytes(F, E, slots, w, 64.0 * F, 0.0) - (F * slots * w if args.coarse else 0.0)
     op_ms = float(np.mean([i["ms_op_avg"] for i in infos]))
     upd_ms = float(np.mean([i["ms_update_avg"] for i in infos]))
     achieved = b_op / (op_ms * 1e-3) / 1e9
     upd_bytes = F * slots * w * 8.0  # x, p, r, q, dinv read; x, r, z written
-    roofline = {"bound": "hbm", "kernel": "k_b_op<%s> (batched condensed operator)" % ("float" if w == 4 else "double"),
+    kname = ("k_c_op<%s> (batched condensed operator, two-level path)" if args.coarse else
+             "k_b_op<%s> (batched condensed operator)") % ("float" if w == 4 else "double")
+    roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": b_op, "avg_launch_us": round(op_ms * 1e3, 2), "live_slots": slots,
                 "update_kernel": {"avg_launch_us": round(upd_ms * 1e3, 2), "bytes_per_launch": upd_bytes,
-                                  "achieved_GBs": round(upd_bytes / (upd_ms * 1e-3) / 1e9, 1)}}
+                                  "achieved_GBs": round(upd_bytes / (upd_ms * 1e-3) / 1e9, 1)},
+                "coarse_solve_and_direction_us": round(float(info["ms_coarse_avg"]) * 1e3, 2)}
 
     # --------------------------------------- secondary: 1M mixed + transport
     c2 = None
@@ -230,7 +293,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(mesh, Kh[int(order[R // 2])], cs, bct, bcp, float(it.mean()), args.cpu_cg_iters)
+        cpu = cpu_baseline(mesh, Kh[int(order[R // 2])], cs, bct, bcp, float(np.mean(it_plain)), args.cpu_cg_iters)
 
     out = {
         "metric": METRIC,
@@ -247,13 +310,18 @@ def main():
         "data": "synthetic: seeded lognormal K realizations on a generated fractured cube mesh",
         "config": {
             "workload": "Monte-Carlo batch: %d realizations (sigma %.1f-%.1f, corr length %.2f-%.2f) of steady MH "
-                        "Darcy on a %d-tet cube with %d fracture triangles; batched FP%s CG%s to rel residual %g, "
-                        "accepted at <= 1e-8 rel flux error"
+                        "Darcy on a %d-tet cube with %d fracture triangles; batched learned-preconditioned FP%s "
+                        "flexible CG%s to rel residual %g, accepted at <= 1e-8 rel flux error"
                         % (R, sig.min(), sig.max(), ell.min(), ell.max(), int((gm.elem_dim == 3).sum()),
                            int((gm.elem_dim == 2).sum()), "32" if args.precision == 1 else "64",
                            " + FP64 reliable updates" if args.precision == 1 else "", args.rtol),
             "elements": E, "edges": F, "sides": S, "edge_nnz": nnz, "realizations": R, "slots": slots,
-            "method": "batched Jacobi-CG (matrix-free condensed operator), slot refill, single-solve tail",
+            "method": ("batched flexible CG, two-level preconditioner (Jacobi + aggregation coarse space, %s), "
+                       "matrix-free condensed operator, slot refill, single-solve tail"
+                       % {0: "off", 1: "constant column", 2: "constant + learned GNN column"}[args.coarse]),
+            "learned_basis_training": training,
+            "infer_s_per_step": round(t_infer / args.steps, 4),
+            "compare_batch_forms": compare,
             "iterations": {"mean": float(it.mean()), "min": int(it.min()), "max": int(it.max())},
             "converged": int(info["n_converged"]), "handovers": int(info["handovers"]),
             "flux_rel_err_max": max(flux_errs), "flux_err_sample": acc_ids,
@@ -340,7 +408,7 @@ def cpu_baseline(mesh, K, cs, bct, bcp, gpu_iters, cg_iters):
     algorithm: full MH assembly, two-stage sparse Schur, Jacobi-CG, resolve)
     on one realization of the same mesh: everything except CG runs in full, CG
     for a bounded number of iterations; the solve time is extrapolated to the
-    batch's mean iteration count."""
+    mean Jacobi-CG iteration count of the sampled realizations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     om = O.Mesh(arrays=mesh)
@@ -353,7 +421,7 @@ def cpu_baseline(mesh, K, cs, bct, bcp, gpu_iters, cg_iters):
     t_solve = st["t_assembly"] + st["t_schur"] + st["t_resolve"] + per_it * gpu_iters
     return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": "oracle on one realization of the same mesh: assembly %.2fs + Schur %.2fs + resolve %.2fs in "
-                      "full, %d Jacobi-CG iterations (%.1f ms each) extrapolated to the batch's mean %.0f iterations; "
+                      "full, %d Jacobi-CG iterations (%.1f ms each) extrapolated to the plain path's mean %.0f iterations; "
                       "wall clock, 1 thread"
                       % (st["t_assembly"], st["t_schur"], st["t_resolve"], st["iters"], 1e3 * per_it, gpu_iters)}
 

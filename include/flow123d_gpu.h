@@ -170,6 +170,8 @@ typedef struct {
     int handovers;                   /* realizations finished by the single-solve FP64 path */
     long long handover_iterations;   /* CG iterations they took there */
     double ms_handover;              /* wall time of that phase */
+    double ms_coarse_avg;            /* two-level: coarse solve + direction kernel per iteration (sampled) */
+    double ms_coarse_setup;          /* two-level: coarse operators and inverses, all refills */
 } fl_batch_info;
 
 /* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as

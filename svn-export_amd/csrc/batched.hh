@@ -39,6 +39,8 @@ struct BatchStats {
     int handovers = 0;              // realizations finished by the single-solve path
     long handover_iterations = 0;   // their iterations there
     double ms_handover = 0;         // wall time of the hand-over phase
+    double ms_coarse_avg = 0;       // live samples: coarse fold + direction kernel per iteration
+    double ms_coarse_setup = 0;     // coarse operators and their inverses, all refills
     long total_iterations = 0;  // summed over realizations
     double ms_setup = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
     double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels
@@ -52,10 +54,12 @@ struct BatchStats {
 struct BatchScal {
     double *alpha, *beta, *rz, *rz_prev, *rr, *tol2, *bnorm2, *rr_hist, *rr_ru;
     int *status, *it, *n_ru, *par, *it_ru, *active_count;  // par: buffer holding the current search direction
+    int *zp;  // two-level path: 1 when the slot's direction must be re-formed after the last fold
 };
 
 // Batched CG status per realization
-enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6 };
+// kPause: paused for a reliable update after a 10x residual drop (two-level FP32 path)
+enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6, kPause = 7 };
 
 class DarcyBatchGPU {
    public:

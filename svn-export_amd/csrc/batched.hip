@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(256) k_b_update(BArgs<V> a) {
 // the FP32 residual is still accurate.
 constexpr double kRuDrop2 = 0.01;
 __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
-    return st == kCandidate || (st == kRun && sc.rr[r] < kRuDrop2 * sc.rr_ru[r]);
+    return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < kRuDrop2 * sc.rr_ru[r]);
 }
 
 // mixed precision: x64 += x32, x32 = 0 for the selected realizations
@@ -1004,7 +1004,7 @@ __global__ void __launch_bounds__(256) k_c_update(BArgs<V> a, CArgs c) {
     double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
     if (__any(act)) {
         const V alpha = act ? (V)a.sc.alpha[slot] : V(0);
-        const V *p = a.sc.par[slot] ? a.p0 : a.p1;  // the buffer the operator just wrote
+        const V *p = a.p0;  // the two-level path keeps its direction in place
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
             if (!act) continue;
@@ -1087,7 +1087,7 @@ __global__ void __launch_bounds__(256) k_c_resid(BArgs<V> a, CArgs c, const doub
     agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
 }
 
-// q = S p with p = z + beta p_old, z = D^-1 r + Z y formed on the fly; partial p.q
+// q = S p for the two-level path (the direction is formed by k_c_zp); partial p.q
 template <class V>
 __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
     const int lane = threadIdx.x & 63;
@@ -1095,11 +1095,7 @@ __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
     const bool act = a.sc.status[r] == kRun;
     double v[1] = {0.0};
     if (__any(act)) {
-        const V beta = act ? (V)a.sc.beta[r] : V(0);
-        const int par = a.sc.par[r];
-        const V *pin = par ? a.p1 : a.p0;
-        V *pout = par ? a.p0 : a.p1;
-        const XTwoLevel<V> xs{a.r, a.dinv, pin, c.ncol == 2 ? c.phi : nullptr, c.y, c.agg, beta, (size_t)a.Rp, r};
+        const XPlain<V> xs{a.p0};
         int s0, st, en;
         row_range(a.F, a.G, a.xcd, s0, st, en);
         for (int row = s0; row < en; row += st) {
@@ -1107,10 +1103,8 @@ __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
                 const V q = op_row<V>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
                                       a.opT64, a.K64, row, xs, a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
-                const V p = xs(row, o);
-                pout[o] = p;
                 a.q[o] = q;
-                v[0] += (double)p * (double)q;
+                v[0] += (double)a.p0[o] * (double)q;
             }
         }
     }
@@ -1118,23 +1112,47 @@ __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
     block_part<1>(v, dst, a.Rp);
 }
 
+// New direction of the slots flagged by the last fold, in place:
+// p = z + beta p with z = D^-1 r + Z y (two-level preconditioner)
+template <class V, int NCOL>
+__global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.y * 64 + lane;
+    const bool act = a.sc.zp[slot] != 0;
+    if (!__any(act)) return;
+    const V beta = act ? (V)a.sc.beta[slot] : V(0);
+    const XTwoLevel<V> xs{a.r, a.dinv, a.p0, NCOL == 2 ? c.phi : nullptr, c.y, c.agg, beta, (size_t)a.Rp, slot};
+    int s0, st, en;
+    row_range(a.F, a.G, a.xcd, s0, st, en);
+    for (int row = s0; row < en; row += st) {
+        if (!act) continue;
+        const size_t o = (size_t)row * a.Rp + slot;
+        a.p0[o] = xs(row, o);
+    }
+}
+
 // Per-slot fold of the aggregate partials, coarse solve y = (Z^T S Z)^-1 Z^T r
 // (FP32 inverse, FP64 accumulation), then the scalar step of CG for the slot.
 // One block per slot.  MODE: kFoldInit (selected slots), kFoldBeta (running
 // slots, flexible beta), kFoldRU (reliable-update selection).
+constexpr int kCoarseThreads = 1024;  // 16 waves per slot: the coarse inverse streams at full rate
+
 template <int MODE>
-__global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const double *__restrict__ part1,
+__global__ void __launch_bounds__(kCoarseThreads) k_c_fold(BatchScal sc, CArgs c, const double *__restrict__ part1,
                                                 const double *__restrict__ part2, size_t Rp, double rtol2,
                                                 double atol2, int max_it, int mixed, const int *__restrict__ lsel) {
     __shared__ double srx[2048], sqx[2048];
-    __shared__ double red[5][4];
+    __shared__ double red[5][kCoarseThreads / 64];
     const int slot = blockIdx.x;
     const int st0 = sc.status[slot];
     bool act;
     if (MODE == kFoldInit) act = lsel[slot] != 0;
     else if (MODE == kFoldBeta) act = st0 == kRun;
     else act = ru_pick(sc, slot, st0);
-    if (!act) return;  // block-uniform
+    if (!act) {  // block-uniform
+        if (threadIdx.x == 0) sc.zp[slot] = 0;
+        return;
+    }
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int g = tid; g < c.nagg; g += blockDim.x) {
@@ -1149,9 +1167,20 @@ __global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const dou
     }
     __syncthreads();
     const float *Ai = c.ainv + (size_t)slot * c.nc * c.nc;
-    for (int i = tid; i < c.nc; i += blockDim.x) {
-        double yi = 0.0;
-        for (int j = 0; j < c.nc; j++) yi += (double)Ai[(size_t)j * c.nc + i] * srx[j];
+    const int nc = c.nc;
+    for (int i = tid; i < nc; i += blockDim.x) {
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+        int j = 0;
+        for (; j + 4 <= nc; j += 4) {
+            const float a0 = Ai[(size_t)j * nc + i], a1 = Ai[(size_t)(j + 1) * nc + i];
+            const float a2 = Ai[(size_t)(j + 2) * nc + i], a3 = Ai[(size_t)(j + 3) * nc + i];
+            y0 += (double)a0 * srx[j];
+            y1 += (double)a1 * srx[j + 1];
+            y2 += (double)a2 * srx[j + 2];
+            y3 += (double)a3 * srx[j + 3];
+        }
+        for (; j < nc; j++) y0 += (double)Ai[(size_t)j * nc + i] * srx[j];
+        const double yi = (y0 + y1) + (y2 + y3);
         c.y[(size_t)i * Rp + slot] = yi;
         s[3] += yi * srx[i];
         s[4] += yi * sqx[i];
@@ -1164,7 +1193,11 @@ __global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const dou
     __syncthreads();
     if (tid != 0) return;
     double t[5];
-    for (int k = 0; k < 5; k++) t[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+    for (int k = 0; k < 5; k++) {
+        double x = 0.0;
+        for (int w = 0; w < kCoarseThreads / 64; w++) x += red[k][w];
+        t[k] = x;
+    }
     const double rz = t[0] + t[3], rr = t[1];
     int st = st0;
     if (MODE == kFoldInit) {
@@ -1185,7 +1218,6 @@ __global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const dou
     } else if (MODE == kFoldBeta) {
         const int it = sc.it[slot] + 1;
         sc.it[slot] = it;
-        sc.par[slot] ^= 1;
         const double zq = t[2] + t[4];
         sc.rz_prev[slot] = sc.rz[slot];
         sc.rz[slot] = rz;
@@ -1196,6 +1228,7 @@ __global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const dou
         else if (rr <= sc.tol2[slot]) st = mixed ? kCandidate : kConverged;
         else if (it >= max_it) st = kMaxIt;
         else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
+        else if (mixed && rr < kRuDrop2 * sc.rr_ru[slot]) st = kPause;  // direction kept for the update
     } else {  // reliable update
         sc.n_ru[slot] += 1;
         sc.rr_ru[slot] = rr;
@@ -1210,9 +1243,12 @@ __global__ void __launch_bounds__(256) k_c_fold(BatchScal sc, CArgs c, const dou
                 st = (rr > 0.01 * sc.rr_hist[slot]) ? kFp64 : kRun;
                 sc.rr_hist[slot] = rr;
             }
-        } else if (sc.it[slot] >= max_it) st = kMaxIt;
+        } else if (rr <= sc.tol2[slot]) st = kConverged;
+        else if (sc.it[slot] >= max_it) st = kMaxIt;
+        else st = kRun;
     }
     sc.status[slot] = st;
+    sc.zp[slot] = st == kRun ? 1 : 0;
 }
 
 // slot-major arrays ([slot][n]): out[s] = in[src[s]]
@@ -1263,6 +1299,7 @@ __global__ void k_b_perm_scal(BatchScal in, BatchScal out, const int *__restrict
     out.it[s] = in.it[f];
     out.n_ru[s] = in.n_ru[f];
     out.it_ru[s] = in.it_ru[f];
+    out.zp[s] = in.zp[f];
     out.par[s] = 0;  // directions are moved into buffer 0
 }
 
@@ -1563,7 +1600,7 @@ void DarcyBatchGPU::alloc32() {
 
 void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
     double *scd = dev_alloc<double>(9 * (size_t)Rp_);
-    int *sci = dev_alloc<int>(6 * (size_t)Rp_ + 2 * nchunk_);
+    int *sci = dev_alloc<int>(7 * (size_t)Rp_ + 2 * nchunk_);
     owned_.push_back(scd);
     owned_.push_back(sci);
     sc.alpha = scd; sc.beta = scd + Rp_; sc.rz = scd + 2 * Rp_; sc.rz_prev = scd + 3 * Rp_;
@@ -1571,7 +1608,8 @@ void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
     sc.rr_ru = scd + 8 * Rp_;
     sc.status = sci; sc.it = sci + Rp_; sc.n_ru = sci + 2 * Rp_; sc.par = sci + 3 * Rp_;
     sc.it_ru = sci + 4 * Rp_;
-    sc.active_count = sci + 6 * Rp_;
+    sc.zp = sci + 5 * Rp_;
+    sc.active_count = sci + 7 * Rp_;
 }
 
 void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
@@ -1835,7 +1873,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const dim3 gs(64 * nch_);
+        const dim3 gs(64 * nch_), blk(kCoarseThreads);
         if (mode == kFoldInit)
             hipLaunchKernelGGL(k_c_fold<kFoldInit>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
                                (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
@@ -1846,7 +1884,16 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             hipLaunchKernelGGL(k_c_fold<kFoldRU>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
                                (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
     };
-    // one CG iteration; ev: optional events around the operator and the update
+    auto zp64 = [&]() {
+        if (NC == 1) hipLaunchKernelGGL((k_c_zp<double, 1>), grow(), blk, 0, stream_, a64, ca);
+        else hipLaunchKernelGGL((k_c_zp<double, 2>), grow(), blk, 0, stream_, a64, ca);
+    };
+    auto zp32 = [&]() {
+        if (NC == 1) hipLaunchKernelGGL((k_c_zp<float, 1>), grow(), blk, 0, stream_, a32, ca);
+        else hipLaunchKernelGGL((k_c_zp<float, 2>), grow(), blk, 0, stream_, a32, ca);
+    };
+    // one CG iteration; ev: optional events around the operator, the update and
+    // the fold (with the coarse solve and the direction kernel)
     auto iter64 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
         if (NC) hipLaunchKernelGGL(k_c_op<double>, grow(), blk, 0, stream_, a64, ca);
@@ -1858,8 +1905,13 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         else if (NC == 2) hipLaunchKernelGGL((k_c_update<double, 2>), gagg(), blk, 0, stream_, a64, ca);
         else hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        if (NC) cfold(kFoldBeta, 0, nosel);
-        else fold(kFoldBeta, 0, nosel);
+        if (NC) {
+            cfold(kFoldBeta, 0, nosel);
+            zp64();
+        } else {
+            fold(kFoldBeta, 0, nosel);
+        }
+        if (ev) FL_HIP(hipEventRecord(ev[4], stream_));
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
@@ -1872,8 +1924,13 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         else if (NC == 2) hipLaunchKernelGGL((k_c_update<float, 2>), gagg(), blk, 0, stream_, a32, ca);
         else hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
-        if (NC) cfold(kFoldBeta, 1, nosel);
-        else fold(kFoldBeta, 1, nosel);
+        if (NC) {
+            cfold(kFoldBeta, 1, nosel);
+            zp32();
+        } else {
+            fold(kFoldBeta, 1, nosel);
+        }
+        if (ev) FL_HIP(hipEventRecord(ev[4], stream_));
     };
     auto reliable_update = [&]() {
         hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_, d_x32_);
@@ -1885,6 +1942,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                 hipLaunchKernelGGL((k_c_resid<float, 2>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
                                    (const double *)d_x_);
             cfold(kFoldRU, 1, nosel);
+            zp32();
             return;
         }
         hipLaunchKernelGGL((k_b_resid<float, 0>), grow(), blk, 0, stream_, a32, -1, nosel, (const int *)d_op_off_,
@@ -1941,7 +1999,11 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                            (const int *)d_rs_off_, (const double *)d_rsT_, (const double *)d_K_, d_b_, d_dinv_,
                            d_dinv32_, (const int *)d_lsel_);
         if (NC) {
+            const auto tc = std::chrono::steady_clock::now();
             coarse_setup(pairs);
+            FL_HIP(hipStreamSynchronize(stream_));
+            st.ms_coarse_setup +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
             if (mixed && NC == 1)
                 hipLaunchKernelGGL((k_c_init<float, 1>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
                                    d_x_, (const int *)d_lsel_);
@@ -1955,6 +2017,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                 hipLaunchKernelGGL((k_c_init<double, 2>), gagg(), blk, 0, stream_, a64, ca, (const double *)d_b_,
                                    (double *)nullptr, (const int *)d_lsel_);
             cfold(kFoldInit, mixed, d_lsel_);
+            if (mixed) zp32();
+            else zp64();
         } else {
             if (mixed)
                 hipLaunchKernelGGL(k_b_init<float>, grow(), blk, 0, stream_, a32, (const double *)d_b_, d_x_,
@@ -2067,7 +2131,9 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         st.ms_handover += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     };
     // move live slots to the front when that frees whole 64-slot chunks
-    auto live = [&](int s) { return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64); };
+    auto live = [&](int s) {
+        return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64 || stv[s] == kPause);
+    };
     auto maybe_compact = [&](int phase) {
         if (nch_ <= 1) return;
         std::vector<int> src;
@@ -2113,33 +2179,37 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     }
     // live kernel timing: the first iterations run eagerly between events
     const int kSample = 4;
-    while ((int)ev_.size() < 4 * kSample) {
+    while ((int)ev_.size() < 5 * kSample) {
         hipEvent_t e;
         FL_HIP(hipEventCreate(&e));
         ev_.push_back(e);
     }
     for (int k = 0; k < kSample; k++) {
-        if (mixed) iter32(&ev_[4 * k]);
-        else iter64(&ev_[4 * k]);
+        if (mixed) iter32(&ev_[5 * k]);
+        else iter64(&ev_[5 * k]);
     }
     if (mixed) reliable_update();
     {
-        double so = 0, su = 0;
+        double so = 0, su = 0, sf = 0;
         FL_HIP(hipStreamSynchronize(stream_));
         for (int k = 0; k < kSample; k++) {
-            float t1 = 0, t2 = 0;
-            FL_HIP(hipEventElapsedTime(&t1, ev_[4 * k], ev_[4 * k + 1]));
-            FL_HIP(hipEventElapsedTime(&t2, ev_[4 * k + 2], ev_[4 * k + 3]));
+            float t1 = 0, t2 = 0, t3 = 0;
+            FL_HIP(hipEventElapsedTime(&t1, ev_[5 * k], ev_[5 * k + 1]));
+            FL_HIP(hipEventElapsedTime(&t2, ev_[5 * k + 2], ev_[5 * k + 3]));
+            FL_HIP(hipEventElapsedTime(&t3, ev_[5 * k + 3], ev_[5 * k + 4]));
             so += t1;
             su += t2;
+            sf += t3;
         }
         st.ms_op_avg = so / kSample;
         st.ms_update_avg = su / kSample;
+        st.ms_coarse_avg = sf / kSample;
         st.op_samples = kSample;
     }
     // main loop: graph chunks; harvest finished slots, refill them from the
     // queue, compact once the queue is empty
     auto terminal = [&](int v) { return v == kConverged || v == kMaxIt || v == kBreakdown; };
+    std::vector<int> free_slots;  // harvested slots waiting for a refill
     long guard = 0;
     const long guard_max = ((long)max_it + chunk_) * ((R_ + Rp_ - 1) / Rp_ + 2) / chunk_ + 16;
     while (true) {
@@ -2171,20 +2241,27 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             nf64 = 0;
         }
         if (!done.empty()) {
-            {
-                std::vector<int> h;
-                for (size_t k = 0; k < done.size(); k += 2)
-                    if (done[k + 1] >= 0) {
-                        h.push_back(done[k]);
-                        h.push_back(done[k + 1]);
-                    }
-                harvest(h, mixed);
+            std::vector<int> h;
+            for (size_t k = 0; k < done.size(); k += 2) {
+                if (done[k + 1] >= 0) {
+                    h.push_back(done[k]);
+                    h.push_back(done[k + 1]);
+                }
+                free_slots.push_back(done[k]);
             }
+            harvest(h, mixed);
+        }
+        // refill: at once with the one-level preconditioner; in groups of 16 with
+        // the two-level one (each refill factorizes the new coarse operators)
+        const size_t left = queue.size() - next;
+        if (!free_slots.empty() && left > 0 &&
+            (!NC || nrun == 0 || free_slots.size() >= std::min<size_t>(16, left))) {
             std::vector<int> pairs;
-            for (size_t k = 0; k < done.size() && next < queue.size(); k += 2) {
-                pairs.push_back(done[k]);
+            for (size_t k = 0; k < free_slots.size() && next < queue.size(); k++) {
+                pairs.push_back(free_slots[k]);
                 pairs.push_back(queue[next++]);
             }
+            free_slots.erase(free_slots.begin(), free_slots.begin() + pairs.size() / 2);
             if (!pairs.empty()) {
                 // refilled slots may lie beyond the live chunks after a compaction
                 int hi = 0;

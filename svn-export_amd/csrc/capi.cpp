@@ -300,6 +300,8 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->handovers = st.handovers;
             info->handover_iterations = st.handover_iterations;
             info->ms_handover = st.ms_handover;
+            info->ms_coarse_avg = st.ms_coarse_avg;
+            info->ms_coarse_setup = st.ms_coarse_setup;
         }
         return 0;
     });

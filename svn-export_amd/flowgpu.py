@@ -44,7 +44,8 @@ class BatchInfo(C.Structure):
                 ("ms_setup", C.c_double), ("ms_solve", C.c_double), ("ms_resolve", C.c_double),
                 ("ms_total", C.c_double), ("ms_op_avg", C.c_double), ("ms_update_avg", C.c_double),
                 ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int), ("compactions", C.c_int),
-                ("handovers", C.c_int), ("handover_iterations", C.c_longlong), ("ms_handover", C.c_double)]
+                ("handovers", C.c_int), ("handover_iterations", C.c_longlong), ("ms_handover", C.c_double),
+                ("ms_coarse_avg", C.c_double), ("ms_coarse_setup", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
