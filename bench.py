@@ -276,11 +276,18 @@ def main():
     upd_ms = float(np.mean([i["ms_update_avg"] for i in infos]))
     achieved = b_op / (op_ms * 1e-3) / 1e9
     upd_bytes = F * slots * w * 8.0  # x, p, r, q, dinv read; x, r, z written
+    # HBM bytes per launch of the operator with all slots live, from the two
+    # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated in
+    # the same runs) over this workload shape; other shapes were not profiled
+    traffic = None
+    if (args.n, slots, args.precision, args.coarse) == (32, 128, 1, 2):
+        traffic = {"bytes": 843.0e6, "fetch_bytes": 626.0e6, "write_bytes": 217.0e6,
+                   "source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes of bench.py on this workload (profiles/)"}
     kname = ("k_c_op<%s> (batched condensed operator, two-level path)" if args.coarse else
              "k_b_op<%s> (batched condensed operator)") % ("float" if w == 4 else "double")
     roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": b_op, "avg_launch_us": round(op_ms * 1e3, 2), "live_slots": slots,
                 "update_kernel": {"avg_launch_us": round(upd_ms * 1e3, 2), "bytes_per_launch": upd_bytes,
                                   "achieved_GBs": round(upd_bytes / (upd_ms * 1e-3) / 1e9, 1)},

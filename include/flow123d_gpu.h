@@ -195,6 +195,10 @@ const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
  * autotuner that picks grid and mapping by timing the operator kernel. */
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
+/* Once the queue is empty and at most n realizations are still iterating,
+ * finish them one by one with the single-solve FP64 CG (warm start from the
+ * batch iterate); 0 disables.  Default 16. */
+int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n);
 /* Two-level preconditioner of the batched CG: ncol 0 = Jacobi only, 1 = plus
  * one constant coarse column per aggregate, 2 = plus a second column phi per
  * aggregate (fl_darcy_batch_set_coarse_basis; e.g. the learned basis of
@@ -240,10 +244,6 @@ int fl_gnn_train(fl_gnn *g, const double *K, const double *lambda, int n, int ep
 int fl_gnn_infer(fl_gnn *g, const double *K, int R, float *phi);
 /* inputs of the last forward (host FP32 [E][16], [F][16]; either may be NULL) */
 int fl_gnn_features(fl_gnn *g, float *xe, float *xf);
-/* Once the queue is empty and at most n realizations are still iterating,
- * finish them one by one with the single-solve FP64 CG (warm start from the
- * batch iterate); 0 disables.  Default 16. */
-int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n);
 
 /* ---- explicit upwind transport -------------------------------------------- */
 
