@@ -54,7 +54,7 @@ struct BatchStats {
 struct BatchScal {
     double *alpha, *beta, *rz, *rz_prev, *rr, *tol2, *bnorm2, *rr_hist, *rr_ru;
     int *status, *it, *n_ru, *par, *it_ru, *active_count;  // par: buffer holding the current search direction
-    int *zp;  // two-level path: 1 when the slot's direction must be re-formed after the last fold
+    int *zp;  // two-level path, per slot: bit 0 re-form the direction, bit 1 add alpha p to x
 };
 
 // Batched CG status per realization
@@ -94,6 +94,10 @@ class DarcyBatchGPU {
     void autotune();
     // live-realization count at which the rest go to the single-solve path (0: never)
     void set_handover(int n) { handover_ = n < 0 ? 0 : n; }
+    // Two-level path: once the queue is empty and at most 16 (then 4)
+    // realizations are live, continue them in the batch with 16 (4) lanes per
+    // edge row instead of handing them over (default on)
+    void set_narrow(bool on) { narrow_ = on; }
     // Two-level preconditioner: ncol 0 off (Jacobi), 1 one constant column per
     // aggregate, 2 constant + basis column phi per aggregate
     void set_coarse(int ncol);
@@ -108,7 +112,7 @@ class DarcyBatchGPU {
    private:
     void build();
     void alloc32();
-    void tune();
+    void tune(int precision);  // launch shape for the operator kernel of this precision
     void setup_system();  // table, rhs, Jacobi diagonal for the current K
     void free_graphs();
     void alloc_scal(BatchScal &sc);
@@ -132,12 +136,22 @@ class DarcyBatchGPU {
     int dim_start_[4] = {0, 0, 0, 0};
     // per realization
     double *d_K_ = nullptr, *d_b_ = nullptr, *d_dinv_ = nullptr;
-    float *d_K32_ = nullptr, *d_dinv32_ = nullptr;
+    float *d_K32_ = nullptr;
+    unsigned short *d_dinv32_ = nullptr, *d_sp16_ = nullptr;  // FP32 path: D^-1 as bfloat16, its spare
     double *d_x_ = nullptr, *d_r_ = nullptr, *d_z_ = nullptr, *d_p0_ = nullptr, *d_p1_ = nullptr, *d_q_ = nullptr;
     float *d_x32_ = nullptr, *d_r32_ = nullptr, *d_z32_ = nullptr, *d_p32a_ = nullptr, *d_p32b_ = nullptr,
           *d_q32_ = nullptr;
     double *d_sol_ = nullptr, *d_ext_ = nullptr;
     double *d_part_ = nullptr;  // 3 partial arrays of 2048 * Rp
+    double *d_pre_ = nullptr;   // first-stage fold sums, 3 arrays of 32 * Rp
+    double *d_cpart_ = nullptr; // coarse-solve sums, 5 x (row blocks) x Rp
+    int lw_ = 64;               // lanes per row of the two-level iteration kernels (64, 16, 4)
+    bool narrow_ = true;        // two-level path: narrow lanes for the tail instead of the hand-over
+    bool trace_ = false;        // per-chunk progress lines on stderr
+    int opfast_ = 2;            // two-level operator: 0 general rows, 1 pair rows, 2 inlined pair records (2 chunks per wave), 3 the same, 1 chunk
+    int *d_frec_ = nullptr;     // FP32 pair-row records (20 words per row)
+    bool unroll_ = true;        // two-level update: two rows per step
+    int qs_ = 4;                // column split of the coarse solve with all slots busy
     BatchScal sc_{}, sc2_{};
     int *h_count_ = nullptr;  // pinned
     // slot -> realization (-1: empty), live chunk count, spare buffers
@@ -152,7 +166,8 @@ class DarcyBatchGPU {
     bool have_phi_ = false;
     std::vector<int> h_perm_, h_agg_;
     int *d_agg_ = nullptr, *d_aptr_ = nullptr, *d_nbptr_ = nullptr, *d_nb_ = nullptr, *d_lbmap_ = nullptr;
-    float *d_phiAll_ = nullptr, *d_phi_ = nullptr, *d_ainv_ = nullptr, *d_spphi_ = nullptr, *d_spainv_ = nullptr;
+    float *d_phiAll_ = nullptr, *d_ainv_ = nullptr, *d_spainv_ = nullptr;
+    unsigned short *d_phi_ = nullptr, *d_spphi_ = nullptr;  // phi per slot, bfloat16 [F][Rp]
     double *d_Asp_ = nullptr, *d_rc_ = nullptr, *d_qc_ = nullptr, *d_y_ = nullptr, *d_spy_ = nullptr;
     double *d_part3_ = nullptr, *d_Apack_ = nullptr;
     double **d_Aptrs_ = nullptr;

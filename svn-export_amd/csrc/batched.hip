@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <numeric>
 
@@ -22,9 +23,31 @@ namespace fl {
 namespace {
 
 constexpr int kWaves = 4;            // waves per block of the row kernels
+constexpr int kAggWaves = 8;         // waves per block of the aggregate-blocked kernels (one aggregate per block)
 constexpr int kFoldThreads = 1024;   // 16 waves per fold block
 constexpr int kMaxGrid = 2048;       // partial arrays are sized for this many blocks
 constexpr int kMaxConn = 16;         // element index packing e * 16 + local index
+
+// Per-slot preconditioner arrays of the FP32 path (Jacobi diagonal D^-1, the
+// coarse column phi) are stored as bfloat16: they define the preconditioner,
+// not the operator, so rounding them changes M slightly but consistently
+// (every kernel reads the same stored values) and halves their traffic.
+__host__ __device__ inline unsigned short f_to_bf16(float f) {  // round to nearest even, finite inputs
+    unsigned b;
+    __builtin_memcpy(&b, &f, 4);
+    return (unsigned short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
+template <class V>
+struct Pre {  // storage of D^-1 in the V path
+    using T = V;
+    __device__ static V ld(const T *p, size_t o) { return p[o]; }
+};
+template <>
+struct Pre<float> {
+    using T = unsigned short;
+    __device__ static float ld(const T *p, size_t o) { return bf16_to_f(p[o]); }
+};
 
 // Element table layouts (offsets in values):
 //  operator table, element without vb neighbours (nvb = 0):
@@ -48,7 +71,9 @@ struct BArgs {
     int F, Rp, G, xcd;
     const int *row_ptr, *row_c, *meta, *op_off, *conn_off, *conn;
     const int *rec;  // packed row records (16 ints per row)
-    const V *opT, *K, *dinv;
+    const int *frec; // FP32 tetrahedron-pair row records (20 words per row) or null
+    const V *opT, *K;
+    const typename Pre<V>::T *dinv;
     const double *opT64, *K64;  // FP64 table and K (elements with vb neighbours)
     V *x, *r, *z, *p0, *p1, *q;
     double *part0, *part1, *part2;
@@ -129,9 +154,9 @@ struct XFly {
 template <class V>
 struct XTwoLevel {
     const V *__restrict__ r;
-    const V *__restrict__ dinv;
+    const typename Pre<V>::T *__restrict__ dinv;
     const V *__restrict__ p;
-    const float *__restrict__ phi;  // [F][Rp] or null (constant column only)
+    const unsigned short *__restrict__ phi;  // [F][Rp] bfloat16, or null (constant column only)
     const double *__restrict__ y;   // [nc][Rp]
     const int *__restrict__ agg;    // row -> aggregate
     V beta;
@@ -140,10 +165,11 @@ struct XTwoLevel {
     __device__ __forceinline__ V z(int j, size_t idx) const {
         const int a = agg[j];
         if (phi) {
-            const double c = y[(size_t)(2 * a) * Rp + slot] + (double)phi[idx] * y[(size_t)(2 * a + 1) * Rp + slot];
-            return dinv[idx] * r[idx] + (V)c;
+            const double c =
+                y[(size_t)(2 * a) * Rp + slot] + (double)bf16_to_f(phi[idx]) * y[(size_t)(2 * a + 1) * Rp + slot];
+            return Pre<V>::ld(dinv, idx) * r[idx] + (V)c;
         }
-        return dinv[idx] * r[idx] + (V)y[(size_t)a * Rp + slot];
+        return Pre<V>::ld(dinv, idx) * r[idx] + (V)y[(size_t)a * Rp + slot];
     }
     __device__ __forceinline__ V operator()(int j, size_t idx) const { return z(j, idx) + beta * p[idx]; }
 };
@@ -266,6 +292,65 @@ __device__ __forceinline__ V op_row(const int *__restrict__ rec, const int *__re
     return y;
 }
 
+// The common row of a tetrahedral mesh: two tetrahedra without vb
+// neighbours (record contributions with n0 = 4).  Same arithmetic and order
+// as op_row; both contributions' K and x gathers are issued before any of
+// them is used.
+template <class V, class X>
+__device__ __forceinline__ V op_row_pair(const int *__restrict__ R, const V *__restrict__ opT,
+                                         const V *__restrict__ Kv, const X &xs, size_t Rp, int r) {
+    const int i1 = R[1], i2 = R[9];
+    const V *T1 = opT + R[3], *T2 = opT + R[11];
+    const V K1 = Kv[(size_t)R[0] * Rp + r], K2 = Kv[(size_t)R[8] * Rp + r];
+    V a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        a[j] = xs(R[4 + j], (size_t)R[4 + j] * Rp + r);
+        b[j] = xs(R[12 + j], (size_t)R[12 + j] * Rp + r);
+    }
+    V t1 = V(0), t2 = V(0), x1 = V(0), x2 = V(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        t1 += T1[i1 * 4 + j] * a[j];
+        t2 += T2[i2 * 4 + j] * b[j];
+        if (j == i1) x1 = a[j];
+        if (j == i2) x2 = b[j];
+    }
+    V y = K1 * t1 + T1[16 + i1] * x1;
+    y += K2 * t2 + T2[16 + i2] * x2;
+    return y;
+}
+
+// Row records of the FP32 operator for tetrahedron-pair rows, with the row of
+// each element block inlined so the operator needs no table load:
+//   [0] e1 (-1: not a pair row, use the general path), [1] e2,
+//   [2..5] rows of e1's edges, [6..9] rows of e2's edges,
+//   [10..13] P0_e1 row i1, [14..17] P0_e2 row i2 (FP32 bits), [18] dlt_e1[i1], [19] dlt_e2[i2]
+// (the diagonal term multiplies the row's own value).
+constexpr int kFastRec = 20;
+__global__ void k_b_fastrec(int F, const int *__restrict__ rec, const float *__restrict__ opT,
+                            int *__restrict__ out) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= F) return;
+    const int *R = rec + (size_t)row * kRecInts;
+    int *O = out + (size_t)row * kFastRec;
+    if (!(R[0] >= 0 && R[2] == 4 && R[8] >= 0 && R[10] == 4)) {
+        O[0] = -1;
+        return;
+    }
+    for (int k = 0; k < 2; k++) {
+        const int *Q = R + 8 * k;
+        const float *T = opT + Q[3];
+        const int i = Q[1];
+        O[k] = Q[0];
+        for (int j = 0; j < 4; j++) {
+            O[2 + 4 * k + j] = Q[4 + j];
+            O[10 + 4 * k + j] = __float_as_int(T[i * 4 + j]);
+        }
+        O[18 + k] = __float_as_int(T[16 + i]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Table of K-independent element factors (element_local at K = 1)
 // ---------------------------------------------------------------------------
@@ -357,7 +442,7 @@ __global__ void k_b_slot_K(const double *__restrict__ K, int E, int Sp, const in
 // Per-realization FP32 rows [real][n] into slot columns dst[n][slot] for the
 // listed (slot, real) pairs (LDS transpose, 64 entries x 64 pairs per block)
 __global__ void k_b_slot_col(const float *__restrict__ src, int n, int Sp, const int *__restrict__ pairs, int npairs,
-                             float *__restrict__ dst) {
+                             unsigned short *__restrict__ dst) {
     __shared__ float t[64][65];
     const int e0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -371,7 +456,7 @@ __global__ void k_b_slot_col(const float *__restrict__ src, int n, int Sp, const
     const int slot = pairs[2 * pi];
     for (int j = wid; j < 64; j += 4) {
         const int e = e0 + j;
-        if (e < n) dst[(size_t)e * Sp + slot] = t[lane][j];
+        if (e < n) dst[(size_t)e * Sp + slot] = f_to_bf16(t[lane][j]);
     }
 }
 
@@ -417,7 +502,7 @@ __global__ void __launch_bounds__(256) k_b_rhs(int F, size_t Rp, const int *__re
                                                const int *__restrict__ row_c, const int *__restrict__ meta,
                                                const int *__restrict__ rs_off, const double *__restrict__ rsT,
                                                const double *__restrict__ Kv, double *__restrict__ b,
-                                               double *__restrict__ dinv, float *__restrict__ dinv32,
+                                               double *__restrict__ dinv, unsigned short *__restrict__ dinv32,
                                                const int *__restrict__ lsel) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.y * 64 + lane;
@@ -455,7 +540,7 @@ __global__ void __launch_bounds__(256) k_b_rhs(int F, size_t Rp, const int *__re
         b[o] = sb;
         const double di = sd != 0.0 ? 1.0 / sd : 1.0;
         dinv[o] = di;
-        dinv32[o] = (float)di;
+        dinv32[o] = f_to_bf16((float)di);
     }
 }
 
@@ -474,7 +559,7 @@ __global__ void __launch_bounds__(256) k_b_init(BArgs<V> a, const double *__rest
         const size_t o = (size_t)row * a.Rp + r;
         const double b = b64[o];
         const V bv = (V)b;
-        const V z = a.dinv[o] * bv;
+        const V z = Pre<V>::ld(a.dinv, o) * bv;
         a.x[o] = V(0);
         if (x64) x64[o] = 0.0;
         a.r[o] = bv;
@@ -537,7 +622,7 @@ __global__ void __launch_bounds__(256) k_b_update(BArgs<V> a) {
                 a.x[o] += alpha * p[o];
                 const V rv = a.r[o] - alpha * a.q[o];
                 a.r[o] = rv;
-                const V z = a.dinv[o] * rv;
+                const V z = Pre<V>::ld(a.dinv, o) * rv;
                 a.z[o] = z;
                 v[0] += (double)rv * (double)z;
                 v[1] += (double)rv * (double)rv;
@@ -557,19 +642,39 @@ __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
     return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < kRuDrop2 * sc.rr_ru[r]);
 }
 
+// Lanes per edge row of the two-level iteration kernels.  W = 64: one row per
+// wave, lane = slot of the block's 64-slot chunk.  Once the queue is empty
+// and at most W realizations are left, all in slots [0, W), the kernels run
+// with W = 16 or 4: 64 / W rows per wave, lane = row offset * W + slot, so the
+// tail of a batch does not sweep every row with idle lanes.
+template <int W>
+struct LaneMap {
+    static constexpr int P = 64 / W;  // rows per wave
+    __device__ static int slot(int lane) { return W == 64 ? (int)blockIdx.y * 64 + lane : (lane & (W - 1)); }
+    __device__ static int sub(int lane) { return W == 64 ? 0 : lane / W; }
+    // sum over the lanes of one slot (every such lane gets the same sum)
+    __device__ static double fold(double x) {
+        for (int m = W; m < 64; m <<= 1) x += __shfl_xor(x, m, 64);
+        return x;
+    }
+};
+
 // mixed precision: x64 += x32, x32 = 0 for the selected realizations
 // (lsel: the listed slots instead, when they leave the FP32 phase)
+template <int W>
 __global__ void __launch_bounds__(256) k_b_accum(int F, int G, int xcd, size_t Rp, BatchScal sc,
                                                  const int *__restrict__ lsel, double *__restrict__ x64,
                                                  float *__restrict__ x32) {
+    using L = LaneMap<W>;
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.y * 64 + lane;
+    const int r = L::slot(lane), sub = L::sub(lane);
     const bool act = lsel ? lsel[r] != 0 : ru_pick(sc, r, sc.status[r]);
     if (!__any(act)) return;
     int s0, st, en;
-    row_range(F, G, xcd, s0, st, en);
-    for (int row = s0; row < en; row += st) {
-        if (act) {
+    row_range((F + L::P - 1) / L::P, G, xcd, s0, st, en);
+    for (int g = s0; g < en; g += st) {
+        const int row = g * L::P + sub;
+        if (act && row < F) {
             const size_t o = (size_t)row * Rp + r;
             x64[o] += (double)x32[o];
             x32[o] = 0.0f;
@@ -605,7 +710,7 @@ __global__ void __launch_bounds__(256) k_b_resid(BArgs<V> a, int sel, const int 
                 v[1] += rr * rr;
                 if (MODE != 2) {
                     const V rv = (V)rr;
-                    const V z = a.dinv[o] * rv;
+                    const V z = Pre<V>::ld(a.dinv, o) * rv;
                     a.r[o] = rv;
                     a.z[o] = z;
                     v[0] += (double)rv * (double)z;
@@ -739,6 +844,40 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
     }
 }
 
+// First stage of a fold over many block partials: block (s, chunk) sums rows
+// [s G / kPre, (s + 1) G / kPre) of each given array in a fixed order, so the
+// per-slot fold reads kPre rows instead of G (a G-row chain on one CU per
+// chunk is latency-bound).
+constexpr int kPre = 32;
+__global__ void __launch_bounds__(256) k_b_prefold(const double *__restrict__ p0, const double *__restrict__ p1,
+                                                   const double *__restrict__ p2, int G, int Rp,
+                                                   double *__restrict__ o0, double *__restrict__ o1,
+                                                   double *__restrict__ o2) {
+    __shared__ double sh[3][kWaves][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int slot = blockIdx.y * 64 + lane;
+    const int g0 = blockIdx.x * G / kPre, g1 = (blockIdx.x + 1) * G / kPre;
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int g = g0 + wid; g < g1; g += kWaves) {
+        const size_t o = (size_t)g * Rp + slot;
+        if (p0) s[0] += p0[o];
+        if (p1) s[1] += p1[o];
+        if (p2) s[2] += p2[o];
+    }
+    for (int k = 0; k < 3; k++) sh[k][wid][lane] = s[k];
+    __syncthreads();
+    if (wid != 0) return;
+    double *const out[3] = {o0, o1, o2};
+    const double *const in[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; k++) {
+        if (!in[k]) continue;
+        double t = sh[k][0][lane];
+        for (int w = 1; w < kWaves; w++) t += sh[k][w][lane];
+        out[k][(size_t)blockIdx.x * Rp + slot] = t;
+    }
+}
+
 // Reconstruction of element pressure and side fluxes (schur.cc:714-773) for
 // the listed (slot, realization) pairs: edge traces lam and K read from the
 // slot, output written to the realization's column, sol[idx * Rout + real].
@@ -839,7 +978,7 @@ constexpr int kNbMax = 32;  // neighbour aggregates per aggregate (27 on a regul
 struct CArgs {
     int ncol, nagg, nc;
     const int *agg, *aptr, *nbptr, *nb, *lbmap;  // lbmap[a * nagg + b]: local index of b among a's neighbours
-    const float *phi;                            // [F][Rp] (ncol 2) or null
+    const unsigned short *phi;                   // [F][Rp] bfloat16 (ncol 2) or null
     const float *ainv;                           // [Rp][nc][nc]
     double *rc, *qc, *y;                         // [nc][Rp]
     double *part3;                               // [nagg][Rp]
@@ -875,7 +1014,7 @@ __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, con
     if (__any(act)) {
         for (int row = c.aptr[ag]; row < c.aptr[ag + 1]; row++) {
             if (!act) continue;
-            const double zf1 = NCOL == 2 ? (double)c.phi[(size_t)row * a.Rp + slot] : 0.0;
+            const double zf1 = NCOL == 2 ? (double)bf16_to_f(c.phi[(size_t)row * a.Rp + slot]) : 0.0;
             for (int cc = a.row_ptr[row]; cc < a.row_ptr[row + 1]; cc++) {
                 const int code = a.row_c[cc];
                 const int e = code >> 4, i = code & 15;
@@ -891,7 +1030,7 @@ __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, con
                     if constexpr (NCOL == 1) {
                         acc[lb][0][lane] += sij;
                     } else {
-                        const double zg1 = (double)c.phi[(size_t)g * a.Rp + slot];
+                        const double zg1 = (double)bf16_to_f(c.phi[(size_t)g * a.Rp + slot]);
                         acc[lb][0][lane] += sij;
                         acc[lb][1][lane] += sij * zg1;
                         acc[lb][2][lane] += zf1 * sij;
@@ -942,7 +1081,7 @@ __global__ void k_c_unpack(int nc, const double *__restrict__ Apack, const int *
 
 // phi of the selected slots: per aggregate zero mean and unit mean square
 // (keeps the two coarse columns independent and scaled alike)
-__global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, float *__restrict__ phi) {
+__global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, unsigned short *__restrict__ phi) {
     const int lane = threadIdx.x;
     const int slot = blockIdx.y * 64 + lane;
     const int ag = blockIdx.x;
@@ -950,7 +1089,7 @@ __global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, f
     const int r0 = c.aptr[ag], r1 = c.aptr[ag + 1];
     double s1 = 0.0, s2 = 0.0;
     for (int row = r0; row < r1; row++) {
-        const double v = phi[(size_t)row * Rp + slot];
+        const double v = bf16_to_f(phi[(size_t)row * Rp + slot]);
         s1 += v;
         s2 += v * v;
     }
@@ -959,17 +1098,34 @@ __global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, f
     const double sc = var > 1e-24 * (s2 / n + 1e-300) ? 1.0 / sqrt(var) : 0.0;
     for (int row = r0; row < r1; row++) {
         const size_t o = (size_t)row * Rp + slot;
-        phi[o] = (float)(((double)phi[o] - mean) * sc);
+        phi[o] = f_to_bf16((float)(((double)bf16_to_f(phi[o]) - mean) * sc));
     }
 }
 
 // Shared tail of the aggregate-blocked kernels: block reduction of three
-// partials and the 2 x ncol restrictions, written per aggregate.
-template <int NCOL>
-__device__ __forceinline__ void agg_part(const double (&v)[3], const double (&rc)[2], const double (&qc)[2],
-                                         double *p1, double *p2, double *p3, CArgs &c, size_t Rp, int slot) {
-    __shared__ double sh[3 + 2 * NCOL][kWaves][64];
+// partials and the 2 x ncol restrictions, written per aggregate.  With W < 64
+// the lanes of one slot are summed first; the other lanes write zeros into the
+// partials of slots that are idle in that mode.
+template <int NCOL, int W = 64>
+__device__ __forceinline__ void agg_part(const double (&vin)[3], const double (&rcin)[2], const double (&qcin)[2],
+                                         double *p1, double *p2, double *p3, CArgs &c, size_t Rp) {
+    __shared__ double sh[3 + 2 * NCOL][kAggWaves][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double v[3] = {vin[0], vin[1], vin[2]}, rc[2] = {rcin[0], rcin[1]}, qc[2] = {qcin[0], qcin[1]};
+    if (W < 64) {
+        const bool keep = LaneMap<W>::sub(lane) == 0;
+        // every lane takes part in the shuffles; only then the selection
+        for (int k = 0; k < 3; k++) {
+            const double f = LaneMap<W>::fold(v[k]);
+            v[k] = keep ? f : 0.0;
+        }
+        for (int s = 0; s < NCOL; s++) {
+            const double f = LaneMap<W>::fold(rc[s]), h = LaneMap<W>::fold(qc[s]);
+            rc[s] = keep ? f : 0.0;
+            qc[s] = keep ? h : 0.0;
+        }
+    }
+    const int slot = blockIdx.y * 64 + lane;
     for (int k = 0; k < 3; k++) sh[k][wid][lane] = v[k];
     for (int s = 0; s < NCOL; s++) {
         sh[3 + s][wid][lane] = rc[s];
@@ -980,7 +1136,7 @@ __device__ __forceinline__ void agg_part(const double (&v)[3], const double (&rc
         double t[3 + 2 * NCOL];
         for (int k = 0; k < 3 + 2 * NCOL; k++) {
             double x = sh[k][0][lane];
-            for (int w = 1; w < kWaves; w++) x += sh[k][w][lane];
+            for (int w = 1; w < kAggWaves; w++) x += sh[k][w][lane];
             t[k] = x;
         }
         const size_t o = (size_t)blockIdx.x * Rp + slot;
@@ -994,44 +1150,88 @@ __device__ __forceinline__ void agg_part(const double (&v)[3], const double (&rc
     }
 }
 
-// x += alpha p; r -= alpha q; partials D^-1 r.r, r.r, D^-1 r.q and the
-// restrictions Z^T r, Z^T q, one block per (aggregate, 64 slots)
-template <class V, int NCOL>
-__global__ void __launch_bounds__(256) k_c_update(BArgs<V> a, CArgs c) {
+// r -= alpha q; partials D^-1 r.r, r.r, D^-1 r.q and the restrictions Z^T r,
+// Z^T q, one block per (aggregate, 64 slots).  x += alpha p is done by the
+// direction kernel k_c_zp, which reads p anyway.
+template <class V, int NCOL, int W, int U>
+__global__ void __launch_bounds__(kAggWaves * 64) k_c_update(BArgs<V> a, CArgs c) {
+    using L = LaneMap<W>;
     const int lane = threadIdx.x & 63;
-    const int slot = blockIdx.y * 64 + lane;
+    const int slot = L::slot(lane), sub = L::sub(lane);
     const bool act = a.sc.status[slot] == kRun;
     double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
-    if (__any(act)) {
+    if (U == 2 && W == 64 && __any(act)) {
+        // two rows per step, all loads first (row and row + kAggWaves)
         const V alpha = act ? (V)a.sc.alpha[slot] : V(0);
-        const V *p = a.p0;  // the two-level path keeps its direction in place
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
+        const int r1 = c.aptr[blockIdx.x + 1];
+        for (int row = c.aptr[blockIdx.x] + wid; row < r1; row += 2 * kAggWaves) {
             if (!act) continue;
+            const bool two = row + kAggWaves < r1;
+            const size_t o0 = (size_t)row * a.Rp + slot, o1 = two ? o0 + (size_t)kAggWaves * a.Rp : o0;
+            const V q0 = a.q[o0], q1 = a.q[o1];
+            const V r0 = a.r[o0], r1v = a.r[o1];
+            const V d0 = Pre<V>::ld(a.dinv, o0), d1 = Pre<V>::ld(a.dinv, o1);
+            double f0 = 0.0, f1 = 0.0;
+            if constexpr (NCOL == 2) {
+                f0 = (double)bf16_to_f(c.phi[o0]);
+                f1 = (double)bf16_to_f(c.phi[o1]);
+            }
+            const V rv0 = r0 - alpha * q0, rv1 = r1v - alpha * q1;
+            a.r[o0] = rv0;
+            const V dr0 = d0 * rv0, dr1 = d1 * rv1;
+            v[0] += (double)dr0 * (double)rv0;
+            v[1] += (double)rv0 * (double)rv0;
+            v[2] += (double)dr0 * (double)q0;
+            rc[0] += (double)rv0;
+            qc[0] += (double)q0;
+            if constexpr (NCOL == 2) {
+                rc[1] += f0 * (double)rv0;
+                qc[1] += f0 * (double)q0;
+            }
+            if (two) {
+                a.r[o1] = rv1;
+                v[0] += (double)dr1 * (double)rv1;
+                v[1] += (double)rv1 * (double)rv1;
+                v[2] += (double)dr1 * (double)q1;
+                rc[0] += (double)rv1;
+                qc[0] += (double)q1;
+                if constexpr (NCOL == 2) {
+                    rc[1] += f1 * (double)rv1;
+                    qc[1] += f1 * (double)q1;
+                }
+            }
+        }
+    } else if (__any(act)) {
+        const V alpha = act ? (V)a.sc.alpha[slot] : V(0);
+        const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int r1 = c.aptr[blockIdx.x + 1];
+        for (int g = c.aptr[blockIdx.x] + wid * L::P; g < r1; g += kAggWaves * L::P) {
+            const int row = g + sub;
+            if (!act || row >= r1) continue;
             const size_t o = (size_t)row * a.Rp + slot;
-            a.x[o] += alpha * p[o];
             const V q = a.q[o];
             const V rv = a.r[o] - alpha * q;
             a.r[o] = rv;
-            const V dr = a.dinv[o] * rv;
+            const V dr = Pre<V>::ld(a.dinv, o) * rv;
             v[0] += (double)dr * (double)rv;
             v[1] += (double)rv * (double)rv;
             v[2] += (double)dr * (double)q;
             rc[0] += (double)rv;
             qc[0] += (double)q;
             if constexpr (NCOL == 2) {
-                const double ph = (double)c.phi[o];
+                const double ph = (double)bf16_to_f(c.phi[o]);
                 rc[1] += ph * (double)rv;
                 qc[1] += ph * (double)q;
             }
         }
     }
-    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+    agg_part<NCOL, W>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
 }
 
 // cold start of the selected slots: x = 0, r = b, p = 0; partials and Z^T b
 template <class V, int NCOL>
-__global__ void __launch_bounds__(256) k_c_init(BArgs<V> a, CArgs c, const double *__restrict__ b64, double *x64,
+__global__ void __launch_bounds__(kAggWaves * 64) k_c_init(BArgs<V> a, CArgs c, const double *__restrict__ b64, double *x64,
                                                 const int *__restrict__ lsel) {
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.y * 64 + lane;
@@ -1039,7 +1239,7 @@ __global__ void __launch_bounds__(256) k_c_init(BArgs<V> a, CArgs c, const doubl
     double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
     if (__any(act)) {
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
+        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kAggWaves) {
             if (!act) continue;
             const size_t o = (size_t)row * a.Rp + slot;
             const double b = b64[o];
@@ -1049,141 +1249,273 @@ __global__ void __launch_bounds__(256) k_c_init(BArgs<V> a, CArgs c, const doubl
             a.r[o] = bv;
             a.p0[o] = V(0);
             a.p1[o] = V(0);
-            v[0] += (double)(a.dinv[o] * bv) * (double)bv;
+            v[0] += (double)(Pre<V>::ld(a.dinv, o) * bv) * (double)bv;
             v[1] += b * b;
             rc[0] += (double)bv;
-            if (NCOL == 2) rc[1] += (double)c.phi[o] * (double)bv;
+            if (NCOL == 2) rc[1] += (double)bf16_to_f(c.phi[o]) * (double)bv;
         }
     }
-    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
 }
 
 // True FP64 residual of the reliable-update selection (mixed precision):
 // r = b - S64 x64 stored in V, partials and Z^T r
-template <class V, int NCOL>
-__global__ void __launch_bounds__(256) k_c_resid(BArgs<V> a, CArgs c, const double *__restrict__ b64,
+template <class V, int NCOL, int W>
+__global__ void __launch_bounds__(kAggWaves * 64) k_c_resid(BArgs<V> a, CArgs c, const double *__restrict__ b64,
                                                  const double *__restrict__ x64) {
+    using L = LaneMap<W>;
     const int lane = threadIdx.x & 63;
-    const int slot = blockIdx.y * 64 + lane;
+    const int slot = L::slot(lane), sub = L::sub(lane);
     const bool act = ru_pick(a.sc, slot, a.sc.status[slot]);
     double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
     if (__any(act)) {
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const XPlain<double> xs{x64};
-        for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kWaves) {
-            if (!act) continue;
+        const int r1 = c.aptr[blockIdx.x + 1];
+        for (int g = c.aptr[blockIdx.x] + wid * L::P; g < r1; g += kAggWaves * L::P) {
+            const int row = g + sub;
+            if (!act || row >= r1) continue;
             const double ax = op_row<double>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
                                              a.opT64, a.K64, a.opT64, a.K64, row, xs, a.Rp, slot);
             const size_t o = (size_t)row * a.Rp + slot;
             const double rr = b64[o] - ax;
             const V rv = (V)rr;
             a.r[o] = rv;
-            v[0] += (double)(a.dinv[o] * rv) * (double)rv;
+            v[0] += (double)(Pre<V>::ld(a.dinv, o) * rv) * (double)rv;
             v[1] += rr * rr;
             rc[0] += (double)rv;
-            if (NCOL == 2) rc[1] += (double)c.phi[o] * (double)rv;
+            if (NCOL == 2) rc[1] += (double)bf16_to_f(c.phi[o]) * (double)rv;
         }
     }
-    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp, slot);
+    agg_part<NCOL, W>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
 }
 
 // q = S p for the two-level path (the direction is formed by k_c_zp); partial p.q
-template <class V>
+// PAIR 0: general row path; 1: tetrahedron-pair rows with all gathers issued
+// first.
+template <class V, int W, int PAIR>
 __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
+    using L = LaneMap<W>;
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.y * 64 + lane;
+    const int r = L::slot(lane), sub = L::sub(lane);
     const bool act = a.sc.status[r] == kRun;
     double v[1] = {0.0};
     if (__any(act)) {
         const XPlain<V> xs{a.p0};
         int s0, st, en;
-        row_range(a.F, a.G, a.xcd, s0, st, en);
-        for (int row = s0; row < en; row += st) {
-            if (act) {
-                const V q = op_row<V>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
-                                      a.opT64, a.K64, row, xs, a.Rp, r);
+        row_range((a.F + L::P - 1) / L::P, a.G, a.xcd, s0, st, en);
+        for (int g = s0; g < en; g += st) {
+            const int row = g * L::P + sub;
+            if (act && row < a.F) {
+                const int *R = a.rec + (size_t)row * kRecInts;
+                V q;
+                if (PAIR != 0 && R[0] >= 0 && R[2] == 4 && R[8] >= 0 && R[10] == 4)
+                    q = op_row_pair<V>(R, a.opT, a.K, xs, a.Rp, r);
+                else
+                    q = op_row<V>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
+                                  a.opT64, a.K64, row, xs, a.Rp, r);
                 const size_t o = (size_t)row * a.Rp + r;
                 a.q[o] = q;
                 v[0] += (double)a.p0[o] * (double)q;
             }
         }
     }
+    if (W < 64) {
+        const double f = L::fold(v[0]);
+        v[0] = sub == 0 ? f : 0.0;
+    }
     double *const dst[1] = {a.part0};
     block_part<1>(v, dst, a.Rp);
 }
 
-// New direction of the slots flagged by the last fold, in place:
-// p = z + beta p with z = D^-1 r + Z y (two-level preconditioner)
-template <class V, int NCOL>
-__global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
-    const int lane = threadIdx.x & 63;
-    const int slot = blockIdx.y * 64 + lane;
-    const bool act = a.sc.zp[slot] != 0;
-    if (!__any(act)) return;
-    const V beta = act ? (V)a.sc.beta[slot] : V(0);
-    const XTwoLevel<V> xs{a.r, a.dinv, a.p0, NCOL == 2 ? c.phi : nullptr, c.y, c.agg, beta, (size_t)a.Rp, slot};
-    int s0, st, en;
-    row_range(a.F, a.G, a.xcd, s0, st, en);
-    for (int row = s0; row < en; row += st) {
-        if (!act) continue;
-        const size_t o = (size_t)row * a.Rp + slot;
-        a.p0[o] = xs(row, o);
+// FP32 operator with one row per wave for CH 64-slot chunks at once (grid
+// (G, chunks / CH)): the row's record is read once for all CH chunks, the
+// pair rows take their block rows from the inlined records, and the next
+// row's record is loaded while the current row's gathers are in flight.
+// Partial p.q per (block, slot) as k_c_op.
+template <int CH>
+__global__ void __launch_bounds__(256) k_c_op2(BArgs<float> a, CArgs c) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int sl[CH];
+    bool act[CH];
+    bool any = false;
+#pragma unroll
+    for (int h = 0; h < CH; h++) {
+        sl[h] = (blockIdx.y * CH + h) * 64 + lane;
+        act[h] = a.sc.status[sl[h]] == kRun;
+        any = any || act[h];
+    }
+    double v[CH];
+#pragma unroll
+    for (int h = 0; h < CH; h++) v[h] = 0.0;
+    if (__any(any)) {
+        // record words live in lanes 0..19 of one register (vector loads,
+        // counted in order) and are broadcast with readlane
+        const float *__restrict__ p = a.p0;
+        const float *__restrict__ Kv = a.K;
+        const size_t Rp = a.Rp;
+        int s0, st, en;
+        row_range(a.F, a.G, a.xcd, s0, st, en);
+        const int wl = lane < kFastRec ? lane : 0;
+        int cur = s0 < en ? a.frec[(size_t)s0 * kFastRec + wl] : 0;
+        for (int row = s0; row < en; row += st) {
+            const int nrow = row + st;
+            const int nxt = nrow < en ? a.frec[(size_t)nrow * kFastRec + wl] : 0;
+            int C[kFastRec];
+#pragma unroll
+            for (int k = 0; k < kFastRec; k++) C[k] = __builtin_amdgcn_readlane(cur, k);
+            if (C[0] >= 0) {
+                float xr[CH], K1[CH], K2[CH], xa[CH][4], xb[CH][4];
+#pragma unroll
+                for (int h = 0; h < CH; h++) {
+                    if (!act[h]) continue;
+                    xr[h] = p[(size_t)row * Rp + sl[h]];
+                    K1[h] = Kv[(size_t)C[0] * Rp + sl[h]];
+                    K2[h] = Kv[(size_t)C[1] * Rp + sl[h]];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        xa[h][j] = p[(size_t)C[2 + j] * Rp + sl[h]];
+                        xb[h][j] = p[(size_t)C[6 + j] * Rp + sl[h]];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < CH; h++) {
+                    if (!act[h]) continue;
+                    float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        t1 += __int_as_float(C[10 + j]) * xa[h][j];
+                        t2 += __int_as_float(C[14 + j]) * xb[h][j];
+                    }
+                    float q = K1[h] * t1 + __int_as_float(C[18]) * xr[h];
+                    q += K2[h] * t2 + __int_as_float(C[19]) * xr[h];
+                    a.q[(size_t)row * Rp + sl[h]] = q;
+                    v[h] += (double)xr[h] * (double)q;
+                }
+            } else {
+                const XPlain<float> xs{p};
+#pragma unroll
+                for (int h = 0; h < CH; h++) {
+                    if (!act[h]) continue;
+                    const float q = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
+                                                  a.opT, a.K, a.opT64, a.K64, row, xs, Rp, sl[h]);
+                    const size_t o = (size_t)row * Rp + sl[h];
+                    a.q[o] = q;
+                    v[h] += (double)p[o] * (double)q;
+                }
+            }
+            cur = nxt;
+        }
+    }
+    __shared__ double sh[CH][kWaves][64];
+#pragma unroll
+    for (int h = 0; h < CH; h++) sh[h][wid][lane] = v[h];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            double t = sh[h][0][lane];
+            for (int w = 1; w < kWaves; w++) t += sh[h][w][lane];
+            a.part0[(size_t)blockIdx.x * a.Rp + sl[h]] = t;
+        }
     }
 }
 
-// Per-slot fold of the aggregate partials, coarse solve y = (Z^T S Z)^-1 Z^T r
-// (FP32 inverse, FP64 accumulation), then the scalar step of CG for the slot.
-// One block per slot.  MODE: kFoldInit (selected slots), kFoldBeta (running
-// slots, flexible beta), kFoldRU (reliable-update selection).
-constexpr int kCoarseThreads = 1024;  // 16 waves per slot: the coarse inverse streams at full rate
+// Per slot, by the zp flags of the last fold: bit 1 x += alpha p (the
+// iteration's own step), bit 0 the new direction in place, p = z + beta p
+// with z = D^-1 r + Z y (two-level preconditioner)
+template <class V, int NCOL, int W>
+__global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
+    using L = LaneMap<W>;
+    const int lane = threadIdx.x & 63;
+    const int slot = L::slot(lane), sub = L::sub(lane);
+    const int fl = a.sc.zp[slot];
+    if (!__any(fl != 0)) return;
+    const V beta = (fl & 1) ? (V)a.sc.beta[slot] : V(0);
+    const V alpha = (fl & 2) ? (V)a.sc.alpha[slot] : V(0);
+    const XTwoLevel<V> xs{a.r, a.dinv, a.p0, NCOL == 2 ? c.phi : nullptr, c.y, c.agg, beta, (size_t)a.Rp, slot};
+    int s0, st, en;
+    row_range((a.F + L::P - 1) / L::P, a.G, a.xcd, s0, st, en);
+    for (int g = s0; g < en; g += st) {
+        const int row = g * L::P + sub;
+        if (fl == 0 || row >= a.F) continue;
+        const size_t o = (size_t)row * a.Rp + slot;
+        const V pold = a.p0[o];
+        if (fl & 2) a.x[o] += alpha * pold;
+        if (fl & 1) a.p0[o] = xs.z(row, o) + beta * pold;
+    }
+}
+
+// Coarse solve y = (Z^T S Z)^-1 Z^T r of the slots the fold will step (FP32
+// inverse, FP64 accumulation), split over row blocks of the inverse so that
+// many CUs stream it: block (part, slot) forms the kCoarseThreads / QS rows
+// [part * rows, (part + 1) * rows) of y, QS threads per row (each one QS-th of
+// the columns), and writes its share of the per-slot sums
+//   0: D^-1 r . r   1: r . r   2: D^-1 r . q   3: y . Z^T r   4: y . Z^T q
+// to cpart[(k * parts + part) * Rp + slot].  QS = 4 with all slots busy;
+// QS = 16 for the few slots of a narrowed tail (more, shorter blocks per
+// slot).  MODE: kFoldInit (selected slots), kFoldBeta (running slots),
+// kFoldRU (reliable-update selection).
+constexpr int kCoarseThreads = 1024;
+constexpr int kMaxCoarseParts = 2048 / (kCoarseThreads / 16);  // nc <= 2048, QS <= 16
 
 template <int MODE>
-__global__ void __launch_bounds__(kCoarseThreads) k_c_fold(BatchScal sc, CArgs c, const double *__restrict__ part1,
-                                                const double *__restrict__ part2, size_t Rp, double rtol2,
-                                                double atol2, int max_it, int mixed, const int *__restrict__ lsel) {
-    __shared__ double srx[2048], sqx[2048];
-    __shared__ double red[5][kCoarseThreads / 64];
-    const int slot = blockIdx.x;
+__device__ __forceinline__ bool coarse_act(const BatchScal &sc, int slot, const int *__restrict__ lsel) {
     const int st0 = sc.status[slot];
-    bool act;
-    if (MODE == kFoldInit) act = lsel[slot] != 0;
-    else if (MODE == kFoldBeta) act = st0 == kRun;
-    else act = ru_pick(sc, slot, st0);
-    if (!act) {  // block-uniform
-        if (threadIdx.x == 0) sc.zp[slot] = 0;
-        return;
-    }
+    if (MODE == kFoldInit) return lsel[slot] != 0;
+    if (MODE == kFoldBeta) return st0 == kRun;
+    return ru_pick(sc, slot, st0);
+}
+
+template <int MODE, int QS>
+__global__ void __launch_bounds__(kCoarseThreads) k_c_solve(BatchScal sc, CArgs c, const double *__restrict__ part1,
+                                                            const double *__restrict__ part2, size_t Rp,
+                                                            const int *__restrict__ lsel, double *__restrict__ cpart) {
+    constexpr int kRows = kCoarseThreads / QS;
+    __shared__ double srx[2048];
+    __shared__ double ys[QS][kRows];
+    __shared__ double red[5][kCoarseThreads / 64];
+    const int part = blockIdx.x, nparts = gridDim.x, slot = blockIdx.y;
+    if (!coarse_act<MODE>(sc, slot, lsel)) return;  // block-uniform
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int g = tid; g < c.nagg; g += blockDim.x) {
-        const size_t o = (size_t)g * Rp + slot;
-        s[0] += part1[o];
-        s[1] += part2[o];
-        if (MODE == kFoldBeta) s[2] += c.part3[o];
-    }
-    for (int i = tid; i < c.nc; i += blockDim.x) {
-        srx[i] = c.rc[(size_t)i * Rp + slot];
-        sqx[i] = MODE == kFoldBeta ? c.qc[(size_t)i * Rp + slot] : 0.0;
-    }
-    __syncthreads();
-    const float *Ai = c.ainv + (size_t)slot * c.nc * c.nc;
     const int nc = c.nc;
-    for (int i = tid; i < nc; i += blockDim.x) {
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-        int j = 0;
-        for (; j + 4 <= nc; j += 4) {
-            const float a0 = Ai[(size_t)j * nc + i], a1 = Ai[(size_t)(j + 1) * nc + i];
-            const float a2 = Ai[(size_t)(j + 2) * nc + i], a3 = Ai[(size_t)(j + 3) * nc + i];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    {  // this block's share of the aggregate partials
+        const int g0 = part * c.nagg / nparts, g1 = (part + 1) * c.nagg / nparts;
+        for (int g = g0 + tid; g < g1; g += kCoarseThreads) {
+            const size_t o = (size_t)g * Rp + slot;
+            s[0] += part1[o];
+            s[1] += part2[o];
+            if (MODE == kFoldBeta) s[2] += c.part3[o];
+        }
+    }
+    for (int i = tid; i < nc; i += kCoarseThreads) srx[i] = c.rc[(size_t)i * Rp + slot];
+    __syncthreads();
+    const int il = tid % kRows, q = tid / kRows;
+    const int i = part * kRows + il;
+    const int jq = (nc + QS - 1) / QS, j0 = q * jq, j1 = min(nc, j0 + jq);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    if (i < nc) {
+        const float *Ai = c.ainv + (size_t)slot * nc * nc + i;  // column i = row i (symmetric)
+        int j = j0;
+        for (; j + 4 <= j1; j += 4) {
+            const float a0 = Ai[(size_t)j * nc], a1 = Ai[(size_t)(j + 1) * nc];
+            const float a2 = Ai[(size_t)(j + 2) * nc], a3 = Ai[(size_t)(j + 3) * nc];
             y0 += (double)a0 * srx[j];
             y1 += (double)a1 * srx[j + 1];
             y2 += (double)a2 * srx[j + 2];
             y3 += (double)a3 * srx[j + 3];
         }
-        for (; j < nc; j++) y0 += (double)Ai[(size_t)j * nc + i] * srx[j];
-        const double yi = (y0 + y1) + (y2 + y3);
+        for (; j < j1; j++) y0 += (double)Ai[(size_t)j * nc] * srx[j];
+    }
+    ys[q][il] = (y0 + y1) + (y2 + y3);
+    __syncthreads();
+    if (q == 0 && i < nc) {
+        double yi = ys[0][il];
+        for (int k = 1; k < QS; k++) yi += ys[k][il];
         c.y[(size_t)i * Rp + slot] = yi;
         s[3] += yi * srx[i];
-        s[4] += yi * sqx[i];
+        if (MODE == kFoldBeta) s[4] += yi * c.qc[(size_t)i * Rp + slot];
     }
     for (int k = 0; k < 5; k++) {
         double x = s[k];
@@ -1191,11 +1523,30 @@ __global__ void __launch_bounds__(kCoarseThreads) k_c_fold(BatchScal sc, CArgs c
         if (lane == 0) red[k][wid] = x;
     }
     __syncthreads();
-    if (tid != 0) return;
+    if (tid < 5) {
+        double x = 0.0;
+        for (int w = 0; w < kCoarseThreads / 64; w++) x += red[tid][w];
+        cpart[((size_t)tid * nparts + part) * Rp + slot] = x;
+    }
+}
+
+// Scalar step of CG per slot from the coarse-solve sums (one lane per slot).
+// zp flags for the direction kernel: bit 0 re-form the direction, bit 1 add
+// alpha p to x (the update of x is deferred to the direction kernel, which
+// reads p anyway).
+template <int MODE>
+__global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int nparts, size_t Rp, double rtol2,
+                        double atol2, int max_it, int mixed, const int *__restrict__ lsel) {
+    const int slot = blockIdx.x * 64 + threadIdx.x;
+    const int st0 = sc.status[slot];
+    if (!coarse_act<MODE>(sc, slot, lsel)) {
+        sc.zp[slot] = 0;
+        return;
+    }
     double t[5];
     for (int k = 0; k < 5; k++) {
         double x = 0.0;
-        for (int w = 0; w < kCoarseThreads / 64; w++) x += red[k][w];
+        for (int p = 0; p < nparts; p++) x += cpart[((size_t)k * nparts + p) * Rp + slot];
         t[k] = x;
     }
     const double rz = t[0] + t[3], rr = t[1];
@@ -1248,7 +1599,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_c_fold(BatchScal sc, CArgs c
         else st = kRun;
     }
     sc.status[slot] = st;
-    sc.zp[slot] = st == kRun ? 1 : 0;
+    sc.zp[slot] = (st == kRun ? 1 : 0) | (MODE == kFoldBeta ? 2 : 0);
 }
 
 // slot-major arrays ([slot][n]): out[s] = in[src[s]]
@@ -1320,11 +1671,13 @@ static BArgs<V> make_args(int F, int Rp, int G, int xcd, const int *rec, const i
                           const int *meta,
                           const int *op_off, const int *conn_off, const int *conn, const V *opT, const V *K,
                           const double *opT64, const double *K64,
-                          const V *dinv, V *x, V *r, V *z, V *p0, V *p1, V *q, double *part, const BatchScal &sc) {
+                          const typename Pre<V>::T *dinv, V *x, V *r, V *z, V *p0, V *p1, V *q, double *part,
+                          const BatchScal &sc) {
     BArgs<V> a;
     a.F = F; a.Rp = Rp; a.G = G; a.xcd = xcd;
     a.row_ptr = row_ptr; a.row_c = row_c; a.meta = meta; a.op_off = op_off; a.conn_off = conn_off; a.conn = conn;
     a.rec = rec;
+    a.frec = nullptr;
     a.opT = opT; a.K = K; a.dinv = dinv;
     a.opT64 = opT64; a.K64 = K64;
     a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
@@ -1532,6 +1885,8 @@ void DarcyBatchGPU::build() {
         }
     }
     d_rec_ = b_up(owned_, rec, stream_);
+    d_frec_ = dev_alloc<int>((size_t)F * kFastRec);
+    owned_.push_back(d_frec_);
     std::vector<int> dperm;
     for (int d = 1; d <= 3; d++) {
         for (int e = 0; e < E; e++)
@@ -1562,7 +1917,8 @@ void DarcyBatchGPU::build() {
     d_K32_ = af((size_t)E * Rp_);
     d_b_ = ad(nv);
     d_dinv_ = ad(nv);
-    d_dinv32_ = af(nv);
+    d_dinv32_ = dev_alloc<unsigned short>(nv);
+    owned_.push_back(d_dinv32_);
     d_x_ = ad(nv); d_r_ = ad(nv); d_z_ = ad(nv); d_p0_ = ad(nv); d_p1_ = ad(nv); d_q_ = ad(nv);
     d_sol_ = ad(((size_t)S_ + E + F) * Rout_);
     d_Kall_ = ad((size_t)R_ * E);
@@ -1570,9 +1926,19 @@ void DarcyBatchGPU::build() {
     d_pairs_ = ai(2 * (size_t)Rp_);
     d_Ksave_ = ad(E);
     if (const char *h = std::getenv("FL_BATCH_HANDOVER")) handover_ = std::max(0, std::atoi(h));
+    if (const char *w = std::getenv("FL_BATCH_NARROW")) narrow_ = std::atoi(w) != 0;
+    if (const char *w = std::getenv("FL_BATCH_OPPAIR")) opfast_ = std::min(3, std::max(0, std::atoi(w)));
+    if (const char *w = std::getenv("FL_BATCH_UNROLL")) unroll_ = std::atoi(w) != 0;
+    if (const char *w = std::getenv("FL_BATCH_QS")) {
+        const int q = std::atoi(w);
+        qs_ = (q == 8 || q == 16) ? q : 4;
+    }
+    trace_ = std::getenv("FL_BATCH_TRACE") != nullptr;
     d_lsel_ = ai(Rp_);
     d_ext_ = ad((size_t)S_ + E + F);
     d_part_ = ad(3 * (size_t)kMaxGrid * Rp_);
+    d_pre_ = ad(3 * (size_t)kPre * Rp_);
+    d_cpart_ = ad(5 * (size_t)kMaxCoarseParts * Rp_);
     alloc_scal(sc_);
     alloc_scal(sc2_);
     d_src_ = ai(Rp_);
@@ -1586,6 +1952,10 @@ void DarcyBatchGPU::build() {
             tuned_ = true;
         }
     }
+    if (const char *c = std::getenv("FL_BATCH_CHUNK")) {  // iterations per captured chunk only (shape still tuned)
+        const int cv = std::atoi(c);
+        if (cv >= 1 && cv <= 1024) chunk_ = cv;
+    }
     FL_HIP(hipStreamSynchronize(s));
 }
 
@@ -1595,6 +1965,8 @@ void DarcyBatchGPU::alloc32() {
     auto af = [&](size_t n) { float *p = dev_alloc<float>(n); owned_.push_back(p); return p; };
     d_x32_ = af(nv); d_r32_ = af(nv); d_z32_ = af(nv); d_p32a_ = af(nv); d_p32b_ = af(nv); d_q32_ = af(nv);
     d_sp32_ = af(nv);
+    d_sp16_ = dev_alloc<unsigned short>(nv);
+    owned_.push_back(d_sp16_);
     d_spK32_ = af((size_t)E_ * Rp_);
 }
 
@@ -1642,7 +2014,9 @@ void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
         perm32(d_x32_);
         perm32(d_r32_);
         perm32(d_z32_);
-        perm32(d_dinv32_);
+        hipLaunchKernelGGL(k_b_perm<unsigned short>, gF, blk, 0, stream_, (const unsigned short *)d_dinv32_, d_sp16_,
+                           (size_t)F_, (size_t)Rp_, (const int *)d_src_);
+        std::swap(d_dinv32_, d_sp16_);
         hipLaunchKernelGGL(k_b_perm_p<float>, gF, blk, 0, stream_, (const float *)d_p32a_, (const float *)d_p32b_,
                            d_sp32_, (size_t)F_, (size_t)Rp_, (const int *)d_src_, (const int *)sc_.par);
         std::swap(d_p32a_, d_sp32_);
@@ -1658,7 +2032,8 @@ void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
                            d_spainv_, (size_t)nc_ * nc_, (const int *)d_src_);
         std::swap(d_ainv_, d_spainv_);
         if (ncol_ == 2) {
-            hipLaunchKernelGGL(k_b_perm<float>, gF, blk, 0, stream_, (const float *)d_phi_, d_spphi_, (size_t)F_,
+            hipLaunchKernelGGL(k_b_perm<unsigned short>, gF, blk, 0, stream_, (const unsigned short *)d_phi_, d_spphi_,
+                               (size_t)F_,
                                (size_t)Rp_, (const int *)d_src_);
             std::swap(d_phi_, d_spphi_);
         }
@@ -1723,8 +2098,9 @@ void DarcyBatchGPU::set_coarse(int ncol) {
         rocblas_set_stream(h, stream_);
     }
     if (ncol == 2 && !d_phi_) {
-        d_phi_ = af(nv);
-        d_spphi_ = af(nv);
+        auto ah = [&](size_t n) { unsigned short *p = dev_alloc<unsigned short>(n); owned_.push_back(p); return p; };
+        d_phi_ = ah(nv);
+        d_spphi_ = ah(nv);
     }
     nc_ = ncol * nagg_;
 }
@@ -1814,6 +2190,8 @@ void DarcyBatchGPU::setup_system() {
     }
     hipLaunchKernelGGL(k_b_to_float, dim3(std::min(grid_for(op_total_), 4096)), dim3(kBlock), 0, stream_,
                        (size_t)op_total_, (const double *)d_tab_, d_tab32_);
+    hipLaunchKernelGGL(k_b_fastrec, dim3(grid_for(F_)), dim3(kBlock), 0, stream_, F_, (const int *)d_rec_,
+                       (const float *)d_tab32_, d_frec_);
     FL_HIP(hipGetLastError());
 }
 
@@ -1832,9 +2210,10 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     FL_HIP(hipEventRecord(e0, stream_));
     setup_system();
     FL_HIP(hipEventRecord(e1, stream_));
-    const dim3 blk(256), bfold(kFoldThreads);
+    const dim3 blk(256), bfold(kFoldThreads), blkA(kAggWaves * 64);
     const double rtol2 = rtol * rtol, atol2 = atol * atol;
     nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
+    lw_ = 64;        // lanes per row of the two-level kernels (narrowed for the tail)
     BArgs<double> a64;
     BArgs<float> a32;
     CArgs ca{};
@@ -1851,14 +2230,26 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         a32 = make_args<float>(F_, Rp_, G, X, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_, d_conn_,
                                d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_, d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_,
                                d_q32_, d_part_, sc_);
+        a32.frec = d_frec_;
     };
     rebuild();
     auto grow = [&]() { return dim3(G, nch_); };
     auto fold = [&](int mode, int mixed, const int *lsel) {
         const double *q0 = a64.part0, *q1 = a64.part1, *q2 = a64.part2;
         const dim3 gf(nch_);
-#define FL_FOLD(M)                                                                                                 \
-    hipLaunchKernelGGL(k_b_fold<M>, gf, bfold, 0, stream_, sc_, q0, q1, q2, G, Rp_, R_, rtol2, atol2, max_it, mixed, \
+        int Gf = G;
+        if (G > 2 * kPre) {  // two-stage: kPre partial rows per slot for the fold proper
+            double *o0 = d_pre_, *o1 = d_pre_ + (size_t)kPre * Rp_, *o2 = d_pre_ + 2 * (size_t)kPre * Rp_;
+            hipLaunchKernelGGL(k_b_prefold, dim3(kPre, nch_), blk, 0, stream_, mode == kFoldAlpha ? q0 : nullptr,
+                               (mode != kFoldAlpha && mode != kFoldFinal) ? q1 : nullptr,
+                               mode != kFoldAlpha ? q2 : nullptr, G, Rp_, o0, o1, o2);
+            q0 = o0;
+            q1 = o1;
+            q2 = o2;
+            Gf = kPre;
+        }
+#define FL_FOLD(M)                                                                                                  \
+    hipLaunchKernelGGL(k_b_fold<M>, gf, bfold, 0, stream_, sc_, q0, q1, q2, Gf, Rp_, R_, rtol2, atol2, max_it, mixed, \
                        lsel)
         switch (mode) {
             case kFoldInit: FL_FOLD(kFoldInit); break;
@@ -1873,36 +2264,64 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const dim3 gs(64 * nch_), blk(kCoarseThreads);
-        if (mode == kFoldInit)
-            hipLaunchKernelGGL(k_c_fold<kFoldInit>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
-                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
-        else if (mode == kFoldBeta)
-            hipLaunchKernelGGL(k_c_fold<kFoldBeta>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
-                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
-        else
-            hipLaunchKernelGGL(k_c_fold<kFoldRU>, gs, blk, 0, stream_, sc_, ca, (const double *)a64.part1,
-                               (const double *)a64.part2, (size_t)Rp_, rtol2, atol2, max_it, mixed, lsel);
+        const int qs = lw_ < 64 ? 16 : qs_;
+        const int parts = (nc_ + kCoarseThreads / qs - 1) / (kCoarseThreads / qs);
+        const dim3 gs(parts, 64 * nch_), blkc(kCoarseThreads), gfin(nch_), b64(64);
+        const double *p1 = a64.part1, *p2 = a64.part2;
+#define FL_CFOLD(M)                                                                                                   \
+    if (qs == 16)                                                                                                     \
+        hipLaunchKernelGGL((k_c_solve<M, 16>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);  \
+    else if (qs == 8)                                                                                                 \
+        hipLaunchKernelGGL((k_c_solve<M, 8>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);   \
+    else                                                                                                              \
+        hipLaunchKernelGGL((k_c_solve<M, 4>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);   \
+    hipLaunchKernelGGL(k_c_fin<M>, gfin, b64, 0, stream_, sc_, (const double *)d_cpart_, parts, (size_t)Rp_, rtol2,    \
+                       atol2, max_it, mixed, lsel)
+        if (mode == kFoldInit) {
+            FL_CFOLD(kFoldInit);
+        } else if (mode == kFoldBeta) {
+            FL_CFOLD(kFoldBeta);
+        } else {
+            FL_CFOLD(kFoldRU);
+        }
+#undef FL_CFOLD
     };
+    // two-level kernels at the current lanes per row
+#define FL_LW(...)                      \
+    switch (lw_) {                      \
+        case 16: {                      \
+            constexpr int LW = 16;      \
+            __VA_ARGS__;                \
+        } break;                        \
+        case 4: {                       \
+            constexpr int LW = 4;       \
+            __VA_ARGS__;                \
+        } break;                        \
+        default: {                      \
+            constexpr int LW = 64;      \
+            __VA_ARGS__;                \
+        }                               \
+    }
     auto zp64 = [&]() {
-        if (NC == 1) hipLaunchKernelGGL((k_c_zp<double, 1>), grow(), blk, 0, stream_, a64, ca);
-        else hipLaunchKernelGGL((k_c_zp<double, 2>), grow(), blk, 0, stream_, a64, ca);
+        if (NC == 1) FL_LW(hipLaunchKernelGGL((k_c_zp<double, 1, LW>), grow(), blk, 0, stream_, a64, ca))
+        else FL_LW(hipLaunchKernelGGL((k_c_zp<double, 2, LW>), grow(), blk, 0, stream_, a64, ca))
     };
     auto zp32 = [&]() {
-        if (NC == 1) hipLaunchKernelGGL((k_c_zp<float, 1>), grow(), blk, 0, stream_, a32, ca);
-        else hipLaunchKernelGGL((k_c_zp<float, 2>), grow(), blk, 0, stream_, a32, ca);
+        if (NC == 1) FL_LW(hipLaunchKernelGGL((k_c_zp<float, 1, LW>), grow(), blk, 0, stream_, a32, ca))
+        else FL_LW(hipLaunchKernelGGL((k_c_zp<float, 2, LW>), grow(), blk, 0, stream_, a32, ca))
     };
     // one CG iteration; ev: optional events around the operator, the update and
     // the fold (with the coarse solve and the direction kernel)
     auto iter64 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC) hipLaunchKernelGGL(k_c_op<double>, grow(), blk, 0, stream_, a64, ca);
+        if (NC && opfast_) FL_LW(hipLaunchKernelGGL((k_c_op<double, LW, 1>), grow(), blk, 0, stream_, a64, ca))
+        else if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<double, LW, 0>), grow(), blk, 0, stream_, a64, ca))
         else hipLaunchKernelGGL(k_b_op<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
         fold(kFoldAlpha, 0, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        if (NC == 1) hipLaunchKernelGGL((k_c_update<double, 1>), gagg(), blk, 0, stream_, a64, ca);
-        else if (NC == 2) hipLaunchKernelGGL((k_c_update<double, 2>), gagg(), blk, 0, stream_, a64, ca);
+        if (NC == 1) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 1, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 1, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
+        else if (NC == 2) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 2, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 2, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
         else hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
         if (NC) {
@@ -1915,13 +2334,18 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC) hipLaunchKernelGGL(k_c_op<float>, grow(), blk, 0, stream_, a32, ca);
+        if (NC && opfast_ == 2 && lw_ == 64 && nch_ % 2 == 0)
+            hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ >= 2 && lw_ == 64)
+            hipLaunchKernelGGL(k_c_op2<1>, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 1>), grow(), blk, 0, stream_, a32, ca))
+        else if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 0>), grow(), blk, 0, stream_, a32, ca))
         else hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
         fold(kFoldAlpha, 1, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        if (NC == 1) hipLaunchKernelGGL((k_c_update<float, 1>), gagg(), blk, 0, stream_, a32, ca);
-        else if (NC == 2) hipLaunchKernelGGL((k_c_update<float, 2>), gagg(), blk, 0, stream_, a32, ca);
+        if (NC == 1) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 1, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 1, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
+        else if (NC == 2) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 2, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 2, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
         else hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
         if (NC) {
@@ -1933,18 +2357,20 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         if (ev) FL_HIP(hipEventRecord(ev[4], stream_));
     };
     auto reliable_update = [&]() {
-        hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_, d_x32_);
         if (NC) {
+            FL_LW(hipLaunchKernelGGL(k_b_accum<LW>, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_,
+                                     d_x32_))
             if (NC == 1)
-                hipLaunchKernelGGL((k_c_resid<float, 1>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
-                                   (const double *)d_x_);
+                FL_LW(hipLaunchKernelGGL((k_c_resid<float, 1, LW>), gagg(), blkA, 0, stream_, a32, ca,
+                                         (const double *)d_b_, (const double *)d_x_))
             else
-                hipLaunchKernelGGL((k_c_resid<float, 2>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
-                                   (const double *)d_x_);
+                FL_LW(hipLaunchKernelGGL((k_c_resid<float, 2, LW>), gagg(), blkA, 0, stream_, a32, ca,
+                                         (const double *)d_b_, (const double *)d_x_))
             cfold(kFoldRU, 1, nosel);
             zp32();
             return;
         }
+        hipLaunchKernelGGL(k_b_accum<64>, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, nosel, d_x_, d_x32_);
         hipLaunchKernelGGL((k_b_resid<float, 0>), grow(), blk, 0, stream_, a32, -1, nosel, (const int *)d_op_off_,
                            (const double *)d_tab_, (const double *)d_K_, (const double *)d_b_,
                            (const double *)d_x_);
@@ -1955,7 +2381,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     auto graph = [&](int kind) -> hipGraphExec_t {
         hipGraphExec_t &ex = kind ? g32_ : g64_;
         long long &key = kind ? g32_key_ : g64_key_;
-        const long long want = (long long)max_it * 65536 + nch_;
+        const long long want = ((long long)max_it << 24) | ((long long)nch_ << 8) | lw_;
         if (ex && key == want) return ex;
         if (ex) (void)hipGraphExecDestroy(ex);
         ex = nullptr;
@@ -2005,16 +2431,16 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             st.ms_coarse_setup +=
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
             if (mixed && NC == 1)
-                hipLaunchKernelGGL((k_c_init<float, 1>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                hipLaunchKernelGGL((k_c_init<float, 1>), gagg(), blkA, 0, stream_, a32, ca, (const double *)d_b_,
                                    d_x_, (const int *)d_lsel_);
             else if (mixed)
-                hipLaunchKernelGGL((k_c_init<float, 2>), gagg(), blk, 0, stream_, a32, ca, (const double *)d_b_,
+                hipLaunchKernelGGL((k_c_init<float, 2>), gagg(), blkA, 0, stream_, a32, ca, (const double *)d_b_,
                                    d_x_, (const int *)d_lsel_);
             else if (NC == 1)
-                hipLaunchKernelGGL((k_c_init<double, 1>), gagg(), blk, 0, stream_, a64, ca, (const double *)d_b_,
+                hipLaunchKernelGGL((k_c_init<double, 1>), gagg(), blkA, 0, stream_, a64, ca, (const double *)d_b_,
                                    (double *)nullptr, (const int *)d_lsel_);
             else
-                hipLaunchKernelGGL((k_c_init<double, 2>), gagg(), blk, 0, stream_, a64, ca, (const double *)d_b_,
+                hipLaunchKernelGGL((k_c_init<double, 2>), gagg(), blkA, 0, stream_, a64, ca, (const double *)d_b_,
                                    (double *)nullptr, (const int *)d_lsel_);
             cfold(kFoldInit, mixed, d_lsel_);
             if (mixed) zp32();
@@ -2038,7 +2464,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         set_pairs(pairs);
         const int np = (int)pairs.size() / 2;
         if (mixed)
-            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
+            hipLaunchKernelGGL(k_b_accum<64>, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
                                d_x_, d_x32_);
         hipLaunchKernelGGL((k_b_resid<double, 2>), grow(), blk, 0, stream_, a64, -2, (const int *)d_lsel_,
                            (const int *)d_op_off_, (const double *)d_tab_, (const double *)d_K_,
@@ -2091,7 +2517,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         const auto th0 = std::chrono::steady_clock::now();
         set_pairs(pairs);
         if (mixed_now)
-            hipLaunchKernelGGL(k_b_accum, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
+            hipLaunchKernelGGL(k_b_accum<64>, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
                                d_x_, d_x32_);
         FL_HIP(hipMemcpyAsync(stv.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipMemcpyAsync(itv.data(), sc_.it, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -2150,6 +2576,25 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         rebuild();
         read_status();
     };
+    // two-level tail: live realizations into slots [0, n), then 16 or 4 lanes
+    // per row once n fits
+    auto maybe_narrow = [&](int phase) {
+        std::vector<int> src;
+        src.reserve(Rp_);
+        for (int s = 0; s < Rp_; s++)
+            if (live(s)) src.push_back(s);
+        const int n = (int)src.size();
+        const int w = n <= 4 ? 4 : (n <= 16 ? 16 : 64);
+        if (w >= lw_) return;
+        for (int s = 0; s < Rp_; s++)
+            if (!live(s)) src.push_back(s);
+        permute_slots(src, phase);
+        nch_ = 1;
+        lw_ = w;
+        st.compactions++;
+        rebuild();
+        read_status();
+    };
     // queue: hardest realizations first (largest ln K variance), so the
     // longest solves start early and the final tail is short
     std::vector<int> queue(R_);
@@ -2174,7 +2619,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         refill(pairs, mixed);
     }
     if (!tuned_) {
-        tune();  // times the operator on the filled slots; the CG state is unchanged
+        tune(precision);  // times the operator on the filled slots; the CG state is unchanged
         rebuild();
     }
     // live kernel timing: the first iterations run eagerly between events
@@ -2276,6 +2721,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             read_status();
         }
         if (nrun == 0) break;
+        const bool narrow = narrow_ && NC > 0;
         if (next >= queue.size() && nrun + nf64 <= handover_) {
             std::vector<int> pairs;
             for (int sl = 0; sl < Rp_; sl++)
@@ -2287,6 +2733,11 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             break;
         }
         if (next >= queue.size() && nrun + nf64 <= 64 * (nch_ - 1)) maybe_compact(mixed);
+        if (narrow && next >= queue.size()) maybe_narrow(mixed);
+        if (trace_)
+            std::fprintf(stderr, "batch %.1f ms: running %d, chunks %d, lanes/row %d, queued %zu\n",
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(),
+                         nrun, nch_, lw_, queue.size() - next);
         if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
         FL_HIP(hipGraphLaunch(graph(mixed), stream_));
         st.chunks++;
@@ -2327,7 +2778,7 @@ void DarcyBatchGPU::autotune() {
     (void)solve(1e-30, 0.0, 1, 1);  // fills the slots, tunes, one iteration
 }
 
-void DarcyBatchGPU::tune() {
+void DarcyBatchGPU::tune(int precision) {
     alloc32();
     const int cand_g[] = {512, 1024, 2048};
     float best = 1e30f;
@@ -2340,10 +2791,23 @@ void DarcyBatchGPU::tune() {
             BArgs<float> a = make_args<float>(F_, Rp_, G, x, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_,
                                               d_conn_off_, d_conn_, d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_,
                                               d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_, d_q32_, d_part_, sc_);
+            a.frec = d_frec_;
+            const CArgs ca{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_,
+                           d_qc_, d_y_, d_part3_};
             const dim3 grow(G, nch_), blk(256);
-            hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);  // warm
+            // the operator kernel the iterations will run (it only rewrites q
+            // and, for the one-level kernel, the spare direction buffer)
+            auto op = [&]() {
+                if (ncol_ && opfast_ == 2 && precision == 1 && nch_ % 2 == 0)
+                    hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a, ca);
+                else if (ncol_ && opfast_ >= 2 && precision == 1)
+                    hipLaunchKernelGGL(k_c_op2<1>, grow, blk, 0, stream_, a, ca);
+                else
+                    hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);
+            };
+            op();  // warm
             FL_HIP(hipEventRecord(t0, stream_));
-            for (int k = 0; k < 3; k++) hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);
+            for (int k = 0; k < 3; k++) op();
             FL_HIP(hipEventRecord(t1, stream_));
             FL_HIP(hipEventSynchronize(t1));
             float ms = 0;

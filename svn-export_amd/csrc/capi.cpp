@@ -340,6 +340,13 @@ int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n) {
     });
 }
 
+int fl_darcy_batch_set_narrow(fl_darcy_batch *b, int on) {
+    return guard([&] {
+        b->g->set_narrow(on != 0);
+        return 0;
+    });
+}
+
 int fl_darcy_batch_set_coarse(fl_darcy_batch *b, int ncol) {
     return guard([&] {
         b->g->set_coarse(ncol);

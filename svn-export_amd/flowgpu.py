@@ -86,6 +86,7 @@ SIGNATURES = {
     "fl_darcy_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fl_darcy_batch_autotune": (C.c_int, [C.c_void_p]),
     "fl_darcy_batch_set_handover": (C.c_int, [C.c_void_p, C.c_int]),
+    "fl_darcy_batch_set_narrow": (C.c_int, [C.c_void_p, C.c_int]),
     "fl_darcy_batch_set_coarse": (C.c_int, [C.c_void_p, C.c_int]),
     "fl_darcy_batch_set_coarse_basis": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fl_darcy_batch_aggregates": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -335,6 +336,10 @@ class DarcyBatch:
 
     def set_handover(self, n):
         _check(lib().fl_darcy_batch_set_handover(self.h, int(n)))
+
+    def set_narrow(self, on):
+        """Two-level tail in the batch with 16 / 4 lanes per row (default) instead of the hand-over."""
+        _check(lib().fl_darcy_batch_set_narrow(self.h, 1 if on else 0))
 
     def set_coarse(self, ncol):
         """Two-level preconditioner: 0 Jacobi, 1 constant per aggregate, 2 constant + basis phi."""

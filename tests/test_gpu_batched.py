@@ -262,3 +262,42 @@ def test_batch_two_level(cube6):
         print("  r=%d rel err flux %.2e pressure %.2e trace %.2e; two-column flux %.2e" % (r, *errs, e2), flush=True)
         assert max(errs) <= 1e-8 and e2 <= 1e-8
     g.set_conductivity(d["K"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_two_level_narrow_tail(cube6, precision):
+    """The two-level tail continues in the batch with 16, then 4 lanes per edge
+    row (several rows per wavefront): every realization's solution equals the
+    one of the full-width run and of the single FP64 solve."""
+    om, gm, d, g = cube6
+    R = 80
+    K = cases.mc_conductivity(om, R, seed=12)
+    b = G.DarcyBatch(g, R, slots=64)
+    b.set_conductivity(K)
+    b.set_handover(0)
+    b.set_coarse(1)
+    b.set_narrow(False)
+    wide = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=precision)
+    S, E = om.n_sides, om.n_elements
+    xw = [b.get_solution_vector(r) for r in range(R)]
+    b.set_narrow(True)
+    nar = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=precision)
+    xn = [b.get_solution_vector(r) for r in range(R)]
+    print("precision %d: iterations wide %d, narrow %d; narrowings+compactions %d / %d" % (
+        precision, wide["total_iterations"], nar["total_iterations"], nar["compactions"], wide["compactions"]),
+        flush=True)
+    assert wide["converged"].all() and nar["converged"].all()
+    assert nar["compactions"] > wide["compactions"]  # the tail was narrowed (16, then 4 lanes per row)
+    worst = max(rel(a[:S], c[:S]) for a, c in zip(xn, xw))
+    print("  worst flux difference narrow vs wide %.2e" % worst, flush=True)
+    assert worst <= 1e-8
+    hard = int(np.argmax(nar["iterations"]))  # the longest solve ran its end in the narrowest mode
+    for r in (0, hard):
+        g.set_conductivity(K[r])
+        g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=100000)
+        x0 = g.get_solution_vector()
+        errs = [rel(a, c) for a, c in zip(parts(xn[r], S, E), parts(x0, S, E))]
+        print("  r=%d (%d iterations) rel err flux %.2e pressure %.2e trace %.2e" % (
+            r, nar["iterations"][r], *errs), flush=True)
+        assert max(errs) <= 1e-8
+    g.set_conductivity(d["K"])
