@@ -121,7 +121,8 @@ def main():
                     help="two-level preconditioner columns per aggregate: 0 Jacobi, 1 constant, 2 constant + learned")
     ap.add_argument("--train-realizations", type=int, default=128)
     ap.add_argument("--train-epochs", type=int, default=20)
-    ap.add_argument("--handover", type=int, default=16)
+    ap.add_argument("--handover", type=int, default=-1,
+                    help="live realizations handed to the single-solve path (-1: library default)")
     ap.add_argument("--compare", type=int, default=1, help="also time the Jacobi and constant-coarse batch forms")
     ap.add_argument("--plain-sample", type=int, default=16, help="realizations timed on the plain path")
     ap.add_argument("--accuracy-sample", type=int, default=4)

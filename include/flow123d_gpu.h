@@ -197,12 +197,13 @@ int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chun
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
 /* Once the queue is empty and at most n realizations are still iterating,
  * finish them one by one with the single-solve FP64 CG (warm start from the
- * batch iterate); 0 disables.  Default 16. */
+ * batch iterate); 0 disables; negative restores the default: 16, or 4 when the
+ * two-level tail runs narrowed (fl_darcy_batch_set_narrow). */
 int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n);
 /* Two-level preconditioner only: once the queue is empty and at most 16 (then
  * 4) realizations are still iterating, they continue in the batch with 16 (4)
- * lanes per edge row, keeping the coarse space, instead of being handed over.
- * on = 1 (default) enables it; with 0 the hand-over rule above applies. */
+ * lanes per edge row, keeping the coarse space; the hand-over threshold above
+ * still applies first.  on = 1 (default) enables it. */
 int fl_darcy_batch_set_narrow(fl_darcy_batch *b, int on);
 /* Two-level preconditioner of the batched CG: ncol 0 = Jacobi only, 1 = plus
  * one constant coarse column per aggregate, 2 = plus a second column phi per

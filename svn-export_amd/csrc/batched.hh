@@ -55,6 +55,8 @@ struct BatchScal {
     double *alpha, *beta, *rz, *rz_prev, *rr, *tol2, *bnorm2, *rr_hist, *rr_ru;
     int *status, *it, *n_ru, *par, *it_ru, *active_count;  // par: buffer holding the current search direction
     int *zp;  // two-level path, per slot: bit 0 re-form the direction, bit 1 add alpha p to x
+    double ru_drop2;  // FP32 path: reliable update once ||r||^2 fell below ru_drop2 times its value at the last one
+    int beta_fr;      // two-level path: 1 Fletcher-Reeves beta (fixed preconditioner), 0 flexible (Polak-Ribiere)
 };
 
 // Batched CG status per realization
@@ -92,8 +94,9 @@ class DarcyBatchGPU {
     // over candidate shapes and keeps the fastest.
     void set_launch(int grid, int xcd_map, int chunk);
     void autotune();
-    // live-realization count at which the rest go to the single-solve path (0: never)
-    void set_handover(int n) { handover_ = n < 0 ? 0 : n; }
+    // live-realization count at which the rest go to the single-solve path
+    // (0: never; negative: 16, or 4 when the two-level tail runs narrowed)
+    void set_handover(int n) { handover_ = n < 0 ? -1 : n; }
     // Two-level path: once the queue is empty and at most 16 (then 4)
     // realizations are live, continue them in the batch with 16 (4) lanes per
     // edge row instead of handing them over (default on)
@@ -152,13 +155,15 @@ class DarcyBatchGPU {
     int *d_frec_ = nullptr;     // FP32 pair-row records (20 words per row)
     bool unroll_ = true;        // two-level update: two rows per step
     int qs_ = 4;                // column split of the coarse solve with all slots busy
+    double ru_drop2_ = 0.1;     // reliable-update trigger (squared residual drop)
+    bool beta_fr_ = true;       // two-level beta: Fletcher-Reeves (fixed SPD preconditioner) or flexible
     BatchScal sc_{}, sc2_{};
     int *h_count_ = nullptr;  // pinned
     // slot -> realization (-1: empty), live chunk count, spare buffers
     std::vector<int> rid_;
     double *d_Kall_ = nullptr, *d_var_ = nullptr;  // K [R][E] as given; difficulty proxy [R]
     double *d_Ksave_ = nullptr;                    // the base solver's K while it is borrowed
-    int handover_ = 16;  // live realizations at or below which the single-solve path finishes them
+    int handover_ = -1;  // live realizations at or below which the single-solve path finishes them (-1: default)
     // two-level preconditioner
     void coarse_setup(const std::vector<int> &pairs);  // coarse inverses of the listed slots
     int ncol_ = 0, nagg_ = 0, nc_ = 0, nbmax_ = 0;

@@ -637,9 +637,9 @@ __global__ void __launch_bounds__(256) k_b_update(BArgs<V> a) {
 // convergence, or whose recursive residual norm dropped 10x since its last
 // true residual.  Updating only then keeps the recurrence undisturbed while
 // the FP32 residual is still accurate.
-constexpr double kRuDrop2 = 0.01;
+constexpr double kRuDrop2 = 0.1;  // default squared drop (sqrt(10) in norm); BatchScal::ru_drop2 at run time
 __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
-    return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < kRuDrop2 * sc.rr_ru[r]);
+    return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < sc.ru_drop2 * sc.rr_ru[r]);
 }
 
 // Lanes per edge row of the two-level iteration kernels.  W = 64: one row per
@@ -1421,6 +1421,188 @@ __global__ void __launch_bounds__(256) k_c_op2(BArgs<float> a, CArgs c) {
     }
 }
 
+// As k_c_op2<1>, two rows per step (row and row + stride): the records of
+// both rows arrive in one register (lanes 0..19 and 32..51) and all gathers of
+// both rows are issued before any is used.
+__global__ void __launch_bounds__(256) k_c_op3(BArgs<float> a, CArgs c) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int sl = blockIdx.y * 64 + lane;
+    const bool act = a.sc.status[sl] == kRun;
+    double v = 0.0;
+    if (__any(act)) {
+        const float *__restrict__ p = a.p0;
+        const float *__restrict__ Kv = a.K;
+        const size_t Rp = a.Rp;
+        int s0, st, en;
+        row_range(a.F, a.G, a.xcd, s0, st, en);
+        const int half = lane >> 5, wl = (lane & 31) < kFastRec ? (lane & 31) : 0;
+        auto rec_of = [&](int row0) {
+            const int rr = row0 + half * st;
+            return rr < en ? a.frec[(size_t)rr * kFastRec + wl] : -1;
+        };
+        int cur = s0 < en ? rec_of(s0) : -1;
+        for (int row = s0; row < en; row += 2 * st) {
+            const int nxt = row + 2 * st < en ? rec_of(row + 2 * st) : -1;
+            const int rowB = row + st;
+            const bool hasB = rowB < en;
+            const int eA = __builtin_amdgcn_readlane(cur, 0), eB = __builtin_amdgcn_readlane(cur, 32);
+            if (act && eA >= 0 && (!hasB || eB >= 0)) {
+                float xa[2][4], xb[2][4], K1[2], K2[2], xr[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (h == 1 && !hasB) break;
+                    const int b = 32 * h, rw = h ? rowB : row;
+                    xr[h] = p[(size_t)rw * Rp + sl];
+                    K1[h] = Kv[(size_t)__builtin_amdgcn_readlane(cur, b + 0) * Rp + sl];
+                    K2[h] = Kv[(size_t)__builtin_amdgcn_readlane(cur, b + 1) * Rp + sl];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        xa[h][j] = p[(size_t)__builtin_amdgcn_readlane(cur, b + 2 + j) * Rp + sl];
+                        xb[h][j] = p[(size_t)__builtin_amdgcn_readlane(cur, b + 6 + j) * Rp + sl];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (h == 1 && !hasB) break;
+                    const int b = 32 * h, rw = h ? rowB : row;
+                    float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        t1 += __int_as_float(__builtin_amdgcn_readlane(cur, b + 10 + j)) * xa[h][j];
+                        t2 += __int_as_float(__builtin_amdgcn_readlane(cur, b + 14 + j)) * xb[h][j];
+                    }
+                    float q = K1[h] * t1 + __int_as_float(__builtin_amdgcn_readlane(cur, b + 18)) * xr[h];
+                    q += K2[h] * t2 + __int_as_float(__builtin_amdgcn_readlane(cur, b + 19)) * xr[h];
+                    a.q[(size_t)rw * Rp + sl] = q;
+                    v += (double)xr[h] * (double)q;
+                }
+            } else if (act) {
+                const XPlain<float> xs{p};
+                for (int h = 0; h < 2; h++) {
+                    if (h == 1 && !hasB) break;
+                    const int rw = h ? rowB : row;
+                    const float q = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
+                                                  a.opT, a.K, a.opT64, a.K64, rw, xs, Rp, sl);
+                    const size_t o = (size_t)rw * Rp + sl;
+                    a.q[o] = q;
+                    v += (double)p[o] * (double)q;
+                }
+            }
+            cur = nxt;
+        }
+    }
+    __shared__ double sh[kWaves][64];
+    sh[wid][lane] = v;
+    __syncthreads();
+    if (wid == 0) {
+        double t = sh[0][lane];
+        for (int w = 1; w < kWaves; w++) t += sh[w][lane];
+        a.part0[(size_t)blockIdx.x * a.Rp + sl] = t;
+    }
+}
+
+// FP32 operator of the narrowed tail (W lanes per row, 64 / W rows per wave)
+// from the inlined pair records: each lane group reads its row's record
+// (identical addresses within the group), the next group row's record is
+// loaded while the current gathers fly.
+template <int W>
+__global__ void __launch_bounds__(256) k_c_opn(BArgs<float> a, CArgs c) {
+    using L = LaneMap<W>;
+    const int lane = threadIdx.x & 63;
+    const int r = L::slot(lane), sub = L::sub(lane);
+    const bool act = a.sc.status[r] == kRun;
+    double v[1] = {0.0};
+    if (__any(act)) {
+        const float *__restrict__ p = a.p0;
+        const float *__restrict__ Kv = a.K;
+        const size_t Rp = a.Rp;
+        int s0, st, en;
+        row_range((a.F + L::P - 1) / L::P, a.G, a.xcd, s0, st, en);
+        int C[kFastRec], N[kFastRec];
+        {
+            const int row = s0 * L::P + sub;
+            const bool ok = s0 < en && row < a.F;
+#pragma unroll
+            for (int k = 0; k < kFastRec; k++) C[k] = ok ? a.frec[(size_t)row * kFastRec + k] : -1;
+        }
+        for (int g = s0; g < en; g += st) {
+            const int row = g * L::P + sub;
+            {
+                const int nrow = (g + st) * L::P + sub;
+                const bool ok = g + st < en && nrow < a.F;
+#pragma unroll
+                for (int k = 0; k < kFastRec; k++) N[k] = ok ? a.frec[(size_t)nrow * kFastRec + k] : -1;
+            }
+            if (act && row < a.F) {
+                const size_t o = (size_t)row * Rp + r;
+                const float xr = p[o];
+                float q;
+                if (C[0] >= 0) {
+                    const float K1 = Kv[(size_t)C[0] * Rp + r], K2 = Kv[(size_t)C[1] * Rp + r];
+                    float xa[4], xb[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        xa[j] = p[(size_t)C[2 + j] * Rp + r];
+                        xb[j] = p[(size_t)C[6 + j] * Rp + r];
+                    }
+                    float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        t1 += __int_as_float(C[10 + j]) * xa[j];
+                        t2 += __int_as_float(C[14 + j]) * xb[j];
+                    }
+                    q = K1 * t1 + __int_as_float(C[18]) * xr;
+                    q += K2 * t2 + __int_as_float(C[19]) * xr;
+                } else {
+                    const XPlain<float> xs{p};
+                    q = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
+                                      a.opT64, a.K64, row, xs, Rp, r);
+                }
+                a.q[o] = q;
+                v[0] += (double)xr * (double)q;
+            }
+#pragma unroll
+            for (int k = 0; k < kFastRec; k++) C[k] = N[k];
+        }
+    }
+    {
+        const double f = L::fold(v[0]);
+        v[0] = sub == 0 ? f : 0.0;
+    }
+    double *const dst[1] = {a.part0};
+    block_part<1>(v, dst, a.Rp);
+}
+
+// FP32 direction, FP64 operator: q = S64 p with the FP64 element table and
+// K, rounded to FP32 once (isolates the operator's rounding from the FP32
+// recursion's).
+struct XWiden {
+    const float *__restrict__ x;
+    __device__ __forceinline__ double operator()(int, size_t idx) const { return (double)x[idx]; }
+};
+__global__ void __launch_bounds__(256) k_c_op_w(BArgs<float> a, CArgs c) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.y * 64 + lane;
+    const bool act = a.sc.status[r] == kRun;
+    double v[1] = {0.0};
+    if (__any(act)) {
+        const XWiden xs{a.p0};
+        int s0, st, en;
+        row_range(a.F, a.G, a.xcd, s0, st, en);
+        for (int row = s0; row < en; row += st) {
+            if (!act) continue;
+            const double q = op_row<double>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
+                                            a.opT64, a.K64, a.opT64, a.K64, row, xs, a.Rp, r);
+            const size_t o = (size_t)row * a.Rp + r;
+            const float qf = (float)q;
+            a.q[o] = qf;
+            v[0] += (double)a.p0[o] * (double)qf;
+        }
+    }
+    double *const dst[1] = {a.part0};
+    block_part<1>(v, dst, a.Rp);
+}
+
 // Per slot, by the zp flags of the last fold: bit 1 x += alpha p (the
 // iteration's own step), bit 0 the new direction in place, p = z + beta p
 // with z = D^-1 r + Z y (two-level preconditioner)
@@ -1573,13 +1755,13 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         sc.rz_prev[slot] = sc.rz[slot];
         sc.rz[slot] = rz;
         // flexible CG: beta = z_new . (r_new - r_old) / z_old . r_old, r_new - r_old = -alpha q
-        sc.beta[slot] = -sc.alpha[slot] * zq / sc.rz_prev[slot];
+        sc.beta[slot] = sc.beta_fr ? rz / sc.rz_prev[slot] : -sc.alpha[slot] * zq / sc.rz_prev[slot];
         sc.rr[slot] = rr;
         if (!(rr == rr)) st = kBreakdown;
         else if (rr <= sc.tol2[slot]) st = mixed ? kCandidate : kConverged;
         else if (it >= max_it) st = kMaxIt;
         else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
-        else if (mixed && rr < kRuDrop2 * sc.rr_ru[slot]) st = kPause;  // direction kept for the update
+        else if (mixed && rr < sc.ru_drop2 * sc.rr_ru[slot]) st = kPause;  // direction kept for the update
     } else {  // reliable update
         sc.n_ru[slot] += 1;
         sc.rr_ru[slot] = rr;
@@ -1925,9 +2107,14 @@ void DarcyBatchGPU::build() {
     d_var_ = ad(R_);
     d_pairs_ = ai(2 * (size_t)Rp_);
     d_Ksave_ = ad(E);
-    if (const char *h = std::getenv("FL_BATCH_HANDOVER")) handover_ = std::max(0, std::atoi(h));
+    if (const char *h = std::getenv("FL_BATCH_HANDOVER")) handover_ = std::max(-1, std::atoi(h));
     if (const char *w = std::getenv("FL_BATCH_NARROW")) narrow_ = std::atoi(w) != 0;
-    if (const char *w = std::getenv("FL_BATCH_OPPAIR")) opfast_ = std::min(3, std::max(0, std::atoi(w)));
+    if (const char *w = std::getenv("FL_BATCH_RUDROP")) {  // drop in norm that triggers a reliable update
+        const double d = std::atof(w);
+        if (d > 1.0) ru_drop2_ = 1.0 / (d * d);
+    }
+    if (const char *w = std::getenv("FL_BATCH_BETA_FR")) beta_fr_ = std::atoi(w) != 0;
+    if (const char *w = std::getenv("FL_BATCH_OPPAIR")) opfast_ = std::min(5, std::max(0, std::atoi(w)));
     if (const char *w = std::getenv("FL_BATCH_UNROLL")) unroll_ = std::atoi(w) != 0;
     if (const char *w = std::getenv("FL_BATCH_QS")) {
         const int q = std::atoi(w);
@@ -1982,6 +2169,8 @@ void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
     sc.it_ru = sci + 4 * Rp_;
     sc.zp = sci + 5 * Rp_;
     sc.active_count = sci + 7 * Rp_;
+    sc.ru_drop2 = kRuDrop2;
+    sc.beta_fr = 0;
 }
 
 void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
@@ -2214,6 +2403,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const double rtol2 = rtol * rtol, atol2 = atol * atol;
     nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
     lw_ = 64;        // lanes per row of the two-level kernels (narrowed for the tail)
+    sc_.ru_drop2 = sc2_.ru_drop2 = ru_drop2_;
+    sc_.beta_fr = sc2_.beta_fr = beta_fr_ ? 1 : 0;
     BArgs<double> a64;
     BArgs<float> a32;
     CArgs ca{};
@@ -2334,7 +2525,15 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC && opfast_ == 2 && lw_ == 64 && nch_ % 2 == 0)
+        if (NC && opfast_ >= 2 && lw_ == 16)
+            hipLaunchKernelGGL(k_c_opn<16>, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ >= 2 && lw_ == 4)
+            hipLaunchKernelGGL(k_c_opn<4>, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ == 5 && lw_ == 64)
+            hipLaunchKernelGGL(k_c_op_w, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ == 4 && lw_ == 64)
+            hipLaunchKernelGGL(k_c_op3, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ == 2 && lw_ == 64 && nch_ % 2 == 0)
             hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a32, ca);
         else if (NC && opfast_ >= 2 && lw_ == 64)
             hipLaunchKernelGGL(k_c_op2<1>, grow(), blk, 0, stream_, a32, ca);
@@ -2722,7 +2921,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         }
         if (nrun == 0) break;
         const bool narrow = narrow_ && NC > 0;
-        if (next >= queue.size() && nrun + nf64 <= handover_) {
+        const int hand = handover_ >= 0 ? handover_ : (narrow ? 4 : 16);
+        if (next >= queue.size() && nrun + nf64 <= hand) {
             std::vector<int> pairs;
             for (int sl = 0; sl < Rp_; sl++)
                 if (rid_[sl] >= 0 && !terminal(stv[sl])) {
@@ -2798,7 +2998,9 @@ void DarcyBatchGPU::tune(int precision) {
             // the operator kernel the iterations will run (it only rewrites q
             // and, for the one-level kernel, the spare direction buffer)
             auto op = [&]() {
-                if (ncol_ && opfast_ == 2 && precision == 1 && nch_ % 2 == 0)
+                if (ncol_ && opfast_ == 4 && precision == 1)
+                    hipLaunchKernelGGL(k_c_op3, grow, blk, 0, stream_, a, ca);
+                else if (ncol_ && opfast_ == 2 && precision == 1 && nch_ % 2 == 0)
                     hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a, ca);
                 else if (ncol_ && opfast_ >= 2 && precision == 1)
                     hipLaunchKernelGGL(k_c_op2<1>, grow, blk, 0, stream_, a, ca);

@@ -72,6 +72,7 @@ class GnnGPU {
     float *d_dHf_[2] = {}, *d_dHe_[2] = {}, *d_dM_ = nullptr, *d_wpart_ = nullptr, *d_lossbuf_ = nullptr;
     unsigned short *d_dZ_ = nullptr;
     double *d_stats_ = nullptr;
+    double *d_lkpart_ = nullptr;  // partial ln K sums
     int adam_t_ = 0;
 };
 

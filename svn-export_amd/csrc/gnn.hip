@@ -234,10 +234,13 @@ __global__ void __launch_bounds__(256) k_g_out_bwd(int n, const bf16 *__restrict
 }
 
 // ln K statistics (mean, standard deviation), one block
-__global__ void k_g_logk_stats(int E, const double *__restrict__ K, double *__restrict__ out) {
-    __shared__ double sh[2][16];
+// Mean and standard deviation of ln K over the elements, in two stages:
+// kLkParts blocks write partial sums, one wave folds them in a fixed order.
+constexpr int kLkParts = 128;
+__global__ void __launch_bounds__(256) k_g_logk_part(int E, const double *__restrict__ K, double *__restrict__ part) {
+    __shared__ double sh[2][4];
     double s1 = 0.0, s2 = 0.0;
-    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const double v = log(K[e]);
         s1 += v;
         s2 += v * v;
@@ -253,18 +256,28 @@ __global__ void k_g_logk_stats(int E, const double *__restrict__ K, double *__re
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-            a += sh[0][k];
-            b += sh[1][k];
-        }
-        const double mu = a / E;
-        out[0] = mu;
-        out[1] = sqrt(fmax(b / E - mu * mu, 1e-24));
+        part[2 * blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        part[2 * blockIdx.x + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     }
 }
 
-// element features: standardised ln K, its square, dimension one-hot, ln cs / 10, bias
+__global__ void k_g_logk_stats(int E, const double *__restrict__ part, double *__restrict__ out) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = threadIdx.x; k < kLkParts; k += 64) {
+        s1 += part[2 * k];
+        s2 += part[2 * k + 1];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+    }
+    if (threadIdx.x == 0) {
+        const double mu = s1 / E;
+        out[0] = mu;
+        out[1] = sqrt(fmax(s2 / E - mu * mu, 1e-24));
+    }
+}
+
 __global__ void k_g_feat_e(int E, const double *__restrict__ K, const double *__restrict__ cs,
                            const int *__restrict__ dim, const double *__restrict__ st, bf16 *__restrict__ xe) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
@@ -534,6 +547,8 @@ GnnGPU::GnnGPU(DarcyBatchGPU &batch, unsigned seed) : b_(batch) {
     d_lossbuf_ = af(Fp_);
     d_stats_ = dev_alloc<double>(5 * (size_t)nagg_ + 8);
     owned_.push_back(d_stats_);
+    d_lkpart_ = dev_alloc<double>(2 * (size_t)kLkParts);
+    owned_.push_back(d_lkpart_);
     // parameters (oracle/gnn.py param_shapes order)
     std::vector<std::pair<int, int>> shp = {{NF, H}, {H, 0}, {NF, H}, {H, 0}};
     for (int l = 1; l <= 3; l++) {
@@ -637,7 +652,9 @@ void GnnGPU::get_features(float *xe, float *xf) const {
 void GnnGPU::forward(const double *dK, float *dphi_out) {
     hipStream_t s = stream_;
     const dim3 blk(256);
-    hipLaunchKernelGGL(k_g_logk_stats, dim3(1), dim3(1024), 0, s, E_, dK, d_stats_ + 5 * (size_t)nagg_);
+    hipLaunchKernelGGL(k_g_logk_part, dim3(kLkParts), dim3(256), 0, s, E_, dK, d_lkpart_);
+    hipLaunchKernelGGL(k_g_logk_stats, dim3(1), dim3(64), 0, s, E_, (const double *)d_lkpart_,
+                       d_stats_ + 5 * (size_t)nagg_);
     hipLaunchKernelGGL(k_g_feat_e, dim3(std::min(grid_for(E_), 2048)), blk, 0, s, E_, dK, b_.base().dev().cs,
                        (const int *)d_dim_, (const double *)(d_stats_ + 5 * (size_t)nagg_), d_xe_);
     const float *P = d_P_;
