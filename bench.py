@@ -99,12 +99,28 @@ def mc_fields(torch, cen, scale, R, seed, n_features=256, sigma=(0.5, 3.0), corr
     return K, sig.cpu().numpy(), ell.cpu().numpy()
 
 
-def op_bytes(F, E, slots, w, rec_bytes, table_bytes):
-    """Algorithmic bytes of one batched-operator launch with every slot live:
-    z and the previous direction read once per row and slot, the new direction
-    and q written, K read once per element and slot, plus the shared row
-    records and element table."""
-    return F * slots * 4.0 * w + E * slots * w + rec_bytes + table_bytes
+def op_bytes(F, E, slots, w, coarse):
+    """Algorithmic bytes of one batched-operator launch with every slot live.
+    Two-level path (k_c_op2 in FP32: the direction p read once per row and
+    slot, q written, K read once per element and slot, one 80-byte inlined
+    record per row; in FP64 the 64-byte records plus the element table).
+    One-level path (k_b_op): z and the previous direction read, the new
+    direction and q written, K, 64-byte records."""
+    if coarse:
+        rec = 80.0 * F if w == 4.0 else 64.0 * F + 20 * 8.0 * E
+        return 2.0 * F * slots * w + E * slots * w + rec
+    return 4.0 * F * slots * w + E * slots * w + 64.0 * F
+
+
+def update_bytes(F, slots, w, coarse):
+    """Algorithmic bytes of one update launch: two-level (k_c_update): q and r
+    read, r written, D^-1 (bfloat16 in FP32, else FP64) and, with two coarse
+    columns, phi (bfloat16) read; one-level (k_b_update): x, p, r, q, D^-1
+    read, x, r, z written."""
+    if coarse:
+        dw = 2.0 if w == 4.0 else 8.0
+        return F * slots * (3.0 * w + dw + (2.0 if coarse == 2 else 0.0))
+    return 8.0 * F * slots * w
 
 
 def main():
@@ -271,21 +287,29 @@ def main():
     # --------------------------------------------------------------- roofline
     w = 4.0 if args.precision == 1 else 8.0
     slots = min(((args.slots + 63) // 64) * 64, ((R + 63) // 64) * 64)
-    # two-level path: the operator gathers one vector (p) instead of z and the previous direction
-    b_op = op_bytes(F, E, slots, w, 64.0 * F, 0.0) - (F * slots * w if args.coarse else 0.0)
+    b_op = op_bytes(F, E, slots, w, args.coarse)
     op_ms = float(np.mean([i["ms_op_avg"] for i in infos]))
     upd_ms = float(np.mean([i["ms_update_avg"] for i in infos]))
     achieved = b_op / (op_ms * 1e-3) / 1e9
-    upd_bytes = F * slots * w * 8.0  # x, p, r, q, dinv read; x, r, z written
-    # HBM bytes per launch of the operator with all slots live, from the two
-    # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated in
-    # the same runs) over this workload shape; other shapes were not profiled
+    upd_bytes = update_bytes(F, slots, w, args.coarse)
+    # Bytes per launch of the operator with all slots live, from the two
+    # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated
+    # in the same runs) over this workload, per launch shape measured;
+    # FETCH_SIZE counts L2 -> fabric requests, Infinity-Cache hits included
+    measured = {(32, 128, 1, 2, 1024, 0): (1836.5e6, 217.1e6), (32, 128, 1, 2, 2048, 1): (1074.0e6, 217.1e6)}
+    key = (args.n, slots, args.precision, args.coarse, int(info["grid"]), int(info["xcd_map"]))
     traffic = None
-    if (args.n, slots, args.precision, args.coarse) == (32, 128, 1, 2):
-        traffic = {"bytes": 843.0e6, "fetch_bytes": 626.0e6, "write_bytes": 217.0e6,
-                   "source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes of bench.py on this workload (profiles/)"}
-    kname = ("k_c_op<%s> (batched condensed operator, two-level path)" if args.coarse else
-             "k_b_op<%s> (batched condensed operator)") % ("float" if w == 4 else "double")
+    if key in measured:
+        fb, wb = measured[key]
+        traffic = {"bytes": fb + wb, "fetch_bytes": fb, "write_bytes": wb,
+                   "source": "rocprofv3 FETCH_SIZE / WRITE_SIZE passes, launch grid %d, row mapping %d (profiles/)"
+                   % (key[4], key[5])}
+    if args.coarse and w == 4.0:
+        kname = "k_c_op2<2> (batched condensed operator, two-level path, FP32)"
+    elif args.coarse:
+        kname = "k_c_op<double, 64, 1> (batched condensed operator, two-level path, FP64)"
+    else:
+        kname = "k_b_op<%s> (batched condensed operator)" % ("float" if w == 4 else "double")
     roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -57,6 +57,7 @@ struct BatchScal {
     int *zp;  // two-level path, per slot: bit 0 re-form the direction, bit 1 add alpha p to x
     double ru_drop2;  // FP32 path: reliable update once ||r||^2 fell below ru_drop2 times its value at the last one
     int beta_fr;      // two-level path: 1 Fletcher-Reeves beta (fixed preconditioner), 0 flexible (Polak-Ribiere)
+    int no_pause;     // FP32 two-level path: 1 keeps iterating after the drop, the update replaces r only
 };
 
 // Batched CG status per realization
@@ -155,8 +156,10 @@ class DarcyBatchGPU {
     int *d_frec_ = nullptr;     // FP32 pair-row records (20 words per row)
     bool unroll_ = true;        // two-level update: two rows per step
     int qs_ = 4;                // column split of the coarse solve with all slots busy
-    double ru_drop2_ = 0.1;     // reliable-update trigger (squared residual drop)
+    double ru_drop2_ = 1.0;     // reliable-update trigger (squared residual drop)
     bool beta_fr_ = true;       // two-level beta: Fletcher-Reeves (fixed SPD preconditioner) or flexible
+    int early_ = 0;             // realizations handed to the single-solve worker when the queue runs empty
+    bool no_pause_ = true;      // reliable update as residual replacement, no pause before it
     BatchScal sc_{}, sc2_{};
     int *h_count_ = nullptr;  // pinned
     // slot -> realization (-1: empty), live chunk count, spare buffers
@@ -170,6 +173,9 @@ class DarcyBatchGPU {
     bool coarse_ok_ = false;  // aggregation usable (neighbour counts within the kernel limit)
     bool have_phi_ = false;
     std::vector<int> h_perm_, h_agg_;
+    std::vector<int> h_nbptr_, h_nbv_, h_lbmap_;  // neighbour aggregates (host copies)
+    int *d_tptr_ = nullptr, *d_tagg_ = nullptr;    // coarse-block term lists: offsets per block, block aggregate
+    int2 *d_terms_ = nullptr;                      // (row, e << 8 | i << 4 | j) per term
     int *d_agg_ = nullptr, *d_aptr_ = nullptr, *d_nbptr_ = nullptr, *d_nb_ = nullptr, *d_lbmap_ = nullptr;
     float *d_phiAll_ = nullptr, *d_ainv_ = nullptr, *d_spainv_ = nullptr;
     unsigned short *d_phi_ = nullptr, *d_spphi_ = nullptr;  // phi per slot, bfloat16 [F][Rp]
@@ -188,7 +194,7 @@ class DarcyBatchGPU {
     double *d_sp64_ = nullptr, *d_spK_ = nullptr;
     float *d_sp32_ = nullptr, *d_spK32_ = nullptr;
     // launch shape
-    int grid_ = 512, xcd_map_ = 0, chunk_ = 16;
+    int grid_ = 512, xcd_map_ = 0, chunk_ = 0;  // chunk 0: automatic
     bool tuned_ = false;
     hipGraphExec_t g64_ = nullptr, g32_ = nullptr;
     long long g64_key_ = -1, g32_key_ = -1;  // (iteration limit, live chunks) of the captured graphs

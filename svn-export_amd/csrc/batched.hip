@@ -8,8 +8,14 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <condition_variable>
 #include <cstdlib>
+#include <deque>
+#include <exception>
+#include <functional>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -637,7 +643,9 @@ __global__ void __launch_bounds__(256) k_b_update(BArgs<V> a) {
 // convergence, or whose recursive residual norm dropped 10x since its last
 // true residual.  Updating only then keeps the recurrence undisturbed while
 // the FP32 residual is still accurate.
-constexpr double kRuDrop2 = 0.1;  // default squared drop (sqrt(10) in norm); BatchScal::ru_drop2 at run time
+// Default reliable-update trigger (squared residual drop since the last one):
+// 1 replaces the residual of every running realization at every chunk end.
+constexpr double kRuDrop2 = 1.0;
 __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
     return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < sc.ru_drop2 * sc.rr_ru[r]);
 }
@@ -998,52 +1006,48 @@ __device__ __forceinline__ double elem_entry(const double *__restrict__ T, int n
 
 // Coarse operator Z^T S Z of the selected slots, as per-aggregate blocks
 // Asp[((a * kNbMax + lb) * ncol^2 + s * ncol + t) * Rp + slot] (lb: local
-// index of the neighbour aggregate).  One wave per (aggregate, 64 slots);
-// each wave owns its accumulators, so the sums are in a fixed order.
+// index of the neighbour aggregate b).  One wave per (block (a, b), 64
+// slots): the block is the sum, over the precomputed terms (row of a, element
+// contribution (e, i) of the row, local column j of e whose edge lies in b),
+// of z_s(row) S_e(K)_ij z_t(edge j), taken in row / contribution / column
+// order, accumulated in registers.
 template <int NCOL>
 __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, const int *__restrict__ lsel,
-                                                   double *__restrict__ Asp) {
-    __shared__ double acc[kNbMax][NCOL * NCOL][64];
+                                                   const int *__restrict__ tptr, const int2 *__restrict__ terms,
+                                                   const int *__restrict__ tagg, double *__restrict__ Asp) {
     const int lane = threadIdx.x;
     const int slot = blockIdx.y * 64 + lane;
-    const int ag = blockIdx.x;
+    const int tg = blockIdx.x;  // global neighbour slot: aggregate tagg[tg], local index tg - nbptr[a]
     const bool act = lsel[slot] != 0;
-    const int nb0 = c.nbptr[ag], nnb = c.nbptr[ag + 1] - nb0;
-    for (int k = 0; k < nnb; k++)
-        for (int q = 0; q < NCOL * NCOL; q++) acc[k][q][lane] = 0.0;
-    if (__any(act)) {
-        for (int row = c.aptr[ag]; row < c.aptr[ag + 1]; row++) {
-            if (!act) continue;
-            const double zf1 = NCOL == 2 ? (double)bf16_to_f(c.phi[(size_t)row * a.Rp + slot]) : 0.0;
-            for (int cc = a.row_ptr[row]; cc < a.row_ptr[row + 1]; cc++) {
-                const int code = a.row_c[cc];
-                const int e = code >> 4, i = code & 15;
-                const int mt = a.meta[e];
-                const int n0 = mt & 15, nvb = (mt >> 4) & 255;
-                const double *T = a.opT64 + a.op_off[e];
-                const double K = a.K64[(size_t)e * a.Rp + slot];
-                const int *cn = a.conn + a.conn_off[e];
-                for (int j = 0; j < n0 + nvb; j++) {
-                    const int g = cn[j];
-                    const int lb = c.lbmap[(size_t)ag * c.nagg + c.agg[g]];
-                    const double sij = elem_entry(T, n0, nvb, K, i, j);
-                    if constexpr (NCOL == 1) {
-                        acc[lb][0][lane] += sij;
-                    } else {
-                        const double zg1 = (double)bf16_to_f(c.phi[(size_t)g * a.Rp + slot]);
-                        acc[lb][0][lane] += sij;
-                        acc[lb][1][lane] += sij * zg1;
-                        acc[lb][2][lane] += zf1 * sij;
-                        acc[lb][3][lane] += zf1 * sij * zg1;
-                    }
-                }
-            }
+    if (!__any(act)) return;
+    const int ag = tagg[tg], lb = tg - c.nbptr[ag];
+    double acc[NCOL * NCOL];
+#pragma unroll
+    for (int q = 0; q < NCOL * NCOL; q++) acc[q] = 0.0;
+#pragma unroll 4
+    for (int k = tptr[tg]; k < tptr[tg + 1]; k++) {
+        const int2 u = terms[k];
+        const int row = u.x, e = u.y >> 8, i = (u.y >> 4) & 15, j = u.y & 15;
+        const int mt = a.meta[e];
+        const int n0 = mt & 15, nvb = (mt >> 4) & 255;
+        const double *T = a.opT64 + a.op_off[e];
+        const double K = a.K64[(size_t)e * a.Rp + slot];
+        const double sij = elem_entry(T, n0, nvb, K, i, j);
+        if constexpr (NCOL == 1) {
+            acc[0] += sij;
+        } else {
+            const int g = a.conn[a.conn_off[e] + j];
+            const double zf1 = (double)bf16_to_f(c.phi[(size_t)row * a.Rp + slot]);
+            const double zg1 = (double)bf16_to_f(c.phi[(size_t)g * a.Rp + slot]);
+            acc[0] += sij;
+            acc[1] += sij * zg1;
+            acc[2] += zf1 * sij;
+            acc[3] += zf1 * sij * zg1;
         }
     }
     if (!act) return;
-    for (int k = 0; k < nnb; k++)
-        for (int q = 0; q < NCOL * NCOL; q++)
-            Asp[((size_t)(ag * kNbMax + k) * (NCOL * NCOL) + q) * a.Rp + slot] = acc[k][q][lane];
+#pragma unroll
+    for (int q = 0; q < NCOL * NCOL; q++) Asp[((size_t)(ag * kNbMax + lb) * (NCOL * NCOL) + q) * a.Rp + slot] = acc[q];
 }
 
 // Dense coarse matrices of the listed slots, packed [np][nc][nc] for the
@@ -1761,13 +1765,13 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         else if (rr <= sc.tol2[slot]) st = mixed ? kCandidate : kConverged;
         else if (it >= max_it) st = kMaxIt;
         else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
-        else if (mixed && rr < sc.ru_drop2 * sc.rr_ru[slot]) st = kPause;  // direction kept for the update
+        else if (mixed && !sc.no_pause && rr < sc.ru_drop2 * sc.rr_ru[slot]) st = kPause;  // direction kept
     } else {  // reliable update
         sc.n_ru[slot] += 1;
         sc.rr_ru[slot] = rr;
         sc.it_ru[slot] = sc.it[slot];
+        if (st0 != kRun) sc.beta[slot] = rz / sc.rz_prev[slot];  // re-formed direction below
         sc.rz[slot] = rz;
-        sc.beta[slot] = rz / sc.rz_prev[slot];
         sc.rr[slot] = rr;
         if (!(rr == rr)) st = kBreakdown;
         else if (st0 == kCandidate) {
@@ -1781,7 +1785,10 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         else st = kRun;
     }
     sc.status[slot] = st;
-    sc.zp[slot] = (st == kRun ? 1 : 0) | (MODE == kFoldBeta ? 2 : 0);
+    // a slot that kept iterating up to its reliable update keeps its direction
+    // (residual replacement); paused and candidate slots re-form theirs
+    const bool keep = MODE == kFoldRU && st0 == kRun;
+    sc.zp[slot] = (st == kRun && !keep ? 1 : 0) | (MODE == kFoldBeta ? 2 : 0);
 }
 
 // slot-major arrays ([slot][n]): out[s] = in[src[s]]
@@ -2017,7 +2024,7 @@ void DarcyBatchGPU::build() {
             nbv.insert(nbv.end(), nb[a].begin(), nb[a].end());
             nbptr[a + 1] = (int)nbv.size();
         }
-        coarse_ok_ = nbmax_ <= kNbMax && n <= 1024;
+        coarse_ok_ = nbmax_ <= kNbMax && n <= 1024 && E < (1 << 23);  // term packing e << 8
         std::vector<int> lbmap;
         if (coarse_ok_) {
             lbmap.assign((size_t)n * n, -1);
@@ -2026,6 +2033,9 @@ void DarcyBatchGPU::build() {
         } else {
             lbmap.assign(1, -1);
         }
+        h_nbptr_ = nbptr;
+        h_nbv_ = nbv;
+        h_lbmap_ = lbmap;
         d_agg_ = b_up(owned_, agg, stream_);
         d_aptr_ = b_up(owned_, aptr, stream_);
         d_nbptr_ = b_up(owned_, nbptr, stream_);
@@ -2042,6 +2052,42 @@ void DarcyBatchGPU::build() {
             for (int i = conn_off[e]; i < conn_off[e + 1]; i++) row_c[fp[conn[i]]++] = e * kMaxConn + (i - conn_off[e]);
     }
     ncontrib_ = conn_off[E];
+    if (coarse_ok_) {
+        // terms of the coarse blocks (a, b), grouped by block, in row /
+        // contribution / column order: (row, e << 8 | i << 4 | j)
+        const int ntg = (int)h_nbv_.size();
+        std::vector<int> cnt(ntg + 1, 0), tagg(ntg);
+        for (int a = 0; a < nagg_; a++)
+            for (int k = h_nbptr_[a]; k < h_nbptr_[a + 1]; k++) tagg[k] = a;
+        auto target = [&](int row, int g) { return h_nbptr_[h_agg_[row]] + h_lbmap_[(size_t)h_agg_[row] * nagg_ + h_agg_[g]]; };
+        for (int pass = 0; pass < 2; pass++) {
+            std::vector<int> fill;
+            std::vector<int2> terms;
+            if (pass == 1) {
+                for (int t = 0; t < ntg; t++) cnt[t + 1] += cnt[t];
+                fill.assign(cnt.begin(), cnt.end() - 1);
+                terms.resize(cnt[ntg]);
+            }
+            for (int row = 0; row < F; row++)
+                for (int cc = row_ptr[row]; cc < row_ptr[row + 1]; cc++) {
+                    const int e = row_c[cc] / kMaxConn, i = row_c[cc] % kMaxConn;
+                    for (int j = 0; j < conn_off[e + 1] - conn_off[e]; j++) {
+                        const int t = target(row, conn[conn_off[e] + j]);
+                        if (pass == 0) cnt[t + 1]++;
+                        else terms[fill[t]++] = int2{row, (e << 8) | (i << 4) | j};
+                    }
+                }
+            if (pass == 1) {
+                d_tptr_ = b_up(owned_, cnt, stream_);
+                d_tagg_ = b_up(owned_, tagg, stream_);
+                int2 *dt = dev_alloc<int2>(terms.size());
+                owned_.push_back(dt);
+                FL_HIP(hipMemcpyAsync(dt, terms.data(), terms.size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
+                FL_HIP(hipStreamSynchronize(stream_));
+                d_terms_ = dt;
+            }
+        }
+    }
     std::vector<int> rec((size_t)F * kRecInts, -1);
     for (int f = 0; f < F; f++) {
         int *Rr = &rec[(size_t)f * kRecInts];
@@ -2114,6 +2160,8 @@ void DarcyBatchGPU::build() {
         if (d > 1.0) ru_drop2_ = 1.0 / (d * d);
     }
     if (const char *w = std::getenv("FL_BATCH_BETA_FR")) beta_fr_ = std::atoi(w) != 0;
+    if (const char *w = std::getenv("FL_BATCH_EARLY")) early_ = std::max(0, std::atoi(w));
+    if (const char *w = std::getenv("FL_BATCH_NOPAUSE")) no_pause_ = std::atoi(w) != 0;
     if (const char *w = std::getenv("FL_BATCH_OPPAIR")) opfast_ = std::min(5, std::max(0, std::atoi(w)));
     if (const char *w = std::getenv("FL_BATCH_UNROLL")) unroll_ = std::atoi(w) != 0;
     if (const char *w = std::getenv("FL_BATCH_QS")) {
@@ -2171,6 +2219,7 @@ void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
     sc.active_count = sci + 7 * Rp_;
     sc.ru_drop2 = kRuDrop2;
     sc.beta_fr = 0;
+    sc.no_pause = 0;
 }
 
 void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
@@ -2324,11 +2373,15 @@ void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
                                         d_conn_off_, d_conn_, d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_,
                                         d_p0_, d_p1_, d_q_, d_part_, sc_);
     if (ncol_ == 1) {
-        hipLaunchKernelGGL(k_c_assemble<1>, gagg, dim3(64), 0, stream_, a, c, (const int *)d_lsel_, d_Asp_);
+        hipLaunchKernelGGL(k_c_assemble<1>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
+                           (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
+                           d_Asp_);
         hipLaunchKernelGGL(k_c_pack<1>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
     } else {
-        hipLaunchKernelGGL(k_c_assemble<2>, gagg, dim3(64), 0, stream_, a, c, (const int *)d_lsel_, d_Asp_);
+        hipLaunchKernelGGL(k_c_assemble<2>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
+                           (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
+                           d_Asp_);
         hipLaunchKernelGGL(k_c_pack<2>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
     }
@@ -2351,7 +2404,7 @@ void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
 
 void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
     if (grid < 8 || grid > kMaxGrid || grid % 8) throw Error(-1, "batch grid must be a multiple of 8 in [8, 2048]");
-    if (chunk < 1 || chunk > 1024) throw Error(-1, "batch chunk must be in [1, 1024]");
+    if (chunk < 0 || chunk > 1024) throw Error(-1, "batch chunk must be in [0, 1024] (0: automatic)");
     if (grid != grid_ || xcd_map != xcd_map_ || chunk != chunk_) free_graphs();
     if (xcd_map < 0 || xcd_map > 2) throw Error(-1, "batch row mapping must be 0, 1 or 2");
     grid_ = grid;
@@ -2405,11 +2458,14 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     lw_ = 64;        // lanes per row of the two-level kernels (narrowed for the tail)
     sc_.ru_drop2 = sc2_.ru_drop2 = ru_drop2_;
     sc_.beta_fr = sc2_.beta_fr = beta_fr_ ? 1 : 0;
+    sc_.no_pause = sc2_.no_pause = no_pause_ ? 1 : 0;
     BArgs<double> a64;
     BArgs<float> a32;
     CArgs ca{};
     int G = grid_, X = xcd_map_;
     const int NC = ncol_;  // two-level preconditioner columns (0: Jacobi)
+    // iterations per captured chunk (automatic: 32 two-level, 16 one-level)
+    const int CH = chunk_ > 0 ? chunk_ : (NC ? 32 : 16);
     auto rebuild = [&]() {
         G = grid_;
         X = xcd_map_;
@@ -2586,7 +2642,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         ex = nullptr;
         hipGraph_t gr;
         FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk_; k++) {
+        for (int k = 0; k < CH; k++) {
             if (kind) iter32(nullptr);
             else iter64(nullptr);
         }
@@ -2707,53 +2763,125 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             rid_[sl] = -1;
         }
     };
-    // Hand the last live realizations to the single-solve FP64 path,
-    // warm-started from the batch iterate: a batched iteration costs a sweep
-    // over all rows whatever the number of live lanes, a single solve's
-    // iteration touches one realization only.
+    // Hand realizations to the single-solve FP64 path, warm-started from the
+    // batch iterate: a batched iteration costs a sweep over all rows whatever
+    // the number of live lanes, a single solve's iteration touches one
+    // realization only.  The single solves run in a worker thread of their
+    // own on the base solver's stream, so the batch keeps iterating its other
+    // slots meanwhile; their results are merged when the batch is done.
+    struct HandJob {
+        int real, it0, n_ru;
+        double bb, tol;
+        double *x0;
+    };
+    struct HandResult {
+        int real, it0, n_ru;
+        double bb, tol;
+        SolveStats ss;
+    };
+    std::mutex hmu;
+    std::condition_variable hcv;
+    std::deque<HandJob> hjobs;
+    std::vector<HandResult> hres;
+    std::vector<double *> hfree;  // extraction buffers ready for reuse
+    bool hclose = false;
+    std::exception_ptr herr;
+    std::thread hworker;
+    double hbusy_ms = 0.0;
+    int hdev = 0;
+    FL_HIP(hipGetDevice(&hdev));
+    auto hand_join = [&]() {
+        if (!hworker.joinable()) return;
+        {
+            std::lock_guard<std::mutex> lk(hmu);
+            hclose = true;
+        }
+        hcv.notify_all();
+        hworker.join();
+    };
+    struct JoinOnExit {  // the worker never outlives this call, also on an exception
+        std::function<void()> f;
+        ~JoinOnExit() { f(); }
+    } join_on_exit{hand_join};
+    auto hand_worker = [&]() {
+        (void)hipSetDevice(hdev);
+        const size_t n = (size_t)S_ + E_ + F_;
+        while (true) {
+            HandJob j;
+            {
+                std::unique_lock<std::mutex> lk(hmu);
+                hcv.wait(lk, [&] { return hclose || !hjobs.empty(); });
+                if (hjobs.empty()) return;
+                j = hjobs.front();
+                hjobs.pop_front();
+            }
+            try {
+                const auto t0 = std::chrono::steady_clock::now();
+                base_.set_conductivity_device(d_Kall_ + (size_t)j.real * E_);
+                const int left = std::max(1, max_it - j.it0);
+                SolveStats ss = base_.solve_warm(j.x0, rtol, atol, left, j.bb);
+                hipLaunchKernelGGL(k_b_put_col, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0,
+                                   base_.stream(), n, (size_t)Rout_, j.real, base_.device_solution(), d_sol_);
+                FL_HIP(hipGetLastError());
+                FL_HIP(hipStreamSynchronize(base_.stream()));
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                std::lock_guard<std::mutex> lk(hmu);
+                hres.push_back(HandResult{j.real, j.it0, j.n_ru, j.bb, j.tol, ss});
+                hfree.push_back(j.x0);
+                hbusy_ms += ms;
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(hmu);
+                if (!herr) herr = std::current_exception();
+                hfree.push_back(j.x0);
+            }
+        }
+    };
     auto handover = [&](const std::vector<int> &pairs, int mixed_now) {
         if (pairs.empty()) return;
-        const auto th0 = std::chrono::steady_clock::now();
         set_pairs(pairs);
         if (mixed_now)
             hipLaunchKernelGGL(k_b_accum<64>, grow(), blk, 0, stream_, F_, G, X, (size_t)Rp_, sc_, (const int *)d_lsel_,
                                d_x_, d_x32_);
-        FL_HIP(hipMemcpyAsync(stv.data(), sc_.status, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipMemcpyAsync(itv.data(), sc_.it, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipMemcpyAsync(ruv.data(), sc_.n_ru, Rp_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipMemcpyAsync(bbv.data(), sc_.bnorm2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipMemcpyAsync(tolv.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
-        if (!ksaved) {
+        if (!ksaved) {  // before the worker first changes the base solver's K
             FL_HIP(hipMemcpyAsync(d_Ksave_, base_.device_K(), E_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
             FL_HIP(hipStreamSynchronize(stream_));
             ksaved = true;
         }
-        const size_t n = (size_t)S_ + E_ + F_;
+        std::vector<HandJob> js;
         for (size_t k = 0; k < pairs.size(); k += 2) {
             const int sl = pairs[k], r = pairs[k + 1];
+            double *buf = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(hmu);
+                if (!hfree.empty()) {
+                    buf = hfree.back();
+                    hfree.pop_back();
+                }
+            }
+            if (!buf) {
+                buf = dev_alloc<double>((size_t)F_);
+                owned_.push_back(buf);
+            }
             hipLaunchKernelGGL(k_b_extract_x, dim3(std::min(grid_for(F_), 4096)), dim3(kBlock), 0, stream_, F_,
-                               (size_t)Rp_, sl, (const int *)d_perm_, (const double *)d_x_, d_ext_);
-            FL_HIP(hipStreamSynchronize(stream_));
-            base_.set_conductivity_device(d_Kall_ + (size_t)r * E_);
-            const int left = std::max(1, max_it - itv[sl]);
-            SolveStats ss = base_.solve_warm(d_ext_, rtol, atol, left, bbv[sl]);
-            hipLaunchKernelGGL(k_b_put_col, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0, stream_, n,
-                               (size_t)Rout_, r, base_.device_solution(), d_sol_);
-            FL_HIP(hipGetLastError());
-            h_status_[r] = ss.converged ? kConverged : kMaxIt;
-            h_it_[r] = itv[sl] + ss.iterations;
-            h_ru_[r] = ruv[sl];
-            h_rr_[r] = ss.residual * ss.residual;
-            h_bb_[r] = bbv[sl];
-            h_tol_[r] = tolv[sl];
-            rid_[sl] = -1;
+                               (size_t)Rp_, sl, (const int *)d_perm_, (const double *)d_x_, buf);
             hipLaunchKernelGGL(k_b_status_fill, dim3(1), dim3(64), 0, stream_, sc_.status + sl, 1, (int)kIdle);
+            js.push_back(HandJob{r, itv[sl], ruv[sl], bbv[sl], tolv[sl], buf});
+            rid_[sl] = -1;
             st.handovers++;
-            st.handover_iterations += ss.iterations;
         }
-        FL_HIP(hipStreamSynchronize(stream_));
-        st.ms_handover += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
+        FL_HIP(hipGetLastError());
+        FL_HIP(hipStreamSynchronize(stream_));  // extractions complete before the worker reads them
+        {
+            std::lock_guard<std::mutex> lk(hmu);
+            for (const HandJob &j : js) hjobs.push_back(j);
+        }
+        if (!hworker.joinable()) hworker = std::thread(hand_worker);
+        hcv.notify_all();
     };
     // move live slots to the front when that frees whole 64-slot chunks
     auto live = [&](int s) {
@@ -2854,8 +2982,9 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     // queue, compact once the queue is empty
     auto terminal = [&](int v) { return v == kConverged || v == kMaxIt || v == kBreakdown; };
     std::vector<int> free_slots;  // harvested slots waiting for a refill
+    bool early_done = false;
     long guard = 0;
-    const long guard_max = ((long)max_it + chunk_) * ((R_ + Rp_ - 1) / Rp_ + 2) / chunk_ + 16;
+    const long guard_max = ((long)max_it + CH) * ((R_ + Rp_ - 1) / Rp_ + 2) / CH + 16;
     while (true) {
         read_status();
         std::vector<int> done, stalled;
@@ -2919,6 +3048,28 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             }
             read_status();
         }
+        if (!early_done && early_ > 0 && next >= queue.size() && nrun > early_) {
+            // the queue just ran empty: the realizations furthest from their
+            // tolerance go to the single-solve worker now, so the batch drains
+            // the rest while they finish beside it
+            early_done = true;
+            FL_HIP(hipMemcpyAsync(rrv.data(), sc_.rr, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            FL_HIP(hipMemcpyAsync(tolv.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            FL_HIP(hipStreamSynchronize(stream_));
+            std::vector<std::pair<double, int>> far;
+            for (int sl = 0; sl < Rp_; sl++)
+                if (rid_[sl] >= 0 && (stv[sl] == kRun || stv[sl] == kPause))
+                    far.push_back({-(rrv[sl] / std::max(tolv[sl], 1e-300)), sl});
+            std::sort(far.begin(), far.end());
+            std::vector<int> pairs;
+            for (int k = 0; k < early_ && k < (int)far.size(); k++) {
+                pairs.push_back(far[k].second);
+                pairs.push_back(rid_[far[k].second]);
+            }
+            handover(pairs, mixed);
+            nrun -= (int)pairs.size() / 2;
+            read_status();
+        }
         if (nrun == 0) break;
         const bool narrow = narrow_ && NC > 0;
         const int hand = handover_ >= 0 ? handover_ : (narrow ? 4 : 16);
@@ -2942,6 +3093,18 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         FL_HIP(hipGraphLaunch(graph(mixed), stream_));
         st.chunks++;
     }
+    hand_join();
+    if (herr) std::rethrow_exception(herr);
+    for (const HandResult &h : hres) {
+        h_status_[h.real] = h.ss.converged ? kConverged : kMaxIt;
+        h_it_[h.real] = h.it0 + h.ss.iterations;
+        h_ru_[h.real] = h.n_ru;
+        h_rr_[h.real] = h.ss.residual * h.ss.residual;
+        h_bb_[h.real] = h.bb;
+        h_tol_[h.real] = h.tol;
+        st.handover_iterations += h.ss.iterations;
+    }
+    st.ms_handover = hbusy_ms;
     if (ksaved) base_.set_conductivity_device(d_Ksave_);
     FL_HIP(hipStreamSynchronize(stream_));
     float t01 = 0;
@@ -2965,7 +3128,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     }
     st.grid = grid_;
     st.xcd_map = xcd_map_;
-    st.chunk = chunk_;
+    st.chunk = CH;
     return st;
 }
 

@@ -75,6 +75,75 @@ __global__ void __launch_bounds__(256) k_g_fwd(const bf16 *__restrict__ X1, cons
     }
 }
 
+// Inference form of a message-passing layer: Y = relu(X1 W1 + M W2 + bias)
+// with M[i] = mean of S[j] over j in idx[ptr[i] .. ptr[i + 1]), formed in the
+// A-operand loads with the summation order and rounding of k_g_mean (so the
+// result equals k_g_mean followed by k_g_fwd<64, 64>), and the output tile
+// written through LDS as 32-byte row segments.  n rows, buffers padded to 64.
+__global__ void __launch_bounds__(256) k_g_fwd_mean(int n, const bf16 *__restrict__ X1, const bf16 *__restrict__ W1t,
+                                                    const int *__restrict__ ptr, const int *__restrict__ idx,
+                                                    const bf16 *__restrict__ S, const bf16 *__restrict__ W2t,
+                                                    const float *__restrict__ bias, bf16 *__restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) bf16 tile[4][16][H + 8];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = l & 15, kg = l >> 4;
+    const size_t rb = (size_t)blockIdx.x * 64 + w * 16;
+    const int row = (int)rb + r;
+    int a0 = 0, a1 = 0;
+    if (row < n) {
+        a0 = ptr[row];
+        a1 = ptr[row + 1];
+    }
+    float m[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) m[c] = 0.f;
+    for (int k = a0; k < a1; k++) {
+        const bf16 *src = S + (size_t)idx[k] * H + 4 * kg;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const s4 v = *(const s4 *)(src + 16 * q);
+#pragma unroll
+            for (int c = 0; c < 4; c++) m[4 * q + c] += bf2f((bf16)v[c]);
+        }
+    }
+    f4 acc[4];
+    for (int t = 0; t < 4; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < H; k0 += 16) {
+        const s4 a = *(const s4 *)(X1 + (rb + r) * H + k0 + kg * 4);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const s4 b = *(const s4 *)(W1t + (size_t)(t * 16 + r) * H + k0 + kg * 4);
+            acc[t] = mfma(a, b, acc[t]);
+        }
+    }
+    const int cnt = a1 - a0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        s4 a;
+#pragma unroll
+        for (int c = 0; c < 4; c++) a[c] = (short)f2bf(cnt > 0 ? m[4 * q + c] / (float)cnt : 0.f);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const s4 b = *(const s4 *)(W2t + (size_t)(t * 16 + r) * H + 16 * q + kg * 4);
+            acc[t] = mfma(a, b, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int col = t * 16 + r;
+        const float bb = bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; i++) tile[w][kg * 4 + i][col] = f2bf(fmaxf(acc[t][i] + bb, 0.f));
+    }
+    __syncthreads();
+    const int rr = l >> 2, seg = (l & 3) * 16;
+    const uint4 *src = (const uint4 *)&tile[w][rr][seg];
+    uint4 *dst = (uint4 *)(Y + (rb + rr) * H + seg);
+    dst[0] = src[0];
+    dst[1] = src[1];
+}
+
 // dX[N][K1] (= or +=) dZ[N][64] W^T with W[K1][64]
 template <int K1>
 __global__ void __launch_bounds__(256) k_g_dx(const bf16 *__restrict__ dZ, const bf16 *__restrict__ W,
@@ -788,8 +857,47 @@ void GnnGPU::adam_step(double lr) {
     pack_weights();
 }
 
+// Inference for R realizations: the forward pass without the stored
+// aggregation tensors (k_g_fwd_mean), phi written straight into dphi, one
+// stream synchronization at the end.
 void GnnGPU::infer(const double *dK, int R, float *dphi) {
-    for (int r = 0; r < R; r++) forward(dK + (size_t)r * E_, dphi + (size_t)r * F_);
+    hipStream_t s = stream_;
+    const dim3 blk(256);
+    const float *P = d_P_;
+    const dim3 gF(std::min((F_ + 3) / 4, 8192));
+    for (int r = 0; r < R; r++) {
+        const double *K = dK + (size_t)r * E_;
+        hipLaunchKernelGGL(k_g_logk_part, dim3(kLkParts), dim3(256), 0, s, E_, K, d_lkpart_);
+        hipLaunchKernelGGL(k_g_logk_stats, dim3(1), dim3(64), 0, s, E_, (const double *)d_lkpart_,
+                           d_stats_ + 5 * (size_t)nagg_);
+        hipLaunchKernelGGL(k_g_feat_e, dim3(std::min(grid_for(E_), 2048)), blk, 0, s, E_, K, b_.base().dev().cs,
+                           (const int *)d_dim_, (const double *)(d_stats_ + 5 * (size_t)nagg_), d_xe_);
+        hipLaunchKernelGGL((k_g_fwd<NF, 0>), dim3(Ep_ / 64), blk, 0, s, (const bf16 *)d_xe_,
+                           (const bf16 *)(d_Wt_ + off_[0]), (const bf16 *)nullptr, (const bf16 *)nullptr,
+                           P + off_[1], d_He_[0]);
+        hipLaunchKernelGGL((k_g_fwd<NF, 0>), dim3(Fp_ / 64), blk, 0, s, (const bf16 *)d_xf_,
+                           (const bf16 *)(d_Wt_ + off_[2]), (const bf16 *)nullptr, (const bf16 *)nullptr,
+                           P + off_[3], d_Hf_[0]);
+        int k = 4;
+        for (int l = 1; l <= 3; l++) {
+            hipLaunchKernelGGL(k_g_fwd_mean, dim3(Fp_ / 64), blk, 0, s, F_, (const bf16 *)d_Hf_[l - 1],
+                               (const bf16 *)(d_Wt_ + off_[k]), (const int *)d_fptr_, (const int *)d_fel_,
+                               (const bf16 *)d_He_[l - 1], (const bf16 *)(d_Wt_ + off_[k + 1]), P + off_[k + 2],
+                               d_Hf_[l]);
+            k += 3;
+            if (l < 3) {
+                hipLaunchKernelGGL(k_g_fwd_mean, dim3(Ep_ / 64), blk, 0, s, E_, (const bf16 *)d_He_[l - 1],
+                                   (const bf16 *)(d_Wt_ + off_[k]), (const int *)d_eptr_, (const int *)d_eedge_,
+                                   (const bf16 *)d_Hf_[l], (const bf16 *)(d_Wt_ + off_[k + 1]), P + off_[k + 2],
+                                   d_He_[l]);
+                k += 3;
+            }
+        }
+        hipLaunchKernelGGL(k_g_out, gF, blk, 0, s, F_, (const bf16 *)d_Hf_[3], P, off_[off_.size() - 2],
+                           dphi + (size_t)r * F_);
+    }
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace fl

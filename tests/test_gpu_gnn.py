@@ -104,3 +104,19 @@ def test_gnn_training_and_gain(setup):
     assert i2["converged"].all()
     assert t1 < t0 and t2 < t0
     b.set_coarse(0)
+
+
+def test_gnn_inference_equals_forward(setup):
+    """The inference path (aggregation fused into the layer kernels, no stored
+    aggregation tensors) gives bit-for-bit the forward pass used in training."""
+    import torch
+    om, gm, g, b, K, Kd, lam = setup
+    net = G.LearnedCoarseBasis(b, seed=5)
+    R = 6
+    phi = torch.zeros((R, om.n_edges), dtype=torch.float32, device="cuda")
+    net.infer(Kd.data_ptr(), R, phi.data_ptr())
+    one = torch.zeros(om.n_edges, dtype=torch.float32, device="cuda")
+    for r in range(R):
+        net.forward(Kd[r].data_ptr(), one.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(one, phi[r]), r
