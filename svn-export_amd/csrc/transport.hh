@@ -59,6 +59,8 @@ struct TmDev {
     const double *meas, *cs, *por, *flux, *edge_flow, *bc_conc;
     const double *src_density, *src_sigma, *src_conc;  // [E][m] interleaved (when has_src)
     double *val, *bcv, *partial, *mb;
+    double *ell_val;        // first entries of every row, column-major [5][E]
+    int *ell_col, *ell_len; // their columns [5][E], row lengths [E]
     unsigned *count;
 };
 
@@ -92,7 +94,13 @@ class TransportGPU {
     void mb_initial();
     void flush_flux();
     int step_grid() const;
-    static constexpr int kMaxStepGrid = 1024;
+    static constexpr int kMaxStepGrid = 16384;
+    bool quad_ = true;  // step kernel with one lane per (row, substance) for 2..4 substances
+    bool rows2_ = false; // quad step kernel: two rows per lane group and step
+    bool xcd_split_ = true;  // quad step kernel: each XCD one contiguous eighth of the rows
+    int grid_cap_ = 0;       // step grid limit (0: 8192)
+    double *d_acc_ = nullptr;  // per-block running boundary-flux / source totals of the quad step kernel
+    bool acc_dirty_ = false;
 
     const HostMesh &m_;
     hipStream_t stream_ = nullptr;

@@ -1,5 +1,6 @@
 // GPU explicit upwind transport.  See transport.hh.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -188,6 +189,22 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
     }
 }
 
+// Column-major copy of the first kEll entries of every row of the scaled
+// matrix (ell_val / ell_col [kEll][E], row length in ell_len), refreshed after
+// every rescale: the step kernel reads it coalesced, without the row-pointer
+// chain; longer rows (fracture and channel elements) take the rest from CSR.
+constexpr int kEll = 5;
+__global__ void k_tm_ell(const TmDev t) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        const int k0 = t.ptr[i], n = t.ptr[i + 1] - k0;
+        t.ell_len[i] = n;
+        for (int j = 0; j < kEll; j++) {
+            t.ell_val[(size_t)j * t.E + i] = j < n ? t.val[k0 + j] : 0.0;
+            t.ell_col[(size_t)j * t.E + i] = j < n ? t.col[k0 + j] : i;
+        }
+    }
+}
+
 // Folds one step's per-block partials (G blocks x 2M: boundary mass flux and
 // source term per substance) in a fixed order and adds them, times that
 // step's length, to the integrated quantities (mass_balance.cc:208-221).
@@ -268,6 +285,120 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
             out_buf[(size_t)blockIdx.x * 2 * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
 }
 
+// The same step with one lane per (row, substance): four lanes per row, 64
+// rows per block, so the concentration loads and stores of a wave are 16
+// whole rows (contiguous); the matrix comes from its column-major copy.  Per
+// (row, substance) the arithmetic and its order are those of k_step
+// (bit-identical concentrations).  The boundary-flux and source sums of the
+// block, times this step's length, are added to the block's own running
+// totals acc[block][2M]; k_flux_acc folds those over the blocks at output
+// times, so no block carries a cross-block fold on the step's critical path.
+template <int M, int RW>
+__global__ void __launch_bounds__(256) k_step_q(const TmDev t, const double *__restrict__ c_old,
+                                               double *__restrict__ c_new, double dt, double tdiff,
+                                               double *__restrict__ acc_buf, int xcd_split) {
+    const int s = threadIdx.x & 3;
+    double bfl = 0.0, bsr = 0.0;  // boundary flux, source mass of substance s
+    // rows: each XCD one contiguous eighth (xcd_split), or strided over the grid
+    const int xcd = xcd_split ? (blockIdx.x & 7) : 0, lb = xcd_split ? (blockIdx.x >> 3) : blockIdx.x;
+    const int nlb = xcd_split ? (gridDim.x >> 3) : gridDim.x;
+    const int per = xcd_split ? (t.E + 7) >> 3 : t.E;
+    const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
+    const int stride = nlb * 64;
+    if (s < M) {
+        for (int i0 = r_lo + lb * 64 + (threadIdx.x >> 2); i0 < r_hi; i0 += RW * stride) {
+            double acc[RW];
+            int rows[RW];
+#pragma unroll
+            for (int u = 0; u < RW; u++) {
+                const int i = i0 + u * stride;
+                rows[u] = i < r_hi ? i : -1;
+                acc[u] = 0.0;
+                if (i >= r_hi) continue;
+                const size_t o = (size_t)i * M + s;
+                acc[u] = t.bcv[o];
+                if (t.has_src) {
+                    const double dens = t.src_density[o], sig = t.src_sigma[o];
+                    const double diff = t.src_conc[o] - c_old[o];
+                    const double snd = diff > 0.0 ? (dens + diff * sig) : dens;
+                    acc[u] = acc[u] + snd * dt;
+                    bsr += snd * t.meas[i];
+                } else {
+                    acc[u] = acc[u] + 0.0;
+                }
+            }
+            // the padded entries of the column-major copy are (0, own row):
+            // adding 0 * c leaves every finite sum unchanged, so the fixed
+            // five-entry loop issues all loads of the RW rows together
+            double av[RW][kEll];
+            int ac[RW][kEll], ln[RW], b0[RW], b1[RW];
+#pragma unroll
+            for (int u = 0; u < RW; u++) {
+                const int i = rows[u] < 0 ? r_lo : rows[u];
+                ln[u] = t.ell_len[i];
+                b0[u] = t.bl_ptr[i];
+                b1[u] = t.bl_ptr[i + 1];
+#pragma unroll
+                for (int j = 0; j < kEll; j++) {
+                    av[u][j] = t.ell_val[(size_t)j * t.E + i];
+                    ac[u][j] = t.ell_col[(size_t)j * t.E + i];
+                }
+            }
+            double cv[RW][kEll];
+#pragma unroll
+            for (int u = 0; u < RW; u++)
+#pragma unroll
+                for (int j = 0; j < kEll; j++) cv[u][j] = c_old[(size_t)ac[u][j] * M + s];
+#pragma unroll
+            for (int u = 0; u < RW; u++) {
+                const int i = rows[u];
+                if (i < 0) continue;
+                const int ne = min(ln[u], kEll);
+#pragma unroll
+                for (int j = 0; j < kEll; j++)
+                    if (j < ne) acc[u] += av[u][j] * cv[u][j];
+                if (ln[u] > kEll) {
+                    const int k1 = t.ptr[i + 1];
+                    for (int k = t.ptr[i] + kEll; k < k1; k++) acc[u] += t.val[k] * c_old[(size_t)t.col[k] * M + s];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < RW; u++) {
+                const int i = rows[u];
+                if (i < 0) continue;
+                const size_t o = (size_t)i * M + s;
+                c_new[o] = acc[u];
+                for (int k = b0[u]; k < b1[u]; k++) {
+                    const double wf = t.flux[t.bl_side[k]];
+                    bfl += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + t.bl_bcele[k]] : wf * c_old[o];
+                }
+            }
+        }
+    }
+    for (int m = 4; m < 64; m <<= 1) {
+        bfl += __shfl_xor(bfl, m, 64);
+        bsr += __shfl_xor(bsr, m, 64);
+    }
+    __shared__ double sh[2 * M][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane < M) {
+        sh[lane][wid] = bfl;
+        sh[M + lane][wid] = bsr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 2 * M; k++)
+            acc_buf[(size_t)blockIdx.x * 2 * M + k] += (((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3]) * tdiff;
+}
+
+// Output time: the blocks' running totals of k_step_q folded in a fixed order
+// into the integrated quantities, then cleared.  One block.
+template <int M>
+__global__ void __launch_bounds__(256) k_flux_acc(const TmDev t, double *__restrict__ acc_buf, int G) {
+    fold_flux<M>(t, acc_buf, G, 1.0);
+    for (int k = threadIdx.x; k < G * 2 * M; k += blockDim.x) acc_buf[k] = 0.0;
+}
+
 template <int M>
 __global__ void __launch_bounds__(256) k_flux_flush(const TmDev t, const double *__restrict__ buf, int G,
                                                    double tdiff) {
@@ -288,6 +419,10 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
                            const double *init_conc, double end_time, double save_step)
     : m_(m), E_(m.n_elements()), nsub_(nsub), NBC_(m.n_bc_elements()) {
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (const char *q = std::getenv("FL_TRANSPORT_QUAD")) quad_ = std::atoi(q) != 0;
+    if (const char *q = std::getenv("FL_TRANSPORT_ROWS2")) rows2_ = std::atoi(q) != 0;
+    if (const char *q = std::getenv("FL_TRANSPORT_XCD")) xcd_split_ = std::atoi(q) != 0;
+    if (const char *q = std::getenv("FL_TRANSPORT_GRID")) grid_cap_ = std::max(8, std::atoi(q) / 8 * 8);
     const int E = E_, S = m.n_sides(), F = m.n_edges(), V = m.n_vb();
     // matrix pattern: rows in the reference's insertion order, later entries
     // overwrite earlier ones (INSERT_VALUES), then columns sorted per row.
@@ -372,10 +507,19 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     t.flux = d_flux_;
     t.edge_flow = alloc_d(F);
     t.val = alloc_d(nnz_);
+    t.ell_val = alloc_d((size_t)kEll * E);
+    {
+        int *p = dev_alloc<int>(((size_t)kEll + 1) * E);
+        owned_.push_back(p);
+        t.ell_col = p;
+        t.ell_len = p + (size_t)kEll * E;
+    }
     t.bcv = alloc_d((size_t)E * nsub);
     d_c0_ = alloc_d((size_t)E * nsub);
     d_c1_ = alloc_d((size_t)E * nsub);
     t.partial = alloc_d((size_t)2 * kMaxStepGrid * 8 + 4096);
+    d_acc_ = alloc_d((size_t)kMaxStepGrid * 8);
+    FL_HIP(hipMemsetAsync(d_acc_, 0, (size_t)kMaxStepGrid * 8 * sizeof(double), s));
     t.mb = alloc_d(1 + 2 * nsub);
     unsigned *cnt = dev_alloc<unsigned>(1);
     owned_.push_back(cnt);
@@ -482,9 +626,21 @@ void TransportGPU::mb_initial() {
     mb_init_ = false;
 }
 
-int TransportGPU::step_grid() const { return std::max(8, std::min(grid_for(E_) / 8 * 8, kMaxStepGrid)); }
+int TransportGPU::step_grid() const {
+    const int rows_per_block = (quad_ && nsub_ >= 2) ? 64 : kBlock;
+    const int cap = grid_cap_ > 0 ? std::min(grid_cap_, kMaxStepGrid) : 8192;
+    return std::max(8, std::min((E_ + rows_per_block - 1) / rows_per_block / 8 * 8, cap));
+}
 
 void TransportGPU::flush_flux() {
+    if (acc_dirty_) {
+        const int G = step_grid();
+        if (nsub_ == 2) hipLaunchKernelGGL(k_flux_acc<2>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
+        else if (nsub_ == 3) hipLaunchKernelGGL(k_flux_acc<3>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
+        else hipLaunchKernelGGL(k_flux_acc<4>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
+        FL_HIP(hipGetLastError());
+        acc_dirty_ = false;
+    }
     if (!pending_) return;
     const int G = step_grid();
     const double *last = t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8;
@@ -513,8 +669,10 @@ void TransportGPU::one_step() {
         if (scaled_) {
             double fac = tg_.estimate_dt() / tg_.time_step;
             hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 0, fac);
+            hipLaunchKernelGGL(k_tm_ell, dim3(gridS), dim3(kBlock), 0, stream_, t_);
         } else {
             hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 1, tg_.estimate_dt());
+            hipLaunchKernelGGL(k_tm_ell, dim3(gridS), dim3(kBlock), 0, stream_, t_);
             scaled_ = true;
         }
         need_rescale_ = false;
@@ -528,17 +686,37 @@ void TransportGPU::one_step() {
     double *out = t_.partial + (size_t)(step_parity_ ? 1 : 0) * kMaxStepGrid * 8;
     const double *fold = pending_ ? t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8 : nullptr;
     double ftd = pending_tdiff_, dt = tg_.time_step;
-    switch (nsub_) {
-        case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-        case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-        case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-        case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-        default: throw Error(-5, "transport supports 1..4 substances per handle");
+    if (quad_ && nsub_ >= 2) {
+#define FL_STEPQ(MM, RR) \
+    hipLaunchKernelGGL((k_step_q<MM, RR>), dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, tdiff, d_acc_, \
+                       xcd_split_ ? 1 : 0)
+        if (rows2_) {
+            if (nsub_ == 2) FL_STEPQ(2, 2);
+            else if (nsub_ == 3) FL_STEPQ(3, 2);
+            else FL_STEPQ(4, 2);
+        } else {
+            if (nsub_ == 2) FL_STEPQ(2, 1);
+            else if (nsub_ == 3) FL_STEPQ(3, 1);
+            else FL_STEPQ(4, 1);
+        }
+#undef FL_STEPQ
+    } else {
+        switch (nsub_) {
+            case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+            case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+            case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+            case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+            default: throw Error(-5, "transport supports 1..4 substances per handle");
+        }
     }
     FL_HIP(hipGetLastError());
-    pending_ = true;
-    pending_tdiff_ = tdiff;
-    step_parity_ = !step_parity_;
+    if (quad_ && nsub_ >= 2) {
+        acc_dirty_ = true;  // totals stay in the per-block accumulators until the output time
+    } else {
+        pending_ = true;
+        pending_tdiff_ = tdiff;
+        step_parity_ = !step_parity_;
+    }
     cur_is_c0_ = !cur_is_c0_;
     stats_.steps++;
 }
