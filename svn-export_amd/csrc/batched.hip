@@ -1762,7 +1762,7 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         sc.beta[slot] = sc.beta_fr ? rz / sc.rz_prev[slot] : -sc.alpha[slot] * zq / sc.rz_prev[slot];
         sc.rr[slot] = rr;
         if (!(rr == rr)) st = kBreakdown;
-        else if (rr <= sc.tol2[slot]) st = mixed ? kCandidate : kConverged;
+        else if (rr <= sc.tol2[slot]) st = kCandidate;  // confirmed on the true residual at the chunk end
         else if (it >= max_it) st = kMaxIt;
         else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
         else if (mixed && !sc.no_pause && rr < sc.ru_drop2 * sc.rr_ru[slot]) st = kPause;  // direction kept
@@ -2456,7 +2456,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const double rtol2 = rtol * rtol, atol2 = atol * atol;
     nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
     lw_ = 64;        // lanes per row of the two-level kernels (narrowed for the tail)
-    sc_.ru_drop2 = sc2_.ru_drop2 = ru_drop2_;
+    // FP64 runs no residual replacement: only candidates are checked
+    sc_.ru_drop2 = sc2_.ru_drop2 = precision == 1 ? ru_drop2_ : 0.0;
     sc_.beta_fr = sc2_.beta_fr = beta_fr_ ? 1 : 0;
     sc_.no_pause = sc2_.no_pause = no_pause_ ? 1 : 0;
     BArgs<double> a64;
@@ -2631,6 +2632,18 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                            (const double *)d_x_);
         fold(kFoldRU, 1, nosel);
     };
+    // two-level FP64: realizations whose recursive residual met the tolerance
+    // are confirmed (or resumed) on the true residual b - S x
+    auto candidate_check64 = [&]() {
+        if (NC == 1)
+            FL_LW(hipLaunchKernelGGL((k_c_resid<double, 1, LW>), gagg(), blkA, 0, stream_, a64, ca,
+                                     (const double *)d_b_, (const double *)d_x_))
+        else
+            FL_LW(hipLaunchKernelGGL((k_c_resid<double, 2, LW>), gagg(), blkA, 0, stream_, a64, ca,
+                                     (const double *)d_b_, (const double *)d_x_))
+        cfold(kFoldRU, 0, nosel);
+        zp64();
+    };
     // graph of `chunk_` iterations (kind 0: FP64; kind 1: FP32 + reliable update),
     // re-captured when the launch width or the iteration limit changes
     auto graph = [&](int kind) -> hipGraphExec_t {
@@ -2647,6 +2660,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             else iter64(nullptr);
         }
         if (kind) reliable_update();
+        else if (NC) candidate_check64();
         FL_HIP(hipStreamEndCapture(stream_, &gr));
         FL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
         (void)hipGraphDestroy(gr);
