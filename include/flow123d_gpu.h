@@ -25,6 +25,7 @@ extern "C" {
 typedef struct fl_mesh fl_mesh;
 typedef struct fl_darcy fl_darcy;
 typedef struct fl_transport fl_transport;
+typedef struct fl_element_data fl_element_data;
 
 /* Message of the last failed call on this thread. */
 const char *fl_last_error(void);
@@ -63,6 +64,54 @@ int fl_mesh_topology(const fl_mesh *m, int *elem_dim, int *elem_id, int *elem_re
 /* Region table (RegionDB, src/mesh/region.cc:105-171): ids, boundary flag,
  * index among boundary or bulk regions; labels '\n'-separated into buf. */
 int fl_mesh_regions(const fl_mesh *m, int *ids, int *is_boundary, int *sub_idx, char *buf, int buflen);
+
+/* Mesh::elements_id_maps (src/mesh/mesh.cc:781-812): gmsh ids of the E bulk
+ * elements (boundary = 0; an error unless strictly increasing) or of the NBC
+ * boundary elements, -1 for the implicit ones (boundary = 1). */
+int fl_mesh_element_ids(const fl_mesh *m, int boundary, int *ids);
+
+/* ---- element-wise data in gmsh files ------------------------------------- */
+
+/* FieldElementwise::init_from_input + set_mesh
+ * (src/fields/field_elementwise_impl.hh:67-76, 121-137) over a
+ * GmshMeshReader opened on gmsh_file (src/mesh/msh_gmshreader.cc:51-57): the
+ * field `field_name` with n_comp values per element on the bulk (boundary = 0)
+ * or boundary (boundary = 1) elements of m.  Values start at zero.  The mesh
+ * is only read here. */
+int fl_element_data_open(const fl_mesh *m, const char *gmsh_file, const char *field_name, int n_comp, int boundary,
+                         fl_element_data **out);
+
+/* FieldElementwise::set_time (field_elementwise_impl.hh:92-117) ->
+ * GmshMeshReader::read_element_data (msh_gmshreader.cc:306-397): applies, in
+ * file order, every not yet applied $ElementData section of the field whose
+ * time is <= time * (1 + 2 eps); rows are matched to the element ids in
+ * increasing order.  *updated = 1 when a section was applied.  time = +inf
+ * changes nothing.  A field with no section at all is an error on the first call. */
+int fl_element_data_set_time(fl_element_data *f, double time, int *updated);
+
+/* FieldElementwise::value over all elements (field_elementwise_impl.hh:144-156):
+ * values[n_entities * n_comp], element by element. */
+int fl_element_data_values(const fl_element_data *f, double *values);
+
+/* info[3] = n_entities, n_comp, rows stored by the last section applied;
+ * the warnings of the last set_time, '\n'-separated, into buf. */
+int fl_element_data_info(const fl_element_data *f, int *info, char *buf, int buflen);
+
+void fl_element_data_free(fl_element_data *f);
+
+/* OutputMSH::write_head (src/io/output_msh.cc:53-98, 424-435): creates path
+ * with $MeshFormat, $Nodes (index + 1, %g) and $Elements (bulk elements,
+ * index + 1, gmsh type, 3 tags: region, region, partition). */
+int fl_msh_write_head(const fl_mesh *m, const char *path);
+
+/* OutputMSH::write_msh_elem_data with write_msh_ascii_cont_data
+ * (output_msh.cc:100-188, 383-408): appends to path one $ElementData section
+ * "name_[units]" at `time` with time index `step`, n_values rows
+ * "id v1 .. vn" (ids 1..n_values; rows of more than one value end with a
+ * blank).  precision = significant digits of the time and the values; the
+ * reference's stream default is 6. */
+int fl_msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
+                              int n_comp, int n_values, const double *values, int precision);
 
 /* ---- steady MH Darcy flow -------------------------------------------------- */
 

@@ -1,5 +1,6 @@
 // C ABI over the host mesh module and the HIP solvers (include/flow123d_gpu.h).
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 
@@ -9,6 +10,7 @@
 #include "darcy.hh"
 #include "gnn.hh"
 #include "mesh.hh"
+#include "msh_data.hh"
 #include "transport.hh"
 
 struct fl_mesh {
@@ -27,6 +29,13 @@ struct fl_gnn {
 struct fl_transport {
     std::unique_ptr<fl::TransportGPU> g;
     fl_mesh *mesh;
+};
+struct fl_element_data {
+    fl::ElementDataReader reader;
+    std::vector<int> ids;
+    std::vector<double> data;
+    std::string field;
+    int n_comp = 1;
 };
 
 static thread_local std::string g_err;
@@ -132,6 +141,87 @@ int fl_mesh_regions(const fl_mesh *mh, int *ids, int *is_boundary, int *sub_idx,
     }
     std::snprintf(buf, buflen, "%s", all.c_str());
     return 0;
+}
+
+int fl_mesh_element_ids(const fl_mesh *mh, int boundary, int *ids) {
+    return guard([&] {
+        std::vector<int> v;
+        std::string err = fl::element_id_map(mh->m, boundary != 0, v);
+        if (!err.empty()) throw fl::Error(-1, err);
+        if (!v.empty()) std::memcpy(ids, v.data(), v.size() * sizeof(int));
+        return 0;
+    });
+}
+
+int fl_element_data_open(const fl_mesh *m, const char *gmsh_file, const char *field_name, int n_comp, int boundary,
+                         fl_element_data **out) {
+    return guard([&] {
+        if (!m || !gmsh_file || !field_name || !out) throw fl::Error(-3, "null argument");
+        if (n_comp < 1) throw fl::Error(-3, "n_comp must be positive");
+        std::unique_ptr<fl_element_data> f(new fl_element_data());
+        std::string err = fl::element_id_map(m->m, boundary != 0, f->ids);
+        if (err.empty()) err = f->reader.open(gmsh_file);
+        if (!err.empty()) throw fl::Error(-1, err);
+        f->field = field_name;
+        f->n_comp = n_comp;
+        f->data.assign(f->ids.size() * (size_t)n_comp, 0.0);
+        *out = f.release();
+        return 0;
+    });
+}
+
+int fl_element_data_set_time(fl_element_data *f, double time, int *updated) {
+    return guard([&] {
+        if (updated) *updated = 0;
+        if (time == std::numeric_limits<double>::infinity()) return 0;
+        fl::MshDataHeader search;
+        search.actual = false;
+        search.field_name = f->field;
+        search.n_components = (unsigned)f->n_comp;
+        search.n_entities = (unsigned)f->ids.size();
+        search.time = time;
+        std::string err = f->reader.read(search, f->data.data(), f->ids);
+        if (!err.empty()) throw fl::Error(-1, err);
+        if (updated) *updated = search.actual ? 1 : 0;
+        return 0;
+    });
+}
+
+int fl_element_data_values(const fl_element_data *f, double *values) {
+    if (!f->data.empty()) std::memcpy(values, f->data.data(), f->data.size() * sizeof(double));
+    return 0;
+}
+
+int fl_element_data_info(const fl_element_data *f, int *info, char *buf, int buflen) {
+    info[0] = (int)f->ids.size();
+    info[1] = f->n_comp;
+    info[2] = f->reader.n_read;
+    if (buf && buflen > 0) {
+        std::string all;
+        for (const auto &w : f->reader.warnings) all += w + "\n";
+        std::snprintf(buf, (size_t)buflen, "%s", all.c_str());
+    }
+    return 0;
+}
+
+void fl_element_data_free(fl_element_data *f) { delete f; }
+
+int fl_msh_write_head(const fl_mesh *m, const char *path) {
+    return guard([&] {
+        std::string err = fl::msh_write_head(m->m, path);
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
+}
+
+int fl_msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
+                              int n_comp, int n_values, const double *values, int precision) {
+    return guard([&] {
+        std::string err =
+            fl::msh_write_element_data(path, name, units, time, step, n_comp, n_values, values, precision);
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
 }
 
 int fl_darcy_create(fl_mesh *m, fl_darcy **out) {

@@ -34,7 +34,7 @@ int region_add_unnamed(HostMesh &m, int id) {
     return region_add(m, id, "region_" + std::to_string(id), false);
 }
 
-static void add_element(HostMesh &m, int id, int dim, int region_id, const int *nodes) {
+static void add_element(HostMesh &m, int id, int dim, int region_id, const int *nodes, int pid = 0) {
     int r = region_add_unnamed(m, region_id);
     if (m.regions[r].boundary) {
         m.bc_id.push_back(id);
@@ -45,6 +45,7 @@ static void add_element(HostMesh &m, int id, int dim, int region_id, const int *
         m.el_id.push_back(id);
         m.el_dim.push_back(dim);
         m.el_region.push_back(r);
+        m.el_pid.push_back(pid);
         for (int k = 0; k < 4; k++) m.el_nodes.push_back(k <= dim ? nodes[k] : -1);
     }
 }
@@ -124,10 +125,11 @@ std::string read_gmsh(const char *path, HostMesh &m) {
         int dim = gmsh_dim(type);
         if (dim < 0) return "unsupported element type " + std::to_string(type);
         if (ntags < 2) return "at least two element tags are required";
-        int region = 0;
+        int region = 0, pid = 0;
         for (int t = 0; t < ntags; t++) {
             long v = std::strtol(q, &q, 10);
             if (t == 0) region = (int)v;
+            if (t == 2) pid = (int)v;  // partition number (msh_gmshreader.cc:174-175)
         }
         int nodes[4] = {-1, -1, -1, -1};
         for (int k = 0; k <= dim; k++) {
@@ -136,7 +138,7 @@ std::string read_gmsh(const char *path, HostMesh &m) {
             if (it == node_index.end()) return "unknown node id in element " + std::to_string(id);
             nodes[k] = it->second;
         }
-        add_element(m, id, dim, region, nodes);
+        add_element(m, id, dim, region, nodes, pid);
     }
     m.n_file_bc = (int)m.bc_dim.size();
     return "";

@@ -34,6 +34,7 @@ struct HostMesh {
     // bulk elements (file order)
     std::vector<int> el_id, el_dim, el_region;  // el_region: index into regions
     std::vector<int> el_nodes;                  // 4 * E, -1 padded
+    std::vector<int> el_pid;                    // partition tag (third gmsh tag, 0 when absent)
     // boundary elements (file ones first, implicit ones appended)
     std::vector<int> bc_id, bc_dim, bc_region, bc_nodes;  // bc_nodes: 4 * NBC
     int n_file_bc = 0;

@@ -61,6 +61,16 @@ SIGNATURES = {
     "fl_mesh_counts": (C.c_int, [C.c_void_p, _i32p]),
     "fl_mesh_topology": (C.c_int, [C.c_void_p] + [_i32p] * 13),
     "fl_mesh_regions": (C.c_int, [C.c_void_p, _i32p, _i32p, _i32p, C.c_char_p, C.c_int]),
+    "fl_mesh_element_ids": (C.c_int, [C.c_void_p, C.c_int, _i32p]),
+    "fl_element_data_open": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    "fl_element_data_set_time": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
+    "fl_element_data_values": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_element_data_info": (C.c_int, [C.c_void_p, _i32p, C.c_char_p, C.c_int]),
+    "fl_element_data_free": (None, [C.c_void_p]),
+    "fl_msh_write_head": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "fl_msh_write_element_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
+                                            C.c_int, _f64p, C.c_int]),
     "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "fl_darcy_free": (None, [C.c_void_p]),
     "fl_darcy_set_data": (C.c_int, [C.c_void_p, _f64p, _f64p, C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p,
@@ -214,6 +224,88 @@ class Mesh:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.fl_mesh_free(self.h)
             self.h = None
+
+
+def element_ids(mesh, boundary=False):
+    """Mesh::elements_id_maps (mesh.cc:781-812): gmsh ids of the bulk elements, or of
+    the boundary elements with -1 for the implicit ones."""
+    n = mesh.n_bc_elements if boundary else mesh.n_elements
+    out = np.zeros(max(n, 1), np.int32)
+    _check(lib().fl_mesh_element_ids(mesh.h, 1 if boundary else 0, out))
+    return out[:n]
+
+
+class FieldElementwise:
+    """Element-wise field read from gmsh `$ElementData` sections
+    (FieldElementwise, field_elementwise_impl.hh:67-156, over
+    GmshMeshReader::read_element_data, msh_gmshreader.cc:306-397).
+
+    ``set_time(t)`` applies every section of the field with time <= t not yet
+    applied and returns whether the values changed; ``value()`` returns them,
+    shape (n_entities,) for one component, else (n_entities, n_comp)."""
+
+    def __init__(self, mesh, gmsh_file, field_name, n_comp=1, boundary=False):
+        self.mesh = mesh
+        self.n_comp = int(n_comp)
+        h = C.c_void_p()
+        _check(lib().fl_element_data_open(mesh.h, os.fsencode(gmsh_file), field_name.encode(), self.n_comp,
+                                          1 if boundary else 0, C.byref(h)))
+        self.h = h
+        self.n_entities = mesh.n_bc_elements if boundary else mesh.n_elements
+
+    def set_time(self, t):
+        u = C.c_int()
+        _check(lib().fl_element_data_set_time(self.h, float(t), C.byref(u)))
+        return bool(u.value)
+
+    def value(self):
+        out = np.zeros(max(self.n_entities * self.n_comp, 1))
+        _check(lib().fl_element_data_values(self.h, out))
+        out = out[:self.n_entities * self.n_comp]
+        return out if self.n_comp == 1 else out.reshape(self.n_entities, self.n_comp)
+
+    def info(self):
+        """(rows stored by the last section applied, warnings of the last set_time)."""
+        i = np.zeros(3, np.int32)
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().fl_element_data_info(self.h, i, buf, len(buf)))
+        return int(i[2]), [w for w in buf.value.decode().split("\n") if w]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fl_element_data_free(self.h)
+            self.h = None
+
+
+class OutputMSH:
+    """gmsh 2.x ASCII output of per-element results (OutputMSH, output_msh.cc:53-98,
+    100-188, 383-467): the mesh head once, then one `$ElementData` section per field
+    and output time.  ``precision`` = significant digits (the reference's 6 by default)."""
+
+    def __init__(self, mesh, path, precision=6):
+        self.mesh, self.path, self.precision = mesh, os.fsencode(path), int(precision)
+        self.header_written = False
+        self.current_step = 0
+
+    def write_head(self):
+        _check(lib().fl_msh_write_head(self.mesh.h, self.path))
+        self.header_written = True
+
+    def write_elem_data(self, name, units, values, time, step):
+        v = np.ascontiguousarray(values, np.float64)
+        n = v.shape[0]
+        ncomp = 1 if v.ndim == 1 else int(np.prod(v.shape[1:]))
+        _check(lib().fl_msh_write_element_data(self.path, name.encode(), units.encode(), float(time), int(step),
+                                               ncomp, n, v.reshape(-1) if n else np.zeros(1), self.precision))
+
+    def write_data(self, time, fields):
+        """OutputMSH::write_data(time) (output_msh.cc:440-462): `fields` is a list of
+        (name, units, values[E] or values[E, n_comp]) in registration order."""
+        if not self.header_written:
+            self.write_head()
+        for name, units, v in fields:
+            self.write_elem_data(name, units, v, time, self.current_step)
+        self.current_step += 1
 
 
 class DarcyFlowMH_Steady:
