@@ -26,6 +26,7 @@ typedef struct fl_mesh fl_mesh;
 typedef struct fl_darcy fl_darcy;
 typedef struct fl_transport fl_transport;
 typedef struct fl_element_data fl_element_data;
+typedef struct fl_old_bcd fl_old_bcd;
 
 /* Message of the last failed call on this thread. */
 const char *fl_last_error(void);
@@ -112,6 +113,20 @@ int fl_msh_write_head(const fl_mesh *m, const char *path);
  * reference's stream default is 6. */
 int fl_msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
                               int n_comp, int n_values, const double *values, int precision);
+
+/* OldBcdInput::read_flow (src/flow/old_bcd.cc:44-205): the legacy flow
+ * boundary file (.fbc) onto the NBC boundary elements of m: bc_type (0 none,
+ * 1 Dirichlet, 2 Neumann, 3 Newton/Robin), bc_pressure, bc_flux, bc_sigma, ready
+ * for fl_darcy_set_data; elements without a row keep zeros.  The handle keeps
+ * the file's row ids for fl_old_bcd_read_transport. */
+int fl_old_bcd_read_flow(const fl_mesh *m, const char *path, int *bc_type, double *bc_pressure, double *bc_flux,
+                         double *bc_sigma, fl_old_bcd **out);
+
+/* OldBcdInput::read_transport (old_bcd.cc:209-250): the legacy transport
+ * boundary file (.tbc), bc_conc[n_subst][NBC] as fl_transport_create takes it. */
+int fl_old_bcd_read_transport(const fl_old_bcd *b, const char *path, int n_subst, double *bc_conc);
+
+void fl_old_bcd_free(fl_old_bcd *b);
 
 /* ---- steady MH Darcy flow -------------------------------------------------- */
 

@@ -11,6 +11,7 @@
 #include "gnn.hh"
 #include "mesh.hh"
 #include "msh_data.hh"
+#include "old_bcd.hh"
 #include "transport.hh"
 
 struct fl_mesh {
@@ -29,6 +30,10 @@ struct fl_gnn {
 struct fl_transport {
     std::unique_ptr<fl::TransportGPU> g;
     fl_mesh *mesh;
+};
+struct fl_old_bcd {
+    fl::OldBcd b;
+    const fl_mesh *mesh;
 };
 struct fl_element_data {
     fl::ElementDataReader reader;
@@ -223,6 +228,29 @@ int fl_msh_write_element_data(const char *path, const char *name, const char *un
         return 0;
     });
 }
+
+int fl_old_bcd_read_flow(const fl_mesh *m, const char *path, int *bc_type, double *bc_pressure, double *bc_flux,
+                         double *bc_sigma, fl_old_bcd **out) {
+    return guard([&] {
+        std::unique_ptr<fl_old_bcd> h(new fl_old_bcd());
+        h->mesh = m;
+        std::string err = h->b.read_flow(m->m, path, bc_type, bc_pressure, bc_flux, bc_sigma);
+        if (!err.empty()) throw fl::Error(-1, err);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int fl_old_bcd_read_transport(const fl_old_bcd *h, const char *path, int n_subst, double *bc_conc) {
+    return guard([&] {
+        if (n_subst < 1) throw fl::Error(-3, "n_subst must be positive");
+        std::string err = h->b.read_transport(h->mesh->m, path, n_subst, bc_conc);
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
+}
+
+void fl_old_bcd_free(fl_old_bcd *h) { delete h; }
 
 int fl_darcy_create(fl_mesh *m, fl_darcy **out) {
     return guard([&] {

@@ -71,6 +71,9 @@ SIGNATURES = {
     "fl_msh_write_head": (C.c_int, [C.c_void_p, C.c_char_p]),
     "fl_msh_write_element_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
                                             C.c_int, _f64p, C.c_int]),
+    "fl_old_bcd_read_flow": (C.c_int, [C.c_void_p, C.c_char_p, _i32p, _f64p, _f64p, _f64p, C.POINTER(C.c_void_p)]),
+    "fl_old_bcd_read_transport": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _f64p]),
+    "fl_old_bcd_free": (None, [C.c_void_p]),
     "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "fl_darcy_free": (None, [C.c_void_p]),
     "fl_darcy_set_data": (C.c_int, [C.c_void_p, _f64p, _f64p, C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p,
@@ -275,6 +278,41 @@ class FieldElementwise:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.fl_element_data_free(self.h)
             self.h = None
+
+
+class OldBcdInput:
+    """Legacy boundary files (OldBcdInput, old_bcd.cc:44-250).  ``read_flow(path)``
+    returns per boundary element (bc_type, bc_pressure, bc_flux, bc_sigma);
+    ``read_transport(path, n_subst)`` returns bc_conc[n_subst][NBC]."""
+
+    def __init__(self, mesh):
+        self.mesh, self.h = mesh, None
+
+    def read_flow(self, path):
+        NB = self.mesh.n_bc_elements
+        t = np.zeros(max(NB, 1), np.int32)
+        p, f, s = np.zeros(max(NB, 1)), np.zeros(max(NB, 1)), np.zeros(max(NB, 1))
+        h = C.c_void_p()
+        _check(lib().fl_old_bcd_read_flow(self.mesh.h, os.fsencode(path), t, p, f, s, C.byref(h)))
+        self._free()
+        self.h = h
+        return t[:NB], p[:NB], f[:NB], s[:NB]
+
+    def read_transport(self, path, n_subst=1):
+        if self.h is None:
+            raise FlowError("read_flow must come first: transport rows refer to the flow file's rows")
+        NB = self.mesh.n_bc_elements
+        c = np.zeros(max(n_subst * NB, 1))
+        _check(lib().fl_old_bcd_read_transport(self.h, os.fsencode(path), int(n_subst), c))
+        return c[:n_subst * NB].reshape(n_subst, NB)
+
+    def _free(self):
+        if self.h and _LIB is not None:
+            _LIB.fl_old_bcd_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        self._free()
 
 
 class OutputMSH:
