@@ -128,6 +128,24 @@ int fl_old_bcd_read_transport(const fl_old_bcd *b, const char *path, int n_subst
 
 void fl_old_bcd_free(fl_old_bcd *b);
 
+/* ---- balance files --------------------------------------------------------- */
+
+/* DarcyFlowMHOutput::water_balance (src/flow/darcy_flow_mh_output.cc:671-773):
+ * appends one water-balance block to path from per-region arrays in the
+ * region order of m (boundary index / bulk index): bal, plus, minus
+ * [n_boundary_regions], src [n_bulk_regions]. */
+int fl_write_water_balance(const fl_mesh *m, const char *path, double time, const double *bal, const double *plus,
+                           const double *minus, const double *src);
+
+/* MassBalance::output (src/transport/mass_balance.cc:242-354): appends one
+ * mass-balance block; arrays as fl_transport_mass_balance returns them
+ * (bal/plus/minus[n_subst][n_boundary_regions], mass/src[n_subst][n_bulk_regions],
+ * totals[n_subst][4] = initial mass, integrated flux, integrated sources,
+ * current mass), subst_names[n_subst]. */
+int fl_write_mass_balance(const fl_mesh *m, const char *path, double time, double initial_time, int n_subst,
+                          const char *const *subst_names, const double *bal, const double *plus, const double *minus,
+                          const double *mass, const double *src, const double *totals);
+
 /* ---- steady MH Darcy flow -------------------------------------------------- */
 
 /* DarcyFlowMH_Steady::DarcyFlowMH_Steady (src/flow/darcy_flow_mh.cc:249-367):
@@ -205,6 +223,10 @@ int fl_darcy_water_balance(fl_darcy *d, double *balance, double *outflow, double
  * nnz columns / values, and the rhs.  nnz via fl_darcy_edge_nnz. */
 int fl_darcy_edge_nnz(fl_darcy *d);
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs);
+
+/* DarcyFlowMHOutput::water_balance for the current solution: boundary side
+ * fluxes and the |e| cs f sources per region, one block appended to path. */
+int fl_darcy_write_water_balance(fl_darcy *d, const char *path, double time);
 
 /* ---- batched Monte-Carlo solves ------------------------------------------- */
 
@@ -354,6 +376,12 @@ int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl);
  * sources, current mass. */
 int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass, double *src,
                               double *totals);
+
+/* MassBalance::output at the current time of the transport: one block
+ * appended to path.  Before the first step the block shows the initial state
+ * (initial mass = current mass, interval [t, t]), as the reference's output at
+ * the start time does. */
+int fl_transport_write_mass_balance(fl_transport *t, const char *path, const char *const *subst_names);
 
 /* Concentration sources (ConvectionTransport::compute_concentration_sources,
  * transport.cc:501-544): sources_density, sources_sigma, sources_conc, each

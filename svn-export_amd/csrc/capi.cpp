@@ -12,6 +12,7 @@
 #include "mesh.hh"
 #include "msh_data.hh"
 #include "old_bcd.hh"
+#include "balance_out.hh"
 #include "transport.hh"
 
 struct fl_mesh {
@@ -252,6 +253,39 @@ int fl_old_bcd_read_transport(const fl_old_bcd *h, const char *path, int n_subst
 
 void fl_old_bcd_free(fl_old_bcd *h) { delete h; }
 
+int fl_write_water_balance(const fl_mesh *m, const char *path, double time, const double *bal, const double *plus,
+                           const double *minus, const double *src) {
+    return guard([&] {
+        std::vector<fl::RegionRow> bnd, blk;
+        fl::region_sets(m->m, bnd, blk);
+        std::string err = fl::write_water_balance(path, time, bnd, bal, plus, minus, blk, src);
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
+}
+
+int fl_write_mass_balance(const fl_mesh *m, const char *path, double time, double initial_time, int n_subst,
+                          const char *const *subst_names, const double *bal, const double *plus, const double *minus,
+                          const double *mass, const double *src, const double *totals) {
+    return guard([&] {
+        if (n_subst < 1) throw fl::Error(-3, "n_subst must be positive");
+        std::vector<fl::RegionRow> bnd, blk;
+        fl::region_sets(m->m, bnd, blk);
+        std::vector<std::string> names(n_subst);
+        std::vector<double> a(n_subst), b(n_subst), c(n_subst);
+        for (int k = 0; k < n_subst; k++) {
+            names[k] = subst_names && subst_names[k] ? subst_names[k] : "substance_" + std::to_string(k);
+            a[k] = totals[4 * k];
+            c[k] = totals[4 * k + 1];
+            b[k] = totals[4 * k + 2];
+        }
+        std::string err = fl::write_mass_balance(path, time, initial_time, n_subst, names, bnd, bal, plus, minus,
+                                                 blk, mass, src, a.data(), b.data(), c.data());
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
+}
+
 int fl_darcy_create(fl_mesh *m, fl_darcy **out) {
     return guard([&] {
         check_device();
@@ -339,6 +373,20 @@ int fl_darcy_element_balance(fl_darcy *d, double *out) {
 int fl_darcy_water_balance(fl_darcy *d, double *b, double *p, double *m) {
     return guard([&] {
         d->g->water_balance(b, p, m);
+        return 0;
+    });
+}
+
+int fl_darcy_write_water_balance(fl_darcy *d, const char *path, double time) {
+    return guard([&] {
+        const fl::HostMesh &m = d->mesh->m;
+        std::vector<double> b(m.n_boundary_regions + 1), p(b.size()), mi(b.size()), src(m.n_bulk_regions + 1);
+        d->g->water_balance(b.data(), p.data(), mi.data());
+        d->g->source_balance(src.data());
+        std::vector<fl::RegionRow> bnd, blk;
+        fl::region_sets(m, bnd, blk);
+        std::string err = fl::write_water_balance(path, time, bnd, b.data(), p.data(), mi.data(), blk, src.data());
+        if (!err.empty()) throw fl::Error(-1, err);
         return 0;
     });
 }
@@ -682,6 +730,20 @@ int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double
         t->g->mass_balance(bal, plus, minus, mass, src, totals);
         return 0;
     });
+}
+
+int fl_transport_write_mass_balance(fl_transport *t, const char *path, const char *const *subst_names) {
+    const fl::HostMesh &m = t->mesh->m;
+    const int ns = t->g->n_subst(), nbr = m.n_boundary_regions, nblk = m.n_bulk_regions;
+    std::vector<double> bal((size_t)ns * nbr + 1), plus(bal.size()), minus(bal.size()),
+        mass((size_t)ns * nblk + 1), src(mass.size()), tot((size_t)4 * ns);
+    int rc = guard([&] {
+        t->g->mass_balance(bal.data(), plus.data(), minus.data(), mass.data(), src.data(), tot.data());
+        return 0;
+    });
+    if (rc) return rc;
+    return fl_write_mass_balance(t->mesh, path, t->g->time(), t->g->balance_initial_time(), ns, subst_names,
+                                 bal.data(), plus.data(), minus.data(), mass.data(), src.data(), tot.data());
 }
 
 int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc) {

@@ -80,6 +80,7 @@ class DarcyGPU {
     void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
     void element_balance(double *out) const;                // [E]
     void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region
+    void source_balance(double *src) const;                                // per bulk region
     double solution_precision(double rtol, double atol) const;
     int nnz() const { return nnz_; }
     const HostMesh &mesh() const { return m_; }

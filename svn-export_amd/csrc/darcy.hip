@@ -759,6 +759,17 @@ void DarcyGPU::water_balance(double *bal, double *plus, double *minus) const {
     }
 }
 
+// DarcyFlowMHOutput::water_balance sources (darcy_flow_mh_output.cc:743-751):
+// |e| cs f summed per bulk region
+void DarcyGPU::source_balance(double *src) const {
+    const HostMesh &m = m_;
+    std::fill(src, src + m.n_bulk_regions, 0.0);
+    if (h_cs_.empty()) return;
+    for (int e = 0; e < m.n_elements(); e++)
+        src[m.regions[m.el_region[e]].sub_idx] +=
+            element_measure(m, m.el_dim[e], &m.el_nodes[4 * e]) * h_cs_[e] * h_f_[e];
+}
+
 double DarcyGPU::solution_precision(double rtol, double atol) const {
     // darcy_flow_mh.cc:463-482 uses the norm of the first Schur complement's
     // rhs; the condensed system's rhs norm is reported here.

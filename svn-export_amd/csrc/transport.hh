@@ -80,6 +80,8 @@ class TransportGPU {
     void mass_balance(double *bal, double *plus, double *minus, double *mass, double *src, double *totals);
     double time() const { return tg_.time; }
     double dt() const { return tg_.time_step; }
+    double balance_initial_time() const { return mb_init_ ? tg_.time : mb_t0_; }
+    int n_subst() const { return nsub_; }
     double cfl() const { return cfl_max_step_; }
     int nnz() const { return nnz_; }
     int nsub() const { return nsub_; }
@@ -117,6 +119,7 @@ class TransportGPU {
     bool matrix_built_ = false, setup_done_ = false, scaled_ = false, need_rescale_ = true, flux_set_ = false;
     bool mb_init_ = true;
     double last_time_ = -1;
+    double mb_t0_ = 0;  // time of the first balance (MassBalance::initial_time)
     std::vector<double> h_meas_, h_cs_, h_por_, h_bc_conc_, initial_mass_;
     std::vector<double> h_src_density_, h_src_sigma_, h_src_conc_;
     std::vector<int> h_el_region_sub_;

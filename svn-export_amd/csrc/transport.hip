@@ -622,7 +622,7 @@ void TransportGPU::mb_initial() {
     initial_mass_.assign(nsub_, 0.0);
     for (int k = 0; k < nsub_; k++)
         for (int i = 0; i < E_; i++) initial_mass_[k] += h_por_[i] * h_cs_[i] * c[(size_t)i * nsub_ + k] * h_meas_[i];
-    last_time_ = tg_.time;
+    last_time_ = mb_t0_ = tg_.time;
     mb_init_ = false;
 }
 
@@ -805,7 +805,7 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
         for (int i = 0; i < E_; i++) {
             int r = h_el_region_sub_[i];
             mass[k * nblk + r] += h_por_[i] * h_cs_[i] * cur[(size_t)i * ns + k] * h_meas_[i];
-            if (t_.has_src && stats_.steps > 0) {
+            if (t_.has_src) {  // before the first step the previous concentration is zero (transport.cc:433-435)
                 size_t j = (size_t)k * E_ + i;
                 double diff = h_src_conc_[j] - prev[(size_t)i * ns + k];
                 double snd = diff > 0.0 ? (h_src_density_[j] + diff * h_src_sigma_[j]) : h_src_density_[j];
@@ -815,7 +815,7 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
     for (int k = 0; k < ns; k++) {
         double mt = 0.0;
         for (int r = 0; r < nblk; r++) mt += mass[k * nblk + r];
-        totals[4 * k + 0] = k < (int)initial_mass_.size() ? initial_mass_[k] : 0.0;
+        totals[4 * k + 0] = k < (int)initial_mass_.size() ? initial_mass_[k] : mt;  // first output: current mass
         totals[4 * k + 1] = mb[1 + k];
         totals[4 * k + 2] = mb[1 + ns + k];
         totals[4 * k + 3] = mt;

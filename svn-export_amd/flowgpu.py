@@ -74,6 +74,11 @@ SIGNATURES = {
     "fl_old_bcd_read_flow": (C.c_int, [C.c_void_p, C.c_char_p, _i32p, _f64p, _f64p, _f64p, C.POINTER(C.c_void_p)]),
     "fl_old_bcd_read_transport": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _f64p]),
     "fl_old_bcd_free": (None, [C.c_void_p]),
+    "fl_write_water_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, _f64p, _f64p, _f64p, _f64p]),
+    "fl_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                        _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    "fl_darcy_write_water_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "fl_darcy_free": (None, [C.c_void_p]),
     "fl_darcy_set_data": (C.c_int, [C.c_void_p, _f64p, _f64p, C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p,
@@ -315,6 +320,32 @@ class OldBcdInput:
         self._free()
 
 
+def _names(names):
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    return arr, C.cast(arr, C.c_void_p)
+
+
+def write_water_balance(mesh, path, time, bal, plus, minus, src):
+    """One DarcyFlowMHOutput::water_balance block (darcy_flow_mh_output.cc:671-773) from
+    per-region arrays (boundary regions, then bulk regions, in the mesh's region order)."""
+    nb, nk = max(mesh.n_boundary_regions, 1), max(mesh.n_bulk_regions, 1)
+    pad = lambda a, n: np.resize(np.ascontiguousarray(a, np.float64).reshape(-1), n) if np.size(a) else np.zeros(n)
+    _check(lib().fl_write_water_balance(mesh.h, os.fsencode(path), float(time), pad(bal, nb), pad(plus, nb),
+                                        pad(minus, nb), pad(src, nk)))
+
+
+def write_mass_balance(mesh, path, time, initial_time, names, mb):
+    """One MassBalance::output block (mass_balance.cc:242-354) from a mass_balance()
+    dictionary; names = substance names."""
+    ns = len(names)
+    keep, ptr = _names(names)
+    f = lambda k, n: np.resize(np.ascontiguousarray(mb[k], np.float64).reshape(-1), max(n, 1))
+    nb, nk = ns * mesh.n_boundary_regions, ns * mesh.n_bulk_regions
+    _check(lib().fl_write_mass_balance(mesh.h, os.fsencode(path), float(time), float(initial_time), ns, ptr,
+                                       f("bal", nb), f("plus", nb), f("minus", nb), f("mass", nk), f("src", nk),
+                                       f("totals", 4 * ns)))
+
+
 class OutputMSH:
     """gmsh 2.x ASCII output of per-element results (OutputMSH, output_msh.cc:53-98,
     100-188, 383-467): the mesh head once, then one `$ElementData` section per field
@@ -397,6 +428,10 @@ class DarcyFlowMH_Steady:
         _check(lib().fl_darcy_water_balance(self.h, b, p, m))
         k = self.mesh.n_boundary_regions
         return b[:k], p[:k], m[:k]
+
+    def write_water_balance(self, path, time=float("inf")):
+        """DarcyFlowMHOutput::water_balance block for this solution (steady: time inf)."""
+        _check(lib().fl_darcy_write_water_balance(self.h, os.fsencode(path), float(time)))
 
     def edge_system(self):
         nnz = lib().fl_darcy_edge_nnz(self.h)
@@ -613,6 +648,11 @@ class ConvectionTransport:
         return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
                     minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
                     src=src[:ns * nblk].reshape(ns, nblk), totals=tot.reshape(ns, 4))
+
+    def write_mass_balance(self, path, names):
+        """MassBalance::output block at the current time (mass_balance.cc:242-354)."""
+        keep, ptr = _names(names)
+        _check(lib().fl_transport_write_mass_balance(self.h, os.fsencode(path), ptr))
 
     def set_sources(self, density, sigma, conc):
         """sources_density, sources_sigma, sources_conc, each [n_subst][E] (transport.cc:501-544)."""
