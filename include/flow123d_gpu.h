@@ -340,7 +340,7 @@ int fl_gnn_features(fl_gnn *g, float *xe, float *xf);
 /* ---- explicit upwind transport -------------------------------------------- */
 
 /* ConvectionTransport::ConvectionTransport (src/transport/transport.cc:105-188)
- * with TransportOperatorSplitting's time governor: n_subst substances (1..4),
+ * with TransportOperatorSplitting's time governor: n_subst substances (1..8),
  * mobile porosity por[E], cross_section cs[E], bc_conc[n_subst][NBC],
  * init_conc[n_subst][E], end_time, save_step (output marks). */
 int fl_transport_create(fl_mesh *m, int n_subst, const double *por, const double *cs, const double *bc_conc,
@@ -382,6 +382,19 @@ int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double
  * (initial mass = current mass, interval [t, t]), as the reference's output at
  * the start time does. */
 int fl_transport_write_mass_balance(fl_transport *t, const char *path, const char *const *subst_names);
+
+/* Linear_reaction (src/reaction/linear_reaction.cc:144-300) as
+ * TransportOperatorSplitting drives it (transport_operator_splitting.cc:244-287):
+ * n_decays first-order decays, decay i with parent substance parent[i],
+ * products products[product_ptr[i] .. product_ptr[i+1]) with branch_ratio[..],
+ * and half_life[i] (a kinetic constant k is half_life = ln 2 / k,
+ * linear_reaction.cc:214-224).  At every fl_transport_update_solution the
+ * matrix R is rebuilt from the step estimate dt: identity, R[p][p] =
+ * 0.5^(dt/half_life), R[p][product] = (1 - 0.5^(dt/half_life)) * ratio; after
+ * each convection step every element's concentrations become c R.
+ * n_decays = 0 switches reactions off. */
+int fl_transport_set_linear_reactions(fl_transport *t, int n_decays, const int *parent, const int *product_ptr,
+                                      const int *products, const double *half_life, const double *branch_ratio);
 
 /* Concentration sources (ConvectionTransport::compute_concentration_sources,
  * transport.cc:501-544): sources_density, sources_sigma, sources_conc, each

@@ -1350,6 +1350,9 @@ struct Transport {
     std::vector<double> conc, pconc;  // [nsub][E]
     std::vector<double> bcvcorr;    // [nsub][E]
     std::vector<double> src_density, src_sigma, src_conc;  // [nsub][E], empty = no sources
+    // Linear_reaction: decays (parent, products, half-life, branch ratios), matrix [nsub][nsub]
+    std::vector<int> r_parent, r_pptr, r_prod;
+    std::vector<double> r_hl, r_branch, reaction_matrix;
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -1557,6 +1560,37 @@ static void one_step(Transport &T) {
 }
 
 // set_target_time: transport.cc:718-733
+// Linear_reaction::set_time_step -> allocate_reaction_matrix + modify_reaction_matrix
+// (linear_reaction.cc:73-95, 144-165, 278-285)
+static void set_reaction_time_step(Transport &T, double time_step) {
+    int n = T.nsub;
+    T.reaction_matrix.assign((size_t)n * n, 0.0);
+    for (int rows = 0; rows < n; rows++)
+        for (int cols = 0; cols < n; cols++) T.reaction_matrix[(size_t)rows * n + cols] = rows == cols ? 1.0 : 0.0;
+    for (size_t i_decay = 0; i_decay < T.r_parent.size(); i_decay++) {
+        int index_par = T.r_parent[i_decay];
+        double rel_step = time_step / T.r_hl[i_decay];
+        T.reaction_matrix[(size_t)index_par * n + index_par] = std::pow(0.5, rel_step);
+        for (int k = T.r_pptr[i_decay]; k < T.r_pptr[i_decay + 1]; k++)
+            T.reaction_matrix[(size_t)index_par * n + T.r_prod[k]] = (1 - std::pow(0.5, rel_step)) * T.r_branch[k];
+    }
+}
+
+// Linear_reaction::compute_one_step / compute_reaction (linear_reaction.cc:167-183, 287-300)
+static void compute_reactions(Transport &T) {
+    int n = T.nsub;
+    std::vector<double> prev_conc(n);
+    for (int e = 0; e < T.E; e++) {
+        for (int cols = 0; cols < n; cols++) {
+            prev_conc[cols] = T.conc[(size_t)cols * T.E + e];
+            T.conc[(size_t)cols * T.E + e] = 0.0;
+        }
+        for (int rows = 0; rows < n; rows++)
+            for (int cols = 0; cols < n; cols++)
+                T.conc[(size_t)rows * T.E + e] += prev_conc[cols] * T.reaction_matrix[(size_t)cols * n + rows];
+    }
+}
+
 static void set_target(Transport &T, double t) {
     T.tg.add_mark(t);
     T.tg.set_upper_constraint(T.cfl_max_step);
@@ -1605,6 +1639,17 @@ void *orc_transport_create(void *mesh, const double *side_flux, const double *cs
 }
 void orc_transport_free(void *h) { delete (Transport *)h; }
 
+// Linear reactions: n decays, parent[n], products in CSR (pptr[n+1], prod[]), half-life[n], branch ratios
+void orc_transport_set_reactions(void *h, int n, const int *parent, const int *pptr, const int *prod,
+                                 const double *half_life, const double *branch) {
+    Transport &T = *(Transport *)h;
+    T.r_parent.assign(parent, parent + n);
+    T.r_pptr.assign(pptr, pptr + n + 1);
+    T.r_prod.assign(prod, prod + pptr[n]);
+    T.r_hl.assign(half_life, half_life + n);
+    T.r_branch.assign(branch, branch + pptr[n]);
+}
+
 void orc_transport_set_sources(void *h, const double *density, const double *sigma, const double *conc) {
     Transport &T = *(Transport *)h;
     size_t n = (size_t)T.nsub * T.E;
@@ -1619,10 +1664,12 @@ int orc_transport_update(void *h, double t) {
     Transport &T = *(Transport *)h;
     if (T.mb_initial) mb_calculate(T, T.tg.time);  // first_time_call output at t=0
     set_target(T, t);
+    if (!T.r_parent.empty()) set_reaction_time_step(T, T.tg.estimate_dt());
     int steps = 0;
     double t0 = now_s();
     while (T.tg.lt(t)) {
         one_step(T);
+        if (!T.r_parent.empty()) compute_reactions(T);
         mb_calculate(T, T.tg.time);
         steps++;
     }

@@ -63,6 +63,7 @@ def lib():
         L.orc_transport_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.orc_transport_set_reactions.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]
         _LIB = L
     return _LIB
 
@@ -246,6 +247,15 @@ class Transport:
     def set_sources(self, density, sigma, conc):
         self._src = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (density, sigma, conc)]
         lib().orc_transport_set_sources(self.h, *self._src)
+
+    def set_reactions(self, parent, products, half_life, branch):
+        """parent[n]; products: list of lists of substance indices; branch: same shape."""
+        pptr = np.concatenate([[0], np.cumsum([len(p) for p in products])]).astype(np.int32)
+        self._react = [np.ascontiguousarray(parent, np.int32), pptr,
+                       np.ascontiguousarray([q for p in products for q in p], np.int32),
+                       np.ascontiguousarray(half_life, np.float64),
+                       np.ascontiguousarray([b for p in branch for b in p], np.float64)]
+        lib().orc_transport_set_reactions(self.h, len(parent), *self._react)
 
     def step_seconds(self):
         return lib().orc_transport_step_seconds(self.h)

@@ -663,7 +663,7 @@ int fl_transport_create(fl_mesh *m, int n_subst, const double *por, const double
                         const double *init_conc, double end_time, double save_step, fl_transport **out) {
     return guard([&] {
         check_device();
-        if (n_subst < 1 || n_subst > 4) throw fl::Error(-5, "n_subst must be in 1..4");
+        if (n_subst < 1 || n_subst > fl::TransportGPU::kMaxSubst) throw fl::Error(-5, "n_subst must be in 1..8");
         auto *t = new fl_transport();
         t->mesh = m;
         try {
@@ -744,6 +744,14 @@ int fl_transport_write_mass_balance(fl_transport *t, const char *path, const cha
     if (rc) return rc;
     return fl_write_mass_balance(t->mesh, path, t->g->time(), t->g->balance_initial_time(), ns, subst_names,
                                  bal.data(), plus.data(), minus.data(), mass.data(), src.data(), tot.data());
+}
+
+int fl_transport_set_linear_reactions(fl_transport *t, int n_decays, const int *parent, const int *product_ptr,
+                                      const int *products, const double *half_life, const double *branch_ratio) {
+    return guard([&] {
+        t->g->set_linear_reactions(n_decays, parent, product_ptr, products, half_life, branch_ratio);
+        return 0;
+    });
 }
 
 int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc) {

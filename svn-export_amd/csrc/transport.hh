@@ -58,6 +58,8 @@ struct TmDev {
     const int *bl_ptr, *bl_side, *bl_bcele;  // representative boundary sides per element
     const double *meas, *cs, *por, *flux, *edge_flow, *bc_conc;
     const double *src_density, *src_sigma, *src_conc;  // [E][m] interleaved (when has_src)
+    const double *react;                               // [m][m] reaction matrix R[from][to] (when has_react)
+    int has_react;
     double *val, *bcv, *partial, *mb;
     double *ell_val;        // first entries of every row, column-major [5][E]
     int *ell_col, *ell_len; // their columns [5][E], row lengths [E]
@@ -66,12 +68,19 @@ struct TmDev {
 
 class TransportGPU {
    public:
+    static constexpr int kMaxSubst = 8;  // substances per handle
     // por, cs [E]; bc_conc [nsub][NBC]; init_conc [nsub][E] (host arrays)
     TransportGPU(const HostMesh &m, int nsub, const double *por, const double *cs, const double *bc_conc,
                  const double *init_conc, double end_time, double save_step);
     ~TransportGPU();
     // sources_density / sources_sigma / sources_conc, each [nsub][E] (transport.cc:501-544)
     void set_sources(const double *density, const double *sigma, const double *conc);
+    // Linear_reaction (linear_reaction.cc:144-220): decays i with parent[i],
+    // products[pptr[i]..pptr[i+1]), half_life[i] and branch ratios; the matrix is
+    // rebuilt from the step estimate at every update_solution and applied after
+    // each convection step.  n_decays = 0 switches reactions off.
+    void set_linear_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                              const double *half_life, const double *branch);
     void set_velocity_host(const double *side_flux);      // [S]
     void set_velocity_device(const double *d_side_flux);  // device [S] (e.g. the Darcy solution)
     double prepare();                                      // build the matrix now; returns the CFL step
@@ -97,7 +106,14 @@ class TransportGPU {
     void flush_flux();
     int step_grid() const;
     static constexpr int kMaxStepGrid = 16384;
+    static constexpr size_t kPartStride = (size_t)kMaxStepGrid * 2 * kMaxSubst;  // per-step partials
     bool quad_ = true;  // step kernel with one lane per (row, substance) for 2..4 substances
+    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_; }
+    void build_reaction_matrix(double dt);
+    bool react_on_ = false;
+    std::vector<int> r_parent_, r_pptr_, r_prod_;
+    std::vector<double> r_hl_, r_branch_, h_react_;
+    double *d_react_ = nullptr;
     bool rows2_ = false; // quad step kernel: two rows per lane group and step
     bool xcd_split_ = true;  // quad step kernel: each XCD one contiguous eighth of the rows
     int grid_cap_ = 0;       // step grid limit (0: 8192)

@@ -264,6 +264,16 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
             const double *cj = c_old + (size_t)t.col[k] * M;
             for (int s = 0; s < M; s++) acc[s] += a * cj[s];
         }
+        if (t.has_react) {
+            // Linear_reaction::compute_reaction (linear_reaction.cc:167-183): c_to = sum over
+            // c_from * R[from][to], from in increasing order, after the convection step
+            double r[M];
+            for (int q = 0; q < M; q++) {
+                r[q] = 0.0;
+                for (int s = 0; s < M; s++) r[q] += acc[s] * t.react[s * M + q];
+            }
+            for (int q = 0; q < M; q++) acc[q] = r[q];
+        }
         for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
         for (int k = t.bl_ptr[i]; k < t.bl_ptr[i + 1]; k++) {
             double wf = t.flux[t.bl_side[k]];
@@ -483,6 +493,7 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     hipStream_t s = stream_;
     TmDev &t = t_;
     t.E = E; t.nsub = nsub; t.NBC = NBC_; t.has_src = 0;
+    t.react = nullptr; t.has_react = 0;
     t.ptr = upv(owned_, ptr, s); t.col = upv(owned_, col, s); t.kind = upv(owned_, kind, s);
     t.s1 = upv(owned_, s1, s); t.s2 = upv(owned_, s2, s); t.diag = upv(owned_, diag, s);
     t.side_off = upv(owned_, m.side_off, s); t.side_edge = upv(owned_, m.side_edge, s);
@@ -517,7 +528,7 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     t.bcv = alloc_d((size_t)E * nsub);
     d_c0_ = alloc_d((size_t)E * nsub);
     d_c1_ = alloc_d((size_t)E * nsub);
-    t.partial = alloc_d((size_t)2 * kMaxStepGrid * 8 + 4096);
+    t.partial = alloc_d(2 * kPartStride + 4096);
     d_acc_ = alloc_d((size_t)kMaxStepGrid * 8);
     FL_HIP(hipMemsetAsync(d_acc_, 0, (size_t)kMaxStepGrid * 8 * sizeof(double), s));
     t.mb = alloc_d(1 + 2 * nsub);
@@ -562,6 +573,53 @@ void TransportGPU::set_sources(const double *density, const double *sigma, const
     t_.src_conc = upv(owned_, c, stream_);
     t_.has_src = 1;
     FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void TransportGPU::set_linear_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                                        const double *half_life, const double *branch) {
+    if (n_decays <= 0) {
+        react_on_ = false;
+        t_.has_react = 0;
+        return;
+    }
+    for (int i = 0; i < n_decays; i++) {
+        if (parent[i] < 0 || parent[i] >= nsub_) throw Error(-3, "Wrong name of parent substance in the " +
+                                                                  std::to_string(i) + "-th reaction.");
+        if (pptr[i + 1] <= pptr[i]) throw Error(-3, "Empty array of products in the " + std::to_string(i) + "-th reaction.");
+        if (!(half_life[i] > 0.0)) throw Error(-3, "half-life of the " + std::to_string(i) + "-th reaction must be positive");
+        for (int k = pptr[i]; k < pptr[i + 1]; k++)
+            if (products[k] < 0 || products[k] >= nsub_) throw Error(-3, "Wrong product in the " + std::to_string(i) + "-th reaction.");
+    }
+    r_parent_.assign(parent, parent + n_decays);
+    r_pptr_.assign(pptr, pptr + n_decays + 1);
+    r_prod_.assign(products, products + pptr[n_decays]);
+    r_hl_.assign(half_life, half_life + n_decays);
+    r_branch_.assign(branch, branch + pptr[n_decays]);
+    if (!d_react_) {
+        d_react_ = dev_alloc<double>((size_t)kMaxSubst * kMaxSubst);
+        owned_.push_back(d_react_);
+    }
+    react_on_ = true;
+    t_.react = d_react_;
+    t_.has_react = 1;
+    build_reaction_matrix(tg_.estimate_dt());
+}
+
+// Linear_reaction::modify_reaction_matrix (linear_reaction.cc:144-165): identity,
+// then per decay R[p][p] = 0.5^(dt / half_life), R[p][product_k] = (1 - that) * ratio_k
+void TransportGPU::build_reaction_matrix(double dt) {
+    const int n = nsub_;
+    h_react_.assign((size_t)n * n, 0.0);
+    for (int k = 0; k < n; k++) h_react_[(size_t)k * n + k] = 1.0;
+    for (size_t i = 0; i < r_parent_.size(); i++) {
+        const int p = r_parent_[i];
+        const double rel = dt / r_hl_[i];
+        h_react_[(size_t)p * n + p] = std::pow(0.5, rel);
+        for (int k = r_pptr_[i]; k < r_pptr_[i + 1]; k++)
+            h_react_[(size_t)p * n + r_prod_[k]] = (1 - std::pow(0.5, rel)) * r_branch_[k];
+    }
+    FL_HIP(hipMemcpyAsync(d_react_, h_react_.data(), h_react_.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));  // h_react_ is rebuilt at the next output interval
 }
 
 void TransportGPU::set_velocity_host(const double *side_flux) {
@@ -627,7 +685,7 @@ void TransportGPU::mb_initial() {
 }
 
 int TransportGPU::step_grid() const {
-    const int rows_per_block = (quad_ && nsub_ >= 2) ? 64 : kBlock;
+    const int rows_per_block = use_quad() ? 64 : kBlock;
     const int cap = grid_cap_ > 0 ? std::min(grid_cap_, kMaxStepGrid) : 8192;
     return std::max(8, std::min((E_ + rows_per_block - 1) / rows_per_block / 8 * 8, cap));
 }
@@ -643,12 +701,12 @@ void TransportGPU::flush_flux() {
     }
     if (!pending_) return;
     const int G = step_grid();
-    const double *last = t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8;
+    const double *last = t_.partial + (step_parity_ ? 0 : 1) * kPartStride;
     switch (nsub_) {
-        case 1: hipLaunchKernelGGL(k_flux_flush<1>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
-        case 2: hipLaunchKernelGGL(k_flux_flush<2>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
-        case 3: hipLaunchKernelGGL(k_flux_flush<3>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
-        case 4: hipLaunchKernelGGL(k_flux_flush<4>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+#define FL_FLUSH(MM) \
+    case MM: hipLaunchKernelGGL(k_flux_flush<MM>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+        FL_FLUSH(1) FL_FLUSH(2) FL_FLUSH(3) FL_FLUSH(4) FL_FLUSH(5) FL_FLUSH(6) FL_FLUSH(7) FL_FLUSH(8)
+#undef FL_FLUSH
     }
     FL_HIP(hipGetLastError());
     pending_ = false;
@@ -683,10 +741,10 @@ void TransportGPU::one_step() {
     const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
     double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
     const int G = step_grid();
-    double *out = t_.partial + (size_t)(step_parity_ ? 1 : 0) * kMaxStepGrid * 8;
-    const double *fold = pending_ ? t_.partial + (size_t)(step_parity_ ? 0 : 1) * kMaxStepGrid * 8 : nullptr;
+    double *out = t_.partial + (step_parity_ ? 1 : 0) * kPartStride;
+    const double *fold = pending_ ? t_.partial + (step_parity_ ? 0 : 1) * kPartStride : nullptr;
     double ftd = pending_tdiff_, dt = tg_.time_step;
-    if (quad_ && nsub_ >= 2) {
+    if (use_quad()) {
 #define FL_STEPQ(MM, RR) \
     hipLaunchKernelGGL((k_step_q<MM, RR>), dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, tdiff, d_acc_, \
                        xcd_split_ ? 1 : 0)
@@ -702,15 +760,15 @@ void TransportGPU::one_step() {
 #undef FL_STEPQ
     } else {
         switch (nsub_) {
-            case 1: hipLaunchKernelGGL(k_step<1>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-            case 2: hipLaunchKernelGGL(k_step<2>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-            case 3: hipLaunchKernelGGL(k_step<3>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-            case 4: hipLaunchKernelGGL(k_step<4>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
-            default: throw Error(-5, "transport supports 1..4 substances per handle");
+#define FL_STEP(MM) \
+    case MM: hipLaunchKernelGGL(k_step<MM>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+            FL_STEP(1) FL_STEP(2) FL_STEP(3) FL_STEP(4) FL_STEP(5) FL_STEP(6) FL_STEP(7) FL_STEP(8)
+#undef FL_STEP
+            default: throw Error(-5, "transport supports 1..8 substances per handle");
         }
     }
     FL_HIP(hipGetLastError());
-    if (quad_ && nsub_ >= 2) {
+    if (use_quad()) {
         acc_dirty_ = true;  // totals stay in the per-block accumulators until the output time
     } else {
         pending_ = true;
@@ -730,6 +788,8 @@ int TransportGPU::update_solution(double target) {
     tg_.set_upper_constraint(cfl_max_step_);
     tg_.fix_dt_until_mark();
     need_rescale_ = true;
+    // decayRad->set_time_step(convection->time().estimate_dt()) (transport_operator_splitting.cc:258)
+    if (react_on_) build_reaction_matrix(tg_.estimate_dt());
     hipEvent_t a, b;
     FL_HIP(hipEventCreate(&a)); FL_HIP(hipEventCreate(&b));
     FL_HIP(hipEventRecord(a, stream_));

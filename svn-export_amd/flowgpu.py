@@ -8,6 +8,7 @@ Every call goes through libflow123d_gpu.so; there is no CPU fallback: a
 missing library or device raises.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -135,6 +136,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
     "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 6),
     "fl_transport_set_sources": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
+    "fl_transport_set_linear_reactions": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]),
     "fl_transport_nnz": (C.c_int, [C.c_void_p]),
     "fl_transport_matrix": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p]),
     "fl_transport_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
@@ -648,6 +650,27 @@ class ConvectionTransport:
         return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
                     minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
                     src=src[:ns * nblk].reshape(ns, nblk), totals=tot.reshape(ns, 4))
+
+    def set_linear_reactions(self, decays, names=None):
+        """Linear_reaction decays (linear_reaction.cc:200-258): a list of dicts with
+        parent, products, optional branch_ratios (default one product, ratio 1.0) and
+        half_life or kinetic (half_life = ln 2 / kinetic).  Substances are given by
+        index or, with `names`, by name."""
+        idx = (lambda x: names.index(x) if isinstance(x, str) else int(x))
+        parent, pptr, prod, hl, br = [], [0], [], [], []
+        for d in decays:
+            parent.append(idx(d["parent"]))
+            ps = [idx(p) for p in d["products"]]
+            ratios = d.get("branch_ratios", [1.0])
+            if len(ratios) != len(ps):
+                raise FlowError("Number of branches %d has to match number of products %d" % (len(ratios), len(ps)))
+            prod += ps
+            br += list(ratios)
+            pptr.append(len(prod))
+            hl.append(d["half_life"] if "half_life" in d else math.log(2) / d["kinetic"])
+        self._react = [np.array(parent, np.int32), np.array(pptr, np.int32), np.array(prod or [0], np.int32),
+                       np.array(hl or [1.0], np.float64), np.array(br or [0.0], np.float64)]
+        _check(lib().fl_transport_set_linear_reactions(self.h, len(parent), *self._react))
 
     def write_mass_balance(self, path, names):
         """MassBalance::output block at the current time (mass_balance.cc:242-354)."""
