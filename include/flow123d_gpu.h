@@ -199,6 +199,23 @@ typedef struct {
 /* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433). */
 int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *opts, fl_solve_info *info);
 
+/* DarcyFlowMH_Unsteady (lumped = 0) or DarcyFlowLMH_Unsteady (lumped = 1)
+ * setup_time_term (darcy_flow_mh.cc:1601-1658, 1680-1749) on the data of
+ * fl_darcy_set_data: storativity[E], init_pressure[E].  The solution becomes
+ * the initial state: element pressures = init_pressure (MH), or edge traces =
+ * the sum of init_pressure / n_sides over each edge's sides (LMH), the rest 0. */
+int fl_darcy_set_unsteady(fl_darcy *d, int lumped, const double *storativity, const double *init_pressure);
+
+/* One implicit Euler step of length dt from the current solution
+ * (update_solution with modify_system, darcy_flow_mh.cc:1660-1677 / 1751-1770,
+ * and the LMH postprocess :1772-1827): MH adds S|e|/dt to each element's
+ * pressure row and S|e|/dt p_prev to its right-hand side; LMH adds
+ * |e| S cs / dt / n_sides per side to its edge's row and the same times the
+ * previous trace to its right-hand side, then corrects every side flux by the
+ * lumped storage and source terms.  A plain fl_darcy_update_solution afterwards
+ * solves the steady problem again. */
+int fl_darcy_unsteady_step(fl_darcy *d, double dt, const fl_solver_opts *opts, fl_solve_info *info);
+
 /* DarcyFlowMH_Steady::get_solution_vector (darcy_flow_mh.cc:485-500):
  * copies the [q | p | lambda] vector (S+E+F doubles) to out. */
 int fl_darcy_get_solution(fl_darcy *d, double *out);

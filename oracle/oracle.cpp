@@ -793,6 +793,10 @@ struct Darcy {
     CSR IA2, IAB2, C2;  // C2: final edge system (F)
     std::vector<double> rhsC1_el, rhsC2;
     std::vector<double> sol;  // [q | p | lambda]
+    // unsteady models (darcy_flow_mh.cc:1601-1827): 1 MH, 2 LMH
+    int unsteady = 0;
+    std::vector<double> stor, steady_diag, new_diag, steady_rhs, prev_sol;
+    double dt = 0;
     int iters = 0;
     double resid = 0, bnorm1 = 0;
     double t_assembly = 0, t_schur = 0, t_solve = 0, t_resolve = 0;
@@ -1149,6 +1153,22 @@ void *orc_darcy_create(void *mesh, const double *K, const double *cs, const doub
 }
 void orc_darcy_free(void *h) { delete (Darcy *)h; }
 
+// The assembled system solved through both Schur stages (darcy_flow_mh.cc:397-418).
+static int solve_assembled(Darcy &D, int method, double rtol, double atol, int max_it);
+
+// Diagonal of the assembled MH matrix set (insert) or added (MatDiagonalSet).
+static void diag_apply(Darcy &D, const std::vector<double> &d, bool add) {
+    for (int i = 0; i < D.N; i++) {
+        bool found = false;
+        for (int k = D.M.ptr[i]; k < D.M.ptr[i + 1]; k++)
+            if (D.M.col[k] == i) {
+                D.M.val[k] = add ? D.M.val[k] + d[i] : d[i];
+                found = true;
+            }
+        if (!found && d[i] != 0.0) { g_err = "missing diagonal entry"; }
+    }
+}
+
 // Full steady solve (update_solution, darcy_flow_mh.cc:375-433, n_schurs=2).
 // method 0: Jacobi-CG on the edge system; method 1: dense LU of the edge system
 // (brute force, small meshes); method 2: dense LU of the FULL MH system.
@@ -1168,6 +1188,11 @@ int orc_darcy_solve(void *h, int method, double rtol, double atol, int max_it) {
         D.sol = x;
         return 0;
     }
+    return solve_assembled(D, method, rtol, atol, max_it);
+}
+
+static int solve_assembled(Darcy &D, int method, double rtol, double atol, int max_it) {
+    double t1 = now_s();
     make_schurs(D);
     double t2 = now_s();
     D.t_schur = t2 - t1;
@@ -1194,6 +1219,75 @@ int orc_darcy_solve(void *h, int method, double rtol, double atol, int max_it) {
     std::copy(x2.begin(), x2.end(), D.sol.begin() + S);
     D.t_resolve = now_s() - t3;
     return D.iters;
+}
+
+// DarcyFlowMH_Unsteady::setup_time_term (darcy_flow_mh.cc:1614-1658), lumped = 0, or
+// DarcyFlowLMH_Unsteady::setup_time_term (:1695-1749), lumped = 1: the steady
+// system is assembled, its diagonal and rhs saved, the time term of step dt added
+// to the diagonal and the initial condition put into the solution.
+void orc_darcy_unsteady_setup(void *h, int lumped, const double *storativity, const double *init_pressure,
+                              double dt) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    assemble(D);
+    D.unsteady = lumped ? 2 : 1;
+    D.stor.assign(storativity, storativity + D.E);
+    D.dt = dt;
+    D.steady_diag.assign(D.N, 0.0);
+    for (int i = 0; i < D.N; i++) D.steady_diag[i] = D.M.get(i, i);
+    D.sol.assign(D.N, 0.0);
+    D.new_diag.assign(D.N, 0.0);
+    for (int e = 0; e < D.E; e++) {
+        const Element &el = m.element[e];
+        double meas = elem_measure(m, el);
+        if (!lumped) {
+            D.sol[D.S + e] = init_pressure[e];
+            D.new_diag[D.S + e] = -D.stor[e] * meas / dt;
+        } else {
+            int ns = el.n_sides();
+            for (int i = 0; i < ns; i++) {
+                int edge_row = D.S + D.E + el.edge_idx[i];
+                D.new_diag[edge_row] += -meas * D.stor[e] * D.cs[e] / dt / ns;
+                D.sol[edge_row] += init_pressure[e] / ns;
+            }
+        }
+    }
+    diag_apply(D, D.new_diag, true);
+    D.prev_sol = D.sol;
+    D.steady_rhs = D.rhs;
+}
+
+// One step: update_solution with modify_system (darcy_flow_mh.cc:1660-1677 MH,
+// :1751-1770 LMH) and, for LMH, postprocess (:1772-1827).  Returns CG iterations.
+int orc_darcy_unsteady_step(void *h, double dt, int method, double rtol, double atol, int max_it) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    if (dt != D.dt) {
+        diag_apply(D, D.steady_diag, false);
+        for (double &v : D.new_diag) v *= D.dt / dt;
+        diag_apply(D, D.new_diag, true);
+        D.dt = dt;
+    }
+    for (int i = 0; i < D.N; i++) D.rhs[i] = D.new_diag[i] * D.sol[i] + D.steady_rhs[i];
+    D.prev_sol = D.sol;
+    int it = solve_assembled(D, method, rtol, atol, max_it);
+    if (D.unsteady == 2) {
+        for (size_t f = 0; f < m.edges.size(); f++) {
+            int row = D.S + D.E + (int)f;
+            double new_pressure = D.sol[row], old_pressure = D.prev_sol[row];
+            for (auto &sd : m.edges[f].sides) {
+                const Element &el = m.element[sd.first];
+                double time_coef = -elem_measure(m, el) * D.cs[sd.first] * D.stor[sd.first] / dt / el.n_sides();
+                D.sol[m.side_offset[sd.first] + sd.second] += time_coef * (new_pressure - old_pressure);
+            }
+        }
+        for (int e = 0; e < D.E; e++) {
+            const Element &el = m.element[e];
+            for (int i = 0; i < el.n_sides(); i++)
+                D.sol[m.side_offset[e] + i] += -1.0 * elem_measure(m, el) * D.cs[e] * D.f[e] / el.n_sides();
+        }
+    }
+    return it;
 }
 
 int orc_darcy_size(void *h, int *out) {

@@ -63,6 +63,9 @@ def lib():
         L.orc_transport_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
+        L.orc_darcy_unsteady_step.restype = C.c_int
+        L.orc_darcy_unsteady_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int]
         L.orc_transport_set_reactions.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]
         _LIB = L
     return _LIB
@@ -165,6 +168,21 @@ class Darcy:
         lib().orc_darcy_stats(self.h, st)
         self.stats = dict(iters=int(st[0]), resid=st[1], bnorm=st[2], t_assembly=st[3], t_schur=st[4],
                           t_solve=st[5], t_resolve=st[6])
+        return it
+
+    def set_unsteady(self, lumped, storativity, init_pressure, dt):
+        self._unst = [np.ascontiguousarray(storativity, np.float64), np.ascontiguousarray(init_pressure, np.float64)]
+        lib().orc_darcy_unsteady_setup(self.h, int(lumped), self._unst[0], self._unst[1], float(dt))
+        sz = np.zeros(5, np.int32)
+        lib().orc_darcy_size(self.h, sz)
+        self.S, self.E, self.F, self.N, self.nnz = [int(v) for v in sz]
+        self.solution = np.zeros(self.N)
+        lib().orc_darcy_solution(self.h, self.solution)
+
+    def unsteady_step(self, dt, method=0, rtol=1e-14, atol=0.0, max_it=100000):
+        it = lib().orc_darcy_unsteady_step(self.h, float(dt), method, rtol, atol, max_it)
+        self.solution = np.zeros(self.N)
+        lib().orc_darcy_solution(self.h, self.solution)
         return it
 
     def edge_system(self):

@@ -320,6 +320,8 @@ int fl_darcy_set_conductivity(fl_darcy *d, const double *K) {
     });
 }
 
+static void fill_info(const fl::SolveStats &st, fl_solve_info *info);
+
 int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info *info) {
     return guard([&] {
         double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
@@ -327,6 +329,32 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
         int prec = o ? o->precision : 0;
         if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
         fl::SolveStats st = d->g->solve(rtol, atol, max_it, prec);
+        fill_info(st, info);
+        return 0;
+    });
+}
+
+int fl_darcy_set_unsteady(fl_darcy *d, int lumped, const double *storativity, const double *init_pressure) {
+    return guard([&] {
+        d->g->set_unsteady(lumped, storativity, init_pressure);
+        return 0;
+    });
+}
+
+int fl_darcy_unsteady_step(fl_darcy *d, double dt, const fl_solver_opts *o, fl_solve_info *info) {
+    return guard([&] {
+        double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
+        int max_it = o ? o->max_it : 10000;
+        int prec = o ? o->precision : 0;
+        if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
+        fl::SolveStats st = d->g->unsteady_step(dt, rtol, atol, max_it, prec);
+        fill_info(st, info);
+        return 0;
+    });
+}
+
+static void fill_info(const fl::SolveStats &st, fl_solve_info *info) {
+    {
         if (info) {
             info->iterations = st.iterations;
             info->converged = st.converged;
@@ -341,8 +369,7 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
             info->kernel_samples = st.kernel_samples;
             info->reliable_updates = st.reliable_updates;
         }
-        return 0;
-    });
+    }
 }
 
 int fl_darcy_get_solution(fl_darcy *d, double *out) {

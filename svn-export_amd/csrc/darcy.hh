@@ -59,6 +59,10 @@ struct DarcyDev {
     const double *bc_p, *bc_flux, *bc_sigma;
     // work
     double *blk, *rhsc, *val, *rhs, *dinv, *sol;
+    // unsteady models, null when steady: MH time coefficient S|e|/dt and previous
+    // pressure per element; LMH time coefficient sum |e| S cs / dt / n_sides and
+    // previous trace per edge (edge order)
+    const double *tc, *pprev, *te, *lprev;
 };
 
 class DarcyGPU {
@@ -79,6 +83,12 @@ class DarcyGPU {
     void get_solution(double *out) const;                   // [S+E+F]
     void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
     void element_balance(double *out) const;                // [E]
+    // Unsteady models (darcy_flow_mh.cc:1601-1827): lumped 0 = MH (time term on the
+    // element pressures), 1 = LMH (on the edge traces); the solution becomes the
+    // initial state.  unsteady_step advances it by one implicit Euler step of dt.
+    void set_unsteady(int lumped, const double *storativity, const double *init_pressure);
+    SolveStats unsteady_step(double dt, double rtol, double atol, int max_it, int precision = 0);
+    bool unsteady() const { return unsteady_ != 0; }
     void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region
     void source_balance(double *src) const;                                // per bulk region
     double solution_precision(double rtol, double atol) const;
@@ -121,6 +131,13 @@ class DarcyGPU {
     std::vector<hipEvent_t> ev_pool_;
     std::vector<double> h_bc_measure_;
     std::vector<double> h_cs_, h_f_, h_sigma_;
+    // unsteady state
+    void set_time_coefficients(double dt);
+    int unsteady_ = 0;  // 1 MH, 2 LMH
+    double dt_ = 0;
+    std::vector<double> h_stor_, h_meas_;
+    double *d_tc_ = nullptr, *d_pprev_ = nullptr, *d_te_ = nullptr, *d_lprev_ = nullptr, *d_stor_ = nullptr,
+           *d_meas_ = nullptr;
 };
 
 }  // namespace fl

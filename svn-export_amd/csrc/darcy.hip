@@ -25,13 +25,17 @@ __global__ void __launch_bounds__(256) k_assemble(const DarcyDev dv) {
     int vb0 = dv.el_vb_ptr[e], nvb = dv.el_vb_ptr[e + 1] - vb0;
     double V = 0.0;
     for (int k = 0; k < nvb; k++) V += vb_coef(dv, e, dv.el_vb[vb0 + k]);
-    double Dd = L.s + V;
+    double Dd = L.s + V, h = L.h;
+    if (dv.tc) {  // MH time term on the element pressure row (darcy_flow_mh.cc:1641-1652, 1671-1673)
+        Dd = Dd + dv.tc[e];
+        h = h + dv.tc[e] * dv.pprev[e];
+    }
     double w[n0];
     for (int i = 0; i < n0; i++) w[i] = L.c[i] * L.a[i];
     const int m = n0 + nvb;
     double *B = dv.blk + dv.blk_off[e];
     double *r = dv.rhsc + dv.conn_off[e];
-    double hD = L.h / Dd;
+    double hD = h / Dd;
     for (int i = 0; i < n0; i++) {
         for (int j = 0; j < n0; j++) {
             double v = L.c[i] * L.c[j] * L.inv[i * n0 + j] - w[i] * w[j] / Dd;
@@ -72,6 +76,11 @@ __global__ void k_gather_rhs(const DarcyDev dv) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int j = dv.rptr[f]; j < dv.rptr[f + 1]; j++) s += dv.rhsc[dv.ridx[j]];
+        if (dv.te) {  // LMH time term on the edge trace row (darcy_flow_mh.cc:1718-1727, 1762-1764)
+            const int fe = dv.rperm[f];
+            s += dv.te[fe] * dv.lprev[fe];
+            dv.val[dv.diag_slot[f]] += dv.te[fe];
+        }
         dv.rhs[f] = s;
         double d = dv.val[dv.diag_slot[f]];
         dv.dinv[f] = d != 0.0 ? 1.0 / d : 1.0;
@@ -98,20 +107,40 @@ __global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double
         V += vk;
         wl += vk * lam[dv.vb_edge[n]];
     }
-    double Dd = L.s + V;
+    double Dd = L.s + V, h = L.h;
+    if (dv.tc) {
+        Dd = Dd + dv.tc[e];
+        h = h + dv.tc[e] * dv.pprev[e];
+    }
     double cl[n0];
     double own = 0.0;
     for (int i = 0; i < n0; i++) {
         cl[i] = L.c[i] * lam[dv.side_edge[so + i]];
         own += L.c[i] * L.a[i] * lam[dv.side_edge[so + i]];
     }
-    double pe = (L.h + (own + wl)) / Dd;
+    double pe = (h + (own + wl)) / Dd;
     for (int i = 0; i < n0; i++) {
         double t = 0.0;
         for (int j = 0; j < n0; j++) t += L.inv[i * n0 + j] * cl[j];
         dv.sol[so + i] = L.Ag[i] + pe * L.a[i] - t;
     }
     dv.sol[dv.S + e] = pe;
+}
+
+// DarcyFlowLMH_Unsteady::postprocess (darcy_flow_mh.cc:1772-1827): every side flux
+// gets the lumped storage term of its edge, then the element's source share
+__global__ void k_lmh_post(const DarcyDev dv, const double *__restrict__ meas, const double *__restrict__ stor,
+                           double dt) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < dv.E; e += gridDim.x * blockDim.x) {
+        const int so = dv.side_off[e], ns = dv.side_off[e + 1] - so;
+        const double *lam = dv.sol + dv.S + dv.E;
+        for (int i = 0; i < ns; i++) {
+            const int f = dv.side_edge[so + i];
+            const double time_coef = -meas[e] * dv.cs[e] * stor[e] / dt / ns;
+            dv.sol[so + i] += time_coef * (lam[f] - dv.lprev[f]);
+        }
+        for (int i = 0; i < ns; i++) dv.sol[so + i] += -1.0 * meas[e] * dv.cs[e] * dv.f[e] / ns;
+    }
 }
 
 // initial guess in edge order into the permuted CG vector
@@ -698,6 +727,96 @@ SolveStats DarcyGPU::solve(double rtol, double atol, int max_it, int precision) 
     FL_HIP(hipEventElapsedTime(&t23, e2, e3));
     st.ms_assembly = t01; st.ms_cg = t12; st.ms_resolve = t23; st.ms_total = t01 + t12 + t23;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+    return st;
+}
+
+void DarcyGPU::set_unsteady(int lumped, const double *storativity, const double *init_pressure) {
+    if (!have_data_) throw Error(-4, "Darcy data not set");
+    const HostMesh &m = m_;
+    const int E = m.n_elements(), S = m.n_sides(), F = m.n_edges();
+    unsteady_ = lumped ? 2 : 1;
+    h_stor_.assign(storativity, storativity + E);
+    h_meas_.resize(E);
+    for (int e = 0; e < E; e++) h_meas_[e] = element_measure(m, m.el_dim[e], &m.el_nodes[4 * e]);
+    if (!d_tc_) {
+        d_tc_ = dev_alloc<double>(E); d_pprev_ = dev_alloc<double>(E); d_stor_ = dev_alloc<double>(E);
+        d_meas_ = dev_alloc<double>(E); d_te_ = dev_alloc<double>(F); d_lprev_ = dev_alloc<double>(F);
+        for (void *p : {(void *)d_tc_, (void *)d_pprev_, (void *)d_stor_, (void *)d_meas_, (void *)d_te_,
+                        (void *)d_lprev_})
+            owned_.push_back(p);
+    }
+    FL_HIP(hipMemcpyAsync(d_stor_, h_stor_.data(), E * sizeof(double), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipMemcpyAsync(d_meas_, h_meas_.data(), E * sizeof(double), hipMemcpyHostToDevice, stream_));
+    // initial condition (setup_time_term): element pressures (MH) or edge traces
+    // sum of init / n_sides over the edge's sides (LMH); the rest zero
+    std::vector<double> sol((size_t)S + E + F, 0.0);
+    if (unsteady_ == 1) {
+        for (int e = 0; e < E; e++) sol[S + e] = init_pressure[e];
+    } else {
+        for (int e = 0; e < E; e++) {
+            const int ns = m.el_dim[e] + 1;
+            for (int i = 0; i < ns; i++) sol[(size_t)S + E + m.side_edge[m.side_off[e] + i]] += init_pressure[e] / ns;
+        }
+    }
+    FL_HIP(hipMemcpyAsync(d_sol_, sol.data(), sol.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    dt_ = 0;
+    dv_.tc = dv_.pprev = dv_.te = dv_.lprev = nullptr;
+}
+
+// time coefficients for step dt, positive as the condensed system carries them:
+// MH S|e|/dt per element (darcy_flow_mh.cc:1643-1644), LMH sum over an edge's
+// sides of |e| S cs / dt / n_sides (:1722-1724)
+void DarcyGPU::set_time_coefficients(double dt) {
+    const HostMesh &m = m_;
+    const int E = m.n_elements(), F = m.n_edges();
+    if (unsteady_ == 1) {
+        std::vector<double> tc(E);
+        for (int e = 0; e < E; e++) tc[e] = h_stor_[e] * h_meas_[e] / dt;
+        FL_HIP(hipMemcpyAsync(d_tc_, tc.data(), E * sizeof(double), hipMemcpyHostToDevice, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<double> te(F, 0.0);
+        for (int e = 0; e < E; e++) {
+            const int ns = m.el_dim[e] + 1;
+            for (int i = 0; i < ns; i++)
+                te[m.side_edge[m.side_off[e] + i]] += h_meas_[e] * h_stor_[e] * h_cs_[e] / dt / ns;
+        }
+        FL_HIP(hipMemcpyAsync(d_te_, te.data(), F * sizeof(double), hipMemcpyHostToDevice, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+    }
+    dt_ = dt;
+}
+
+// DarcyFlowMH::update_solution for the unsteady models: modify_system
+// (darcy_flow_mh.cc:1660-1677 / 1751-1770), both Schur stages, the solve and,
+// for LMH, postprocess (:1772-1827)
+SolveStats DarcyGPU::unsteady_step(double dt, double rtol, double atol, int max_it, int precision) {
+    if (!unsteady_) throw Error(-4, "unsteady model not set");
+    if (!(dt > 0.0)) throw Error(-3, "time step must be positive");
+    if (dt != dt_) set_time_coefficients(dt);
+    const int S = m_.n_sides(), E = m_.n_elements(), F = m_.n_edges();
+    if (unsteady_ == 1) {
+        FL_HIP(hipMemcpyAsync(d_pprev_, d_sol_ + S, E * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        dv_.tc = d_tc_; dv_.pprev = d_pprev_;
+    } else {
+        FL_HIP(hipMemcpyAsync(d_lprev_, d_sol_ + S + E, F * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        dv_.te = d_te_; dv_.lprev = d_lprev_;
+    }
+    SolveStats st;
+    try {
+        st = solve(rtol, atol, max_it, precision);
+        if (unsteady_ == 2) {
+            hipLaunchKernelGGL(k_lmh_post, dim3(std::min(grid_for(E), 4096)), dim3(kBlock), 0, stream_, dv_,
+                               (const double *)d_meas_, (const double *)d_stor_, dt);
+            FL_HIP(hipGetLastError());
+            FL_HIP(hipStreamSynchronize(stream_));
+        }
+    } catch (...) {
+        dv_.tc = dv_.pprev = dv_.te = dv_.lprev = nullptr;
+        throw;
+    }
+    dv_.tc = dv_.pprev = dv_.te = dv_.lprev = nullptr;  // a later steady solve sees the steady system
     return st;
 }
 

@@ -86,6 +86,8 @@ SIGNATURES = {
                                     _f64p]),
     "fl_darcy_set_conductivity": (C.c_int, [C.c_void_p, _f64p]),
     "fl_darcy_update_solution": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
+    "fl_darcy_set_unsteady": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p]),
+    "fl_darcy_unsteady_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
     "fl_darcy_get_solution": (C.c_int, [C.c_void_p, _f64p]),
     "fl_darcy_device_solution": (C.c_void_p, [C.c_void_p]),
     "fl_darcy_solution_precision": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
@@ -408,6 +410,21 @@ class DarcyFlowMH_Steady:
         o = SolverOpts(r_tol, a_tol, max_it, precision)
         info = SolveInfo()
         _check(lib().fl_darcy_update_solution(self.h, C.byref(o), C.byref(info)))
+        self.info = info.as_dict()
+        return self.info
+
+    def set_unsteady(self, lumped, storativity, init_pressure):
+        """Unsteady MH (lumped False) or LMH (lumped True) from the data already set;
+        the solution becomes the initial state (darcy_flow_mh.cc:1601-1749)."""
+        E = self.mesh.n_elements
+        self._unst = [_f64(storativity, E), _f64(init_pressure, E)]
+        _check(lib().fl_darcy_set_unsteady(self.h, 1 if lumped else 0, *self._unst))
+
+    def unsteady_step(self, dt, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
+        """One implicit Euler step of length dt (darcy_flow_mh.cc:375-433, 1660-1827)."""
+        o = SolverOpts(r_tol, a_tol, max_it, precision)
+        info = SolveInfo()
+        _check(lib().fl_darcy_unsteady_step(self.h, float(dt), C.byref(o), C.byref(info)))
         self.info = info.as_dict()
         return self.info
 
