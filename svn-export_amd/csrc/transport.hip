@@ -154,9 +154,14 @@ __global__ void __launch_bounds__(256) k_tm_build(const TmDev t) {
     if (grid_reduce<1, MaxOp>(v, t.partial, t.count) && threadIdx.x == 0) t.mb[0] = v[0];
 }
 
-// set_boundary_conditions (transport.cc:455-495) followed by VecScale(est)
+// set_boundary_conditions (transport.cc:455-495) followed by VecScale(est).  Only
+// rows with boundary sides can have a non-zero value; they keep it compactly at
+// slot bl_ptr[i] (the row's first boundary-side entry), every other row's value
+// is zero and is not stored.
 __global__ void k_set_bc(const TmDev t, double est) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        const int b0 = t.bl_ptr[i];
+        if (t.bl_ptr[i + 1] == b0) continue;
         for (int s = 0; s < t.nsub; s++) {
             double acc = 0.0;
             for (int sd = t.side_off[i]; sd < t.side_off[i + 1]; sd++) {
@@ -168,7 +173,7 @@ __global__ void k_set_bc(const TmDev t, double est) {
                     acc += t.bc_conc[(size_t)s * t.NBC + t.bnd_bcele[b]] * aij;
                 }
             }
-            t.bcv[(size_t)i * t.nsub + s] = acc * est;
+            t.bcv[(size_t)b0 * t.nsub + s] = acc * est;
         }
     }
 }
@@ -184,8 +189,8 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
             if (k == dk) v = v + 1.0;
             t.val[k] = v;
         }
-        if (!first)
-            for (int s = 0; s < t.nsub; s++) t.bcv[(size_t)i * t.nsub + s] *= fac;
+        if (!first && t.bl_ptr[i + 1] > t.bl_ptr[i])
+            for (int s = 0; s < t.nsub; s++) t.bcv[(size_t)t.bl_ptr[i] * t.nsub + s] *= fac;
     }
 }
 
@@ -246,7 +251,8 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
     const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
     for (int i = r_lo + lb * blockDim.x + threadIdx.x; i < r_hi; i += nlb * blockDim.x) {
         double acc[M];
-        for (int s = 0; s < M; s++) acc[s] = t.bcv[(size_t)i * M + s];
+        const int bb0 = t.bl_ptr[i], bb1 = t.bl_ptr[i + 1];
+        for (int s = 0; s < M; s++) acc[s] = bb1 > bb0 ? t.bcv[(size_t)bb0 * M + s] : 0.0;
         if (t.has_src) {
             // compute_concentration_sources (transport.cc:501-544), dt of this step
             for (int s = 0; s < M; s++) {
@@ -275,7 +281,7 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
             for (int q = 0; q < M; q++) acc[q] = r[q];
         }
         for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
-        for (int k = t.bl_ptr[i]; k < t.bl_ptr[i + 1]; k++) {
+        for (int k = bb0; k < bb1; k++) {
             double wf = t.flux[t.bl_side[k]];
             int bce = t.bl_bcele[k];
             for (int s = 0; s < M; s++)
@@ -318,15 +324,18 @@ __global__ void __launch_bounds__(256) k_step_q(const TmDev t, const double *__r
     if (s < M) {
         for (int i0 = r_lo + lb * 64 + (threadIdx.x >> 2); i0 < r_hi; i0 += RW * stride) {
             double acc[RW];
-            int rows[RW];
+            int rows[RW], b0[RW], b1[RW];
 #pragma unroll
             for (int u = 0; u < RW; u++) {
                 const int i = i0 + u * stride;
                 rows[u] = i < r_hi ? i : -1;
                 acc[u] = 0.0;
+                b0[u] = b1[u] = 0;
                 if (i >= r_hi) continue;
                 const size_t o = (size_t)i * M + s;
-                acc[u] = t.bcv[o];
+                b0[u] = t.bl_ptr[i];
+                b1[u] = t.bl_ptr[i + 1];
+                acc[u] = b1[u] > b0[u] ? t.bcv[(size_t)b0[u] * M + s] : 0.0;
                 if (t.has_src) {
                     const double dens = t.src_density[o], sig = t.src_sigma[o];
                     const double diff = t.src_conc[o] - c_old[o];
@@ -341,13 +350,11 @@ __global__ void __launch_bounds__(256) k_step_q(const TmDev t, const double *__r
             // adding 0 * c leaves every finite sum unchanged, so the fixed
             // five-entry loop issues all loads of the RW rows together
             double av[RW][kEll];
-            int ac[RW][kEll], ln[RW], b0[RW], b1[RW];
+            int ac[RW][kEll], ln[RW];
 #pragma unroll
             for (int u = 0; u < RW; u++) {
                 const int i = rows[u] < 0 ? r_lo : rows[u];
                 ln[u] = t.ell_len[i];
-                b0[u] = t.bl_ptr[i];
-                b1[u] = t.bl_ptr[i + 1];
 #pragma unroll
                 for (int j = 0; j < kEll; j++) {
                     av[u][j] = t.ell_val[(size_t)j * t.E + i];
@@ -525,7 +532,7 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
         t.ell_col = p;
         t.ell_len = p + (size_t)kEll * E;
     }
-    t.bcv = alloc_d((size_t)E * nsub);
+    t.bcv = alloc_d((size_t)bl_side.size() * nsub);  // compact: one slot group per row with boundary sides
     d_c0_ = alloc_d((size_t)E * nsub);
     d_c1_ = alloc_d((size_t)E * nsub);
     t.partial = alloc_d(2 * kPartStride + 4096);
