@@ -400,6 +400,15 @@ int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double
  * the start time does. */
 int fl_transport_write_mass_balance(fl_transport *t, const char *path, const char *const *subst_names);
 
+/* Time-dependent boundary concentrations (a bc_conc field given as a list of
+ * data times, e.g. old_boundary_file entries with `time`): bc_conc[n_subst][NBC]
+ * takes effect at the start of the next step, as data_.set_time does at the start
+ * of ConvectionTransport::compute_one_step (transport.cc:598-627: the boundary
+ * vector is rebuilt and scaled by the step estimate).  Call it after the
+ * fl_transport_update_solution that reached the data time; balances read at that
+ * output time still use the previous values, as the reference's do. */
+int fl_transport_set_bc_conc(fl_transport *t, const double *bc_conc);
+
 /* Linear_reaction (src/reaction/linear_reaction.cc:144-300) as
  * TransportOperatorSplitting drives it (transport_operator_splitting.cc:244-287):
  * n_decays first-order decays, decay i with parent substance parent[i],

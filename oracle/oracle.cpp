@@ -1441,6 +1441,8 @@ struct Transport {
     std::vector<double> flux;  // side flux [S]
     std::vector<double> cs, por;
     std::vector<double> bc_conc;    // [nsub][NBC]
+    std::vector<double> bc_pending; // new boundary data, read by the next step
+    bool bc_changed = false;
     std::vector<double> conc, pconc;  // [nsub][E]
     std::vector<double> bcvcorr;    // [nsub][E]
     std::vector<double> src_density, src_sigma, src_conc;  // [nsub][E], empty = no sources
@@ -1604,6 +1606,18 @@ static void mb_calculate(Transport &T, double time) {
 
 // compute_one_step: transport.cc:593-715
 static void one_step(Transport &T) {
+    // data_.set_time at the step's start: boundary data given since the last step apply now
+    bool bc_changed = false;
+    if (T.bc_changed) {
+        T.bc_conc = T.bc_pending;
+        T.bc_changed = false;
+        bc_changed = true;
+    }
+    if (T.matrix_built && bc_changed) {  // transport.cc:620-626
+        set_bc(T);
+        double est = T.tg.estimate_dt();
+        for (double &v : T.bcvcorr) v *= est;
+    }
     if (!T.matrix_built) {
         build_tm(T);
         T.tg.set_upper_constraint(T.cfl_max_step);
@@ -1742,6 +1756,14 @@ void orc_transport_set_reactions(void *h, int n, const int *parent, const int *p
     T.r_prod.assign(prod, prod + pptr[n]);
     T.r_hl.assign(half_life, half_life + n);
     T.r_branch.assign(branch, branch + pptr[n]);
+}
+
+// Boundary concentrations bc_conc[nsub][NBC] from the next step on (a time-dependent
+// bc_conc field whose new data time has been reached).
+void orc_transport_set_bc_conc(void *h, const double *bc_conc) {
+    Transport &T = *(Transport *)h;
+    T.bc_pending.assign(bc_conc, bc_conc + T.bc_conc.size());
+    T.bc_changed = true;
 }
 
 void orc_transport_set_sources(void *h, const double *density, const double *sigma, const double *conc) {

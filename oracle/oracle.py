@@ -63,6 +63,7 @@ def lib():
         L.orc_transport_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
         L.orc_darcy_unsteady_step.restype = C.c_int
         L.orc_darcy_unsteady_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int]
@@ -265,6 +266,10 @@ class Transport:
     def set_sources(self, density, sigma, conc):
         self._src = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (density, sigma, conc)]
         lib().orc_transport_set_sources(self.h, *self._src)
+
+    def set_bc_conc(self, bc_conc):
+        self._bcc = np.ascontiguousarray(bc_conc, np.float64).reshape(-1)
+        lib().orc_transport_set_bc_conc(self.h, self._bcc)
 
     def set_reactions(self, parent, products, half_life, branch):
         """parent[n]; products: list of lists of substance indices; branch: same shape."""

@@ -75,6 +75,9 @@ class TransportGPU {
     ~TransportGPU();
     // sources_density / sources_sigma / sources_conc, each [nsub][E] (transport.cc:501-544)
     void set_sources(const double *density, const double *sigma, const double *conc);
+    // new boundary concentrations [nsub][NBC], read by the next step (a time-dependent
+    // bc_conc field whose next data time has been reached)
+    void set_bc_conc(const double *bc_conc);
     // Linear_reaction (linear_reaction.cc:144-220): decays i with parent[i],
     // products[pptr[i]..pptr[i+1]), half_life[i] and branch ratios; the matrix is
     // rebuilt from the step estimate at every update_solution and applied after
@@ -108,6 +111,9 @@ class TransportGPU {
     static constexpr int kMaxStepGrid = 16384;
     static constexpr size_t kPartStride = (size_t)kMaxStepGrid * 2 * kMaxSubst;  // per-step partials
     bool quad_ = true;  // step kernel with one lane per (row, substance) for 2..4 substances
+    double *d_bc_conc_ = nullptr;
+    std::vector<double> h_bc_next_;
+    bool bc_pending_ = false;
     bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_; }
     void build_reaction_matrix(double dt);
     bool react_on_ = false;

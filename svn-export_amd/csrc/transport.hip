@@ -518,7 +518,8 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     t.meas = upv(owned_, h_meas_, s); t.cs = upv(owned_, h_cs_, s); t.por = upv(owned_, h_por_, s);
     std::vector<double> bcc = h_bc_conc_;
     if (bcc.empty()) bcc.push_back(0);
-    t.bc_conc = upv(owned_, bcc, s);
+    d_bc_conc_ = upv(owned_, bcc, s);
+    t.bc_conc = d_bc_conc_;
     t.src_density = t.src_sigma = t.src_conc = nullptr;
     auto alloc_d = [&](size_t n) { double *p = dev_alloc<double>(n); owned_.push_back(p); return p; };
     d_flux_ = alloc_d(S);
@@ -629,6 +630,11 @@ void TransportGPU::build_reaction_matrix(double dt) {
     FL_HIP(hipStreamSynchronize(stream_));  // h_react_ is rebuilt at the next output interval
 }
 
+void TransportGPU::set_bc_conc(const double *bc_conc) {
+    h_bc_next_.assign(bc_conc, bc_conc + (size_t)nsub_ * NBC_);
+    bc_pending_ = true;
+}
+
 void TransportGPU::set_velocity_host(const double *side_flux) {
     FL_HIP(hipMemcpyAsync(d_flux_, side_flux, m_.n_sides() * sizeof(double), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
@@ -722,6 +728,18 @@ void TransportGPU::flush_flux() {
 // ConvectionTransport::compute_one_step (transport.cc:593-715)
 void TransportGPU::one_step() {
     const int gridS = std::min(grid_for(E_), 4096);
+    // data_.set_time at the step's start (transport.cc:600): boundary data given since
+    // the last step apply now; with the matrix in place only the boundary vector is
+    // rebuilt (transport.cc:620-626)
+    bool bc_changed = false;
+    if (bc_pending_) {
+        h_bc_conc_ = h_bc_next_;
+        FL_HIP(hipMemcpyAsync(d_bc_conc_, h_bc_conc_.data(), h_bc_conc_.size() * sizeof(double),
+                              hipMemcpyHostToDevice, stream_));
+        bc_pending_ = false;
+        bc_changed = true;
+    }
+    if (setup_done_ && bc_changed) set_bc();
     if (!matrix_built_) build_matrix();
     if (!setup_done_) {
         setup_done_ = true;

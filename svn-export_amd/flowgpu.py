@@ -138,6 +138,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
     "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 6),
     "fl_transport_set_sources": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
+    "fl_transport_set_bc_conc": (C.c_int, [C.c_void_p, _f64p]),
     "fl_transport_set_linear_reactions": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]),
     "fl_transport_nnz": (C.c_int, [C.c_void_p]),
     "fl_transport_matrix": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p]),
@@ -667,6 +668,11 @@ class ConvectionTransport:
         return dict(bal=bal[:ns * nbr].reshape(ns, nbr), plus=plus[:ns * nbr].reshape(ns, nbr),
                     minus=minus[:ns * nbr].reshape(ns, nbr), mass=mass[:ns * nblk].reshape(ns, nblk),
                     src=src[:ns * nblk].reshape(ns, nblk), totals=tot.reshape(ns, 4))
+
+    def set_bc_conc(self, bc_conc):
+        """New boundary concentrations [n_subst][NBC] from the next step on (transport.cc:598-627)."""
+        self._bcc_next = _f64(bc_conc, self.nsub * self.mesh.n_bc_elements)
+        _check(lib().fl_transport_set_bc_conc(self.h, self._bcc_next))
 
     def set_linear_reactions(self, decays, names=None):
         """Linear_reaction decays (linear_reaction.cc:200-258): a list of dicts with
