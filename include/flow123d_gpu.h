@@ -422,6 +422,16 @@ int fl_transport_set_bc_conc(fl_transport *t, const double *bc_conc);
 int fl_transport_set_linear_reactions(fl_transport *t, int n_decays, const int *parent, const int *product_ptr,
                                       const int *products, const double *half_life, const double *branch_ratio);
 
+/* Pade_approximant (src/reaction/pade_approximant.cc:97-290): the decays of
+ * fl_transport_set_linear_reactions with the step matrix the (nom_pol_deg,
+ * den_pol_deg) Pade approximant of exp(A dt), A[p][p] = -ln2 dt/half_life,
+ * A[p][product] = ln2 dt/half_life * ratio: N(A) and D(A) by Horner, P =
+ * (D^-1 N)^T, each element's concentrations become P c after every convection
+ * step.  Each polynomial is evaluated with its own degree. */
+int fl_transport_set_pade_reactions(fl_transport *t, int n_decays, const int *parent, const int *product_ptr,
+                                    const int *products, const double *half_life, const double *branch_ratio,
+                                    int nom_pol_deg, int den_pol_deg);
+
 /* Concentration sources (ConvectionTransport::compute_concentration_sources,
  * transport.cc:501-544): sources_density, sources_sigma, sources_conc, each
  * [n_subst][E]; the step adds (density + max(conc_src - c, 0) sigma) dt. */

@@ -1449,6 +1449,7 @@ struct Transport {
     // Linear_reaction: decays (parent, products, half-life, branch ratios), matrix [nsub][nsub]
     std::vector<int> r_parent, r_pptr, r_prod;
     std::vector<double> r_hl, r_branch, reaction_matrix;
+    int pade_nom = -1, pade_den = -1;  // Pade_approximant degrees, -1: Linear_reaction
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -1684,7 +1685,63 @@ static void set_reaction_time_step(Transport &T, double time_step) {
     }
 }
 
-// Linear_reaction::compute_one_step / compute_reaction (linear_reaction.cc:167-183, 287-300)
+static double fact(int n) { return n <= 1 ? 1.0 : n * fact(n - 1); }
+
+// Pade_approximant::modify_reaction_matrix (pade_approximant.cc:97-239) with
+// evaluate_matrix_polynomial (:241-263): reaction_matrix[rows][cols] = (D^-1 N)[cols][rows]
+static void set_pade_time_step(Transport &T, double time_step) {
+    const int n = T.nsub, p = T.pade_nom, q = T.pade_den;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (size_t d = 0; d < T.r_parent.size(); d++) {
+        int par = T.r_parent[d];
+        double rel_step = time_step / T.r_hl[d];
+        A[(size_t)par * n + par] = -std::log(2.0) * rel_step;
+        for (int k = T.r_pptr[d]; k < T.r_pptr[d + 1]; k++)
+            A[(size_t)par * n + T.r_prod[k]] = std::log(2.0) * rel_step * T.r_branch[k];
+    }
+    auto evaluate = [&](int deg, const std::vector<double> &coef) {
+        std::vector<double> P((size_t)n * n, 0.0);
+        for (int i = deg; i >= 0; i--) {
+            std::vector<double> Q((size_t)n * n, 0.0);  // P * A
+            for (int r = 0; r < n; r++)
+                for (int k = 0; k < n; k++)
+                    for (int c = 0; c < n; c++) Q[(size_t)r * n + c] += P[(size_t)r * n + k] * A[(size_t)k * n + c];
+            for (int r = 0; r < n; r++) Q[(size_t)r * n + r] += coef[i];
+            P = Q;
+        }
+        return P;
+    };
+    std::vector<double> nc(p + 1), dc(q + 1);
+    for (int j = p; j >= 0; j--) nc[j] = fact(p + q - j) * fact(p) / (fact(p + q) * fact(j) * fact(p - j));
+    for (int i = q; i >= 0; i--) dc[i] = std::pow(-1.0, i) * fact(p + q - i) * fact(q) / (fact(p + q) * fact(i) * fact(q - i));
+    std::vector<double> Nm = evaluate(p, nc), Dm = evaluate(q, dc);
+    // Gauss-Jordan on [D | N]: afterwards the right block is D^-1 N
+    std::vector<double> W((size_t)n * 2 * n);
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) {
+            W[(size_t)r * 2 * n + c] = Dm[(size_t)r * n + c];
+            W[(size_t)r * 2 * n + n + c] = Nm[(size_t)r * n + c];
+        }
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(W[(size_t)r * 2 * n + c]) > std::fabs(W[(size_t)piv * 2 * n + c])) piv = r;
+        for (int k = 0; k < 2 * n; k++) std::swap(W[(size_t)c * 2 * n + k], W[(size_t)piv * 2 * n + k]);
+        double d = W[(size_t)c * 2 * n + c];
+        for (int k = 0; k < 2 * n; k++) W[(size_t)c * 2 * n + k] /= d;
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            double f = W[(size_t)r * 2 * n + c];
+            for (int k = 0; k < 2 * n; k++) W[(size_t)r * 2 * n + k] -= f * W[(size_t)c * 2 * n + k];
+        }
+    }
+    T.reaction_matrix.assign((size_t)n * n, 0.0);
+    for (int rows = 0; rows < n; rows++)
+        for (int cols = 0; cols < n; cols++) T.reaction_matrix[(size_t)rows * n + cols] = W[(size_t)cols * 2 * n + n + rows];
+}
+
+// Linear_reaction::compute_one_step / compute_reaction (linear_reaction.cc:167-183, 287-300);
+// Pade_approximant::compute_reaction (pade_approximant.cc:274-290) multiplies the other way round
 static void compute_reactions(Transport &T) {
     int n = T.nsub;
     std::vector<double> prev_conc(n);
@@ -1693,9 +1750,15 @@ static void compute_reactions(Transport &T) {
             prev_conc[cols] = T.conc[(size_t)cols * T.E + e];
             T.conc[(size_t)cols * T.E + e] = 0.0;
         }
-        for (int rows = 0; rows < n; rows++)
+        if (T.pade_nom >= 0) {
             for (int cols = 0; cols < n; cols++)
-                T.conc[(size_t)rows * T.E + e] += prev_conc[cols] * T.reaction_matrix[(size_t)cols * n + rows];
+                for (int rows = 0; rows < n; rows++)
+                    T.conc[(size_t)cols * T.E + e] += T.reaction_matrix[(size_t)cols * n + rows] * prev_conc[rows];
+        } else {
+            for (int rows = 0; rows < n; rows++)
+                for (int cols = 0; cols < n; cols++)
+                    T.conc[(size_t)rows * T.E + e] += prev_conc[cols] * T.reaction_matrix[(size_t)cols * n + rows];
+        }
     }
 }
 
@@ -1756,6 +1819,13 @@ void orc_transport_set_reactions(void *h, int n, const int *parent, const int *p
     T.r_prod.assign(prod, prod + pptr[n]);
     T.r_hl.assign(half_life, half_life + n);
     T.r_branch.assign(branch, branch + pptr[n]);
+    T.pade_nom = T.pade_den = -1;
+}
+// The same decays with the Pade approximant of the given degrees.
+void orc_transport_set_pade(void *h, int nom_deg, int den_deg) {
+    Transport &T = *(Transport *)h;
+    T.pade_nom = nom_deg;
+    T.pade_den = den_deg;
 }
 
 // Boundary concentrations bc_conc[nsub][NBC] from the next step on (a time-dependent
@@ -1780,7 +1850,10 @@ int orc_transport_update(void *h, double t) {
     Transport &T = *(Transport *)h;
     if (T.mb_initial) mb_calculate(T, T.tg.time);  // first_time_call output at t=0
     set_target(T, t);
-    if (!T.r_parent.empty()) set_reaction_time_step(T, T.tg.estimate_dt());
+    if (!T.r_parent.empty()) {
+        if (T.pade_nom >= 0) set_pade_time_step(T, T.tg.estimate_dt());
+        else set_reaction_time_step(T, T.tg.estimate_dt());
+    }
     int steps = 0;
     double t0 = now_s();
     while (T.tg.lt(t)) {

@@ -64,6 +64,7 @@ def lib():
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
+        L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
         L.orc_darcy_unsteady_step.restype = C.c_int
         L.orc_darcy_unsteady_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int]
@@ -279,6 +280,10 @@ class Transport:
                        np.ascontiguousarray(half_life, np.float64),
                        np.ascontiguousarray([b for p in branch for b in p], np.float64)]
         lib().orc_transport_set_reactions(self.h, len(parent), *self._react)
+
+    def set_pade(self, nom_deg, den_deg):
+        """Use the Pade approximant of the given degrees for the decays already set."""
+        lib().orc_transport_set_pade(self.h, int(nom_deg), int(den_deg))
 
     def step_seconds(self):
         return lib().orc_transport_step_seconds(self.h)

@@ -773,6 +773,16 @@ int fl_transport_write_mass_balance(fl_transport *t, const char *path, const cha
                                  bal.data(), plus.data(), minus.data(), mass.data(), src.data(), tot.data());
 }
 
+int fl_transport_set_pade_reactions(fl_transport *t, int n_decays, const int *parent, const int *product_ptr,
+                                    const int *products, const double *half_life, const double *branch_ratio,
+                                    int nom_pol_deg, int den_pol_deg) {
+    return guard([&] {
+        t->g->set_pade_reactions(n_decays, parent, product_ptr, products, half_life, branch_ratio, nom_pol_deg,
+                                 den_pol_deg);
+        return 0;
+    });
+}
+
 int fl_transport_set_bc_conc(fl_transport *t, const double *bc_conc) {
     return guard([&] {
         t->g->set_bc_conc(bc_conc);

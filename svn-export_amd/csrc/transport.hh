@@ -84,6 +84,10 @@ class TransportGPU {
     // each convection step.  n_decays = 0 switches reactions off.
     void set_linear_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
                               const double *half_life, const double *branch);
+    // Pade_approximant (pade_approximant.cc:97-290): the same decays, the step matrix the
+    // (nom_deg, den_deg) Pade approximant of exp(A dt)
+    void set_pade_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                            const double *half_life, const double *branch, int nom_deg, int den_deg);
     void set_velocity_host(const double *side_flux);      // [S]
     void set_velocity_device(const double *d_side_flux);  // device [S] (e.g. the Darcy solution)
     double prepare();                                      // build the matrix now; returns the CFL step
@@ -116,7 +120,10 @@ class TransportGPU {
     bool bc_pending_ = false;
     bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_; }
     void build_reaction_matrix(double dt);
-    bool react_on_ = false;
+    void define_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                          const double *half_life, const double *branch);
+    bool react_on_ = false, pade_ = false;
+    int pade_nom_ = 0, pade_den_ = 0;
     std::vector<int> r_parent_, r_pptr_, r_prod_;
     std::vector<double> r_hl_, r_branch_, h_react_;
     double *d_react_ = nullptr;

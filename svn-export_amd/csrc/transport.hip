@@ -585,6 +585,12 @@ void TransportGPU::set_sources(const double *density, const double *sigma, const
 
 void TransportGPU::set_linear_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
                                         const double *half_life, const double *branch) {
+    pade_ = false;
+    define_reactions(n_decays, parent, pptr, products, half_life, branch);
+}
+
+void TransportGPU::define_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                                    const double *half_life, const double *branch) {
     if (n_decays <= 0) {
         react_on_ = false;
         t_.has_react = 0;
@@ -613,10 +619,104 @@ void TransportGPU::set_linear_reactions(int n_decays, const int *parent, const i
     build_reaction_matrix(tg_.estimate_dt());
 }
 
+void TransportGPU::set_pade_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
+                                      const double *half_life, const double *branch, int nom_deg, int den_deg) {
+    if (n_decays > 0 && (nom_deg < 0 || den_deg < 0 || nom_deg + den_deg < 1 || nom_deg > 10 || den_deg > 10))
+        throw Error(-3, "Pade degrees must be in 0..10 with a positive sum");
+    pade_nom_ = nom_deg;
+    pade_den_ = den_deg;
+    pade_ = n_decays > 0;
+    try {
+        define_reactions(n_decays, parent, pptr, products, half_life, branch);
+    } catch (...) {
+        pade_ = false;
+        throw;
+    }
+}
+
+static double factorial(int n) {
+    double f = 1.0;
+    for (int k = 2; k <= n; k++) f *= k;
+    return f;
+}
+
+// Pade_approximant::modify_reaction_matrix (pade_approximant.cc:97-239): A with
+// A[p][p] = -ln2 dt/T, A[p][product] = ln2 dt/T ratio; N(A) and D(A) the (nom, den)
+// Pade polynomials of exp evaluated by Horner; the step matrix is P = (D^-1 N)^T and
+// each element's concentrations become P c (pade_approximant.cc:274-290).  Stored
+// here transposed, R[from][to] = P[to][from], so the step kernel's c_to = sum c_from
+// R[from][to] forms the same products in the same order.
+static void pade_matrix(int n, const std::vector<int> &parent, const std::vector<int> &pptr,
+                        const std::vector<int> &prod, const std::vector<double> &hl, const std::vector<double> &branch,
+                        int p, int q, double dt, std::vector<double> &R) {
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (size_t i = 0; i < parent.size(); i++) {
+        const double rel = dt / hl[i];
+        A[(size_t)parent[i] * n + parent[i]] = -std::log(2.0) * rel;
+        for (int k = pptr[i]; k < pptr[i + 1]; k++) A[(size_t)parent[i] * n + prod[k]] = std::log(2.0) * rel * branch[k];
+    }
+    auto poly = [&](int deg, bool den) {
+        std::vector<double> P((size_t)n * n, 0.0), T((size_t)n * n);
+        for (int j = deg; j >= 0; j--) {
+            const double c = den ? std::pow(-1.0, j) * factorial(p + q - j) * factorial(q) /
+                                       (factorial(p + q) * factorial(j) * factorial(q - j))
+                                 : factorial(p + q - j) * factorial(p) /
+                                       (factorial(p + q) * factorial(j) * factorial(p - j));
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    double v = 0.0;
+                    for (int k = 0; k < n; k++) v += P[(size_t)a * n + k] * A[(size_t)k * n + b];
+                    T[(size_t)a * n + b] = v;
+                }
+            P = T;
+            for (int a = 0; a < n; a++) P[(size_t)a * n + a] += c;
+        }
+        return P;
+    };
+    std::vector<double> Nm = poly(p, false), Dm = poly(q, true);
+    // column r of N solved against D (LU with partial pivoting): X = D^-1 N, P[r][c] = X[c][r]
+    std::vector<double> LU = Dm;
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; k++) {
+        int m = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(LU[(size_t)i * n + k]) > std::fabs(LU[(size_t)m * n + k])) m = i;
+        piv[k] = m;
+        if (m != k)
+            for (int j = 0; j < n; j++) std::swap(LU[(size_t)k * n + j], LU[(size_t)m * n + j]);
+        if (LU[(size_t)k * n + k] == 0.0) throw Error(-3, "singular Pade denominator");
+        for (int i = k + 1; i < n; i++) {
+            LU[(size_t)i * n + k] /= LU[(size_t)k * n + k];
+            for (int j = k + 1; j < n; j++) LU[(size_t)i * n + j] -= LU[(size_t)i * n + k] * LU[(size_t)k * n + j];
+        }
+    }
+    R.assign((size_t)n * n, 0.0);
+    std::vector<double> x(n);
+    for (int r = 0; r < n; r++) {
+        for (int i = 0; i < n; i++) x[i] = Nm[(size_t)i * n + r];
+        for (int k = 0; k < n; k++) std::swap(x[k], x[piv[k]]);
+        for (int i = 1; i < n; i++)
+            for (int j = 0; j < i; j++) x[i] -= LU[(size_t)i * n + j] * x[j];
+        for (int i = n - 1; i >= 0; i--) {
+            for (int j = i + 1; j < n; j++) x[i] -= LU[(size_t)i * n + j] * x[j];
+            x[i] /= LU[(size_t)i * n + i];
+        }
+        // P[r][c] = x[c]; R[from = c][to = r] = P[r][c]
+        for (int c = 0; c < n; c++) R[(size_t)c * n + r] = x[c];
+    }
+}
+
 // Linear_reaction::modify_reaction_matrix (linear_reaction.cc:144-165): identity,
 // then per decay R[p][p] = 0.5^(dt / half_life), R[p][product_k] = (1 - that) * ratio_k
 void TransportGPU::build_reaction_matrix(double dt) {
     const int n = nsub_;
+    if (pade_) {
+        pade_matrix(n, r_parent_, r_pptr_, r_prod_, r_hl_, r_branch_, pade_nom_, pade_den_, dt, h_react_);
+        FL_HIP(hipMemcpyAsync(d_react_, h_react_.data(), h_react_.size() * sizeof(double), hipMemcpyHostToDevice,
+                              stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+        return;
+    }
     h_react_.assign((size_t)n * n, 0.0);
     for (int k = 0; k < n; k++) h_react_[(size_t)k * n + k] = 1.0;
     for (size_t i = 0; i < r_parent_.size(); i++) {

@@ -139,6 +139,8 @@ SIGNATURES = {
     "fl_transport_mass_balance": (C.c_int, [C.c_void_p] + [_f64p] * 6),
     "fl_transport_set_sources": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
     "fl_transport_set_bc_conc": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_transport_set_pade_reactions": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, C.c_int,
+                                                  C.c_int]),
     "fl_transport_set_linear_reactions": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]),
     "fl_transport_nnz": (C.c_int, [C.c_void_p]),
     "fl_transport_matrix": (C.c_int, [C.c_void_p, _i32p, _i32p, _f64p]),
@@ -674,11 +676,21 @@ class ConvectionTransport:
         self._bcc_next = _f64(bc_conc, self.nsub * self.mesh.n_bc_elements)
         _check(lib().fl_transport_set_bc_conc(self.h, self._bcc_next))
 
+    def set_pade_reactions(self, decays, names=None, nom_pol_deg=2, den_pol_deg=2):
+        """Pade_approximant (pade_approximant.cc:97-290) with the decays of set_linear_reactions."""
+        self._decay_arrays(decays, names)
+        _check(lib().fl_transport_set_pade_reactions(self.h, self._n_decays, *self._react, int(nom_pol_deg),
+                                                     int(den_pol_deg)))
+
     def set_linear_reactions(self, decays, names=None):
         """Linear_reaction decays (linear_reaction.cc:200-258): a list of dicts with
         parent, products, optional branch_ratios (default one product, ratio 1.0) and
         half_life or kinetic (half_life = ln 2 / kinetic).  Substances are given by
         index or, with `names`, by name."""
+        self._decay_arrays(decays, names)
+        _check(lib().fl_transport_set_linear_reactions(self.h, self._n_decays, *self._react))
+
+    def _decay_arrays(self, decays, names):
         idx = (lambda x: names.index(x) if isinstance(x, str) else int(x))
         parent, pptr, prod, hl, br = [], [0], [], [], []
         for d in decays:
@@ -691,9 +703,9 @@ class ConvectionTransport:
             br += list(ratios)
             pptr.append(len(prod))
             hl.append(d["half_life"] if "half_life" in d else math.log(2) / d["kinetic"])
-        self._react = [np.array(parent, np.int32), np.array(pptr, np.int32), np.array(prod or [0], np.int32),
+        self._n_decays = len(parent)
+        self._react = [np.array(parent or [0], np.int32), np.array(pptr, np.int32), np.array(prod or [0], np.int32),
                        np.array(hl or [1.0], np.float64), np.array(br or [0.0], np.float64)]
-        _check(lib().fl_transport_set_linear_reactions(self.h, len(parent), *self._react))
 
     def write_mass_balance(self, path, names):
         """MassBalance::output block at the current time (mass_balance.cc:242-354)."""
