@@ -1828,6 +1828,14 @@ void orc_transport_set_pade(void *h, int nom_deg, int den_deg) {
     T.pade_den = den_deg;
 }
 
+// New flux field (set_velocity_field): the next step rebuilds the transport matrix,
+// the step constraint and the boundary vector (transport.cc:605-619).
+void orc_transport_set_velocity(void *h, const double *side_flux) {
+    Transport &T = *(Transport *)h;
+    T.flux.assign(side_flux, side_flux + T.flux.size());
+    T.matrix_built = false;
+}
+
 // Boundary concentrations bc_conc[nsub][NBC] from the next step on (a time-dependent
 // bc_conc field whose new data time has been reached).
 void orc_transport_set_bc_conc(void *h, const double *bc_conc) {

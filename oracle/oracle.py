@@ -64,6 +64,7 @@ def lib():
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
+        L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
         L.orc_darcy_unsteady_step.restype = C.c_int
@@ -267,6 +268,10 @@ class Transport:
     def set_sources(self, density, sigma, conc):
         self._src = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (density, sigma, conc)]
         lib().orc_transport_set_sources(self.h, *self._src)
+
+    def set_velocity(self, side_flux):
+        self._keep[0] = np.ascontiguousarray(side_flux, np.float64).reshape(-1).copy()
+        lib().orc_transport_set_velocity(self.h, self._keep[0])
 
     def set_bc_conc(self, bc_conc):
         self._bcc = np.ascontiguousarray(bc_conc, np.float64).reshape(-1)
