@@ -8,7 +8,13 @@
 //   ConvectionTransport + TransportOperatorSplitting
 //   (src/transport/transport.h:81-290, transport_operator_splitting.hh:37-144)
 //     set_velocity_field(), update_solution(), concentration matrix,
-//     mass balance.
+//     mass balance; sources, linear / Pade reactions, time-dependent
+//     boundary data.
+//   DarcyFlowMH_Unsteady / DarcyFlowLMH_Unsteady (darcy_flow_mh.cc:1601-1827)
+//   FieldElementwise (src/fields/field_elementwise_impl.hh:67-156)
+//   OldBcdInput (src/flow/old_bcd.cc:44-250)
+//   OutputMSH (src/io/output_msh.cc:53-467), water / mass balance files
+//     (darcy_flow_mh_output.cc:671-773, mass_balance.cc:242-354)
 //
 // Errors: the reference terminates through xprintf(Err) (system.hh:74-109);
 // here a failed call throws fl::GpuError carrying fl_last_error().
@@ -93,14 +99,128 @@ class DarcyFlowMH_Steady {
     double edge_scalar(unsigned edge) const { return solution_[(size_t)n_sides_ + n_elem_ + edge]; }
     const fl_solve_info &last_solve() const { return last_; }
     fl_darcy *handle() const { return d_; }
+    // DarcyFlowMHOutput::water_balance: one block appended to path
+    void write_water_balance(const std::string &path, double time) {
+        check(fl_darcy_write_water_balance(d_, path.c_str(), time), "water balance");
+    }
 
-   private:
+   protected:
     GpuMesh &mesh_;
     fl_darcy *d_ = nullptr;
     fl_solver_opts solver_;
     fl_solve_info last_{};
     std::vector<double> solution_;
     unsigned n_sides_ = 0, n_elem_ = 0;
+};
+
+// DarcyFlowMH_Unsteady (lumped = false) / DarcyFlowLMH_Unsteady (lumped = true):
+// update_solution() advances by one implicit Euler step of dt.
+class DarcyFlowUnsteady : public DarcyFlowMH_Steady {
+   public:
+    DarcyFlowUnsteady(GpuMesh &mesh, const DarcyData &data, bool lumped, const std::vector<double> &storativity,
+                      const std::vector<double> &init_pressure, double dt,
+                      fl_solver_opts solver = {1e-7, 1e-9, 10000, 0})
+        : DarcyFlowMH_Steady(mesh, data, solver), dt_(dt) {
+        check(fl_darcy_set_unsteady(d_, lumped ? 1 : 0, storativity.data(), init_pressure.data()), "set_unsteady");
+        check(fl_darcy_get_solution(d_, solution_.data()), "get_solution");
+    }
+    void set_dt(double dt) { dt_ = dt; }
+    void update_solution() {
+        check(fl_darcy_unsteady_step(d_, dt_, &solver_, &last_), "unsteady step");
+        check(fl_darcy_get_solution(d_, solution_.data()), "get_solution");
+        time_ += dt_;
+    }
+    double time() const { return time_; }
+
+   private:
+    double dt_, time_ = 0.0;
+};
+
+// FieldElementwise: values of one gmsh $ElementData field on the bulk or boundary elements.
+class FieldElementwise {
+   public:
+    FieldElementwise(GpuMesh &mesh, const std::string &gmsh_file, const std::string &field_name, int n_comp = 1,
+                     bool boundary = false) {
+        check(fl_element_data_open(mesh.handle(), gmsh_file.c_str(), field_name.c_str(), n_comp, boundary ? 1 : 0,
+                                   &f_),
+              "element data");
+        int info[3];
+        check(fl_element_data_info(f_, info, nullptr, 0), "element data info");
+        n_ = (size_t)info[0] * info[1];
+    }
+    ~FieldElementwise() { fl_element_data_free(f_); }
+    FieldElementwise(const FieldElementwise &) = delete;
+    FieldElementwise &operator=(const FieldElementwise &) = delete;
+    bool set_time(double t) {
+        int updated = 0;
+        check(fl_element_data_set_time(f_, t, &updated), "element data set_time");
+        return updated != 0;
+    }
+    std::vector<double> value() const {
+        std::vector<double> v(n_);
+        check(fl_element_data_values(f_, v.data()), "element data values");
+        return v;
+    }
+
+   private:
+    fl_element_data *f_ = nullptr;
+    size_t n_ = 0;
+};
+
+// OldBcdInput: legacy .fbc (into DarcyData) and .tbc (bc_conc[n_subst][NBC]) files.
+class OldBcdInput {
+   public:
+    explicit OldBcdInput(GpuMesh &mesh) : mesh_(mesh) {}
+    ~OldBcdInput() {
+        if (b_) fl_old_bcd_free(b_);
+    }
+    OldBcdInput(const OldBcdInput &) = delete;
+    OldBcdInput &operator=(const OldBcdInput &) = delete;
+    void read_flow(const std::string &path, DarcyData &d) {
+        const unsigned nb = mesh_.n_bc_elements();
+        d.bc_type.assign(nb, 0);
+        d.bc_pressure.assign(nb, 0.0);
+        d.bc_flux.assign(nb, 0.0);
+        d.bc_robin_sigma.assign(nb, 0.0);
+        if (b_) fl_old_bcd_free(b_), b_ = nullptr;
+        check(fl_old_bcd_read_flow(mesh_.handle(), path.c_str(), d.bc_type.data(), d.bc_pressure.data(),
+                                   d.bc_flux.data(), d.bc_robin_sigma.data(), &b_),
+              "flow bcd");
+    }
+    std::vector<double> read_transport(const std::string &path, int n_subst) {
+        std::vector<double> c((size_t)n_subst * mesh_.n_bc_elements());
+        check(fl_old_bcd_read_transport(b_, path.c_str(), n_subst, c.data()), "transport bcd");
+        return c;
+    }
+
+   private:
+    GpuMesh &mesh_;
+    fl_old_bcd *b_ = nullptr;
+};
+
+// OutputMSH: the mesh head once, then $ElementData sections (values at `precision`
+// significant digits; the reference's stream default is 6).
+class OutputMSH {
+   public:
+    OutputMSH(GpuMesh &mesh, const std::string &path, int precision = 6)
+        : mesh_(mesh), path_(path), precision_(precision) {}
+    void write_elem_data(const std::string &name, const std::string &units, const double *values, int n_comp,
+                         int n_values, double time) {
+        if (!head_) {
+            check(fl_msh_write_head(mesh_.handle(), path_.c_str()), "msh head");
+            head_ = true;
+        }
+        check(fl_msh_write_element_data(path_.c_str(), name.c_str(), units.c_str(), time, step_, n_comp, n_values,
+                                        values, precision_),
+              "msh element data");
+    }
+    void next_step() { step_++; }
+
+   private:
+    GpuMesh &mesh_;
+    std::string path_;
+    int precision_, step_ = 0;
+    bool head_ = false;
 };
 
 class ConvectionTransport {
@@ -134,6 +254,35 @@ class ConvectionTransport {
         double t = 0;
         fl_transport_time(t_, &t, nullptr, nullptr);
         return t;
+    }
+    // ConvectionTransport sources_density / sources_sigma / sources_conc, each [n_subst][E]
+    void set_sources(const std::vector<double> &density, const std::vector<double> &sigma,
+                     const std::vector<double> &conc) {
+        check(fl_transport_set_sources(t_, density.data(), sigma.data(), conc.data()), "sources");
+    }
+    // Linear_reaction / Pade_approximant decays: parent[i], products in CSR form
+    void set_reactions(const std::vector<int> &parent, const std::vector<int> &product_ptr,
+                       const std::vector<int> &products, const std::vector<double> &half_life,
+                       const std::vector<double> &branch_ratio, int pade_nom = -1, int pade_den = -1) {
+        if (pade_nom < 0)
+            check(fl_transport_set_linear_reactions(t_, (int)parent.size(), parent.data(), product_ptr.data(),
+                                                    products.data(), half_life.data(), branch_ratio.data()),
+                  "reactions");
+        else
+            check(fl_transport_set_pade_reactions(t_, (int)parent.size(), parent.data(), product_ptr.data(),
+                                                  products.data(), half_life.data(), branch_ratio.data(), pade_nom,
+                                                  pade_den),
+                  "reactions");
+    }
+    // new boundary data from the next step on
+    void set_bc_conc(const std::vector<double> &bc_conc) {
+        check(fl_transport_set_bc_conc(t_, bc_conc.data()), "bc_conc");
+    }
+    // MassBalance::output: one block appended to path
+    void write_mass_balance(const std::string &path, const std::vector<std::string> &names) {
+        std::vector<const char *> p;
+        for (const auto &n : names) p.push_back(n.c_str());
+        check(fl_transport_write_mass_balance(t_, path.c_str(), p.data()), "mass balance");
     }
 
    private:

@@ -181,3 +181,21 @@ def test_reactions_with_flow_generated_mesh(tmp_path, nsub):
         assert np.array_equal(T0.conc(), T1.conc()), t
     m0, m1 = T0.mass_balance(), T1.mass_balance()
     assert np.allclose(m1["totals"], m0["totals"], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.gpu
+def test_cpp_adapter_decay_chain(tmp_path):
+    """examples/decay_chain.cpp: the same case through the C++ adapter (legacy files,
+    steady flow, reactions, MSH output stream); its output file equals the
+    reference's transport_decay.msh byte for byte."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(G.LIB_PATH), "examples", "decay_chain")
+    assert os.path.exists(exe), "build() compiles the example"
+    out, mb = str(tmp_path / "t.msh"), str(tmp_path / "mb.txt")
+    r = subprocess.run([exe, cases.ref_path("test12.msh"), cases.ref_path("test12.fbc"), cases.ref_path("test12.tbc"),
+                        out, mb], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "decay chain:" in r.stdout
+    assert open(out).read() == open(cases.ref_path("test12_transport_decay.msh")).read()
+    assert open(mb).read().count("MASS BALANCE") == 21
