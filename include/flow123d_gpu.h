@@ -241,6 +241,14 @@ int fl_darcy_water_balance(fl_darcy *d, double *balance, double *outflow, double
 int fl_darcy_edge_nnz(fl_darcy *d);
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs);
 
+/* DarcyFlowMHOutput::make_element_scalar / make_element_vector
+ * (src/flow/darcy_flow_mh_output.cc:261-325) for the current solution:
+ * pressure[E], piezo_head[E] (pressure + z of the element centre) and
+ * velocity[3E], the Darcy velocity at the element centre from the RT0 side
+ * fluxes divided by the cross-section (MHFEValues::RT0_value,
+ * mh_fe_values.cc:76-118). */
+int fl_darcy_element_output(fl_darcy *d, double *pressure, double *piezo_head, double *velocity);
+
 /* DarcyFlowMHOutput::water_balance for the current solution: boundary side
  * fluxes and the |e| cs f sources per region, one block appended to path. */
 int fl_darcy_write_water_balance(fl_darcy *d, const char *path, double time);

@@ -1324,6 +1324,83 @@ double orc_darcy_mh_residual(void *h) {
     return std::sqrt(rn) / std::max(std::sqrt(bn), 1e-300);
 }
 
+// MHFEValues::RT0_value (mh_fe_values.cc:76-118) with the basis coefficients of
+// basis_functions_triangle / _tetrahedron (:268-295, :379-397).
+static void rt0_value(const Mesh &m, const Element &e, const double *x, int face, double *out) {
+    if (e.dim == 1) {
+        double lv[3], d[3];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), lv);
+        double ll = dot3(lv, lv);
+        if (face == 0) {
+            sub3(x, m.P(e.node[1]), d);
+            double s = -norm3(d);
+            for (int k = 0; k < 3; k++) out[k] = s * lv[k] / ll;
+        } else {
+            sub3(x, m.P(e.node[0]), d);
+            double s = norm3(d);
+            for (int k = 0; k < 3; k++) out[k] = s * lv[k] / ll;
+        }
+    } else if (e.dim == 2) {
+        double ex[3], ac[3], ez[3], ey[3], u[3], v[3], n[3], t[3], b[3];
+        sub3(m.P(e.node[1]), m.P(e.node[0]), ex);
+        normalize3(ex);
+        sub3(m.P(e.node[2]), m.P(e.node[0]), ac);
+        cross3(ex, ac, ez);
+        normalize3(ez);
+        cross3(ez, ex, ey);
+        normalize3(ey);
+        // node_coordinates_triangle (:218-262)
+        sub3(m.P(e.node[1]), m.P(e.node[0]), u);
+        sub3(m.P(e.node[2]), m.P(e.node[0]), v);
+        cross3(u, v, n);
+        normalize3(n);
+        t[0] = u[0]; t[1] = u[1]; t[2] = u[2];
+        normalize3(t);
+        cross3(n, u, b);
+        normalize3(b);
+        double nod[3][2] = {{0.0, 0.0}, {dot3(t, u), dot3(b, u)}, {dot3(t, v), dot3(b, v)}};
+        int i0 = face, i1 = (face + 1) % 3, i2 = (face + 2) % 3;
+        double alfa = nod[i2][0], beta = nod[i2][1];
+        double gama = 1.0 / std::fabs((nod[i0][0] - nod[i2][0]) * (nod[i1][1] - nod[i0][1]) +
+                                      (nod[i0][1] - nod[i2][1]) * (nod[i0][0] - nod[i1][0]));
+        double w[3];
+        sub3(x, m.P(e.node[0]), w);
+        double cx = dot3(w, ex) - alfa, cy = dot3(w, ey) - beta;
+        for (int k = 0; k < 3; k++) out[k] = gama * (cx * ex[k] + cy * ey[k]);
+    } else {
+        const double *pf = m.P(e.node[face]);
+        double nrm[3], sc[3];
+        side_normal(m, e, face, nrm);
+        side_centre(m, e, face, sc);
+        double delta = 1.0 / (side_measure(m, e, face) * (nrm[0] * sc[0] + nrm[1] * sc[1] + nrm[2] * sc[2] -
+                                                          nrm[0] * pf[0] - nrm[1] * pf[1] - nrm[2] * pf[2]));
+        for (int k = 0; k < 3; k++) out[k] = delta * (x[k] - pf[k]);
+    }
+}
+
+// DarcyFlowMHOutput::make_element_scalar / make_element_vector
+// (darcy_flow_mh_output.cc:261-325): pressure, p + z of the centre, centre velocity.
+void orc_darcy_element_output(void *h, double *pressure, double *head, double *velocity) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    for (int e = 0; e < D.E; e++) {
+        const Element &el = m.element[e];
+        double c[3];
+        elem_centre(m, el, c);
+        double p = D.sol[D.S + e];
+        pressure[e] = p;
+        head[e] = p + c[2];
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int li = 0; li < el.n_sides(); li++) {
+            double b[3];
+            rt0_value(m, el, c, li, b);
+            double q = D.sol[m.side_offset[e] + li];
+            for (int k = 0; k < 3; k++) f[k] += q * b[k] / D.cs[e];
+        }
+        for (int k = 0; k < 3; k++) velocity[3 * e + k] = f[k];
+    }
+}
+
 // Water balance per boundary region: DarcyFlowMHOutput::water_balance,
 // darcy_flow_mh_output.cc:671-773.  out arrays indexed by boundary region
 // sub-index: balance, plus (outflow), minus (inflow).

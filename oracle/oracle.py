@@ -64,6 +64,7 @@ def lib():
         L.orc_transport_mass_balance.argtypes = [C.c_void_p] + [_f64p] * 6
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
+        L.orc_darcy_element_output.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
@@ -172,6 +173,12 @@ class Darcy:
         self.stats = dict(iters=int(st[0]), resid=st[1], bnorm=st[2], t_assembly=st[3], t_schur=st[4],
                           t_solve=st[5], t_resolve=st[6])
         return it
+
+    def element_output(self):
+        E = self.mesh.n_elements
+        p, h, v = np.zeros(E), np.zeros(E), np.zeros(3 * E)
+        lib().orc_darcy_element_output(self.h, p, h, v)
+        return p, h, v.reshape(E, 3)
 
     def set_unsteady(self, lumped, storativity, init_pressure, dt):
         self._unst = [np.ascontiguousarray(storativity, np.float64), np.ascontiguousarray(init_pressure, np.float64)]

@@ -418,6 +418,13 @@ int fl_darcy_write_water_balance(fl_darcy *d, const char *path, double time) {
     });
 }
 
+int fl_darcy_element_output(fl_darcy *d, double *pressure, double *piezo_head, double *velocity) {
+    return guard([&] {
+        d->g->element_output(pressure, piezo_head, velocity);
+        return 0;
+    });
+}
+
 int fl_darcy_edge_nnz(fl_darcy *d) { return d->g->nnz(); }
 
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs) {

@@ -91,6 +91,8 @@ class DarcyGPU {
     bool unsteady() const { return unsteady_ != 0; }
     void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region
     void source_balance(double *src) const;                                // per bulk region
+    // element pressure [E], piezometric head [E], centre velocity [3E] of the current solution
+    void element_output(double *pressure, double *head, double *velocity) const;
     double solution_precision(double rtol, double atol) const;
     int nnz() const { return nnz_; }
     const HostMesh &mesh() const { return m_; }

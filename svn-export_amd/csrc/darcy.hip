@@ -127,6 +127,39 @@ __global__ void __launch_bounds__(256) k_resolve(const DarcyDev dv, const double
     dv.sol[dv.S + e] = pe;
 }
 
+// DarcyFlowMHOutput::make_element_scalar / make_element_vector
+// (darcy_flow_mh_output.cc:261-325): element pressure, piezometric head p + z of
+// the centre, and the Darcy velocity at the centre, sum_i q_i RT0_i(centre) / cs.
+template <int D>
+__global__ void __launch_bounds__(256) k_elem_output(const DarcyDev dv, double *__restrict__ pres,
+                                                    double *__restrict__ head, double *__restrict__ vel) {
+    int idx = dv.dim_start[D - 1] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= dv.dim_start[D]) return;
+    int e = dv.perm[idx];
+    Elem<D> el;
+    for (int i = 0; i <= D; i++) {
+        int n = dv.el_nodes[4 * e + i];
+        el.P[i] = V3{dv.xyz[3 * n], dv.xyz[3 * n + 1], dv.xyz[3 * n + 2]};
+    }
+    V3 c = elem_centre<D>(el);
+    const double p = dv.sol[dv.S + e];
+    pres[e] = p;
+    head[e] = p + c.z;
+    const int so = dv.side_off[e];
+    const double cs = dv.cs[e];
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    for (int i = 0; i <= D; i++) {
+        V3 b = rt0_value<D>(el, c, i);
+        const double q = dv.sol[so + i];
+        vx += q * b.x / cs;
+        vy += q * b.y / cs;
+        vz += q * b.z / cs;
+    }
+    vel[3 * e] = vx;
+    vel[3 * e + 1] = vy;
+    vel[3 * e + 2] = vz;
+}
+
 // DarcyFlowLMH_Unsteady::postprocess (darcy_flow_mh.cc:1772-1827): every side flux
 // gets the lumped storage term of its edge, then the element's source share
 __global__ void k_lmh_post(const DarcyDev dv, const double *__restrict__ meas, const double *__restrict__ stor,
@@ -818,6 +851,29 @@ SolveStats DarcyGPU::unsteady_step(double dt, double rtol, double atol, int max_
     }
     dv_.tc = dv_.pprev = dv_.te = dv_.lprev = nullptr;  // a later steady solve sees the steady system
     return st;
+}
+
+void DarcyGPU::element_output(double *pressure, double *head, double *velocity) const {
+    const DarcyDev &d = dv_;
+    const int E = m_.n_elements();
+    double *buf = dev_alloc<double>((size_t)5 * E);
+    for (int D = 1; D <= 3; D++) {
+        int n = d.dim_start[D] - d.dim_start[D - 1];
+        if (n <= 0) continue;
+        dim3 g(grid_for(n)), b(kBlock);
+        if (D == 1) hipLaunchKernelGGL(k_elem_output<1>, g, b, 0, stream_, d, buf, buf + E, buf + 2 * E);
+        if (D == 2) hipLaunchKernelGGL(k_elem_output<2>, g, b, 0, stream_, d, buf, buf + E, buf + 2 * E);
+        if (D == 3) hipLaunchKernelGGL(k_elem_output<3>, g, b, 0, stream_, d, buf, buf + E, buf + 2 * E);
+    }
+    std::vector<double> h((size_t)5 * E);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpyAsync(h.data(), buf, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream_);
+    dev_free(buf);
+    if (err != hipSuccess) throw Error(-2, std::string("element output: ") + hipGetErrorString(err));
+    std::copy(h.begin(), h.begin() + E, pressure);
+    std::copy(h.begin() + E, h.begin() + 2 * E, head);
+    std::copy(h.begin() + 2 * E, h.end(), velocity);
 }
 
 void DarcyGPU::get_solution(double *out) const {

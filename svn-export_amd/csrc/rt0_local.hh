@@ -193,6 +193,47 @@ __device__ __forceinline__ void local_matrix(const Elem<D> &el, double scale, co
     }
 }
 
+// MHFEValues::RT0_value (mh_fe_values.cc:76-118): the RT0 basis function of
+// side `face` at point x, with the basis coefficients local_matrix_triangle /
+// local_matrix_tetrahedron set up (the triangle's in its own plane frame).
+template <int D>
+__device__ __forceinline__ V3 rt0_value(const Elem<D> &el, V3 x, int face) {
+    if constexpr (D == 1) {
+        V3 lv = vsub(el.P[1], el.P[0]);
+        double ll = vdot(lv, lv);
+        double s = face == 0 ? -vnorm(vsub(x, el.P[1])) : vnorm(vsub(x, el.P[0]));
+        return V3{s * lv.x / ll, s * lv.y / ll, s * lv.z / ll};
+    } else if constexpr (D == 2) {
+        V3 ex = vnormalize(vsub(el.P[1], el.P[0]));
+        V3 ac = vsub(el.P[2], el.P[0]);
+        V3 ez = vnormalize(vcross(ex, ac));
+        V3 ey = vnormalize(vcross(ez, ex));
+        // node coordinates in the plane frame (node_coordinates_triangle)
+        V3 u1 = vsub(el.P[1], el.P[0]), u2 = vsub(el.P[2], el.P[0]);
+        V3 n = vnormalize(vcross(u1, u2));
+        V3 tt = vnormalize(u1);
+        V3 b = vnormalize(vcross(n, u1));
+        double nod[3][2] = {{0.0, 0.0}, {vdot(tt, u1), vdot(b, u1)}, {vdot(tt, u2), vdot(b, u2)}};
+        const int i = face, i1 = (face + 1) % 3, i2 = (face + 2) % 3;
+        double x0 = nod[i][0], y0 = nod[i][1], x1 = nod[i1][0], y1 = nod[i1][1], x2 = nod[i2][0], y2 = nod[i2][1];
+        double al = x2, be = y2, ga = 1.0 / fabs((x0 - x2) * (y1 - y0) + (y0 - y2) * (x0 - x1));
+        V3 u = vsub(x, el.P[0]);
+        double cx = vdot(u, ex) - al, cy = vdot(u, ey) - be;
+        return V3{ga * (cx * ex.x + cy * ey.x), ga * (cx * ex.y + cy * ey.y), ga * (cx * ex.z + cy * ey.z)};
+    } else {
+        V3 ec = elem_centre<D>(el);
+        V3 s0 = el.P[side_node(3, face, 0)], s1 = el.P[side_node(3, face, 1)], s2 = el.P[side_node(3, face, 2)];
+        V3 nrm = vnormalize(vcross(vsub(s1, s0), vsub(s2, s0)));
+        if (vdot(nrm, vsub(ec, s0)) > 0.0) nrm = V3{-nrm.x, -nrm.y, -nrm.z};
+        V3 sc = side_centre<D>(el, face);
+        V3 pf = el.P[face];
+        double sm = 0.5 * vnorm(vcross(vsub(s1, s0), vsub(s2, s0)));
+        double de = 1.0 / (sm * (nrm.x * sc.x + nrm.y * sc.y + nrm.z * sc.z - nrm.x * pf.x - nrm.y * pf.y -
+                                 nrm.z * pf.z));
+        return V3{de * (x.x - pf.x), de * (x.y - pf.y), de * (x.z - pf.z)};
+    }
+}
+
 // MatrixInverse (math_fce.cc:124-230); returns det
 template <int N>
 __device__ __forceinline__ double small_inverse(const double *a, double *b) {

@@ -79,6 +79,7 @@ SIGNATURES = {
     "fl_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                         _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
     "fl_darcy_write_water_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "fl_darcy_element_output": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
     "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "fl_darcy_free": (None, [C.c_void_p]),
@@ -450,6 +451,14 @@ class DarcyFlowMH_Steady:
         _check(lib().fl_darcy_water_balance(self.h, b, p, m))
         k = self.mesh.n_boundary_regions
         return b[:k], p[:k], m[:k]
+
+    def element_output(self):
+        """(pressure[E], piezo_head[E], velocity[E, 3]) of the current solution
+        (DarcyFlowMHOutput::make_element_scalar / make_element_vector)."""
+        E = self.mesh.n_elements
+        p, h, v = np.zeros(E), np.zeros(E), np.zeros(3 * E)
+        _check(lib().fl_darcy_element_output(self.h, p, h, v))
+        return p, h, v.reshape(E, 3)
 
     def write_water_balance(self, path, time=float("inf")):
         """DarcyFlowMHOutput::water_balance block for this solution (steady: time inf)."""
