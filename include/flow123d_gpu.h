@@ -128,6 +128,11 @@ int fl_old_bcd_read_transport(const fl_old_bcd *b, const char *path, int n_subst
 
 void fl_old_bcd_free(fl_old_bcd *b);
 
+/* OutputMSH::write_msh_node_data (output_msh.cc:327-381): appends one
+ * $NodeData section, laid out as fl_msh_write_element_data's rows. */
+int fl_msh_write_node_data(const char *path, const char *name, const char *units, double time, int step,
+                           int n_comp, int n_values, const double *values, int precision);
+
 /* ---- balance files --------------------------------------------------------- */
 
 /* DarcyFlowMHOutput::water_balance (src/flow/darcy_flow_mh_output.cc:671-773):
@@ -248,6 +253,11 @@ int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *r
  * fluxes divided by the cross-section (MHFEValues::RT0_value,
  * mh_fe_values.cc:76-118). */
 int fl_darcy_element_output(fl_darcy *d, double *pressure, double *piezo_head, double *velocity);
+
+/* DarcyFlowMHOutput::make_node_scalar_param (darcy_flow_mh_output.cc:385-480):
+ * pressure_nodes[n_nodes], the element pressures and side traces around each
+ * node averaged with the reference's distance weights. */
+int fl_darcy_node_pressure(fl_darcy *d, double *pressure_nodes);
 
 /* DarcyFlowMHOutput::water_balance for the current solution: boundary side
  * fluxes and the |e| cs f sources per region, one block appended to path. */

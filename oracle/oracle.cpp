@@ -1401,6 +1401,49 @@ void orc_darcy_element_output(void *h, double *pressure, double *head, double *v
     }
 }
 
+// DarcyFlowMHOutput::make_node_scalar_param (darcy_flow_mh_output.cc:424-541):
+// first pass sums the distances node -> element centre and node -> side centre and
+// counts them, second pass adds element pressure and side trace with weight
+// (1 - d / sum d) / (count - 1); elements first, then the sides of every element.
+void orc_darcy_node_pressure(void *h, double *out) {
+    Darcy &D = *(Darcy *)h;
+    Mesh &m = *D.m;
+    const int N = (int)m.X.size() / 3;
+    std::vector<int> ne(N, 0), ns(N, 0);
+    std::vector<double> de(N, 0.0), ds(N, 0.0);
+    std::fill(out, out + N, 0.0);
+    auto dist = [&](int v, const double *c) {
+        const double *p = m.P(v);
+        return std::sqrt((p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) + (p[2] - c[2]) * (p[2] - c[2]));
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        for (int e = 0; e < D.E; e++) {
+            const Element &el = m.element[e];
+            double c[3];
+            elem_centre(m, el, c);
+            for (int k = 0; k < el.n_nodes(); k++) {
+                int v = el.node[k];
+                double d = dist(v, c);
+                if (pass == 0) de[v] += d, ne[v]++;
+                else out[v] += D.sol[D.S + e] * (1 - d / (de[v] + ds[v])) / (ne[v] + ns[v] - 1);
+            }
+        }
+        for (int e = 0; e < D.E; e++) {
+            const Element &el = m.element[e];
+            for (int s = 0; s < el.n_sides(); s++) {
+                double c[3];
+                side_centre(m, el, s, c);
+                for (int k = 0; k < el.dim; k++) {
+                    int v = snode(m, el, s, k);
+                    double d = dist(v, c);
+                    if (pass == 0) ds[v] += d, ns[v]++;
+                    else out[v] += D.sol[D.S + D.E + el.edge_idx[s]] * (1 - d / (de[v] + ds[v])) / (ne[v] + ns[v] - 1);
+                }
+            }
+        }
+    }
+}
+
 // Water balance per boundary region: DarcyFlowMHOutput::water_balance,
 // darcy_flow_mh_output.cc:671-773.  out arrays indexed by boundary region
 // sub-index: balance, plus (outflow), minus (inflow).

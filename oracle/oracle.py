@@ -65,6 +65,7 @@ def lib():
         L.orc_transport_set_sources.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
         L.orc_darcy_element_output.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.orc_darcy_node_pressure.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
@@ -179,6 +180,11 @@ class Darcy:
         p, h, v = np.zeros(E), np.zeros(E), np.zeros(3 * E)
         lib().orc_darcy_element_output(self.h, p, h, v)
         return p, h, v.reshape(E, 3)
+
+    def node_pressure(self):
+        out = np.zeros(max(self.mesh.n_nodes, 1))
+        lib().orc_darcy_node_pressure(self.h, out)
+        return out[:self.mesh.n_nodes]
 
     def set_unsteady(self, lumped, storativity, init_pressure, dt):
         self._unst = [np.ascontiguousarray(storativity, np.float64), np.ascontiguousarray(init_pressure, np.float64)]

@@ -425,6 +425,23 @@ int fl_darcy_element_output(fl_darcy *d, double *pressure, double *piezo_head, d
     });
 }
 
+int fl_darcy_node_pressure(fl_darcy *d, double *pressure_nodes) {
+    return guard([&] {
+        d->g->node_pressure(pressure_nodes);
+        return 0;
+    });
+}
+
+int fl_msh_write_node_data(const char *path, const char *name, const char *units, double time, int step,
+                           int n_comp, int n_values, const double *values, int precision) {
+    return guard([&] {
+        std::string err =
+            fl::msh_write_node_data(path, name, units, time, step, n_comp, n_values, values, precision);
+        if (!err.empty()) throw fl::Error(-1, err);
+        return 0;
+    });
+}
+
 int fl_darcy_edge_nnz(fl_darcy *d) { return d->g->nnz(); }
 
 int fl_darcy_edge_system(fl_darcy *d, int *ptr, int *col, double *val, double *rhs) {

@@ -93,6 +93,8 @@ class DarcyGPU {
     void source_balance(double *src) const;                                // per bulk region
     // element pressure [E], piezometric head [E], centre velocity [3E] of the current solution
     void element_output(double *pressure, double *head, double *velocity) const;
+    // pressure interpolated to the nodes [n_nodes] (make_node_scalar_param)
+    void node_pressure(double *out);
     double solution_precision(double rtol, double atol) const;
     int nnz() const { return nnz_; }
     const HostMesh &mesh() const { return m_; }
@@ -133,6 +135,8 @@ class DarcyGPU {
     std::vector<hipEvent_t> ev_pool_;
     std::vector<double> h_bc_measure_;
     std::vector<double> h_cs_, h_f_, h_sigma_;
+    int *d_inc_ptr_ = nullptr, *d_inc_nel_ = nullptr, *d_inc_src_ = nullptr;  // node incidences
+    double *d_inc_dist_ = nullptr;
     // unsteady state
     void set_time_coefficients(double dt);
     int unsteady_ = 0;  // 1 MH, 2 LMH

@@ -160,6 +160,29 @@ __global__ void __launch_bounds__(256) k_elem_output(const DarcyDev dv, double *
     vel[3 * e + 2] = vz;
 }
 
+// DarcyFlowMHOutput::make_node_scalar_param (darcy_flow_mh_output.cc:385-480):
+// the pressure at a node is the average of the element pressures and side traces
+// around it, weighted by (1 - d / sum d) / (count - 1), d the distance from the node
+// to the element or side centre.  One lane per node gathers its incidences, the
+// elements (in element order) then the sides (in side order), so the sums run in
+// the reference's order.  inc_val[k] = d, inc_src[k] = solution index
+// (S + e for an element, S + E + edge for a side); inc_ptr splits each node's list
+// after its elements (nel[node]).
+__global__ void k_node_pressure(int n_nodes, const int *__restrict__ inc_ptr, const int *__restrict__ nel,
+                                const double *__restrict__ inc_dist, const int *__restrict__ inc_src,
+                                const double *__restrict__ sol, double *__restrict__ out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n_nodes; v += gridDim.x * blockDim.x) {
+        const int k0 = inc_ptr[v], k1 = inc_ptr[v + 1], ke = k0 + nel[v];
+        double se = 0.0, ss = 0.0;
+        for (int k = k0; k < ke; k++) se += inc_dist[k];
+        for (int k = ke; k < k1; k++) ss += inc_dist[k];
+        const int cnt = k1 - k0;
+        double r = 0.0;
+        for (int k = k0; k < k1; k++) r += sol[inc_src[k]] * (1 - inc_dist[k] / (se + ss)) / (cnt - 1);
+        out[v] = r;
+    }
+}
+
 // DarcyFlowLMH_Unsteady::postprocess (darcy_flow_mh.cc:1772-1827): every side flux
 // gets the lumped storage term of its edge, then the element's source share
 __global__ void k_lmh_post(const DarcyDev dv, const double *__restrict__ meas, const double *__restrict__ stor,
@@ -874,6 +897,70 @@ void DarcyGPU::element_output(double *pressure, double *head, double *velocity) 
     std::copy(h.begin(), h.begin() + E, pressure);
     std::copy(h.begin() + E, h.begin() + 2 * E, head);
     std::copy(h.begin() + 2 * E, h.end(), velocity);
+}
+
+void DarcyGPU::node_pressure(double *out) {
+    const HostMesh &m = m_;
+    const int N = m.n_nodes(), E = m.n_elements(), S = m.n_sides();
+    if (!d_inc_ptr_) {
+        // node incidences: elements in element order, then sides in side order
+        auto dist = [&](int node, double cx, double cy, double cz) {
+            const double *p = &m.xyz[3 * (size_t)node];
+            return std::sqrt((p[0] - cx) * (p[0] - cx) + (p[1] - cy) * (p[1] - cy) + (p[2] - cz) * (p[2] - cz));
+        };
+        std::vector<std::vector<std::pair<double, int>>> el(N), sd(N);
+        static const int kSN[4][4][3] = {{{0, 0, 0}}, {{0, 0, 0}, {1, 0, 0}}, {{0, 1, 0}, {1, 2, 0}, {0, 2, 0}},
+                                         {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}}};
+        for (int e = 0; e < E; e++) {
+            const int d = m.el_dim[e];
+            double c[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k <= d; k++)
+                for (int j = 0; j < 3; j++) c[j] += m.xyz[3 * (size_t)m.el_nodes[4 * e + k] + j];
+            for (int j = 0; j < 3; j++) c[j] /= (d + 1);
+            for (int k = 0; k <= d; k++) {
+                const int v = m.el_nodes[4 * e + k];
+                el[v].push_back({dist(v, c[0], c[1], c[2]), S + e});
+            }
+        }
+        for (int e = 0; e < E; e++) {
+            const int d = m.el_dim[e];
+            for (int i = 0; i <= d; i++) {
+                double c[3] = {0.0, 0.0, 0.0};
+                for (int k = 0; k < d; k++)
+                    for (int j = 0; j < 3; j++) c[j] += m.xyz[3 * (size_t)m.el_nodes[4 * e + kSN[d][i][k]] + j];
+                for (int j = 0; j < 3; j++) c[j] /= d;
+                const int src = S + E + m.side_edge[m.side_off[e] + i];
+                for (int k = 0; k < d; k++) {
+                    const int v = m.el_nodes[4 * e + kSN[d][i][k]];
+                    sd[v].push_back({dist(v, c[0], c[1], c[2]), src});
+                }
+            }
+        }
+        std::vector<int> ptr(N + 1, 0), nel(N), src;
+        std::vector<double> dd;
+        for (int v = 0; v < N; v++) {
+            nel[v] = (int)el[v].size();
+            for (auto &x : el[v]) dd.push_back(x.first), src.push_back(x.second);
+            for (auto &x : sd[v]) dd.push_back(x.first), src.push_back(x.second);
+            ptr[v + 1] = (int)dd.size();
+        }
+        if (dd.empty()) dd.push_back(0.0), src.push_back(0);
+        d_inc_ptr_ = dev_upload(ptr.data(), ptr.size(), stream_);
+        d_inc_nel_ = dev_upload(nel.data(), nel.size(), stream_);
+        d_inc_dist_ = dev_upload(dd.data(), dd.size(), stream_);
+        d_inc_src_ = dev_upload(src.data(), src.size(), stream_);
+        for (void *q : {(void *)d_inc_ptr_, (void *)d_inc_nel_, (void *)d_inc_dist_, (void *)d_inc_src_})
+            owned_.push_back(q);
+    }
+    double *buf = dev_alloc<double>(N);
+    hipLaunchKernelGGL(k_node_pressure, dim3(std::min(grid_for(N), 4096)), dim3(kBlock), 0, stream_, N,
+                       (const int *)d_inc_ptr_, (const int *)d_inc_nel_, (const double *)d_inc_dist_,
+                       (const int *)d_inc_src_, (const double *)d_sol_, buf);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpyAsync(out, buf, N * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream_);
+    dev_free(buf);
+    if (err != hipSuccess) throw Error(-2, std::string("node pressure: ") + hipGetErrorString(err));
 }
 
 void DarcyGPU::get_solution(double *out) const {

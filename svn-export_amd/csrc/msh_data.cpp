@@ -340,14 +340,15 @@ std::string msh_write_head(const HostMesh &m, const char *path) {
     return ok ? "" : std::string("write error on ") + path;
 }
 
-std::string msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
-                                   int n_comp, int n_values, const double *values, int precision) {
-    if (n_comp < 1 || n_values < 0) return "bad element data shape";
+static std::string write_data_section(const char *section, const char *path, const char *name, const char *units,
+                                      double time, int step, int n_comp, int n_values, const double *values,
+                                      int precision) {
+    if (n_comp < 1 || n_values < 0) return "bad data shape";
     if (precision < 1 || precision > 17) return "precision must be in 1..17";
     FILE *f = std::fopen(path, "a");
     if (!f) return std::string("can not open output file ") + path;
     const int p = precision;
-    std::fprintf(f, "$ElementData\n1\n\"%s_[%s]\"\n1\n%.*g\n3\n%d\n%d\n%d\n", name, units, p, time, step, n_comp,
+    std::fprintf(f, "$%s\n1\n\"%s_[%s]\"\n1\n%.*g\n3\n%d\n%d\n%d\n", section, name, units, p, time, step, n_comp,
                  n_values);
     for (int i = 0; i < n_values; i++) {
         const double *v = values + (size_t)i * n_comp;
@@ -360,10 +361,20 @@ std::string msh_write_element_data(const char *path, const char *name, const cha
             std::fputc('\n', f);
         }
     }
-    std::fprintf(f, "$EndElementData\n");
+    std::fprintf(f, "$End%s\n", section);
     bool ok = !std::ferror(f);
     ok = (std::fclose(f) == 0) && ok;
     return ok ? "" : std::string("write error on ") + path;
+}
+
+std::string msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
+                                   int n_comp, int n_values, const double *values, int precision) {
+    return write_data_section("ElementData", path, name, units, time, step, n_comp, n_values, values, precision);
+}
+
+std::string msh_write_node_data(const char *path, const char *name, const char *units, double time, int step,
+                                int n_comp, int n_values, const double *values, int precision) {
+    return write_data_section("NodeData", path, name, units, time, step, n_comp, n_values, values, precision);
 }
 
 }  // namespace fl

@@ -90,4 +90,9 @@ std::string msh_write_head(const HostMesh &m, const char *path);
 std::string msh_write_element_data(const char *path, const char *name, const char *units, double time, int step,
                                    int n_comp, int n_values, const double *values, int precision);
 
+// OutputMSH::write_msh_node_data (output_msh.cc:327-381): one $NodeData section,
+// rows "id v1 .. vn" with ids 1..n_values, as the element data.
+std::string msh_write_node_data(const char *path, const char *name, const char *units, double time, int step,
+                                int n_comp, int n_values, const double *values, int precision);
+
 }  // namespace fl

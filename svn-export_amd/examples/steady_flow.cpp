@@ -1,8 +1,11 @@
 // Drives the GPU path the way HC_ExplicitSequential::run_simulation
 // (src/coupling/hc_explicit_sequential.cc:151-235) drives the reference:
 // one steady Darcy solve, then the transport step loop to each output time.
-// Usage: steady_flow <mesh.msh>  -- unit conductivity, Dirichlet 1 on the first
-// boundary region and 0 on the second, one substance entering with c = 1.
+// Usage: steady_flow <mesh.msh> [flow_output.msh]  -- unit conductivity, Dirichlet 1
+// on the first boundary region and 0 on the second, one substance entering with
+// c = 1; with a second argument the flow's output fields are written there as
+// DarcyFlowMHOutput::output writes them (node pressure, element pressure,
+// piezometric head, centre velocity).
 #include <cstdio>
 #include <vector>
 
@@ -10,7 +13,7 @@
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s mesh.msh\n", argv[0]);
+        std::fprintf(stderr, "usage: %s mesh.msh [flow_output.msh]\n", argv[0]);
         return 2;
     }
     try {
@@ -53,6 +56,15 @@ int main(int argc, char **argv) {
         const fl_solve_info &i = darcy.last_solve();
         std::printf("darcy: %u unknowns, %d CG iterations, residual %.3e, %.3f ms\n", n, i.iterations, i.residual,
                     i.ms_total);
+        if (argc > 2) {
+            std::vector<double> pn, pe, ph, v;
+            darcy.output_fields(pn, pe, ph, v);
+            fl::OutputMSH out(mesh, argv[2]);
+            out.write_node_data("pressure_nodes", "L", pn.data(), 1, (int)pn.size(), 0.0);
+            out.write_elem_data("pressure_elements", "L", pe.data(), 1, E, 0.0);
+            out.write_elem_data("piezo_head_elements", "L", ph.data(), 1, E, 0.0);
+            out.write_elem_data("velocity_elements", "L/T", v.data(), 3, E, 0.0);
+        }
         std::vector<double> bc(NBC, 1.0), ic(E, 0.0);
         fl::ConvectionTransport tr(mesh, 1, std::vector<double>(E, 1.0), d.cross_section, bc, ic, 1.0, 0.25);
         tr.set_velocity_field(darcy);

@@ -48,6 +48,7 @@ class GpuMesh {
         check(fl_mesh_counts(m_, c.data()), "counts");
         return c;
     }
+    unsigned n_nodes() const { return counts()[0]; }
     unsigned n_elements() const { return counts()[1]; }
     unsigned n_bc_elements() const { return counts()[2]; }
     unsigned n_sides() const { return counts()[3]; }
@@ -102,6 +103,17 @@ class DarcyFlowMH_Steady {
     // DarcyFlowMHOutput::water_balance: one block appended to path
     void write_water_balance(const std::string &path, double time) {
         check(fl_darcy_write_water_balance(d_, path.c_str(), time), "water balance");
+    }
+    // DarcyFlowMHOutput::output fields (darcy_flow_mh_output.cc:261-325, 424-541):
+    // node pressure [n_nodes], element pressure and piezometric head [E], centre velocity [3E]
+    void output_fields(std::vector<double> &pressure_nodes, std::vector<double> &pressure,
+                       std::vector<double> &piezo_head, std::vector<double> &velocity) {
+        pressure_nodes.assign(mesh_.n_nodes(), 0.0);
+        pressure.assign(n_elem_, 0.0);
+        piezo_head.assign(n_elem_, 0.0);
+        velocity.assign(3 * (size_t)n_elem_, 0.0);
+        check(fl_darcy_node_pressure(d_, pressure_nodes.data()), "node pressure");
+        check(fl_darcy_element_output(d_, pressure.data(), piezo_head.data(), velocity.data()), "element output");
     }
 
    protected:
@@ -198,25 +210,35 @@ class OldBcdInput {
     fl_old_bcd *b_ = nullptr;
 };
 
-// OutputMSH: the mesh head once, then $ElementData sections (values at `precision`
-// significant digits; the reference's stream default is 6).
+// OutputMSH: the mesh head once, then $NodeData / $ElementData sections (values at
+// `precision` significant digits; the reference's stream default is 6).
 class OutputMSH {
    public:
     OutputMSH(GpuMesh &mesh, const std::string &path, int precision = 6)
         : mesh_(mesh), path_(path), precision_(precision) {}
     void write_elem_data(const std::string &name, const std::string &units, const double *values, int n_comp,
                          int n_values, double time) {
-        if (!head_) {
-            check(fl_msh_write_head(mesh_.handle(), path_.c_str()), "msh head");
-            head_ = true;
-        }
+        head();
         check(fl_msh_write_element_data(path_.c_str(), name.c_str(), units.c_str(), time, step_, n_comp, n_values,
                                         values, precision_),
               "msh element data");
     }
+    void write_node_data(const std::string &name, const std::string &units, const double *values, int n_comp,
+                         int n_values, double time) {
+        head();
+        check(fl_msh_write_node_data(path_.c_str(), name.c_str(), units.c_str(), time, step_, n_comp, n_values,
+                                     values, precision_),
+              "msh node data");
+    }
     void next_step() { step_++; }
 
    private:
+    void head() {
+        if (!head_) {
+            check(fl_msh_write_head(mesh_.handle(), path_.c_str()), "msh head");
+            head_ = true;
+        }
+    }
     GpuMesh &mesh_;
     std::string path_;
     int precision_, step_ = 0;

@@ -80,6 +80,9 @@ SIGNATURES = {
                                         _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
     "fl_darcy_write_water_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "fl_darcy_element_output": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
+    "fl_darcy_node_pressure": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_msh_write_node_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
+                                         C.c_int, _f64p, C.c_int]),
     "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "fl_darcy_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "fl_darcy_free": (None, [C.c_void_p]),
@@ -375,6 +378,13 @@ class OutputMSH:
         _check(lib().fl_msh_write_element_data(self.path, name.encode(), units.encode(), float(time), int(step),
                                                ncomp, n, v.reshape(-1) if n else np.zeros(1), self.precision))
 
+    def write_node_data(self, name, units, values, time, step):
+        v = np.ascontiguousarray(values, np.float64)
+        n = v.shape[0]
+        ncomp = 1 if v.ndim == 1 else int(np.prod(v.shape[1:]))
+        _check(lib().fl_msh_write_node_data(self.path, name.encode(), units.encode(), float(time), int(step),
+                                            ncomp, n, v.reshape(-1) if n else np.zeros(1), self.precision))
+
     def write_data(self, time, fields):
         """OutputMSH::write_data(time) (output_msh.cc:440-462): `fields` is a list of
         (name, units, values[E] or values[E, n_comp]) in registration order."""
@@ -459,6 +469,12 @@ class DarcyFlowMH_Steady:
         p, h, v = np.zeros(E), np.zeros(E), np.zeros(3 * E)
         _check(lib().fl_darcy_element_output(self.h, p, h, v))
         return p, h, v.reshape(E, 3)
+
+    def node_pressure(self):
+        """Pressure interpolated to the nodes (DarcyFlowMHOutput::make_node_scalar_param)."""
+        out = np.zeros(max(self.mesh.n_nodes, 1))
+        _check(lib().fl_darcy_node_pressure(self.h, out))
+        return out[:self.mesh.n_nodes]
 
     def write_water_balance(self, path, time=float("inf")):
         """DarcyFlowMHOutput::water_balance block for this solution (steady: time inf)."""

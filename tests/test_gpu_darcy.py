@@ -159,15 +159,36 @@ def test_darcy_repeat_and_new_realization(tmp_path):
     assert rel(c, D0.solution) < 1e-9
 
 
-def test_cpp_adapter_example():
-    """The C++ adapter (DarcyFlowMH_Steady / ConvectionTransport method set) over the C ABI."""
+def test_cpp_adapter_example(tmp_path):
+    """The C++ adapter (DarcyFlowMH_Steady / ConvectionTransport method set) over the
+    C ABI; the flow output file it writes (node pressure, element pressure, head,
+    velocity) against the oracle's fields on the same data."""
     import subprocess
     exe = os.path.join(os.path.dirname(G.LIB_PATH), "examples", "steady_flow")
     assert os.path.exists(exe), "build() compiles the example"
-    out = subprocess.run([exe, cases.ref_path("test01_mesh.msh")], capture_output=True, text=True, timeout=120)
+    msh = str(tmp_path / "flow.msh")
+    out = subprocess.run([exe, cases.ref_path("test01_mesh.msh"), msh], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "darcy:" in out.stdout and "transport:" in out.stdout
+    om = O.Mesh(cases.ref_path("test01_mesh.msh"))
+    E, NB = om.n_elements, om.n_bc_elements
+    bnd = [r["id"] for r in om.regions if r["boundary"] and r["id"] != -3][:2]
+    bct = np.isin(om.bc_region, bnd).astype(np.int32)
+    bcp = (om.bc_region == bnd[0]).astype(float)
+    D = O.Darcy(om, np.ones(E), np.ones(E), np.ones(E), np.zeros(E), bct, bcp, np.zeros(NB), np.zeros(NB))
+    D.solve(0, rtol=1e-15)
+    sec = {}
+    for part in open(msh).read().split("Data\n1\n\"")[1:]:
+        L = part.split("\n")
+        n = int(L[6])
+        sec[L[0].split("_[")[0]] = np.array([[float(x) for x in l.split()[1:]] for l in L[7:7 + n]])
+    assert sorted(sec) == ["piezo_head_elements", "pressure_elements", "pressure_nodes", "velocity_elements"]
+    p, h, v = D.element_output()
+    for name, ref in [("pressure_nodes", D.node_pressure()), ("pressure_elements", p), ("piezo_head_elements", h),
+                      ("velocity_elements", v)]:
+        got = sec[name].reshape(ref.shape)
+        assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-9 * np.max(np.abs(ref))), name
 
 
 @pytest.mark.parametrize("case,mesh", REF_CASES)
