@@ -455,6 +455,24 @@ int fl_transport_set_pade_reactions(fl_transport *t, int n_decays, const int *pa
  * [n_subst][E]; the step adds (density + max(conc_src - c, 0) sigma) dt. */
 int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc);
 
+/* Dual porosity and sorption of ConvectionTransport (dual_porosity and
+ * sorption_enable; transport_dual_porosity / transport_sorption / compute_sorption,
+ * transport.cc:1026-1160), applied after every convection step and before the
+ * decays (which then act on the immobile phase too, linear_reaction.cc:295-301).
+ * por_imm[E] and phi[E] (null: 1.0); alpha, sorp_type (0 none, 1 linear,
+ * 2 freundlich, 3 langmuir; reaction/isotherm.hh:71-76), sorp_coef0, sorp_coef1
+ * each [n_subst][E], null where the model reading them is off.  Set before the
+ * first step; the immobile and sorbed phases start at zero.  The region mass of
+ * the balance then sums every phase in use times por_m (transport.cc:1354-1362). */
+int fl_transport_set_dual_porosity_sorption(fl_transport *t, int dual_porosity, int sorption, const double *por_imm,
+                                            const double *alpha, const double *phi, const int *sorp_type,
+                                            const double *sorp_coef0, const double *sorp_coef1);
+
+/* get_concentration_matrix()[phase] (transport.h:58-63, 107-173): phase 0 mobile,
+ * 1 immobile, 2 mobile sorbed, 3 immobile sorbed; out[n_subst][E] (zero for a
+ * phase not in use). */
+int fl_transport_get_conc_phase(fl_transport *t, int phase, double *out);
+
 /* Scaled transport matrix tm = I + dt*A (create_transport_matrix_mpi,
  * transport.cc:788-943, after the scaling at :628-648). */
 int fl_transport_nnz(fl_transport *t);

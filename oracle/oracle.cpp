@@ -1570,6 +1570,12 @@ struct Transport {
     std::vector<int> r_parent, r_pptr, r_prod;
     std::vector<double> r_hl, r_branch, reaction_matrix;
     int pade_nom = -1, pade_den = -1;  // Pade_approximant degrees, -1: Linear_reaction
+    // dual porosity and sorption (transport.cc:1026-1132): immobile, mobile-sorbed and
+    // immobile-sorbed concentrations [nsub][E]; por_imm, phi [E]; alpha, isotherm
+    // type and coefficients [nsub][E]
+    bool dual = false, sorp = false;
+    std::vector<double> c_imm, c_msorb, c_isorb, por_imm, phi, alpha, sorp_c0, sorp_c1;
+    std::vector<int> sorp_type;
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -1699,7 +1705,14 @@ static void mb_calculate(Transport &T, double time) {
         for (int e = 0; e < T.E; e++) {
             const Element &el = m.element[e];
             int r = m.rdb.regs[el.region].sub_idx;
-            T.mass[k * T.nblk + r] += T.por[e] * T.cs[e] * T.conc[(size_t)k * T.E + e] * elem_measure(m, el);
+            // the concentrations of every phase in use, summed, times por_m (transport.cc:1354-1362)
+            size_t j = (size_t)k * T.E + e;
+            double sum_sol_phases = 0;
+            sum_sol_phases += T.conc[j];
+            if (T.dual) sum_sol_phases += T.c_imm[j];
+            if (T.sorp) sum_sol_phases += T.c_msorb[j];
+            if (T.dual && T.sorp) sum_sol_phases += T.c_isorb[j];
+            T.mass[k * T.nblk + r] += T.por[e] * T.cs[e] * sum_sol_phases * elem_measure(m, el);
             if (!T.src_density.empty()) {
                 // compute_concentration_sources_for_mass_balance (transport.cc:546-590) with pconc
                 size_t j = (size_t)k * T.E + e;
@@ -1726,6 +1739,74 @@ static void mb_calculate(Transport &T, double time) {
 }
 
 // compute_one_step: transport.cc:593-715
+// ConvectionTransport::transport_dual_porosity (transport.cc:1026-1057): the mobile and
+// immobile concentrations relax towards their porosity-weighted average with
+// rate alpha (por_m + por_imm) / (por_m por_imm) over the step dt.
+static void dual_porosity(Transport &T, int i, int k, double dt) {
+    size_t j = (size_t)k * T.E + i;
+    double por_m = T.por[i], por_imm = T.por_imm[i], alpha = T.alpha[j];
+    double pcm = T.conc[j], pci = T.c_imm[j];
+    double conc_avg = ((por_m * pcm) + (por_imm * pci)) / (por_m + por_imm);
+    if ((conc_avg != 0.0) && (por_imm != 0.0)) {
+        double f = std::exp(-alpha * ((por_m + por_imm) / (por_m * por_imm)) * dt);
+        T.conc[j] = (pcm - conc_avg) * f + conc_avg;
+        T.c_imm[j] = (pci - conc_avg) * f + conc_avg;
+    }
+}
+
+// ConvectionTransport::compute_sorption (transport.cc:1098-1160): splits the pore
+// mass conc_avg into the dissolved concentration concx (linear, Freundlich by at
+// most 20 Newton steps from the kink cz, Langmuir by at most 5) and the sorbed
+// concentration (conc_avg - concx) Nv.  Isotherm types: 0 none, 1 linear,
+// 2 freundlich, 3 langmuir (reaction/isotherm.hh:71-76).
+static void compute_sorption(double conc_avg, double c0, double c1, int type, double *concx, double *concx_sorb,
+                             double Nv, double N) {
+    double Kx = c0 * N;
+    const double ad = 1e4, tolerence = 1e-8;
+    double pNR = 0, NR, cz, tcz;
+    if (type == 1) {
+        *concx = conc_avg / (1 + Kx);
+    } else if (type == 2) {
+        cz = std::pow(ad / (Kx * c1), 1 / (c1 - 1));
+        tcz = ad / c1;
+        NR = cz;
+        for (int i = 0; i < 20; i++) {
+            NR -= (NR + ((NR > cz) ? Kx * std::pow(NR, c1) : tcz * NR) - conc_avg) /
+                  (1 + ((NR > cz) ? c1 * Kx * std::pow(NR, c1 - 1) : tcz));
+            if ((NR <= cz) || (std::fabs(NR - pNR) < tolerence * NR)) break;
+            pNR = NR;
+        }
+        *concx = NR;
+    } else if (type == 3) {
+        NR = 0;
+        for (int i = 0; i < 5; i++) {
+            NR -= (NR + (N * NR * c1 * c0) / (1 + NR * c0) - conc_avg) /
+                  (1 + N * c0 * c1 / std::pow((1 + NR * c0), 2));
+            if (std::fabs(NR - pNR) < tolerence * NR) break;
+            pNR = NR;
+        }
+        *concx = NR;
+    }
+    *concx_sorb = (conc_avg - *concx) * Nv;
+}
+
+// ConvectionTransport::transport_sorption (transport.cc:1061-1094)
+static void sorption(Transport &T, int i, int k) {
+    size_t j = (size_t)k * T.E + i;
+    double phi = T.phi[i], por_m = T.por[i], por_imm = T.por_imm[i];
+    double n = 1 - (por_m + por_imm), Nm = por_m, Nimm = por_imm;
+    double conc_avg = T.conc[j] + T.c_msorb[j] * n / Nm;
+    if (conc_avg != 0)
+        compute_sorption(conc_avg, T.sorp_c0[j], T.sorp_c1[j], T.sorp_type[j], &T.conc[j], &T.c_msorb[j], Nm / n,
+                         n * phi / Nm);
+    if (T.dual && (por_imm != 0)) {
+        double conc_avg_imm = T.c_imm[j] + T.c_isorb[j] * n / Nimm;
+        if (conc_avg_imm != 0)
+            compute_sorption(conc_avg_imm, T.sorp_c0[j], T.sorp_c1[j], T.sorp_type[j], &T.c_imm[j], &T.c_isorb[j],
+                             Nimm / n, n * (1 - phi) / Nimm);
+    }
+}
+
 static void one_step(Transport &T) {
     // data_.set_time at the step's start: boundary data given since the last step apply now
     bool bc_changed = false;
@@ -1784,6 +1865,12 @@ static void one_step(Transport &T) {
             for (int p = T.tm.ptr[i]; p < T.tm.ptr[i + 1]; p++) s += T.tm.val[p] * cp[T.tm.col[p]];
             c[i] = s;
         }
+        // transport.cc:700-709: dual porosity then sorption, element by element
+        if (T.dual || T.sorp)
+            for (int i = 0; i < T.E; i++) {
+                if (T.dual) dual_porosity(T, i, k, dt);
+                if (T.sorp) sorption(T, i, k);
+            }
     }
     T.steps++;
 }
@@ -1862,24 +1949,29 @@ static void set_pade_time_step(Transport &T, double time_step) {
 
 // Linear_reaction::compute_one_step / compute_reaction (linear_reaction.cc:167-183, 287-300);
 // Pade_approximant::compute_reaction (pade_approximant.cc:274-290) multiplies the other way round
-static void compute_reactions(Transport &T) {
+// With dual porosity the immobile phase reacts too (linear_reaction.cc:298-300).
+static void react_phase(Transport &T, std::vector<double> &conc) {
     int n = T.nsub;
     std::vector<double> prev_conc(n);
     for (int e = 0; e < T.E; e++) {
         for (int cols = 0; cols < n; cols++) {
-            prev_conc[cols] = T.conc[(size_t)cols * T.E + e];
-            T.conc[(size_t)cols * T.E + e] = 0.0;
+            prev_conc[cols] = conc[(size_t)cols * T.E + e];
+            conc[(size_t)cols * T.E + e] = 0.0;
         }
         if (T.pade_nom >= 0) {
             for (int cols = 0; cols < n; cols++)
                 for (int rows = 0; rows < n; rows++)
-                    T.conc[(size_t)cols * T.E + e] += T.reaction_matrix[(size_t)cols * n + rows] * prev_conc[rows];
+                    conc[(size_t)cols * T.E + e] += T.reaction_matrix[(size_t)cols * n + rows] * prev_conc[rows];
         } else {
             for (int rows = 0; rows < n; rows++)
                 for (int cols = 0; cols < n; cols++)
-                    T.conc[(size_t)rows * T.E + e] += prev_conc[cols] * T.reaction_matrix[(size_t)cols * n + rows];
+                    conc[(size_t)rows * T.E + e] += prev_conc[cols] * T.reaction_matrix[(size_t)cols * n + rows];
         }
     }
+}
+static void compute_reactions(Transport &T) {
+    react_phase(T, T.conc);
+    if (T.dual) react_phase(T, T.c_imm);
 }
 
 static void set_target(Transport &T, double t) {
@@ -1962,6 +2054,41 @@ void orc_transport_set_bc_conc(void *h, const double *bc_conc) {
     Transport &T = *(Transport *)h;
     T.bc_pending.assign(bc_conc, bc_conc + T.bc_conc.size());
     T.bc_changed = true;
+}
+
+// Dual porosity and sorption (ConvectionTransport's dual_porosity / sorption_enable):
+// por_imm[E], phi[E]; alpha, sorp_type, sorp_coef0, sorp_coef1 [nsub][E] (null when
+// the model that reads them is off).  The other phases start at zero.
+void orc_transport_set_phases(void *h, int dual, int sorp, const double *por_imm, const double *alpha,
+                              const double *phi, const int *sorp_type, const double *sorp_coef0,
+                              const double *sorp_coef1) {
+    Transport &T = *(Transport *)h;
+    size_t n = (size_t)T.nsub * T.E;
+    T.dual = dual != 0;
+    T.sorp = sorp != 0;
+    T.por_imm.assign(por_imm, por_imm + T.E);
+    T.alpha.assign(n, 0.0);
+    if (alpha) T.alpha.assign(alpha, alpha + n);
+    T.phi.assign(T.E, 1.0);
+    if (phi) T.phi.assign(phi, phi + T.E);
+    T.sorp_type.assign(n, 0);
+    T.sorp_c0.assign(n, 0.0);
+    T.sorp_c1.assign(n, 0.0);
+    if (sorp_type) T.sorp_type.assign(sorp_type, sorp_type + n);
+    if (sorp_coef0) T.sorp_c0.assign(sorp_coef0, sorp_coef0 + n);
+    if (sorp_coef1) T.sorp_c1.assign(sorp_coef1, sorp_coef1 + n);
+    T.c_imm.assign(n, 0.0);
+    T.c_msorb.assign(n, 0.0);
+    T.c_isorb.assign(n, 0.0);
+}
+// Concentrations of one phase (0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile
+// sorbed; transport.h:58-63) [nsub][E].
+void orc_transport_conc_phase(void *h, int phase, double *out) {
+    Transport &T = *(Transport *)h;
+    const std::vector<double> *v[4] = {&T.conc, &T.c_imm, &T.c_msorb, &T.c_isorb};
+    const std::vector<double> &c = *v[phase];
+    if (c.empty()) std::memset(out, 0, (size_t)T.nsub * T.E * sizeof(double));
+    else std::memcpy(out, c.data(), c.size() * sizeof(double));
 }
 
 void orc_transport_set_sources(void *h, const double *density, const double *sigma, const double *conc) {

@@ -66,6 +66,9 @@ def lib():
         L.orc_transport_set_bc_conc.argtypes = [C.c_void_p, _f64p]
         L.orc_darcy_element_output.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_darcy_node_pressure.argtypes = [C.c_void_p, _f64p]
+        L.orc_transport_set_phases.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_transport_conc_phase.argtypes = [C.c_void_p, C.c_int, _f64p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
@@ -298,6 +301,26 @@ class Transport:
                        np.ascontiguousarray(half_life, np.float64),
                        np.ascontiguousarray([b for p in branch for b in p], np.float64)]
         lib().orc_transport_set_reactions(self.h, len(parent), *self._react)
+
+    def set_phases(self, dual, sorption, por_imm, alpha=None, phi=None, sorp_type=None, sorp_coef0=None,
+                   sorp_coef1=None):
+        """Dual porosity / sorption: por_imm, phi [E]; alpha, sorp_type (0 none,
+        1 linear, 2 freundlich, 3 langmuir), sorp_coef0/1 [nsub][E]."""
+        E = self.mesh.n_elements
+
+        def arr(a, dt, shape):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dt).reshape(-1)
+
+        self._phases = [arr(por_imm, np.float64, (E,)), arr(alpha, np.float64, (self.nsub, E)),
+                        arr(phi, np.float64, (E,)), arr(sorp_type, np.int32, (self.nsub, E)),
+                        arr(sorp_coef0, np.float64, (self.nsub, E)), arr(sorp_coef1, np.float64, (self.nsub, E))]
+        ptr = [None if a is None else a.ctypes.data for a in self._phases[1:]]
+        lib().orc_transport_set_phases(self.h, int(dual), int(sorption), self._phases[0], *ptr)
+
+    def conc_phase(self, phase):
+        out = np.zeros(self.nsub * self.mesh.n_elements)
+        lib().orc_transport_conc_phase(self.h, int(phase), out)
+        return out.reshape(self.nsub, -1)
 
     def set_pade(self, nom_deg, den_deg):
         """Use the Pade approximant of the given degrees for the decays already set."""

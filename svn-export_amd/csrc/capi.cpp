@@ -829,6 +829,22 @@ int fl_transport_set_sources(fl_transport *t, const double *density, const doubl
     });
 }
 
+int fl_transport_set_dual_porosity_sorption(fl_transport *t, int dual_porosity, int sorption, const double *por_imm,
+                                            const double *alpha, const double *phi, const int *sorp_type,
+                                            const double *sorp_coef0, const double *sorp_coef1) {
+    return guard([&] {
+        t->g->set_phases(dual_porosity != 0, sorption != 0, por_imm, alpha, phi, sorp_type, sorp_coef0, sorp_coef1);
+        return 0;
+    });
+}
+
+int fl_transport_get_conc_phase(fl_transport *t, int phase, double *out) {
+    return guard([&] {
+        t->g->get_conc_phase(phase, out);
+        return 0;
+    });
+}
+
 int fl_transport_nnz(fl_transport *t) { return t->g->nnz(); }
 
 int fl_transport_matrix(fl_transport *t, int *ptr, int *col, double *val) {

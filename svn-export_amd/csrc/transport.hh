@@ -88,6 +88,15 @@ class TransportGPU {
     // (nom_deg, den_deg) Pade approximant of exp(A dt)
     void set_pade_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
                             const double *half_life, const double *branch, int nom_deg, int den_deg);
+    // Dual porosity and sorption (dual_porosity / sorption_enable, transport.cc:1026-1160):
+    // por_imm, phi [E]; alpha, sorp_type (0 none, 1 linear, 2 freundlich, 3 langmuir),
+    // sorp_coef0, sorp_coef1 [nsub][E] (null where the model that reads them is off).
+    // Applied after every convection step, before the decays, which then act on the
+    // immobile phase too.  The immobile and sorbed phases start at zero.
+    void set_phases(bool dual, bool sorption, const double *por_imm, const double *alpha, const double *phi,
+                    const int *sorp_type, const double *sorp_coef0, const double *sorp_coef1);
+    // phase 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed (transport.h:58-63) [nsub][E]
+    void get_conc_phase(int phase, double *out) const;
     void set_velocity_host(const double *side_flux);      // [S]
     void set_velocity_device(const double *d_side_flux);  // device [S] (e.g. the Darcy solution)
     double prepare();                                      // build the matrix now; returns the CFL step
@@ -118,7 +127,16 @@ class TransportGPU {
     double *d_bc_conc_ = nullptr;
     std::vector<double> h_bc_next_;
     bool bc_pending_ = false;
-    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_; }
+    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_ && !phases_on(); }
+    bool phases_on() const { return dual_ || sorp_; }
+    bool dual_ = false, sorp_ = false;
+    // per element [E][m] interleaved like the concentrations: immobile, mobile sorbed,
+    // immobile sorbed concentrations; alpha, isotherm type and coefficients; per element
+    // por_imm and phi
+    double *d_imm_ = nullptr, *d_msorb_ = nullptr, *d_isorb_ = nullptr;
+    const double *d_alpha_ = nullptr, *d_sc0_ = nullptr, *d_sc1_ = nullptr, *d_por_imm_ = nullptr, *d_phi_ = nullptr;
+    const int *d_stype_ = nullptr;
+    std::vector<double> h_por_imm_;
     void build_reaction_matrix(double dt);
     void define_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
                           const double *half_life, const double *branch);

@@ -301,6 +301,107 @@ __global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__res
             out_buf[(size_t)blockIdx.x * 2 * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
 }
 
+// ConvectionTransport::compute_sorption (transport.cc:1098-1160): the pore mass
+// conc_avg split into dissolved concx and sorbed (conc_avg - concx) Nv; linear,
+// Freundlich (at most 20 Newton steps from the kink cz) or Langmuir (at most 5).
+__device__ inline void d_sorption(double conc_avg, double c0, double c1, int type, double &concx,
+                                  double &concx_sorb, double Nv, double N) {
+    const double Kx = c0 * N;
+    const double ad = 1e4, tolerence = 1e-8;
+    double pNR = 0, NR;
+    if (type == 1) {
+        concx = conc_avg / (1 + Kx);
+    } else if (type == 2) {
+        const double cz = pow(ad / (Kx * c1), 1 / (c1 - 1)), tcz = ad / c1;
+        NR = cz;
+        for (int i = 0; i < 20; i++) {
+            NR -= (NR + ((NR > cz) ? Kx * pow(NR, c1) : tcz * NR) - conc_avg) /
+                  (1 + ((NR > cz) ? c1 * Kx * pow(NR, c1 - 1) : tcz));
+            if ((NR <= cz) || (fabs(NR - pNR) < tolerence * NR)) break;
+            pNR = NR;
+        }
+        concx = NR;
+    } else if (type == 3) {
+        NR = 0;
+        for (int i = 0; i < 5; i++) {
+            NR -= (NR + (N * NR * c1 * c0) / (1 + NR * c0) - conc_avg) / (1 + N * c0 * c1 / pow((1 + NR * c0), 2));
+            if (fabs(NR - pNR) < tolerence * NR) break;
+            pNR = NR;
+        }
+        concx = NR;
+    }
+    concx_sorb = (conc_avg - concx) * Nv;
+}
+
+struct PhaseDev {
+    double *imm, *msorb, *isorb;
+    const double *alpha, *c0, *c1, *por_imm, *phi;
+    const int *type;
+    int dual, sorp;
+};
+
+// After the convection step, per element and substance in the reference's order
+// (transport.cc:700-709): transport_dual_porosity (transport.cc:1026-1057), then
+// transport_sorption (transport.cc:1061-1094); then the decays on the mobile and,
+// with dual porosity, the immobile phase (linear_reaction.cc:295-301).  Element-
+// wise and HBM-bound: every array is [E][m] so a wave reads whole rows.
+template <int M>
+__global__ void __launch_bounds__(256) k_phases(const TmDev t, const PhaseDev ph, double *__restrict__ c, double dt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
+        const double por_m = t.por[i], por_imm = ph.por_imm[i];
+        double cm[M], ci[M];
+        for (int s = 0; s < M; s++) {
+            const size_t j = (size_t)i * M + s;
+            cm[s] = c[j];
+            ci[s] = ph.dual ? ph.imm[j] : 0.0;
+            if (ph.dual) {
+                const double pcm = cm[s], pci = ci[s];
+                const double conc_avg = ((por_m * pcm) + (por_imm * pci)) / (por_m + por_imm);
+                if ((conc_avg != 0.0) && (por_imm != 0.0)) {
+                    const double f = exp(-ph.alpha[j] * ((por_m + por_imm) / (por_m * por_imm)) * dt);
+                    cm[s] = (pcm - conc_avg) * f + conc_avg;
+                    ci[s] = (pci - conc_avg) * f + conc_avg;
+                }
+            }
+            if (ph.sorp) {
+                const double phi = ph.phi[i];
+                const double n = 1 - (por_m + por_imm), Nm = por_m, Nimm = por_imm;
+                double ms = ph.msorb[j];
+                const double conc_avg = cm[s] + ms * n / Nm;
+                if (conc_avg != 0) d_sorption(conc_avg, ph.c0[j], ph.c1[j], ph.type[j], cm[s], ms, Nm / n, n * phi / Nm);
+                ph.msorb[j] = ms;
+                if (ph.dual && (por_imm != 0)) {
+                    double is = ph.isorb[j];
+                    const double conc_avg_imm = ci[s] + is * n / Nimm;
+                    if (conc_avg_imm != 0)
+                        d_sorption(conc_avg_imm, ph.c0[j], ph.c1[j], ph.type[j], ci[s], is, Nimm / n,
+                                   n * (1 - phi) / Nimm);
+                    ph.isorb[j] = is;
+                }
+            }
+        }
+        if (t.has_react) {
+            double r[M];
+            for (int q = 0; q < M; q++) {
+                r[q] = 0.0;
+                for (int s = 0; s < M; s++) r[q] += cm[s] * t.react[s * M + q];
+            }
+            for (int q = 0; q < M; q++) cm[q] = r[q];
+            if (ph.dual) {
+                for (int q = 0; q < M; q++) {
+                    r[q] = 0.0;
+                    for (int s = 0; s < M; s++) r[q] += ci[s] * t.react[s * M + q];
+                }
+                for (int q = 0; q < M; q++) ci[q] = r[q];
+            }
+        }
+        for (int s = 0; s < M; s++) {
+            c[(size_t)i * M + s] = cm[s];
+            if (ph.dual) ph.imm[(size_t)i * M + s] = ci[s];
+        }
+    }
+}
+
 // The same step with one lane per (row, substance): four lanes per row, 64
 // rows per block, so the concentration loads and stores of a wave are 16
 // whole rows (contiguous); the matrix comes from its column-major copy.  Per
@@ -884,12 +985,26 @@ void TransportGPU::one_step() {
         }
 #undef FL_STEPQ
     } else {
+        // with dual porosity / sorption the decays run after them, in k_phases
+        TmDev ts = t_;
+        if (phases_on()) ts.has_react = 0;
         switch (nsub_) {
 #define FL_STEP(MM) \
-    case MM: hipLaunchKernelGGL(k_step<MM>, dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, fold, ftd, G, out); break;
+    case MM: hipLaunchKernelGGL(k_step<MM>, dim3(G), dim3(kBlock), 0, stream_, ts, cold, cnew, dt, fold, ftd, G, out); break;
             FL_STEP(1) FL_STEP(2) FL_STEP(3) FL_STEP(4) FL_STEP(5) FL_STEP(6) FL_STEP(7) FL_STEP(8)
 #undef FL_STEP
             default: throw Error(-5, "transport supports 1..8 substances per handle");
+        }
+        if (phases_on()) {
+            FL_HIP(hipGetLastError());
+            PhaseDev ph{d_imm_, d_msorb_, d_isorb_, d_alpha_, d_sc0_, d_sc1_, d_por_imm_, d_phi_, d_stype_,
+                        dual_ ? 1 : 0, sorp_ ? 1 : 0};
+            switch (nsub_) {
+#define FL_PH(MM) \
+    case MM: hipLaunchKernelGGL(k_phases<MM>, dim3(gridS), dim3(kBlock), 0, stream_, t_, ph, cnew, dt); break;
+                FL_PH(1) FL_PH(2) FL_PH(3) FL_PH(4) FL_PH(5) FL_PH(6) FL_PH(7) FL_PH(8)
+#undef FL_PH
+            }
         }
     }
     FL_HIP(hipGetLastError());
@@ -936,6 +1051,62 @@ int TransportGPU::update_solution(double target) {
     return steps;
 }
 
+void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, const double *alpha,
+                              const double *phi, const int *sorp_type, const double *sorp_coef0,
+                              const double *sorp_coef1) {
+    if (stats_.steps > 0) throw Error(-3, "dual porosity / sorption must be set before the first step");
+    if (!por_imm) throw Error(-3, "por_imm is required");
+    if (dual && !alpha) throw Error(-3, "dual porosity needs alpha");
+    if (sorption && (!sorp_type || !sorp_coef0 || !sorp_coef1)) throw Error(-3, "sorption needs its isotherms");
+    const size_t n = (size_t)E_ * nsub_;
+    auto inter = [&](const double *a, double dflt) {
+        std::vector<double> v(n, dflt);
+        if (a)
+            for (int k = 0; k < nsub_; k++)
+                for (int i = 0; i < E_; i++) v[(size_t)i * nsub_ + k] = a[(size_t)k * E_ + i];
+        return v;
+    };
+    std::vector<int> ty(n, 0);
+    if (sorp_type)
+        for (int k = 0; k < nsub_; k++)
+            for (int i = 0; i < E_; i++) {
+                const int v = sorp_type[(size_t)k * E_ + i];
+                if (v < 0 || v > 3) throw Error(-3, "sorption type must be 0..3 (none, linear, freundlich, langmuir)");
+                ty[(size_t)i * nsub_ + k] = v;
+            }
+    h_por_imm_.assign(por_imm, por_imm + E_);
+    d_por_imm_ = upv(owned_, h_por_imm_, stream_);
+    std::vector<double> ph(E_, 1.0);
+    if (phi) ph.assign(phi, phi + E_);
+    d_phi_ = upv(owned_, ph, stream_);
+    d_alpha_ = upv(owned_, inter(alpha, 0.0), stream_);
+    d_sc0_ = upv(owned_, inter(sorp_coef0, 0.0), stream_);
+    d_sc1_ = upv(owned_, inter(sorp_coef1, 0.0), stream_);
+    d_stype_ = upv(owned_, ty, stream_);
+    std::vector<double> zero(n, 0.0);
+    d_imm_ = upv(owned_, zero, stream_);
+    d_msorb_ = upv(owned_, zero, stream_);
+    d_isorb_ = upv(owned_, zero, stream_);
+    dual_ = dual;
+    sorp_ = sorption;
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void TransportGPU::get_conc_phase(int phase, double *out) const {
+    if (phase < 0 || phase > 3) throw Error(-3, "phase must be 0..3");
+    const double *src = phase == 0 ? (cur_is_c0_ ? d_c0_ : d_c1_) : phase == 1 ? d_imm_ : phase == 2 ? d_msorb_ : d_isorb_;
+    const size_t n = (size_t)E_ * nsub_;
+    if (!src) {
+        std::fill(out, out + n, 0.0);
+        return;
+    }
+    std::vector<double> c(n);
+    FL_HIP(hipMemcpyAsync(c.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    for (int k = 0; k < nsub_; k++)
+        for (int i = 0; i < E_; i++) out[(size_t)k * E_ + i] = c[(size_t)i * nsub_ + k];
+}
+
 void TransportGPU::get_conc(double *out) const {
     std::vector<double> c((size_t)E_ * nsub_);
     FL_HIP(hipMemcpyAsync(c.data(), cur_is_c0_ ? d_c0_ : d_c1_, c.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -966,6 +1137,20 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
     FL_HIP(hipMemcpyAsync(mb.data(), t_.mb, mb.size() * 8, hipMemcpyDeviceToHost, stream_));
     std::vector<double> flux(m.n_sides());
     FL_HIP(hipMemcpyAsync(flux.data(), d_flux_, flux.size() * 8, hipMemcpyDeviceToHost, stream_));
+    // the mass counts every phase in use (transport.cc:1354-1362)
+    std::vector<double> imm, msorb, isorb;
+    if (dual_) {
+        imm.resize(cur.size());
+        FL_HIP(hipMemcpyAsync(imm.data(), d_imm_, imm.size() * 8, hipMemcpyDeviceToHost, stream_));
+    }
+    if (sorp_) {
+        msorb.resize(cur.size());
+        FL_HIP(hipMemcpyAsync(msorb.data(), d_msorb_, msorb.size() * 8, hipMemcpyDeviceToHost, stream_));
+    }
+    if (dual_ && sorp_) {
+        isorb.resize(cur.size());
+        FL_HIP(hipMemcpyAsync(isorb.data(), d_isorb_, isorb.size() * 8, hipMemcpyDeviceToHost, stream_));
+    }
     FL_HIP(hipStreamSynchronize(stream_));
     if (stats_.steps == 0) std::fill(prev.begin(), prev.end(), 0.0);
     std::fill(bal, bal + ns * nbr, 0.0);
@@ -989,7 +1174,13 @@ void TransportGPU::mass_balance(double *bal, double *plus, double *minus, double
     for (int k = 0; k < ns; k++)
         for (int i = 0; i < E_; i++) {
             int r = h_el_region_sub_[i];
-            mass[k * nblk + r] += h_por_[i] * h_cs_[i] * cur[(size_t)i * ns + k] * h_meas_[i];
+            const size_t j = (size_t)i * ns + k;
+            double sum_sol_phases = 0;
+            sum_sol_phases += cur[j];
+            if (dual_) sum_sol_phases += imm[j];
+            if (sorp_) sum_sol_phases += msorb[j];
+            if (dual_ && sorp_) sum_sol_phases += isorb[j];
+            mass[k * nblk + r] += h_por_[i] * h_cs_[i] * sum_sol_phases * h_meas_[i];
             if (t_.has_src) {  // before the first step the previous concentration is zero (transport.cc:433-435)
                 size_t j = (size_t)k * E_ + i;
                 double diff = h_src_conc_[j] - prev[(size_t)i * ns + k];

@@ -282,6 +282,23 @@ class ConvectionTransport {
                      const std::vector<double> &conc) {
         check(fl_transport_set_sources(t_, density.data(), sigma.data(), conc.data()), "sources");
     }
+    // dual_porosity / sorption_enable (transport.cc:1026-1160): por_imm, phi [E];
+    // alpha, sorp_type, sorp_coef0, sorp_coef1 [n_subst][E] (empty where unused)
+    void set_dual_porosity_sorption(bool dual_porosity, bool sorption, const std::vector<double> &por_imm,
+                                    const std::vector<double> &alpha, const std::vector<double> &phi,
+                                    const std::vector<int> &sorp_type, const std::vector<double> &sorp_coef0,
+                                    const std::vector<double> &sorp_coef1) {
+        auto p = [](const auto &v) { return v.empty() ? nullptr : v.data(); };
+        check(fl_transport_set_dual_porosity_sorption(t_, dual_porosity, sorption, por_imm.data(), p(alpha), p(phi),
+                                                      p(sorp_type), p(sorp_coef0), p(sorp_coef1)),
+              "dual porosity / sorption");
+    }
+    // get_concentration_matrix()[phase]: 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed
+    std::vector<double> concentrations(int phase) const {
+        std::vector<double> c((size_t)n_subst_ * n_elem_);
+        check(fl_transport_get_conc_phase(t_, phase, c.data()), "conc phase");
+        return c;
+    }
     // Linear_reaction / Pade_approximant decays: parent[i], products in CSR form
     void set_reactions(const std::vector<int> &parent, const std::vector<int> &product_ptr,
                        const std::vector<int> &products, const std::vector<double> &half_life,

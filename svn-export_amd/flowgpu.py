@@ -81,6 +81,9 @@ SIGNATURES = {
     "fl_darcy_write_water_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "fl_darcy_element_output": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p]),
     "fl_darcy_node_pressure": (C.c_int, [C.c_void_p, _f64p]),
+    "fl_transport_set_dual_porosity_sorption": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_void_p,
+                                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fl_transport_get_conc_phase": (C.c_int, [C.c_void_p, C.c_int, _f64p]),
     "fl_msh_write_node_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
                                          C.c_int, _f64p, C.c_int]),
     "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
@@ -742,6 +745,29 @@ class ConvectionTransport:
         n = self.nsub * self.mesh.n_elements
         self._src = [_f64(density, n), _f64(sigma, n), _f64(conc, n)]
         _check(lib().fl_transport_set_sources(self.h, *self._src))
+
+    def set_dual_porosity_sorption(self, dual_porosity, sorption, por_imm, alpha=None, phi=None, sorp_type=None,
+                                   sorp_coef0=None, sorp_coef1=None):
+        """dual_porosity / sorption_enable (transport.cc:1026-1160): por_imm, phi [E];
+        alpha, sorp_type (0 none, 1 linear, 2 freundlich, 3 langmuir), sorp_coef0/1
+        [n_subst][E]; scalars and per-substance columns broadcast."""
+        E, ns = self.mesh.n_elements, self.nsub
+
+        def arr(a, dt, shape):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dt).reshape(-1)
+
+        self._phases = [arr(por_imm, np.float64, (E,)), arr(alpha, np.float64, (ns, E)), arr(phi, np.float64, (E,)),
+                        arr(sorp_type, np.int32, (ns, E)), arr(sorp_coef0, np.float64, (ns, E)),
+                        arr(sorp_coef1, np.float64, (ns, E))]
+        ptr = [None if a is None else a.ctypes.data for a in self._phases[1:]]
+        _check(lib().fl_transport_set_dual_porosity_sorption(self.h, int(bool(dual_porosity)), int(bool(sorption)),
+                                                             self._phases[0], *ptr))
+
+    def conc_phase(self, phase):
+        """get_concentration_matrix()[phase]: 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed."""
+        out = np.zeros(self.nsub * self.mesh.n_elements)
+        _check(lib().fl_transport_get_conc_phase(self.h, int(phase), out))
+        return out.reshape(self.nsub, -1)
 
     def stats(self):
         ms, st = C.c_double(), C.c_int()

@@ -71,6 +71,26 @@ def darcy_case(name, om, seed=0):
                 bcp[b] = cen[b, 0] - cen[b, 2]
             elif blab[b] == ".channel":
                 bct[b] = 2
+    elif name.startswith("test05_"):
+        # tests/05_flow_boundary_conditions/flow_{dirichlet,neumann,robin}.con: exact
+        # solution u = xy; formula data evaluated at the boundary element centre
+        # (darcy_flow_mh.cc:577-589)
+        variant = name[len("test05_"):]
+        for b in range(NB):
+            x, y = cen[b, 0], cen[b, 1]
+            if blab[b] == ".bc_east":
+                bct[b], bcp[b] = 1, y
+            elif blab[b] == ".bc_west":
+                bct[b], bcp[b] = 1, 0.0
+            elif variant == "dirichlet":
+                bct[b], bcp[b] = 1, (0.0 if blab[b] == ".bc_south" else x)
+            elif variant == "neumann":
+                bct[b], bcf[b] = 2, (x if blab[b] == ".bc_south" else -x)
+            elif variant == "robin":
+                bct[b], bcs[b] = 3, 0.5
+                bcp[b] = -2.0 * x if blab[b] == ".bc_south" else 3.0 * x
+            else:
+                raise KeyError(name)
     elif name == "test13":
         # tests/13_edge_solute_mixing: anisotropy per region, old .fbc Dirichlet data
         an = {17: 1.0, 18: 0.1, 19: 1.0}
