@@ -455,6 +455,14 @@ int fl_transport_set_pade_reactions(fl_transport *t, int n_decays, const int *pa
  * [n_subst][E]; the step adds (density + max(conc_src - c, 0) sigma) dt. */
 int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc);
 
+/* Boundary concentrations given as a function of time (a time-dependent
+ * FieldFormula for bc_conc, field_formula_impl.hh:70-122, read by
+ * ConvectionTransport::compute_one_step at every step's start, transport.cc:598-627):
+ * fn(t, bc_conc[n_subst][NBC], user) is called on the host at the start time t of
+ * every step and its data drive that step.  fn null: the last data stay fixed. */
+typedef void (*fl_bc_conc_fn)(double time, double *bc_conc, void *user);
+int fl_transport_set_bc_conc_function(fl_transport *t, fl_bc_conc_fn fn, void *user);
+
 /* Dual porosity and sorption of ConvectionTransport (dual_porosity and
  * sorption_enable; transport_dual_porosity / transport_sorption / compute_sorption,
  * transport.cc:1026-1160), applied after every convection step and before the

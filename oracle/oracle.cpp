@@ -1563,6 +1563,10 @@ struct Transport {
     std::vector<double> bc_conc;    // [nsub][NBC]
     std::vector<double> bc_pending; // new boundary data, read by the next step
     bool bc_changed = false;
+    // boundary data as a function of time (a time-dependent FieldFormula): evaluated
+    // at every step's start time
+    void (*bc_fn)(double, double *, void *) = nullptr;
+    void *bc_user = nullptr;
     std::vector<double> conc, pconc;  // [nsub][E]
     std::vector<double> bcvcorr;    // [nsub][E]
     std::vector<double> src_density, src_sigma, src_conc;  // [nsub][E], empty = no sources
@@ -1808,8 +1812,14 @@ static void sorption(Transport &T, int i, int k) {
 }
 
 static void one_step(Transport &T) {
-    // data_.set_time at the step's start: boundary data given since the last step apply now
+    // data_.set_time at the step's start: boundary data given since the last step apply now;
+    // a time-dependent formula changes at every new time (field_formula_impl.hh:70-122)
     bool bc_changed = false;
+    if (T.bc_fn) {
+        T.bc_pending.assign(T.bc_conc.size(), 0.0);
+        T.bc_fn(T.tg.time, T.bc_pending.data(), T.bc_user);
+        T.bc_changed = true;
+    }
     if (T.bc_changed) {
         T.bc_conc = T.bc_pending;
         T.bc_changed = false;
@@ -2089,6 +2099,14 @@ void orc_transport_conc_phase(void *h, int phase, double *out) {
     const std::vector<double> &c = *v[phase];
     if (c.empty()) std::memset(out, 0, (size_t)T.nsub * T.E * sizeof(double));
     else std::memcpy(out, c.data(), c.size() * sizeof(double));
+}
+
+// Boundary concentrations as a function of time: fn(t, bc_conc[nsub][NBC], user) at the
+// start of every step (fn null: back to fixed data).
+void orc_transport_set_bc_function(void *h, void (*fn)(double, double *, void *), void *user) {
+    Transport &T = *(Transport *)h;
+    T.bc_fn = fn;
+    T.bc_user = user;
 }
 
 void orc_transport_set_sources(void *h, const double *density, const double *sigma, const double *conc) {

@@ -69,6 +69,7 @@ def lib():
         L.orc_transport_set_phases.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_conc_phase.argtypes = [C.c_void_p, C.c_int, _f64p]
+        L.orc_transport_set_bc_function.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
@@ -232,6 +233,9 @@ class Darcy:
             self.h = None
 
 
+BC_FN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
 class Transport:
     """ConvectionTransport + TransportOperatorSplitting loop restated (transport.cc)."""
 
@@ -321,6 +325,16 @@ class Transport:
         out = np.zeros(self.nsub * self.mesh.n_elements)
         lib().orc_transport_conc_phase(self.h, int(phase), out)
         return out.reshape(self.nsub, -1)
+
+    def set_bc_function(self, fn):
+        """Boundary concentrations fn(t) -> [nsub][NBC], evaluated at every step's start."""
+        shape = (self.nsub, self.mesh.n_bc_elements)
+
+        def cb(t, out, user):
+            np.ctypeslib.as_array(out, shape=shape)[:] = fn(t)
+
+        self._bc_fn = BC_FN(cb)
+        lib().orc_transport_set_bc_function(self.h, C.cast(self._bc_fn, C.c_void_p), None)
 
     def set_pade(self, nom_deg, den_deg):
         """Use the Pade approximant of the given degrees for the decays already set."""

@@ -838,6 +838,13 @@ int fl_transport_set_dual_porosity_sorption(fl_transport *t, int dual_porosity, 
     });
 }
 
+int fl_transport_set_bc_conc_function(fl_transport *t, fl_bc_conc_fn fn, void *user) {
+    return guard([&] {
+        t->g->set_bc_function(fn, user);
+        return 0;
+    });
+}
+
 int fl_transport_get_conc_phase(fl_transport *t, int phase, double *out) {
     return guard([&] {
         t->g->get_conc_phase(phase, out);

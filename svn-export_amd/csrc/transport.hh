@@ -78,6 +78,11 @@ class TransportGPU {
     // new boundary concentrations [nsub][NBC], read by the next step (a time-dependent
     // bc_conc field whose next data time has been reached)
     void set_bc_conc(const double *bc_conc);
+    // boundary concentrations as a function of time (a time-dependent FieldFormula,
+    // field_formula_impl.hh:70-122): fn(t, bc_conc[nsub][NBC], user) at every step's
+    // start time, the data then used for that step; fn null: fixed data again
+    using BcFn = void (*)(double, double *, void *);
+    void set_bc_function(BcFn fn, void *user) { bc_fn_ = fn; bc_user_ = user; }
     // Linear_reaction (linear_reaction.cc:144-220): decays i with parent[i],
     // products[pptr[i]..pptr[i+1]), half_life[i] and branch ratios; the matrix is
     // rebuilt from the step estimate at every update_solution and applied after
@@ -127,6 +132,8 @@ class TransportGPU {
     double *d_bc_conc_ = nullptr;
     std::vector<double> h_bc_next_;
     bool bc_pending_ = false;
+    BcFn bc_fn_ = nullptr;
+    void *bc_user_ = nullptr;
     bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_ && !phases_on(); }
     bool phases_on() const { return dual_ || sorp_; }
     bool dual_ = false, sorp_ = false;

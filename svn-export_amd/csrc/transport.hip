@@ -931,9 +931,16 @@ void TransportGPU::one_step() {
     const int gridS = std::min(grid_for(E_), 4096);
     // data_.set_time at the step's start (transport.cc:600): boundary data given since
     // the last step apply now; with the matrix in place only the boundary vector is
-    // rebuilt (transport.cc:620-626)
+    // rebuilt (transport.cc:620-626); a function of time gives new data at every step
     bool bc_changed = false;
+    if (bc_fn_) {
+        h_bc_next_.assign((size_t)nsub_ * NBC_, 0.0);
+        bc_fn_(tg_.time, h_bc_next_.data(), bc_user_);
+        bc_pending_ = true;
+    }
     if (bc_pending_) {
+        // the previous upload from h_bc_conc_ has finished before it is overwritten
+        FL_HIP(hipStreamSynchronize(stream_));
         h_bc_conc_ = h_bc_next_;
         FL_HIP(hipMemcpyAsync(d_bc_conc_, h_bc_conc_.data(), h_bc_conc_.size() * sizeof(double),
                               hipMemcpyHostToDevice, stream_));

@@ -84,6 +84,7 @@ SIGNATURES = {
     "fl_transport_set_dual_porosity_sorption": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_void_p,
                                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fl_transport_get_conc_phase": (C.c_int, [C.c_void_p, C.c_int, _f64p]),
+    "fl_transport_set_bc_conc_function": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fl_msh_write_node_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
                                          C.c_int, _f64p, C.c_int]),
     "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
@@ -643,6 +644,9 @@ class LearnedCoarseBasis:
             self.h = None
 
 
+BC_CONC_FN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
 class ConvectionTransport:
     """GPU replacement of ConvectionTransport + TransportOperatorSplitting (transport.cc)."""
 
@@ -762,6 +766,21 @@ class ConvectionTransport:
         ptr = [None if a is None else a.ctypes.data for a in self._phases[1:]]
         _check(lib().fl_transport_set_dual_porosity_sorption(self.h, int(bool(dual_porosity)), int(bool(sorption)),
                                                              self._phases[0], *ptr))
+
+    def set_bc_conc_function(self, fn):
+        """Boundary concentrations fn(t) -> [n_subst][NBC] at every step's start time
+        (a time-dependent bc_conc formula); None: keep the last data fixed."""
+        shape = (self.nsub, self.mesh.n_bc_elements)
+        if fn is None:
+            self._bc_fn = None
+            _check(lib().fl_transport_set_bc_conc_function(self.h, None, None))
+            return
+
+        def cb(t, out, user):
+            np.ctypeslib.as_array(out, shape=shape)[:] = fn(t)
+
+        self._bc_fn = BC_CONC_FN(cb)
+        _check(lib().fl_transport_set_bc_conc_function(self.h, C.cast(self._bc_fn, C.c_void_p), None))
 
     def conc_phase(self, phase):
         """get_concentration_matrix()[phase]: 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed."""
