@@ -455,6 +455,23 @@ int fl_transport_set_pade_reactions(fl_transport *t, int n_decays, const int *pa
  * [n_subst][E]; the step adds (density + max(conc_src - c, 0) sigma) dt. */
 int fl_transport_set_sources(fl_transport *t, const double *density, const double *sigma, const double *conc);
 
+/* Sorption of the reaction module (Sorption, reaction/sorption.cc:71-285 with
+ * Isotherm, isotherm.cc:33-258; the "adsorptions" record of
+ * TransportOperatorSplitting, transport_operator_splitting.cc:178-192), applied
+ * after every step and after the decays: species k on substance subst[k];
+ * sorption_type (0 none, 1 linear, 2 freundlich, 3 langmuir), mult_coef,
+ * second_param [n_bulk_regions][n]; rock_density [n_bulk_regions]; molar_mass and
+ * solubility [n] (solubility null or <= 0: no limit); solvent_dens; substeps
+ * (the reference's default 1000) table intervals.  The porosity must be constant
+ * on each bulk region and dual porosity off.  Set before the first step. */
+int fl_transport_set_reaction_sorption(fl_transport *t, int n, const int *subst, const int *sorption_type,
+                                       const double *mult_coef, const double *second_param,
+                                       const double *rock_density, const double *molar_mass,
+                                       const double *solubility, double solvent_dens, int substeps);
+
+/* Sorbed concentrations of the reaction module's species, out[n][E]. */
+int fl_transport_get_reaction_sorbed(fl_transport *t, double *out);
+
 /* Boundary concentrations given as a function of time (a time-dependent
  * FieldFormula for bc_conc, field_formula_impl.hh:70-122, read by
  * ConvectionTransport::compute_one_step at every step's start, transport.cc:598-627):

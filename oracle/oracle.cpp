@@ -1580,6 +1580,17 @@ struct Transport {
     bool dual = false, sorp = false;
     std::vector<double> c_imm, c_msorb, c_isorb, por_imm, phi, alpha, sorp_c0, sorp_c1;
     std::vector<int> sorp_type;
+    // reaction-module sorption (reaction/sorption.cc, isotherm.cc): per species its
+    // substance, per (bulk region, species) the isotherm and its interpolation table
+    struct Iso {
+        int type = 0;
+        double c0 = 0, c1 = 0, scale_aqua = 0, scale_sorbed = 0, rho = 1, limit = 0, step = 0;
+        bool limited = false;
+        std::vector<double> table;
+    };
+    std::vector<int> rs_subst;
+    std::vector<Iso> rs_iso;           // [n_bulk_regions][n_species]
+    std::vector<double> rs_sorbed;     // [n_species][E]
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -1808,6 +1819,84 @@ static void sorption(Transport &T, int i, int k) {
         if (conc_avg_imm != 0)
             compute_sorption(conc_avg_imm, T.sorp_c0[j], T.sorp_c1[j], T.sorp_type[j], &T.c_imm[j], &T.c_isorb[j],
                              Nimm / n, n * (1 - phi) / Nimm);
+    }
+}
+
+// Isotherm functors (isotherm.hh:100-170): 0 none, 1 linear, 2 freundlich, 3 langmuir
+static double iso_func(const Transport::Iso &I, double x) {
+    if (I.type == 1) return I.c0 * x;
+    if (I.type == 2) return I.c0 * std::pow(x, I.c1);
+    if (I.type == 3) return I.c0 * (I.c1 * x) / (I.c1 * x + 1);
+    return 0.0;
+}
+// Isotherm::solve_conc (isotherm.cc:86-109): the aqueous concentration where
+// scale_sorbed f(c / rho) + scale_aqua c = total_mass on [0, critical mass / scale_aqua
+// + 1e-5].  The reference brackets with boost's TOMS 748 to 30 bits and takes the
+// bracket's midpoint; this restatement bisects to the last bit (the two agree to
+// about 1e-9 of the value).
+static void iso_solve(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
+    double total = I.scale_aqua * c_aqua + I.scale_sorbed * c_sorbed;
+    double critic = I.limit * I.scale_aqua + iso_func(I, I.limit / I.rho) * I.scale_sorbed;
+    double a = 0.0, b = critic / I.scale_aqua + 0.00001;
+    auto F = [&](double c) { return I.scale_sorbed * iso_func(I, c / I.rho) + I.scale_aqua * c - total; };
+    double fa = F(a), fb = F(b);
+    // an end point that is a root is returned as it is (both ends of the bracket)
+    if (fa == 0.0) b = a;
+    else if (fb == 0.0) a = b;
+    for (int it = 0; it < 200 && a != b; it++) {
+        double m = 0.5 * (a + b);
+        if (!(m > a && m < b)) break;
+        double fm = F(m);
+        if ((fm < 0) == (fa < 0)) a = m, fa = fm;
+        else b = m;
+    }
+    c_aqua = (a + b) / 2;
+    c_sorbed = (total - I.scale_aqua * c_aqua) / I.scale_sorbed;
+}
+// Isotherm::make_table (isotherm.cc:220-242): n + 1 rotated sorbed values at equidistant
+// total masses up to the mass of the table limit (1 where no solubility limit is set)
+static void iso_table(Transport::Iso &I, int n) {
+    if (I.limit <= 0.0) I.limit = 1.0;
+    double mass_limit = I.scale_aqua * I.limit + I.scale_sorbed * iso_func(I, I.limit / I.rho);
+    I.step = mass_limit / n;
+    double mass = 0.0;
+    I.table.clear();
+    for (int i = 0; i <= n; i++) {
+        double c_aqua = mass / I.scale_aqua, c_sorbed = 0.0;
+        iso_solve(I, c_aqua, c_sorbed);
+        I.table.push_back(c_sorbed * I.scale_aqua - c_aqua * I.scale_sorbed);
+        mass = mass + I.step;
+    }
+}
+// Isotherm::compute_projection (isotherm.cc:56-84): the total mass's position in the
+// table, linear interpolation of the rotated coordinate, rotation back; beyond the
+// table the solubility limit (precipitate, :244-258) or a direct solve.
+static void iso_project(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
+    double inv_a = I.scale_aqua / (I.scale_aqua * I.scale_aqua + I.scale_sorbed * I.scale_sorbed);
+    double inv_s = I.scale_sorbed / (I.scale_aqua * I.scale_aqua + I.scale_sorbed * I.scale_sorbed);
+    double total = (I.scale_aqua * c_aqua + I.scale_sorbed * c_sorbed);
+    double steps = total / I.step;
+    int idx = (int)std::floor(steps);
+    if (idx < 0) throw std::runtime_error("total_mass seems to have negative value");
+    if ((size_t)idx < I.table.size() - 1) {
+        double rot = I.table[idx] + (steps - idx) * (I.table[idx + 1] - I.table[idx]);
+        c_aqua = (total * inv_a - rot * inv_s);
+        c_sorbed = (total * inv_s + rot * inv_a);
+    } else if (I.limited) {
+        c_aqua = I.limit;
+        c_sorbed = (total - I.scale_aqua * I.limit) / I.scale_sorbed;
+    } else {
+        iso_solve(I, c_aqua, c_sorbed);
+    }
+}
+// Sorption::compute_one_step (sorption.cc:230-285): every element, every species
+static void reaction_sorption(Transport &T) {
+    Mesh &m = *T.m;
+    size_t ns = T.rs_subst.size();
+    for (int e = 0; e < T.E; e++) {
+        int r = m.rdb.regs[m.element[e].region].sub_idx;
+        for (size_t k = 0; k < ns; k++)
+            iso_project(T.rs_iso[r * ns + k], T.conc[(size_t)T.rs_subst[k] * T.E + e], T.rs_sorbed[k * T.E + e]);
     }
 }
 
@@ -2101,6 +2190,46 @@ void orc_transport_conc_phase(void *h, int phase, double *out) {
     else std::memcpy(out, c.data(), c.size() * sizeof(double));
 }
 
+// Reaction-module sorption (Sorption, transport_operator_splitting.cc:178-192, 244-287):
+// n species on substances subst[n]; per bulk region (sub-index order) and species the
+// isotherm type, mult_coef, second_param [nreg][n]; rock_density[nreg]; molar_mass[n],
+// solubility[n] (<= 0 or null: no limit); porosity por_m constant per region, por_imm 0,
+// phi 1 (no dual porosity).  Returns 0, or -1 with por_m varying inside a region.
+int orc_transport_set_reaction_sorption(void *h, int n, const int *subst, const int *type, const double *mult,
+                                        const double *second, const double *rock_density, const double *molar_mass,
+                                        const double *solubility, double solvent_dens, int substeps) {
+    Transport &T = *(Transport *)h;
+    Mesh &m = *T.m;
+    int nreg = m.rdb.n_bulk;
+    std::vector<double> por(nreg, -1.0);
+    for (int e = 0; e < T.E; e++) {
+        int r = m.rdb.regs[m.element[e].region].sub_idx;
+        if (por[r] >= 0 && por[r] != T.por[e]) return -1;
+        por[r] = T.por[e];
+    }
+    T.rs_subst.assign(subst, subst + n);
+    T.rs_iso.assign((size_t)nreg * n, Transport::Iso());
+    for (int r = 0; r < nreg; r++)
+        for (int k = 0; k < n; k++) {
+            Transport::Iso &I = T.rs_iso[(size_t)r * n + k];
+            I.type = type[r * n + k];
+            I.c0 = mult[r * n + k];
+            I.c1 = second[r * n + k];
+            I.rho = solvent_dens;
+            I.scale_aqua = por[r];
+            I.scale_sorbed = 1.0 * (1 - por[r] - 0.0) * rock_density[r] * molar_mass[k];
+            I.limit = solubility ? solubility[k] : 0.0;
+            I.limited = I.limit > 0.0;
+            iso_table(I, I.type == 0 ? 1 : substeps);
+        }
+    T.rs_sorbed.assign((size_t)n * T.E, 0.0);
+    return 0;
+}
+void orc_transport_reaction_sorbed(void *h, double *out) {
+    Transport &T = *(Transport *)h;
+    std::memcpy(out, T.rs_sorbed.data(), T.rs_sorbed.size() * sizeof(double));
+}
+
 // Boundary concentrations as a function of time: fn(t, bc_conc[nsub][NBC], user) at the
 // start of every step (fn null: back to fixed data).
 void orc_transport_set_bc_function(void *h, void (*fn)(double, double *, void *), void *user) {
@@ -2132,6 +2261,7 @@ int orc_transport_update(void *h, double t) {
     while (T.tg.lt(t)) {
         one_step(T);
         if (!T.r_parent.empty()) compute_reactions(T);
+        if (!T.rs_subst.empty()) reaction_sorption(T);
         mb_calculate(T, T.tg.time);
         steps++;
     }

@@ -70,6 +70,9 @@ def lib():
                                                C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_conc_phase.argtypes = [C.c_void_p, C.c_int, _f64p]
         L.orc_transport_set_bc_function.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_transport_set_reaction_sorption.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                                          _f64p, _f64p, C.c_double, C.c_int]
+        L.orc_transport_reaction_sorbed.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_darcy_unsteady_setup.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double]
@@ -335,6 +338,28 @@ class Transport:
 
         self._bc_fn = BC_FN(cb)
         lib().orc_transport_set_bc_function(self.h, C.cast(self._bc_fn, C.c_void_p), None)
+
+    def set_reaction_sorption(self, subst, types, mult, second, rock_density, molar_mass, solubility=None,
+                              solvent_dens=1.0, substeps=1000):
+        """Sorption of the reaction module: species on substances subst[n]; types, mult,
+        second [n_bulk_regions][n]; rock_density [n_bulk_regions]; molar_mass, solubility [n]."""
+        n = len(subst)
+        nreg = max(r["sub_idx"] for r in self.mesh.regions if not r["boundary"]) + 1
+        self._rs = [np.ascontiguousarray(subst, np.int32),
+                    np.ascontiguousarray(np.broadcast_to(types, (nreg, n)), np.int32).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(mult, (nreg, n)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(second, (nreg, n)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(rock_density, (nreg,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(molar_mass, (n,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(-1.0 if solubility is None else solubility, (n,)),
+                                         np.float64)]
+        if lib().orc_transport_set_reaction_sorption(self.h, n, *self._rs, float(solvent_dens), int(substeps)):
+            raise ValueError("porosity must be constant on each bulk region")
+
+    def reaction_sorbed(self):
+        out = np.zeros(len(self._rs[0]) * self.mesh.n_elements)
+        lib().orc_transport_reaction_sorbed(self.h, out)
+        return out.reshape(len(self._rs[0]), -1)
 
     def set_pade(self, nom_deg, den_deg):
         """Use the Pade approximant of the given degrees for the decays already set."""

@@ -838,6 +838,24 @@ int fl_transport_set_dual_porosity_sorption(fl_transport *t, int dual_porosity, 
     });
 }
 
+int fl_transport_set_reaction_sorption(fl_transport *t, int n, const int *subst, const int *sorption_type,
+                                       const double *mult_coef, const double *second_param,
+                                       const double *rock_density, const double *molar_mass,
+                                       const double *solubility, double solvent_dens, int substeps) {
+    return guard([&] {
+        t->g->set_reaction_sorption(n, subst, sorption_type, mult_coef, second_param, rock_density, molar_mass,
+                                    solubility, solvent_dens, substeps);
+        return 0;
+    });
+}
+
+int fl_transport_get_reaction_sorbed(fl_transport *t, double *out) {
+    return guard([&] {
+        t->g->get_reaction_sorbed(out);
+        return 0;
+    });
+}
+
 int fl_transport_set_bc_conc_function(fl_transport *t, fl_bc_conc_fn fn, void *user) {
     return guard([&] {
         t->g->set_bc_function(fn, user);

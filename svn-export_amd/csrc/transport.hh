@@ -100,6 +100,16 @@ class TransportGPU {
     // immobile phase too.  The immobile and sorbed phases start at zero.
     void set_phases(bool dual, bool sorption, const double *por_imm, const double *alpha, const double *phi,
                     const int *sorp_type, const double *sorp_coef0, const double *sorp_coef1);
+    // Sorption of the reaction module (reaction/sorption.cc, isotherm.cc) after every step:
+    // species k on substance subst[k]; per bulk region (sub-index order) and species the
+    // isotherm type (0 none, 1 linear, 2 freundlich, 3 langmuir), mult_coef, second_param
+    // [nreg][n]; rock_density [nreg]; molar_mass, solubility [n] (<= 0: no limit).  The
+    // porosity must be constant on each region (the reference then tabulates the
+    // isotherm per region); no dual porosity.
+    void set_reaction_sorption(int n, const int *subst, const int *type, const double *mult, const double *second,
+                               const double *rock_density, const double *molar_mass, const double *solubility,
+                               double solvent_dens, int substeps);
+    void get_reaction_sorbed(double *out) const;  // [n][E]
     // phase 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed (transport.h:58-63) [nsub][E]
     void get_conc_phase(int phase, double *out) const;
     void set_velocity_host(const double *side_flux);      // [S]
@@ -134,7 +144,11 @@ class TransportGPU {
     bool bc_pending_ = false;
     BcFn bc_fn_ = nullptr;
     void *bc_user_ = nullptr;
-    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_ && !phases_on(); }
+    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_ && !phases_on() && !rs_n_; }
+    int rs_n_ = 0, rs_len_ = 0;   // reaction sorption: species, table length
+    const int *d_rs_subst_ = nullptr, *d_el_region_ = nullptr;
+    const double *d_rs_par_ = nullptr, *d_rs_table_ = nullptr;  // [nreg*n][8] parameters, [nreg*n][len] tables
+    double *d_rs_sorbed_ = nullptr;                                // [E][n]
     bool phases_on() const { return dual_ || sorp_; }
     bool dual_ = false, sorp_ = false;
     // per element [E][m] interleaved like the concentrations: immobile, mobile sorbed,

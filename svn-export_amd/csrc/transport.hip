@@ -402,6 +402,83 @@ __global__ void __launch_bounds__(256) k_phases(const TmDev t, const PhaseDev ph
     }
 }
 
+// Isotherm of the reaction module's sorption (isotherm.hh:100-170, isotherm.cc):
+// parameters per (region, species): type, c0 (mult_coef), c1 (second_param),
+// rho (solvent density), scale_aqua (porosity), scale_sorbed (phi (1 - porosity)
+// rock density molar mass), limit (solubility, or 1 for the table), limited, step.
+struct IsoPar {
+    int type;
+    double c0, c1, rho, sa, ss, limit, step;
+    bool limited;
+};
+__host__ __device__ inline double iso_f(const IsoPar &I, double x) {
+    if (I.type == 1) return I.c0 * x;
+    if (I.type == 2) return I.c0 * pow(x, I.c1);
+    if (I.type == 3) return I.c0 * (I.c1 * x) / (I.c1 * x + 1);
+    return 0.0;
+}
+// Isotherm::solve_conc (isotherm.cc:86-109): the root c of ss f(c / rho) + sa c = total
+// on [0, critical mass / sa + 1e-5], by bisection to the last bit (the reference uses a
+// 30-bit TOMS 748 bracket); an end point that is a root is returned exactly.
+__host__ __device__ inline void iso_solve(const IsoPar &I, double &c_aqua, double &c_sorbed) {
+    const double total = I.sa * c_aqua + I.ss * c_sorbed;
+    const double critic = I.limit * I.sa + iso_f(I, I.limit / I.rho) * I.ss;
+    double a = 0.0, b = critic / I.sa + 0.00001;
+    const double fa0 = I.ss * iso_f(I, a / I.rho) + I.sa * a - total;
+    const double fb0 = I.ss * iso_f(I, b / I.rho) + I.sa * b - total;
+    if (fa0 == 0.0) b = a;
+    else if (fb0 == 0.0) a = b;
+    const bool neg_a = fa0 < 0;
+    for (int it = 0; it < 200 && a != b; it++) {
+        const double m = 0.5 * (a + b);
+        if (!(m > a && m < b)) break;
+        const double fm = I.ss * iso_f(I, m / I.rho) + I.sa * m - total;
+        if ((fm < 0) == neg_a) a = m;
+        else b = m;
+    }
+    c_aqua = (a + b) / 2;
+    c_sorbed = (total - I.sa * c_aqua) / I.ss;
+}
+
+// Sorption::compute_one_step (sorption.cc:263-285) through Isotherm::compute_projection
+// (isotherm.cc:56-84): per element and species, the total mass's place in the region's
+// table, the rotated sorbed coordinate interpolated and rotated back; beyond the table
+// the solubility limit or a direct solve.  par: [nreg*n] IsoPar, tab: [nreg*n][len].
+template <int M>
+__global__ void __launch_bounds__(256) k_rsorp(int E, int n, const int *__restrict__ subst,
+                                              const int *__restrict__ region, const IsoPar *__restrict__ par,
+                                              const double *__restrict__ tab, int len, double *__restrict__ c,
+                                              double *__restrict__ sorbed) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
+        const int r = region[i];
+        for (int k = 0; k < n; k++) {
+            const IsoPar I = par[r * n + k];
+            const double *T = tab + (size_t)(r * n + k) * len;
+            double &ca = c[(size_t)i * M + subst[k]];
+            double &cs = sorbed[(size_t)i * n + k];
+            double c_aqua = ca, c_sorbed = cs;
+            const double den = I.sa * I.sa + I.ss * I.ss;
+            const double inv_a = I.sa / den, inv_s = I.ss / den;
+            const double total = (I.sa * c_aqua + I.ss * c_sorbed);
+            const double steps = total / I.step;
+            const int idx = (int)floor(steps);
+            const int tl = I.type == 0 ? 2 : len;
+            if (idx >= 0 && idx < tl - 1) {
+                const double rot = T[idx] + (steps - idx) * (T[idx + 1] - T[idx]);
+                c_aqua = (total * inv_a - rot * inv_s);
+                c_sorbed = (total * inv_s + rot * inv_a);
+            } else if (idx >= 0 && I.limited) {
+                c_aqua = I.limit;
+                c_sorbed = (total - I.sa * I.limit) / I.ss;
+            } else if (idx >= 0) {
+                iso_solve(I, c_aqua, c_sorbed);
+            }
+            ca = c_aqua;
+            cs = c_sorbed;
+        }
+    }
+}
+
 // The same step with one lane per (row, substance): four lanes per row, 64
 // rows per block, so the concentration loads and stores of a wave are 16
 // whole rows (contiguous); the matrix comes from its column-major copy.  Per
@@ -1015,6 +1092,18 @@ void TransportGPU::one_step() {
         }
     }
     FL_HIP(hipGetLastError());
+    if (rs_n_) {
+        switch (nsub_) {
+#define FL_RS(MM)                                                                                                 \
+    case MM:                                                                                                      \
+        hipLaunchKernelGGL(k_rsorp<MM>, dim3(gridS), dim3(kBlock), 0, stream_, E_, rs_n_, d_rs_subst_, d_el_region_, \
+                           (const IsoPar *)d_rs_par_, d_rs_table_, rs_len_, cnew, d_rs_sorbed_);                   \
+        break;
+            FL_RS(1) FL_RS(2) FL_RS(3) FL_RS(4) FL_RS(5) FL_RS(6) FL_RS(7) FL_RS(8)
+#undef FL_RS
+        }
+        FL_HIP(hipGetLastError());
+    }
     if (use_quad()) {
         acc_dirty_ = true;  // totals stay in the per-block accumulators until the output time
     } else {
@@ -1097,6 +1186,80 @@ void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, c
     dual_ = dual;
     sorp_ = sorption;
     FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void TransportGPU::set_reaction_sorption(int n, const int *subst, const int *type, const double *mult,
+                                         const double *second, const double *rock_density, const double *molar_mass,
+                                         const double *solubility, double solvent_dens, int substeps) {
+    if (stats_.steps > 0) throw Error(-3, "sorption must be set before the first step");
+    if (n < 1 || n > nsub_) throw Error(-3, "sorbing species must be 1..n_subst");
+    if (dual_) throw Error(-3, "the reaction module's sorption is not combined with dual porosity here");
+    if (substeps < 1 || substeps > 100000) throw Error(-3, "substeps must be in 1..100000");
+    const int nreg = m_.n_bulk_regions;
+    std::vector<double> por(nreg, -1.0);
+    for (int i = 0; i < E_; i++) {
+        const int r = h_el_region_sub_[i];
+        if (por[r] >= 0 && por[r] != h_por_[i]) throw Error(-3, "porosity must be constant on each bulk region");
+        por[r] = h_por_[i];
+    }
+    std::vector<int> sub(subst, subst + n);
+    for (int k = 0; k < n; k++)
+        if (sub[k] < 0 || sub[k] >= nsub_) throw Error(-3, "Wrong name of " + std::to_string(k) + "-th adsorbing specie.");
+    const int len = substeps + 1;
+    std::vector<IsoPar> par((size_t)nreg * n);
+    std::vector<double> tab((size_t)nreg * n * len, 0.0);
+    for (int r = 0; r < nreg; r++)
+        for (int k = 0; k < n; k++) {
+            const int t = type[r * n + k];
+            if (t < 0 || t > 3) throw Error(-3, "sorption type must be 0..3 (none, linear, freundlich, langmuir)");
+            IsoPar I{};
+            I.type = t;
+            I.c0 = mult[r * n + k];
+            I.c1 = second[r * n + k];
+            I.rho = solvent_dens;
+            I.sa = por[r] < 0 ? 1.0 : por[r];
+            I.ss = 1.0 * (1 - I.sa - 0.0) * rock_density[r] * molar_mass[k];
+            if (I.ss == 0.0) throw Error(-3, "scale_sorbed (phi (1 - por_m - por_imm) rock_density molar_mass) is zero");
+            I.limit = solubility ? solubility[k] : 0.0;
+            I.limited = I.limit > 0.0;
+            // Isotherm::make_table (isotherm.cc:220-242)
+            const int steps = t == 0 ? 1 : substeps;
+            if (I.limit <= 0.0) I.limit = 1.0;
+            const double mass_limit = I.sa * I.limit + I.ss * iso_f(I, I.limit / I.rho);
+            I.step = mass_limit / steps;
+            double mass = 0.0;
+            for (int j = 0; j <= steps; j++) {
+                double ca = mass / I.sa, cs = 0.0;
+                iso_solve(I, ca, cs);
+                tab[((size_t)r * n + k) * len + j] = cs * I.sa - ca * I.ss;
+                mass = mass + I.step;
+            }
+            par[(size_t)r * n + k] = I;
+        }
+    std::vector<int> reg(h_el_region_sub_.begin(), h_el_region_sub_.end());
+    d_el_region_ = upv(owned_, reg, stream_);
+    d_rs_subst_ = upv(owned_, sub, stream_);
+    IsoPar *dp = dev_alloc<IsoPar>(par.size());
+    owned_.push_back(dp);
+    FL_HIP(hipMemcpyAsync(dp, par.data(), par.size() * sizeof(IsoPar), hipMemcpyHostToDevice, stream_));
+    d_rs_par_ = (const double *)dp;
+    d_rs_table_ = upv(owned_, tab, stream_);
+    std::vector<double> zero((size_t)E_ * n, 0.0);
+    d_rs_sorbed_ = upv(owned_, zero, stream_);
+    rs_n_ = n;
+    rs_len_ = len;
+    FL_HIP(hipStreamSynchronize(stream_));
+}
+
+void TransportGPU::get_reaction_sorbed(double *out) const {
+    const size_t n = (size_t)E_ * rs_n_;
+    std::vector<double> c(n);
+    if (n) {
+        FL_HIP(hipMemcpyAsync(c.data(), d_rs_sorbed_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipStreamSynchronize(stream_));
+    }
+    for (int k = 0; k < rs_n_; k++)
+        for (int i = 0; i < E_; i++) out[(size_t)k * E_ + i] = c[(size_t)i * rs_n_ + k];
 }
 
 void TransportGPU::get_conc_phase(int phase, double *out) const {

@@ -85,6 +85,9 @@ SIGNATURES = {
                                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fl_transport_get_conc_phase": (C.c_int, [C.c_void_p, C.c_int, _f64p]),
     "fl_transport_set_bc_conc_function": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fl_transport_set_reaction_sorption": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                                     C.c_void_p, C.c_double, C.c_int]),
+    "fl_transport_get_reaction_sorbed": (C.c_int, [C.c_void_p, _f64p]),
     "fl_msh_write_node_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
                                          C.c_int, _f64p, C.c_int]),
     "fl_transport_write_mass_balance": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
@@ -781,6 +784,29 @@ class ConvectionTransport:
 
         self._bc_fn = BC_CONC_FN(cb)
         _check(lib().fl_transport_set_bc_conc_function(self.h, C.cast(self._bc_fn, C.c_void_p), None))
+
+    def set_reaction_sorption(self, subst, types, mult, second, rock_density, molar_mass, solubility=None,
+                              solvent_dens=1.0, substeps=1000):
+        """The reaction module's sorption: species on substances subst[n]; types
+        (0 none, 1 linear, 2 freundlich, 3 langmuir), mult, second [n_bulk_regions][n];
+        rock_density [n_bulk_regions]; molar_mass, solubility [n]."""
+        n = len(subst)
+        nreg = self.mesh.n_bulk_regions
+        self._rs = [np.ascontiguousarray(subst, np.int32),
+                    np.ascontiguousarray(np.broadcast_to(types, (nreg, n)), np.int32).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(mult, (nreg, n)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(second, (nreg, n)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(rock_density, (nreg,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(molar_mass, (n,)), np.float64),
+                    None if solubility is None else np.ascontiguousarray(np.broadcast_to(solubility, (n,)), np.float64)]
+        sol = None if self._rs[6] is None else self._rs[6].ctypes.data
+        _check(lib().fl_transport_set_reaction_sorption(self.h, n, *self._rs[:6], sol, float(solvent_dens),
+                                                        int(substeps)))
+
+    def reaction_sorbed(self):
+        out = np.zeros(len(self._rs[0]) * self.mesh.n_elements)
+        _check(lib().fl_transport_get_reaction_sorbed(self.h, out))
+        return out.reshape(len(self._rs[0]), -1)
 
     def conc_phase(self, phase):
         """get_concentration_matrix()[phase]: 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed."""

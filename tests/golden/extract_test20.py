@@ -4,7 +4,9 @@ into tests/golden/ref:
   * test20_A_mobile.txt: the substance A (no sorption; B and C are equal to it in
     the reference's output) of ref_output/test_20.con/transport_test_20/*.vtu at
     t = 0, 2.5, .., 15, one output time a line ("time v1 v2 ..." with the values
-    as printed)."""
+    as printed);
+  * test20_sorbing.txt: the sorbing substances As-lin, Bs-lang, Cs-freund at
+    t = 7.5 and 15, one a line ("name time v1 v2 ...")."""
 import os
 import re
 import shutil
@@ -25,6 +27,16 @@ def main():
         m = re.search(r'Name="A_mobile_\[M/L\^3\]"[^>]*>\s*(.*?)\s*</DataArray>', txt, re.S)
         rows.append(" ".join([t] + m.group(1).split()))
     with open(os.path.join(OUT, "test20_A_mobile.txt"), "w") as fh:
+        fh.write("\n".join(rows) + "\n")
+    rows = []
+    for k, (t, f) in enumerate(re.findall(r'timestep="([^"]+)" group="" part="0" file="([^"]+)"', pvd)):
+        if k not in (15, 30):
+            continue
+        txt = open(os.path.join(d, f)).read()
+        for name in ("As-lin", "Bs-lang", "Cs-freund"):
+            m = re.search(r'Name="%s_mobile_\[M/L\^3\]"[^>]*>\s*(.*?)\s*</DataArray>' % re.escape(name), txt, re.S)
+            rows.append(" ".join([name, t] + m.group(1).split()))
+    with open(os.path.join(OUT, "test20_sorbing.txt"), "w") as fh:
         fh.write("\n".join(rows) + "\n")
 
 
