@@ -152,7 +152,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(the Monte-Carlo realizations would shard one share per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
