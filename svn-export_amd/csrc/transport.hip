@@ -1154,6 +1154,7 @@ void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, c
     if (!por_imm) throw Error(-3, "por_imm is required");
     if (dual && !alpha) throw Error(-3, "dual porosity needs alpha");
     if (sorption && (!sorp_type || !sorp_coef0 || !sorp_coef1)) throw Error(-3, "sorption needs its isotherms");
+    if (dual && rs_n_) throw Error(-3, "the reaction module's sorption is not combined with dual porosity here");
     const size_t n = (size_t)E_ * nsub_;
     auto inter = [&](const double *a, double dflt) {
         std::vector<double> v(n, dflt);
