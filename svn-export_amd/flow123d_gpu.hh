@@ -293,6 +293,24 @@ class ConvectionTransport {
                                                       p(sorp_type), p(sorp_coef0), p(sorp_coef1)),
               "dual porosity / sorption");
     }
+    // the reaction module's sorption (Sorption, sorption.cc:71-285): species on substances subst;
+    // type, mult_coef, second_param [n_bulk_regions][n]; rock_density [n_bulk_regions];
+    // molar_mass, solubility [n] (empty solubility: no limit)
+    void set_reaction_sorption(const std::vector<int> &subst, const std::vector<int> &type,
+                               const std::vector<double> &mult_coef, const std::vector<double> &second_param,
+                               const std::vector<double> &rock_density, const std::vector<double> &molar_mass,
+                               const std::vector<double> &solubility, double solvent_dens = 1.0,
+                               int substeps = 1000) {
+        check(fl_transport_set_reaction_sorption(t_, (int)subst.size(), subst.data(), type.data(), mult_coef.data(),
+                                                 second_param.data(), rock_density.data(), molar_mass.data(),
+                                                 solubility.empty() ? nullptr : solubility.data(), solvent_dens,
+                                                 substeps),
+              "reaction sorption");
+    }
+    // bc_conc as a function of time: fn(t, bc_conc[n_subst][NBC], user) at every step's start
+    void set_bc_conc_function(fl_bc_conc_fn fn, void *user) {
+        check(fl_transport_set_bc_conc_function(t_, fn, user), "bc function");
+    }
     // get_concentration_matrix()[phase]: 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed
     std::vector<double> concentrations(int phase) const {
         std::vector<double> c((size_t)n_subst_ * n_elem_);
