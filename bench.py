@@ -132,7 +132,9 @@ def main():
     ap.add_argument("--realizations", type=int, default=256)
     ap.add_argument("--slots", type=int, default=128)
     ap.add_argument("--precision", type=int, default=1, help="batched CG: 0 FP64, 1 FP32 + FP64 reliable updates")
-    ap.add_argument("--rtol", type=float, default=1e-12)
+    ap.add_argument("--rtol", type=float, default=3e-13)
+    ap.add_argument("--norm", type=int, default=1,
+                    help="stop-test residual norm: 0 ||r||_2, 1 natural Jacobi norm sqrt(r^T D^-1 r)")
     ap.add_argument("--coarse", type=int, default=2,
                     help="two-level preconditioner columns per aggregate: 0 Jacobi, 1 constant, 2 constant + learned")
     ap.add_argument("--train-realizations", type=int, default=128)
@@ -141,12 +143,12 @@ def main():
                     help="live realizations handed to the single-solve path (-1: library default)")
     ap.add_argument("--compare", type=int, default=1, help="also time the Jacobi and constant-coarse batch forms")
     ap.add_argument("--plain-sample", type=int, default=16, help="realizations timed on the plain path")
-    ap.add_argument("--accuracy-sample", type=int, default=4)
+    ap.add_argument("--accuracy-sample", type=int, default=0,
+                    help="realizations checked against the FP64 reference solve (0: all)")
     ap.add_argument("--c2", type=int, default=1, help="run the 1M-tet drop-in + transport section")
     ap.add_argument("--c2-n", type=int, default=55)
     ap.add_argument("--transport-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cg-iters", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -191,7 +193,7 @@ def main():
         Kt, _, _ = mc_fields(torch, cen, scale, NT, seed=99)
         bt = G.DarcyBatch(darcy, NT, slots=args.slots)
         bt.set_conductivity(Kt.data_ptr())
-        bt.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=0)
+        bt.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=0, norm=args.norm)
         lam = torch.zeros((NT, F), dtype=torch.float64, device="cuda")
         bt.lambda_device(lam.data_ptr())
         torch.cuda.synchronize()
@@ -224,7 +226,8 @@ def main():
             batch.set_coarse(coarse)
             torch.cuda.synchronize()
             ti = time.perf_counter() - t
-            inf = batch.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=precision)
+            inf = batch.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=precision,
+                                        norm=args.norm)
             torch.cuda.synchronize()
             if k >= warmup:
                 t_inf += ti
@@ -252,20 +255,47 @@ def main():
         # leave the chosen form's solutions in the batch for the acceptance check
         run_batch(args.coarse, args.precision, 1, 0, False)
 
-    # ------------------------------------------- acceptance: flux error sample
+    # --------------------------------- acceptance: every realization checked
+    # reference = the single-solve FP64 drop-in path to rel residual 1e-15 (the
+    # path the parity tests pin to the oracle), run after the timed region
     Kh = K.cpu().numpy()
     order = np.argsort(-sig)  # hardest (largest sigma) first
-    acc_ids = [int(order[i * (R - 1) // max(args.accuracy_sample - 1, 1)]) for i in range(args.accuracy_sample)]
-    flux_errs = []
+    acc_ids = list(range(R)) if args.accuracy_sample <= 0 else \
+        [int(order[i * (R - 1) // max(args.accuracy_sample - 1, 1)]) for i in range(args.accuracy_sample)]
+    errs = np.zeros((len(acc_ids), 3))
     ref = G.DarcyFlowMH_Steady(gm)
-    for r in acc_ids:
-        ref.set_data(Kh[r], cs, sigma_f, f_src, bct, bcp, zb, zb)
+    ref.set_data(Kh[0], cs, sigma_f, f_src, bct, bcp, zb, zb)
+    t_acc = time.perf_counter()
+    for k, r in enumerate(acc_ids):
+        ref.set_conductivity(Kh[r])
         ref.update_solution(r_tol=1e-15, a_tol=0.0, max_it=200000)
         xr = ref.get_solution_vector()
         xb = batch.get_solution_vector(r)
-        flux_errs.append(float(np.linalg.norm(xb[:S] - xr[:S]) / np.linalg.norm(xr[:S])))
+        for j, (lo, hi) in enumerate(((0, S), (S, S + E), (S + E, S + E + F))):
+            errs[k, j] = np.linalg.norm(xb[lo:hi] - xr[lo:hi]) / np.linalg.norm(xr[lo:hi])
+    # a realization above the bar: how far apart are two FP64 solutions of it
+    # (the 2-norm reference above and the natural-norm single solve to 1e-16)?
+    # That spread is the floor of any FP64 method on that field
+    not_acc = []
+    for k in np.flatnonzero(errs[:, 0] > 1e-8):
+        r = acc_ids[k]
+        ref.set_conductivity(Kh[r])
+        ref.update_solution(r_tol=1e-15, a_tol=0.0, max_it=200000)
+        xa = ref.get_solution_vector()
+        ref.update_solution(r_tol=1e-16, a_tol=0.0, max_it=300000, norm=1)
+        xn = ref.get_solution_vector()
+        not_acc.append({"realization": int(r), "sigma": round(float(sig[r]), 3), "corr_len": round(float(ell[r]), 3),
+                        "flux_rel_err": float(errs[k, 0]),
+                        "fp64_floor_flux_rel": float(np.linalg.norm(xa[:S] - xn[:S]) / np.linalg.norm(xa[:S]))})
     del ref
-    log("flux rel err (sample %s): %s" % (acc_ids, ["%.2e" % e for e in flux_errs]))
+    t_acc = time.perf_counter() - t_acc
+    n_acc = int((errs[:, 0] <= 1e-8).sum())
+    log("flux rel err over %d realizations (%.1f s): max %.2e median %.2e; pressure max %.2e; trace max %.2e; "
+        "accepted %d" % (len(acc_ids), t_acc, errs[:, 0].max(), np.median(errs[:, 0]), errs[:, 1].max(),
+                         errs[:, 2].max(), n_acc))
+    # only accepted solves count (all of them when every realization was checked)
+    if len(acc_ids) == R:
+        value = n_acc * args.steps / elapsed
 
     # ------------------------------------------------ plain drop-in, same data
     plain_ids = [int(order[i * (R - 1) // max(args.plain_sample - 1, 1)]) for i in range(args.plain_sample)]
@@ -274,7 +304,7 @@ def main():
         darcy.set_conductivity(Kh[r])
         torch.cuda.synchronize()
         t = time.perf_counter()
-        pi = darcy.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=0)
+        pi = darcy.update_solution(r_tol=args.rtol, a_tol=0.0, max_it=100000, precision=0, norm=args.norm)
         torch.cuda.synchronize()
         t_plain += time.perf_counter() - t
         it_plain.append(pi["iterations"])
@@ -319,13 +349,23 @@ def main():
                 "coarse_solve_and_direction_us": round(float(info["ms_coarse_avg"]) * 1e3, 2)}
 
     # --------------------------------------- secondary: 1M mixed + transport
+    peak_copy = stream_peak(torch)
+    log("stream copy: %.1f GB/s" % peak_copy)
+    roofline["measured_copy_GBs"] = peak_copy
+    roofline["frac_of_measured_copy"] = round(achieved / peak_copy, 4)
     c2 = None
     if args.c2:
         c2 = c2_section(torch, args)
+        if c2 and "transport" in c2:
+            tr = c2["transport"]
+            tr["measured_copy_GBs"] = peak_copy
+            tr["frac_of_measured_copy"] = round(tr["achieved_GBs"] / peak_copy, 4)
 
     cpu = None
+    c2_cpu = c2.pop("_cpu", None) if c2 else None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(mesh, Kh[int(order[R // 2])], cs, bct, bcp, float(np.mean(it_plain)), args.cpu_cg_iters)
+        cpu = cpu_baseline(torch, args, mesh, Kh[int(order[R // 2])], cs, bct, bcp,
+                           {"transport": True, **c2_cpu} if c2_cpu else None)
 
     out = {
         "metric": METRIC,
@@ -343,12 +383,14 @@ def main():
         "config": {
             "workload": "Monte-Carlo batch: %d realizations (sigma %.1f-%.1f, corr length %.2f-%.2f) of steady MH "
                         "Darcy on a %d-tet cube with %d fracture triangles; batched learned-preconditioned FP%s "
-                        "flexible CG%s to rel residual %g, accepted at <= 1e-8 rel flux error"
+                        "CG%s to rel residual %g (%s), accepted at <= 1e-8 rel flux error"
                         % (R, sig.min(), sig.max(), ell.min(), ell.max(), int((gm.elem_dim == 3).sum()),
                            int((gm.elem_dim == 2).sum()), "32" if args.precision == 1 else "64",
-                           " + FP64 reliable updates" if args.precision == 1 else "", args.rtol),
+                           " + FP64 reliable updates" if args.precision == 1 else "", args.rtol,
+                           "natural Jacobi norm" if args.norm else "2-norm"),
             "elements": E, "edges": F, "sides": S, "edge_nnz": nnz, "realizations": R, "slots": slots,
-            "method": ("batched flexible CG, two-level preconditioner (Jacobi + aggregation coarse space, %s), "
+            "method": ("batched preconditioned CG (Fletcher-Reeves beta: the two-level preconditioner is one fixed SPD "
+                       "operator per realization), two-level preconditioner (Jacobi + aggregation coarse space, %s), "
                        "matrix-free condensed operator, slot refill, single-solve tail"
                        % {0: "off", 1: "constant column", 2: "constant + learned GNN column"}[args.coarse]),
             "learned_basis_training": training,
@@ -356,9 +398,15 @@ def main():
             "compare_batch_forms": compare,
             "iterations": {"mean": float(it.mean()), "min": int(it.min()), "max": int(it.max())},
             "converged": int(info["n_converged"]), "handovers": int(info["handovers"]),
-            "flux_rel_err_max": max(flux_errs), "flux_err_sample": acc_ids,
-            "flux_err_reference": "single-solve FP64 GPU CG to rel residual 1e-15 of the same realization",
-            "accepted": bool(max(flux_errs) <= 1e-8),
+            "accuracy": {"checked": len(acc_ids), "accepted": n_acc,
+                         "flux_rel_err_max": float(errs[:, 0].max()),
+                         "flux_rel_err_median": float(np.median(errs[:, 0])),
+                         "pressure_rel_err_max": float(errs[:, 1].max()),
+                         "trace_rel_err_max": float(errs[:, 2].max()),
+                         "not_accepted": not_acc,
+                         "reference": "single-solve FP64 GPU CG to rel residual 1e-15 of the same realization "
+                                      "(the drop-in path, pinned to the CPU oracle by the parity tests)"},
+            "accepted": bool(n_acc == len(acc_ids)),
             "plain_cg": {"solves_per_s": round(plain_rate, 4), "sample": len(plain_ids),
                          "mean_iterations": float(np.mean(it_plain)),
                          "path": "single-solve FP64 Jacobi-CG on the assembled SELL edge system (drop-in)"},
@@ -370,6 +418,24 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def stream_peak(torch, nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth on this device: a plain device-to-device copy
+    of nbytes (read + write counted), timed with events over reps copies."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    return round(gbs, 1)
 
 
 def c2_section(torch, args):
@@ -432,30 +498,153 @@ def c2_section(torch, args):
                                                           max(abs(tot[0, 3]), 1e-300))}
         log("c2 transport: %d steps, %.4f ms/step, %.1f GB/s" % (nsteps, per, step_bytes / (per * 1e-3) / 1e9))
         del tr
+        # inputs of the CPU baseline's transport leg (not part of the JSON line)
+        out["_cpu"] = {"mesh": mesh, "flux": darcy.get_solution_vector()[:S], "cs": cs, "cfl": cfl}
     return out
 
 
-def cpu_baseline(mesh, K, cs, bct, bcp, gpu_iters, cg_iters):
-    """Times the FP64 oracle (a single-threaded restatement of the reference
-    algorithm: full MH assembly, two-stage sparse Schur, Jacobi-CG, resolve)
-    on one realization of the same mesh: everything except CG runs in full, CG
-    for a bounded number of iterations; the solve time is extrapolated to the
-    mean Jacobi-CG iteration count of the sampled realizations."""
+def host_cores():
+    """threads the CPU baseline may use: the affinity set, capped by OMP_NUM_THREADS"""
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs, warm=1):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(torch, args, mesh, K, cs, bct, bcp, c2):
+    """The CPU FP64 oracle (a restatement of the reference algorithm: full MH
+    assembly, two generic sparse Schur stages, CG, resolve) timed on this
+    host, with the preconditioners labelled: Jacobi (1 thread and all
+    threads of this process) and IC(0), the closest textbook analogue of the
+    reference's default ILU(3) (src/la/solve.cc:370).  Wall clock.
+    (1) headline mesh, one realization (median sigma) solved in full;
+    (2) config 1 (unit square, 20,000 triangles): median of 5 solves after a
+        warm-up per variant, beside the GPU's FP64 drop-in and FP32 (config 3)
+        solves of the same problem, whose head / flux / mass-balance errors
+        are measured against this oracle solution;
+    (3) config 2 transport: 20 steps of 4 substances on the GPU fluxes."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    nth = host_cores()
+    out = {"host": {"cores_available": nth, "cpu_model": cpu_model()}}
+    rtol = args.rtol
+
+    # (1) headline mesh
     om = O.Mesh(arrays=mesh)
     E, NB = om.n_elements, om.n_bc_elements
     z = np.zeros(NB)
-    D = O.Darcy(om, K, cs, np.ones(E), np.zeros(E), bct, bcp, z, z)
-    D.solve(0, rtol=1e-30, atol=0.0, max_it=cg_iters)
-    st = D.stats
-    per_it = st["t_solve"] / max(st["iters"], 1)
-    t_solve = st["t_assembly"] + st["t_schur"] + st["t_resolve"] + per_it * gpu_iters
-    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle on one realization of the same mesh: assembly %.2fs + Schur %.2fs + resolve %.2fs in "
-                      "full, %d Jacobi-CG iterations (%.1f ms each) extrapolated to the plain path's mean %.0f iterations; "
-                      "wall clock, 1 thread"
-                      % (st["t_assembly"], st["t_schur"], st["t_resolve"], st["iters"], 1e3 * per_it, gpu_iters)}
+    head = {}
+    for name, meth, th in (("ic0_1thread", 3, 1), ("jacobi_%dthreads" % nth, 0, nth)):
+        O.set_threads(th)
+        D = O.Darcy(om, K, cs, np.ones(E), np.zeros(E), bct, bcp, z, z)
+        t = time.perf_counter()
+        D.solve(meth, rtol=rtol, atol=0.0, max_it=100000)
+        t = time.perf_counter() - t
+        st = D.stats
+        head[name] = {"solves_per_s": round(1.0 / t, 5), "s": round(t, 3), "iterations": st["iters"],
+                      "threads": th, "t_assembly": round(st["t_assembly"], 3), "t_schur": round(st["t_schur"], 3),
+                      "t_cg": round(st["t_solve"], 3), "t_resolve": round(st["t_resolve"], 3)}
+        log("cpu oracle headline mesh %s: %.2f s (%d iterations)" % (name, t, st["iters"]))
+        del D
+    O.set_threads(1)
+    best = max(head, key=lambda k: head[k]["solves_per_s"])
+    out["headline_mesh"] = head
+
+    # (2) config 1 on the CPU and the GPU
+    m1 = meshgen.square_mesh(100)
+    cen1, bulk1 = meshgen.bulk_centroids(m1)
+    K1 = meshgen.lognormal_field(cen1, sigma=1.0, corr_len=0.1, seed=0)
+    om1, gm1 = O.Mesh(arrays=m1), G.Mesh(arrays=m1)
+    E1, NB1 = om1.n_elements, om1.n_bc_elements
+    breg1 = m1["elem_region"][~bulk1]
+    bct1, bcp1 = bc_arrays(NB1, np.zeros((len(breg1), 3)), breg1)
+    z1, one1, zero1 = np.zeros(NB1), np.ones(E1), np.zeros(E1)
+    c1 = {"elements": E1, "edges": om1.n_edges, "cpu": {}, "gpu": {}}
+    for name, meth, th in (("jacobi_1thread", 0, 1), ("jacobi_%dthreads" % nth, 0, nth), ("ic0_1thread", 3, 1)):
+        O.set_threads(th)
+        D = O.Darcy(om1, K1, one1, one1, zero1, bct1, bcp1, z1, z1)
+        med, ts = median_time(lambda: D.solve(meth, rtol=1e-12, atol=0.0, max_it=100000), 5)
+        c1["cpu"][name] = {"solves_per_s": round(1.0 / med, 3), "median_s": round(med, 4), "runs": 5,
+                           "iterations": D.stats["iters"], "threads": th}
+        log("cpu oracle C1 %s: median %.3f s (%d iterations)" % (name, med, D.stats["iters"]))
+        del D
+    O.set_threads(1)
+    D = O.Darcy(om1, K1, one1, one1, zero1, bct1, bcp1, z1, z1)
+    D.solve(0, rtol=1e-15, atol=0.0, max_it=200000)
+    x_or, bal_or = D.solution.copy(), D.element_balance()
+    S1 = om1.n_sides
+    g1 = G.DarcyFlowMH_Steady(gm1)
+    g1.set_data(K1, one1, one1, zero1, bct1, bcp1, z1, z1)
+    for name, prec in (("fp64", 0), ("fp32_reliable_fp64", 1)):
+        def solve():
+            g1.update_solution(r_tol=1e-12, a_tol=0.0, max_it=100000, precision=prec)
+            torch.cuda.synchronize()
+        med, _ = median_time(solve, 5)
+        info = g1.info
+        x = g1.get_solution_vector()
+        bal = g1.element_balance()
+        flux_scale = np.abs(x_or[:S1]).max()
+        c1["gpu"][name] = {
+            "solves_per_s": round(1.0 / med, 2), "median_ms": round(1e3 * med, 3), "runs": 5,
+            "iterations": info["iterations"],
+            "err_vs_oracle": {
+                "flux_rel": float(np.linalg.norm(x[:S1] - x_or[:S1]) / np.linalg.norm(x_or[:S1])),
+                "pressure_head_rel": float(np.linalg.norm(x[S1:S1 + E1] - x_or[S1:S1 + E1]) /
+                                           np.linalg.norm(x_or[S1:S1 + E1])),
+                "trace_rel": float(np.linalg.norm(x[S1 + E1:] - x_or[S1 + E1:]) / np.linalg.norm(x_or[S1 + E1:])),
+                "mass_balance_max_abs_diff_rel_to_max_flux": float(np.abs(bal - bal_or).max() / flux_scale),
+                "mass_balance_max_abs_rel_to_max_flux": float(np.abs(bal).max() / flux_scale)}}
+        log("gpu C1 %s: median %.2f ms (%d iterations), flux err vs oracle %.2e"
+            % (name, 1e3 * med, info["iterations"], c1["gpu"][name]["err_vs_oracle"]["flux_rel"]))
+    out["c1_unit_square"] = c1
+
+    # (3) config 2 transport steps on the CPU, same fluxes as the GPU run
+    if c2 is not None and "transport" in c2:
+        om2 = O.Mesh(arrays=c2["mesh"])
+        E2, NB2 = om2.n_elements, om2.n_bc_elements
+        cfl = c2["cfl"]
+        T = O.Transport(om2, c2["flux"], c2["cs"], np.ones(E2), np.ones((4, NB2)), np.zeros((4, E2)),
+                        21 * cfl, 21 * cfl)
+        t = time.perf_counter()
+        T.update(cfl)  # matrix assembly + first step
+        t_build = time.perf_counter() - t
+        t = time.perf_counter()
+        n = T.update(21 * cfl)
+        t_steps = time.perf_counter() - t
+        out["c2_transport"] = {"steps_per_s": round(n / t_steps, 3), "steps_timed": n, "substances": 4,
+                               "matrix_and_first_step_s": round(t_build, 3), "threads": 1}
+        log("cpu oracle C2 transport: %d steps in %.2f s" % (n, t_steps))
+        del T, om2
+
+    b = head[best]
+    return {"value": b["solves_per_s"], "unit": "solves/s", "cores": b["threads"], "kind": "port",
+            "sample": "oracle (CPU FP64 restatement) on one realization of the headline mesh (median sigma), solved in "
+                      "full to rel residual %g: %s (assembly %.2f s, Schur %.2f s, CG %.2f s / %d iterations, resolve "
+                      "%.2f s); wall clock on %s; other variants and configs 1-2 in 'detail'"
+                      % (rtol, best.replace("_", " "), b["t_assembly"], b["t_schur"], b["t_cg"], b["iterations"],
+                         b["t_resolve"], out["host"]["cpu_model"]),
+            "detail": out}
 
 
 if __name__ == "__main__":

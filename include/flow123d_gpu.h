@@ -1,8 +1,25 @@
 /*
  * C ABI of the MI355X steady mixed-hybrid Darcy solve and explicit upwind
  * transport step.  Plain pointers and sizes only; every call returns 0 on
- * success or a negative status, with the message in fl_last_error().
+ * success or a negative status, with the message in fl_last_error().  A
+ * positive status is a warning: the solve calls return FL_NOT_CONVERGED when
+ * CG stopped at its iteration limit; the solution (the last iterate) is still
+ * available, as in the reference, which prints KSP's convergence reason and
+ * goes on (src/la/solve.cc:454-458).
  * Handles are not thread-safe; each handle owns one HIP stream.
+ *
+ * Statuses with a fixed meaning:
+ *   FL_NOT_CONVERGED        1  iteration limit reached (warning, see above)
+ *   FL_E_SINGULAR_ELEMENT -10  a local RT0 matrix is singular: |det| below
+ *                              NUM_ZERO (the reference stops there,
+ *                              src/flow/mh_fe_values.cc:54-59 with
+ *                              src/system/math_fce.cc:147,168,211) or not
+ *                              finite (a zero-volume element); the message
+ *                              names the element id
+ *   FL_E_BAD_FIELD        -11  a conductivity or cross-section value is not
+ *                              finite and positive (the message names the
+ *                              element, and the realization for a batch)
+ *   FL_E_DIVERGED         -12  the CG residual became NaN or infinite
  *
  * Each entry point names the Flow123d 1.7 interface it replaces
  * (paths relative to the Flow123d source tree).
@@ -21,6 +38,11 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+#define FL_NOT_CONVERGED 1
+#define FL_E_SINGULAR_ELEMENT (-10)
+#define FL_E_BAD_FIELD (-11)
+#define FL_E_DIVERGED (-12)
 
 typedef struct fl_mesh fl_mesh;
 typedef struct fl_darcy fl_darcy;
@@ -164,7 +186,8 @@ void fl_darcy_free(fl_darcy *d);
  * per bulk element: conductivity K, cross_section cs, sigma, water_source_density f,
  * anisotropy (9 per element, row-major; NULL = identity);
  * per boundary element: bc_type (0 none, 1 dirichlet, 2 neumann, 3 robin),
- * bc_pressure, bc_flux, bc_robin_sigma. */
+ * bc_pressure, bc_flux, bc_robin_sigma.  FL_E_BAD_FIELD unless every K and
+ * cs is finite and positive (the reference does not check them). */
 int fl_darcy_set_data(fl_darcy *d, const double *K, const double *cs, const double *aniso, const double *sigma,
                       const double *f, const int *bc_type, const double *bc_pressure, const double *bc_flux,
                       const double *bc_robin_sigma);
@@ -174,12 +197,20 @@ int fl_darcy_set_conductivity(fl_darcy *d, const double *K);
 
 /* Solver options (Solver record, src/la/solve.cc:69-84): r_tol, a_tol, max_it;
  * precision: 0 = FP64 CG (the drop-in path), 1 = FP32 CG with FP64 reliable
- * updates (the stop test is always on the true FP64 residual). */
+ * updates (the stop test is always on the true FP64 residual);
+ * norm: residual norm of the stop test ||r|| <= max(r_tol ||b||, a_tol):
+ * 0 = the 2-norm, 1 = the natural norm of the Jacobi preconditioner,
+ * sqrt(r^T D^-1 r) with D the diagonal of the edge system (PETSc's
+ * KSP_NORM_NATURAL; the reference's default KSP CG tests a preconditioned
+ * norm, solve.cc:370).  On strongly heterogeneous K the natural norm tracks the
+ * flux error far better than the 2-norm.  Residuals reported back are in the
+ * same norm. */
 typedef struct {
     double r_tol;
     double a_tol;
     int max_it;
     int precision;
+    int norm;
 } fl_solver_opts;
 
 /* Result of a solve: CG iterations, final residual 2-norm, rhs 2-norm,
@@ -201,7 +232,9 @@ typedef struct {
     int reliable_updates;
 } fl_solve_info;
 
-/* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433). */
+/* DarcyFlowMH_Steady::update_solution (darcy_flow_mh.cc:375-433).  Returns
+ * FL_NOT_CONVERGED with info->converged = 0 at the iteration limit,
+ * FL_E_SINGULAR_ELEMENT or FL_E_DIVERGED as described at the top. */
 int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *opts, fl_solve_info *info);
 
 /* DarcyFlowMH_Unsteady (lumped = 0) or DarcyFlowLMH_Unsteady (lumped = 1)

@@ -782,6 +782,7 @@ static CSR axpy_union(const CSR &C, double alpha, const CSR &X, double beta) {
 struct Darcy {
     Mesh *m;
     int E, S, F, N;
+    int bad = -1;  // first element with a singular local matrix (assembly)
     std::vector<double> K, cs, sigma, f, aniso;  // per element
     std::vector<int> bc_type;                     // per bc element (0 none,1 dir,2 neu,3 robin)
     std::vector<double> bc_p, bc_flux, bc_sigma;  // per bc element
@@ -817,7 +818,9 @@ static void assemble(Darcy &D) {
         const Element &el = m.element[e];
         int ns = el.n_sides();
         int el_row = S + e;
-        local_matrix(m, el, D.K[e], D.cs[e], D.aniso.empty() ? nullptr : &D.aniso[9 * e], loc, inv);
+        const double det = local_matrix(m, el, D.K[e], D.cs[e], D.aniso.empty() ? nullptr : &D.aniso[9 * e], loc, inv);
+        // MHFEValues::update (mh_fe_values.cc:54-59): a singular local matrix is fatal
+        if (std::fabs(det) < NUM_ZERO && D.bad < 0) D.bad = e;
         double csec = D.cs[e];
         double ec[3];
         elem_centre(m, el, ec);
@@ -940,6 +943,161 @@ static int cg(const CSR &A, const double *b, double *x, double rtol, double atol
     return it;
 }
 
+// CPU baseline variants of the KSP solve (timing only; the parity tests use
+// the serial Jacobi-CG above).  The reference's serial default is CG with
+// PETSc ILU(3) (src/la/solve.cc:370); PETSc is not available here, and on
+// this SPD system the closest preconditioner with a textbook definition is
+// IC(0): the incomplete Cholesky factor on the lower pattern of A (no fill),
+// applied by a forward and a backward triangular solve.
+static int g_threads = 1;
+
+// Jacobi-CG as `cg`, each loop split over g_threads OpenMP threads (the dot
+// products then sum in a thread-dependent order).
+static int cg_threads(const CSR &A, const double *b, double *x, double rtol, double atol, int max_it, double &res) {
+    const int n = A.nr, nt = g_threads;
+    std::vector<double> r(n), z(n), p(n), q(n), dinv(n);
+    for (int i = 0; i < n; i++) {
+        double d = A.get(i, i);
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    }
+    auto mv = [&](const double *v, double *out) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int i = 0; i < n; i++) {
+            double t = 0.0;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++) t += A.val[k] * v[A.col[k]];
+            out[i] = t;
+        }
+    };
+    mv(x, q.data());
+    double bn = 0, rr = 0, rz = 0;
+#pragma omp parallel for num_threads(nt) reduction(+ : bn, rr, rz) schedule(static)
+    for (int i = 0; i < n; i++) {
+        r[i] = b[i] - q[i];
+        z[i] = dinv[i] * r[i];
+        p[i] = z[i];
+        bn += b[i] * b[i];
+        rr += r[i] * r[i];
+        rz += r[i] * z[i];
+    }
+    bn = std::sqrt(bn);
+    const double tol = std::max(rtol * bn, atol);
+    int it = 0;
+    res = std::sqrt(rr);
+    while (res > tol && it < max_it) {
+        mv(p.data(), q.data());
+        double pq = 0;
+#pragma omp parallel for num_threads(nt) reduction(+ : pq) schedule(static)
+        for (int i = 0; i < n; i++) pq += p[i] * q[i];
+        const double alpha = rz / pq;
+        double rr2 = 0, rz2 = 0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rr2, rz2) schedule(static)
+        for (int i = 0; i < n; i++) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            z[i] = dinv[i] * r[i];
+            rr2 += r[i] * r[i];
+            rz2 += r[i] * z[i];
+        }
+        const double beta = rz2 / rz;
+        rz = rz2;
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        res = std::sqrt(rr2);
+        it++;
+    }
+    return it;
+}
+
+// IC(0)-preconditioned CG (serial).  L (lower, with diagonal) on the pattern of
+// the lower triangle of A: l_ij = (a_ij - sum_k<j l_ik l_jk) / l_jj,
+// l_ii = sqrt(a_ii - sum_k<i l_ik^2); a non-positive pivot falls back to a_ii.
+static int cg_ic0(const CSR &A, const double *b, double *x, double rtol, double atol, int max_it, double &res) {
+    const int n = A.nr;
+    std::vector<int> lp(n + 1, 0), lc;
+    std::vector<double> lv;
+    for (int i = 0; i < n; i++) {
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+            if (A.col[k] <= i) {
+                lc.push_back(A.col[k]);
+                lv.push_back(A.val[k]);
+            }
+        lp[i + 1] = (int)lc.size();
+    }
+    // row-oriented factorization: rows are sorted, the diagonal is each row's last entry
+    std::vector<int> pos(n, -1);
+    for (int i = 0; i < n; i++) {
+        for (int k = lp[i]; k < lp[i + 1]; k++) pos[lc[k]] = k;
+        for (int k = lp[i]; k < lp[i + 1] - 1; k++) {
+            const int j = lc[k];
+            double s = lv[k];
+            for (int m = lp[j]; m < lp[j + 1] - 1; m++) {
+                const int c = lc[m];
+                if (pos[c] >= lp[i] && pos[c] < k) s -= lv[pos[c]] * lv[m];
+            }
+            lv[k] = s / lv[lp[j + 1] - 1];
+        }
+        const int d = lp[i + 1] - 1;
+        double s = lv[d];
+        for (int k = lp[i]; k < d; k++) s -= lv[k] * lv[k];
+        lv[d] = s > 0.0 ? std::sqrt(s) : std::sqrt(std::fabs(A.get(i, i)));
+        for (int k = lp[i]; k < lp[i + 1]; k++) pos[lc[k]] = -1;
+    }
+    // transpose of L for the backward solve
+    std::vector<int> up(n + 1, 0), uc(lc.size());
+    std::vector<double> uv(lc.size());
+    for (int k = 0; k < (int)lc.size(); k++) up[lc[k] + 1]++;
+    for (int i = 0; i < n; i++) up[i + 1] += up[i];
+    {
+        std::vector<int> fill(up.begin(), up.end() - 1);
+        for (int i = 0; i < n; i++)
+            for (int k = lp[i]; k < lp[i + 1]; k++) {
+                const int j = lc[k], w = fill[j]++;
+                uc[w] = i;
+                uv[w] = lv[k];
+            }
+    }
+    auto prec = [&](const double *rv, double *zv) {
+        for (int i = 0; i < n; i++) {  // L y = r
+            double t = rv[i];
+            for (int k = lp[i]; k < lp[i + 1] - 1; k++) t -= lv[k] * zv[lc[k]];
+            zv[i] = t / lv[lp[i + 1] - 1];
+        }
+        for (int i = n - 1; i >= 0; i--) {  // L^T z = y; row i of L^T starts with its diagonal
+            double t = zv[i];
+            for (int k = up[i] + 1; k < up[i + 1]; k++) t -= uv[k] * zv[uc[k]];
+            zv[i] = t / uv[up[i]];
+        }
+    };
+    std::vector<double> r(n), z(n), p(n), q(n);
+    spmv(A, x, q.data());
+    double bn = 0, rr = 0;
+    for (int i = 0; i < n; i++) { r[i] = b[i] - q[i]; bn += b[i] * b[i]; rr += r[i] * r[i]; }
+    bn = std::sqrt(bn);
+    const double tol = std::max(rtol * bn, atol);
+    prec(r.data(), z.data());
+    double rz = 0;
+    for (int i = 0; i < n; i++) { p[i] = z[i]; rz += r[i] * z[i]; }
+    int it = 0;
+    res = std::sqrt(rr);
+    while (res > tol && it < max_it) {
+        spmv(A, p.data(), q.data());
+        double pq = 0;
+        for (int i = 0; i < n; i++) pq += p[i] * q[i];
+        const double alpha = rz / pq;
+        rr = 0;
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; rr += r[i] * r[i]; }
+        prec(r.data(), z.data());
+        double rz_new = 0;
+        for (int i = 0; i < n; i++) rz_new += r[i] * z[i];
+        const double beta = rz_new / rz;
+        rz = rz_new;
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        res = std::sqrt(rr);
+        it++;
+    }
+    return it;
+}
+
 // Dense LU with partial pivoting (brute-force check on tiny systems).
 static bool dense_solve(std::vector<double> A, std::vector<double> b, int n, std::vector<double> &x) {
     for (int k = 0; k < n; k++) {
@@ -1009,6 +1167,9 @@ using namespace orc;
 extern "C" {
 
 const char *orc_last_error() { return g_err.c_str(); }
+
+// threads of the Jacobi-CG in orc_darcy_solve (CPU baseline timing; 1 = the serial restatement)
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 void *orc_mesh_load(const char *path) {
     Mesh *m = new Mesh();
@@ -1175,7 +1336,12 @@ static void diag_apply(Darcy &D, const std::vector<double> &d, bool add) {
 int orc_darcy_solve(void *h, int method, double rtol, double atol, int max_it) {
     Darcy &D = *(Darcy *)h;
     double t0 = now_s();
+    D.bad = -1;
     assemble(D);
+    if (D.bad >= 0) {
+        g_err = "Singular local matrix of the element " + std::to_string(D.m->element[D.bad].id);
+        return -1;
+    }
     double t1 = now_s();
     D.t_assembly = t1 - t0;
     D.sol.assign(D.N, 0.0);
@@ -1204,6 +1370,10 @@ static int solve_assembled(Darcy &D, int method, double rtol, double atol, int m
             for (int k = D.C2.ptr[i]; k < D.C2.ptr[i + 1]; k++) A[(size_t)i * F + D.C2.col[k]] = D.C2.val[k];
         if (!dense_solve(A, D.rhsC2, F, lam)) { g_err = "singular"; return -1; }
         D.iters = 0;
+    } else if (method == 3) {
+        D.iters = cg_ic0(D.C2, D.rhsC2.data(), lam.data(), rtol, atol, max_it, D.resid);
+    } else if (g_threads > 1) {
+        D.iters = cg_threads(D.C2, D.rhsC2.data(), lam.data(), rtol, atol, max_it, D.resid);
     } else {
         D.iters = cg(D.C2, D.rhsC2.data(), lam.data(), rtol, atol, max_it, D.resid);
     }

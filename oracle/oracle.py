@@ -40,6 +40,7 @@ def lib():
         L.orc_darcy_create.restype = C.c_void_p
         L.orc_darcy_create.argtypes = [C.c_void_p, _f64p, _f64p, C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p, _f64p]
         L.orc_darcy_free.argtypes = [C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_darcy_solve.restype = C.c_int
         L.orc_darcy_solve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
         L.orc_darcy_size.argtypes = [C.c_void_p, _i32p]
@@ -154,6 +155,11 @@ class Mesh:
             self.h = None
 
 
+def set_threads(n):
+    """OpenMP threads of the Jacobi-CG (CPU baseline timing only)."""
+    lib().orc_set_threads(int(n))
+
+
 class Darcy:
     """DarcyFlowMH_Steady restated (darcy_flow_mh.cc); BC arrays are per bc element."""
 
@@ -168,6 +174,8 @@ class Darcy:
                                     self._bct, bc_p, bc_flux, bc_sigma)
 
     def solve(self, method=0, rtol=1e-14, atol=0.0, max_it=100000):
+        """method 0: Jacobi-CG (threaded when set_threads(n > 1)), 1: dense LU of
+        the edge system, 2: dense LU of the whole MH system, 3: IC(0)-CG."""
         it = lib().orc_darcy_solve(self.h, method, rtol, atol, max_it)
         if it < 0:
             raise RuntimeError(lib().orc_last_error().decode())

@@ -58,6 +58,8 @@ struct BatchScal {
     double ru_drop2;  // FP32 path: reliable update once ||r||^2 fell below ru_drop2 times its value at the last one
     int beta_fr;      // two-level path: 1 Fletcher-Reeves beta (fixed preconditioner), 0 flexible (Polak-Ribiere)
     int no_pause;     // FP32 two-level path: 1 keeps iterating after the drop, the update replaces r only
+    int nat;          // stop-test norm: 0 ||r||_2, 1 sqrt(r^T D^-1 r) (Jacobi natural norm); rr, tol2,
+                      // bnorm2, rr_ru and rr_hist are squares in that norm
 };
 
 // Batched CG status per realization
@@ -76,7 +78,8 @@ class DarcyBatchGPU {
     void set_conductivity(const double *K, bool device_ptr);
     // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates (FP64 finish
     // for realizations whose FP32 residual stagnates)
-    BatchStats solve(double rtol, double atol, int max_it, int precision);
+    // norm: stop-test residual norm as DarcyGPU::solve
+    BatchStats solve(double rtol, double atol, int max_it, int precision, int norm = 0);
     void get_solution(int r, double *out) const;  // [S+E+F] of realization r
     void diagnostics(int *status, int *n_ru) const;  // [R] each, either may be null
     const double *device_solution() const { return d_sol_; }  // [(S+E+F)][Rout] interleaved

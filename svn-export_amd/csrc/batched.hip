@@ -716,6 +716,7 @@ __global__ void __launch_bounds__(256) k_b_resid(BArgs<V> a, int sel, const int 
                 const size_t o = (size_t)row * a.Rp + r;
                 const double rr = b64[o] - ax;
                 v[1] += rr * rr;
+                if (MODE == 2) v[0] += (double)Pre<V>::ld(a.dinv, o) * rr * rr;
                 if (MODE != 2) {
                     const V rv = (V)rr;
                     const V z = Pre<V>::ld(a.dinv, o) * rv;
@@ -752,7 +753,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
         const size_t o = (size_t)g * Rp + r;
         if (MODE == kFoldAlpha) s[0] += part0[o];
         else {
-            if (MODE != kFoldFinal) s[1] += part1[o];
+            s[1] += part1[o];
             s[2] += part2[o];
         }
     }
@@ -766,14 +767,15 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
     }
     int st = sc.status[r];
     const bool picked = !lsel || lsel[r] != 0;  // Init / Final touch only the selected slots
+    // s[1] = r.z = r^T D^-1 r, s[2] = r.r; nr: the square of the stop-test norm
+    const double nr = sc.nat ? s[1] : s[2];
     if (MODE == kFoldInit) {
         if (picked) {
-            // s[1] = r.z, s[2] = r.r = b.b
-            sc.bnorm2[r] = s[2];
-            sc.tol2[r] = fmax(rtol2 * s[2], atol2);
+            sc.bnorm2[r] = nr;
+            sc.tol2[r] = fmax(rtol2 * nr, atol2);
             sc.rz[r] = s[1];
             sc.rz_prev[r] = 1.0;
-            sc.rr[r] = s[2];
+            sc.rr[r] = nr;
             sc.beta[r] = 0.0;
             sc.alpha[r] = 0.0;
             sc.it[r] = 0;
@@ -781,8 +783,8 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
             sc.it_ru[r] = 0;
             sc.par[r] = 0;
             sc.rr_hist[r] = INFINITY;
-            sc.rr_ru[r] = s[2];
-            st = s[2] <= sc.tol2[r] ? kConverged : kRun;
+            sc.rr_ru[r] = nr;
+            st = nr <= sc.tol2[r] ? kConverged : kRun;
             sc.status[r] = st;
         }
     } else if (MODE == kFoldAlpha) {
@@ -799,7 +801,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
             const int it = sc.it[r] + 1;
             sc.it[r] = it;
             sc.par[r] ^= 1;
-            const double rz = s[1], rr = s[2];
+            const double rz = s[1], rr = nr;
             sc.rz_prev[r] = sc.rz[r];
             sc.rz[r] = rz;
             sc.beta[r] = rz / sc.rz_prev[r];
@@ -812,7 +814,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
         }
     } else if (MODE == kFoldRU) {
         if (ru_pick(sc, r, st)) {
-            const double rz = s[1], rr = s[2];
+            const double rz = s[1], rr = nr;
             sc.n_ru[r] += 1;
             sc.rr_ru[r] = rr;
             sc.it_ru[r] = sc.it[r];
@@ -836,14 +838,14 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
         if (st == kFp64) {
             sc.rz[r] = s[1];
             sc.rz_prev[r] = 1.0;
-            sc.rr[r] = s[2];
+            sc.rr[r] = nr;
             sc.beta[r] = 0.0;
             sc.par[r] = 0;
-            st = s[2] <= sc.tol2[r] ? kConverged : kRun;
+            st = nr <= sc.tol2[r] ? kConverged : kRun;
             sc.status[r] = st;
         }
     } else {  // final true residual
-        if (picked) sc.rr[r] = s[2];
+        if (picked) sc.rr[r] = nr;
     }
     const unsigned long long run = __ballot(st == kRun), f64 = __ballot(st == kFp64);
     if (lane == 0) {
@@ -1735,7 +1737,8 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         for (int p = 0; p < nparts; p++) x += cpart[((size_t)k * nparts + p) * Rp + slot];
         t[k] = x;
     }
-    const double rz = t[0] + t[3], rr = t[1];
+    // t[0] = r^T D^-1 r (Jacobi part), t[1] = r.r; rr: square of the stop-test norm
+    const double rz = t[0] + t[3], rr = sc.nat ? t[0] : t[1];
     int st = st0;
     if (MODE == kFoldInit) {
         sc.bnorm2[slot] = rr;
@@ -2414,9 +2417,19 @@ void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
 }
 
 void DarcyBatchGPU::set_conductivity(const double *K, bool device_ptr) {
+    have_K_ = false;
     FL_HIP(hipMemcpyAsync(d_Kall_, K, (size_t)R_ * E_ * sizeof(double),
                           device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
-    FL_HIP(hipStreamSynchronize(stream_));
+    const long i = first_not_positive_device(d_Kall_, (long)R_ * E_, base_.check_flag(), stream_);
+    if (i >= 0) {
+        double x = 0.0;
+        FL_HIP(hipMemcpy(&x, d_Kall_ + i, sizeof(double), hipMemcpyDeviceToHost));
+        char buf[64];
+        snprintf(buf, sizeof buf, "%.17g", x);
+        throw Error(kErrBadField, "conductivity of realization " + std::to_string(i / E_) + ", element " +
+                                      std::to_string(base_.mesh().el_id[i % E_]) + " is not finite and positive: " +
+                                      buf);
+    }
     have_K_ = true;
 }
 
@@ -2435,10 +2448,12 @@ void DarcyBatchGPU::setup_system() {
     hipLaunchKernelGGL(k_b_fastrec, dim3(grid_for(F_)), dim3(kBlock), 0, stream_, F_, (const int *)d_rec_,
                        (const float *)d_tab32_, d_frec_);
     FL_HIP(hipGetLastError());
+    FL_HIP(hipStreamSynchronize(stream_));
+    base_.check_singular();
 }
 
 
-BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precision) {
+BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precision, int norm) {
     if (!have_K_) throw Error(-4, "batch conductivities not set");
     if (precision != 0 && precision != 1) throw Error(-1, "batch precision must be 0 (FP64) or 1 (FP32 + FP64)");
     if (max_it < 1) throw Error(-1, "max_it must be positive");
@@ -2460,6 +2475,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     sc_.ru_drop2 = sc2_.ru_drop2 = precision == 1 ? ru_drop2_ : 0.0;
     sc_.beta_fr = sc2_.beta_fr = beta_fr_ ? 1 : 0;
     sc_.no_pause = sc2_.no_pause = no_pause_ ? 1 : 0;
+    if (norm != 0 && norm != 1) throw Error(-1, "stop norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
+    sc_.nat = sc2_.nat = norm;
     BArgs<double> a64;
     BArgs<float> a32;
     CArgs ca{};
@@ -2489,7 +2506,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         if (G > 2 * kPre) {  // two-stage: kPre partial rows per slot for the fold proper
             double *o0 = d_pre_, *o1 = d_pre_ + (size_t)kPre * Rp_, *o2 = d_pre_ + 2 * (size_t)kPre * Rp_;
             hipLaunchKernelGGL(k_b_prefold, dim3(kPre, nch_), blk, 0, stream_, mode == kFoldAlpha ? q0 : nullptr,
-                               (mode != kFoldAlpha && mode != kFoldFinal) ? q1 : nullptr,
+                               mode != kFoldAlpha ? q1 : nullptr,
                                mode != kFoldAlpha ? q2 : nullptr, G, Rp_, o0, o1, o2);
             q0 = o0;
             q1 = o1;
@@ -2833,7 +2850,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                 const auto t0 = std::chrono::steady_clock::now();
                 base_.set_conductivity_device(d_Kall_ + (size_t)j.real * E_);
                 const int left = std::max(1, max_it - j.it0);
-                SolveStats ss = base_.solve_warm(j.x0, rtol, atol, left, j.bb);
+                SolveStats ss = base_.solve_warm(j.x0, rtol, atol, left, j.bb, norm);
                 hipLaunchKernelGGL(k_b_put_col, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0,
                                    base_.stream(), n, (size_t)Rout_, j.real, base_.device_solution(), d_sol_);
                 FL_HIP(hipGetLastError());

@@ -1,4 +1,5 @@
 // C ABI over the host mesh module and the HIP solvers (include/flow123d_gpu.h).
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -322,15 +323,32 @@ int fl_darcy_set_conductivity(fl_darcy *d, const double *K) {
 
 static void fill_info(const fl::SolveStats &st, fl_solve_info *info);
 
+// 0 converged; a positive warning when the iteration limit stopped CG (the
+// solution is still the last iterate); an error when the residual is not finite
+static int solve_status(const fl::SolveStats &st) {
+    if (st.converged) return 0;
+    char buf[160];
+    if (!std::isfinite(st.residual)) {
+        snprintf(buf, sizeof buf, "CG diverged: residual %g after %d iterations", st.residual, st.iterations);
+        throw fl::Error(fl::kErrDiverged, buf);
+    }
+    snprintf(buf, sizeof buf, "CG stopped at the iteration limit: %d iterations, residual %.6g of rhs norm %.6g",
+             st.iterations, st.residual, st.bnorm);
+    g_err = buf;
+    return (int)fl::kStatusNotConverged;
+}
+
 int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info *info) {
     return guard([&] {
         double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
         int max_it = o ? o->max_it : 10000;  // solve.cc:69-84 defaults
         int prec = o ? o->precision : 0;
+        const int norm = o ? o->norm : 0;
+        if (norm != 0 && norm != 1) throw fl::Error(-8, "norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
         if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
-        fl::SolveStats st = d->g->solve(rtol, atol, max_it, prec);
+        fl::SolveStats st = d->g->solve(rtol, atol, max_it, prec, norm);
         fill_info(st, info);
-        return 0;
+        return solve_status(st);
     });
 }
 
@@ -346,10 +364,12 @@ int fl_darcy_unsteady_step(fl_darcy *d, double dt, const fl_solver_opts *o, fl_s
         double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
         int max_it = o ? o->max_it : 10000;
         int prec = o ? o->precision : 0;
+        const int norm = o ? o->norm : 0;
+        if (norm != 0 && norm != 1) throw fl::Error(-8, "norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
         if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
-        fl::SolveStats st = d->g->unsteady_step(dt, rtol, atol, max_it, prec);
+        fl::SolveStats st = d->g->unsteady_step(dt, rtol, atol, max_it, prec, norm);
         fill_info(st, info);
-        return 0;
+        return solve_status(st);
     });
 }
 
@@ -486,8 +506,10 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
         double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
         int max_it = o ? o->max_it : 10000;
         int prec = o ? o->precision : 0;
+        const int norm = o ? o->norm : 0;
+        if (norm != 0 && norm != 1) throw fl::Error(-8, "norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
         if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
-        fl::BatchStats st = b->g->solve(rtol, atol, max_it, prec);
+        fl::BatchStats st = b->g->solve(rtol, atol, max_it, prec, norm);
         const int R = b->g->n_real();
         int nconv = 0;
         for (int r = 0; r < R; r++) {
@@ -520,7 +542,12 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->ms_coarse_avg = st.ms_coarse_avg;
             info->ms_coarse_setup = st.ms_coarse_setup;
         }
-        return 0;
+        if (nconv == R) return 0;
+        int ndiv = 0;
+        for (int r = 0; r < R; r++) ndiv += !st.converged[r] && !std::isfinite(st.residual[r]);
+        g_err = std::to_string(R - nconv) + " of " + std::to_string(R) + " realizations did not converge (" +
+                std::to_string(ndiv) + " with a residual that is not finite)";
+        return (int)fl::kStatusNotConverged;
     });
 }
 

@@ -25,9 +25,12 @@ namespace fl {
 constexpr int kCgBlock = 256;
 
 // done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
+// norm: 0 the stop test is on ||r||_2 / ||b||_2; 1 on the natural norm of the
+// Jacobi preconditioner, sqrt(r^T D^-1 r) / sqrt(b^T D^-1 b) (PETSc's
+// KSP_NORM_NATURAL); rr, bnorm2, rr_ru and tol2 are squares in that norm.
 struct CGState {
     double tol2, rr, bnorm2, rr_ru;
-    int done, it, max_it, pad2;
+    int done, it, max_it, norm;
 };
 
 template <class V>
@@ -107,19 +110,21 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_init(CGDev<V> a, V *p0) {
 // bnorm2 > 0: norm^2 of b given (warm start, the initial residual is not b)
 template <class V>
 __global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double rtol2, double atol2, int max_it,
-                                                     double bnorm2) {
-    const double *const parts[1] = {a.part_rr0};
-    double t[1];
-    fold_partials<1>(parts, G, t);
+                                                     double bnorm2, int norm) {
+    const double *const parts[2] = {a.part_rr0, a.part_rz0};
+    double t[2];
+    fold_partials<2>(parts, G, t);
     if (threadIdx.x == 0) {
-        double bb = bnorm2 > 0.0 ? bnorm2 : t[0];
+        const double r0 = norm ? t[1] : t[0];
+        double bb = bnorm2 > 0.0 ? bnorm2 : r0;
         a.st->bnorm2 = bb;
-        a.st->rr = t[0];
+        a.st->rr = r0;
         a.st->tol2 = fmax(rtol2 * bb, atol2);
         a.st->max_it = max_it;
         a.st->it = 0;
         a.st->done = 0;
-        a.st->rr_ru = t[0];
+        a.st->norm = norm;
+        a.st->rr_ru = r0;
     }
 }
 
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
     const double *const parts[3] = {cur_rz, cur_rr, old_rz};
     double s[3];
     fold_partials<3>(parts, G, s);
-    double rz = s[0], rr = s[1];
+    double rz = s[0], rr = st->norm ? s[0] : s[1];
     if (rr <= st->tol2 || k >= st->max_it || !(rr == rr)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->done = 1;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -12,6 +13,32 @@ struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
+
+// Status codes of the C ABI beyond the generic ones (-1 input, -2 HIP, ...).
+// A positive status is a warning: the call completed and its outputs are
+// valid, but a solver stopped before its tolerance (the reference prints
+// PETSc's convergence reason and continues, solve.cc:454-458).
+enum Status : int {
+    kStatusNotConverged = 1,
+    kErrSingularElement = -10,  // |det| of a local RT0 matrix below NUM_ZERO (mh_fe_values.cc:54-59)
+    kErrBadField = -11,         // conductivity / cross-section not finite and positive
+    kErrDiverged = -12,         // CG residual became NaN or infinite
+};
+
+// NUM_ZERO of the reference (math_fce.h:51): the singularity threshold of
+// the small-matrix inverses (math_fce.cc:147,168,211).
+constexpr double kNumZero = DBL_MIN / DBL_EPSILON;
+
+// Index of the first value that is not finite and positive, or -1.
+inline long first_not_positive(const double *v, long n) {
+    for (long i = 0; i < n; i++)
+        if (!(v[i] > 0.0 && v[i] <= DBL_MAX)) return i;
+    return -1;
+}
+
+// The same for a device array (one check kernel + a 4-byte copy back;
+// synchronizes the stream).  `flag` is one device int of scratch.
+long first_not_positive_device(const double *v, long n, int *flag, hipStream_t s);
 
 #define FL_HIP(call)                                                                                   \
     do {                                                                                               \

@@ -3,6 +3,7 @@
 // Schur elimination (assembly_steady_mh_matrix darcy_flow_mh.cc:525-694 +
 // make_schur1/2 :1009-1131).
 #pragma once
+#include "common.hh"
 #include "darcy.hh"
 #include "rt0_local.hh"
 
@@ -36,7 +37,10 @@ __device__ __forceinline__ void element_local(const DarcyDev &dv, int e, double 
     }
     double loc[(D + 1) * (D + 1)];
     local_matrix<D>(el, scale, Rp, loc);
-    small_inverse<D + 1>(loc, L.inv);
+    const double det = small_inverse<D + 1>(loc, L.inv);
+    // a singular local matrix stops the reference (mh_fe_values.cc:54-59); here
+    // the smallest such element is recorded and the host reports it
+    if (dv.bad && !(fabs(det) >= kNumZero)) atomicMin(dv.bad, e);
     V3 ec = elem_centre<D>(el);
     int so = dv.side_off[e];
     double g[D + 1];

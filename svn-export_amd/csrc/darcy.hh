@@ -63,12 +63,19 @@ struct DarcyDev {
     // pressure per element; LMH time coefficient sum |e| S cs / dt / n_sides and
     // previous trace per edge (edge order)
     const double *tc, *pprev, *te, *lprev;
+    // smallest element with a singular local matrix (INT_MAX: none), or null
+    int *bad;
 };
 
 class DarcyGPU {
    public:
     explicit DarcyGPU(const HostMesh &m);
     ~DarcyGPU();
+    // kErrBadField unless every value of v[E] (host or device) is finite and > 0
+    void check_field(const char *what, const double *v, bool device) const;
+    // kErrSingularElement if an element's local matrix was singular (resets the flag)
+    void check_singular();
+    int *check_flag() const { return d_flag_ + 1; }  // one device int of scratch for field checks
     // per-element fields [E] (aniso: 9*E or null), per-bc-element BC data [NBC]
     void set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
                   const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma);
@@ -77,9 +84,11 @@ class DarcyGPU {
     const double *device_K() const { return d_K_; }
     // FP64 CG warm-started from the edge traces x0 (device, edge order); bnorm2
     // = ||b||^2 of the edge system (the stop test is relative to it)
-    SolveStats solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2);
-    // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates
-    SolveStats solve(double rtol, double atol, int max_it, int precision = 0);
+    SolveStats solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2, int norm = 0);
+    // precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates.  norm: the
+    // stop test's residual norm, 0 ||r||_2, 1 sqrt(r^T D^-1 r) (D = diag of the
+    // edge system); the reported residual and rhs norm are in that norm
+    SolveStats solve(double rtol, double atol, int max_it, int precision = 0, int norm = 0);
     void get_solution(double *out) const;                   // [S+E+F]
     void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
     void element_balance(double *out) const;                // [E]
@@ -87,7 +96,7 @@ class DarcyGPU {
     // element pressures), 1 = LMH (on the edge traces); the solution becomes the
     // initial state.  unsteady_step advances it by one implicit Euler step of dt.
     void set_unsteady(int lumped, const double *storativity, const double *init_pressure);
-    SolveStats unsteady_step(double dt, double rtol, double atol, int max_it, int precision = 0);
+    SolveStats unsteady_step(double dt, double rtol, double atol, int max_it, int precision = 0, int norm = 0);
     bool unsteady() const { return unsteady_ != 0; }
     void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region
     void source_balance(double *src) const;                                // per bulk region
@@ -123,7 +132,9 @@ class DarcyGPU {
     double *d_sol_ = nullptr;
     double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
+    int *d_flag_ = nullptr;
     int cg_grid_ = 1024;
+    int stop_norm_ = 0;
     int ru_every_ = 100;
     float *d_val32_ = nullptr, *d_dinv32_ = nullptr, *d_x32_ = nullptr, *d_r32_ = nullptr, *d_z32_ = nullptr;
     float *d_q32_ = nullptr, *d_p32a_ = nullptr, *d_p32b_ = nullptr;

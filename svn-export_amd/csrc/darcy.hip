@@ -2,6 +2,7 @@
 // functions each phase replaces.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -208,6 +209,25 @@ __global__ void k_gather_x0(const DarcyDev dv, const double *__restrict__ x0, do
 __global__ void k_scatter_lambda(const DarcyDev dv, const double *__restrict__ x) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < dv.F; f += gridDim.x * blockDim.x)
         dv.sol[dv.S + dv.E + dv.rperm[f]] = x[f];
+}
+
+// smallest index i with v[i] not finite and positive into *flag (INT_MAX: none)
+__global__ void k_first_not_positive(const double *__restrict__ v, long n, int *flag) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        if (!(v[i] > 0.0 && v[i] <= DBL_MAX)) atomicMin(flag, (int)std::min(i, (long)INT_MAX - 1));
+}
+
+__global__ void k_flag_reset(int *flag) { *flag = INT_MAX; }
+
+long first_not_positive_device(const double *v, long n, int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(1), 0, s, flag);
+    if (n > 0)
+        hipLaunchKernelGGL(k_first_not_positive, dim3(std::min(grid_for(n), 4096)), dim3(kBlock), 0, s, v, n, flag);
+    FL_HIP(hipGetLastError());
+    int h = INT_MAX;
+    FL_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    return h == INT_MAX ? -1 : (long)h;
 }
 
 // ---------------------------------------------------------------------------
@@ -453,6 +473,10 @@ void DarcyGPU::build_pattern() {
     d_part_ = alloc_d(5 * (size_t)cg_grid_);
     d_state_ = dev_alloc<CGState>(1);
     owned_.push_back(d_state_);
+    d_flag_ = dev_alloc<int>(2);  // [0] singular-element flag, [1] field-check scratch
+    owned_.push_back(d_flag_);
+    hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(1), 0, s, d_flag_);
+    d.bad = d_flag_;
     d_K_ = alloc_d(E); d_cs_ = alloc_d(E); d_sigma_ = alloc_d(E); d_f_ = alloc_d(E);
     d_bc_type_ = dev_alloc<int>(std::max(NBC, 1));
     owned_.push_back(d_bc_type_);
@@ -462,9 +486,35 @@ void DarcyGPU::build_pattern() {
     FL_HIP(hipStreamSynchronize(s));
 }
 
+void DarcyGPU::check_field(const char *what, const double *v, bool device) const {
+    const int E = m_.n_elements();
+    const long i = device ? first_not_positive_device(v, E, d_flag_ + 1, stream_) : first_not_positive(v, E);
+    if (i < 0) return;
+    double x = 0.0;
+    if (device) FL_HIP(hipMemcpy(&x, v + i, sizeof(double), hipMemcpyDeviceToHost));
+    else x = v[i];
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.17g", x);
+    throw Error(kErrBadField, std::string(what) + " of the element " + std::to_string(m_.el_id[i]) +
+                                  " is not finite and positive: " + buf);
+}
+
+void DarcyGPU::check_singular() {
+    int h = INT_MAX;
+    FL_HIP(hipMemcpyAsync(&h, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    if (h == INT_MAX) return;
+    hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(1), 0, stream_, d_flag_);
+    FL_HIP(hipStreamSynchronize(stream_));
+    throw Error(kErrSingularElement, "Singular local matrix of the element " + std::to_string(m_.el_id[h]) +
+                                         " (|det| < NUM_ZERO or not finite)");
+}
+
 void DarcyGPU::set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
                         const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma) {
     const int E = m_.n_elements(), NBC = m_.n_bc_elements();
+    check_field("conductivity", K, false);
+    check_field("cross_section", cs, false);
     hipStream_t s = stream_;
     FL_HIP(hipMemcpyAsync(d_K_, K, E * sizeof(double), hipMemcpyHostToDevice, s));
     FL_HIP(hipMemcpyAsync(d_cs_, cs, E * sizeof(double), hipMemcpyHostToDevice, s));
@@ -494,17 +544,20 @@ void DarcyGPU::set_data(const double *K, const double *cs, const double *aniso, 
 }
 
 void DarcyGPU::set_conductivity(const double *K) {
+    check_field("conductivity", K, false);
     FL_HIP(hipMemcpyAsync(d_K_, K, m_.n_elements() * sizeof(double), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
 }
 
 void DarcyGPU::set_conductivity_device(const double *dK) {
+    check_field("conductivity", dK, true);
     FL_HIP(hipMemcpyAsync(d_K_, dK, m_.n_elements() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
 }
 
-SolveStats DarcyGPU::solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2) {
+SolveStats DarcyGPU::solve_warm(const double *d_x0, double rtol, double atol, int max_it, double bnorm2, int norm) {
     if (!have_data_) throw Error(-4, "Darcy data not set");
+    stop_norm_ = norm;
     SolveStats st;
     auto t0 = std::chrono::steady_clock::now();
     assemble();
@@ -532,6 +585,7 @@ void DarcyGPU::assemble() {
     int g2 = std::min(grid_for(d.F), 8192);
     hipLaunchKernelGGL(k_gather_rhs, dim3(g2), dim3(kBlock), 0, stream_, d);
     FL_HIP(hipGetLastError());
+    check_singular();
 }
 
 int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm, double bnorm2) {
@@ -551,7 +605,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
     if (warm) hipLaunchKernelGGL(k_cg_init_warm, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
     else hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
     hipLaunchKernelGGL(k_cg_setup<double>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
-                       max_it, warm ? bnorm2 : 0.0);
+                       max_it, warm ? bnorm2 : 0.0, stop_norm_);
     double *pin = d_p0_, *pout = d_p1_;
     CGState h{};
     int launched = 0;
@@ -651,7 +705,7 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
     };
     reliable_update(0);  // r = b, with the true FP64 norms
     hipLaunchKernelGGL(k_cg_setup<float>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
-                       max_it, 0.0);
+                       max_it, 0.0, stop_norm_);
     CGState h{};
     int k = 0, last_ru = 0, chunk = 32, n_ru = 1;
     const double delta2 = 0.01;  // true residual recomputed after every 10x drop
@@ -763,8 +817,10 @@ void DarcyGPU::resolve() {
     FL_HIP(hipGetLastError());
 }
 
-SolveStats DarcyGPU::solve(double rtol, double atol, int max_it, int precision) {
+SolveStats DarcyGPU::solve(double rtol, double atol, int max_it, int precision, int norm) {
     if (!have_data_) throw Error(-4, "Darcy data not set");
+    if (norm != 0 && norm != 1) throw Error(-1, "stop norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
+    stop_norm_ = norm;
     SolveStats st;
     hipEvent_t e0, e1, e2, e3;
     FL_HIP(hipEventCreate(&e0)); FL_HIP(hipEventCreate(&e1)); FL_HIP(hipEventCreate(&e2)); FL_HIP(hipEventCreate(&e3));
@@ -847,7 +903,7 @@ void DarcyGPU::set_time_coefficients(double dt) {
 // DarcyFlowMH::update_solution for the unsteady models: modify_system
 // (darcy_flow_mh.cc:1660-1677 / 1751-1770), both Schur stages, the solve and,
 // for LMH, postprocess (:1772-1827)
-SolveStats DarcyGPU::unsteady_step(double dt, double rtol, double atol, int max_it, int precision) {
+SolveStats DarcyGPU::unsteady_step(double dt, double rtol, double atol, int max_it, int precision, int norm) {
     if (!unsteady_) throw Error(-4, "unsteady model not set");
     if (!(dt > 0.0)) throw Error(-3, "time step must be positive");
     if (dt != dt_) set_time_coefficients(dt);
@@ -861,7 +917,7 @@ SolveStats DarcyGPU::unsteady_step(double dt, double rtol, double atol, int max_
     }
     SolveStats st;
     try {
-        st = solve(rtol, atol, max_it, precision);
+        st = solve(rtol, atol, max_it, precision, norm);
         if (unsteady_ == 2) {
             hipLaunchKernelGGL(k_lmh_post, dim3(std::min(grid_for(E), 4096)), dim3(kBlock), 0, stream_, dv_,
                                (const double *)d_meas_, (const double *)d_stor_, dt);

@@ -27,7 +27,7 @@ int main(int argc, char **argv) {
         fl::OldBcdInput bcd(mesh);
         bcd.read_flow(argv[2], d);
         std::vector<double> bc_conc = bcd.read_transport(argv[3], NS);
-        fl::DarcyFlowMH_Steady darcy(mesh, d, fl_solver_opts{1e-14, 1e-14, 100000, 0});
+        fl::DarcyFlowMH_Steady darcy(mesh, d, fl_solver_opts{1e-14, 1e-14, 100000, 0, 0});
         darcy.update_solution();
         std::vector<double> init((size_t)NS * E);
         for (int s = 0; s < NS; s++)

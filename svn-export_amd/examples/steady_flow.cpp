@@ -48,7 +48,7 @@ int main(int argc, char **argv) {
             if (bcr[b] == first) { d.bc_type[b] = 1; d.bc_pressure[b] = 1.0; }
             if (bcr[b] == second) { d.bc_type[b] = 1; d.bc_pressure[b] = 0.0; }
         }
-        fl::DarcyFlowMH_Steady darcy(mesh, d, fl_solver_opts{1e-12, 0.0, 100000, 0});
+        fl::DarcyFlowMH_Steady darcy(mesh, d, fl_solver_opts{1e-12, 0.0, 100000, 0, 0});
         darcy.update_solution();
         double *sol;
         unsigned n;

@@ -32,7 +32,7 @@ struct GpuError : std::runtime_error {
 };
 
 inline void check(int rc, const char *what) {
-    if (rc != 0) throw GpuError(std::string(what) + ": " + fl_last_error());
+    if (rc < 0) throw GpuError(std::string(what) + ": " + fl_last_error());
 }
 
 class GpuMesh {
@@ -68,7 +68,7 @@ struct DarcyData {
 
 class DarcyFlowMH_Steady {
    public:
-    DarcyFlowMH_Steady(GpuMesh &mesh, const DarcyData &data, fl_solver_opts solver = {1e-7, 1e-9, 10000, 0})
+    DarcyFlowMH_Steady(GpuMesh &mesh, const DarcyData &data, fl_solver_opts solver = {1e-7, 1e-9, 10000, 0, 0})
         : mesh_(mesh), solver_(solver) {
         check(fl_darcy_create(mesh.handle(), &d_), "darcy create");
         check(fl_darcy_set_data(d_, data.conductivity.data(), data.cross_section.data(),
@@ -84,6 +84,8 @@ class DarcyFlowMH_Steady {
     ~DarcyFlowMH_Steady() { fl_darcy_free(d_); }
 
     // darcy_flow_mh.cc:375-433
+    // like the reference (solve.cc:454-458) an iteration limit is reported, not
+    // fatal: last_solve().converged tells, the solution is the last iterate
     void update_solution() {
         check(fl_darcy_update_solution(d_, &solver_, &last_), "update_solution");
         check(fl_darcy_get_solution(d_, solution_.data()), "get_solution");
@@ -131,7 +133,7 @@ class DarcyFlowUnsteady : public DarcyFlowMH_Steady {
    public:
     DarcyFlowUnsteady(GpuMesh &mesh, const DarcyData &data, bool lumped, const std::vector<double> &storativity,
                       const std::vector<double> &init_pressure, double dt,
-                      fl_solver_opts solver = {1e-7, 1e-9, 10000, 0})
+                      fl_solver_opts solver = {1e-7, 1e-9, 10000, 0, 0})
         : DarcyFlowMH_Steady(mesh, data, solver), dt_(dt) {
         check(fl_darcy_set_unsteady(d_, lumped ? 1 : 0, storativity.data(), init_pressure.data()), "set_unsteady");
         check(fl_darcy_get_solution(d_, solution_.data()), "get_solution");

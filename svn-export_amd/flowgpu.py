@@ -26,7 +26,8 @@ class FlowError(RuntimeError):
 
 
 class SolverOpts(C.Structure):
-    _fields_ = [("r_tol", C.c_double), ("a_tol", C.c_double), ("max_it", C.c_int), ("precision", C.c_int)]
+    _fields_ = [("r_tol", C.c_double), ("a_tol", C.c_double), ("max_it", C.c_int), ("precision", C.c_int),
+                ("norm", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -173,9 +174,21 @@ def lib():
     return _LIB
 
 
+# status codes (include/flow123d_gpu.h)
+NOT_CONVERGED = 1
+E_SINGULAR_ELEMENT = -10
+E_BAD_FIELD = -11
+E_DIVERGED = -12
+
+
 def _check(rc):
-    if rc != 0:
-        raise FlowError("flow123d_gpu error %d: %s" % (rc, lib().fl_last_error().decode()))
+    """Raises on a negative status; a positive one (a solver stopped at its
+    iteration limit, the result still valid) is returned to the caller."""
+    if rc < 0:
+        err = FlowError("flow123d_gpu error %d: %s" % (rc, lib().fl_last_error().decode()))
+        err.status = rc
+        raise err
+    return rc
 
 
 def _f64(a, n=None):
@@ -426,12 +439,14 @@ class DarcyFlowMH_Steady:
     def set_conductivity(self, K):
         _check(lib().fl_darcy_set_conductivity(self.h, _f64(K, self.E)))
 
-    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
-        """precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates."""
-        o = SolverOpts(r_tol, a_tol, max_it, precision)
+    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0, norm=0):
+        """precision 0: FP64 CG; 1: FP32 CG with FP64 reliable updates.
+        norm 0: stop test on ||r||_2; 1: on the natural Jacobi norm sqrt(r^T D^-1 r)."""
+        o = SolverOpts(r_tol, a_tol, max_it, precision, norm)
         info = SolveInfo()
-        _check(lib().fl_darcy_update_solution(self.h, C.byref(o), C.byref(info)))
+        st = _check(lib().fl_darcy_update_solution(self.h, C.byref(o), C.byref(info)))
         self.info = info.as_dict()
+        self.info["status"] = st
         return self.info
 
     def set_unsteady(self, lumped, storativity, init_pressure):
@@ -441,12 +456,13 @@ class DarcyFlowMH_Steady:
         self._unst = [_f64(storativity, E), _f64(init_pressure, E)]
         _check(lib().fl_darcy_set_unsteady(self.h, 1 if lumped else 0, *self._unst))
 
-    def unsteady_step(self, dt, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
+    def unsteady_step(self, dt, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0, norm=0):
         """One implicit Euler step of length dt (darcy_flow_mh.cc:375-433, 1660-1827)."""
-        o = SolverOpts(r_tol, a_tol, max_it, precision)
+        o = SolverOpts(r_tol, a_tol, max_it, precision, norm)
         info = SolveInfo()
-        _check(lib().fl_darcy_unsteady_step(self.h, float(dt), C.byref(o), C.byref(info)))
+        st = _check(lib().fl_darcy_unsteady_step(self.h, float(dt), C.byref(o), C.byref(info)))
         self.info = info.as_dict()
+        self.info["status"] = st
         return self.info
 
     def get_solution_vector(self):
@@ -521,15 +537,16 @@ class DarcyBatch:
             Kh = _f64(K, self.n_real * self.E)
             _check(lib().fl_darcy_batch_set_conductivity(self.h, Kh.ctypes.data, 0))
 
-    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0):
-        o = SolverOpts(r_tol, a_tol, max_it, precision)
+    def update_solution(self, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0, norm=0):
+        """norm as DarcyFlowMH_Steady.update_solution"""
+        o = SolverOpts(r_tol, a_tol, max_it, precision, norm)
         info = BatchInfo()
         it = np.zeros(self.n_real, np.int32)
         conv = np.zeros(self.n_real, np.int32)
         res = np.zeros(self.n_real)
-        _check(lib().fl_darcy_batch_update_solution(self.h, C.byref(o), C.byref(info), it, conv, res))
+        st = _check(lib().fl_darcy_batch_update_solution(self.h, C.byref(o), C.byref(info), it, conv, res))
         self.info = info.as_dict()
-        self.info.update(iterations=it, converged=conv, residual=res)
+        self.info.update(iterations=it, converged=conv, residual=res, status=st)
         return self.info
 
     def get_solution_vector(self, r):

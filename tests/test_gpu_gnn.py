@@ -73,17 +73,27 @@ def test_gnn_forward_and_gradient_match_oracle(setup):
     rel = np.linalg.norm(gr - gr_ref) / np.linalg.norm(gr_ref)
     print("loss %.5f oracle %.5f; gradient cosine %.5f rel err %.2e" % (L, L_ref, cos, rel), flush=True)
     assert abs(L - L_ref) < 2e-2 * abs(L_ref)
-    assert cos > 0.99 and rel < 0.15
+    # BF16 operands with FP32 accumulation against the FP64 oracle
+    assert cos > 0.998 and rel < 0.05
 
 
-def test_gnn_training_and_gain(setup):
-    """Train on 64 solver-generated pairs, evaluate the learned coarse column
-    on 32 held-out realizations against the constant-only coarse space."""
-    import torch
+@pytest.fixture(scope="module")
+def trained(setup):
+    """the network trained on the first 64 solver-generated pairs"""
     om, gm, g, b, K, Kd, lam = setup
     net = G.LearnedCoarseBasis(b, seed=7)
     ntr = 64
     losses = net.train(Kd.data_ptr(), lam.data_ptr(), ntr, 40, 8, 3e-3)
+    return net, ntr, losses
+
+
+def test_gnn_training_and_gain(setup, trained):
+    """Trained on 64 solver-generated pairs, the learned coarse column is
+    evaluated on 32 held-out realizations: fewer iterations than the
+    constant-only coarse space, which has fewer than Jacobi."""
+    import torch
+    om, gm, g, b, K, Kd, lam = setup
+    net, ntr, losses = trained
     print("training loss: first %.4f last %.4f" % (losses[0], losses[-1]), flush=True)
     assert losses[-1] < 0.8 * losses[0]
     R = K.shape[0]
@@ -102,8 +112,41 @@ def test_gnn_training_and_gain(setup):
     t0, t1, t2 = (int(i["iterations"][held].sum()) for i in (i0, i1, i2))
     print("held-out iterations: jacobi %d, constant coarse %d, learned coarse %d" % (t0, t1, t2), flush=True)
     assert i2["converged"].all()
-    assert t1 < t0 and t2 < t0
+    assert t1 < t0 and t2 < t1
     b.set_coarse(0)
+
+
+def test_learned_fp32_batch_every_realization_vs_oracle(setup, trained):
+    """The chosen form in full on 64 new realizations: learned coarse column,
+    FP32 CG with FP64 reliable updates, slot refill and narrowed tail; every
+    realization's fluxes, pressures and traces against the FP64 oracle."""
+    import torch
+    om, gm, g, b, K, Kd, lam = setup
+    net, _, _ = trained
+    d = cases.darcy_case("cube", om, seed=1)
+    R = 64
+    KR = cases.mc_conductivity(om, R, seed=31)
+    KRd = torch.tensor(KR, dtype=torch.float64, device="cuda")
+    phi = torch.zeros((R, om.n_edges), dtype=torch.float32, device="cuda")
+    net.infer(KRd.data_ptr(), R, phi.data_ptr())
+    torch.cuda.synchronize()
+    bb = G.DarcyBatch(g, R, slots=64)
+    bb.set_conductivity(KRd.data_ptr())
+    bb.set_coarse_basis(phi.data_ptr())
+    bb.set_coarse(2)
+    info = bb.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=1)
+    assert info["status"] == 0 and info["converged"].all()
+    S, E = om.n_sides, om.n_elements
+    worst = np.zeros(3)
+    for r in range(R):
+        D0 = O.Darcy(om, KR[r], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+        D0.solve(0, rtol=1e-15, max_it=200000)
+        x0, x1 = D0.solution, bb.get_solution_vector(r)
+        for k, (lo, hi) in enumerate(((0, S), (S, S + E), (S + E, len(x0)))):
+            worst[k] = max(worst[k], np.linalg.norm(x1[lo:hi] - x0[lo:hi]) / np.linalg.norm(x0[lo:hi]))
+    print("64 realizations, worst rel err vs oracle: flux %.2e pressure %.2e trace %.2e; iterations mean %.0f max %d"
+          % (*worst, info["iterations"].mean(), info["iterations"].max()), flush=True)
+    assert (worst <= 1e-8).all()
 
 
 def test_gnn_inference_equals_forward(setup):

@@ -229,3 +229,28 @@ def test_transport_sources_balance():
         T2.set_sources(np.zeros(E), np.zeros(E), np.zeros(E))
         T1.update(0.5); T2.update(0.5)
         assert np.array_equal(T1.conc(), T2.conc())
+
+
+def test_cpu_baseline_solvers_agree(tmp_path):
+    """The CPU-baseline variants of the edge solve (IC(0)-CG, threaded
+    Jacobi-CG) reach the dense LU solution of the same edge system; IC(0) needs
+    fewer iterations than Jacobi (a wrong factor would not)."""
+    import meshgen
+    m = meshgen.square_mesh(24)
+    om = O.Mesh(arrays=m)
+    d = cases.darcy_case("square", om, seed=3)
+    args = (om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    D = O.Darcy(*args)
+    D.solve(1)
+    x_lu = D.solution.copy()
+    S = om.n_sides
+    it = {}
+    for meth, th in ((0, 1), (0, 4), (3, 1)):
+        O.set_threads(th)
+        D = O.Darcy(*args)
+        it[(meth, th)] = D.solve(meth, rtol=1e-13)
+        e = np.linalg.norm(D.solution - x_lu) / np.linalg.norm(x_lu)
+        assert e < 1e-10, (meth, th, e)
+    O.set_threads(1)
+    assert it[(3, 1)] < 0.6 * it[(0, 1)]
+    assert abs(it[(0, 4)] - it[(0, 1)]) <= 3
