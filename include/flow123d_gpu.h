@@ -237,6 +237,18 @@ typedef struct {
  * FL_E_SINGULAR_ELEMENT or FL_E_DIVERGED as described at the top. */
 int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *opts, fl_solve_info *info);
 
+/* No counterpart in the reference.  For callers that cannot allocate during
+ * the call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fl_darcy_update_solution in FP64 (opts->precision must
+ * be 0) that allocates nothing and creates no event.  Its CG partial sums and
+ * state live in a file-scope device array of the library, 5 x 8192 + 8
+ * doubles (328 KiB), held by each GPU the library runs on until the process
+ * ends; a handle whose CG grid exceeds 8192 blocks is rejected.  Every call on
+ * a GPU shares that array, so calls that overlap on one GPU race.  The kernel
+ * duration fields of info stay 0.  Same statuses and results as
+ * fl_darcy_update_solution. */
+int fl_darcy_update_solution_noalloc(fl_darcy *d, const fl_solver_opts *opts, fl_solve_info *info);
+
 /* DarcyFlowMH_Unsteady (lumped = 0) or DarcyFlowLMH_Unsteady (lumped = 1)
  * setup_time_term (darcy_flow_mh.cc:1601-1658, 1680-1749) on the data of
  * fl_darcy_set_data: storativity[E], init_pressure[E].  The solution becomes

@@ -352,6 +352,20 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
     });
 }
 
+int fl_darcy_update_solution_noalloc(fl_darcy *d, const fl_solver_opts *o, fl_solve_info *info) {
+    return guard([&] {
+        double rtol = o ? o->r_tol : 1e-7, atol = o ? o->a_tol : 1e-9;
+        int max_it = o ? o->max_it : 10000;
+        const int prec = o ? o->precision : 0;
+        const int norm = o ? o->norm : 0;
+        if (prec != 0) throw fl::Error(-8, "the no-alloc solve runs FP64 CG only (precision 0)");
+        if (norm != 0 && norm != 1) throw fl::Error(-8, "norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
+        fl::SolveStats st = d->g->solve_noalloc(rtol, atol, max_it, norm);
+        fill_info(st, info);
+        return solve_status(st);
+    });
+}
+
 int fl_darcy_set_unsteady(fl_darcy *d, int lumped, const double *storativity, const double *init_pressure) {
     return guard([&] {
         d->g->set_unsteady(lumped, storativity, init_pressure);

@@ -89,6 +89,10 @@ class DarcyGPU {
     // stop test's residual norm, 0 ||r||_2, 1 sqrt(r^T D^-1 r) (D = diag of the
     // edge system); the reported residual and rhs norm are in that norm
     SolveStats solve(double rtol, double atol, int max_it, int precision = 0, int norm = 0);
+    // The FP64 solve with no allocation and no event during the call: the CG
+    // partial sums and state live in a file-scope device array shared by all
+    // handles on the GPU (calls that overlap on one GPU race); no timings.
+    SolveStats solve_noalloc(double rtol, double atol, int max_it, int norm = 0);
     void get_solution(double *out) const;                   // [S+E+F]
     void get_edge_system(int *ptr, int *col, double *val, double *rhs) const;
     void element_balance(double *out) const;                // [E]
@@ -135,6 +139,11 @@ class DarcyGPU {
     int *d_flag_ = nullptr;
     int cg_grid_ = 1024;
     int stop_norm_ = 0;
+    int device_ = 0;
+    // CG partial sums and state of the current solve (null: the handle's own)
+    double *work_part_ = nullptr;
+    CGState *work_state_ = nullptr;
+    bool sample_ = true;  // sample kernel durations with events
     int ru_every_ = 100;
     float *d_val32_ = nullptr, *d_dinv32_ = nullptr, *d_x32_ = nullptr, *d_r32_ = nullptr, *d_z32_ = nullptr;
     float *d_q32_ = nullptr, *d_p32a_ = nullptr, *d_p32b_ = nullptr;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "common.hh"
@@ -231,6 +232,41 @@ long first_not_positive_device(const double *v, long n, int *flag, hipStream_t s
 }
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve (DarcyGPU::solve_noalloc): the five CG
+// partial-sum arrays of up to kNoAllocGrid blocks, then the CG state.  One
+// copy per GPU for the life of the process; every call on a GPU uses it, so
+// calls that overlap on one GPU race.  Each call writes every partial and the
+// state before reading them (k_cg_init, k_cg_setup), so nothing has to start
+// at zero.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocGrid = 8192;
+constexpr size_t kNoAllocDoubles = 5 * (size_t)kNoAllocGrid + 8;
+static_assert(sizeof(CGState) <= 8 * sizeof(double), "CG state must fit the scratch tail");
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64];  // device address of `scratch`, per device
+
+// Device address of `scratch` on `device`, looked up once per device with that
+// device current (the array's name is only a host-side stand-in).
+static double *scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) throw Error(-2, "device index out of range for the no-alloc scratch");
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        FL_HIP(hipGetDevice(&cur));
+        if (cur != device) FL_HIP(hipSetDevice(device));
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            throw Error(-2, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 template <class T>
@@ -241,6 +277,7 @@ static T *up(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s) {
 }
 
 DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
+    FL_HIP(hipGetDevice(&device_));
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     if (const char *g = std::getenv("FL_CG_GRID")) {  // tuning knob: CG grid size (multiple of 8)
         int v = std::atoi(g);
@@ -599,9 +636,11 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
     a.b = dv_.rhs;
     a.x = d_x_; a.r = d_r_; a.z = d_z_; a.q = d_q_;
     const int G = cg_grid_;
-    a.part_pq = d_part_; a.part_rz0 = d_part_ + G; a.part_rz1 = d_part_ + 2 * G;
-    a.part_rr0 = d_part_ + 3 * G; a.part_rr1 = d_part_ + 4 * G;
-    a.st = d_state_;
+    double *part = work_part_ ? work_part_ : d_part_;
+    CGState *state = work_state_ ? work_state_ : d_state_;
+    a.part_pq = part; a.part_rz0 = part + G; a.part_rz1 = part + 2 * G;
+    a.part_rr0 = part + 3 * G; a.part_rr1 = part + 4 * G;
+    a.st = state;
     if (warm) hipLaunchKernelGGL(k_cg_init_warm, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
     else hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, d_p0_);
     hipLaunchKernelGGL(k_cg_setup<double>, dim3(1), dim3(kCgBlock), 0, stream_, a, G, rtol * rtol, atol * atol,
@@ -626,7 +665,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
     while (true) {
         for (int k = 0; k < chunk; k++) {
             int itx = launched + k;
-            bool samp = (itx % kSampleEvery) == 1;
+            bool samp = sample_ && (itx % kSampleEvery) == 1;
             int e0 = 0;
             if (samp) {
                 e0 = next_ev(); next_ev(); next_ev();
@@ -645,7 +684,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
         }
         launched += chunk;
         FL_HIP(hipGetLastError());
-        FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(&h, state, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
         if (h.done || launched > max_it + chunk) break;
         if (chunk < 256) chunk *= 2;
@@ -839,6 +878,35 @@ SolveStats DarcyGPU::solve(double rtol, double atol, int max_it, int precision, 
     FL_HIP(hipEventElapsedTime(&t23, e2, e3));
     st.ms_assembly = t01; st.ms_cg = t12; st.ms_resolve = t23; st.ms_total = t01 + t12 + t23;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+    return st;
+}
+
+SolveStats DarcyGPU::solve_noalloc(double rtol, double atol, int max_it, int norm) {
+    if (!have_data_) throw Error(-4, "Darcy data not set");
+    if (norm != 0 && norm != 1) throw Error(-1, "stop norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
+    if (cg_grid_ > kNoAllocGrid)
+        throw Error(-1, "CG grid " + std::to_string(cg_grid_) + " exceeds the no-alloc scratch (at most " +
+                            std::to_string(kNoAllocGrid) + " blocks)");
+    double *base = scratch_address(device_);
+    stop_norm_ = norm;
+    work_part_ = base;
+    work_state_ = reinterpret_cast<CGState *>(base + 5 * (size_t)kNoAllocGrid);
+    sample_ = false;
+    SolveStats st;
+    try {
+        assemble();
+        cg(rtol, atol, max_it, st);
+        resolve();
+        FL_HIP(hipStreamSynchronize(stream_));
+    } catch (...) {
+        work_part_ = nullptr;
+        work_state_ = nullptr;
+        sample_ = true;
+        throw;
+    }
+    work_part_ = nullptr;
+    work_state_ = nullptr;
+    sample_ = true;
     return st;
 }
 

@@ -98,6 +98,7 @@ SIGNATURES = {
                                     _f64p]),
     "fl_darcy_set_conductivity": (C.c_int, [C.c_void_p, _f64p]),
     "fl_darcy_update_solution": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
+    "fl_darcy_update_solution_noalloc": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
     "fl_darcy_set_unsteady": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p]),
     "fl_darcy_unsteady_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
     "fl_darcy_get_solution": (C.c_int, [C.c_void_p, _f64p]),
