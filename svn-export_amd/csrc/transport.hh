@@ -7,16 +7,19 @@
 //    PETSc AIJ row) is built on the host once per mesh.
 //  * k_tm_scale: tm := dt*A + I, or (tm - I)*dt'/dt + I on a time-step change
 //    (transport.cc:628-648).
-//  * k_step: c_new = tm * c_old + bcvcorr + sources for all substances at
-//    once; concentrations are substance-interleaved [E][m] so one gathered
-//    column feeds every substance and the matrix row is read once per step.
-//    The sum runs in the row's column order with no fused multiply-add, which
-//    is the sequential MatMultAdd order (transport.cc:668-683).  Each XCD
-//    works on one contiguous eighth of the rows so the gathered columns stay
-//    in its L2.  The same kernel reduces the boundary mass flux and the source
-//    term of the cumulative mass balance (MassBalance::calculate,
-//    mass_balance.cc:79-239) without grid-wide synchronisation: step j's
-//    per-block partials are folded by block 0 of step j+1.
+//  * k_tm_compact: the scaled matrix without its exact zeros (an upwind row
+//    keeps only its inflow entries), compacted per group of 64 rows.
+//  * k_step_c: c_new = tm * c_old + bcvcorr + sources for all substances at
+//    once, one lane per row; concentrations are substance-interleaved [E][m] so
+//    one gathered column feeds every substance and the matrix row is read once
+//    per step.  The sum runs in the row's column order with no fused
+//    multiply-add, which is the sequential MatMultAdd order
+//    (transport.cc:668-683).  Each XCD works on one contiguous eighth of the
+//    rows so the gathered columns stay in its L2.  The same kernel adds each
+//    block's boundary mass flux and source term, times the step length, to the
+//    block's running totals of the cumulative mass balance
+//    (MassBalance::calculate, mass_balance.cc:79-239); they are folded over the
+//    blocks in a fixed order at output times.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -61,8 +64,17 @@ struct TmDev {
     const double *react;                               // [m][m] reaction matrix R[from][to] (when has_react)
     int has_react;
     double *val, *bcv, *partial, *mb;
-    double *ell_val;        // first entries of every row, column-major [5][E]
-    int *ell_col, *ell_len; // their columns [5][E], row lengths [E]
+    // the scaled matrix without its exact zeros (upwind: outflow faces), in
+    // column order per row: the first kEll entries column-major in
+    // ell_val / ell_col [kEll][E] (padding: value 0, own column), the rest of a
+    // longer row compacted inside its group of 64 rows, cval / ccol
+    // [cst[i], cst[i] + cln[i] - kEll); cln[i] = the row's count
+    double *ell_val, *cval;
+    int *ell_col, *ccol, *cst;
+    unsigned char *cln;
+    int *bidx;          // per row: slot of its boundary vector entry (bl_ptr), -1 without boundary sides
+    const int *bl_elem; // element of each representative boundary side
+    int nbl;            // number of representative boundary sides
     unsigned *count;
 };
 
@@ -131,20 +143,20 @@ class TransportGPU {
 
    private:
     void build_matrix();
+    void compact_matrix();
     void set_bc();
     void one_step();
     void mb_initial();
     void flush_flux();
     int step_grid() const;
-    static constexpr int kMaxStepGrid = 16384;
-    static constexpr size_t kPartStride = (size_t)kMaxStepGrid * 2 * kMaxSubst;  // per-step partials
-    bool quad_ = true;  // step kernel with one lane per (row, substance) for 2..4 substances
+    int step_row_blocks() const;
+    static constexpr int kMaxStepGrid = 16384;  // row blocks of the step kernel
+    static constexpr int kMaxBndBlocks = 64;    // boundary-side blocks after them
     double *d_bc_conc_ = nullptr;
     std::vector<double> h_bc_next_;
     bool bc_pending_ = false;
     BcFn bc_fn_ = nullptr;
     void *bc_user_ = nullptr;
-    bool use_quad() const { return quad_ && nsub_ >= 2 && nsub_ <= 4 && !react_on_ && !phases_on() && !rs_n_; }
     int rs_n_ = 0, rs_len_ = 0;   // reaction sorption: species, table length
     const int *d_rs_subst_ = nullptr, *d_el_region_ = nullptr;
     const double *d_rs_par_ = nullptr, *d_rs_table_ = nullptr;  // [nreg*n][8] parameters, [nreg*n][len] tables
@@ -166,10 +178,7 @@ class TransportGPU {
     std::vector<int> r_parent_, r_pptr_, r_prod_;
     std::vector<double> r_hl_, r_branch_, h_react_;
     double *d_react_ = nullptr;
-    bool rows2_ = false; // quad step kernel: two rows per lane group and step
-    bool xcd_split_ = true;  // quad step kernel: each XCD one contiguous eighth of the rows
-    int grid_cap_ = 0;       // step grid limit (0: 8192)
-    double *d_acc_ = nullptr;  // per-block running boundary-flux / source totals of the quad step kernel
+    double *d_acc_ = nullptr;  // per-block running boundary-flux / source totals of the step kernel
     bool acc_dirty_ = false;
 
     const HostMesh &m_;
@@ -179,8 +188,6 @@ class TransportGPU {
     TmDev t_{};
     double *d_c0_ = nullptr, *d_c1_ = nullptr, *d_flux_ = nullptr;
     bool cur_is_c0_ = true;
-    bool pending_ = false, step_parity_ = false;
-    double pending_tdiff_ = 0.0;
     // state
     TimeGovernorR tg_;
     double cfl_max_step_ = 0;

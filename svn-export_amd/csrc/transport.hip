@@ -194,19 +194,54 @@ __global__ void k_tm_scale(const TmDev t, int first, double fac) {
     }
 }
 
-// Column-major copy of the first kEll entries of every row of the scaled
-// matrix (ell_val / ell_col [kEll][E], row length in ell_len), refreshed after
-// every rescale: the step kernel reads it coalesced, without the row-pointer
-// chain; longer rows (fracture and channel elements) take the rest from CSR.
-constexpr int kEll = 5;
-__global__ void k_tm_ell(const TmDev t) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
-        const int k0 = t.ptr[i], n = t.ptr[i + 1] - k0;
-        t.ell_len[i] = n;
-        for (int j = 0; j < kEll; j++) {
-            t.ell_val[(size_t)j * t.E + i] = j < n ? t.val[k0 + j] : 0.0;
-            t.ell_col[(size_t)j * t.E + i] = j < n ? t.col[k0 + j] : i;
+// The scaled matrix without its exact zeros (an upwind row has entries only
+// for its inflow faces; the others are inserted as zeros, transport.cc:851-866).
+// Skipping a term 0 * c leaves every finite sum bitwise unchanged (up to the
+// sign of an exact zero), so the step still reproduces the sequential
+// MatMultAdd.  The first kEll non-zeros of each row go column-major to
+// ell_val / ell_col (read coalesced, with no row-pointer load before them);
+// the rest of a longer row is compacted inside its group of 64 rows: one wave
+// per group, a wave prefix sum of the surplus counts places the rows one after
+// another from the group's first CSR slot, so no grid-wide scan is needed.
+constexpr int kEll = 4;
+__global__ void __launch_bounds__(256) k_tm_compact(const TmDev t) {
+    const int lane = threadIdx.x & 63;
+    const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (g * 64 >= t.E) return;  // wave-uniform
+    const int i = g * 64 + lane;
+    int n = 0, k0 = 0, k1 = 0;
+    if (i < t.E) {
+        k0 = t.ptr[i];
+        k1 = t.ptr[i + 1];
+        for (int k = k0; k < k1; k++) n += t.val[k] != 0.0 ? 1 : 0;
+    }
+    const int extra = n > kEll ? n - kEll : 0;
+    int x = extra;  // inclusive prefix sum over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (i >= t.E) return;
+    int w = t.ptr[g * 64] + x - extra;
+    t.cst[i] = w;
+    t.cln[i] = (unsigned char)n;
+    int u = 0;
+    for (int k = k0; k < k1; k++) {
+        const double v = t.val[k];
+        if (v == 0.0) continue;
+        if (u < kEll) {
+            t.ell_val[(size_t)u * t.E + i] = v;
+            t.ell_col[(size_t)u * t.E + i] = t.col[k];
+        } else {
+            t.cval[w] = v;
+            t.ccol[w] = t.col[k];
+            w++;
         }
+        u++;
+    }
+    for (; u < kEll; u++) {
+        t.ell_val[(size_t)u * t.E + i] = 0.0;
+        t.ell_col[(size_t)u * t.E + i] = i;
     }
 }
 
@@ -231,74 +266,6 @@ __device__ __forceinline__ void fold_flux(const TmDev &t, const double *__restri
     if (threadIdx.x == 0)
         for (int s = 0; s < 2 * M; s++) t.mb[1 + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
     __syncthreads();
-}
-
-// c_new = tm c_old + (bcvcorr + sources) for all M substances; per-block
-// partials of the boundary mass flux (transport.cc:1299-1337) and of the
-// source term (transport.cc:1339-1366) of this step into out_buf; block 0
-// first folds the previous step's partials (fold_buf).
-template <int M>
-__global__ void __launch_bounds__(256) k_step(const TmDev t, const double *__restrict__ c_old,
-                                             double *__restrict__ c_new, double dt,
-                                             const double *__restrict__ fold_buf, double fold_tdiff, int G,
-                                             double *__restrict__ out_buf) {
-    if (fold_buf && blockIdx.x == 0) fold_flux<M>(t, fold_buf, G, fold_tdiff);
-    double bf[2 * M];
-    for (int s = 0; s < 2 * M; s++) bf[s] = 0.0;
-    // XCD-aware rows: block b runs on XCD b % 8, which owns one eighth of the rows
-    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = gridDim.x >> 3;
-    const int per = (t.E + 7) >> 3;
-    const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
-    for (int i = r_lo + lb * blockDim.x + threadIdx.x; i < r_hi; i += nlb * blockDim.x) {
-        double acc[M];
-        const int bb0 = t.bl_ptr[i], bb1 = t.bl_ptr[i + 1];
-        for (int s = 0; s < M; s++) acc[s] = bb1 > bb0 ? t.bcv[(size_t)bb0 * M + s] : 0.0;
-        if (t.has_src) {
-            // compute_concentration_sources (transport.cc:501-544), dt of this step
-            for (int s = 0; s < M; s++) {
-                double dens = t.src_density[(size_t)i * M + s], sig = t.src_sigma[(size_t)i * M + s];
-                double diff = t.src_conc[(size_t)i * M + s] - c_old[(size_t)i * M + s];
-                double snd = diff > 0.0 ? (dens + diff * sig) : dens;
-                acc[s] = acc[s] + snd * dt;
-                bf[M + s] += snd * t.meas[i];
-            }
-        } else {
-            for (int s = 0; s < M; s++) acc[s] = acc[s] + 0.0;
-        }
-        for (int k = t.ptr[i]; k < t.ptr[i + 1]; k++) {
-            double a = t.val[k];
-            const double *cj = c_old + (size_t)t.col[k] * M;
-            for (int s = 0; s < M; s++) acc[s] += a * cj[s];
-        }
-        if (t.has_react) {
-            // Linear_reaction::compute_reaction (linear_reaction.cc:167-183): c_to = sum over
-            // c_from * R[from][to], from in increasing order, after the convection step
-            double r[M];
-            for (int q = 0; q < M; q++) {
-                r[q] = 0.0;
-                for (int s = 0; s < M; s++) r[q] += acc[s] * t.react[s * M + q];
-            }
-            for (int q = 0; q < M; q++) acc[q] = r[q];
-        }
-        for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
-        for (int k = bb0; k < bb1; k++) {
-            double wf = t.flux[t.bl_side[k]];
-            int bce = t.bl_bcele[k];
-            for (int s = 0; s < M; s++)
-                bf[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * c_old[(size_t)i * M + s];
-        }
-    }
-    __shared__ double sh[2 * M][4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int s = 0; s < 2 * M; s++) {
-        double v = bf[s];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if (lane == 0) sh[s][wid] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int s = 0; s < 2 * M; s++)
-            out_buf[(size_t)blockIdx.x * 2 * M + s] = ((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3];
 }
 
 // ConvectionTransport::compute_sorption (transport.cc:1098-1160): the pore mass
@@ -479,125 +446,160 @@ __global__ void __launch_bounds__(256) k_rsorp(int E, int n, const int *__restri
     }
 }
 
-// The same step with one lane per (row, substance): four lanes per row, 64
-// rows per block, so the concentration loads and stores of a wave are 16
-// whole rows (contiguous); the matrix comes from its column-major copy.  Per
-// (row, substance) the arithmetic and its order are those of k_step
-// (bit-identical concentrations).  The boundary-flux and source sums of the
-// block, times this step's length, are added to the block's own running
-// totals acc[block][2M]; k_flux_acc folds those over the blocks at output
-// times, so no block carries a cross-block fold on the step's critical path.
-template <int M, int RW>
-__global__ void __launch_bounds__(256) k_step_q(const TmDev t, const double *__restrict__ c_old,
-                                               double *__restrict__ c_new, double dt, double tdiff,
-                                               double *__restrict__ acc_buf, int xcd_split) {
-    const int s = threadIdx.x & 3;
-    double bfl = 0.0, bsr = 0.0;  // boundary flux, source mass of substance s
-    // rows: each XCD one contiguous eighth (xcd_split), or strided over the grid
-    const int xcd = xcd_split ? (blockIdx.x & 7) : 0, lb = xcd_split ? (blockIdx.x >> 3) : blockIdx.x;
-    const int nlb = xcd_split ? (gridDim.x >> 3) : gridDim.x;
-    const int per = xcd_split ? (t.E + 7) >> 3 : t.E;
-    const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
-    const int stride = nlb * 64;
-    if (s < M) {
-        for (int i0 = r_lo + lb * 64 + (threadIdx.x >> 2); i0 < r_hi; i0 += RW * stride) {
-            double acc[RW];
-            int rows[RW], b0[RW], b1[RW];
+// row of M doubles at p (16-byte aligned for even M) in 16-byte loads
+template <int M>
+__device__ __forceinline__ void load_row(const double *__restrict__ p, double (&v)[M]) {
+    if constexpr (M % 2 == 0) {
+        const double2 *q = reinterpret_cast<const double2 *>(p);
 #pragma unroll
-            for (int u = 0; u < RW; u++) {
-                const int i = i0 + u * stride;
-                rows[u] = i < r_hi ? i : -1;
-                acc[u] = 0.0;
-                b0[u] = b1[u] = 0;
-                if (i >= r_hi) continue;
-                const size_t o = (size_t)i * M + s;
-                b0[u] = t.bl_ptr[i];
-                b1[u] = t.bl_ptr[i + 1];
-                acc[u] = b1[u] > b0[u] ? t.bcv[(size_t)b0[u] * M + s] : 0.0;
-                if (t.has_src) {
-                    const double dens = t.src_density[o], sig = t.src_sigma[o];
-                    const double diff = t.src_conc[o] - c_old[o];
-                    const double snd = diff > 0.0 ? (dens + diff * sig) : dens;
-                    acc[u] = acc[u] + snd * dt;
-                    bsr += snd * t.meas[i];
-                } else {
-                    acc[u] = acc[u] + 0.0;
-                }
-            }
-            // the padded entries of the column-major copy are (0, own row):
-            // adding 0 * c leaves every finite sum unchanged, so the fixed
-            // five-entry loop issues all loads of the RW rows together
-            double av[RW][kEll];
-            int ac[RW][kEll], ln[RW];
-#pragma unroll
-            for (int u = 0; u < RW; u++) {
-                const int i = rows[u] < 0 ? r_lo : rows[u];
-                ln[u] = t.ell_len[i];
-#pragma unroll
-                for (int j = 0; j < kEll; j++) {
-                    av[u][j] = t.ell_val[(size_t)j * t.E + i];
-                    ac[u][j] = t.ell_col[(size_t)j * t.E + i];
-                }
-            }
-            double cv[RW][kEll];
-#pragma unroll
-            for (int u = 0; u < RW; u++)
-#pragma unroll
-                for (int j = 0; j < kEll; j++) cv[u][j] = c_old[(size_t)ac[u][j] * M + s];
-#pragma unroll
-            for (int u = 0; u < RW; u++) {
-                const int i = rows[u];
-                if (i < 0) continue;
-                const int ne = min(ln[u], kEll);
-#pragma unroll
-                for (int j = 0; j < kEll; j++)
-                    if (j < ne) acc[u] += av[u][j] * cv[u][j];
-                if (ln[u] > kEll) {
-                    const int k1 = t.ptr[i + 1];
-                    for (int k = t.ptr[i] + kEll; k < k1; k++) acc[u] += t.val[k] * c_old[(size_t)t.col[k] * M + s];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < RW; u++) {
-                const int i = rows[u];
-                if (i < 0) continue;
-                const size_t o = (size_t)i * M + s;
-                c_new[o] = acc[u];
-                for (int k = b0[u]; k < b1[u]; k++) {
-                    const double wf = t.flux[t.bl_side[k]];
-                    bfl += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + t.bl_bcele[k]] : wf * c_old[o];
-                }
-            }
+        for (int h = 0; h < M / 2; h++) {
+            const double2 x = q[h];
+            v[2 * h] = x.x;
+            v[2 * h + 1] = x.y;
         }
+    } else {
+#pragma unroll
+        for (int s = 0; s < M; s++) v[s] = p[s];
     }
-    for (int m = 4; m < 64; m <<= 1) {
-        bfl += __shfl_xor(bfl, m, 64);
-        bsr += __shfl_xor(bsr, m, 64);
-    }
+}
+
+// ConvectionTransport::compute_one_step (transport.cc:593-715) for all M
+// substances: c_new = tm c_old + (bcvcorr + sources), then the reaction
+// matrix when decays are on (linear_reaction.cc:167-183).  One lane per row
+// carries the row's M concentrations: the row's matrix entries (k_tm_compact)
+// are read once, each gathered column is one contiguous M * 8 B read that
+// feeds every substance, and the sum runs in the row's column order with no
+// fused multiply-add, as the sequential MatMultAdd (transport.cc:668-683).
+// All loads of a row are issued before any is used.  The first `row_blocks`
+// blocks do the rows, each XCD one contiguous eighth of them (block b runs on
+// XCD b % 8), so neighbour gathers stay in its L2; the blocks after them
+// reduce the boundary mass flux over the boundary sides (transport.cc:1299-1337),
+// so the row path carries no boundary loop.  Each block adds its sums (flux,
+// and with sources the source term, transport.cc:1339-1366), times the step
+// length, to its own running totals acc[block][2M]; k_flux_acc folds them over
+// the blocks at output times in a fixed order.
+template <int M>
+__global__ void __launch_bounds__(256) k_step_c(const TmDev t, const double *__restrict__ c_old,
+                                               double *__restrict__ c_new, double dt, double tdiff,
+                                               double *__restrict__ acc_buf, int row_blocks) {
     __shared__ double sh[2 * M][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane < M) {
-        sh[lane][wid] = bfl;
-        sh[M + lane][wid] = bsr;
+    if ((int)blockIdx.x >= row_blocks) {
+        // boundary sides: inflow carries the boundary concentration, outflow the
+        // element's concentration of the previous step
+        double bfl[M];
+#pragma unroll
+        for (int s = 0; s < M; s++) bfl[s] = 0.0;
+        const int nb = gridDim.x - row_blocks;
+        for (int k = (blockIdx.x - row_blocks) * blockDim.x + threadIdx.x; k < t.nbl; k += nb * blockDim.x) {
+            const double wf = t.flux[t.bl_side[k]];
+            const int bce = t.bl_bcele[k];
+            double ce[M];
+            load_row<M>(c_old + (size_t)t.bl_elem[k] * M, ce);
+#pragma unroll
+            for (int s = 0; s < M; s++) bfl[s] += wf < 0 ? wf * t.bc_conc[(size_t)s * t.NBC + bce] : wf * ce[s];
+        }
+#pragma unroll
+        for (int s = 0; s < M; s++) {
+            double v = bfl[s];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) sh[s][wid] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int s = 0; s < M; s++)
+                acc_buf[(size_t)blockIdx.x * 2 * M + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
+        return;
+    }
+    double bsr[M];
+#pragma unroll
+    for (int s = 0; s < M; s++) bsr[s] = 0.0;
+    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nlb = row_blocks >> 3;
+    const int per = (t.E + 7) >> 3;
+    const int r_lo = xcd * per, r_hi = min(t.E, r_lo + per);
+    for (int i = r_lo + lb * (int)blockDim.x + (int)threadIdx.x; i < r_hi; i += nlb * (int)blockDim.x) {
+        const int n = t.cln[i], bi = t.bidx[i];
+        int cc[kEll];
+        double av[kEll];
+#pragma unroll
+        for (int u = 0; u < kEll; u++) {
+            cc[u] = t.ell_col[(size_t)u * t.E + i];
+            av[u] = t.ell_val[(size_t)u * t.E + i];
+        }
+        double acc[M];
+#pragma unroll
+        for (int s = 0; s < M; s++) acc[s] = bi >= 0 ? t.bcv[(size_t)bi * M + s] : 0.0;
+        double cv[kEll][M];
+#pragma unroll
+        for (int u = 0; u < kEll; u++)
+            if (u < n) load_row<M>(c_old + (size_t)cc[u] * M, cv[u]);
+        if (t.has_src) {
+            // compute_concentration_sources (transport.cc:501-544), dt of this step
+            double ci[M];
+            load_row<M>(c_old + (size_t)i * M, ci);
+#pragma unroll
+            for (int s = 0; s < M; s++) {
+                const size_t o = (size_t)i * M + s;
+                const double dens = t.src_density[o], sig = t.src_sigma[o];
+                const double diff = t.src_conc[o] - ci[s];
+                const double snd = diff > 0.0 ? (dens + diff * sig) : dens;
+                acc[s] = acc[s] + snd * dt;
+                bsr[s] += snd * t.meas[i];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < M; s++) acc[s] = acc[s] + 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kEll; u++)
+            if (u < n) {
+#pragma unroll
+                for (int s = 0; s < M; s++) acc[s] += av[u] * cv[u][s];
+            }
+        if (n > kEll) {  // rows of fracture and channel elements
+            const int k0 = t.cst[i], k1 = k0 + n - kEll;
+            for (int k = k0; k < k1; k++) {
+                const double a = t.cval[k];
+                double c2[M];
+                load_row<M>(c_old + (size_t)t.ccol[k] * M, c2);
+#pragma unroll
+                for (int s = 0; s < M; s++) acc[s] += a * c2[s];
+            }
+        }
+        if (t.has_react) {
+            // Linear_reaction::compute_reaction (linear_reaction.cc:167-183): c_to = sum over
+            // c_from * R[from][to], from in increasing order, after the convection step
+            double r[M];
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+                r[q] = 0.0;
+                for (int s = 0; s < M; s++) r[q] += acc[s] * t.react[s * M + q];
+            }
+#pragma unroll
+            for (int q = 0; q < M; q++) acc[q] = r[q];
+        }
+#pragma unroll
+        for (int s = 0; s < M; s++) c_new[(size_t)i * M + s] = acc[s];
+    }
+    if (!t.has_src) return;  // block-uniform
+#pragma unroll
+    for (int s = 0; s < M; s++) {
+        double v = bsr[s];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) sh[s][wid] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 0; k < 2 * M; k++)
-            acc_buf[(size_t)blockIdx.x * 2 * M + k] += (((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3]) * tdiff;
+        for (int s = 0; s < M; s++)
+            acc_buf[(size_t)blockIdx.x * 2 * M + M + s] += (((sh[s][0] + sh[s][1]) + sh[s][2]) + sh[s][3]) * tdiff;
 }
 
-// Output time: the blocks' running totals of k_step_q folded in a fixed order
+// Output time: the blocks' running totals of k_step_c folded in a fixed order
 // into the integrated quantities, then cleared.  One block.
 template <int M>
 __global__ void __launch_bounds__(256) k_flux_acc(const TmDev t, double *__restrict__ acc_buf, int G) {
     fold_flux<M>(t, acc_buf, G, 1.0);
     for (int k = threadIdx.x; k < G * 2 * M; k += blockDim.x) acc_buf[k] = 0.0;
-}
-
-template <int M>
-__global__ void __launch_bounds__(256) k_flux_flush(const TmDev t, const double *__restrict__ buf, int G,
-                                                   double tdiff) {
-    fold_flux<M>(t, buf, G, tdiff);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,10 +616,6 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
                            const double *init_conc, double end_time, double save_step)
     : m_(m), E_(m.n_elements()), nsub_(nsub), NBC_(m.n_bc_elements()) {
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    if (const char *q = std::getenv("FL_TRANSPORT_QUAD")) quad_ = std::atoi(q) != 0;
-    if (const char *q = std::getenv("FL_TRANSPORT_ROWS2")) rows2_ = std::atoi(q) != 0;
-    if (const char *q = std::getenv("FL_TRANSPORT_XCD")) xcd_split_ = std::atoi(q) != 0;
-    if (const char *q = std::getenv("FL_TRANSPORT_GRID")) grid_cap_ = std::max(8, std::atoi(q) / 8 * 8);
     const int E = E_, S = m.n_sides(), F = m.n_edges(), V = m.n_vb();
     // matrix pattern: rows in the reference's insertion order, later entries
     // overwrite earlier ones (INSERT_VALUES), then columns sorted per row.
@@ -669,6 +667,12 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
             }
         bl_ptr[i + 1] = (int)bl_side.size();
     }
+    const int nbl = (int)bl_side.size();
+    std::vector<int> bl_elem(std::max(nbl, 1), 0), bidx(E, -1);
+    for (int i = 0; i < E; i++) {
+        if (bl_ptr[i + 1] > bl_ptr[i]) bidx[i] = bl_ptr[i];
+        for (int k = bl_ptr[i]; k < bl_ptr[i + 1]; k++) bl_elem[k] = i;
+    }
     if (bl_side.empty()) { bl_side.push_back(0); bl_bcele.push_back(0); }
     h_meas_.resize(E);
     for (int i = 0; i < E; i++) h_meas_[i] = element_measure(m, m.el_dim[i], &m.el_nodes[4 * i]);
@@ -693,6 +697,9 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     t.el_vb = upv(owned_, elvb, s);
     t.vb_hside = upv(owned_, vb_hside, s);
     t.bl_ptr = upv(owned_, bl_ptr, s); t.bl_side = upv(owned_, bl_side, s); t.bl_bcele = upv(owned_, bl_bcele, s);
+    t.bl_elem = upv(owned_, bl_elem, s);
+    t.bidx = upv(owned_, bidx, s);
+    t.nbl = nbl;
     t.meas = upv(owned_, h_meas_, s); t.cs = upv(owned_, h_cs_, s); t.por = upv(owned_, h_por_, s);
     std::vector<double> bcc = h_bc_conc_;
     if (bcc.empty()) bcc.push_back(0);
@@ -704,19 +711,27 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     t.flux = d_flux_;
     t.edge_flow = alloc_d(F);
     t.val = alloc_d(nnz_);
+    for (int i = 0; i < E; i++)
+        if (ptr[i + 1] - ptr[i] > 255)
+            throw Error(-5, "transport matrix row of element " + std::to_string(m.el_id[i]) + " has more than 255 entries");
+    t.cval = alloc_d(nnz_);
     t.ell_val = alloc_d((size_t)kEll * E);
     {
-        int *p = dev_alloc<int>(((size_t)kEll + 1) * E);
+        int *p = dev_alloc<int>((size_t)nnz_ + E + (size_t)kEll * E);
         owned_.push_back(p);
-        t.ell_col = p;
-        t.ell_len = p + (size_t)kEll * E;
+        t.ccol = p;
+        t.cst = p + nnz_;
+        t.ell_col = p + nnz_ + E;
+        unsigned char *l = dev_alloc<unsigned char>(E);
+        owned_.push_back(l);
+        t.cln = l;
     }
     t.bcv = alloc_d((size_t)bl_side.size() * nsub);  // compact: one slot group per row with boundary sides
     d_c0_ = alloc_d((size_t)E * nsub);
     d_c1_ = alloc_d((size_t)E * nsub);
-    t.partial = alloc_d(2 * kPartStride + 4096);
-    d_acc_ = alloc_d((size_t)kMaxStepGrid * 8);
-    FL_HIP(hipMemsetAsync(d_acc_, 0, (size_t)kMaxStepGrid * 8 * sizeof(double), s));
+    t.partial = alloc_d(4096);
+    d_acc_ = alloc_d((size_t)(kMaxStepGrid + kMaxBndBlocks) * 2 * kMaxSubst);
+    FL_HIP(hipMemsetAsync(d_acc_, 0, (size_t)(kMaxStepGrid + kMaxBndBlocks) * 2 * kMaxSubst * sizeof(double), s));
     t.mb = alloc_d(1 + 2 * nsub);
     unsigned *cnt = dev_alloc<unsigned>(1);
     owned_.push_back(cnt);
@@ -975,32 +990,33 @@ void TransportGPU::mb_initial() {
     mb_init_ = false;
 }
 
+// one lane per row: enough blocks to cover the rows in one sweep, a multiple
+// of 8 (one share per XCD); then the boundary-side blocks
+int TransportGPU::step_row_blocks() const {
+    const int g = (((E_ + kBlock - 1) / kBlock + 7) / 8) * 8;
+    return std::max(8, std::min(g, kMaxStepGrid));
+}
 int TransportGPU::step_grid() const {
-    const int rows_per_block = use_quad() ? 64 : kBlock;
-    const int cap = grid_cap_ > 0 ? std::min(grid_cap_, kMaxStepGrid) : 8192;
-    return std::max(8, std::min((E_ + rows_per_block - 1) / rows_per_block / 8 * 8, cap));
+    const int nb = t_.nbl > 0 ? std::min((t_.nbl + kBlock - 1) / kBlock, kMaxBndBlocks) : 0;
+    return step_row_blocks() + nb;
 }
 
 void TransportGPU::flush_flux() {
-    if (acc_dirty_) {
-        const int G = step_grid();
-        if (nsub_ == 2) hipLaunchKernelGGL(k_flux_acc<2>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
-        else if (nsub_ == 3) hipLaunchKernelGGL(k_flux_acc<3>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
-        else hipLaunchKernelGGL(k_flux_acc<4>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G);
-        FL_HIP(hipGetLastError());
-        acc_dirty_ = false;
-    }
-    if (!pending_) return;
+    if (!acc_dirty_) return;
     const int G = step_grid();
-    const double *last = t_.partial + (step_parity_ ? 0 : 1) * kPartStride;
     switch (nsub_) {
 #define FL_FLUSH(MM) \
-    case MM: hipLaunchKernelGGL(k_flux_flush<MM>, dim3(1), dim3(kBlock), 0, stream_, t_, last, G, pending_tdiff_); break;
+    case MM: hipLaunchKernelGGL(k_flux_acc<MM>, dim3(1), dim3(kBlock), 0, stream_, t_, d_acc_, G); break;
         FL_FLUSH(1) FL_FLUSH(2) FL_FLUSH(3) FL_FLUSH(4) FL_FLUSH(5) FL_FLUSH(6) FL_FLUSH(7) FL_FLUSH(8)
 #undef FL_FLUSH
     }
     FL_HIP(hipGetLastError());
-    pending_ = false;
+    acc_dirty_ = false;
+}
+
+void TransportGPU::compact_matrix() {
+    const int waves = (E_ + 63) / 64;
+    hipLaunchKernelGGL(k_tm_compact, dim3((waves + 3) / 4), dim3(kBlock), 0, stream_, t_);
 }
 
 // ConvectionTransport::compute_one_step (transport.cc:593-715)
@@ -1037,12 +1053,11 @@ void TransportGPU::one_step() {
         if (scaled_) {
             double fac = tg_.estimate_dt() / tg_.time_step;
             hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 0, fac);
-            hipLaunchKernelGGL(k_tm_ell, dim3(gridS), dim3(kBlock), 0, stream_, t_);
         } else {
             hipLaunchKernelGGL(k_tm_scale, dim3(gridS), dim3(kBlock), 0, stream_, t_, 1, tg_.estimate_dt());
-            hipLaunchKernelGGL(k_tm_ell, dim3(gridS), dim3(kBlock), 0, stream_, t_);
             scaled_ = true;
         }
+        compact_matrix();
         need_rescale_ = false;
     }
     tg_.next_time();
@@ -1051,47 +1066,33 @@ void TransportGPU::one_step() {
     const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
     double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
     const int G = step_grid();
-    double *out = t_.partial + (step_parity_ ? 1 : 0) * kPartStride;
-    const double *fold = pending_ ? t_.partial + (step_parity_ ? 0 : 1) * kPartStride : nullptr;
-    double ftd = pending_tdiff_, dt = tg_.time_step;
-    if (use_quad()) {
-#define FL_STEPQ(MM, RR) \
-    hipLaunchKernelGGL((k_step_q<MM, RR>), dim3(G), dim3(kBlock), 0, stream_, t_, cold, cnew, dt, tdiff, d_acc_, \
-                       xcd_split_ ? 1 : 0)
-        if (rows2_) {
-            if (nsub_ == 2) FL_STEPQ(2, 2);
-            else if (nsub_ == 3) FL_STEPQ(3, 2);
-            else FL_STEPQ(4, 2);
-        } else {
-            if (nsub_ == 2) FL_STEPQ(2, 1);
-            else if (nsub_ == 3) FL_STEPQ(3, 1);
-            else FL_STEPQ(4, 1);
-        }
-#undef FL_STEPQ
-    } else {
-        // with dual porosity / sorption the decays run after them, in k_phases
-        TmDev ts = t_;
-        if (phases_on()) ts.has_react = 0;
-        switch (nsub_) {
+    const double dt = tg_.time_step;
+    // with dual porosity / sorption the decays run after them, in k_phases
+    TmDev ts = t_;
+    if (phases_on()) ts.has_react = 0;
+    switch (nsub_) {
 #define FL_STEP(MM) \
-    case MM: hipLaunchKernelGGL(k_step<MM>, dim3(G), dim3(kBlock), 0, stream_, ts, cold, cnew, dt, fold, ftd, G, out); break;
-            FL_STEP(1) FL_STEP(2) FL_STEP(3) FL_STEP(4) FL_STEP(5) FL_STEP(6) FL_STEP(7) FL_STEP(8)
+    case MM:                                                                                                  \
+        hipLaunchKernelGGL(k_step_c<MM>, dim3(G), dim3(kBlock), 0, stream_, ts, cold, cnew, dt, tdiff, d_acc_,  \
+                           step_row_blocks());                                                                 \
+        break;
+        FL_STEP(1) FL_STEP(2) FL_STEP(3) FL_STEP(4) FL_STEP(5) FL_STEP(6) FL_STEP(7) FL_STEP(8)
 #undef FL_STEP
-            default: throw Error(-5, "transport supports 1..8 substances per handle");
-        }
-        if (phases_on()) {
-            FL_HIP(hipGetLastError());
-            PhaseDev ph{d_imm_, d_msorb_, d_isorb_, d_alpha_, d_sc0_, d_sc1_, d_por_imm_, d_phi_, d_stype_,
-                        dual_ ? 1 : 0, sorp_ ? 1 : 0};
-            switch (nsub_) {
-#define FL_PH(MM) \
-    case MM: hipLaunchKernelGGL(k_phases<MM>, dim3(gridS), dim3(kBlock), 0, stream_, t_, ph, cnew, dt); break;
-                FL_PH(1) FL_PH(2) FL_PH(3) FL_PH(4) FL_PH(5) FL_PH(6) FL_PH(7) FL_PH(8)
-#undef FL_PH
-            }
-        }
+        default: throw Error(-5, "transport supports 1..8 substances per handle");
     }
     FL_HIP(hipGetLastError());
+    acc_dirty_ = true;  // totals stay in the per-block accumulators until the output time
+    if (phases_on()) {
+        PhaseDev ph{d_imm_, d_msorb_, d_isorb_, d_alpha_, d_sc0_, d_sc1_, d_por_imm_, d_phi_, d_stype_,
+                    dual_ ? 1 : 0, sorp_ ? 1 : 0};
+        switch (nsub_) {
+#define FL_PH(MM) \
+    case MM: hipLaunchKernelGGL(k_phases<MM>, dim3(gridS), dim3(kBlock), 0, stream_, t_, ph, cnew, dt); break;
+            FL_PH(1) FL_PH(2) FL_PH(3) FL_PH(4) FL_PH(5) FL_PH(6) FL_PH(7) FL_PH(8)
+#undef FL_PH
+        }
+        FL_HIP(hipGetLastError());
+    }
     if (rs_n_) {
         switch (nsub_) {
 #define FL_RS(MM)                                                                                                 \
@@ -1103,13 +1104,6 @@ void TransportGPU::one_step() {
 #undef FL_RS
         }
         FL_HIP(hipGetLastError());
-    }
-    if (use_quad()) {
-        acc_dirty_ = true;  // totals stay in the per-block accumulators until the output time
-    } else {
-        pending_ = true;
-        pending_tdiff_ = tdiff;
-        step_parity_ = !step_parity_;
     }
     cur_is_c0_ = !cur_is_c0_;
     stats_.steps++;
