@@ -131,6 +131,8 @@ class DarcyBatchGPU {
     DarcyGPU &base_;
     const DarcyDev &dv_;
     hipStream_t stream_ = nullptr;
+    hipStream_t stream2_ = nullptr;  // the operator's general rows, beside its row tiles
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     int R_, Rp_, Rout_, E_, S_, F_, nchunk_;  // Rp_: slots (multiple of 64); Rout_: R rounded up to 64
     long tab_bytes_ = 0, ncontrib_ = 0;
     std::vector<void *> owned_;
@@ -155,6 +157,11 @@ class DarcyBatchGPU {
     int lw_ = 64;               // lanes per row of the two-level iteration kernels (64, 16, 4)
     bool narrow_ = true;        // two-level path: narrow lanes for the tail instead of the hand-over
     bool trace_ = false;        // per-chunk progress lines on stderr
+    // tiled FP32 operator: general rows (vb-coupled elements) and row tiles
+    void build_op_tiles(const std::vector<int> &rec);
+    const int *d_gen_ = nullptr, *d_grec_ = nullptr, *d_thptr_ = nullptr, *d_thalo_ = nullptr, *d_teptr_ = nullptr, *d_tels_ = nullptr;
+    const unsigned short *d_lrec_ = nullptr;
+    int ngen_ = 0, gen_blocks_ = 0, ntiles_ = 0, tile_T_ = 128, nseg_max_ = 0;
     int opfast_ = 2;            // two-level operator: 0 general rows, 1 pair rows, 2 inlined pair records (2 chunks per wave), 3 the same, 1 chunk
     int *d_frec_ = nullptr;     // FP32 pair-row records (20 words per row)
     bool unroll_ = true;        // two-level update: two rows per step

@@ -16,6 +16,7 @@
 #include <mutex>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -31,7 +32,8 @@ namespace {
 constexpr int kWaves = 4;            // waves per block of the row kernels
 constexpr int kAggWaves = 8;         // waves per block of the aggregate-blocked kernels (one aggregate per block)
 constexpr int kFoldThreads = 1024;   // 16 waves per fold block
-constexpr int kMaxGrid = 2048;       // partial arrays are sized for this many blocks
+constexpr int kMaxGrid = 8192;       // partial arrays are sized for this many blocks
+constexpr int kMaxLaunchGrid = 2048; // blocks of the row-strided kernels
 constexpr int kMaxConn = 16;         // element index packing e * 16 + local index
 
 // Per-slot preconditioner arrays of the FP32 path (Jacobi diagonal D^-1, the
@@ -334,26 +336,33 @@ __device__ __forceinline__ V op_row_pair(const int *__restrict__ R, const V *__r
 //   [10..13] P0_e1 row i1, [14..17] P0_e2 row i2 (FP32 bits), [18] dlt_e1[i1], [19] dlt_e2[i2]
 // (the diagonal term multiplies the row's own value).
 constexpr int kFastRec = 20;
+// rows of one or two tetrahedra without vb neighbours take the inlined record
+__host__ __device__ inline bool op_fast_row(const int *R) {
+    return R[0] >= 0 && R[2] == 4 && (R[8] < 0 || R[10] == 4);
+}
 __global__ void k_b_fastrec(int F, const int *__restrict__ rec, const float *__restrict__ opT,
                             int *__restrict__ out) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= F) return;
     const int *R = rec + (size_t)row * kRecInts;
     int *O = out + (size_t)row * kFastRec;
-    if (!(R[0] >= 0 && R[2] == 4 && R[8] >= 0 && R[10] == 4)) {
+    if (!op_fast_row(R)) {
         O[0] = -1;
         return;
     }
     for (int k = 0; k < 2; k++) {
-        const int *Q = R + 8 * k;
+        // a row with one tetrahedron repeats it with a zero block row and a
+        // zero diagonal term: K * (0 x) + 0 x_row adds +0 to the sum
+        const bool one = k == 1 && R[8] < 0;
+        const int *Q = R + (one ? 0 : 8 * k);
         const float *T = opT + Q[3];
         const int i = Q[1];
         O[k] = Q[0];
         for (int j = 0; j < 4; j++) {
             O[2 + 4 * k + j] = Q[4 + j];
-            O[10 + 4 * k + j] = __float_as_int(T[i * 4 + j]);
+            O[10 + 4 * k + j] = one ? __float_as_int(0.0f) : __float_as_int(T[i * 4 + j]);
         }
-        O[18 + k] = __float_as_int(T[16 + i]);
+        O[18 + k] = one ? __float_as_int(0.0f) : __float_as_int(T[16 + i]);
     }
 }
 
@@ -1427,6 +1436,462 @@ __global__ void __launch_bounds__(256) k_c_op2(BArgs<float> a, CArgs c) {
     }
 }
 
+// Tiles of the FP32 two-level operator (k_c_opl).  Rows are cut into tiles of
+// T consecutive rows (the row order is a Z-curve, so a tile is compact in
+// space).  Per tile, the segments staged in LDS: its own rows (local index
+// row - r0), then its halo rows (rows outside the tile that its rows' elements
+// touch, halo[hptr[t] ..]), then the K of its rows' elements (els[eptr[t] ..]).
+// lrec[row][10]: local indices of the row's two elements and of their 2 x 4
+// edge rows (0xFFFF in [0]: a general row, done by the general blocks).
+struct OpTiles {
+    const int *gen;                  // general rows (elements with vb neighbours)
+    const int *grec;                 // their flattened records [ngen][kGenRec]
+    const int *hptr, *halo, *eptr, *els;
+    const unsigned short *lrec;
+    int ngen, gen_blocks, ntiles, T, nseg_max;
+};
+constexpr int kLrec = 10;
+// flattened general-row records: [nC, kGenC x {e, i, n0, nvb, op_off, rows[kGenR]}]
+constexpr int kGenC = 4, kGenR = 6, kGenStride = 5 + kGenR, kGenRec = 48;
+static_assert(1 + kGenC * kGenStride <= kGenRec, "general record size");
+constexpr int kSlotGroup = 32;  // slots per LDS pass (128 B per staged row)
+
+// LDS bytes of k_c_opl: staged segments, tile records (indices, values) and
+// the partial-sum table
+constexpr int kStageMax = 7;  // 16-byte staging loads per thread (512) and slot group (nseg_max <= 448)
+// the staging buffer holds all kStageMax loads of every thread, so the loads
+// and stores need no bounds test (a test per load would serialize them)
+__host__ __device__ inline size_t opl_lds_bytes(int nseg_max, int T) {
+    return (size_t)kStageMax * 512 * 16 + (((size_t)T * kLrec * 2 + 15) / 16) * 16 + (size_t)T * kLrec * 4 +
+           8 * kSlotGroup * 8 + (size_t)nseg_max * 8;
+}
+
+// FP32 operator q = S p of the two-level path, in two kernels that run
+// concurrently (k_c_opg on a second stream).  Partial p.q per (block, slot)
+// over all slots of the live chunks: rows [0, gen_blocks) of part0 from
+// k_c_opg, rows [gen_blocks, gen_blocks + ntiles) from k_c_opl.
+
+// contribution (S_e(K) x)_i of an element without vb neighbours from gathered
+// x (elem_row_plain's arithmetic and order)
+__device__ __forceinline__ float gen_plain(const float *__restrict__ T, int n0, int i, float K,
+                                           const float (&x)[kGenR]) {
+    float t = 0.0f, xi = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j < n0) {
+            t += T[i * n0 + j] * x[j];
+            if (j == i) xi = x[j];
+        }
+    return K * t + T[n0 * n0 + i] * xi;
+}
+
+// the same for an element with vb neighbours, in FP64 (elem_row_vb's arithmetic)
+__device__ __forceinline__ double gen_vb(const double *__restrict__ T, int n0, int nvb, int i, double K,
+                                         const float (&x)[kGenR]) {
+    const double *C0 = T, *w0 = T + n0 * n0, *dlt = w0 + n0, *sv = dlt + n0, *vv = sv + 2;
+    double ux = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j < n0) ux += w0[j] * (double)x[j];
+    ux *= K;
+    // vb neighbours are x[n0 .. n0 + nvb): a static index over x keeps it in
+    // registers, the table entry is vv[j - n0] (same order as over k)
+    double xi = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGenR; j++)
+        if (j >= n0 && j < n0 + nvb) {
+            const double xk = (double)x[j];
+            ux += vv[j - n0] * xk;
+            if (j == i) xi = xk;
+        }
+    const double g = ux / (K * sv[0] + sv[1]);
+    if (i < n0) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (j < n0) {
+                t += C0[i * n0 + j] * (double)x[j];
+                if (j == i) xi = (double)x[j];
+            }
+        return K * (t - w0[i] * g) + dlt[i] * xi;
+    }
+    return vv[i - n0] * (xi - g);
+}
+
+// The record and operator tables are separate restrict arguments: the compiler
+// then knows the q stores do not alias them and reads them, wave-uniform, with
+// scalar loads.
+// One general row for CH slots (64 apart) from its flattened record: every
+// gather of the first NC contributions first (records past the count hold
+// valid rows), then the contributions in order.  The FP32 conductivity of a
+// plain element is the FP64 one rounded, as the slot loader stores it.
+template <int NC, int CH>
+__device__ __forceinline__ void gen_row(const int (&Gr)[kGenRec], int nc, const float *__restrict__ p,
+                                        const double *__restrict__ K64, const float *__restrict__ opT,
+                                        const double *__restrict__ opT64, size_t Rp, int lane, float (&y)[CH]) {
+    float x[CH][NC][kGenR];
+    double k64[CH][NC];
+#pragma unroll
+    for (int h = 0; h < CH; h++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+#pragma unroll
+            for (int j = 0; j < kGenR; j++) x[h][c][j] = p[(size_t)Gr[1 + c * kGenStride + 5 + j] * Rp + h * 64 + lane];
+            k64[h][c] = K64[(size_t)Gr[1 + c * kGenStride] * Rp + h * 64 + lane];
+        }
+#pragma unroll
+    for (int h = 0; h < CH; h++) {
+        float t = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if (c >= nc) break;
+            const int *Q = Gr + 1 + c * kGenStride;
+            const int i = Q[1], n0 = Q[2], nvb = Q[3], off = Q[4];
+            if (nvb == 0) t += gen_plain(opT + off, n0, i, (float)k64[h][c], x[h][c]);
+            else t += (float)gen_vb(opT64 + off, n0, nvb, i, k64[h][c], x[h][c]);
+        }
+        y[h] = t;
+    }
+}
+
+// k_c_opg: the general rows, one row per wave and one slot per lane of each
+// of the CH live chunks (all chunks' gathers issued together).  The record and
+// operator tables are separate restrict arguments: the compiler then knows the
+// q stores do not alias them and reads them, wave-uniform, with scalar loads.
+template <int CH>
+__global__ void __launch_bounds__(256) k_c_opg(BArgs<float> a, OpTiles ot, const int *__restrict__ gen,
+                                               const int *__restrict__ grec, const float *__restrict__ opT,
+                                               const double *__restrict__ opT64) {
+    __shared__ double sh[kWaves][64];
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t Rp = a.Rp;
+    const XPlain<float> xs{a.p0};
+    const float *__restrict__ p = a.p0;
+    bool act[CH];
+    double v[CH];
+#pragma unroll
+    for (int h = 0; h < CH; h++) {
+        act[h] = a.sc.status[h * 64 + lane] == kRun;
+        v[h] = 0.0;
+    }
+    for (int k = blockIdx.x * kWaves + wid; k < ot.ngen; k += ot.gen_blocks * kWaves) {
+        const int row = gen[k];
+        int Gr[kGenRec];
+#pragma unroll
+        for (int w = 0; w < kGenRec; w++) Gr[w] = grec[(size_t)k * kGenRec + w];
+        const int nc = Gr[0];
+        float q[CH];
+        if (nc < 0) {
+#pragma unroll
+            for (int h = 0; h < CH; h++)
+                q[h] = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn, a.opT, a.K,
+                                     a.opT64, a.K64, row, xs, Rp, h * 64 + lane);
+        } else if (nc <= 2) {
+            gen_row<2, CH>(Gr, nc, p, a.K64, opT, opT64, Rp, lane, q);
+        } else {
+            gen_row<kGenC, CH>(Gr, nc, p, a.K64, opT, opT64, Rp, lane, q);
+        }
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            if (!act[h]) continue;
+            const size_t o = (size_t)row * Rp + h * 64 + lane;
+            a.q[o] = q[h];
+            v[h] += (double)p[o] * (double)q[h];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < CH; h++) {
+        __syncthreads();
+        sh[wid][lane] = v[h];
+        __syncthreads();
+        if (wid == 0) {
+            double t = sh[0][lane];
+            for (int w = 1; w < kWaves; w++) t += sh[w][lane];
+            a.part0[(size_t)blockIdx.x * Rp + h * 64 + lane] = t;
+        }
+    }
+}
+
+// k_c_opl: one tile per block (512 threads).  Tile t of XCD x (block b runs on
+// XCD b % 8) is tile x * (ntiles / 8) + b / 8, so tiles in flight on one XCD
+// are neighbours and share halo lines in its L2.  The tile's records and
+// segment list are staged in LDS once; then per group of 32 slots its own rows,
+// halo rows and element conductivities are loaded into LDS with coalesced
+// 16-byte loads, all issued before any is stored (every value of the tile's
+// neighbourhood read once per slot group), and 8 lanes x 4 slots compute each
+// row from LDS.  Per slot the arithmetic and its order are those of k_c_op2
+// (bitwise the same q).
+constexpr int kOplThreads = 512;
+#ifndef FL_OP_REGISTER_TILES
+#define FL_OP_REGISTER_TILES 0
+#endif
+constexpr bool kOpRegisterTiles = FL_OP_REGISTER_TILES;
+// One slot group's segments of a tile into registers: 8 lanes x 16 bytes per
+// segment row (32 slots), segments past the end clamped to the last one.  The
+// segment row addresses come from LDS each time (kept in registers across
+// the pipeline they would spill).
+// (global address space: a generic load would also count as an LDS access
+// and make every LDS wait of the compute wait for the staged loads)
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) f4v *gf4v_ptr;
+__device__ __forceinline__ void opl_stage(const float *const *segp, int nseg, int tid, int sb,
+                                          f4v (&vv)[kStageMax]) {
+    const float *ptr[kStageMax];
+#pragma unroll
+    for (int u = 0; u < kStageMax; u++) ptr[u] = segp[min((tid >> 3) + u * (kOplThreads / 8), nseg - 1)];
+#pragma unroll
+    for (int u = 0; u < kStageMax; u++) vv[u] = *(gf4v_ptr)(ptr[u] + sb + 4 * (tid & 7));
+}
+
+__global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTiles ot, int nch) {
+    extern __shared__ double lds_d[];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const size_t Rp = a.Rp;
+    const int tb = blockIdx.x;
+    const int tile = (tb & 7) * (ot.ntiles >> 3) + (tb >> 3);
+    double *const part = a.part0 + (size_t)(ot.gen_blocks + blockIdx.x) * Rp;
+    const int r0 = tile * ot.T, nr = max(0, min(ot.T, a.F - r0));
+    const int h0 = ot.hptr[tile], nh = ot.hptr[tile + 1] - h0;
+    const int e0 = ot.eptr[tile], ne = ot.eptr[tile + 1] - e0;
+    const int nseg = nr + nh + ne, eo = nr + nh;
+    if (nr == 0) {  // padding tile: zero partials
+        for (int k = tid; k < nch * 64; k += blockDim.x) part[k] = 0.0;
+        return;
+    }
+    float4 *buf = reinterpret_cast<float4 *>(lds_d);  // [kStageMax * kOplThreads] float4
+    unsigned short *lr = reinterpret_cast<unsigned short *>(buf + kStageMax * kOplThreads);
+    float *lf = reinterpret_cast<float *>(lr + (((size_t)ot.T * kLrec * 2 + 15) / 16) * 8);
+    double *red = reinterpret_cast<double *>(lf + (size_t)ot.T * kLrec);  // [8 waves][32 slots]
+    const float **segp = reinterpret_cast<const float **>(red + 8 * kSlotGroup);  // row of each segment
+    {
+        // tile set-up; indices past the end are clamped to valid ones and their
+        // values dropped, so no load waits for another
+        constexpr int kRecLd = (128 * kLrec + kOplThreads - 1) / kOplThreads;
+        constexpr int kSegLd = (512 + kOplThreads - 1) / kOplThreads;
+        unsigned short rl[kRecLd];
+        int rf[kRecLd];
+        const float *sg[kSegLd];
+#pragma unroll
+        for (int u = 0; u < kRecLd; u++) {
+            const int w = min(tid + u * kOplThreads, nr * kLrec - 1);
+            const int rr = w / kLrec, k = w - rr * kLrec;
+            rl[u] = ot.lrec[(size_t)r0 * kLrec + w];
+            rf[u] = a.frec[(size_t)(r0 + rr) * kFastRec + 10 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kSegLd; u++) {
+            const int g = min(tid + u * kOplThreads, nseg - 1);
+            const int hi = min(max(h0 + g - nr, h0), max(h0, h0 + nh - 1));
+            const int ei = min(max(e0 + g - eo, e0), max(e0, e0 + ne - 1));
+            const int vh = ot.halo[hi], ve = ot.els[ei];
+            sg[u] = g < eo ? a.p0 + (size_t)(g < nr ? r0 + g : vh) * Rp : a.K + (size_t)ve * Rp;
+        }
+#pragma unroll
+        for (int u = 0; u < kRecLd; u++) {
+            const int w = tid + u * kOplThreads;
+            if (w < nr * kLrec) {
+                lr[w] = rl[u];
+                lf[w] = __int_as_float(rf[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSegLd; u++) {
+            const int g = tid + u * kOplThreads;
+            if (g < nseg) segp[g] = sg[u];
+        }
+    }
+    __syncthreads();
+    const int quad = tid & 7, rsub = tid >> 3;  // 64 rows per pass, 4 slots per lane
+    const int nsg = 2 * nch;
+    // software pipeline: the next slot group's segments are loaded into
+    // registers while the current group is computed from LDS.  The run flags
+    // of all groups (nch <= 2 here) are read first, as bits, so no wait inside
+    // the loop waits for the staged loads too.
+    unsigned long long actm = 0;
+    for (int sg = 0; sg < nsg; sg++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            actm |= (unsigned long long)(a.sc.status[sg * kSlotGroup + 4 * quad + c] == kRun) << (4 * sg + c);
+    f4v vv[kStageMax];
+    opl_stage(segp, nseg, tid, 0, vv);
+    for (int sg = 0; sg < nsg; sg++) {
+        const int sb = sg * kSlotGroup;
+        bool act[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) act[c] = (actm >> (4 * sg + c)) & 1;
+        __syncthreads();  // the previous group's rows are computed
+#pragma unroll
+        for (int u = 0; u < kStageMax; u++) reinterpret_cast<f4v *>(buf)[tid + u * kOplThreads] = vv[u];
+        __syncthreads();
+        if (sg + 1 < nsg) opl_stage(segp, nseg, tid, sb + kSlotGroup, vv);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int r = rsub; r < nr; r += kOplThreads / 8) {
+            const unsigned short *R = lr + r * kLrec;
+            if (R[0] == 0xFFFF) continue;
+            const float *Fv = lf + r * kLrec;
+            const float4 xr = buf[r * 8 + quad];
+            const float4 k1 = buf[(eo + R[0]) * 8 + quad], k2 = buf[(eo + R[1]) * 8 + quad];
+            float4 xa[4], xb[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                xa[j] = buf[R[2 + j] * 8 + quad];
+                xb[j] = buf[R[6 + j] * 8 + quad];
+            }
+            const float d1 = Fv[8], d2 = Fv[9];
+            float q[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float x0 = (&xr.x)[c];
+                float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    t1 += Fv[j] * (&xa[j].x)[c];
+                    t2 += Fv[4 + j] * (&xb[j].x)[c];
+                }
+                float qq = (&k1.x)[c] * t1 + d1 * x0;
+                qq += (&k2.x)[c] * t2 + d2 * x0;
+                q[c] = qq;
+                if (act[c]) v[c] += (double)x0 * (double)qq;
+            }
+            float *dst = a.q + (size_t)(r0 + r) * Rp + sb + 4 * quad;
+            if (act[0] && act[1] && act[2] && act[3]) {
+                *reinterpret_cast<float4 *>(dst) = make_float4(q[0], q[1], q[2], q[3]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (act[c]) dst[c] = q[c];
+            }
+        }
+        // sum over the 8 rows of the wave (lanes 8 apart), then over the 8 waves
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            double x = v[c];
+            x += __shfl_xor(x, 8, 64);
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            v[c] = x;
+        }
+        if (lane < 8) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) red[wid * kSlotGroup + 4 * quad + c] = v[c];
+        }
+        __syncthreads();
+        if (tid < kSlotGroup) {
+            double t = red[tid];
+            for (int w = 1; w < kOplThreads / 64; w++) t += red[w * kSlotGroup + tid];
+            part[sb + tid] = t;
+        }
+    }
+}
+
+// Register-tiled alternative to k_c_opl (same tiles, same partial rows): the
+// tile's fast records are staged in LDS, then 16 lanes x 4 slots (16-byte
+// loads) handle one row for all CH chunks, 4 rows per wave, every load of a
+// row issued before any is used; neighbour values come from L2.
+template <int CH>
+__global__ void __launch_bounds__(256) k_c_opr(BArgs<float> a, OpTiles ot) {
+    __shared__ int rec[128 * kFastRec];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const size_t Rp = a.Rp;
+    const int tb = blockIdx.x;
+    const int tile = (tb & 7) * (ot.ntiles >> 3) + (tb >> 3);
+    const int r0 = tile * ot.T, nr = max(0, min(ot.T, a.F - r0));
+    double *const part = a.part0 + (size_t)(ot.gen_blocks + blockIdx.x) * Rp;
+    if (nr == 0) {
+        for (int k = tid; k < CH * 64; k += blockDim.x) part[k] = 0.0;
+        return;
+    }
+    {
+        constexpr int kLd = (128 * kFastRec + 255) / 256;
+        int rv[kLd];
+#pragma unroll
+        for (int u = 0; u < kLd; u++) rv[u] = a.frec[(size_t)r0 * kFastRec + min(tid + u * 256, nr * kFastRec - 1)];
+#pragma unroll
+        for (int u = 0; u < kLd; u++)
+            if (tid + u * 256 < nr * kFastRec) rec[tid + u * 256] = rv[u];
+    }
+    const int grp = wid * 4 + (lane >> 4), quad = lane & 15;
+    bool act[CH][4];
+    double v[CH][4];
+#pragma unroll
+    for (int h = 0; h < CH; h++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            act[h][j] = a.sc.status[h * 64 + 4 * quad + j] == kRun;
+            v[h][j] = 0.0;
+        }
+    __syncthreads();
+    const float *__restrict__ p = a.p0;
+    const float *__restrict__ Kv = a.K;
+    for (int r = grp; r < nr; r += 16) {
+        const int *C = rec + r * kFastRec;
+        const int e1 = C[0];
+        if (e1 < 0) continue;
+        const int e2 = C[1];
+        const size_t row = (size_t)(r0 + r);
+        float4 xr[CH], k1[CH], k2[CH], xa[CH][4], xb[CH][4];
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            const size_t so = (size_t)h * 64 + 4 * quad;
+            xr[h] = *reinterpret_cast<const float4 *>(p + row * Rp + so);
+            k1[h] = *reinterpret_cast<const float4 *>(Kv + (size_t)e1 * Rp + so);
+            k2[h] = *reinterpret_cast<const float4 *>(Kv + (size_t)e2 * Rp + so);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                xa[h][j] = *reinterpret_cast<const float4 *>(p + (size_t)C[2 + j] * Rp + so);
+                xb[h][j] = *reinterpret_cast<const float4 *>(p + (size_t)C[6 + j] * Rp + so);
+            }
+        }
+        float P1[4], P2[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            P1[j] = __int_as_float(C[10 + j]);
+            P2[j] = __int_as_float(C[14 + j]);
+        }
+        const float d1 = __int_as_float(C[18]), d2 = __int_as_float(C[19]);
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            float q[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float x0 = (&xr[h].x)[c];
+                float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    t1 += P1[j] * (&xa[h][j].x)[c];
+                    t2 += P2[j] * (&xb[h][j].x)[c];
+                }
+                float qq = (&k1[h].x)[c] * t1 + d1 * x0;
+                qq += (&k2[h].x)[c] * t2 + d2 * x0;
+                q[c] = qq;
+                if (act[h][c]) v[h][c] += (double)x0 * (double)qq;
+            }
+            float *dst = a.q + row * Rp + (size_t)h * 64 + 4 * quad;
+            if (act[h][0] && act[h][1] && act[h][2] && act[h][3]) {
+                *reinterpret_cast<float4 *>(dst) = make_float4(q[0], q[1], q[2], q[3]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (act[h][c]) dst[c] = q[c];
+            }
+        }
+    }
+    // sum over the 4 row groups of the wave (lanes 16 apart), then over the waves
+    __shared__ double red[kWaves][CH * 64];
+#pragma unroll
+    for (int h = 0; h < CH; h++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            double x = v[h][c];
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            if (lane < 16) red[wid][h * 64 + 4 * quad + c] = x;
+        }
+    __syncthreads();
+    for (int k = tid; k < CH * 64; k += blockDim.x)
+        part[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+}
+
 // As k_c_op2<1>, two rows per step (row and row + stride): the records of
 // both rows arrive in one register (lanes 0..19 and 32..51) and all gathers of
 // both rows are issued before any is used.
@@ -1886,6 +2351,9 @@ DarcyBatchGPU::DarcyBatchGPU(DarcyGPU &base, int R, int slots) : base_(base), dv
     Rp_ = std::min(Rout_, slots > 0 ? (slots + 63) / 64 * 64 : 256);
     nchunk_ = Rp_ / 64;
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    FL_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    FL_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    FL_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     build();
 }
 
@@ -1896,6 +2364,10 @@ DarcyBatchGPU::~DarcyBatchGPU() {
     for (void *p : owned_) dev_free(p);
     if (h_count_) (void)hipHostFree(h_count_);
     if (solver_) (void)rocblas_destroy_handle((rocblas_handle)solver_);
+    if (stream2_) (void)hipStreamSynchronize(stream2_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1904,6 +2376,98 @@ void DarcyBatchGPU::free_graphs() {
     if (g32_) (void)hipGraphExecDestroy(g32_);
     g64_ = g32_ = nullptr;
     g64_key_ = g32_key_ = -1;
+}
+
+// Tiles of the FP32 operator (k_c_opl): general rows, and per tile of T rows
+// its halo rows, its elements and the local-index records of its fast rows.
+// T = 128 rows, halved while a tile's staged segments would not fit the LDS
+// budget; a multiple of 8 tiles.
+void DarcyBatchGPU::build_op_tiles(const std::vector<int> &rec) {
+    const int F = F_;
+    std::vector<int> gen;
+    for (int f = 0; f < F; f++)
+        if (!op_fast_row(&rec[(size_t)f * kRecInts])) gen.push_back(f);
+    ngen_ = (int)gen.size();
+    if (gen.empty()) gen.push_back(0);
+    constexpr int kSegBudget = kStageMax * kOplThreads / 8;  // 56 KiB of staged rows per 32-slot pass
+    std::vector<int> hptr, halo, eptr, els;
+    std::vector<unsigned short> lrec;
+    int T = 128, nseg_max = 0, ntiles = 0;
+    for (;; T /= 2) {
+        ntiles = ((F + T - 1) / T + 7) / 8 * 8;
+        hptr.assign(1, 0);
+        eptr.assign(1, 0);
+        halo.clear();
+        els.clear();
+        lrec.assign((size_t)F * kLrec, 0xFFFF);
+        nseg_max = 0;
+        std::unordered_map<int, int> hloc, eloc;
+        for (int t = 0; t < ntiles; t++) {
+            const int r0 = t * T, r1 = std::min(F, r0 + T), nr = std::max(0, r1 - r0);
+            hloc.clear();
+            eloc.clear();
+            std::vector<int> th, te;
+            auto rloc = [&](int row) {
+                if (row >= r0 && row < r1) return row - r0;
+                auto it = hloc.find(row);
+                if (it != hloc.end()) return it->second;
+                const int l = nr + (int)th.size();
+                th.push_back(row);
+                hloc[row] = l;
+                return l;
+            };
+            auto elo = [&](int e) {
+                auto it = eloc.find(e);
+                if (it != eloc.end()) return it->second;
+                const int l = (int)te.size();
+                te.push_back(e);
+                eloc[e] = l;
+                return l;
+            };
+            for (int f = r0; f < r1; f++) {
+                const int *R = &rec[(size_t)f * kRecInts];
+                if (!op_fast_row(R)) continue;
+                const int *Q2 = R[8] < 0 ? R : R + 8;  // one tetrahedron: repeated (its block row is zero)
+                unsigned short *L = &lrec[(size_t)f * kLrec];
+                L[0] = (unsigned short)elo(R[0]);
+                L[1] = (unsigned short)elo(Q2[0]);
+                for (int j = 0; j < 4; j++) {
+                    L[2 + j] = (unsigned short)rloc(R[4 + j]);
+                    L[6 + j] = (unsigned short)rloc(Q2[4 + j]);
+                }
+            }
+            // element segments follow the halo: the records hold element indices
+            // relative to that start
+            halo.insert(halo.end(), th.begin(), th.end());
+            els.insert(els.end(), te.begin(), te.end());
+            hptr.push_back((int)halo.size());
+            eptr.push_back((int)els.size());
+            nseg_max = std::max(nseg_max, nr + (int)th.size() + (int)te.size());
+        }
+        if (nseg_max <= kSegBudget || T <= 16) break;
+    }
+    static_assert(kSegBudget <= 512, "staging registers cover 512 segments");
+    if (nseg_max > 65535 || opl_lds_bytes(nseg_max, T) > 160 * 1024)
+        throw Error(-5, "batched operator: a row tile's neighbourhood does not fit the LDS");
+    if (halo.empty()) halo.push_back(0);
+    if (els.empty()) els.push_back(0);
+    tile_T_ = T;
+    ntiles_ = ntiles;
+    nseg_max_ = nseg_max;
+    // general rows are long dependent chains: about 3 per wave, as many
+    // blocks as the partial arrays leave
+    gen_blocks_ = ngen_ ? std::min(kMaxGrid - ntiles_, ((ngen_ + 11) / 12 + 7) / 8 * 8) / 8 * 8 : 0;
+    if (gen_blocks_ < 0 || (ngen_ && gen_blocks_ == 0))
+        throw Error(-5, "batched operator: too many row tiles for the partial arrays");
+    hipStream_t s = stream_;
+    d_gen_ = b_up(owned_, gen, s);
+    d_thptr_ = b_up(owned_, hptr, s);
+    d_thalo_ = b_up(owned_, halo, s);
+    d_teptr_ = b_up(owned_, eptr, s);
+    d_tels_ = b_up(owned_, els, s);
+    d_lrec_ = b_up(owned_, lrec, s);
+    FL_HIP(hipFuncSetAttribute((const void *)k_c_opl, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)opl_lds_bytes(nseg_max_, tile_T_)));
 }
 
 void DarcyBatchGPU::build() {
@@ -2116,6 +2680,42 @@ void DarcyBatchGPU::build() {
         }
     }
     d_rec_ = b_up(owned_, rec, stream_);
+    build_op_tiles(rec);
+    {
+        // flattened records of the general rows (k_c_opg): up to kGenC element
+        // contributions in the row's contribution order, each with its element,
+        // local index, size, table offset and connected rows; more contributions
+        // or rows: nC = -1, the row is evaluated by op_row
+        std::vector<int> grec((size_t)std::max(ngen_, 1) * kGenRec, 0);
+        std::vector<int> gen_rows;
+        for (int f = 0; f < F; f++)
+            if (!op_fast_row(&rec[(size_t)f * kRecInts])) gen_rows.push_back(f);
+        for (int k = 0; k < ngen_; k++) {
+            const int f = gen_rows[k];
+            int *G = &grec[(size_t)k * kGenRec];
+            const int nc = row_ptr[f + 1] - row_ptr[f];
+            bool ok = nc <= kGenC;
+            for (int c = 0; c < nc && ok; c++) {
+                const int e = row_c[row_ptr[f] + c] / kMaxConn;
+                ok = conn_off[e + 1] - conn_off[e] <= kGenR;
+            }
+            G[0] = ok ? nc : -1;
+            if (!ok) continue;
+            for (int c = 0; c < nc; c++) {
+                const int code = row_c[row_ptr[f] + c];
+                const int e = code / kMaxConn, i = code % kMaxConn;
+                int *Q = G + 1 + c * kGenStride;
+                Q[0] = e;
+                Q[1] = i;
+                Q[2] = m.el_dim[e] + 1;
+                Q[3] = m.el_vb_ptr[e + 1] - m.el_vb_ptr[e];
+                Q[4] = op_off[e];
+                for (int j = 0; j < kGenR; j++)
+                    Q[5 + j] = conn_off[e] + j < conn_off[e + 1] ? conn[conn_off[e] + j] : f;
+            }
+        }
+        d_grec_ = b_up(owned_, grec, stream_);
+    }
     d_frec_ = dev_alloc<int>((size_t)F * kFastRec);
     owned_.push_back(d_frec_);
     std::vector<int> dperm;
@@ -2406,7 +3006,8 @@ void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
 }
 
 void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
-    if (grid < 8 || grid > kMaxGrid || grid % 8) throw Error(-1, "batch grid must be a multiple of 8 in [8, 2048]");
+    if (grid < 8 || grid > kMaxLaunchGrid || grid % 8)
+        throw Error(-1, "batch grid must be a multiple of 8 in [8, 2048]");
     if (chunk < 0 || chunk > 1024) throw Error(-1, "batch chunk must be in [0, 1024] (0: automatic)");
     if (grid != grid_ || xcd_map != xcd_map_ || chunk != chunk_) free_graphs();
     if (xcd_map < 0 || xcd_map > 2) throw Error(-1, "batch row mapping must be 0, 1 or 2");
@@ -2499,15 +3100,15 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     rebuild();
     auto grow = [&]() { return dim3(G, nch_); };
-    auto fold = [&](int mode, int mixed, const int *lsel) {
+    auto fold = [&](int mode, int mixed, const int *lsel, int gparts = 0) {
         const double *q0 = a64.part0, *q1 = a64.part1, *q2 = a64.part2;
         const dim3 gf(nch_);
-        int Gf = G;
-        if (G > 2 * kPre) {  // two-stage: kPre partial rows per slot for the fold proper
+        int Gf = gparts > 0 ? gparts : G;
+        if (Gf > 2 * kPre) {  // two-stage: kPre partial rows per slot for the fold proper
             double *o0 = d_pre_, *o1 = d_pre_ + (size_t)kPre * Rp_, *o2 = d_pre_ + 2 * (size_t)kPre * Rp_;
             hipLaunchKernelGGL(k_b_prefold, dim3(kPre, nch_), blk, 0, stream_, mode == kFoldAlpha ? q0 : nullptr,
                                mode != kFoldAlpha ? q1 : nullptr,
-                               mode != kFoldAlpha ? q2 : nullptr, G, Rp_, o0, o1, o2);
+                               mode != kFoldAlpha ? q2 : nullptr, Gf, Rp_, o0, o1, o2);
             q0 = o0;
             q1 = o1;
             q2 = o2;
@@ -2571,6 +3172,34 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         if (NC == 1) FL_LW(hipLaunchKernelGGL((k_c_zp<double, 1, LW>), grow(), blk, 0, stream_, a64, ca))
         else FL_LW(hipLaunchKernelGGL((k_c_zp<double, 2, LW>), grow(), blk, 0, stream_, a64, ca))
     };
+    // the FP32 operator: k_c_opg (general rows) on the second stream beside
+    // k_c_opl (row tiles); the fork and join are captured into the graphs
+    auto op_tiled_grid = [&]() { return gen_blocks_ + ntiles_; };
+    auto op_tiled = [&](const BArgs<float> &a) {
+        const OpTiles ot{d_gen_, d_grec_, d_thptr_, d_thalo_, d_teptr_, d_tels_, d_lrec_, ngen_, gen_blocks_,
+                         ntiles_, tile_T_, nseg_max_};
+        if (ngen_) {
+            FL_HIP(hipEventRecord(ev_fork_, stream_));
+            FL_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+            if (nch_ == 1)
+                hipLaunchKernelGGL(k_c_opg<1>, dim3(gen_blocks_), blk, 0, stream2_, a, ot, ot.gen, ot.grec, a.opT,
+                                   a.opT64);
+            else
+                hipLaunchKernelGGL(k_c_opg<2>, dim3(gen_blocks_), blk, 0, stream2_, a, ot, ot.gen, ot.grec, a.opT,
+                                   a.opT64);
+        }
+        if (kOpRegisterTiles && nch_ == 2)
+            hipLaunchKernelGGL(k_c_opr<2>, dim3(ntiles_), blk, 0, stream_, a, ot);
+        else if (kOpRegisterTiles && nch_ == 1)
+            hipLaunchKernelGGL(k_c_opr<1>, dim3(ntiles_), blk, 0, stream_, a, ot);
+        else
+            hipLaunchKernelGGL(k_c_opl, dim3(ntiles_), dim3(kOplThreads), opl_lds_bytes(nseg_max_, tile_T_), stream_,
+                               a, ot, nch_);
+        if (ngen_) {
+            FL_HIP(hipEventRecord(ev_join_, stream2_));
+            FL_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+        }
+    };
     auto zp32 = [&]() {
         if (NC == 1) FL_LW(hipLaunchKernelGGL((k_c_zp<float, 1, LW>), grow(), blk, 0, stream_, a32, ca))
         else FL_LW(hipLaunchKernelGGL((k_c_zp<float, 2, LW>), grow(), blk, 0, stream_, a32, ca))
@@ -2607,15 +3236,12 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             hipLaunchKernelGGL(k_c_op_w, grow(), blk, 0, stream_, a32, ca);
         else if (NC && opfast_ == 4 && lw_ == 64)
             hipLaunchKernelGGL(k_c_op3, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ == 2 && lw_ == 64 && nch_ % 2 == 0)
-            hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ >= 2 && lw_ == 64)
-            hipLaunchKernelGGL(k_c_op2<1>, grow(), blk, 0, stream_, a32, ca);
+        else if (NC && opfast_ >= 2 && lw_ == 64) op_tiled(a32);
         else if (NC && opfast_) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 1>), grow(), blk, 0, stream_, a32, ca))
         else if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 0>), grow(), blk, 0, stream_, a32, ca))
         else hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
-        fold(kFoldAlpha, 1, nosel);
+        fold(kFoldAlpha, 1, nosel, NC && opfast_ >= 2 && lw_ == 64 ? op_tiled_grid() : 0);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
         if (NC == 1) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 1, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 1, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
         else if (NC == 2) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 2, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 2, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
