@@ -56,15 +56,12 @@ struct BatchScal {
     int *status, *it, *n_ru, *par, *it_ru, *active_count;  // par: buffer holding the current search direction
     int *zp;  // two-level path, per slot: bit 0 re-form the direction, bit 1 add alpha p to x
     double ru_drop2;  // FP32 path: reliable update once ||r||^2 fell below ru_drop2 times its value at the last one
-    int beta_fr;      // two-level path: 1 Fletcher-Reeves beta (fixed preconditioner), 0 flexible (Polak-Ribiere)
-    int no_pause;     // FP32 two-level path: 1 keeps iterating after the drop, the update replaces r only
     int nat;          // stop-test norm: 0 ||r||_2, 1 sqrt(r^T D^-1 r) (Jacobi natural norm); rr, tol2,
                       // bnorm2, rr_ru and rr_hist are squares in that norm
 };
 
 // Batched CG status per realization
-// kPause: paused for a reliable update after a 10x residual drop (two-level FP32 path)
-enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6, kPause = 7 };
+enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6 };
 
 class DarcyBatchGPU {
    public:
@@ -119,7 +116,7 @@ class DarcyBatchGPU {
    private:
     void build();
     void alloc32();
-    void tune(int precision);  // launch shape for the operator kernel of this precision
+    void tune();  // launch shape of the row-grid kernels
     void setup_system();  // table, rhs, Jacobi diagonal for the current K
     void free_graphs();
     void alloc_scal(BatchScal &sc);
@@ -156,20 +153,12 @@ class DarcyBatchGPU {
     double *d_cpart_ = nullptr; // coarse-solve sums, 5 x (row blocks) x Rp
     int lw_ = 64;               // lanes per row of the two-level iteration kernels (64, 16, 4)
     bool narrow_ = true;        // two-level path: narrow lanes for the tail instead of the hand-over
-    bool trace_ = false;        // per-chunk progress lines on stderr
     // tiled FP32 operator: general rows (vb-coupled elements) and row tiles
     void build_op_tiles(const std::vector<int> &rec);
     const int *d_gen_ = nullptr, *d_grec_ = nullptr, *d_thptr_ = nullptr, *d_thalo_ = nullptr, *d_teptr_ = nullptr, *d_tels_ = nullptr;
     const unsigned short *d_lrec_ = nullptr;
     int ngen_ = 0, gen_blocks_ = 0, ntiles_ = 0, tile_T_ = 128, nseg_max_ = 0;
-    int opfast_ = 2;            // two-level operator: 0 general rows, 1 pair rows, 2 inlined pair records (2 chunks per wave), 3 the same, 1 chunk
     int *d_frec_ = nullptr;     // FP32 pair-row records (20 words per row)
-    bool unroll_ = true;        // two-level update: two rows per step
-    int qs_ = 4;                // column split of the coarse solve with all slots busy
-    double ru_drop2_ = 1.0;     // reliable-update trigger (squared residual drop)
-    bool beta_fr_ = true;       // two-level beta: Fletcher-Reeves (fixed SPD preconditioner) or flexible
-    int early_ = 0;             // realizations handed to the single-solve worker when the queue runs empty
-    bool no_pause_ = true;      // reliable update as residual replacement, no pause before it
     BatchScal sc_{}, sc2_{};
     int *h_count_ = nullptr;  // pinned
     // slot -> realization (-1: empty), live chunk count, spare buffers

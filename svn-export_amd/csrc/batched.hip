@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(256) k_b_update(BArgs<V> a) {
 // 1 replaces the residual of every running realization at every chunk end.
 constexpr double kRuDrop2 = 1.0;
 __device__ __forceinline__ bool ru_pick(const BatchScal &sc, int r, int st) {
-    return st == kCandidate || st == kPause || (st == kRun && sc.rr[r] < sc.ru_drop2 * sc.rr_ru[r]);
+    return st == kCandidate || (st == kRun && sc.rr[r] < sc.ru_drop2 * sc.rr_ru[r]);
 }
 
 // Lanes per edge row of the two-level iteration kernels.  W = 64: one row per
@@ -1343,99 +1343,6 @@ __global__ void __launch_bounds__(256) k_c_op(BArgs<V> a, CArgs c) {
     block_part<1>(v, dst, a.Rp);
 }
 
-// FP32 operator with one row per wave for CH 64-slot chunks at once (grid
-// (G, chunks / CH)): the row's record is read once for all CH chunks, the
-// pair rows take their block rows from the inlined records, and the next
-// row's record is loaded while the current row's gathers are in flight.
-// Partial p.q per (block, slot) as k_c_op.
-template <int CH>
-__global__ void __launch_bounds__(256) k_c_op2(BArgs<float> a, CArgs c) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int sl[CH];
-    bool act[CH];
-    bool any = false;
-#pragma unroll
-    for (int h = 0; h < CH; h++) {
-        sl[h] = (blockIdx.y * CH + h) * 64 + lane;
-        act[h] = a.sc.status[sl[h]] == kRun;
-        any = any || act[h];
-    }
-    double v[CH];
-#pragma unroll
-    for (int h = 0; h < CH; h++) v[h] = 0.0;
-    if (__any(any)) {
-        // record words live in lanes 0..19 of one register (vector loads,
-        // counted in order) and are broadcast with readlane
-        const float *__restrict__ p = a.p0;
-        const float *__restrict__ Kv = a.K;
-        const size_t Rp = a.Rp;
-        int s0, st, en;
-        row_range(a.F, a.G, a.xcd, s0, st, en);
-        const int wl = lane < kFastRec ? lane : 0;
-        int cur = s0 < en ? a.frec[(size_t)s0 * kFastRec + wl] : 0;
-        for (int row = s0; row < en; row += st) {
-            const int nrow = row + st;
-            const int nxt = nrow < en ? a.frec[(size_t)nrow * kFastRec + wl] : 0;
-            int C[kFastRec];
-#pragma unroll
-            for (int k = 0; k < kFastRec; k++) C[k] = __builtin_amdgcn_readlane(cur, k);
-            if (C[0] >= 0) {
-                float xr[CH], K1[CH], K2[CH], xa[CH][4], xb[CH][4];
-#pragma unroll
-                for (int h = 0; h < CH; h++) {
-                    if (!act[h]) continue;
-                    xr[h] = p[(size_t)row * Rp + sl[h]];
-                    K1[h] = Kv[(size_t)C[0] * Rp + sl[h]];
-                    K2[h] = Kv[(size_t)C[1] * Rp + sl[h]];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        xa[h][j] = p[(size_t)C[2 + j] * Rp + sl[h]];
-                        xb[h][j] = p[(size_t)C[6 + j] * Rp + sl[h]];
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < CH; h++) {
-                    if (!act[h]) continue;
-                    float t1 = 0.0f, t2 = 0.0f;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        t1 += __int_as_float(C[10 + j]) * xa[h][j];
-                        t2 += __int_as_float(C[14 + j]) * xb[h][j];
-                    }
-                    float q = K1[h] * t1 + __int_as_float(C[18]) * xr[h];
-                    q += K2[h] * t2 + __int_as_float(C[19]) * xr[h];
-                    a.q[(size_t)row * Rp + sl[h]] = q;
-                    v[h] += (double)xr[h] * (double)q;
-                }
-            } else {
-                const XPlain<float> xs{p};
-#pragma unroll
-                for (int h = 0; h < CH; h++) {
-                    if (!act[h]) continue;
-                    const float q = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
-                                                  a.opT, a.K, a.opT64, a.K64, row, xs, Rp, sl[h]);
-                    const size_t o = (size_t)row * Rp + sl[h];
-                    a.q[o] = q;
-                    v[h] += (double)p[o] * (double)q;
-                }
-            }
-            cur = nxt;
-        }
-    }
-    __shared__ double sh[CH][kWaves][64];
-#pragma unroll
-    for (int h = 0; h < CH; h++) sh[h][wid][lane] = v[h];
-    __syncthreads();
-    if (wid == 0) {
-#pragma unroll
-        for (int h = 0; h < CH; h++) {
-            double t = sh[h][0][lane];
-            for (int w = 1; w < kWaves; w++) t += sh[h][w][lane];
-            a.part0[(size_t)blockIdx.x * a.Rp + sl[h]] = t;
-        }
-    }
-}
-
 // Tiles of the FP32 two-level operator (k_c_opl).  Rows are cut into tiles of
 // T consecutive rows (the row order is a Z-curve, so a tile is compact in
 // space).  Per tile, the segments staged in LDS: its own rows (local index
@@ -1616,16 +1523,14 @@ __global__ void __launch_bounds__(256) k_c_opg(BArgs<float> a, OpTiles ot, const
 // XCD b % 8) is tile x * (ntiles / 8) + b / 8, so tiles in flight on one XCD
 // are neighbours and share halo lines in its L2.  The tile's records and
 // segment list are staged in LDS once; then per group of 32 slots its own rows,
-// halo rows and element conductivities are loaded into LDS with coalesced
-// 16-byte loads, all issued before any is stored (every value of the tile's
-// neighbourhood read once per slot group), and 8 lanes x 4 slots compute each
-// row from LDS.  Per slot the arithmetic and its order are those of k_c_op2
-// (bitwise the same q).
+// halo rows and element conductivities are loaded with coalesced 16-byte loads
+// (every value of the tile's neighbourhood read once per slot group; the next
+// group's loads are in flight while the current group is computed), stored to
+// LDS, and 8 lanes x 4 slots compute each row from LDS.  Per slot the
+// arithmetic and its order are fixed by the pair-row record (k_b_fastrec):
+// t1 and t2 over the two elements' four rows in order, then
+// q = (K1 t1 + d1 x) + (K2 t2 + d2 x).
 constexpr int kOplThreads = 512;
-#ifndef FL_OP_REGISTER_TILES
-#define FL_OP_REGISTER_TILES 0
-#endif
-constexpr bool kOpRegisterTiles = FL_OP_REGISTER_TILES;
 // One slot group's segments of a tile into registers: 8 lanes x 16 bytes per
 // segment row (32 slots), segments past the end clamped to the last one.  The
 // segment row addresses come from LDS each time (kept in registers across
@@ -1784,194 +1689,6 @@ __global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTile
     }
 }
 
-// Register-tiled alternative to k_c_opl (same tiles, same partial rows): the
-// tile's fast records are staged in LDS, then 16 lanes x 4 slots (16-byte
-// loads) handle one row for all CH chunks, 4 rows per wave, every load of a
-// row issued before any is used; neighbour values come from L2.
-template <int CH>
-__global__ void __launch_bounds__(256) k_c_opr(BArgs<float> a, OpTiles ot) {
-    __shared__ int rec[128 * kFastRec];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const size_t Rp = a.Rp;
-    const int tb = blockIdx.x;
-    const int tile = (tb & 7) * (ot.ntiles >> 3) + (tb >> 3);
-    const int r0 = tile * ot.T, nr = max(0, min(ot.T, a.F - r0));
-    double *const part = a.part0 + (size_t)(ot.gen_blocks + blockIdx.x) * Rp;
-    if (nr == 0) {
-        for (int k = tid; k < CH * 64; k += blockDim.x) part[k] = 0.0;
-        return;
-    }
-    {
-        constexpr int kLd = (128 * kFastRec + 255) / 256;
-        int rv[kLd];
-#pragma unroll
-        for (int u = 0; u < kLd; u++) rv[u] = a.frec[(size_t)r0 * kFastRec + min(tid + u * 256, nr * kFastRec - 1)];
-#pragma unroll
-        for (int u = 0; u < kLd; u++)
-            if (tid + u * 256 < nr * kFastRec) rec[tid + u * 256] = rv[u];
-    }
-    const int grp = wid * 4 + (lane >> 4), quad = lane & 15;
-    bool act[CH][4];
-    double v[CH][4];
-#pragma unroll
-    for (int h = 0; h < CH; h++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            act[h][j] = a.sc.status[h * 64 + 4 * quad + j] == kRun;
-            v[h][j] = 0.0;
-        }
-    __syncthreads();
-    const float *__restrict__ p = a.p0;
-    const float *__restrict__ Kv = a.K;
-    for (int r = grp; r < nr; r += 16) {
-        const int *C = rec + r * kFastRec;
-        const int e1 = C[0];
-        if (e1 < 0) continue;
-        const int e2 = C[1];
-        const size_t row = (size_t)(r0 + r);
-        float4 xr[CH], k1[CH], k2[CH], xa[CH][4], xb[CH][4];
-#pragma unroll
-        for (int h = 0; h < CH; h++) {
-            const size_t so = (size_t)h * 64 + 4 * quad;
-            xr[h] = *reinterpret_cast<const float4 *>(p + row * Rp + so);
-            k1[h] = *reinterpret_cast<const float4 *>(Kv + (size_t)e1 * Rp + so);
-            k2[h] = *reinterpret_cast<const float4 *>(Kv + (size_t)e2 * Rp + so);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                xa[h][j] = *reinterpret_cast<const float4 *>(p + (size_t)C[2 + j] * Rp + so);
-                xb[h][j] = *reinterpret_cast<const float4 *>(p + (size_t)C[6 + j] * Rp + so);
-            }
-        }
-        float P1[4], P2[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            P1[j] = __int_as_float(C[10 + j]);
-            P2[j] = __int_as_float(C[14 + j]);
-        }
-        const float d1 = __int_as_float(C[18]), d2 = __int_as_float(C[19]);
-#pragma unroll
-        for (int h = 0; h < CH; h++) {
-            float q[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float x0 = (&xr[h].x)[c];
-                float t1 = 0.0f, t2 = 0.0f;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    t1 += P1[j] * (&xa[h][j].x)[c];
-                    t2 += P2[j] * (&xb[h][j].x)[c];
-                }
-                float qq = (&k1[h].x)[c] * t1 + d1 * x0;
-                qq += (&k2[h].x)[c] * t2 + d2 * x0;
-                q[c] = qq;
-                if (act[h][c]) v[h][c] += (double)x0 * (double)qq;
-            }
-            float *dst = a.q + row * Rp + (size_t)h * 64 + 4 * quad;
-            if (act[h][0] && act[h][1] && act[h][2] && act[h][3]) {
-                *reinterpret_cast<float4 *>(dst) = make_float4(q[0], q[1], q[2], q[3]);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (act[h][c]) dst[c] = q[c];
-            }
-        }
-    }
-    // sum over the 4 row groups of the wave (lanes 16 apart), then over the waves
-    __shared__ double red[kWaves][CH * 64];
-#pragma unroll
-    for (int h = 0; h < CH; h++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            double x = v[h][c];
-            x += __shfl_xor(x, 16, 64);
-            x += __shfl_xor(x, 32, 64);
-            if (lane < 16) red[wid][h * 64 + 4 * quad + c] = x;
-        }
-    __syncthreads();
-    for (int k = tid; k < CH * 64; k += blockDim.x)
-        part[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
-}
-
-// As k_c_op2<1>, two rows per step (row and row + stride): the records of
-// both rows arrive in one register (lanes 0..19 and 32..51) and all gathers of
-// both rows are issued before any is used.
-__global__ void __launch_bounds__(256) k_c_op3(BArgs<float> a, CArgs c) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int sl = blockIdx.y * 64 + lane;
-    const bool act = a.sc.status[sl] == kRun;
-    double v = 0.0;
-    if (__any(act)) {
-        const float *__restrict__ p = a.p0;
-        const float *__restrict__ Kv = a.K;
-        const size_t Rp = a.Rp;
-        int s0, st, en;
-        row_range(a.F, a.G, a.xcd, s0, st, en);
-        const int half = lane >> 5, wl = (lane & 31) < kFastRec ? (lane & 31) : 0;
-        auto rec_of = [&](int row0) {
-            const int rr = row0 + half * st;
-            return rr < en ? a.frec[(size_t)rr * kFastRec + wl] : -1;
-        };
-        int cur = s0 < en ? rec_of(s0) : -1;
-        for (int row = s0; row < en; row += 2 * st) {
-            const int nxt = row + 2 * st < en ? rec_of(row + 2 * st) : -1;
-            const int rowB = row + st;
-            const bool hasB = rowB < en;
-            const int eA = __builtin_amdgcn_readlane(cur, 0), eB = __builtin_amdgcn_readlane(cur, 32);
-            if (act && eA >= 0 && (!hasB || eB >= 0)) {
-                float xa[2][4], xb[2][4], K1[2], K2[2], xr[2];
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (h == 1 && !hasB) break;
-                    const int b = 32 * h, rw = h ? rowB : row;
-                    xr[h] = p[(size_t)rw * Rp + sl];
-                    K1[h] = Kv[(size_t)__builtin_amdgcn_readlane(cur, b + 0) * Rp + sl];
-                    K2[h] = Kv[(size_t)__builtin_amdgcn_readlane(cur, b + 1) * Rp + sl];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        xa[h][j] = p[(size_t)__builtin_amdgcn_readlane(cur, b + 2 + j) * Rp + sl];
-                        xb[h][j] = p[(size_t)__builtin_amdgcn_readlane(cur, b + 6 + j) * Rp + sl];
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (h == 1 && !hasB) break;
-                    const int b = 32 * h, rw = h ? rowB : row;
-                    float t1 = 0.0f, t2 = 0.0f;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        t1 += __int_as_float(__builtin_amdgcn_readlane(cur, b + 10 + j)) * xa[h][j];
-                        t2 += __int_as_float(__builtin_amdgcn_readlane(cur, b + 14 + j)) * xb[h][j];
-                    }
-                    float q = K1[h] * t1 + __int_as_float(__builtin_amdgcn_readlane(cur, b + 18)) * xr[h];
-                    q += K2[h] * t2 + __int_as_float(__builtin_amdgcn_readlane(cur, b + 19)) * xr[h];
-                    a.q[(size_t)rw * Rp + sl] = q;
-                    v += (double)xr[h] * (double)q;
-                }
-            } else if (act) {
-                const XPlain<float> xs{p};
-                for (int h = 0; h < 2; h++) {
-                    if (h == 1 && !hasB) break;
-                    const int rw = h ? rowB : row;
-                    const float q = op_row<float>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
-                                                  a.opT, a.K, a.opT64, a.K64, rw, xs, Rp, sl);
-                    const size_t o = (size_t)rw * Rp + sl;
-                    a.q[o] = q;
-                    v += (double)p[o] * (double)q;
-                }
-            }
-            cur = nxt;
-        }
-    }
-    __shared__ double sh[kWaves][64];
-    sh[wid][lane] = v;
-    __syncthreads();
-    if (wid == 0) {
-        double t = sh[0][lane];
-        for (int w = 1; w < kWaves; w++) t += sh[w][lane];
-        a.part0[(size_t)blockIdx.x * a.Rp + sl] = t;
-    }
-}
-
 // FP32 operator of the narrowed tail (W lanes per row, 64 / W rows per wave)
 // from the inlined pair records: each lane group reads its row's record
 // (identical addresses within the group), the next group row's record is
@@ -2039,36 +1756,6 @@ __global__ void __launch_bounds__(256) k_c_opn(BArgs<float> a, CArgs c) {
     {
         const double f = L::fold(v[0]);
         v[0] = sub == 0 ? f : 0.0;
-    }
-    double *const dst[1] = {a.part0};
-    block_part<1>(v, dst, a.Rp);
-}
-
-// FP32 direction, FP64 operator: q = S64 p with the FP64 element table and
-// K, rounded to FP32 once (isolates the operator's rounding from the FP32
-// recursion's).
-struct XWiden {
-    const float *__restrict__ x;
-    __device__ __forceinline__ double operator()(int, size_t idx) const { return (double)x[idx]; }
-};
-__global__ void __launch_bounds__(256) k_c_op_w(BArgs<float> a, CArgs c) {
-    const int lane = threadIdx.x & 63;
-    const int r = blockIdx.y * 64 + lane;
-    const bool act = a.sc.status[r] == kRun;
-    double v[1] = {0.0};
-    if (__any(act)) {
-        const XWiden xs{a.p0};
-        int s0, st, en;
-        row_range(a.F, a.G, a.xcd, s0, st, en);
-        for (int row = s0; row < en; row += st) {
-            if (!act) continue;
-            const double q = op_row<double>(a.rec, a.row_ptr, a.row_c, a.meta, a.op_off, a.conn_off, a.conn,
-                                            a.opT64, a.K64, a.opT64, a.K64, row, xs, a.Rp, r);
-            const size_t o = (size_t)row * a.Rp + r;
-            const float qf = (float)q;
-            a.q[o] = qf;
-            v[0] += (double)a.p0[o] * (double)qf;
-        }
     }
     double *const dst[1] = {a.part0};
     block_part<1>(v, dst, a.Rp);
@@ -2223,17 +1910,15 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
     } else if (MODE == kFoldBeta) {
         const int it = sc.it[slot] + 1;
         sc.it[slot] = it;
-        const double zq = t[2] + t[4];
         sc.rz_prev[slot] = sc.rz[slot];
         sc.rz[slot] = rz;
-        // flexible CG: beta = z_new . (r_new - r_old) / z_old . r_old, r_new - r_old = -alpha q
-        sc.beta[slot] = sc.beta_fr ? rz / sc.rz_prev[slot] : -sc.alpha[slot] * zq / sc.rz_prev[slot];
+        // Fletcher-Reeves: the two-level preconditioner is one fixed SPD operator
+        sc.beta[slot] = rz / sc.rz_prev[slot];
         sc.rr[slot] = rr;
         if (!(rr == rr)) st = kBreakdown;
         else if (rr <= sc.tol2[slot]) st = kCandidate;  // confirmed on the true residual at the chunk end
         else if (it >= max_it) st = kMaxIt;
         else if (mixed && it - sc.it_ru[slot] > 1000 + sc.it_ru[slot]) st = kFp64;
-        else if (mixed && !sc.no_pause && rr < sc.ru_drop2 * sc.rr_ru[slot]) st = kPause;  // direction kept
     } else {  // reliable update
         sc.n_ru[slot] += 1;
         sc.rr_ru[slot] = rr;
@@ -2496,8 +2181,8 @@ void DarcyBatchGPU::build() {
     // the same L2 lines.  perm: row -> edge; conn is renumbered to rows.
     std::vector<int> perm(F);
     std::iota(perm.begin(), perm.end(), 0);
-    std::vector<uint64_t> key;  // Z-curve key per edge (empty: edge order kept)
-    if (!std::getenv("FL_BATCH_EDGE_ORDER")) {
+    std::vector<uint64_t> key;  // Z-curve key per edge
+    {
         std::vector<double> ec(3 * (size_t)F, 0.0);
         std::vector<int> cnt(F, 0);
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -2756,22 +2441,6 @@ void DarcyBatchGPU::build() {
     d_var_ = ad(R_);
     d_pairs_ = ai(2 * (size_t)Rp_);
     d_Ksave_ = ad(E);
-    if (const char *h = std::getenv("FL_BATCH_HANDOVER")) handover_ = std::max(-1, std::atoi(h));
-    if (const char *w = std::getenv("FL_BATCH_NARROW")) narrow_ = std::atoi(w) != 0;
-    if (const char *w = std::getenv("FL_BATCH_RUDROP")) {  // drop in norm that triggers a reliable update
-        const double d = std::atof(w);
-        if (d > 1.0) ru_drop2_ = 1.0 / (d * d);
-    }
-    if (const char *w = std::getenv("FL_BATCH_BETA_FR")) beta_fr_ = std::atoi(w) != 0;
-    if (const char *w = std::getenv("FL_BATCH_EARLY")) early_ = std::max(0, std::atoi(w));
-    if (const char *w = std::getenv("FL_BATCH_NOPAUSE")) no_pause_ = std::atoi(w) != 0;
-    if (const char *w = std::getenv("FL_BATCH_OPPAIR")) opfast_ = std::min(5, std::max(0, std::atoi(w)));
-    if (const char *w = std::getenv("FL_BATCH_UNROLL")) unroll_ = std::atoi(w) != 0;
-    if (const char *w = std::getenv("FL_BATCH_QS")) {
-        const int q = std::atoi(w);
-        qs_ = (q == 8 || q == 16) ? q : 4;
-    }
-    trace_ = std::getenv("FL_BATCH_TRACE") != nullptr;
     d_lsel_ = ai(Rp_);
     d_ext_ = ad((size_t)S_ + E + F);
     d_part_ = ad(3 * (size_t)kMaxGrid * Rp_);
@@ -2783,17 +2452,6 @@ void DarcyBatchGPU::build() {
     d_sp64_ = ad(nv);
     d_spK_ = ad((size_t)E * Rp_);
     FL_HIP(hipHostMalloc((void **)&h_count_, 2 * nchunk_ * sizeof(int), hipHostMallocDefault));
-    if (const char *g = std::getenv("FL_BATCH_GRID")) {  // launch-shape override: grid[,xcd[,chunk]]
-        int gv = 0, xv = 0, cv = 16;
-        if (std::sscanf(g, "%d,%d,%d", &gv, &xv, &cv) >= 1) {
-            set_launch(gv, xv, cv);
-            tuned_ = true;
-        }
-    }
-    if (const char *c = std::getenv("FL_BATCH_CHUNK")) {  // iterations per captured chunk only (shape still tuned)
-        const int cv = std::atoi(c);
-        if (cv >= 1 && cv <= 1024) chunk_ = cv;
-    }
     FL_HIP(hipStreamSynchronize(s));
 }
 
@@ -2821,8 +2479,6 @@ void DarcyBatchGPU::alloc_scal(BatchScal &sc) {
     sc.zp = sci + 5 * Rp_;
     sc.active_count = sci + 7 * Rp_;
     sc.ru_drop2 = kRuDrop2;
-    sc.beta_fr = 0;
-    sc.no_pause = 0;
 }
 
 void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
@@ -3073,9 +2729,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     nch_ = nchunk_;  // chunks of 64 slots that hold live realizations (compaction shrinks it)
     lw_ = 64;        // lanes per row of the two-level kernels (narrowed for the tail)
     // FP64 runs no residual replacement: only candidates are checked
-    sc_.ru_drop2 = sc2_.ru_drop2 = precision == 1 ? ru_drop2_ : 0.0;
-    sc_.beta_fr = sc2_.beta_fr = beta_fr_ ? 1 : 0;
-    sc_.no_pause = sc2_.no_pause = no_pause_ ? 1 : 0;
+    sc_.ru_drop2 = sc2_.ru_drop2 = precision == 1 ? kRuDrop2 : 0.0;
     if (norm != 0 && norm != 1) throw Error(-1, "stop norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
     sc_.nat = sc2_.nat = norm;
     BArgs<double> a64;
@@ -3130,7 +2784,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const int qs = lw_ < 64 ? 16 : qs_;
+        const int qs = lw_ < 64 ? 16 : 4;
         const int parts = (nc_ + kCoarseThreads / qs - 1) / (kCoarseThreads / qs);
         const dim3 gs(parts, 64 * nch_), blkc(kCoarseThreads), gfin(nch_), b64(64);
         const double *p1 = a64.part1, *p2 = a64.part2;
@@ -3188,13 +2842,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                 hipLaunchKernelGGL(k_c_opg<2>, dim3(gen_blocks_), blk, 0, stream2_, a, ot, ot.gen, ot.grec, a.opT,
                                    a.opT64);
         }
-        if (kOpRegisterTiles && nch_ == 2)
-            hipLaunchKernelGGL(k_c_opr<2>, dim3(ntiles_), blk, 0, stream_, a, ot);
-        else if (kOpRegisterTiles && nch_ == 1)
-            hipLaunchKernelGGL(k_c_opr<1>, dim3(ntiles_), blk, 0, stream_, a, ot);
-        else
-            hipLaunchKernelGGL(k_c_opl, dim3(ntiles_), dim3(kOplThreads), opl_lds_bytes(nseg_max_, tile_T_), stream_,
-                               a, ot, nch_);
+        hipLaunchKernelGGL(k_c_opl, dim3(ntiles_), dim3(kOplThreads), opl_lds_bytes(nseg_max_, tile_T_), stream_, a,
+                           ot, nch_);
         if (ngen_) {
             FL_HIP(hipEventRecord(ev_join_, stream2_));
             FL_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
@@ -3208,14 +2857,13 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     // the fold (with the coarse solve and the direction kernel)
     auto iter64 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC && opfast_) FL_LW(hipLaunchKernelGGL((k_c_op<double, LW, 1>), grow(), blk, 0, stream_, a64, ca))
-        else if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<double, LW, 0>), grow(), blk, 0, stream_, a64, ca))
+        if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<double, LW, 1>), grow(), blk, 0, stream_, a64, ca))
         else hipLaunchKernelGGL(k_b_op<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
         fold(kFoldAlpha, 0, nosel);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        if (NC == 1) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 1, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 1, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
-        else if (NC == 2) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 2, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 2, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
+        if (NC == 1) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 1, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 1, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
+        else if (NC == 2) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<double, 2, 64, 2>), gagg(), blkA, 0, stream_, a64, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<double, 2, LW, 1>), gagg(), blkA, 0, stream_, a64, ca)) }
         else hipLaunchKernelGGL(k_b_update<double>, grow(), blk, 0, stream_, a64);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
         if (NC) {
@@ -3228,23 +2876,19 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC && opfast_ >= 2 && lw_ == 16)
+        if (NC && lw_ == 16)
             hipLaunchKernelGGL(k_c_opn<16>, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ >= 2 && lw_ == 4)
+        else if (NC && lw_ == 4)
             hipLaunchKernelGGL(k_c_opn<4>, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ == 5 && lw_ == 64)
-            hipLaunchKernelGGL(k_c_op_w, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ == 4 && lw_ == 64)
-            hipLaunchKernelGGL(k_c_op3, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && opfast_ >= 2 && lw_ == 64) op_tiled(a32);
-        else if (NC && opfast_) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 1>), grow(), blk, 0, stream_, a32, ca))
-        else if (NC) FL_LW(hipLaunchKernelGGL((k_c_op<float, LW, 0>), grow(), blk, 0, stream_, a32, ca))
-        else hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
+        else if (NC)
+            op_tiled(a32);
+        else
+            hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
-        fold(kFoldAlpha, 1, nosel, NC && opfast_ >= 2 && lw_ == 64 ? op_tiled_grid() : 0);
+        fold(kFoldAlpha, 1, nosel, NC && lw_ == 64 ? op_tiled_grid() : 0);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
-        if (NC == 1) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 1, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 1, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
-        else if (NC == 2) { if (unroll_ && lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 2, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 2, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
+        if (NC == 1) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 1, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 1, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
+        else if (NC == 2) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 2, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 2, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
         else hipLaunchKernelGGL(k_b_update<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[3], stream_));
         if (NC) {
@@ -3542,7 +3186,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     // move live slots to the front when that frees whole 64-slot chunks
     auto live = [&](int s) {
-        return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64 || stv[s] == kPause);
+        return rid_[s] >= 0 && (stv[s] == kRun || stv[s] == kCandidate || stv[s] == kFp64);
     };
     auto maybe_compact = [&](int phase) {
         if (nch_ <= 1) return;
@@ -3603,7 +3247,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         refill(pairs, mixed);
     }
     if (!tuned_) {
-        tune(precision);  // times the operator on the filled slots; the CG state is unchanged
+        tune();  // times the operator on the filled slots; the CG state is unchanged
         rebuild();
     }
     // live kernel timing: the first iterations run eagerly between events
@@ -3639,7 +3283,6 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     // queue, compact once the queue is empty
     auto terminal = [&](int v) { return v == kConverged || v == kMaxIt || v == kBreakdown; };
     std::vector<int> free_slots;  // harvested slots waiting for a refill
-    bool early_done = false;
     long guard = 0;
     const long guard_max = ((long)max_it + CH) * ((R_ + Rp_ - 1) / Rp_ + 2) / CH + 16;
     while (true) {
@@ -3705,28 +3348,6 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             }
             read_status();
         }
-        if (!early_done && early_ > 0 && next >= queue.size() && nrun > early_) {
-            // the queue just ran empty: the realizations furthest from their
-            // tolerance go to the single-solve worker now, so the batch drains
-            // the rest while they finish beside it
-            early_done = true;
-            FL_HIP(hipMemcpyAsync(rrv.data(), sc_.rr, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            FL_HIP(hipMemcpyAsync(tolv.data(), sc_.tol2, Rp_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            FL_HIP(hipStreamSynchronize(stream_));
-            std::vector<std::pair<double, int>> far;
-            for (int sl = 0; sl < Rp_; sl++)
-                if (rid_[sl] >= 0 && (stv[sl] == kRun || stv[sl] == kPause))
-                    far.push_back({-(rrv[sl] / std::max(tolv[sl], 1e-300)), sl});
-            std::sort(far.begin(), far.end());
-            std::vector<int> pairs;
-            for (int k = 0; k < early_ && k < (int)far.size(); k++) {
-                pairs.push_back(far[k].second);
-                pairs.push_back(rid_[far[k].second]);
-            }
-            handover(pairs, mixed);
-            nrun -= (int)pairs.size() / 2;
-            read_status();
-        }
         if (nrun == 0) break;
         const bool narrow = narrow_ && NC > 0;
         const int hand = handover_ >= 0 ? handover_ : (narrow ? 4 : 16);
@@ -3742,10 +3363,6 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         }
         if (next >= queue.size() && nrun + nf64 <= 64 * (nch_ - 1)) maybe_compact(mixed);
         if (narrow && next >= queue.size()) maybe_narrow(mixed);
-        if (trace_)
-            std::fprintf(stderr, "batch %.1f ms: running %d, chunks %d, lanes/row %d, queued %zu\n",
-                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(),
-                         nrun, nch_, lw_, queue.size() - next);
         if (++guard > guard_max) throw Error(-5, "batched CG did not terminate");
         FL_HIP(hipGraphLaunch(graph(mixed), stream_));
         st.chunks++;
@@ -3798,8 +3415,15 @@ void DarcyBatchGPU::autotune() {
     (void)solve(1e-30, 0.0, 1, 1);  // fills the slots, tunes, one iteration
 }
 
-void DarcyBatchGPU::tune(int precision) {
+void DarcyBatchGPU::tune() {
     alloc32();
+    if (ncol_) {
+        // two-level path: the FP32 operator's grids are fixed by its tiles;
+        // the launch grid serves the direction kernel and the FP64 operator
+        set_launch(1024, 0, chunk_);
+        tuned_ = true;
+        return;
+    }
     const int cand_g[] = {512, 1024, 2048};
     float best = 1e30f;
     int bg = grid_, bx = xcd_map_;
@@ -3811,22 +3435,9 @@ void DarcyBatchGPU::tune(int precision) {
             BArgs<float> a = make_args<float>(F_, Rp_, G, x, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_,
                                               d_conn_off_, d_conn_, d_tab32_, d_K32_, d_tab_, d_K_, d_dinv32_,
                                               d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_, d_q32_, d_part_, sc_);
-            a.frec = d_frec_;
-            const CArgs ca{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_,
-                           d_qc_, d_y_, d_part3_};
             const dim3 grow(G, nch_), blk(256);
-            // the operator kernel the iterations will run (it only rewrites q
-            // and, for the one-level kernel, the spare direction buffer)
-            auto op = [&]() {
-                if (ncol_ && opfast_ == 4 && precision == 1)
-                    hipLaunchKernelGGL(k_c_op3, grow, blk, 0, stream_, a, ca);
-                else if (ncol_ && opfast_ == 2 && precision == 1 && nch_ % 2 == 0)
-                    hipLaunchKernelGGL(k_c_op2<2>, dim3(G, nch_ / 2), blk, 0, stream_, a, ca);
-                else if (ncol_ && opfast_ >= 2 && precision == 1)
-                    hipLaunchKernelGGL(k_c_op2<1>, grow, blk, 0, stream_, a, ca);
-                else
-                    hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a);
-            };
+            // the one-level operator only rewrites q and the spare direction buffer
+            auto op = [&]() { hipLaunchKernelGGL(k_b_op<float>, grow, blk, 0, stream_, a); };
             op();  // warm
             FL_HIP(hipEventRecord(t0, stream_));
             for (int k = 0; k < 3; k++) op();

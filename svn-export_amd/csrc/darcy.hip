@@ -279,10 +279,6 @@ static T *up(std::vector<void *> &own, const std::vector<T> &v, hipStream_t s) {
 DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
     FL_HIP(hipGetDevice(&device_));
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    if (const char *g = std::getenv("FL_CG_GRID")) {  // tuning knob: CG grid size (multiple of 8)
-        int v = std::atoi(g);
-        if (v >= 8 && v <= 8192 && v % 8 == 0) cg_grid_ = v;
-    }
     build_pattern();
 }
 

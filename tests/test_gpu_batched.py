@@ -301,3 +301,31 @@ def test_batch_two_level_narrow_tail(cube6, precision):
             r, nar["iterations"][r], *errs), flush=True)
         assert max(errs) <= 1e-8
     g.set_conductivity(d["K"])
+
+
+@pytest.mark.parametrize("coarse", [0, 1])
+def test_batch_handover8_fp32(cube6, coarse):
+    """Hand-over of the last 8 live realizations in the FP32 batch (one-level,
+    and two-level with its narrowed tail): the batch terminates with every
+    realization converged, equal to the batch run without hand-over."""
+    om, gm, d, g = cube6
+    R = 96
+    K = cases.mc_conductivity(om, R, seed=21)
+    b = G.DarcyBatch(g, R, slots=64)
+    b.set_conductivity(K)
+    b.set_coarse(coarse)
+    b.set_handover(0)
+    i0 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=1)
+    x0 = [b.get_solution_vector(r) for r in range(R)]
+    b.set_handover(8)
+    i1 = b.update_solution(r_tol=1e-13, a_tol=0.0, max_it=100000, precision=1)
+    x1 = [b.get_solution_vector(r) for r in range(R)]
+    print("coarse %d: handovers %d (%d iterations), chunks %d / %d" % (
+        coarse, i1["handovers"], i1["handover_iterations"], i1["chunks"], i0["chunks"]), flush=True)
+    assert i0["converged"].all() and i1["converged"].all()
+    assert i1["handovers"] >= 8
+    S = om.n_sides
+    worst = max(rel(a[:S], c[:S]) for a, c in zip(x1, x0))
+    print("  worst flux difference %.2e" % worst, flush=True)
+    assert worst <= 1e-8
+    g.set_conductivity(d["K"])
