@@ -374,7 +374,10 @@ int fl_darcy_batch_set_narrow(fl_darcy_batch *b, int on);
 /* Two-level preconditioner of the batched CG: ncol 0 = Jacobi only, 1 = plus
  * one constant coarse column per aggregate, 2 = plus a second column phi per
  * aggregate (fl_darcy_batch_set_coarse_basis; e.g. the learned basis of
- * fl_gnn_*).  M^-1 = D^-1 + Z (Z^T S Z)^-1 Z^T, applied in flexible CG. */
+ * fl_gnn_*).  M^-1 = D^-1 + Z (Z^T S Z)^-1 Z^T, one fixed SPD operator per
+ * realization (Fletcher-Reeves CG).  The coarse inverses are formed on the
+ * GPU (blocked FP64 Cholesky, L^-1, L^-T L^-1) and applied from their packed
+ * lower 64 x 64 tiles in FP32. */
 int fl_darcy_batch_set_coarse(fl_darcy_batch *b, int ncol);
 /* phi[n_real][F] (FP32, device pointer, edge order); normalised per aggregate. */
 int fl_darcy_batch_set_coarse_basis(fl_darcy_batch *b, const float *phi);
@@ -384,6 +387,13 @@ int fl_darcy_batch_aggregates(fl_darcy_batch *b, int *edge_agg);
 /* Edge traces lambda of every realization of the last solve into
  * out[n_real][F] (device pointer, edge order). */
 int fl_darcy_batch_lambda(fl_darcy_batch *b, double *out);
+/* Inverse of np SPD matrices a[np][n][n] (host, row-major, lower triangle
+ * read, n <= 2048) by the coarse space's batched Cholesky + inverse kernels,
+ * rounded to FP32 into inv[np][n][n] (host); info[np]: 0, or 1 for a matrix
+ * that is not positive definite (its inverse is returned as zeros).  No
+ * counterpart in the reference: it checks the coarse-space factorization
+ * against a host inverse. */
+int fl_coarse_invert(const double *a, int n, int np, float *inv, int *info);
 
 /* ---- learned coarse basis (graph network) --------------------------------- */
 

@@ -63,6 +63,10 @@ struct BatchScal {
 // Batched CG status per realization
 enum BatchStatus { kRun = 0, kConverged = 1, kCandidate = 2, kFp64 = 3, kMaxIt = 4, kBreakdown = 5, kIdle = 6 };
 
+// Inverse of np SPD matrices on the host (a[np][n][n], lower triangle read)
+// through the coarse space's batched factorization kernels, FP32 out.
+void coarse_invert_host(const double *a, int n, int np, float *inv, int *info);
+
 class DarcyBatchGPU {
    public:
     // base: a configured single solver (set_data called); its mesh, fields and
@@ -150,7 +154,7 @@ class DarcyBatchGPU {
     double *d_sol_ = nullptr, *d_ext_ = nullptr;
     double *d_part_ = nullptr;  // 3 partial arrays of 2048 * Rp
     double *d_pre_ = nullptr;   // first-stage fold sums, 3 arrays of 32 * Rp
-    double *d_cpart_ = nullptr; // coarse-solve sums, 5 x (row blocks) x Rp
+    double *d_cpart_ = nullptr; // coarse-solve sums, 3 x (row tiles) x Rp
     int lw_ = 64;               // lanes per row of the two-level iteration kernels (64, 16, 4)
     bool narrow_ = true;        // two-level path: narrow lanes for the tail instead of the hand-over
     // tiled FP32 operator: general rows (vb-coupled elements) and row tiles
@@ -178,11 +182,11 @@ class DarcyBatchGPU {
     int *d_agg_ = nullptr, *d_aptr_ = nullptr, *d_nbptr_ = nullptr, *d_nb_ = nullptr, *d_lbmap_ = nullptr;
     float *d_phiAll_ = nullptr, *d_ainv_ = nullptr, *d_spainv_ = nullptr;
     unsigned short *d_phi_ = nullptr, *d_spphi_ = nullptr;  // phi per slot, bfloat16 [F][Rp]
-    double *d_Asp_ = nullptr, *d_rc_ = nullptr, *d_qc_ = nullptr, *d_y_ = nullptr, *d_spy_ = nullptr;
-    double *d_part3_ = nullptr, *d_Apack_ = nullptr;
-    double **d_Aptrs_ = nullptr;
-    int *d_info_ = nullptr;
-    void *solver_ = nullptr;  // rocblas handle
+    double *d_Asp_ = nullptr, *d_rc_ = nullptr, *d_y_ = nullptr, *d_spy_ = nullptr;
+    double *d_Apack_ = nullptr, *d_Wpack_ = nullptr;  // coarse matrices / their inverse factors, [Rp][ncp][ncp]
+    double *d_py_ = nullptr;                           // coarse-solve tile products [Rp][nt][nt][64]
+    int *d_info_ = nullptr;                            // per matrix: not positive definite
+    int nt_ = 0;                                       // 64-row tiles of the padded coarse order
     int *d_pairs_ = nullptr, *d_lsel_ = nullptr;   // pair list [2 * slots], slot flags [slots]
     int npairs_ = 0;
     // per-realization results of the last solve

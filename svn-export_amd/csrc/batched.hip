@@ -18,10 +18,9 @@
 #include <thread>
 #include <unordered_map>
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "batched.hh"
+#include "coarse.hh"
 #include "common.hh"
 #include "condense.hh"
 
@@ -996,11 +995,11 @@ constexpr int kNbMax = 32;  // neighbour aggregates per aggregate (27 on a regul
 
 struct CArgs {
     int ncol, nagg, nc;
+    int nt;                                      // 64-row tiles of the padded coarse order
     const int *agg, *aptr, *nbptr, *nb, *lbmap;  // lbmap[a * nagg + b]: local index of b among a's neighbours
     const unsigned short *phi;                   // [F][Rp] bfloat16 (ncol 2) or null
-    const float *ainv;                           // [Rp][nc][nc]
-    double *rc, *qc, *y;                         // [nc][Rp]
-    double *part3;                               // [nagg][Rp]
+    const float *ainv;                           // [Rp][nt (nt + 1) / 2][64][64]: packed lower tiles of the inverse
+    double *rc, *y;                              // [nt * 64][Rp]: Z^T r and the coarse solution
 };
 
 // entry (i, j) of the condensed element block S_e(K) (FP64 operator table)
@@ -1061,37 +1060,27 @@ __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, con
     for (int q = 0; q < NCOL * NCOL; q++) Asp[((size_t)(ag * kNbMax + lb) * (NCOL * NCOL) + q) * a.Rp + slot] = acc[q];
 }
 
-// Dense coarse matrices of the listed slots, packed [np][nc][nc] for the
-// batched Cholesky; an empty coarse column (phi constant on an aggregate)
-// becomes a decoupled unit diagonal, which Z maps to zero.
+// Dense coarse matrices of the listed slots, packed [np][ncp][ncp] (ncp = 64
+// nt) for the batched Cholesky; an empty coarse column (phi constant on an
+// aggregate) becomes a decoupled unit diagonal, which Z maps to zero, and so
+// do the padding rows past nc.
 template <int NCOL>
 __global__ void k_c_pack(CArgs c, const double *__restrict__ Asp, size_t Rp, const int *__restrict__ pairs,
                          double *__restrict__ Apack) {
-    const int i = blockIdx.x, k = blockIdx.y;
+    const int i = blockIdx.x, k = blockIdx.y, ncp = c.nt * 64;
     const int slot = pairs[2 * k];
     const int ai = i / NCOL, s = i % NCOL;
-    double *row = Apack + ((size_t)k * c.nc + i) * c.nc;
-    for (int j = threadIdx.x; j < c.nc; j += blockDim.x) {
-        const int bj = j / NCOL, t = j % NCOL;
-        const int lb = c.lbmap[(size_t)ai * c.nagg + bj];
-        double v = lb >= 0 ? Asp[((size_t)(ai * kNbMax + lb) * (NCOL * NCOL) + s * NCOL + t) * Rp + slot] : 0.0;
+    double *row = Apack + ((size_t)k * ncp + i) * ncp;
+    for (int j = threadIdx.x; j < ncp; j += blockDim.x) {
+        double v = 0.0;
+        if (i < c.nc && j < c.nc) {
+            const int bj = j / NCOL, t = j % NCOL;
+            const int lb = c.lbmap[(size_t)ai * c.nagg + bj];
+            if (lb >= 0) v = Asp[((size_t)(ai * kNbMax + lb) * (NCOL * NCOL) + s * NCOL + t) * Rp + slot];
+        }
         if (i == j && !(v > 0.0)) v = 1.0;
         row[j] = v;
     }
-}
-
-// Inverse from the batched Cholesky (column-major lower triangle) to the full
-// symmetric FP32 matrix of each listed slot; a failed factorization leaves
-// that slot with the one-level (Jacobi) preconditioner.
-__global__ void k_c_unpack(int nc, const double *__restrict__ Apack, const int *__restrict__ info,
-                           const int *__restrict__ info2, const int *__restrict__ pairs, float *__restrict__ ainv) {
-    const int i = blockIdx.x, k = blockIdx.y;
-    const int slot = pairs[2 * k];
-    const double *A = Apack + (size_t)k * nc * nc;
-    float *out = ainv + ((size_t)slot * nc + i) * nc;
-    const bool ok = info[k] == 0 && info2[k] == 0;
-    for (int j = threadIdx.x; j < nc; j += blockDim.x)
-        out[j] = ok ? (float)(i >= j ? A[(size_t)j * nc + i] : A[(size_t)i * nc + j]) : 0.0f;
 }
 
 // phi of the selected slots: per aggregate zero mean and unit mean square
@@ -1117,39 +1106,35 @@ __global__ void k_c_phi_norm(CArgs c, size_t Rp, const int *__restrict__ lsel, u
     }
 }
 
-// Shared tail of the aggregate-blocked kernels: block reduction of three
-// partials and the 2 x ncol restrictions, written per aggregate.  With W < 64
+// Shared tail of the aggregate-blocked kernels: block reduction of two
+// partials and the ncol restrictions, written per aggregate.  With W < 64
 // the lanes of one slot are summed first; the other lanes write zeros into the
 // partials of slots that are idle in that mode.
 template <int NCOL, int W = 64>
-__device__ __forceinline__ void agg_part(const double (&vin)[3], const double (&rcin)[2], const double (&qcin)[2],
-                                         double *p1, double *p2, double *p3, CArgs &c, size_t Rp) {
-    __shared__ double sh[3 + 2 * NCOL][kAggWaves][64];
+__device__ __forceinline__ void agg_part(const double (&vin)[2], const double (&rcin)[2], double *p1, double *p2,
+                                         CArgs &c, size_t Rp) {
+    __shared__ double sh[2 + NCOL][kAggWaves][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double v[3] = {vin[0], vin[1], vin[2]}, rc[2] = {rcin[0], rcin[1]}, qc[2] = {qcin[0], qcin[1]};
+    double v[2] = {vin[0], vin[1]}, rc[2] = {rcin[0], rcin[1]};
     if (W < 64) {
         const bool keep = LaneMap<W>::sub(lane) == 0;
         // every lane takes part in the shuffles; only then the selection
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 2; k++) {
             const double f = LaneMap<W>::fold(v[k]);
             v[k] = keep ? f : 0.0;
         }
         for (int s = 0; s < NCOL; s++) {
-            const double f = LaneMap<W>::fold(rc[s]), h = LaneMap<W>::fold(qc[s]);
+            const double f = LaneMap<W>::fold(rc[s]);
             rc[s] = keep ? f : 0.0;
-            qc[s] = keep ? h : 0.0;
         }
     }
     const int slot = blockIdx.y * 64 + lane;
-    for (int k = 0; k < 3; k++) sh[k][wid][lane] = v[k];
-    for (int s = 0; s < NCOL; s++) {
-        sh[3 + s][wid][lane] = rc[s];
-        sh[3 + NCOL + s][wid][lane] = qc[s];
-    }
+    for (int k = 0; k < 2; k++) sh[k][wid][lane] = v[k];
+    for (int s = 0; s < NCOL; s++) sh[2 + s][wid][lane] = rc[s];
     __syncthreads();
     if (wid == 0) {
-        double t[3 + 2 * NCOL];
-        for (int k = 0; k < 3 + 2 * NCOL; k++) {
+        double t[2 + NCOL];
+        for (int k = 0; k < 2 + NCOL; k++) {
             double x = sh[k][0][lane];
             for (int w = 1; w < kAggWaves; w++) x += sh[k][w][lane];
             t[k] = x;
@@ -1157,16 +1142,12 @@ __device__ __forceinline__ void agg_part(const double (&vin)[3], const double (&
         const size_t o = (size_t)blockIdx.x * Rp + slot;
         p1[o] = t[0];
         p2[o] = t[1];
-        p3[o] = t[2];
-        for (int s = 0; s < NCOL; s++) {
-            c.rc[(size_t)(blockIdx.x * NCOL + s) * Rp + slot] = t[3 + s];
-            c.qc[(size_t)(blockIdx.x * NCOL + s) * Rp + slot] = t[3 + NCOL + s];
-        }
+        for (int s = 0; s < NCOL; s++) c.rc[(size_t)(blockIdx.x * NCOL + s) * Rp + slot] = t[2 + s];
     }
 }
 
-// r -= alpha q; partials D^-1 r.r, r.r, D^-1 r.q and the restrictions Z^T r,
-// Z^T q, one block per (aggregate, 64 slots).  x += alpha p is done by the
+// r -= alpha q; partials D^-1 r.r, r.r and the restriction Z^T r, one block
+// per (aggregate, 64 slots).  x += alpha p is done by the
 // direction kernel k_c_zp, which reads p anyway.
 template <class V, int NCOL, int W, int U>
 __global__ void __launch_bounds__(kAggWaves * 64) k_c_update(BArgs<V> a, CArgs c) {
@@ -1174,7 +1155,7 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_update(BArgs<V> a, CArgs c
     const int lane = threadIdx.x & 63;
     const int slot = L::slot(lane), sub = L::sub(lane);
     const bool act = a.sc.status[slot] == kRun;
-    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    double v[2] = {0.0, 0.0}, rc[2] = {0.0, 0.0};
     if (U == 2 && W == 64 && __any(act)) {
         // two rows per step, all loads first (row and row + kAggWaves)
         const V alpha = act ? (V)a.sc.alpha[slot] : V(0);
@@ -1197,24 +1178,14 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_update(BArgs<V> a, CArgs c
             const V dr0 = d0 * rv0, dr1 = d1 * rv1;
             v[0] += (double)dr0 * (double)rv0;
             v[1] += (double)rv0 * (double)rv0;
-            v[2] += (double)dr0 * (double)q0;
             rc[0] += (double)rv0;
-            qc[0] += (double)q0;
-            if constexpr (NCOL == 2) {
-                rc[1] += f0 * (double)rv0;
-                qc[1] += f0 * (double)q0;
-            }
+            if constexpr (NCOL == 2) rc[1] += f0 * (double)rv0;
             if (two) {
                 a.r[o1] = rv1;
                 v[0] += (double)dr1 * (double)rv1;
                 v[1] += (double)rv1 * (double)rv1;
-                v[2] += (double)dr1 * (double)q1;
                 rc[0] += (double)rv1;
-                qc[0] += (double)q1;
-                if constexpr (NCOL == 2) {
-                    rc[1] += f1 * (double)rv1;
-                    qc[1] += f1 * (double)q1;
-                }
+                if constexpr (NCOL == 2) rc[1] += f1 * (double)rv1;
             }
         }
     } else if (__any(act)) {
@@ -1231,17 +1202,11 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_update(BArgs<V> a, CArgs c
             const V dr = Pre<V>::ld(a.dinv, o) * rv;
             v[0] += (double)dr * (double)rv;
             v[1] += (double)rv * (double)rv;
-            v[2] += (double)dr * (double)q;
             rc[0] += (double)rv;
-            qc[0] += (double)q;
-            if constexpr (NCOL == 2) {
-                const double ph = (double)bf16_to_f(c.phi[o]);
-                rc[1] += ph * (double)rv;
-                qc[1] += ph * (double)q;
-            }
+            if constexpr (NCOL == 2) rc[1] += (double)bf16_to_f(c.phi[o]) * (double)rv;
         }
     }
-    agg_part<NCOL, W>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
+    agg_part<NCOL, W>(v, rc, a.part1, a.part2, c, a.Rp);
 }
 
 // cold start of the selected slots: x = 0, r = b, p = 0; partials and Z^T b
@@ -1251,7 +1216,7 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_init(BArgs<V> a, CArgs c, 
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.y * 64 + lane;
     const bool act = lsel[slot] != 0;
-    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    double v[2] = {0.0, 0.0}, rc[2] = {0.0, 0.0};
     if (__any(act)) {
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         for (int row = c.aptr[blockIdx.x] + wid; row < c.aptr[blockIdx.x + 1]; row += kAggWaves) {
@@ -1270,7 +1235,7 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_init(BArgs<V> a, CArgs c, 
             if (NCOL == 2) rc[1] += (double)bf16_to_f(c.phi[o]) * (double)bv;
         }
     }
-    agg_part<NCOL>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
+    agg_part<NCOL>(v, rc, a.part1, a.part2, c, a.Rp);
 }
 
 // True FP64 residual of the reliable-update selection (mixed precision):
@@ -1282,7 +1247,7 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_resid(BArgs<V> a, CArgs c,
     const int lane = threadIdx.x & 63;
     const int slot = L::slot(lane), sub = L::sub(lane);
     const bool act = ru_pick(a.sc, slot, a.sc.status[slot]);
-    double v[3] = {0.0, 0.0, 0.0}, rc[2] = {0.0, 0.0}, qc[2] = {0.0, 0.0};
+    double v[2] = {0.0, 0.0}, rc[2] = {0.0, 0.0};
     if (__any(act)) {
         const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const XPlain<double> xs{x64};
@@ -1302,7 +1267,7 @@ __global__ void __launch_bounds__(kAggWaves * 64) k_c_resid(BArgs<V> a, CArgs c,
             if (NCOL == 2) rc[1] += (double)bf16_to_f(c.phi[o]) * (double)rv;
         }
     }
-    agg_part<NCOL, W>(v, rc, qc, a.part1, a.part2, c.part3, c, a.Rp);
+    agg_part<NCOL, W>(v, rc, a.part1, a.part2, c, a.Rp);
 }
 
 // q = S p for the two-level path (the direction is formed by k_c_zp); partial p.q
@@ -1786,19 +1751,17 @@ __global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
     }
 }
 
-// Coarse solve y = (Z^T S Z)^-1 Z^T r of the slots the fold will step (FP32
-// inverse, FP64 accumulation), split over row blocks of the inverse so that
-// many CUs stream it: block (part, slot) forms the kCoarseThreads / QS rows
-// [part * rows, (part + 1) * rows) of y, QS threads per row (each one QS-th of
-// the columns), and writes its share of the per-slot sums
-//   0: D^-1 r . r   1: r . r   2: D^-1 r . q   3: y . Z^T r   4: y . Z^T q
-// to cpart[(k * parts + part) * Rp + slot].  QS = 4 with all slots busy;
-// QS = 16 for the few slots of a narrowed tail (more, shorter blocks per
-// slot).  MODE: kFoldInit (selected slots), kFoldBeta (running slots),
-// kFoldRU (reliable-update selection).
-constexpr int kCoarseThreads = 1024;
-constexpr int kMaxCoarseParts = 2048 / (kCoarseThreads / 16);  // nc <= 2048, QS <= 16
-
+// Coarse solve y = (Z^T S Z)^-1 Z^T r of the slots the fold will step, from
+// the packed lower tiles of the FP32 inverse (FP64 accumulation), in two
+// kernels.  k_c_solve_t: block (tile (I, J), slot) streams its 64 x 64 tile
+// once and forms the tile's row product X_IJ r_J for y_I and, off the
+// diagonal, its transposed product X_IJ^T r_I for y_J, into the per-slot
+// table py[slot][block row][block column][64].  k_c_solve_y: block (I, slot)
+// sums y_I over the block columns in order (deterministic) and writes its
+// share of the per-slot sums
+//   0: D^-1 r . r   1: r . r   2: y . Z^T r
+// to cpart[(k * nt + I) * Rp + slot].  MODE: kFoldInit (selected slots),
+// kFoldBeta (running slots), kFoldRU (reliable-update selection).
 template <int MODE>
 __device__ __forceinline__ bool coarse_act(const BatchScal &sc, int slot, const int *__restrict__ lsel) {
     const int st0 = sc.status[slot];
@@ -1807,66 +1770,87 @@ __device__ __forceinline__ bool coarse_act(const BatchScal &sc, int slot, const 
     return ru_pick(sc, slot, st0);
 }
 
-template <int MODE, int QS>
-__global__ void __launch_bounds__(kCoarseThreads) k_c_solve(BatchScal sc, CArgs c, const double *__restrict__ part1,
-                                                            const double *__restrict__ part2, size_t Rp,
-                                                            const int *__restrict__ lsel, double *__restrict__ cpart) {
-    constexpr int kRows = kCoarseThreads / QS;
-    __shared__ double srx[2048];
-    __shared__ double ys[QS][kRows];
-    __shared__ double red[5][kCoarseThreads / 64];
-    const int part = blockIdx.x, nparts = gridDim.x, slot = blockIdx.y;
+template <int MODE>
+__global__ void __launch_bounds__(256) k_c_solve_t(BatchScal sc, CArgs c, size_t Rp, const int *__restrict__ lsel,
+                                                   double *__restrict__ py) {
+    const int t = blockIdx.x, slot = blockIdx.y;
     if (!coarse_act<MODE>(sc, slot, lsel)) return;  // block-uniform
+    int I, J;
+    coarse::tile_of(t, I, J);
+    __shared__ double xc[64], xr[64], red[4][64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int nc = c.nc;
-    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    {  // this block's share of the aggregate partials
-        const int g0 = part * c.nagg / nparts, g1 = (part + 1) * c.nagg / nparts;
-        for (int g = g0 + tid; g < g1; g += kCoarseThreads) {
-            const size_t o = (size_t)g * Rp + slot;
-            s[0] += part1[o];
-            s[1] += part2[o];
-            if (MODE == kFoldBeta) s[2] += c.part3[o];
-        }
-    }
-    for (int i = tid; i < nc; i += kCoarseThreads) srx[i] = c.rc[(size_t)i * Rp + slot];
+    const int nt = c.nt;
+    // tile rows (tid >> 4) + 16 u, columns 4 (tid & 15) .. + 3
+    const float4 *T = reinterpret_cast<const float4 *>(c.ainv + ((size_t)slot * coarse::ntiles_lower(nt) + t) * 4096);
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = T[tid + 256 * u];
+    if (tid < 64) xc[tid] = c.rc[(size_t)(J * 64 + tid) * Rp + slot];
+    else if (tid < 128) xr[tid - 64] = c.rc[(size_t)(I * 64 + tid - 64) * Rp + slot];
     __syncthreads();
-    const int il = tid % kRows, q = tid / kRows;
-    const int i = part * kRows + il;
-    const int jq = (nc + QS - 1) / QS, j0 = q * jq, j1 = min(nc, j0 + jq);
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-    if (i < nc) {
-        const float *Ai = c.ainv + (size_t)slot * nc * nc + i;  // column i = row i (symmetric)
-        int j = j0;
-        for (; j + 4 <= j1; j += 4) {
-            const float a0 = Ai[(size_t)j * nc], a1 = Ai[(size_t)(j + 1) * nc];
-            const float a2 = Ai[(size_t)(j + 2) * nc], a3 = Ai[(size_t)(j + 3) * nc];
-            y0 += (double)a0 * srx[j];
-            y1 += (double)a1 * srx[j + 1];
-            y2 += (double)a2 * srx[j + 2];
-            y3 += (double)a3 * srx[j + 3];
-        }
-        for (; j < j1; j++) y0 += (double)Ai[(size_t)j * nc] * srx[j];
+    const int c0 = 4 * (tid & 15), rb = tid >> 4;
+    double cp[4] = {0.0, 0.0, 0.0, 0.0};
+    double *row_out = py + (((size_t)slot * nt + I) * nt + J) * 64;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int r = rb + 16 * u;
+        double s = (double)v[u].x * xc[c0];
+        s += (double)v[u].y * xc[c0 + 1];
+        s += (double)v[u].z * xc[c0 + 2];
+        s += (double)v[u].w * xc[c0 + 3];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        s += __shfl_xor(s, 8, 64);
+        if ((tid & 15) == 0) row_out[r] = s;
+        const double xi = xr[r];
+        cp[0] += (double)v[u].x * xi;
+        cp[1] += (double)v[u].y * xi;
+        cp[2] += (double)v[u].z * xi;
+        cp[3] += (double)v[u].w * xi;
     }
-    ys[q][il] = (y0 + y1) + (y2 + y3);
+    if (I == J) return;  // block-uniform
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        cp[q] += __shfl_xor(cp[q], 16, 64);
+        cp[q] += __shfl_xor(cp[q], 32, 64);
+    }
+    if (lane < 16)
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[wid][c0 + q] = cp[q];
     __syncthreads();
-    if (q == 0 && i < nc) {
-        double yi = ys[0][il];
-        for (int k = 1; k < QS; k++) yi += ys[k][il];
-        c.y[(size_t)i * Rp + slot] = yi;
-        s[3] += yi * srx[i];
-        if (MODE == kFoldBeta) s[4] += yi * c.qc[(size_t)i * Rp + slot];
+    if (tid < 64)
+        py[(((size_t)slot * nt + J) * nt + I) * 64 + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64) k_c_solve_y(BatchScal sc, CArgs c, const double *__restrict__ part1,
+                                                  const double *__restrict__ part2, size_t Rp,
+                                                  const int *__restrict__ lsel, const double *__restrict__ py,
+                                                  double *__restrict__ cpart) {
+    const int I = blockIdx.x, slot = blockIdx.y, r = threadIdx.x;
+    if (!coarse_act<MODE>(sc, slot, lsel)) return;
+    const int nt = c.nt;
+    const double *p = py + ((size_t)slot * nt + I) * nt * 64 + r;
+    double y = 0.0;
+    for (int J = 0; J < nt; J++) y += p[(size_t)J * 64];
+    double s[3] = {0.0, 0.0, 0.0};
+    const int row = I * 64 + r;
+    if (row < c.nc) {
+        c.y[(size_t)row * Rp + slot] = y;
+        s[2] = y * c.rc[(size_t)row * Rp + slot];
     }
-    for (int k = 0; k < 5; k++) {
+    // this block's share of the aggregate partials
+    const int g0 = I * c.nagg / nt, g1 = (I + 1) * c.nagg / nt;
+    for (int g = g0 + r; g < g1; g += 64) {
+        s[0] += part1[(size_t)g * Rp + slot];
+        s[1] += part2[(size_t)g * Rp + slot];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
         double x = s[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) red[k][wid] = x;
-    }
-    __syncthreads();
-    if (tid < 5) {
-        double x = 0.0;
-        for (int w = 0; w < kCoarseThreads / 64; w++) x += red[tid][w];
-        cpart[((size_t)tid * nparts + part) * Rp + slot] = x;
+        if (r == 0) cpart[((size_t)k * nt + I) * Rp + slot] = x;
     }
 }
 
@@ -1883,14 +1867,15 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         sc.zp[slot] = 0;
         return;
     }
-    double t[5];
-    for (int k = 0; k < 5; k++) {
+    double t[3];
+    for (int k = 0; k < 3; k++) {
         double x = 0.0;
         for (int p = 0; p < nparts; p++) x += cpart[((size_t)k * nparts + p) * Rp + slot];
         t[k] = x;
     }
-    // t[0] = r^T D^-1 r (Jacobi part), t[1] = r.r; rr: square of the stop-test norm
-    const double rz = t[0] + t[3], rr = sc.nat ? t[0] : t[1];
+    // t[0] = r^T D^-1 r (Jacobi part), t[1] = r.r, t[2] = y . Z^T r (coarse part);
+    // rr: square of the stop-test norm
+    const double rz = t[0] + t[2], rr = sc.nat ? t[0] : t[1];
     int st = st0;
     if (MODE == kFoldInit) {
         sc.bnorm2[slot] = rr;
@@ -2048,7 +2033,6 @@ DarcyBatchGPU::~DarcyBatchGPU() {
     for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
     for (void *p : owned_) dev_free(p);
     if (h_count_) (void)hipHostFree(h_count_);
-    if (solver_) (void)rocblas_destroy_handle((rocblas_handle)solver_);
     if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
@@ -2445,7 +2429,7 @@ void DarcyBatchGPU::build() {
     d_ext_ = ad((size_t)S_ + E + F);
     d_part_ = ad(3 * (size_t)kMaxGrid * Rp_);
     d_pre_ = ad(3 * (size_t)kPre * Rp_);
-    d_cpart_ = ad(5 * (size_t)kMaxCoarseParts * Rp_);
+    d_cpart_ = ad(3 * (size_t)coarse::kMaxTiles * Rp_);
     alloc_scal(sc_);
     alloc_scal(sc2_);
     d_src_ = ai(Rp_);
@@ -2526,7 +2510,8 @@ void DarcyBatchGPU::permute_slots(const std::vector<int> &src, int phase) {
                            d_spy_, (size_t)nc_, (size_t)Rp_, (const int *)d_src_);
         std::swap(d_y_, d_spy_);
         hipLaunchKernelGGL(k_b_perm_major<float>, dim3(1024, Rp_), blk, 0, stream_, (const float *)d_ainv_,
-                           d_spainv_, (size_t)nc_ * nc_, (const int *)d_src_);
+                           d_spainv_, (size_t)coarse::ntiles_lower(nt_) * coarse::kT * coarse::kT,
+                           (const int *)d_src_);
         std::swap(d_ainv_, d_spainv_);
         if (ncol_ == 2) {
             hipLaunchKernelGGL(k_b_perm<unsigned short>, gF, blk, 0, stream_, (const unsigned short *)d_phi_, d_spphi_,
@@ -2572,34 +2557,32 @@ void DarcyBatchGPU::set_coarse(int ncol) {
     const size_t nv = (size_t)F_ * Rp_;
     auto ad = [&](size_t n) { double *p = dev_alloc<double>(n); owned_.push_back(p); return p; };
     auto af = [&](size_t n) { float *p = dev_alloc<float>(n); owned_.push_back(p); return p; };
-    const int ncmax = 2 * nagg_;
+    // coarse order padded to 64-row tiles (two columns: the largest)
+    const int ntmax = (2 * nagg_ + coarse::kT - 1) / coarse::kT, ncpmax = ntmax * coarse::kT;
+    const size_t tiles = (size_t)coarse::ntiles_lower(ntmax) * coarse::kT * coarse::kT;
     if (!d_rc_) {
-        d_rc_ = ad((size_t)ncmax * Rp_);
-        d_qc_ = ad((size_t)ncmax * Rp_);
-        d_y_ = ad((size_t)ncmax * Rp_);
-        d_spy_ = ad((size_t)ncmax * Rp_);
-        d_part3_ = ad((size_t)nagg_ * Rp_);
+        d_rc_ = ad((size_t)ncpmax * Rp_);
+        d_y_ = ad((size_t)ncpmax * Rp_);
+        d_spy_ = ad((size_t)ncpmax * Rp_);
         d_Asp_ = ad((size_t)nagg_ * kNbMax * 4 * Rp_);
-        d_ainv_ = af((size_t)Rp_ * ncmax * ncmax);
-        d_spainv_ = af((size_t)Rp_ * ncmax * ncmax);
-        d_Apack_ = ad((size_t)Rp_ * ncmax * ncmax);
-        double **pp = dev_alloc<double *>(Rp_);
-        owned_.push_back(pp);
-        d_Aptrs_ = pp;
-        int *inf = dev_alloc<int>(2 * (size_t)Rp_);  // potrf, potri status
+        d_ainv_ = af((size_t)Rp_ * tiles);
+        d_spainv_ = af((size_t)Rp_ * tiles);
+        d_Apack_ = ad((size_t)Rp_ * ncpmax * ncpmax);
+        d_Wpack_ = ad((size_t)Rp_ * ncpmax * ncpmax);
+        d_py_ = ad((size_t)Rp_ * ntmax * ntmax * coarse::kT);
+        int *inf = dev_alloc<int>((size_t)Rp_);
         owned_.push_back(inf);
         d_info_ = inf;
-        rocblas_handle h;
-        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(-2, "rocblas_create_handle failed");
-        solver_ = h;
-        rocblas_set_stream(h, stream_);
     }
+    nc_ = ncol * nagg_;
+    nt_ = (nc_ + coarse::kT - 1) / coarse::kT;
+    // padding rows of Z^T r stay zero
+    FL_HIP(hipMemsetAsync(d_rc_, 0, (size_t)ncpmax * Rp_ * sizeof(double), stream_));
     if (ncol == 2 && !d_phi_) {
         auto ah = [&](size_t n) { unsigned short *p = dev_alloc<unsigned short>(n); owned_.push_back(p); return p; };
         d_phi_ = ah(nv);
         d_spphi_ = ah(nv);
     }
-    nc_ = ncol * nagg_;
 }
 
 void DarcyBatchGPU::set_coarse_basis(const float *d_phi_edges) {
@@ -2616,13 +2599,13 @@ void DarcyBatchGPU::set_coarse_basis(const float *d_phi_edges) {
 }
 
 // Coarse operators of the listed (slot, realization) pairs: phi columns,
-// Z^T S Z by aggregate blocks, batched Cholesky + inverse, FP32 inverse.
+// Z^T S Z by aggregate blocks, batched Cholesky + inverse (coarse.hh), packed
+// FP32 inverse tiles.
 void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
     const int np = (int)pairs.size() / 2;
     if (!np) return;
     const dim3 gagg(nagg_, nch_), blk(256);
-    CArgs c{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_qc_, d_y_,
-            d_part3_};
+    CArgs c{ncol_, nagg_, nc_, nt_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_y_};
     if (ncol_ == 2) {
         hipLaunchKernelGGL(k_b_slot_col, dim3((F_ + 63) / 64, (np + 63) / 64), blk, 0, stream_,
                            (const float *)d_phiAll_, F_, Rp_, (const int *)d_pairs_, np, d_phi_);
@@ -2635,30 +2618,65 @@ void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
         hipLaunchKernelGGL(k_c_assemble<1>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
                            (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
                            d_Asp_);
-        hipLaunchKernelGGL(k_c_pack<1>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
+        hipLaunchKernelGGL(k_c_pack<1>, dim3(nt_ * coarse::kT, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
     } else {
         hipLaunchKernelGGL(k_c_assemble<2>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
                            (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
                            d_Asp_);
-        hipLaunchKernelGGL(k_c_pack<2>, dim3(nc_, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
+        hipLaunchKernelGGL(k_c_pack<2>, dim3(nt_ * coarse::kT, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
     }
     FL_HIP(hipGetLastError());
-    std::vector<double *> ptrs(np);
-    for (int k = 0; k < np; k++) ptrs[k] = d_Apack_ + (size_t)k * nc_ * nc_;
-    FL_HIP(hipMemcpyAsync(d_Aptrs_, ptrs.data(), np * sizeof(double *), hipMemcpyHostToDevice, stream_));
-    rocblas_handle h = (rocblas_handle)solver_;
-    if (rocsolver_dpotrf_batched(h, rocblas_fill_lower, nc_, d_Aptrs_, nc_, d_info_, np) != rocblas_status_success)
-        throw Error(-2, "rocsolver_dpotrf_batched failed");
-    // the inverse of a failed factorization is discarded by k_c_unpack
-    if (rocsolver_dpotri_batched(h, rocblas_fill_lower, nc_, d_Aptrs_, nc_, d_info_ + Rp_, np) !=
-        rocblas_status_success)
-        throw Error(-2, "rocsolver_dpotri_batched failed");
-    FL_HIP(hipStreamSynchronize(stream_));  // host pointer staging
-    hipLaunchKernelGGL(k_c_unpack, dim3(nc_, np), blk, 0, stream_, nc_, (const double *)d_Apack_,
-                       (const int *)d_info_, (const int *)(d_info_ + Rp_), (const int *)d_pairs_, d_ainv_);
-    FL_HIP(hipGetLastError());
+    coarse::invert_batched(d_Apack_, d_Wpack_, nt_ * coarse::kT, np, d_info_, (const int *)d_pairs_, d_ainv_,
+                           stream_);
+}
+
+void coarse_invert_host(const double *a, int n, int np, float *inv, int *info) {
+    if (n < 1 || n > coarse::kMaxTiles * coarse::kT || np < 1) throw Error(-1, "coarse order must be in [1, 2048]");
+    const int nt = (n + coarse::kT - 1) / coarse::kT, ncp = nt * coarse::kT, ntl = coarse::ntiles_lower(nt);
+    const size_t tile = (size_t)coarse::kT * coarse::kT;
+    std::vector<double> A((size_t)np * ncp * ncp, 0.0);
+    for (int k = 0; k < np; k++)
+        for (int i = 0; i < ncp; i++)
+            for (int j = 0; j < ncp; j++)
+                A[((size_t)k * ncp + i) * ncp + j] =
+                    i < n && j < n ? a[((size_t)k * n + i) * n + j] : (i == j ? 1.0 : 0.0);
+    std::vector<int> pairs(2 * (size_t)np);
+    for (int k = 0; k < np; k++) pairs[2 * k] = pairs[2 * k + 1] = k;
+    hipStream_t s;
+    FL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<void *> own;
+    try {
+        double *dA = dev_upload(A.data(), A.size(), s);
+        own.push_back(dA);
+        double *dW = dev_alloc<double>(A.size());
+        own.push_back(dW);
+        int *dp = dev_upload(pairs.data(), pairs.size(), s), *dinfo = dev_alloc<int>(np);
+        own.push_back(dp);
+        own.push_back(dinfo);
+        float *dinv = dev_alloc<float>((size_t)np * ntl * tile);
+        own.push_back(dinv);
+        coarse::invert_batched(dA, dW, ncp, np, dinfo, dp, dinv, s);
+        std::vector<float> t((size_t)np * ntl * tile);
+        FL_HIP(hipMemcpyAsync(t.data(), dinv, t.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        FL_HIP(hipMemcpyAsync(info, dinfo, np * sizeof(int), hipMemcpyDeviceToHost, s));
+        FL_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < np; k++)
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j <= i; j++) {
+                    const float v = t[((size_t)k * ntl + coarse::tile_index(i / 64, j / 64)) * tile + (i % 64) * 64 +
+                                      j % 64];
+                    inv[((size_t)k * n + i) * n + j] = v;
+                    inv[((size_t)k * n + j) * n + i] = v;
+                }
+    } catch (...) {
+        for (void *p : own) dev_free(p);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    for (void *p : own) dev_free(p);
+    FL_HIP(hipStreamDestroy(s));
 }
 
 void DarcyBatchGPU::set_launch(int grid, int xcd_map, int chunk) {
@@ -2742,8 +2760,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     auto rebuild = [&]() {
         G = grid_;
         X = xcd_map_;
-        ca = CArgs{ncol_, nagg_, nc_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_qc_, d_y_,
-                   d_part3_};
+        ca = CArgs{ncol_, nagg_, nc_, nt_, d_agg_, d_aptr_, d_nbptr_, d_nb_, d_lbmap_, d_phi_, d_ainv_, d_rc_, d_y_};
         a64 = make_args<double>(F_, Rp_, G, X, d_rec_, d_row_ptr_, d_row_c_, d_meta_, d_op_off_, d_conn_off_, d_conn_,
                                 d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_, d_p0_, d_p1_, d_q_, d_part_,
                                 sc_);
@@ -2784,18 +2801,13 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const int qs = lw_ < 64 ? 16 : 4;
-        const int parts = (nc_ + kCoarseThreads / qs - 1) / (kCoarseThreads / qs);
-        const dim3 gs(parts, 64 * nch_), blkc(kCoarseThreads), gfin(nch_), b64(64);
+        const dim3 gt(coarse::ntiles_lower(nt_), 64 * nch_), gy(nt_, 64 * nch_), gfin(nch_), b64(64);
         const double *p1 = a64.part1, *p2 = a64.part2;
-#define FL_CFOLD(M)                                                                                                   \
-    if (qs == 16)                                                                                                     \
-        hipLaunchKernelGGL((k_c_solve<M, 16>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);  \
-    else if (qs == 8)                                                                                                 \
-        hipLaunchKernelGGL((k_c_solve<M, 8>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);   \
-    else                                                                                                              \
-        hipLaunchKernelGGL((k_c_solve<M, 4>), gs, blkc, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel, d_cpart_);   \
-    hipLaunchKernelGGL(k_c_fin<M>, gfin, b64, 0, stream_, sc_, (const double *)d_cpart_, parts, (size_t)Rp_, rtol2,    \
+#define FL_CFOLD(M)                                                                                                  \
+    hipLaunchKernelGGL(k_c_solve_t<M>, gt, blk, 0, stream_, sc_, ca, (size_t)Rp_, lsel, d_py_);                      \
+    hipLaunchKernelGGL(k_c_solve_y<M>, gy, b64, 0, stream_, sc_, ca, p1, p2, (size_t)Rp_, lsel,                     \
+                       (const double *)d_py_, d_cpart_);                                                             \
+    hipLaunchKernelGGL(k_c_fin<M>, gfin, b64, 0, stream_, sc_, (const double *)d_cpart_, nt_, (size_t)Rp_, rtol2,     \
                        atol2, max_it, mixed, lsel)
         if (mode == kFoldInit) {
             FL_CFOLD(kFoldInit);

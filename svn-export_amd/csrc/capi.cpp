@@ -630,6 +630,14 @@ int fl_darcy_batch_aggregates(fl_darcy_batch *b, int *edge_agg) {
     });
 }
 
+int fl_coarse_invert(const double *a, int n, int np, float *inv, int *info) {
+    return guard([&] {
+        if (!a || !inv || !info) throw fl::Error(-1, "null argument");
+        fl::coarse_invert_host(a, n, np, inv, info);
+        return 0;
+    });
+}
+
 int fl_darcy_batch_lambda(fl_darcy_batch *b, double *out) {
     return guard([&] {
         if (!out) throw fl::Error(-1, "null output");

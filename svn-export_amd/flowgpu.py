@@ -125,6 +125,7 @@ SIGNATURES = {
     "fl_darcy_batch_set_coarse_basis": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fl_darcy_batch_aggregates": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fl_darcy_batch_lambda": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fl_coarse_invert": (C.c_int, [_f64p, C.c_int, C.c_int, C.c_void_p, _i32p]),
     "fl_gnn_create": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_void_p)]),
     "fl_gnn_free": (None, [C.c_void_p]),
     "fl_gnn_n_params": (C.c_int, [C.c_void_p]),
@@ -264,6 +265,20 @@ class Mesh:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.fl_mesh_free(self.h)
             self.h = None
+
+
+def coarse_invert(a):
+    """FP32 inverses of a stack of SPD matrices a[np][n][n] by the batched
+    Cholesky + inverse kernels of the two-level preconditioner's coarse space;
+    returns (inv[np][n][n] float32, info[np]: 1 where not positive definite)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim == 2:
+        a = a[None]
+    npm, n = a.shape[0], a.shape[1]
+    inv = np.zeros((npm, n, n), np.float32)
+    info = np.zeros(npm, np.int32)
+    _check(lib().fl_coarse_invert(a, n, npm, inv.ctypes.data_as(C.c_void_p), info))
+    return inv, info
 
 
 def element_ids(mesh, boundary=False):
