@@ -1513,7 +1513,10 @@ __device__ __forceinline__ void opl_stage(const float *const *segp, int nseg, in
     for (int u = 0; u < kStageMax; u++) vv[u] = *(gf4v_ptr)(ptr[u] + sb + 4 * (tid & 7));
 }
 
-__global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTiles ot, int nch) {
+// nsg: 32-slot groups computed (2 nch, or 1 for a narrowed tail of at most 32
+// live slots at the front); partials of the other slots of the nch chunks are
+// zeroed.
+__global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTiles ot, int nch, int nsg) {
     extern __shared__ double lds_d[];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const size_t Rp = a.Rp;
@@ -1572,7 +1575,7 @@ __global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTile
     }
     __syncthreads();
     const int quad = tid & 7, rsub = tid >> 3;  // 64 rows per pass, 4 slots per lane
-    const int nsg = 2 * nch;
+    for (int k = nsg * kSlotGroup + tid; k < nch * 64; k += blockDim.x) part[k] = 0.0;
     // software pipeline: the next slot group's segments are loaded into
     // registers while the current group is computed from LDS.  The run flags
     // of all groups (nch <= 2 here) are read first, as bits, so no wait inside
@@ -2855,7 +2858,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
                                    a.opT64);
         }
         hipLaunchKernelGGL(k_c_opl, dim3(ntiles_), dim3(kOplThreads), opl_lds_bytes(nseg_max_, tile_T_), stream_, a,
-                           ot, nch_);
+                           ot, nch_, lw_ == 64 ? 2 * nch_ : 1);
         if (ngen_) {
             FL_HIP(hipEventRecord(ev_join_, stream2_));
             FL_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
@@ -2888,16 +2891,14 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     };
     auto iter32 = [&](hipEvent_t *ev) {
         if (ev) FL_HIP(hipEventRecord(ev[0], stream_));
-        if (NC && lw_ == 16)
-            hipLaunchKernelGGL(k_c_opn<16>, grow(), blk, 0, stream_, a32, ca);
-        else if (NC && lw_ == 4)
+        if (NC && lw_ == 4)
             hipLaunchKernelGGL(k_c_opn<4>, grow(), blk, 0, stream_, a32, ca);
         else if (NC)
-            op_tiled(a32);
+            op_tiled(a32);  // a tail narrowed to 16 slots computes one 32-slot group
         else
             hipLaunchKernelGGL(k_b_op<float>, grow(), blk, 0, stream_, a32);
         if (ev) FL_HIP(hipEventRecord(ev[1], stream_));
-        fold(kFoldAlpha, 1, nosel, NC && lw_ == 64 ? op_tiled_grid() : 0);
+        fold(kFoldAlpha, 1, nosel, NC && lw_ != 4 ? op_tiled_grid() : 0);
         if (ev) FL_HIP(hipEventRecord(ev[2], stream_));
         if (NC == 1) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 1, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 1, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
         else if (NC == 2) { if (lw_ == 64) hipLaunchKernelGGL((k_c_update<float, 2, 64, 2>), gagg(), blkA, 0, stream_, a32, ca); else FL_LW(hipLaunchKernelGGL((k_c_update<float, 2, LW, 1>), gagg(), blkA, 0, stream_, a32, ca)) }
