@@ -266,6 +266,26 @@ int fl_darcy_set_unsteady(fl_darcy *d, int lumped, const double *storativity, co
  * solves the steady problem again. */
 int fl_darcy_unsteady_step(fl_darcy *d, double dt, const fl_solver_opts *opts, fl_solve_info *info);
 
+/* ---- time coupling of the equations (Equation, equation.hh:113-146) -------
+ * Each Darcy handle carries the reference's TimeGovernor (time_governor.cc).
+ * By default the steady one (darcy_flow_mh.cc:263): t = 0 and dt = inf until
+ * fl_darcy_update_solution, then t = inf, dt = 0 and is_end (a steady handle
+ * still solves again on every fl_darcy_update_solution, e.g. after new data).
+ * fl_darcy_set_time(end_time, init_dt, save_step) after fl_darcy_set_unsteady
+ * gives the unsteady governor of DarcyFlowMH_Unsteady with the input keys
+ * init_dt (fixed step) and the output save_step (its marks; <= 0: none),
+ * its step fixed until the next mark at once (darcy_flow_mh.cc:1606-1608):
+ * every fl_darcy_update_solution is then one implicit step of the
+ * governor's next dt, and nothing once it is at the end.
+ * planned_time = t + estimate_dt, solved_time = t, choose_next_time =
+ * fix_dt_until_mark, set_time_upper_constraint = set_upper_constraint. */
+int fl_darcy_set_time(fl_darcy *d, double end_time, double init_dt, double save_step);
+int fl_darcy_time(fl_darcy *d, double *time, double *dt, int *is_end);
+int fl_darcy_set_time_upper_constraint(fl_darcy *d, double dt);
+int fl_darcy_planned_time(fl_darcy *d, double *time);
+int fl_darcy_solved_time(fl_darcy *d, double *time);
+int fl_darcy_choose_next_time(fl_darcy *d);
+
 /* DarcyFlowMH_Steady::get_solution_vector (darcy_flow_mh.cc:485-500):
  * copies the [q | p | lambda] vector (S+E+F doubles) to out. */
 int fl_darcy_get_solution(fl_darcy *d, double *out);
@@ -456,8 +476,25 @@ int fl_transport_update_solution(fl_transport *t, double target_time, int *steps
 /* ConvectionTransport::get_concentration_matrix (transport.h:107-173): out[n_subst][E]. */
 int fl_transport_get_conc(fl_transport *t, double *out);
 
-/* time(), dt(), cfl_max_step */
+/* time(), dt() of the convection's governor (its CFL steps), cfl_max_step */
 int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl);
+
+/* The operator-splitting equation's own TimeGovernor (transport_operator_splitting.cc:213:
+ * marks at 0, end_time and every save_step of fl_transport_create), as the
+ * coupling loop of HC_ExplicitSequential::run_simulation
+ * (hc_explicit_sequential.cc:186-231) drives it: set_time_upper_constraint
+ * (the water's dt), planned_time (t + estimate_dt), solved_time (t),
+ * choose_next_time (fix_dt_until_mark), and fl_transport_update_solution_step
+ * = TransportOperatorSplitting::update_solution (:244-287): the governor's
+ * next_time, then the convection's CFL steps up to it.  This governor follows
+ * only these calls (fl_transport_update_solution with an explicit target
+ * leaves it). */
+int fl_transport_time_governor(fl_transport *t, double *time, double *dt, int *is_end);
+int fl_transport_set_time_upper_constraint(fl_transport *t, double dt);
+int fl_transport_planned_time(fl_transport *t, double *time);
+int fl_transport_solved_time(fl_transport *t, double *time);
+int fl_transport_choose_next_time(fl_transport *t);
+int fl_transport_update_solution_step(fl_transport *t, int *steps);
 
 /* MassBalance::calculate (mass_balance.cc:79-239) state after the last step:
  * bal/plus/minus[n_subst][n_boundary_regions], mass[n_subst][n_bulk_regions],

@@ -1446,7 +1446,12 @@ __global__ void __launch_bounds__(256) k_c_opg(BArgs<float> a, OpTiles ot, const
         act[h] = a.sc.status[h * 64 + lane] == kRun;
         v[h] = 0.0;
     }
-    for (int k = blockIdx.x * kWaves + wid; k < ot.ngen; k += ot.gen_blocks * kWaves) {
+    // XCD-aware: the blocks of XCD x (block b runs on XCD b % 8) take the x-th
+    // eighth of the general rows, so one L2 holds the neighbour rows a
+    // stretch of the (Z-ordered) general rows shares
+    const int xcd = blockIdx.x & 7, nb8 = ot.gen_blocks >> 3;
+    const int k0 = (int)((long)ot.ngen * xcd / 8), k1 = (int)((long)ot.ngen * (xcd + 1) / 8);
+    for (int k = k0 + (int)(blockIdx.x >> 3) * kWaves + wid; k < k1; k += nb8 * kWaves) {
         const int row = gen[k];
         int Gr[kGenRec];
 #pragma unroll

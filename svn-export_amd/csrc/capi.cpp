@@ -22,6 +22,7 @@ struct fl_mesh {
 struct fl_darcy {
     std::unique_ptr<fl::DarcyGPU> g;
     fl_mesh *mesh;
+    fl::TimeGovernorR tg;  // steady until fl_darcy_set_time
 };
 struct fl_darcy_batch {
     std::unique_ptr<fl::DarcyBatchGPU> g;
@@ -292,6 +293,7 @@ int fl_darcy_create(fl_mesh *m, fl_darcy **out) {
         check_device();
         auto *d = new fl_darcy();
         d->mesh = m;
+        d->tg.init_steady();
         try {
             d->g.reset(new fl::DarcyGPU(m->m));
         } catch (...) {
@@ -346,10 +348,69 @@ int fl_darcy_update_solution(fl_darcy *d, const fl_solver_opts *o, fl_solve_info
         const int norm = o ? o->norm : 0;
         if (norm != 0 && norm != 1) throw fl::Error(-8, "norm must be 0 (2-norm) or 1 (natural Jacobi norm)");
         if (prec != 0 && prec != 1) throw fl::Error(-8, "precision must be 0 (FP64) or 1 (FP32 + FP64 reliable updates)");
-        fl::SolveStats st = d->g->solve(rtol, atol, max_it, prec, norm);
+        fl::SolveStats st;
+        if (!d->tg.steady) {
+            // unsteady governor (fl_darcy_set_time): one step of its dt, nothing
+            // at the end time (DarcyFlowMH_Steady::update_solution, darcy_flow_mh.cc:375-382)
+            if (d->tg.is_end()) {
+                st.converged = 1;
+                fill_info(st, info);
+                return 0;
+            }
+            d->tg.next_time();
+            st = d->g->unsteady_step(d->tg.time_step, rtol, atol, max_it, prec, norm);
+        } else {
+            // steady: solved again on every call (new data), the governor moves
+            // to t = inf, dt = 0 as after the reference's one steady solve
+            st = d->g->solve(rtol, atol, max_it, prec, norm);
+            d->tg.next_time();
+        }
         fill_info(st, info);
         return solve_status(st);
     });
+}
+
+int fl_darcy_set_time(fl_darcy *d, double end_time, double init_dt, double save_step) {
+    return guard([&] {
+        if (!(init_dt > 0.0) || !(end_time >= init_dt) || !std::isfinite(end_time))
+            throw fl::Error(-1, "time: need 0 < init_dt <= end_time < inf");
+        if (!d->g->is_unsteady()) throw fl::Error(-3, "an unsteady time governor needs fl_darcy_set_unsteady first");
+        d->tg.init_fixed(end_time, init_dt, save_step);
+        d->tg.fix_dt_until_mark();  // DarcyFlowMH_Unsteady constructor (darcy_flow_mh.cc:1606-1608)
+        return 0;
+    });
+}
+
+int fl_darcy_time(fl_darcy *d, double *time, double *dt, int *is_end) {
+    if (time) *time = d->tg.time;
+    if (dt) *dt = d->tg.time_step;
+    if (is_end) *is_end = d->tg.is_end() ? 1 : 0;
+    return 0;
+}
+
+int fl_darcy_set_time_upper_constraint(fl_darcy *d, double dt) {
+    return guard([&] {
+        if (!(dt > 0.0)) throw fl::Error(-1, "time step constraint must be positive");
+        (void)d->tg.set_upper_constraint(dt);
+        return 0;
+    });
+}
+
+int fl_darcy_planned_time(fl_darcy *d, double *time) {
+    if (!time) return -1;
+    *time = d->tg.estimate_time();
+    return 0;
+}
+
+int fl_darcy_solved_time(fl_darcy *d, double *time) {
+    if (!time) return -1;
+    *time = d->tg.time;
+    return 0;
+}
+
+int fl_darcy_choose_next_time(fl_darcy *d) {
+    (void)d->tg.fix_dt_until_mark();
+    return 0;
 }
 
 int fl_darcy_update_solution_noalloc(fl_darcy *d, const fl_solver_opts *o, fl_solve_info *info) {
@@ -822,6 +883,47 @@ int fl_transport_time(fl_transport *t, double *time, double *dt, double *cfl) {
     if (dt) *dt = t->g->dt();
     if (cfl) *cfl = t->g->cfl();
     return 0;
+}
+
+int fl_transport_time_governor(fl_transport *t, double *time, double *dt, int *is_end) {
+    const fl::TimeGovernorR &g = t->g->tos();
+    if (time) *time = g.time;
+    if (dt) *dt = g.time_step;
+    if (is_end) *is_end = g.is_end() ? 1 : 0;
+    return 0;
+}
+
+int fl_transport_set_time_upper_constraint(fl_transport *t, double dt) {
+    return guard([&] {
+        if (!(dt > 0.0)) throw fl::Error(-1, "time step constraint must be positive");
+        (void)t->g->tos().set_upper_constraint(dt);
+        return 0;
+    });
+}
+
+int fl_transport_planned_time(fl_transport *t, double *time) {
+    if (!time) return -1;
+    *time = t->g->tos().estimate_time();
+    return 0;
+}
+
+int fl_transport_solved_time(fl_transport *t, double *time) {
+    if (!time) return -1;
+    *time = t->g->tos().time;
+    return 0;
+}
+
+int fl_transport_choose_next_time(fl_transport *t) {
+    (void)t->g->tos().fix_dt_until_mark();
+    return 0;
+}
+
+int fl_transport_update_solution_step(fl_transport *t, int *steps) {
+    return guard([&] {
+        const int n = t->g->update_solution_step();
+        if (steps) *steps = n;
+        return 0;
+    });
 }
 
 int fl_transport_mass_balance(fl_transport *t, double *bal, double *plus, double *minus, double *mass, double *src,

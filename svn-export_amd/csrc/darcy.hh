@@ -100,6 +100,7 @@ class DarcyGPU {
     // element pressures), 1 = LMH (on the edge traces); the solution becomes the
     // initial state.  unsteady_step advances it by one implicit Euler step of dt.
     void set_unsteady(int lumped, const double *storativity, const double *init_pressure);
+    bool is_unsteady() const { return unsteady_ != 0; }
     SolveStats unsteady_step(double dt, double rtol, double atol, int max_it, int precision = 0, int norm = 0);
     bool unsteady() const { return unsteady_ != 0; }
     void water_balance(double *bal, double *plus, double *minus) const;  // per boundary region

@@ -26,26 +26,9 @@
 #include <vector>
 
 #include "mesh.hh"
+#include "time_governor.hh"
 
 namespace fl {
-
-struct TimeGovernorR {
-    double time = 0, end_time = 0, time_step = 0, last_time_step = 0, fixed_dt = 0, end_fixed = 0;
-    double upper = 0, lower = 0, max_ts = 0, min_ts = 0;
-    bool dt_fixed_now = false, dt_changed = true;
-    int level = 0;
-    std::vector<double> marks;
-    void init(double end, double save_step);
-    bool lt(double o) const;
-    bool ge(double o) const;
-    bool is_end() const;
-    void add_mark(double t);
-    double next_mark() const;
-    double estimate_dt() const;
-    int set_upper_constraint(double u);
-    double fix_dt_until_mark();
-    void next_time();
-};
 
 struct TransportStats {
     int steps = 0;
@@ -128,6 +111,12 @@ class TransportGPU {
     void set_velocity_device(const double *d_side_flux);  // device [S] (e.g. the Darcy solution)
     double prepare();                                      // build the matrix now; returns the CFL step
     int update_solution(double target_time);               // steps taken
+    // the operator-splitting equation's own governor (TransportOperatorSplitting
+    // time_, transport_operator_splitting.cc:213): output marks every save_step
+    TimeGovernorR &tos() { return tos_; }
+    // TransportOperatorSplitting::update_solution (transport_operator_splitting.cc:244-287):
+    // its governor's next time, then the convection's CFL steps up to it
+    int update_solution_step();
     void get_conc(double *out) const;                      // [nsub][E]
     void mass_balance(double *bal, double *plus, double *minus, double *mass, double *src, double *totals);
     double time() const { return tg_.time; }
@@ -189,7 +178,7 @@ class TransportGPU {
     double *d_c0_ = nullptr, *d_c1_ = nullptr, *d_flux_ = nullptr;
     bool cur_is_c0_ = true;
     // state
-    TimeGovernorR tg_;
+    TimeGovernorR tg_, tos_;
     double cfl_max_step_ = 0;
     bool matrix_built_ = false, setup_done_ = false, scaled_ = false, need_rescale_ = true, flux_set_ = false;
     bool mb_init_ = true;

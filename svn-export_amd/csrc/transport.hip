@@ -11,88 +11,6 @@
 namespace fl {
 
 // ---------------------------------------------------------------------------
-// TimeGovernor restated (time_governor.cc:75-117, 215-237, 261-355,
-// time_governor.hh:205-211, 347-355; TimeMarks::next time_marks.cc:125-132)
-// ---------------------------------------------------------------------------
-static const double kInf = std::numeric_limits<double>::infinity();
-static const double kEps = std::numeric_limits<double>::epsilon();
-
-void TimeGovernorR::init(double end, double save_step) {
-    time = 0.0;
-    end_time = end;
-    end_fixed = 0.0;
-    max_ts = end;
-    time_step = end;
-    last_time_step = 0.0;
-    upper = kInf;
-    lower = kEps;
-    min_ts = kEps;
-    marks.clear();
-    add_mark(0.0);
-    add_mark(end);
-    for (double t = 0.0; t <= end * (1 + 1e-12); t += save_step) add_mark(t);
-}
-bool TimeGovernorR::lt(double o) const {
-    double b = o - 16 * kEps * std::max(std::fabs(time), std::fabs(o));
-    return !(time >= b);
-}
-bool TimeGovernorR::ge(double o) const {
-    double b = o - 16 * kEps * std::max(std::fabs(time), std::fabs(o));
-    return time >= b;
-}
-bool TimeGovernorR::is_end() const { return ge(end_time) || time == kInf; }
-void TimeGovernorR::add_mark(double t) {
-    auto it = std::lower_bound(marks.begin(), marks.end(), t);
-    if (it != marks.end() && *it == t) return;
-    marks.insert(it, t);
-}
-double TimeGovernorR::next_mark() const {
-    auto it = std::lower_bound(marks.begin(), marks.end(), time);
-    while (it != marks.end() && !lt(*it)) ++it;
-    return it == marks.end() ? kInf : *it;
-}
-double TimeGovernorR::estimate_dt() const {
-    if (time == kInf || is_end()) return 0.0;
-    if (lt(end_fixed)) return fixed_dt;
-    double full = next_mark() - time;
-    double est = std::min(full, upper);
-    est = std::max(est, lower);
-    int n = (int)std::ceil(full / est);
-    int nf = (int)std::floor(full / est);
-    if (std::fabs(full / est - nf) < 1e-14) n = nf;
-    return full / n;
-}
-int TimeGovernorR::set_upper_constraint(double u) {
-    if (upper < u) return -1;
-    if (lower <= u) { upper = u; return 0; }
-    return 1;
-}
-double TimeGovernorR::fix_dt_until_mark() {
-    end_fixed = -kInf;
-    fixed_dt = estimate_dt();
-    dt_fixed_now = true;
-    return end_fixed = next_mark();
-}
-void TimeGovernorR::next_time() {
-    if (time == kInf || is_end()) return;
-    if (lt(end_fixed)) {
-        fixed_dt = (end_fixed - time) / std::round((end_fixed - time) / fixed_dt);
-        last_time_step = time_step;
-        time_step = fixed_dt;
-        if (dt_fixed_now) { dt_fixed_now = false; dt_changed = (last_time_step != time_step); }
-        else dt_changed = false;
-    } else {
-        last_time_step = time_step;
-        time_step = estimate_dt();
-        dt_changed = (last_time_step != time_step);
-    }
-    time += time_step;
-    level++;
-    upper = std::min(end_time - time, max_ts);
-    lower = min_ts;
-}
-
-// ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
 enum : int { kDiag = 0, kSame = 1, kVbLow = 2, kVbHigh = 3 };
@@ -746,6 +664,7 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
     FL_HIP(hipMemcpyAsync(d_c0_, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
     FL_HIP(hipStreamSynchronize(s));
     tg_.init(end_time, save_step);
+    tos_.init(end_time, save_step);
     cfl_max_step_ = end_time;
     h_el_region_sub_.resize(E);
     for (int i = 0; i < E; i++) h_el_region_sub_[i] = m.regions[m.el_region[i]].sub_idx;
@@ -1139,6 +1058,11 @@ int TransportGPU::update_solution(double target) {
     stats_.ms_steps += ms;
     (void)hipEventDestroy(a); (void)hipEventDestroy(b);
     return steps;
+}
+
+int TransportGPU::update_solution_step() {
+    tos_.next_time();
+    return update_solution(tos_.time);
 }
 
 void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, const double *alpha,

@@ -101,6 +101,19 @@ SIGNATURES = {
     "fl_darcy_update_solution_noalloc": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
     "fl_darcy_set_unsteady": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p]),
     "fl_darcy_unsteady_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]),
+    "fl_darcy_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "fl_darcy_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "fl_darcy_set_time_upper_constraint": (C.c_int, [C.c_void_p, C.c_double]),
+    "fl_darcy_planned_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "fl_darcy_solved_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "fl_darcy_choose_next_time": (C.c_int, [C.c_void_p]),
+    "fl_transport_time_governor": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int)]),
+    "fl_transport_set_time_upper_constraint": (C.c_int, [C.c_void_p, C.c_double]),
+    "fl_transport_planned_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "fl_transport_solved_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "fl_transport_choose_next_time": (C.c_int, [C.c_void_p]),
+    "fl_transport_update_solution_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "fl_darcy_get_solution": (C.c_int, [C.c_void_p, _f64p]),
     "fl_darcy_device_solution": (C.c_void_p, [C.c_void_p]),
     "fl_darcy_solution_precision": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
@@ -431,8 +444,38 @@ class OutputMSH:
         self.current_step += 1
 
 
-class DarcyFlowMH_Steady:
+class _TimeCoupling:
+    """Equation time interface (equation.hh:113-146) over the handle's
+    TimeGovernor: time() -> (t, dt, is_end), planned_time, solved_time,
+    choose_next_time, set_time_upper_constraint."""
+    _pfx = None
+
+    def time(self):
+        t, dt, e = C.c_double(), C.c_double(), C.c_int()
+        _check(getattr(lib(), self._pfx + ("time_governor" if self._pfx == "fl_transport_" else "time"))(
+            self.h, C.byref(t), C.byref(dt), C.byref(e)))
+        return t.value, dt.value, bool(e.value)
+
+    def planned_time(self):
+        t = C.c_double()
+        _check(getattr(lib(), self._pfx + "planned_time")(self.h, C.byref(t)))
+        return t.value
+
+    def solved_time(self):
+        t = C.c_double()
+        _check(getattr(lib(), self._pfx + "solved_time")(self.h, C.byref(t)))
+        return t.value
+
+    def choose_next_time(self):
+        _check(getattr(lib(), self._pfx + "choose_next_time")(self.h))
+
+    def set_time_upper_constraint(self, dt):
+        _check(getattr(lib(), self._pfx + "set_time_upper_constraint")(self.h, float(dt)))
+
+
+class DarcyFlowMH_Steady(_TimeCoupling):
     """GPU replacement of DarcyFlowMH_Steady (darcy_flow_mh.cc:249-505)."""
+    _pfx = "fl_darcy_"
 
     def __init__(self, mesh):
         self.mesh = mesh
@@ -471,6 +514,11 @@ class DarcyFlowMH_Steady:
         E = self.mesh.n_elements
         self._unst = [_f64(storativity, E), _f64(init_pressure, E)]
         _check(lib().fl_darcy_set_unsteady(self.h, 1 if lumped else 0, *self._unst))
+
+    def set_time(self, end_time, init_dt, save_step=0.0):
+        """Unsteady governor with a fixed step and output marks every save_step
+        (after set_unsteady): every update_solution is then one step of it."""
+        _check(lib().fl_darcy_set_time(self.h, float(end_time), float(init_dt), float(save_step)))
 
     def unsteady_step(self, dt, r_tol=1e-7, a_tol=1e-9, max_it=10000, precision=0, norm=0):
         """One implicit Euler step of length dt (darcy_flow_mh.cc:375-433, 1660-1827)."""
@@ -683,8 +731,9 @@ class LearnedCoarseBasis:
 BC_CONC_FN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 
-class ConvectionTransport:
+class ConvectionTransport(_TimeCoupling):
     """GPU replacement of ConvectionTransport + TransportOperatorSplitting (transport.cc)."""
+    _pfx = "fl_transport_"
 
     def __init__(self, mesh, por, cs, bc_conc, init_conc, end_time, save_step):
         self.mesh = mesh
@@ -709,9 +758,14 @@ class ConvectionTransport:
         _check(lib().fl_transport_prepare(self.h, C.byref(c)))
         return c.value
 
-    def update_solution(self, t):
+    def update_solution(self, t=None):
+        """Convection up to target time t; without t the operator-splitting
+        step of its own governor (TransportOperatorSplitting::update_solution)."""
         s = C.c_int()
-        _check(lib().fl_transport_update_solution(self.h, t, C.byref(s)))
+        if t is None:
+            _check(lib().fl_transport_update_solution_step(self.h, C.byref(s)))
+        else:
+            _check(lib().fl_transport_update_solution(self.h, t, C.byref(s)))
         return s.value
 
     def times(self):

@@ -99,3 +99,76 @@ def test_case15_gpu():
             T0.set_bc_conc(c["tbc"][1])
             T1.set_bc_conc(c["tbc"][1])
             T2.set_bc_conc(c["tbc"][1])
+
+
+@pytest.mark.gpu
+def test_case15_gpu_coupling_loop():
+    """test15 driven through the equations' time interface exactly as
+    HC_ExplicitSequential::run_simulation (hc_explicit_sequential.cc:186-231)
+    does it: the transport's upper step constraint is the water's dt, the
+    water solves while its solved time is below the velocity interpolation
+    time (theta 0.5 between the transport's solved and planned times), the
+    transport takes its governor's steps on the last velocity field.  Outputs
+    at every save_step mark against the reference's."""
+    c = load_case15()
+    E, gm = c["E"], c["gm"]
+    S = gm.n_sides
+    water = G.DarcyFlowMH_Steady(gm)
+    water.set_data(np.ones(E), np.ones(E), np.ones(E), c["f"], *c["bc"])
+    water.set_unsteady(True, np.ones(E), np.zeros(E))
+    water.set_time(2.0, DT, 0.1)
+    assert water.time() == (0.0, DT, False)
+    trans = G.ConvectionTransport(gm, np.ones(E), np.ones(E), c["tbc"][0], np.zeros(E), 2.0, 0.1)
+    theta = 0.5
+    velocity_changed, n_water, n_trans, checked, bc_switched = False, 0, 0, 0, False
+    while not (water.time()[2] and trans.time()[2]):
+        trans.set_time_upper_constraint(water.time()[1])
+        vit = theta * trans.planned_time() + (1 - theta) * trans.solved_time()
+        if water.solved_time() < vit:
+            water.update_solution(r_tol=1e-15, a_tol=0.0, max_it=100000)
+            water.choose_next_time()
+            velocity_changed = True
+            n_water += 1
+        else:
+            if velocity_changed:
+                trans.set_velocity_field(darcy=water)
+                velocity_changed = False
+            trans.update_solution()
+            n_trans += 1
+            t = trans.solved_time()
+            k = int(round(t / 0.1))
+            if abs(t - 0.1 * k) <= 1e-9:
+                x = water.get_solution_vector()
+                check_output(c, k, x[S:S + E], trans.conc()[0])
+                checked += 1
+                if k == 1 and not bc_switched:
+                    trans.set_bc_conc(c["tbc"][1])
+                    bc_switched = True
+        assert n_water + n_trans <= 2 * STEPS + 20
+    print("water steps %d, transport steps %d, outputs checked %d" % (n_water, n_trans, checked), flush=True)
+    # the transport governor's step: the water's dt as its upper constraint,
+    # rounded to whole steps until its next mark (time_governor.cc:261-300); the
+    # water's fixed step drifts by ulps near t = 2, so the last interval before
+    # the end mark is cut into one more step
+    assert n_water == STEPS and STEPS <= n_trans <= STEPS + 8 and checked == 20
+    assert abs(water.solved_time() - 2.0) < 1e-9 and abs(trans.solved_time() - 2.0) < 1e-9
+
+
+@pytest.mark.gpu
+def test_steady_water_time_governor():
+    """A steady Darcy handle carries the steady TimeGovernor
+    (time_governor.cc:151-168, 261-330): t = 0, dt = inf, planned inf before
+    the solve; t = inf, dt = 0 and at its end after it."""
+    c = load_case15()
+    E, gm = c["E"], c["gm"]
+    w = G.DarcyFlowMH_Steady(gm)
+    w.set_data(np.ones(E), np.ones(E), np.ones(E), c["f"], *c["bc"])
+    assert w.time() == (0.0, np.inf, False)
+    assert w.planned_time() == np.inf and w.solved_time() == 0.0
+    w.set_time_upper_constraint(0.5)  # no effect on a steady governor's estimate
+    w.choose_next_time()
+    assert w.planned_time() == np.inf
+    w.update_solution(r_tol=1e-12, a_tol=0.0, max_it=10000)
+    assert w.time() == (np.inf, 0.0, True) and w.solved_time() == np.inf
+    with pytest.raises(G.FlowError):
+        w.set_time(2.0, 0.01, 0.1)  # an unsteady governor needs the unsteady form
