@@ -344,18 +344,19 @@ def main():
     upd_ms = float(np.mean([i["ms_update_avg"] for i in infos]))
     achieved = b_op / (op_ms * 1e-3) / 1e9
     upd_bytes = update_bytes(F, slots, w, args.coarse)
-    # Bytes per launch of the operator with all slots live, from the two
-    # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated
-    # in the same runs) over this workload, per launch shape measured;
-    # FETCH_SIZE counts L2 -> fabric requests, Infinity-Cache hits included
-    measured = {(32, 128, 1, 2, 1024, 0): (1836.5e6, 217.1e6), (32, 128, 1, 2, 2048, 1): (1074.0e6, 217.1e6)}
-    key = (args.n, slots, args.precision, args.coarse, int(info["grid"]), int(info["xcd_map"]))
+    # Bytes per launch of the operator (k_c_opl + k_c_opg, all slots live) from
+    # the two rocprofv3 --pmc passes over this workload (FETCH_SIZE x 2.0,
+    # WRITE_SIZE x 1.0, calibrated in the same runs by copies of known size);
+    # FETCH_SIZE counts L2 -> fabric requests, Infinity-Cache hits included.
+    # The tiles and hence the launch shape depend on the mesh only.
+    measured = {(32, 128, 1, 2): (498.07e6, 220.65e6)}
+    key = (args.n, slots, args.precision, args.coarse)
     traffic = None
     if key in measured:
         fb, wb = measured[key]
         traffic = {"bytes": fb + wb, "fetch_bytes": fb, "write_bytes": wb,
-                   "source": "rocprofv3 FETCH_SIZE / WRITE_SIZE passes, launch grid %d, row mapping %d (profiles/)"
-                   % (key[4], key[5])}
+                   "ratio_to_algorithmic": round((fb + wb) / op_bytes(F, E, slots, w, args.coarse), 3),
+                   "source": "rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/batched_two_level_cube200k_256real.md)"}
     if args.coarse and w == 4.0:
         kname = ("k_c_opl + k_c_opg (batched condensed operator, two-level path, FP32: LDS row tiles and the "
                  "general rows on a second stream, one fork/join)")
@@ -372,8 +373,11 @@ def main():
                 "coarse_solve_and_direction_us": round(float(info["ms_coarse_avg"]) * 1e3, 2)}
 
     # --------------------------------------- secondary: 1M mixed + transport
-    peak_copy = stream_peak(torch)
-    log("stream copy: %.1f GB/s" % peak_copy)
+    stream = G.stream_bandwidth(1 << 30, 10)
+    peak_copy = round(stream["copy"], 1)
+    log("stream (hand-written kernels, 1 GiB): copy %.1f read %.1f write %.1f GB/s; torch copy %.1f GB/s"
+        % (stream["copy"], stream["read"], stream["write"], stream_peak(torch)))
+    roofline["measured_stream_GBs"] = {k: round(v, 1) for k, v in stream.items()}
     roofline["measured_copy_GBs"] = peak_copy
     roofline["frac_of_measured_copy"] = round(achieved / peak_copy, 4)
     c2 = None
@@ -382,7 +386,7 @@ def main():
         if c2 and "transport" in c2:
             tr = c2["transport"]
             tr["measured_copy_GBs"] = peak_copy
-            tr["frac_of_measured_copy"] = round(tr["achieved_GBs"] / peak_copy, 4)
+            tr["nonzero_frac_of_measured_copy"] = round(tr["nonzero_GBs"] / peak_copy, 4)
 
     cpu = None
     c2_cpu = c2.pop("_cpu", None) if c2 else None
@@ -422,6 +426,9 @@ def main():
             "compare_batch_forms": compare,
             "iterations": {"mean": float(it.mean()), "min": int(it.min()), "max": int(it.max())},
             "converged": int(info["n_converged"]), "handovers": int(info["handovers"]),
+            "phases_s": {k: round(info[m] * 1e-3, 3) for k, m in (
+                ("queue_empty", "ms_queue_empty"), ("narrowed_16", "ms_narrow16"), ("narrowed_4", "ms_narrow4"),
+                ("batch_loop_end", "ms_loop_end"), ("handover_end", "ms_handover_end"), ("total", "ms_total"))},
             "accuracy": {"checked": len(acc_ids), "accepted": n_acc,
                          "flux_rel_err_max": float(errs[:, 0].max()),
                          "flux_rel_err_median": float(np.median(errs[:, 0])),
@@ -509,14 +516,32 @@ def c2_section(torch, args):
         tt = time.perf_counter() - tt0
         ms_dev, _ = tr.stats()
         tnnz = G.lib().fl_transport_nnz(tr.h)
+        # algorithmic bytes of one step, the formula of the matrix the reference
+        # multiplies (its rows hold the outflow faces as stored zeros): entries
+        # 12 B, row pointers, concentrations read, written and the correction
         step_bytes = tnnz * 12.0 + 4.0 * (E + 1) + E * nsub * 8.0 * 3
+        # stricter: only what the step needs, the non-zero entries, one row
+        # length each, concentrations read once and written once, the boundary
+        # correction of the rows with boundary sides
+        _, _, tval = tr.matrix()
+        nz = int(np.count_nonzero(tval))
+        brows = int((gm.elem_bnd.reshape(-1, 4) >= 0).any(1).sum())
+        nz_bytes = nz * 12.0 + 4.0 * E + E * nsub * 16.0 + brows * nsub * 8.0
         c = tr.conc()
         tot = tr.mass_balance()["totals"]
         per = ms_dev / max(nsteps, 1)
+        # HBM traffic of k_step_c<4> from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE
+        # over this workload (profiles/cg_transport_fractured_cube_1M_trace_and_traffic.md)
+        traffic = 105.17e6 + 33.65e6 if (args.c2_n, nsub) == (55, 4) else None
         out["transport"] = {"substances": nsub, "steps": nsteps, "steps_per_s": round(nsteps / tt, 1),
                             "ms_per_step_device": round(per, 4), "nnz": tnnz, "bytes_per_step": step_bytes,
                             "achieved_GBs": round(step_bytes / (per * 1e-3) / 1e9, 1),
                             "frac_of_peak": round(step_bytes / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "nonzero_entries": nz, "nonzero_bytes_per_step": nz_bytes,
+                            "nonzero_GBs": round(nz_bytes / (per * 1e-3) / 1e9, 1),
+                            "nonzero_frac_of_peak": round(nz_bytes / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "traffic_bytes_per_step": traffic,
+                            "traffic_GBs": round(traffic / (per * 1e-3) / 1e9, 1) if traffic else None,
                             "conc_range": [float(c.min()), float(c.max())],
                             "mass_balance_rel_err": float(abs(tot[0, 0] + tot[0, 2] - tot[0, 1] - tot[0, 3]) /
                                                           max(abs(tot[0, 3]), 1e-300))}

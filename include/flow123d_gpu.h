@@ -20,6 +20,13 @@
  *                              finite and positive (the message names the
  *                              element, and the realization for a batch)
  *   FL_E_DIVERGED         -12  the CG residual became NaN or infinite
+ *   FL_E_CALLBACK         -13  a data callback of the caller returned non-zero;
+ *                              the step it was to supply was not taken
+ *   FL_E_SORPTION         -14  the reaction module's sorption met a negative
+ *                              total mass (the reference stops there,
+ *                              src/reaction/isotherm.cc:62) or an isotherm
+ *                              equation with no root in its bracket (where
+ *                              toms748_solve throws, isotherm.cc:127)
  *
  * Each entry point names the Flow123d 1.7 interface it replaces
  * (paths relative to the Flow123d source tree).
@@ -35,6 +42,8 @@
 #ifndef FLOW123D_GPU_H
 #define FLOW123D_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +52,8 @@ extern "C" {
 #define FL_E_SINGULAR_ELEMENT (-10)
 #define FL_E_BAD_FIELD (-11)
 #define FL_E_DIVERGED (-12)
+#define FL_E_CALLBACK (-13)
+#define FL_E_SORPTION (-14)
 
 typedef struct fl_mesh fl_mesh;
 typedef struct fl_darcy fl_darcy;
@@ -358,6 +369,10 @@ typedef struct {
     double ms_handover;              /* wall time of that phase */
     double ms_coarse_avg;            /* two-level: coarse solve + direction kernel per iteration (sampled) */
     double ms_coarse_setup;          /* two-level: coarse operators and inverses, all refills */
+    /* wall-time phases of the solve (ms from its start): the queue ran empty, the
+     * two-level kernels narrowed to 16 / 4 lanes per row (0: never), the batch
+     * loop ended (hand-over issued), the hand-over path finished */
+    double ms_queue_empty, ms_narrow16, ms_narrow4, ms_loop_end, ms_handover_end;
 } fl_batch_info;
 
 /* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as
@@ -554,12 +569,31 @@ int fl_transport_set_sources(fl_transport *t, const double *density, const doubl
  * sorption_type (0 none, 1 linear, 2 freundlich, 3 langmuir), mult_coef,
  * second_param [n_bulk_regions][n]; rock_density [n_bulk_regions]; molar_mass and
  * solubility [n] (solubility null or <= 0: no limit); solvent_dens; substeps
- * (the reference's default 1000) table intervals.  The porosity must be constant
- * on each bulk region and dual porosity off.  Set before the first step. */
+ * (the reference's default 1000) table intervals.  scale_sorbed is
+ * phi (1 - por_m - por_imm) rock_density molar_mass with por_imm and phi of
+ * fl_transport_set_dual_porosity_sorption (defaults 0 and 1, sorption.cc:172-213).
+ * A bulk region on which the porosities por_m, por_imm and phi are constant is
+ * tabulated (Sorption::make_tables, sorption.cc:148-170); on any other region
+ * every element's isotherm is solved directly (Sorption::compute_reaction
+ * without a table, sorption.cc:236-260, isotherm.cc:184-200).  The direct solve
+ * brackets the root on [0, critical mass / por_m + 1e-5] as the reference does
+ * (isotherm.cc:93-99): without a positive solubility that interval is empty and
+ * the first step stops with FL_E_SORPTION, where the reference's root finder
+ * throws.  Dual porosity off.  Set before the first step; the isotherms are made
+ * at the first step. */
 int fl_transport_set_reaction_sorption(fl_transport *t, int n, const int *subst, const int *sorption_type,
                                        const double *mult_coef, const double *second_param,
                                        const double *rock_density, const double *molar_mass,
                                        const double *solubility, double solvent_dens, int substeps);
+
+/* The same sorption with element-wise parameter fields (the reference's
+ * bulk_data given as element data, so no region has constant parameters and
+ * every element is solved directly, sorption.cc:236-260): sorption_type,
+ * mult_coef, second_param [n][E]; rock_density [E]; molar_mass, solubility [n]. */
+int fl_transport_set_reaction_sorption_elementwise(fl_transport *t, int n, const int *subst, const int *sorption_type,
+                                                   const double *mult_coef, const double *second_param,
+                                                   const double *rock_density, const double *molar_mass,
+                                                   const double *solubility, double solvent_dens);
 
 /* Sorbed concentrations of the reaction module's species, out[n][E]. */
 int fl_transport_get_reaction_sorbed(fl_transport *t, double *out);
@@ -568,8 +602,9 @@ int fl_transport_get_reaction_sorbed(fl_transport *t, double *out);
  * FieldFormula for bc_conc, field_formula_impl.hh:70-122, read by
  * ConvectionTransport::compute_one_step at every step's start, transport.cc:598-627):
  * fn(t, bc_conc[n_subst][NBC], user) is called on the host at the start time t of
- * every step and its data drive that step.  fn null: the last data stay fixed. */
-typedef void (*fl_bc_conc_fn)(double time, double *bc_conc, void *user);
+ * every step and its data drive that step; a non-zero return stops the update
+ * with FL_E_CALLBACK before that step.  fn null: the last data stay fixed. */
+typedef int (*fl_bc_conc_fn)(double time, double *bc_conc, void *user);
 int fl_transport_set_bc_conc_function(fl_transport *t, fl_bc_conc_fn fn, void *user);
 
 /* Dual porosity and sorption of ConvectionTransport (dual_porosity and
@@ -597,6 +632,15 @@ int fl_transport_matrix(fl_transport *t, int *ptr, int *col, double *val);
 
 /* Device time (ms) of the transport steps so far and their count. */
 int fl_transport_stats(fl_transport *t, double *ms_steps, int *steps);
+
+/* ---- measurement (no counterpart in the reference) ------------------------ */
+
+/* Achievable HBM bandwidth of the current device, the denominator beside the
+ * vendor peak in roofline fractions: gbs[0] copy (read + write counted),
+ * gbs[1] read only, gbs[2] write only, in GB/s, over buffers of `bytes` each
+ * (>= 1 MiB, a multiple of 16; allocated and freed by the call), the best of
+ * `reps` timed launches after a warm-up. */
+int fl_device_stream_bandwidth(size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

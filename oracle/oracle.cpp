@@ -1759,8 +1759,17 @@ struct Transport {
         std::vector<double> table;
     };
     std::vector<int> rs_subst;
-    std::vector<Iso> rs_iso;           // [n_bulk_regions][n_species]
+    std::vector<Iso> rs_iso;           // [n_bulk_regions][n_species] (tabulated regions)
+    std::vector<Iso> rs_el;            // [E][n_species] (elements of the other regions)
+    std::vector<char> rs_tabled;       // per bulk region
     std::vector<double> rs_sorbed;     // [n_species][E]
+    // the bulk_data fields per element: type, mult_coef, second_param [n][E],
+    // rock_density [E]; molar masses, solubilities [n]
+    std::vector<int> rs_type;
+    std::vector<double> rs_mult, rs_second, rs_rock, rs_molar, rs_sol;
+    double rs_solvent = 1.0;
+    int rs_substeps = 0;
+    bool rs_elementwise = false, rs_built = false;
     CSR tm;
     std::vector<int> diag_pos;
     double cfl_max_step = 0;
@@ -2003,17 +2012,23 @@ static double iso_func(const Transport::Iso &I, double x) {
 // scale_sorbed f(c / rho) + scale_aqua c = total_mass on [0, critical mass / scale_aqua
 // + 1e-5].  The reference brackets with boost's TOMS 748 to 30 bits and takes the
 // bracket's midpoint; this restatement bisects to the last bit (the two agree to
-// about 1e-9 of the value).
-static void iso_solve(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
+// about 1e-9 of the value).  toms748_solve throws for an empty interval and for
+// ends of one sign: false there.
+static bool iso_solve(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
     double total = I.scale_aqua * c_aqua + I.scale_sorbed * c_sorbed;
     double critic = I.limit * I.scale_aqua + iso_func(I, I.limit / I.rho) * I.scale_sorbed;
     double a = 0.0, b = critic / I.scale_aqua + 0.00001;
+    if (!(a < b)) return false;
     auto F = [&](double c) { return I.scale_sorbed * iso_func(I, c / I.rho) + I.scale_aqua * c - total; };
     double fa = F(a), fb = F(b);
+    if (!std::isfinite(fa) || !std::isfinite(fb)) return false;
     // an end point that is a root is returned as it is (both ends of the bracket)
     if (fa == 0.0) b = a;
     else if (fb == 0.0) a = b;
-    for (int it = 0; it < 200 && a != b; it++) {
+    else if ((fa < 0) == (fb < 0)) return false;
+    // up to 1200 halvings: enough to reach the last bit of any double in [0, b]
+    // (a root near zero, as for Freundlich at tiny concentrations, needs hundreds)
+    for (int it = 0; it < 1200 && a != b; it++) {
         double m = 0.5 * (a + b);
         if (!(m > a && m < b)) break;
         double fm = F(m);
@@ -2022,10 +2037,11 @@ static void iso_solve(const Transport::Iso &I, double &c_aqua, double &c_sorbed)
     }
     c_aqua = (a + b) / 2;
     c_sorbed = (total - I.scale_aqua * c_aqua) / I.scale_sorbed;
+    return true;
 }
 // Isotherm::make_table (isotherm.cc:220-242): n + 1 rotated sorbed values at equidistant
 // total masses up to the mass of the table limit (1 where no solubility limit is set)
-static void iso_table(Transport::Iso &I, int n) {
+static bool iso_table(Transport::Iso &I, int n) {
     if (I.limit <= 0.0) I.limit = 1.0;
     double mass_limit = I.scale_aqua * I.limit + I.scale_sorbed * iso_func(I, I.limit / I.rho);
     I.step = mass_limit / n;
@@ -2033,21 +2049,23 @@ static void iso_table(Transport::Iso &I, int n) {
     I.table.clear();
     for (int i = 0; i <= n; i++) {
         double c_aqua = mass / I.scale_aqua, c_sorbed = 0.0;
-        iso_solve(I, c_aqua, c_sorbed);
+        if (!iso_solve(I, c_aqua, c_sorbed)) return false;
         I.table.push_back(c_sorbed * I.scale_aqua - c_aqua * I.scale_sorbed);
         mass = mass + I.step;
     }
+    return true;
 }
 // Isotherm::compute_projection (isotherm.cc:56-84): the total mass's position in the
 // table, linear interpolation of the rotated coordinate, rotation back; beyond the
 // table the solubility limit (precipitate, :244-258) or a direct solve.
-static void iso_project(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
+// 0, or 1 for a negative total mass (UsrErr in the reference), 2 without a root.
+static int iso_project(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
     double inv_a = I.scale_aqua / (I.scale_aqua * I.scale_aqua + I.scale_sorbed * I.scale_sorbed);
     double inv_s = I.scale_sorbed / (I.scale_aqua * I.scale_aqua + I.scale_sorbed * I.scale_sorbed);
     double total = (I.scale_aqua * c_aqua + I.scale_sorbed * c_sorbed);
     double steps = total / I.step;
     int idx = (int)std::floor(steps);
-    if (idx < 0) throw std::runtime_error("total_mass seems to have negative value");
+    if (idx < 0) return 1;
     if ((size_t)idx < I.table.size() - 1) {
         double rot = I.table[idx] + (steps - idx) * (I.table[idx + 1] - I.table[idx]);
         c_aqua = (total * inv_a - rot * inv_s);
@@ -2056,18 +2074,103 @@ static void iso_project(const Transport::Iso &I, double &c_aqua, double &c_sorbe
         c_aqua = I.limit;
         c_sorbed = (total - I.scale_aqua * I.limit) / I.scale_sorbed;
     } else {
-        iso_solve(I, c_aqua, c_sorbed);
+        double a = c_aqua, s = c_sorbed;
+        if (!iso_solve(I, a, s)) return 2;
+        c_aqua = a, c_sorbed = s;
     }
+    return 0;
 }
-// Sorption::compute_one_step (sorption.cc:230-285): every element, every species
-static void reaction_sorption(Transport &T) {
+// Isotherm::compute_reaction (isotherm.cc:184-200): with a table the projection,
+// without one the solubility limit where c_aqua exceeds it, else a direct solve.
+static int iso_reaction(const Transport::Iso &I, double &c_aqua, double &c_sorbed) {
+    if (!I.table.empty()) return iso_project(I, c_aqua, c_sorbed);
+    if (I.limited && c_aqua > I.limit) {
+        double total = (I.scale_aqua * c_aqua + I.scale_sorbed * c_sorbed);
+        c_aqua = I.limit;
+        c_sorbed = (total - I.scale_aqua * I.limit) / I.scale_sorbed;
+        return 0;
+    }
+    double a = c_aqua, s = c_sorbed;
+    if (!iso_solve(I, a, s)) return 2;
+    c_aqua = a, c_sorbed = s;
+    return 0;
+}
+// Sorption::isotherm_reinit (sorption.cc:172-213) from element e's values: scale_aqua
+// = por_m, scale_sorbed = phi (1 - por_m - por_imm) rock_density molar_mass.
+static Transport::Iso iso_reinit(const Transport &T, int e, int k) {
+    Transport::Iso I;
+    double por_m = T.por[e], por_imm = T.por_imm.empty() ? 0.0 : T.por_imm[e], phi = T.phi.empty() ? 1.0 : T.phi[e];
+    I.type = T.rs_type[(size_t)k * T.E + e];
+    I.c0 = T.rs_mult[(size_t)k * T.E + e];
+    I.c1 = T.rs_second[(size_t)k * T.E + e];
+    I.rho = T.rs_solvent;
+    I.scale_aqua = por_m;
+    I.scale_sorbed = phi * (1 - por_m - por_imm) * T.rs_rock[e] * T.rs_molar[k];
+    I.limit = T.rs_sol[k];
+    I.limited = I.limit > 0.0;
+    return I;
+}
+// Sorption::make_tables (sorption.cc:148-170): a region whose parameters are all
+// constant (the reference: FieldConstant on the region; here: equal values, and never
+// for element-wise parameter fields) gets one table per species made from its first
+// element; every element of another region keeps its own isotherm.  Returns 0, or -1
+// with a zero scale_sorbed (UsrErr in the reference) or a table without a root.
+static int make_sorption(Transport &T) {
+    Mesh &m = *T.m;
+    int nreg = m.rdb.n_bulk;
+    size_t ns = T.rs_subst.size();
+    auto same = [&](int i, int j) {
+        double pi_i = T.por_imm.empty() ? 0.0 : T.por_imm[i], pi_j = T.por_imm.empty() ? 0.0 : T.por_imm[j];
+        double ph_i = T.phi.empty() ? 1.0 : T.phi[i], ph_j = T.phi.empty() ? 1.0 : T.phi[j];
+        if (T.por[i] != T.por[j] || pi_i != pi_j || ph_i != ph_j || T.rs_rock[i] != T.rs_rock[j]) return false;
+        for (size_t k = 0; k < ns; k++) {
+            size_t a = k * T.E + i, b = k * T.E + j;
+            if (T.rs_type[a] != T.rs_type[b] || T.rs_mult[a] != T.rs_mult[b] || T.rs_second[a] != T.rs_second[b])
+                return false;
+        }
+        return true;
+    };
+    std::vector<int> first(nreg, -1);
+    T.rs_tabled.assign(nreg, T.rs_elementwise ? 0 : 1);
+    for (int e = 0; e < T.E; e++) {
+        int r = m.rdb.regs[m.element[e].region].sub_idx;
+        if (first[r] < 0) first[r] = e;
+        else if (T.rs_tabled[r] && !same(e, first[r])) T.rs_tabled[r] = 0;
+    }
+    T.rs_iso.assign((size_t)nreg * ns, Transport::Iso());
+    T.rs_el.assign((size_t)T.E * ns, Transport::Iso());
+    for (int e = 0; e < T.E; e++) {
+        int r = m.rdb.regs[m.element[e].region].sub_idx;
+        bool tab = T.rs_tabled[r] != 0;
+        if (tab && e != first[r]) continue;
+        for (size_t k = 0; k < ns; k++) {
+            Transport::Iso I = iso_reinit(T, e, (int)k);
+            if (I.scale_sorbed == 0.0) return -1;
+            if (tab) {
+                if (!iso_table(I, I.type == 0 ? 1 : T.rs_substeps)) return -1;
+                T.rs_iso[r * ns + k] = I;
+            } else {
+                T.rs_el[e * ns + k] = I;
+            }
+        }
+    }
+    T.rs_built = true;
+    return 0;
+}
+// Sorption::compute_one_step (sorption.cc:230-285): every element, every species.
+// 0, or the error of the first failing element (iso_project's codes).
+static int reaction_sorption(Transport &T) {
     Mesh &m = *T.m;
     size_t ns = T.rs_subst.size();
     for (int e = 0; e < T.E; e++) {
         int r = m.rdb.regs[m.element[e].region].sub_idx;
-        for (size_t k = 0; k < ns; k++)
-            iso_project(T.rs_iso[r * ns + k], T.conc[(size_t)T.rs_subst[k] * T.E + e], T.rs_sorbed[k * T.E + e]);
+        for (size_t k = 0; k < ns; k++) {
+            const Transport::Iso &I = T.rs_tabled[r] ? T.rs_iso[r * ns + k] : T.rs_el[e * ns + k];
+            int rc = iso_reaction(I, T.conc[(size_t)T.rs_subst[k] * T.E + e], T.rs_sorbed[k * T.E + e]);
+            if (rc) return rc;
+        }
     }
+    return 0;
 }
 
 static void one_step(Transport &T) {
@@ -2361,38 +2464,53 @@ void orc_transport_conc_phase(void *h, int phase, double *out) {
 }
 
 // Reaction-module sorption (Sorption, transport_operator_splitting.cc:178-192, 244-287):
-// n species on substances subst[n]; per bulk region (sub-index order) and species the
-// isotherm type, mult_coef, second_param [nreg][n]; rock_density[nreg]; molar_mass[n],
-// solubility[n] (<= 0 or null: no limit); porosity por_m constant per region, por_imm 0,
-// phi 1 (no dual porosity).  Returns 0, or -1 with por_m varying inside a region.
+// n species on substances subst[n]; the isotherm type, mult_coef, second_param per bulk
+// region (sub-index order) and species [nreg][n] with rock_density[nreg], or
+// (elementwise) per species and element [n][E] with rock_density[E]; molar_mass[n],
+// solubility[n] (<= 0 or null: no limit, the reference's default -1).  The isotherms
+// are made at the first update, with the por_imm and phi of orc_transport_set_phases.
+static void set_sorption(Transport &T, int n, const int *subst, const int *type, const double *mult,
+                         const double *second, const double *rock_density, const double *molar_mass,
+                         const double *solubility, double solvent_dens, int substeps, bool elementwise) {
+    Mesh &m = *T.m;
+    T.rs_subst.assign(subst, subst + n);
+    size_t ne = (size_t)n * T.E;
+    T.rs_type.assign(ne, 0);
+    T.rs_mult.assign(ne, 0.0);
+    T.rs_second.assign(ne, 0.0);
+    T.rs_rock.assign(T.E, 0.0);
+    for (int e = 0; e < T.E; e++) {
+        int r = m.rdb.regs[m.element[e].region].sub_idx;
+        T.rs_rock[e] = elementwise ? rock_density[e] : rock_density[r];
+        for (int k = 0; k < n; k++) {
+            size_t src = elementwise ? (size_t)k * T.E + e : (size_t)r * n + k;
+            T.rs_type[(size_t)k * T.E + e] = type[src];
+            T.rs_mult[(size_t)k * T.E + e] = mult[src];
+            T.rs_second[(size_t)k * T.E + e] = second[src];
+        }
+    }
+    T.rs_molar.assign(molar_mass, molar_mass + n);
+    T.rs_sol.assign(n, -1.0);
+    if (solubility) T.rs_sol.assign(solubility, solubility + n);
+    T.rs_solvent = solvent_dens;
+    T.rs_substeps = substeps;
+    T.rs_elementwise = elementwise;
+    T.rs_built = false;
+    T.rs_sorbed.assign((size_t)n * T.E, 0.0);
+}
 int orc_transport_set_reaction_sorption(void *h, int n, const int *subst, const int *type, const double *mult,
                                         const double *second, const double *rock_density, const double *molar_mass,
                                         const double *solubility, double solvent_dens, int substeps) {
-    Transport &T = *(Transport *)h;
-    Mesh &m = *T.m;
-    int nreg = m.rdb.n_bulk;
-    std::vector<double> por(nreg, -1.0);
-    for (int e = 0; e < T.E; e++) {
-        int r = m.rdb.regs[m.element[e].region].sub_idx;
-        if (por[r] >= 0 && por[r] != T.por[e]) return -1;
-        por[r] = T.por[e];
-    }
-    T.rs_subst.assign(subst, subst + n);
-    T.rs_iso.assign((size_t)nreg * n, Transport::Iso());
-    for (int r = 0; r < nreg; r++)
-        for (int k = 0; k < n; k++) {
-            Transport::Iso &I = T.rs_iso[(size_t)r * n + k];
-            I.type = type[r * n + k];
-            I.c0 = mult[r * n + k];
-            I.c1 = second[r * n + k];
-            I.rho = solvent_dens;
-            I.scale_aqua = por[r];
-            I.scale_sorbed = 1.0 * (1 - por[r] - 0.0) * rock_density[r] * molar_mass[k];
-            I.limit = solubility ? solubility[k] : 0.0;
-            I.limited = I.limit > 0.0;
-            iso_table(I, I.type == 0 ? 1 : substeps);
-        }
-    T.rs_sorbed.assign((size_t)n * T.E, 0.0);
+    set_sorption(*(Transport *)h, n, subst, type, mult, second, rock_density, molar_mass, solubility, solvent_dens,
+                 substeps, false);
+    return 0;
+}
+int orc_transport_set_reaction_sorption_el(void *h, int n, const int *subst, const int *type, const double *mult,
+                                           const double *second, const double *rock_density,
+                                           const double *molar_mass, const double *solubility,
+                                           double solvent_dens) {
+    set_sorption(*(Transport *)h, n, subst, type, mult, second, rock_density, molar_mass, solubility, solvent_dens,
+                 1, true);
     return 0;
 }
 void orc_transport_reaction_sorbed(void *h, double *out) {
@@ -2417,7 +2535,9 @@ void orc_transport_set_sources(void *h, const double *density, const double *sig
 }
 
 // Advance to target time t (TransportOperatorSplitting::update_solution,
-// transport_operator_splitting.cc:244-287); returns steps taken.
+// transport_operator_splitting.cc:244-287); returns steps taken, or -1 when the
+// sorption isotherms cannot be made (a zero scale_sorbed, a table without a root),
+// -2 when a sorption step fails (negative total mass, no root in the bracket).
 int orc_transport_update(void *h, double t) {
     Transport &T = *(Transport *)h;
     if (T.mb_initial) mb_calculate(T, T.tg.time);  // first_time_call output at t=0
@@ -2428,10 +2548,11 @@ int orc_transport_update(void *h, double t) {
     }
     int steps = 0;
     double t0 = now_s();
+    if (!T.rs_subst.empty() && !T.rs_built && make_sorption(T)) return -1;
     while (T.tg.lt(t)) {
         one_step(T);
         if (!T.r_parent.empty()) compute_reactions(T);
-        if (!T.rs_subst.empty()) reaction_sorption(T);
+        if (!T.rs_subst.empty() && reaction_sorption(T)) return -2;
         mb_calculate(T, T.tg.time);
         steps++;
     }

@@ -73,6 +73,8 @@ def lib():
         L.orc_transport_set_bc_function.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_transport_set_reaction_sorption.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                                           _f64p, _f64p, C.c_double, C.c_int]
+        L.orc_transport_set_reaction_sorption_el.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p,
+                                                             _f64p, _f64p, _f64p, C.c_double]
         L.orc_transport_reaction_sorbed.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_velocity.argtypes = [C.c_void_p, _f64p]
         L.orc_transport_set_pade.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -258,7 +260,12 @@ class Transport:
         self.h = lib().orc_transport_create(mesh.h, sf, cs, por, self.nsub, bcc, ic, end_time, save_step)
 
     def update(self, t):
-        return lib().orc_transport_update(self.h, t)
+        n = lib().orc_transport_update(self.h, t)
+        if n == -1:
+            raise ValueError("sorption: the isotherms cannot be made (zero scale_sorbed or a table without a root)")
+        if n == -2:
+            raise ValueError("sorption: negative total mass or an isotherm equation without a root in its bracket")
+        return n
 
     @property
     def time(self):
@@ -361,8 +368,22 @@ class Transport:
                     np.ascontiguousarray(np.broadcast_to(molar_mass, (n,)), np.float64),
                     np.ascontiguousarray(np.broadcast_to(-1.0 if solubility is None else solubility, (n,)),
                                          np.float64)]
-        if lib().orc_transport_set_reaction_sorption(self.h, n, *self._rs, float(solvent_dens), int(substeps)):
-            raise ValueError("porosity must be constant on each bulk region")
+        lib().orc_transport_set_reaction_sorption(self.h, n, *self._rs, float(solvent_dens), int(substeps))
+
+    def set_reaction_sorption_elementwise(self, subst, types, mult, second, rock_density, molar_mass,
+                                          solubility=None, solvent_dens=1.0):
+        """The same with element-wise parameter fields: types, mult, second [n][E];
+        rock_density [E] (every element's isotherm solved directly)."""
+        n, E = len(subst), self.mesh.n_elements
+        self._rs = [np.ascontiguousarray(subst, np.int32),
+                    np.ascontiguousarray(np.broadcast_to(types, (n, E)), np.int32).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(mult, (n, E)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(second, (n, E)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(rock_density, (E,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(molar_mass, (n,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(-1.0 if solubility is None else solubility, (n,)),
+                                         np.float64)]
+        lib().orc_transport_set_reaction_sorption_el(self.h, n, *self._rs, float(solvent_dens))
 
     def reaction_sorbed(self):
         out = np.zeros(len(self._rs[0]) * self.mesh.n_elements)

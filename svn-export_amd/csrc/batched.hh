@@ -41,6 +41,9 @@ struct BatchStats {
     double ms_handover = 0;         // wall time of the hand-over phase
     double ms_coarse_avg = 0;       // live samples: coarse fold + direction kernel per iteration
     double ms_coarse_setup = 0;     // coarse operators and their inverses, all refills
+    // wall-time phases (ms from the start): queue empty, narrowed to 16 / 4 lanes,
+    // batch loop ended, hand-over path finished
+    double ms_queue_empty = 0, ms_narrow16 = 0, ms_narrow4 = 0, ms_loop_end = 0, ms_handover_end = 0;
     long total_iterations = 0;  // summed over realizations
     double ms_setup = 0, ms_cg = 0, ms_resolve = 0, ms_total = 0;
     double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels

@@ -2744,6 +2744,9 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     BatchStats st;
     st.n_real = R_;
     auto t_start = std::chrono::steady_clock::now();
+    auto since_start = [&]() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    };
     hipEvent_t e0, e1;
     FL_HIP(hipEventCreate(&e0));
     FL_HIP(hipEventCreate(&e1));
@@ -3232,6 +3235,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         const int n = (int)src.size();
         const int w = n <= 4 ? 4 : (n <= 16 ? 16 : 64);
         if (w >= lw_) return;
+        (w == 4 ? st.ms_narrow4 : st.ms_narrow16) = since_start();
         for (int s = 0; s < Rp_; s++)
             if (!live(s)) src.push_back(s);
         permute_slots(src, phase);
@@ -3366,6 +3370,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             }
             read_status();
         }
+        if (next >= queue.size() && st.ms_queue_empty == 0) st.ms_queue_empty = since_start();
         if (nrun == 0) break;
         const bool narrow = narrow_ && NC > 0;
         const int hand = handover_ >= 0 ? handover_ : (narrow ? 4 : 16);
@@ -3385,7 +3390,9 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         FL_HIP(hipGraphLaunch(graph(mixed), stream_));
         st.chunks++;
     }
+    st.ms_loop_end = since_start();
     hand_join();
+    st.ms_handover_end = since_start();
     if (herr) std::rethrow_exception(herr);
     for (const HandResult &h : hres) {
         h_status_[h.real] = h.ss.converged ? kConverged : kMaxIt;

@@ -616,6 +616,11 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->ms_handover = st.ms_handover;
             info->ms_coarse_avg = st.ms_coarse_avg;
             info->ms_coarse_setup = st.ms_coarse_setup;
+            info->ms_queue_empty = st.ms_queue_empty;
+            info->ms_narrow16 = st.ms_narrow16;
+            info->ms_narrow4 = st.ms_narrow4;
+            info->ms_loop_end = st.ms_loop_end;
+            info->ms_handover_end = st.ms_handover_end;
         }
         if (nconv == R) return 0;
         int ndiv = 0;
@@ -995,7 +1000,18 @@ int fl_transport_set_reaction_sorption(fl_transport *t, int n, const int *subst,
                                        const double *solubility, double solvent_dens, int substeps) {
     return guard([&] {
         t->g->set_reaction_sorption(n, subst, sorption_type, mult_coef, second_param, rock_density, molar_mass,
-                                    solubility, solvent_dens, substeps);
+                                    solubility, solvent_dens, substeps, false);
+        return 0;
+    });
+}
+
+int fl_transport_set_reaction_sorption_elementwise(fl_transport *t, int n, const int *subst, const int *sorption_type,
+                                                   const double *mult_coef, const double *second_param,
+                                                   const double *rock_density, const double *molar_mass,
+                                                   const double *solubility, double solvent_dens) {
+    return guard([&] {
+        t->g->set_reaction_sorption(n, subst, sorption_type, mult_coef, second_param, rock_density, molar_mass,
+                                    solubility, solvent_dens, 1, true);
         return 0;
     });
 }
@@ -1034,6 +1050,13 @@ int fl_transport_stats(fl_transport *t, double *ms, int *steps) {
     if (ms) *ms = t->g->stats().ms_steps;
     if (steps) *steps = t->g->stats().steps;
     return 0;
+}
+
+int fl_device_stream_bandwidth(size_t bytes, int reps, double *gbs) {
+    return guard([&] {
+        fl::stream_bandwidth(bytes, reps, gbs);
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@ enum Status : int {
     kErrSingularElement = -10,  // |det| of a local RT0 matrix below NUM_ZERO (mh_fe_values.cc:54-59)
     kErrBadField = -11,         // conductivity / cross-section not finite and positive
     kErrDiverged = -12,         // CG residual became NaN or infinite
+    kErrCallback = -13,         // a caller's data callback reported a failure
+    kErrSorption = -14,         // sorption: negative total mass or an unbracketed isotherm root
 };
 
 // NUM_ZERO of the reference (math_fce.h:51): the singularity threshold of
@@ -136,5 +138,8 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partial, un
     if (threadIdx.x == 0) *count = 0;
     return true;
 }
+
+// Achievable HBM bandwidth (stream.hip): gbs[0] copy, gbs[1] read, gbs[2] write, GB/s.
+void stream_bandwidth(size_t bytes, int reps, double *gbs);
 
 }  // namespace fl

@@ -75,8 +75,9 @@ class TransportGPU {
     void set_bc_conc(const double *bc_conc);
     // boundary concentrations as a function of time (a time-dependent FieldFormula,
     // field_formula_impl.hh:70-122): fn(t, bc_conc[nsub][NBC], user) at every step's
-    // start time, the data then used for that step; fn null: fixed data again
-    using BcFn = void (*)(double, double *, void *);
+    // start time, the data then used for that step (a non-zero return stops the
+    // update before the step); fn null: fixed data again
+    using BcFn = int (*)(double, double *, void *);
     void set_bc_function(BcFn fn, void *user) { bc_fn_ = fn; bc_user_ = user; }
     // Linear_reaction (linear_reaction.cc:144-220): decays i with parent[i],
     // products[pptr[i]..pptr[i+1]), half_life[i] and branch ratios; the matrix is
@@ -96,14 +97,15 @@ class TransportGPU {
     void set_phases(bool dual, bool sorption, const double *por_imm, const double *alpha, const double *phi,
                     const int *sorp_type, const double *sorp_coef0, const double *sorp_coef1);
     // Sorption of the reaction module (reaction/sorption.cc, isotherm.cc) after every step:
-    // species k on substance subst[k]; per bulk region (sub-index order) and species the
-    // isotherm type (0 none, 1 linear, 2 freundlich, 3 langmuir), mult_coef, second_param
-    // [nreg][n]; rock_density [nreg]; molar_mass, solubility [n] (<= 0: no limit).  The
-    // porosity must be constant on each region (the reference then tabulates the
-    // isotherm per region); no dual porosity.
+    // species k on substance subst[k]; the isotherm type (0 none, 1 linear, 2 freundlich,
+    // 3 langmuir), mult_coef, second_param per bulk region (sub-index order) and species
+    // [nreg][n] with rock_density [nreg], or (elementwise) per species and element
+    // [n][E] with rock_density [E]; molar_mass, solubility [n] (<= 0: no limit).
+    // Regions whose parameters and porosities are constant are tabulated, the rest
+    // solved per element; no dual porosity.
     void set_reaction_sorption(int n, const int *subst, const int *type, const double *mult, const double *second,
                                const double *rock_density, const double *molar_mass, const double *solubility,
-                               double solvent_dens, int substeps);
+                               double solvent_dens, int substeps, bool elementwise);
     void get_reaction_sorbed(double *out) const;  // [n][E]
     // phase 0 mobile, 1 immobile, 2 mobile sorbed, 3 immobile sorbed (transport.h:58-63) [nsub][E]
     void get_conc_phase(int phase, double *out) const;
@@ -146,10 +148,19 @@ class TransportGPU {
     bool bc_pending_ = false;
     BcFn bc_fn_ = nullptr;
     void *bc_user_ = nullptr;
-    int rs_n_ = 0, rs_len_ = 0;   // reaction sorption: species, table length
-    const int *d_rs_subst_ = nullptr, *d_el_region_ = nullptr;
-    const double *d_rs_par_ = nullptr, *d_rs_table_ = nullptr;  // [nreg*n][8] parameters, [nreg*n][len] tables
+    // reaction sorption: species, their parameters per element [n][E] (rock
+    // density [E]), made into isotherms at the first step
+    int rs_n_ = 0, rs_substeps_ = 0;
+    bool rs_elementwise_ = false, rs_built_ = false;
+    double rs_solvent_ = 1.0;
+    std::vector<int> rs_subst_, rs_type_;
+    std::vector<double> rs_mult_, rs_second_, rs_rock_, rs_molar_, rs_sol_;
+    void build_reaction_sorption();
+    void check_sorption();
+    const int *d_rs_subst_ = nullptr, *d_rs_el_par_ = nullptr;
+    const double *d_rs_par_ = nullptr, *d_rs_table_ = nullptr;  // isotherms, their tables back to back
     double *d_rs_sorbed_ = nullptr;                                // [E][n]
+    int *d_rs_err_ = nullptr;                                      // error bits, first failing element
     bool phases_on() const { return dual_ || sorp_; }
     bool dual_ = false, sorp_ = false;
     // per element [E][m] interleaved like the concentrations: immobile, mobile sorbed,
@@ -158,7 +169,7 @@ class TransportGPU {
     double *d_imm_ = nullptr, *d_msorb_ = nullptr, *d_isorb_ = nullptr;
     const double *d_alpha_ = nullptr, *d_sc0_ = nullptr, *d_sc1_ = nullptr, *d_por_imm_ = nullptr, *d_phi_ = nullptr;
     const int *d_stype_ = nullptr;
-    std::vector<double> h_por_imm_;
+    std::vector<double> h_por_imm_, h_phi_;
     void build_reaction_matrix(double dt);
     void define_reactions(int n_decays, const int *parent, const int *pptr, const int *products,
                           const double *half_life, const double *branch);

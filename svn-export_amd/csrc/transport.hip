@@ -288,11 +288,13 @@ __global__ void __launch_bounds__(256) k_phases(const TmDev t, const PhaseDev ph
 }
 
 // Isotherm of the reaction module's sorption (isotherm.hh:100-170, isotherm.cc):
-// parameters per (region, species): type, c0 (mult_coef), c1 (second_param),
-// rho (solvent density), scale_aqua (porosity), scale_sorbed (phi (1 - porosity)
-// rock density molar mass), limit (solubility, or 1 for the table), limited, step.
+// type, c0 (mult_coef), c1 (second_param), rho (solvent density), sa (scale_aqua:
+// the mobile porosity), ss (scale_sorbed: phi (1 - por_m - por_imm) rock density
+// molar mass), limit (the solubility; 1 for a table made without one), limited,
+// step (the table's total-mass step); tab = offset of the interpolation table and
+// len its length, or tab = -1 for an isotherm solved directly per element.
 struct IsoPar {
-    int type;
+    int type, tab, len;
     double c0, c1, rho, sa, ss, limit, step;
     bool limited;
 };
@@ -304,17 +306,23 @@ __host__ __device__ inline double iso_f(const IsoPar &I, double x) {
 }
 // Isotherm::solve_conc (isotherm.cc:86-109): the root c of ss f(c / rho) + sa c = total
 // on [0, critical mass / sa + 1e-5], by bisection to the last bit (the reference uses a
-// 30-bit TOMS 748 bracket); an end point that is a root is returned exactly.
-__host__ __device__ inline void iso_solve(const IsoPar &I, double &c_aqua, double &c_sorbed) {
+// 30-bit TOMS 748 bracket); an end point that is a root is returned exactly.  False
+// where toms748_solve throws: an empty interval, or ends of one sign (or not a number).
+__host__ __device__ inline bool iso_solve(const IsoPar &I, double &c_aqua, double &c_sorbed) {
     const double total = I.sa * c_aqua + I.ss * c_sorbed;
     const double critic = I.limit * I.sa + iso_f(I, I.limit / I.rho) * I.ss;
     double a = 0.0, b = critic / I.sa + 0.00001;
+    if (!(a < b)) return false;
     const double fa0 = I.ss * iso_f(I, a / I.rho) + I.sa * a - total;
     const double fb0 = I.ss * iso_f(I, b / I.rho) + I.sa * b - total;
+    if (!(fabs(fa0) <= DBL_MAX) || !(fabs(fb0) <= DBL_MAX)) return false;
     if (fa0 == 0.0) b = a;
     else if (fb0 == 0.0) a = b;
+    else if ((fa0 < 0) == (fb0 < 0)) return false;
     const bool neg_a = fa0 < 0;
-    for (int it = 0; it < 200 && a != b; it++) {
+    // up to 1200 halvings: enough to reach the last bit of any double in [0, b]
+    // (a root near zero, as for Freundlich at tiny concentrations, needs hundreds)
+    for (int it = 0; it < 1200 && a != b; it++) {
         const double m = 0.5 * (a + b);
         if (!(m > a && m < b)) break;
         const double fm = I.ss * iso_f(I, m / I.rho) + I.sa * m - total;
@@ -323,43 +331,73 @@ __host__ __device__ inline void iso_solve(const IsoPar &I, double &c_aqua, doubl
     }
     c_aqua = (a + b) / 2;
     c_sorbed = (total - I.sa * c_aqua) / I.ss;
+    return true;
 }
 
-// Sorption::compute_one_step (sorption.cc:263-285) through Isotherm::compute_projection
-// (isotherm.cc:56-84): per element and species, the total mass's place in the region's
-// table, the rotated sorbed coordinate interpolated and rotated back; beyond the table
-// the solubility limit or a direct solve.  par: [nreg*n] IsoPar, tab: [nreg*n][len].
+// Error bits of the sorption kernel (FL_E_SORPTION): a negative total mass
+// (Isotherm::compute_projection, isotherm.cc:62) and an isotherm equation with
+// no root in its bracket (toms748_solve throws).
+constexpr int kSorpNegMass = 1, kSorpNoRoot = 2;
+
+// Sorption::compute_one_step (sorption.cc:263-285) for every element and species.
+// With a table, Isotherm::compute_projection (isotherm.cc:56-84): the total mass's
+// place in the table, the rotated sorbed coordinate interpolated and rotated back;
+// beyond the table the solubility limit or a direct solve.  Without one (parameters
+// not constant on the region, sorption.cc:236-249), Isotherm::compute_reaction
+// (isotherm.cc:184-200): the solubility limit when c_aqua exceeds it, else a
+// direct solve.  el_par[i]: index of element i's first isotherm in par (its
+// region's, or its own).  A failure leaves the element's values, sets err[0] and
+// the lowest failing element in err[1].
 template <int M>
 __global__ void __launch_bounds__(256) k_rsorp(int E, int n, const int *__restrict__ subst,
-                                              const int *__restrict__ region, const IsoPar *__restrict__ par,
-                                              const double *__restrict__ tab, int len, double *__restrict__ c,
-                                              double *__restrict__ sorbed) {
+                                              const int *__restrict__ el_par, const IsoPar *__restrict__ par,
+                                              const double *__restrict__ tab, double *__restrict__ c,
+                                              double *__restrict__ sorbed, int *__restrict__ err) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
-        const int r = region[i];
+        const int base = el_par[i];
+        int fail = 0;
         for (int k = 0; k < n; k++) {
-            const IsoPar I = par[r * n + k];
-            const double *T = tab + (size_t)(r * n + k) * len;
+            const IsoPar I = par[base + k];
             double &ca = c[(size_t)i * M + subst[k]];
             double &cs = sorbed[(size_t)i * n + k];
             double c_aqua = ca, c_sorbed = cs;
-            const double den = I.sa * I.sa + I.ss * I.ss;
-            const double inv_a = I.sa / den, inv_s = I.ss / den;
             const double total = (I.sa * c_aqua + I.ss * c_sorbed);
-            const double steps = total / I.step;
-            const int idx = (int)floor(steps);
-            const int tl = I.type == 0 ? 2 : len;
-            if (idx >= 0 && idx < tl - 1) {
-                const double rot = T[idx] + (steps - idx) * (T[idx + 1] - T[idx]);
-                c_aqua = (total * inv_a - rot * inv_s);
-                c_sorbed = (total * inv_s + rot * inv_a);
-            } else if (idx >= 0 && I.limited) {
+            bool ok = true;
+            if (I.tab >= 0) {
+                const double *T = tab + I.tab;
+                const double den = I.sa * I.sa + I.ss * I.ss;
+                const double inv_a = I.sa / den, inv_s = I.ss / den;
+                const double steps = total / I.step;
+                const int idx = (int)floor(steps);
+                if (idx < 0) {
+                    fail |= kSorpNegMass;
+                    ok = false;
+                } else if (idx < I.len - 1) {
+                    const double rot = T[idx] + (steps - idx) * (T[idx + 1] - T[idx]);
+                    c_aqua = (total * inv_a - rot * inv_s);
+                    c_sorbed = (total * inv_s + rot * inv_a);
+                } else if (I.limited) {
+                    c_aqua = I.limit;
+                    c_sorbed = (total - I.sa * I.limit) / I.ss;
+                } else {
+                    ok = iso_solve(I, c_aqua, c_sorbed);
+                    if (!ok) fail |= kSorpNoRoot;
+                }
+            } else if (I.limited && c_aqua > I.limit) {
                 c_aqua = I.limit;
                 c_sorbed = (total - I.sa * I.limit) / I.ss;
-            } else if (idx >= 0) {
-                iso_solve(I, c_aqua, c_sorbed);
+            } else {
+                ok = iso_solve(I, c_aqua, c_sorbed);
+                if (!ok) fail |= kSorpNoRoot;
             }
-            ca = c_aqua;
-            cs = c_sorbed;
+            if (ok) {
+                ca = c_aqua;
+                cs = c_sorbed;
+            }
+        }
+        if (fail) {
+            atomicOr(err, fail);
+            atomicMin(err + 1, i);
         }
     }
 }
@@ -947,7 +985,8 @@ void TransportGPU::one_step() {
     bool bc_changed = false;
     if (bc_fn_) {
         h_bc_next_.assign((size_t)nsub_ * NBC_, 0.0);
-        bc_fn_(tg_.time, h_bc_next_.data(), bc_user_);
+        if (bc_fn_(tg_.time, h_bc_next_.data(), bc_user_) != 0)
+            throw Error(kErrCallback, "the boundary concentration function failed at t = " + std::to_string(tg_.time));
         bc_pending_ = true;
     }
     if (bc_pending_) {
@@ -1013,11 +1052,12 @@ void TransportGPU::one_step() {
         FL_HIP(hipGetLastError());
     }
     if (rs_n_) {
+        if (!rs_built_) build_reaction_sorption();
         switch (nsub_) {
 #define FL_RS(MM)                                                                                                 \
     case MM:                                                                                                      \
-        hipLaunchKernelGGL(k_rsorp<MM>, dim3(gridS), dim3(kBlock), 0, stream_, E_, rs_n_, d_rs_subst_, d_el_region_, \
-                           (const IsoPar *)d_rs_par_, d_rs_table_, rs_len_, cnew, d_rs_sorbed_);                   \
+        hipLaunchKernelGGL(k_rsorp<MM>, dim3(gridS), dim3(kBlock), 0, stream_, E_, rs_n_, d_rs_subst_, d_rs_el_par_, \
+                           (const IsoPar *)d_rs_par_, d_rs_table_, cnew, d_rs_sorbed_, d_rs_err_);              \
         break;
             FL_RS(1) FL_RS(2) FL_RS(3) FL_RS(4) FL_RS(5) FL_RS(6) FL_RS(7) FL_RS(8)
 #undef FL_RS
@@ -1039,9 +1079,16 @@ int TransportGPU::update_solution(double target) {
     need_rescale_ = true;
     // decayRad->set_time_step(convection->time().estimate_dt()) (transport_operator_splitting.cc:258)
     if (react_on_) build_reaction_matrix(tg_.estimate_dt());
-    hipEvent_t a, b;
-    FL_HIP(hipEventCreate(&a)); FL_HIP(hipEventCreate(&b));
-    FL_HIP(hipEventRecord(a, stream_));
+    // the events are destroyed on every exit, a failed step included
+    struct Events {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    FL_HIP(hipEventCreate(&ev.a)); FL_HIP(hipEventCreate(&ev.b));
+    FL_HIP(hipEventRecord(ev.a, stream_));
     int steps = 0;
     while (tg_.lt(target)) {
         double t_before = tg_.time;
@@ -1051,12 +1098,12 @@ int TransportGPU::update_solution(double target) {
             throw Error(-7, "transport time step does not advance (dt = " + std::to_string(tg_.time_step) + ")");
     }
     flush_flux();
-    FL_HIP(hipEventRecord(b, stream_));
-    FL_HIP(hipEventSynchronize(b));
+    FL_HIP(hipEventRecord(ev.b, stream_));
+    FL_HIP(hipEventSynchronize(ev.b));
     float ms = 0;
-    FL_HIP(hipEventElapsedTime(&ms, a, b));
+    FL_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
     stats_.ms_steps += ms;
-    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    if (rs_built_) check_sorption();
     return steps;
 }
 
@@ -1093,6 +1140,7 @@ void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, c
     d_por_imm_ = upv(owned_, h_por_imm_, stream_);
     std::vector<double> ph(E_, 1.0);
     if (phi) ph.assign(phi, phi + E_);
+    h_phi_ = ph;
     d_phi_ = upv(owned_, ph, stream_);
     d_alpha_ = upv(owned_, inter(alpha, 0.0), stream_);
     d_sc0_ = upv(owned_, inter(sorp_coef0, 0.0), stream_);
@@ -1107,73 +1155,190 @@ void TransportGPU::set_phases(bool dual, bool sorption, const double *por_imm, c
     FL_HIP(hipStreamSynchronize(stream_));
 }
 
+// Sorption::Sorption (sorption.cc:71-146): the species and their parameters,
+// kept per element ([n][E] like the ABI's arrays); the isotherms are made at the
+// first step, when the porosities of the transport are final.
 void TransportGPU::set_reaction_sorption(int n, const int *subst, const int *type, const double *mult,
                                          const double *second, const double *rock_density, const double *molar_mass,
-                                         const double *solubility, double solvent_dens, int substeps) {
+                                         const double *solubility, double solvent_dens, int substeps,
+                                         bool elementwise) {
     if (stats_.steps > 0) throw Error(-3, "sorption must be set before the first step");
     if (n < 1 || n > nsub_) throw Error(-3, "sorbing species must be 1..n_subst");
     if (dual_) throw Error(-3, "the reaction module's sorption is not combined with dual porosity here");
     if (substeps < 1 || substeps > 100000) throw Error(-3, "substeps must be in 1..100000");
+    for (int k = 0; k < n; k++)
+        if (subst[k] < 0 || subst[k] >= nsub_) throw Error(-3, "Wrong name of " + std::to_string(k) + "-th adsorbing specie.");
     const int nreg = m_.n_bulk_regions;
-    std::vector<double> por(nreg, -1.0);
+    const size_t ne = (size_t)n * E_;
+    rs_type_.resize(ne);
+    rs_mult_.resize(ne);
+    rs_second_.resize(ne);
+    rs_rock_.resize(E_);
     for (int i = 0; i < E_; i++) {
         const int r = h_el_region_sub_[i];
-        if (por[r] >= 0 && por[r] != h_por_[i]) throw Error(-3, "porosity must be constant on each bulk region");
-        por[r] = h_por_[i];
-    }
-    std::vector<int> sub(subst, subst + n);
-    for (int k = 0; k < n; k++)
-        if (sub[k] < 0 || sub[k] >= nsub_) throw Error(-3, "Wrong name of " + std::to_string(k) + "-th adsorbing specie.");
-    const int len = substeps + 1;
-    std::vector<IsoPar> par((size_t)nreg * n);
-    std::vector<double> tab((size_t)nreg * n * len, 0.0);
-    for (int r = 0; r < nreg; r++)
+        rs_rock_[i] = elementwise ? rock_density[i] : rock_density[r];
         for (int k = 0; k < n; k++) {
-            const int t = type[r * n + k];
+            const size_t src = elementwise ? (size_t)k * E_ + i : (size_t)r * n + k;
+            const int t = type[src];
             if (t < 0 || t > 3) throw Error(-3, "sorption type must be 0..3 (none, linear, freundlich, langmuir)");
-            IsoPar I{};
-            I.type = t;
-            I.c0 = mult[r * n + k];
-            I.c1 = second[r * n + k];
-            I.rho = solvent_dens;
-            I.sa = por[r] < 0 ? 1.0 : por[r];
-            I.ss = 1.0 * (1 - I.sa - 0.0) * rock_density[r] * molar_mass[k];
-            if (I.ss == 0.0) throw Error(-3, "scale_sorbed (phi (1 - por_m - por_imm) rock_density molar_mass) is zero");
-            I.limit = solubility ? solubility[k] : 0.0;
-            I.limited = I.limit > 0.0;
+            rs_type_[(size_t)k * E_ + i] = t;
+            rs_mult_[(size_t)k * E_ + i] = mult[src];
+            rs_second_[(size_t)k * E_ + i] = second[src];
+        }
+    }
+    (void)nreg;
+    rs_subst_.assign(subst, subst + n);
+    rs_molar_.assign(molar_mass, molar_mass + n);
+    rs_sol_.assign(n, -1.0);
+    if (solubility) rs_sol_.assign(solubility, solubility + n);
+    rs_solvent_ = solvent_dens;
+    rs_substeps_ = substeps;
+    rs_elementwise_ = elementwise;
+    rs_n_ = n;
+    rs_built_ = false;
+}
+
+// Sorption::make_tables + isotherm_reinit (sorption.cc:148-213): a region whose
+// parameters (rock density, isotherm coefficients, por_m, por_imm, phi) are all
+// constant gets one table per species (Isotherm::make_table, isotherm.cc:202-242);
+// the other regions' elements keep their own isotherms, solved directly.  The
+// reference decides by the field type (FieldConstant) on the region; here the
+// values decide, and the element-wise entry point always takes the direct path.
+void TransportGPU::build_reaction_sorption() {
+    if (dual_) throw Error(-3, "the reaction module's sorption is not combined with dual porosity here");
+    const int n = rs_n_, nreg = m_.n_bulk_regions;
+    auto por_imm = [&](int i) { return h_por_imm_.empty() ? 0.0 : h_por_imm_[i]; };
+    auto phi = [&](int i) { return h_phi_.empty() ? 1.0 : h_phi_[i]; };
+    auto same = [&](int i, int j) {
+        if (h_por_[i] != h_por_[j] || por_imm(i) != por_imm(j) || phi(i) != phi(j) || rs_rock_[i] != rs_rock_[j])
+            return false;
+        for (int k = 0; k < n; k++) {
+            const size_t a = (size_t)k * E_ + i, b = (size_t)k * E_ + j;
+            if (rs_type_[a] != rs_type_[b] || rs_mult_[a] != rs_mult_[b] || rs_second_[a] != rs_second_[b])
+                return false;
+        }
+        return true;
+    };
+    std::vector<int> first(nreg, -1);
+    std::vector<char> tabled(nreg, rs_elementwise_ ? 0 : 1);
+    for (int i = 0; i < E_; i++) {
+        const int r = h_el_region_sub_[i];
+        if (first[r] < 0) first[r] = i;
+        else if (tabled[r] && !same(i, first[r])) tabled[r] = 0;
+    }
+    // Isotherm::reinit from element i's values (sorption.cc:172-213)
+    auto reinit = [&](int i, int k) {
+        IsoPar I{};
+        I.type = rs_type_[(size_t)k * E_ + i];
+        I.c0 = rs_mult_[(size_t)k * E_ + i];
+        I.c1 = rs_second_[(size_t)k * E_ + i];
+        I.rho = rs_solvent_;
+        I.sa = h_por_[i];
+        I.ss = phi(i) * (1 - h_por_[i] - por_imm(i)) * rs_rock_[i] * rs_molar_[k];
+        if (I.ss == 0.0)
+            throw Error(-3, "Parameter scale_sorbed (phi * (1 - por_m - por_imm) * rock_density * molar_mass) of "
+                            "element " + std::to_string(m_.el_id[i]) + " is equal to zero");
+        I.limit = rs_sol_[k];
+        I.limited = I.limit > 0.0;
+        I.tab = -1;
+        I.len = 0;
+        return I;
+    };
+    std::vector<IsoPar> par;
+    std::vector<double> tab;
+    std::vector<int> el_par(E_);
+    for (int r = 0; r < nreg; r++) {
+        if (!tabled[r] || first[r] < 0) continue;
+        for (int k = 0; k < n; k++) {
+            IsoPar I = reinit(first[r], k);
             // Isotherm::make_table (isotherm.cc:220-242)
-            const int steps = t == 0 ? 1 : substeps;
+            const int steps = I.type == 0 ? 1 : rs_substeps_;
             if (I.limit <= 0.0) I.limit = 1.0;
             const double mass_limit = I.sa * I.limit + I.ss * iso_f(I, I.limit / I.rho);
             I.step = mass_limit / steps;
+            I.tab = (int)tab.size();
+            I.len = steps + 1;
             double mass = 0.0;
             for (int j = 0; j <= steps; j++) {
                 double ca = mass / I.sa, cs = 0.0;
-                iso_solve(I, ca, cs);
-                tab[((size_t)r * n + k) * len + j] = cs * I.sa - ca * I.ss;
+                if (!iso_solve(I, ca, cs))
+                    throw Error(kErrSorption, "the isotherm table of bulk region " + std::to_string(r) +
+                                                  " has no root at total mass " + std::to_string(mass));
+                tab.push_back(cs * I.sa - ca * I.ss);
                 mass = mass + I.step;
             }
-            par[(size_t)r * n + k] = I;
+            par.push_back(I);
         }
-    std::vector<int> reg(h_el_region_sub_.begin(), h_el_region_sub_.end());
-    d_el_region_ = upv(owned_, reg, stream_);
-    d_rs_subst_ = upv(owned_, sub, stream_);
+    }
+    // table isotherms first, [region][species] in region order
+    std::vector<int> reg_base(nreg, -1);
+    for (int r = 0, at = 0; r < nreg; r++)
+        if (tabled[r] && first[r] >= 0) {
+            reg_base[r] = at;
+            at += n;
+        }
+    for (int i = 0; i < E_; i++) {
+        const int r = h_el_region_sub_[i];
+        if (reg_base[r] >= 0) {
+            el_par[i] = reg_base[r];
+        } else {
+            el_par[i] = (int)par.size();
+            for (int k = 0; k < n; k++) {
+                const IsoPar I = reinit(i, k);
+                // toms748_solve refuses an empty bracket whatever the concentration
+                const double critic = I.limit * I.sa + iso_f(I, I.limit / I.rho) * I.ss;
+                if (!(critic / I.sa + 0.00001 > 0.0))
+                    throw Error(kErrSorption, "sorption of species " + std::to_string(k) + " at element " +
+                                                  std::to_string(m_.el_id[i]) +
+                                                  " is solved per element and has an empty root bracket "
+                                                  "(no positive solubility limit)");
+                par.push_back(I);
+            }
+        }
+    }
+    if (tab.empty()) tab.push_back(0.0);
+    d_rs_el_par_ = upv(owned_, el_par, stream_);
+    d_rs_subst_ = upv(owned_, rs_subst_, stream_);
     IsoPar *dp = dev_alloc<IsoPar>(par.size());
     owned_.push_back(dp);
     FL_HIP(hipMemcpyAsync(dp, par.data(), par.size() * sizeof(IsoPar), hipMemcpyHostToDevice, stream_));
     d_rs_par_ = (const double *)dp;
     d_rs_table_ = upv(owned_, tab, stream_);
-    std::vector<double> zero((size_t)E_ * n, 0.0);
-    d_rs_sorbed_ = upv(owned_, zero, stream_);
-    rs_n_ = n;
-    rs_len_ = len;
+    if (!d_rs_sorbed_) {
+        std::vector<double> zero((size_t)E_ * n, 0.0);
+        d_rs_sorbed_ = upv(owned_, zero, stream_);
+    }
+    if (!d_rs_err_) {
+        d_rs_err_ = dev_alloc<int>(2);
+        owned_.push_back(d_rs_err_);
+    }
+    const int init[2] = {0, INT_MAX};
+    FL_HIP(hipMemcpyAsync(d_rs_err_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
     FL_HIP(hipStreamSynchronize(stream_));
+    rs_built_ = true;
+}
+
+// The reference stops at a negative total mass (xprintf(UsrErr), isotherm.cc:62)
+// and where toms748_solve finds no bracketed root (it throws); the steps already
+// queued have run, so the failure is reported when the update returns.
+void TransportGPU::check_sorption() {
+    int e[2] = {0, INT_MAX};
+    FL_HIP(hipMemcpyAsync(e, d_rs_err_, sizeof(e), hipMemcpyDeviceToHost, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    if (e[0] == 0) return;
+    const int init[2] = {0, INT_MAX};
+    FL_HIP(hipMemcpyAsync(d_rs_err_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
+    FL_HIP(hipStreamSynchronize(stream_));
+    std::string what = (e[0] & kSorpNegMass) ? "a negative total mass" : "";
+    if (e[0] & kSorpNoRoot) what += std::string(what.empty() ? "" : " and ") + "an isotherm equation without a root in its bracket";
+    throw Error(kErrSorption, "sorption: " + what + " at element " + std::to_string(m_.el_id[e[1]]) +
+                                  " (the first failing element; its concentrations were left as they were)");
 }
 
 void TransportGPU::get_reaction_sorbed(double *out) const {
     const size_t n = (size_t)E_ * rs_n_;
-    std::vector<double> c(n);
-    if (n) {
+    std::vector<double> c(n, 0.0);
+    if (n && d_rs_sorbed_) {
         FL_HIP(hipMemcpyAsync(c.data(), d_rs_sorbed_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
     }

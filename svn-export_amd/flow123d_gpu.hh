@@ -53,6 +53,16 @@ class GpuMesh {
     unsigned n_bc_elements() const { return counts()[2]; }
     unsigned n_sides() const { return counts()[3]; }
     unsigned n_edges() const { return counts()[4]; }
+    // bulk regions (RegionDB::bulk_size): the regions without the boundary flag
+    unsigned n_bulk_regions() const {
+        const int nr = counts()[7];
+        std::vector<int> ids(nr), bnd(nr), sub(nr);
+        std::vector<char> buf(64 * (size_t)(nr + 1));
+        check(fl_mesh_regions(m_, ids.data(), bnd.data(), sub.data(), buf.data(), (int)buf.size()), "regions");
+        unsigned n = 0;
+        for (int b : bnd) n += b == 0;
+        return n;
+    }
 
    private:
     fl_mesh *m_ = nullptr;
@@ -252,7 +262,7 @@ class ConvectionTransport {
     ConvectionTransport(GpuMesh &mesh, int n_subst, const std::vector<double> &por_m,
                         const std::vector<double> &cross_section, const std::vector<double> &bc_conc,
                         const std::vector<double> &init_conc, double end_time, double save_step)
-        : n_subst_(n_subst), n_elem_(mesh.n_elements()) {
+        : n_subst_(n_subst), n_elem_(mesh.n_elements()), n_bulk_(mesh.n_bulk_regions()) {
         check(fl_transport_create(mesh.handle(), n_subst, por_m.data(), cross_section.data(), bc_conc.data(),
                                   init_conc.data(), end_time, save_step, &t_),
               "transport create");
@@ -303,13 +313,41 @@ class ConvectionTransport {
                                const std::vector<double> &rock_density, const std::vector<double> &molar_mass,
                                const std::vector<double> &solubility, double solvent_dens = 1.0,
                                int substeps = 1000) {
+        const size_t n = subst.size(), nr = n_bulk_;
+        if (type.size() != nr * n || mult_coef.size() != nr * n || second_param.size() != nr * n ||
+            rock_density.size() != nr || molar_mass.size() != n || (!solubility.empty() && solubility.size() != n))
+            throw std::invalid_argument("set_reaction_sorption: type, mult_coef, second_param need n_bulk_regions x "
+                                        "species values, rock_density n_bulk_regions, molar_mass and solubility "
+                                        "(or empty) one per species");
         check(fl_transport_set_reaction_sorption(t_, (int)subst.size(), subst.data(), type.data(), mult_coef.data(),
                                                  second_param.data(), rock_density.data(), molar_mass.data(),
                                                  solubility.empty() ? nullptr : solubility.data(), solvent_dens,
                                                  substeps),
               "reaction sorption");
     }
+    // the same with element-wise parameter fields: type, mult_coef, second_param [n][E],
+    // rock_density [E]; every element's isotherm solved directly (sorption.cc:236-260)
+    void set_reaction_sorption_elementwise(const std::vector<int> &subst, const std::vector<int> &type,
+                                           const std::vector<double> &mult_coef,
+                                           const std::vector<double> &second_param,
+                                           const std::vector<double> &rock_density,
+                                           const std::vector<double> &molar_mass,
+                                           const std::vector<double> &solubility, double solvent_dens = 1.0) {
+        const size_t n = subst.size(), ne = n_elem_;
+        if (type.size() != n * ne || mult_coef.size() != n * ne || second_param.size() != n * ne ||
+            rock_density.size() != ne || molar_mass.size() != n || (!solubility.empty() && solubility.size() != n))
+            throw std::invalid_argument("set_reaction_sorption_elementwise: type, mult_coef, second_param need "
+                                        "species x elements values, rock_density one per element, molar_mass and "
+                                        "solubility (or empty) one per species");
+        check(fl_transport_set_reaction_sorption_elementwise(t_, (int)n, subst.data(), type.data(), mult_coef.data(),
+                                                             second_param.data(), rock_density.data(),
+                                                             molar_mass.data(),
+                                                             solubility.empty() ? nullptr : solubility.data(),
+                                                             solvent_dens),
+              "reaction sorption");
+    }
     // bc_conc as a function of time: fn(t, bc_conc[n_subst][NBC], user) at every step's start
+    // (a non-zero return stops the update before that step)
     void set_bc_conc_function(fl_bc_conc_fn fn, void *user) {
         check(fl_transport_set_bc_conc_function(t_, fn, user), "bc function");
     }
@@ -347,7 +385,7 @@ class ConvectionTransport {
    private:
     fl_transport *t_ = nullptr;
     int n_subst_;
-    unsigned n_elem_;
+    unsigned n_elem_, n_bulk_;
 };
 
 }  // namespace fl

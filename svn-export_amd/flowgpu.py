@@ -47,7 +47,9 @@ class BatchInfo(C.Structure):
                 ("ms_total", C.c_double), ("ms_op_avg", C.c_double), ("ms_update_avg", C.c_double),
                 ("grid", C.c_int), ("xcd_map", C.c_int), ("chunk", C.c_int), ("compactions", C.c_int),
                 ("handovers", C.c_int), ("handover_iterations", C.c_longlong), ("ms_handover", C.c_double),
-                ("ms_coarse_avg", C.c_double), ("ms_coarse_setup", C.c_double)]
+                ("ms_coarse_avg", C.c_double), ("ms_coarse_setup", C.c_double), ("ms_queue_empty", C.c_double),
+                ("ms_narrow16", C.c_double), ("ms_narrow4", C.c_double), ("ms_loop_end", C.c_double),
+                ("ms_handover_end", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -88,6 +90,9 @@ SIGNATURES = {
     "fl_transport_set_bc_conc_function": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fl_transport_set_reaction_sorption": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                                      C.c_void_p, C.c_double, C.c_int]),
+    "fl_device_stream_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _f64p]),
+    "fl_transport_set_reaction_sorption_elementwise": (C.c_int, [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, _f64p,
+                                                                 _f64p, _f64p, C.c_void_p, C.c_double]),
     "fl_transport_get_reaction_sorbed": (C.c_int, [C.c_void_p, _f64p]),
     "fl_msh_write_node_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,
                                          C.c_int, _f64p, C.c_int]),
@@ -194,6 +199,8 @@ NOT_CONVERGED = 1
 E_SINGULAR_ELEMENT = -10
 E_BAD_FIELD = -11
 E_DIVERGED = -12
+E_CALLBACK = -13
+E_SORPTION = -14
 
 
 def _check(rc):
@@ -204,6 +211,14 @@ def _check(rc):
         err.status = rc
         raise err
     return rc
+
+
+def stream_bandwidth(nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth (GB/s) of the current device: copy (read + write
+    counted), read only and write only over buffers of nbytes each."""
+    g = np.zeros(3)
+    _check(lib().fl_device_stream_bandwidth(int(nbytes), int(reps), g))
+    return {"copy": float(g[0]), "read": float(g[1]), "write": float(g[2])}
 
 
 def _f64(a, n=None):
@@ -728,7 +743,7 @@ class LearnedCoarseBasis:
             self.h = None
 
 
-BC_CONC_FN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+BC_CONC_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 
 class ConvectionTransport(_TimeCoupling):
@@ -762,10 +777,16 @@ class ConvectionTransport(_TimeCoupling):
         """Convection up to target time t; without t the operator-splitting
         step of its own governor (TransportOperatorSplitting::update_solution)."""
         s = C.c_int()
-        if t is None:
-            _check(lib().fl_transport_update_solution_step(self.h, C.byref(s)))
-        else:
-            _check(lib().fl_transport_update_solution(self.h, t, C.byref(s)))
+        self._bc_exc = None
+        try:
+            if t is None:
+                _check(lib().fl_transport_update_solution_step(self.h, C.byref(s)))
+            else:
+                _check(lib().fl_transport_update_solution(self.h, t, C.byref(s)))
+        except FlowError as err:
+            if self._bc_exc is not None:
+                raise self._bc_exc from err
+            raise
         return s.value
 
     def times(self):
@@ -867,8 +888,16 @@ class ConvectionTransport(_TimeCoupling):
             return
 
         def cb(t, out, user):
-            np.ctypeslib.as_array(out, shape=shape)[:] = fn(t)
+            # an exception cannot cross the C frames: keep it, stop the update
+            # (non-zero return) and re-raise it when the native call returns
+            try:
+                np.ctypeslib.as_array(out, shape=shape)[:] = fn(t)
+                return 0
+            except BaseException as e:  # noqa: B902
+                self._bc_exc = e
+                return 1
 
+        self._bc_exc = None
         self._bc_fn = BC_CONC_FN(cb)
         _check(lib().fl_transport_set_bc_conc_function(self.h, C.cast(self._bc_fn, C.c_void_p), None))
 
@@ -889,6 +918,22 @@ class ConvectionTransport(_TimeCoupling):
         sol = None if self._rs[6] is None else self._rs[6].ctypes.data
         _check(lib().fl_transport_set_reaction_sorption(self.h, n, *self._rs[:6], sol, float(solvent_dens),
                                                         int(substeps)))
+
+    def set_reaction_sorption_elementwise(self, subst, types, mult, second, rock_density, molar_mass,
+                                          solubility=None, solvent_dens=1.0):
+        """The same with element-wise parameter fields: types, mult, second [n][E];
+        rock_density [E]; every element's isotherm is solved directly."""
+        n, E = len(subst), self.mesh.n_elements
+        self._rs = [np.ascontiguousarray(subst, np.int32),
+                    np.ascontiguousarray(np.broadcast_to(types, (n, E)), np.int32).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(mult, (n, E)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(second, (n, E)), np.float64).reshape(-1),
+                    np.ascontiguousarray(np.broadcast_to(rock_density, (E,)), np.float64),
+                    np.ascontiguousarray(np.broadcast_to(molar_mass, (n,)), np.float64),
+                    None if solubility is None else np.ascontiguousarray(np.broadcast_to(solubility, (n,)), np.float64)]
+        sol = None if self._rs[6] is None else self._rs[6].ctypes.data
+        _check(lib().fl_transport_set_reaction_sorption_elementwise(self.h, n, *self._rs[:6], sol,
+                                                                    float(solvent_dens)))
 
     def reaction_sorbed(self):
         out = np.zeros(len(self._rs[0]) * self.mesh.n_elements)

@@ -69,4 +69,8 @@ def test_status_codes_match_header():
     txt = open(os.path.join(ROOT, "include", "flow123d_gpu.h")).read()
     defs = {k: int(v) for k, v in re.findall(r"#define FL_(\w+) \(?(-?\d+)\)?", txt)}
     assert defs == {"NOT_CONVERGED": G.NOT_CONVERGED, "E_SINGULAR_ELEMENT": G.E_SINGULAR_ELEMENT,
-                    "E_BAD_FIELD": G.E_BAD_FIELD, "E_DIVERGED": G.E_DIVERGED}
+                    "E_BAD_FIELD": G.E_BAD_FIELD, "E_DIVERGED": G.E_DIVERGED, "E_CALLBACK": G.E_CALLBACK,
+                    "E_SORPTION": G.E_SORPTION}
+    # and the library's own enumeration
+    src = open(os.path.join(ROOT, "svn-export_amd", "csrc", "common.hh")).read()
+    assert {int(v) for v in re.findall(r"kErr\w+ = (-\d+)", src)} == {v for v in defs.values() if v < 0}
