@@ -221,7 +221,11 @@ def main():
         del bt
         net = G.LearnedCoarseBasis(batch, seed=1)
         t1 = time.perf_counter()
-        losses = net.train(Kt.data_ptr(), lam.data_ptr(), NT, args.train_epochs, 8, args.train_lr)
+        # one epoch per call (the optimizer state carries over), a line per epoch
+        losses = []
+        for ep in range(args.train_epochs):
+            losses.append(float(net.train(Kt.data_ptr(), lam.data_ptr(), NT, 1, 8, args.train_lr)[0]))
+            log("  epoch %d: loss %.5f (%.1f s)" % (ep + 1, losses[-1], time.perf_counter() - t1))
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t1
         training = {"pairs": NT, "epochs": args.train_epochs, "batch": 8, "lr": args.train_lr,

@@ -866,7 +866,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
 // [s G / kPre, (s + 1) G / kPre) of each given array in a fixed order, so the
 // per-slot fold reads kPre rows instead of G (a G-row chain on one CU per
 // chunk is latency-bound).
-constexpr int kPre = 32;
+constexpr int kPre = 128;
 __global__ void __launch_bounds__(256) k_b_prefold(const double *__restrict__ p0, const double *__restrict__ p1,
                                                    const double *__restrict__ p2, int G, int Rp,
                                                    double *__restrict__ o0, double *__restrict__ o1,
