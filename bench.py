@@ -154,7 +154,9 @@ def main():
                     help="two-level preconditioner columns per aggregate: 0 Jacobi, 1 constant, 2 constant + learned")
     ap.add_argument("--train-realizations", type=int, default=2000,
                     help="solver-generated (K, lambda) pairs the network is trained on")
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-epochs", type=int, default=16)
+    ap.add_argument("--train-sigma", type=float, nargs=2, default=(0.5, 3.0),
+                    help="sigma range of the training realizations (the timed ones span 0.5-3)")
     ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--handover", type=int, default=-1,
                     help="live realizations handed to the single-solve path (-1: library default)")
@@ -209,7 +211,7 @@ def main():
         # learns the shape of lambda per aggregate, not its last digits)
         t_tr = time.perf_counter()
         NT = args.train_realizations
-        Kt, _, _ = mc_fields(torch, cen, scale, NT, seed=99)
+        Kt, _, _ = mc_fields(torch, cen, scale, NT, seed=99, sigma=tuple(args.train_sigma))
         bt = G.DarcyBatch(darcy, NT, slots=args.slots)
         bt.set_conductivity(Kt.data_ptr())
         bt.set_coarse(1)
@@ -230,6 +232,7 @@ def main():
         t_train = time.perf_counter() - t1
         training = {"pairs": NT, "epochs": args.train_epochs, "batch": 8, "lr": args.train_lr,
                     "pair_solver": "constant-coarse FP32 batch CG to rel residual 1e-8 (natural norm), seed 99",
+                    "sigma": list(args.train_sigma),
                     "loss_first_epoch": float(losses[0]),
                     "loss_last_epoch": float(losses[-1]), "data_s": round(t_data, 2), "train_s": round(t_train, 2),
                     "params": net.n_params}

@@ -1333,9 +1333,9 @@ constexpr int kSlotGroup = 32;  // slots per LDS pass (128 B per staged row)
 constexpr int kStageMax = 7;  // 16-byte staging loads per thread (512) and slot group (nseg_max <= 448)
 // the staging buffer holds all kStageMax loads of every thread, so the loads
 // and stores need no bounds test (a test per load would serialize them)
+constexpr int kRecWords = 16;  // one row's record in LDS: 5 words of packed indices, 10 values, 1 pad
 __host__ __device__ inline size_t opl_lds_bytes(int nseg_max, int T) {
-    return (size_t)kStageMax * 512 * 16 + (((size_t)T * kLrec * 2 + 15) / 16) * 16 + (size_t)T * kLrec * 4 +
-           8 * kSlotGroup * 8 + (size_t)nseg_max * 8;
+    return (size_t)kStageMax * 512 * 16 + (size_t)T * kRecWords * 4 + 8 * kSlotGroup * 8 + (size_t)nseg_max * 8;
 }
 
 // FP32 operator q = S p of the two-level path, in two kernels that run
@@ -1537,9 +1537,10 @@ __global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTile
         return;
     }
     float4 *buf = reinterpret_cast<float4 *>(lds_d);  // [kStageMax * kOplThreads] float4
-    unsigned short *lr = reinterpret_cast<unsigned short *>(buf + kStageMax * kOplThreads);
-    float *lf = reinterpret_cast<float *>(lr + (((size_t)ot.T * kLrec * 2 + 15) / 16) * 8);
-    double *red = reinterpret_cast<double *>(lf + (size_t)ot.T * kLrec);  // [8 waves][32 slots]
+    // per row 64 bytes: the 10 local indices as 16-bit pairs in words 0-4, the
+    // 10 values in words 5-14, read with four 16-byte LDS loads
+    unsigned *lrw = reinterpret_cast<unsigned *>(buf + kStageMax * kOplThreads);
+    double *red = reinterpret_cast<double *>(lrw + (size_t)ot.T * kRecWords);  // [8 waves][32 slots]
     const float **segp = reinterpret_cast<const float **>(red + 8 * kSlotGroup);  // row of each segment
     {
         // tile set-up; indices past the end are clamped to valid ones and their
@@ -1568,8 +1569,9 @@ __global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTile
         for (int u = 0; u < kRecLd; u++) {
             const int w = tid + u * kOplThreads;
             if (w < nr * kLrec) {
-                lr[w] = rl[u];
-                lf[w] = __int_as_float(rf[u]);
+                const int rr = w / kLrec, k = w - rr * kLrec;
+                reinterpret_cast<unsigned short *>(lrw + rr * kRecWords)[k] = rl[u];
+                lrw[rr * kRecWords + 5 + k] = (unsigned)rf[u];
             }
         }
 #pragma unroll
@@ -1604,9 +1606,17 @@ __global__ void __launch_bounds__(kOplThreads, 4) k_c_opl(BArgs<float> a, OpTile
         if (sg + 1 < nsg) opl_stage(segp, nseg, tid, sb + kSlotGroup, vv);
         double v[4] = {0.0, 0.0, 0.0, 0.0};
         for (int r = rsub; r < nr; r += kOplThreads / 8) {
-            const unsigned short *R = lr + r * kLrec;
-            if (R[0] == 0xFFFF) continue;
-            const float *Fv = lf + r * kLrec;
+            const uint4 *rq = reinterpret_cast<const uint4 *>(lrw + r * kRecWords);
+            const uint4 q0 = rq[0], q1 = rq[1], q2 = rq[2], q3 = rq[3];
+            if ((q0.x & 0xFFFFu) == 0xFFFFu) continue;
+            const unsigned pk[5] = {q0.x, q0.y, q0.z, q0.w, q1.x};
+            int R[kLrec];
+#pragma unroll
+            for (int k = 0; k < kLrec; k++) R[k] = (int)((pk[k >> 1] >> (16 * (k & 1))) & 0xFFFFu);
+            const float Fv[kLrec] = {__uint_as_float(q1.y), __uint_as_float(q1.z), __uint_as_float(q1.w),
+                                     __uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z),
+                                     __uint_as_float(q2.w), __uint_as_float(q3.x), __uint_as_float(q3.y),
+                                     __uint_as_float(q3.z)};
             const float4 xr = buf[r * 8 + quad];
             const float4 k1 = buf[(eo + R[0]) * 8 + quad], k2 = buf[(eo + R[1]) * 8 + quad];
             float4 xa[4], xb[4];
