@@ -986,10 +986,11 @@ __global__ void k_b_extract(size_t n, size_t Rp, int r, const double *__restrict
 // (constant) or two (constant, phi) columns per aggregate.  Aggregates are
 // contiguous row ranges (cells of the row ordering's Z-curve), so the
 // restriction Z^T r is a block reduction of the update kernel, and z is never
-// stored: the operator forms z = D^-1 r + Z y on the fly from r and the
-// coarse correction y.  The conjugate-gradient step uses the flexible
-// (Polak-Ribiere) beta = z_new . (r_new - r_old) / z_old . r_old, which stays
-// robust when the coarse inverse is applied in FP32.
+// stored: the direction kernel forms z = D^-1 r + Z y on the fly from r and
+// the coarse correction y.  M is one fixed SPD operator per realization (the
+// FP32 coarse inverse is symmetric and every kernel reads the same stored
+// values), so the conjugate-gradient step uses the Fletcher-Reeves beta
+// z_new . r_new / z_old . r_old.
 // ---------------------------------------------------------------------------
 constexpr int kNbMax = 32;  // neighbour aggregates per aggregate (27 on a regular Z-curve level)
 
