@@ -397,9 +397,11 @@ const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
 /* Once the queue is empty and at most n realizations are still iterating,
- * finish them one by one with the single-solve FP64 CG (warm start from the
- * batch iterate); 0 disables; negative restores the default: 16, or 4 when the
- * two-level tail runs narrowed (fl_darcy_batch_set_narrow). */
+ * finish them with the single-solve FP64 CG (warm start from the batch
+ * iterate), up to 4 side by side on solvers and streams of their own (copies
+ * of the base solver on the same mesh, made at the first hand-over and kept
+ * by the batch handle); 0 disables; negative restores the default: 16, or 8
+ * when the two-level tail runs narrowed (fl_darcy_batch_set_narrow). */
 int fl_darcy_batch_set_handover(fl_darcy_batch *b, int n);
 /* Two-level preconditioner only: once the queue is empty and at most 16 (then
  * 4) realizations are still iterating, they continue in the batch with 16 (4)

@@ -28,6 +28,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "darcy.hh"
@@ -103,7 +104,7 @@ class DarcyBatchGPU {
     void set_launch(int grid, int xcd_map, int chunk);
     void autotune();
     // live-realization count at which the rest go to the single-solve path
-    // (0: never; negative: 16, or 4 when the two-level tail runs narrowed)
+    // (0: never; negative: 16, or 8 when the two-level tail runs narrowed)
     void set_handover(int n) { handover_ = n < 0 ? -1 : n; }
     // Two-level path: once the queue is empty and at most 16 (then 4)
     // realizations are live, continue them in the batch with 16 (4) lanes per
@@ -133,6 +134,10 @@ class DarcyBatchGPU {
     void set_pairs(const std::vector<int> &pairs);
 
     DarcyGPU &base_;
+    // single-solve copies of base_ for the concurrent hand-over workers (worker
+    // w > 0 uses hand_solvers_[w - 1]; made by the worker at its first job)
+    static constexpr int kHandWorkers = 4;
+    std::vector<std::unique_ptr<DarcyGPU>> hand_solvers_;
     const DarcyDev &dv_;
     hipStream_t stream_ = nullptr;
     hipStream_t stream2_ = nullptr;  // the operator's general rows, beside its row tiles

@@ -3099,9 +3099,12 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     // Hand realizations to the single-solve FP64 path, warm-started from the
     // batch iterate: a batched iteration costs a sweep over all rows whatever
     // the number of live lanes, a single solve's iteration touches one
-    // realization only.  The single solves run in a worker thread of their
-    // own on the base solver's stream, so the batch keeps iterating its other
-    // slots meanwhile; their results are merged when the batch is done.
+    // realization only.  The single solves run in up to kHandWorkers worker
+    // threads, each with a solver and stream of its own (the base solver and
+    // copies of it on the same mesh, made by the workers themselves once and
+    // kept), so the batch keeps iterating its other slots meanwhile and the
+    // handed-over solves run side by side; their results are merged when the
+    // batch is done.
     struct HandJob {
         int real, it0, n_ru;
         double bb, tol;
@@ -3119,26 +3122,44 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     std::vector<double *> hfree;  // extraction buffers ready for reuse
     bool hclose = false;
     std::exception_ptr herr;
-    std::thread hworker;
+    std::vector<std::thread> hworkers;
     double hbusy_ms = 0.0;
     int hdev = 0;
     FL_HIP(hipGetDevice(&hdev));
     auto hand_join = [&]() {
-        if (!hworker.joinable()) return;
+        if (hworkers.empty()) return;
         {
             std::lock_guard<std::mutex> lk(hmu);
             hclose = true;
         }
         hcv.notify_all();
-        hworker.join();
+        for (std::thread &t : hworkers)
+            if (t.joinable()) t.join();
+        hworkers.clear();
     };
     struct JoinOnExit {  // the worker never outlives this call, also on an exception
         std::function<void()> f;
         ~JoinOnExit() { f(); }
     } join_on_exit{hand_join};
-    auto hand_worker = [&]() {
+    if (hand_solvers_.size() < kHandWorkers - 1) hand_solvers_.resize(kHandWorkers - 1);
+    auto hand_worker = [&](int w) {
         (void)hipSetDevice(hdev);
         const size_t n = (size_t)S_ + E_ + F_;
+        DarcyGPU *solver = &base_;
+        if (w > 0) {
+            try {
+                std::unique_ptr<DarcyGPU> &h = hand_solvers_[w - 1];  // this worker's slot only
+                if (!h) {
+                    h = std::make_unique<DarcyGPU>(base_.mesh());
+                    h->copy_data_from(base_);
+                }
+                solver = h.get();
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(hmu);
+                if (!herr) herr = std::current_exception();
+                return;
+            }
+        }
         while (true) {
             HandJob j;
             {
@@ -3150,13 +3171,13 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
             }
             try {
                 const auto t0 = std::chrono::steady_clock::now();
-                base_.set_conductivity_device(d_Kall_ + (size_t)j.real * E_);
+                solver->set_conductivity_device(d_Kall_ + (size_t)j.real * E_);
                 const int left = std::max(1, max_it - j.it0);
-                SolveStats ss = base_.solve_warm(j.x0, rtol, atol, left, j.bb, norm);
+                SolveStats ss = solver->solve_warm(j.x0, rtol, atol, left, j.bb, norm);
                 hipLaunchKernelGGL(k_b_put_col, dim3(std::min(grid_for((long)n), 4096)), dim3(kBlock), 0,
-                                   base_.stream(), n, (size_t)Rout_, j.real, base_.device_solution(), d_sol_);
+                                   solver->stream(), n, (size_t)Rout_, j.real, solver->device_solution(), d_sol_);
                 FL_HIP(hipGetLastError());
-                FL_HIP(hipStreamSynchronize(base_.stream()));
+                FL_HIP(hipStreamSynchronize(solver->stream()));
                 const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 std::lock_guard<std::mutex> lk(hmu);
                 hres.push_back(HandResult{j.real, j.it0, j.n_ru, j.bb, j.tol, ss});
@@ -3209,11 +3230,15 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         }
         FL_HIP(hipGetLastError());
         FL_HIP(hipStreamSynchronize(stream_));  // extractions complete before the worker reads them
+        size_t queued = 0;
         {
             std::lock_guard<std::mutex> lk(hmu);
             for (const HandJob &j : js) hjobs.push_back(j);
+            queued = hjobs.size();
         }
-        if (!hworker.joinable()) hworker = std::thread(hand_worker);
+        // one more worker per waiting job, up to kHandWorkers
+        while (hworkers.size() < (size_t)kHandWorkers && hworkers.size() < queued + 1)
+            hworkers.emplace_back(hand_worker, (int)hworkers.size());
         hcv.notify_all();
     };
     // move live slots to the front when that frees whole 64-slot chunks
@@ -3384,7 +3409,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
         if (next >= queue.size() && st.ms_queue_empty == 0) st.ms_queue_empty = since_start();
         if (nrun == 0) break;
         const bool narrow = narrow_ && NC > 0;
-        const int hand = handover_ >= 0 ? handover_ : (narrow ? 4 : 16);
+        const int hand = handover_ >= 0 ? handover_ : (narrow ? 8 : 16);
         if (next >= queue.size() && nrun + nf64 <= hand) {
             std::vector<int> pairs;
             for (int sl = 0; sl < Rp_; sl++)

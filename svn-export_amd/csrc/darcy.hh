@@ -79,6 +79,10 @@ class DarcyGPU {
     // per-element fields [E] (aniso: 9*E or null), per-bc-element BC data [NBC]
     void set_data(const double *K, const double *cs, const double *aniso, const double *sigma, const double *f,
                   const int *bc_type, const double *bc_p, const double *bc_flux, const double *bc_sigma);
+    // the steady data of another handle on the same mesh but its conductivity
+    // (device copies on this handle's stream; `o` is only read), which the
+    // caller sets before the first solve
+    void copy_data_from(const DarcyGPU &o);
     void set_conductivity(const double *K);  // host array [E]
     void set_conductivity_device(const double *dK);  // device array [E]
     const double *device_K() const { return d_K_; }

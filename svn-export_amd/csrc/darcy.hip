@@ -576,6 +576,39 @@ void DarcyGPU::set_data(const double *K, const double *cs, const double *aniso, 
     have_data_ = true;
 }
 
+void DarcyGPU::copy_data_from(const DarcyGPU &o) {
+    if (&o.m_ != &m_) throw Error(-3, "copy_data_from: the handles are on different meshes");
+    if (!o.have_data_) throw Error(-4, "Darcy data not set");
+    if (o.unsteady_) throw Error(-3, "copy_data_from: the source handle is unsteady");
+    const size_t E = m_.n_elements(), NBC = m_.n_bc_elements();
+    hipStream_t s = stream_;
+    // K is not copied: `o` may be changing it; every solve sets its own
+    FL_HIP(hipMemcpyAsync(d_cs_, o.d_cs_, E * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FL_HIP(hipMemcpyAsync(d_sigma_, o.d_sigma_, E * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FL_HIP(hipMemcpyAsync(d_f_, o.d_f_, E * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (o.dv_.aniso) {
+        if (!d_aniso_) {
+            d_aniso_ = dev_alloc<double>(9 * E);
+            owned_.push_back(d_aniso_);
+        }
+        FL_HIP(hipMemcpyAsync(d_aniso_, o.d_aniso_, 9 * E * sizeof(double), hipMemcpyDeviceToDevice, s));
+        dv_.aniso = d_aniso_;
+    } else {
+        dv_.aniso = nullptr;
+    }
+    if (NBC) {
+        FL_HIP(hipMemcpyAsync(d_bc_type_, o.d_bc_type_, NBC * sizeof(int), hipMemcpyDeviceToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_p_, o.d_bc_p_, NBC * sizeof(double), hipMemcpyDeviceToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_flux_, o.d_bc_flux_, NBC * sizeof(double), hipMemcpyDeviceToDevice, s));
+        FL_HIP(hipMemcpyAsync(d_bc_sigma_, o.d_bc_sigma_, NBC * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    FL_HIP(hipStreamSynchronize(s));
+    h_cs_ = o.h_cs_;
+    h_f_ = o.h_f_;
+    h_sigma_ = o.h_sigma_;
+    have_data_ = true;
+}
+
 void DarcyGPU::set_conductivity(const double *K) {
     check_field("conductivity", K, false);
     FL_HIP(hipMemcpyAsync(d_K_, K, m_.n_elements() * sizeof(double), hipMemcpyHostToDevice, stream_));
