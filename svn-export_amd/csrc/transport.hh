@@ -139,6 +139,23 @@ class TransportGPU {
     void one_step();
     void mb_initial();
     void flush_flux();
+    // the launches of one step (shared by one_step and the captured graphs)
+    void launch_step(const double *cold, double *cnew, double dt, double tdiff, const double *plan);
+    // runs of plain steps replayed from captured graphs (run_planned)
+    static constexpr int kGraphSteps[3] = {64, 16, 4};
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        int steps = 0, parity = 0;
+        std::vector<unsigned char> sig;
+    };
+    std::vector<StepGraph> graphs_;
+    double *d_plan_ = nullptr;  // [kGraphSteps[0]][2] step length and time difference of a replayed run
+    double *h_plan_ = nullptr;  // pinned: the planned steps of one update
+    size_t h_plan_cap_ = 0;
+    bool plain_steps_ready() const;
+    std::vector<unsigned char> step_signature(int steps, int parity) const;
+    int run_planned(double target);
+    void free_step_graphs();
     int step_grid() const;
     int step_row_blocks() const;
     static constexpr int kMaxStepGrid = 16384;  // row blocks of the step kernel

@@ -231,7 +231,9 @@ struct PhaseDev {
 // with dual porosity, the immobile phase (linear_reaction.cc:295-301).  Element-
 // wise and HBM-bound: every array is [E][m] so a wave reads whole rows.
 template <int M>
-__global__ void __launch_bounds__(256) k_phases(const TmDev t, const PhaseDev ph, double *__restrict__ c, double dt) {
+__global__ void __launch_bounds__(256) k_phases(const TmDev t, const PhaseDev ph, double *__restrict__ c, double dt,
+                                               const double *__restrict__ plan) {
+    if (plan) dt = plan[0];  // a captured step reads its step length from the plan
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.E; i += gridDim.x * blockDim.x) {
         const double por_m = t.por[i], por_imm = ph.por_imm[i];
         double cm[M], ci[M];
@@ -437,8 +439,13 @@ __device__ __forceinline__ void load_row(const double *__restrict__ p, double (&
 template <int M>
 __global__ void __launch_bounds__(256) k_step_c(const TmDev t, const double *__restrict__ c_old,
                                                double *__restrict__ c_new, double dt, double tdiff,
-                                               double *__restrict__ acc_buf, int row_blocks) {
+                                               double *__restrict__ acc_buf, int row_blocks,
+                                               const double *__restrict__ plan) {
     __shared__ double sh[2 * M][4];
+    if (plan) {  // a captured step reads its step length and time difference from the plan
+        dt = plan[0];
+        tdiff = plan[1];
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if ((int)blockIdx.x >= row_blocks) {
         // boundary sides: inflow carries the boundary concentration, outflow the
@@ -710,6 +717,8 @@ TransportGPU::TransportGPU(const HostMesh &m, int nsub, const double *por, const
 
 TransportGPU::~TransportGPU() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    free_step_graphs();
+    if (h_plan_) (void)hipHostFree(h_plan_);
     for (void *p : owned_) dev_free(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1023,8 +1032,19 @@ void TransportGPU::one_step() {
     last_time_ = tg_.time;
     const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
     double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
+    if (rs_n_ && !rs_built_) build_reaction_sorption();
+    launch_step(cold, cnew, tg_.time_step, tdiff, nullptr);
+    cur_is_c0_ = !cur_is_c0_;
+    stats_.steps++;
+}
+
+// The kernels of one step from cold to cnew: the convection step (with the
+// decays unless the phases run), dual porosity / sorption, the reaction
+// module's sorption.  dt and tdiff as arguments, or from plan[0..1] on the
+// device when plan is given (a step captured into a graph).
+void TransportGPU::launch_step(const double *cold, double *cnew, double dt, double tdiff, const double *plan) {
+    const int gridS = std::min(grid_for(E_), 4096);
     const int G = step_grid();
-    const double dt = tg_.time_step;
     // with dual porosity / sorption the decays run after them, in k_phases
     TmDev ts = t_;
     if (phases_on()) ts.has_react = 0;
@@ -1032,7 +1052,7 @@ void TransportGPU::one_step() {
 #define FL_STEP(MM) \
     case MM:                                                                                                  \
         hipLaunchKernelGGL(k_step_c<MM>, dim3(G), dim3(kBlock), 0, stream_, ts, cold, cnew, dt, tdiff, d_acc_,  \
-                           step_row_blocks());                                                                 \
+                           step_row_blocks(), plan);                                                           \
         break;
         FL_STEP(1) FL_STEP(2) FL_STEP(3) FL_STEP(4) FL_STEP(5) FL_STEP(6) FL_STEP(7) FL_STEP(8)
 #undef FL_STEP
@@ -1045,14 +1065,13 @@ void TransportGPU::one_step() {
                     dual_ ? 1 : 0, sorp_ ? 1 : 0};
         switch (nsub_) {
 #define FL_PH(MM) \
-    case MM: hipLaunchKernelGGL(k_phases<MM>, dim3(gridS), dim3(kBlock), 0, stream_, t_, ph, cnew, dt); break;
+    case MM: hipLaunchKernelGGL(k_phases<MM>, dim3(gridS), dim3(kBlock), 0, stream_, t_, ph, cnew, dt, plan); break;
             FL_PH(1) FL_PH(2) FL_PH(3) FL_PH(4) FL_PH(5) FL_PH(6) FL_PH(7) FL_PH(8)
 #undef FL_PH
         }
         FL_HIP(hipGetLastError());
     }
     if (rs_n_) {
-        if (!rs_built_) build_reaction_sorption();
         switch (nsub_) {
 #define FL_RS(MM)                                                                                                 \
     case MM:                                                                                                      \
@@ -1064,8 +1083,6 @@ void TransportGPU::one_step() {
         }
         FL_HIP(hipGetLastError());
     }
-    cur_is_c0_ = !cur_is_c0_;
-    stats_.steps++;
 }
 
 // TransportOperatorSplitting::update_solution (transport_operator_splitting.cc:244-287)
@@ -1091,6 +1108,12 @@ int TransportGPU::update_solution(double target) {
     FL_HIP(hipEventRecord(ev.a, stream_));
     int steps = 0;
     while (tg_.lt(target)) {
+        if (plain_steps_ready()) {
+            // every further step to the target is the same kernel sequence:
+            // plan them on a copy of the governor and replay captured graphs
+            steps += run_planned(target);
+            continue;
+        }
         double t_before = tg_.time;
         one_step();
         steps++;
@@ -1105,6 +1128,130 @@ int TransportGPU::update_solution(double target) {
     stats_.ms_steps += ms;
     if (rs_built_) check_sorption();
     return steps;
+}
+
+// Steps whose launches do not depend on host work between them: no boundary
+// data to apply, the matrix built and scaled, the sorption isotherms made.
+bool TransportGPU::plain_steps_ready() const {
+    return !bc_fn_ && !bc_pending_ && matrix_built_ && setup_done_ && !need_rescale_ && (!rs_n_ || rs_built_);
+}
+
+// What a captured step's launches depend on besides the plan: the kernel
+// argument structures, buffers and switches.  A graph is replayed only while
+// this is unchanged.
+std::vector<unsigned char> TransportGPU::step_signature(int steps, int parity) const {
+    std::vector<unsigned char> sig;
+    auto put = [&](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        sig.insert(sig.end(), c, c + n);
+    };
+    put(&t_, sizeof(t_));
+    const void *ptrs[] = {d_c0_, d_c1_, d_acc_, d_plan_, d_imm_, d_msorb_, d_isorb_, d_alpha_, d_sc0_, d_sc1_,
+                          d_por_imm_, d_phi_, d_stype_, d_rs_subst_, d_rs_el_par_, d_rs_par_, d_rs_table_,
+                          d_rs_sorbed_, d_rs_err_};
+    put(ptrs, sizeof(ptrs));
+    const int ints[] = {steps, parity, nsub_, rs_n_, dual_ ? 1 : 0, sorp_ ? 1 : 0, E_};
+    put(ints, sizeof(ints));
+    return sig;
+}
+
+void TransportGPU::free_step_graphs() {
+    for (StepGraph &g : graphs_)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
+// The remaining steps to `target`, all plain: their step lengths and time
+// differences come from a copy of the governor advanced as one_step would,
+// into pinned memory; runs of 64, 16 and 4 steps replay captured graphs (one
+// per run length and starting buffer, each preceded by the copy of its
+// plan), the rest launch step by step.  Results are those of one_step: the
+// same launches with the same values.
+int TransportGPU::run_planned(double target) {
+    TimeGovernorR g = tg_;
+    double last = last_time_;
+    std::vector<double> plan;
+    while (g.lt(target)) {
+        const double t_before = g.time;
+        g.next_time();
+        if (!(g.time > t_before) || !std::isfinite(g.time))
+            throw Error(-7, "transport time step does not advance (dt = " + std::to_string(g.time_step) + ")");
+        plan.push_back(g.time_step);
+        plan.push_back(g.time - last);
+        last = g.time;
+    }
+    const int n = (int)plan.size() / 2;
+    if (n == 0) return 0;
+    if (plan.size() > h_plan_cap_) {  // the previous update's copies have completed (it synchronized)
+        if (h_plan_) (void)hipHostFree(h_plan_);
+        h_plan_ = nullptr;
+        h_plan_cap_ = 0;
+        FL_HIP(hipHostMalloc((void **)&h_plan_, plan.size() * sizeof(double), hipHostMallocDefault));
+        h_plan_cap_ = plan.size();
+    }
+    std::copy(plan.begin(), plan.end(), h_plan_);
+    if (!d_plan_) {
+        d_plan_ = dev_alloc<double>(2 * kGraphSteps[0]);
+        owned_.push_back(d_plan_);
+    }
+    int k = 0;
+    while (k < n) {
+        int S = 0;
+        for (int len : kGraphSteps)
+            if (n - k >= len) {
+                S = len;
+                break;
+            }
+        if (S == 0) {  // the last few steps one by one
+            const double *cold = cur_is_c0_ ? d_c0_ : d_c1_;
+            double *cnew = cur_is_c0_ ? d_c1_ : d_c0_;
+            launch_step(cold, cnew, h_plan_[2 * k], h_plan_[2 * k + 1], nullptr);
+            cur_is_c0_ = !cur_is_c0_;
+            k++;
+            continue;
+        }
+        const int parity = cur_is_c0_ ? 0 : 1;
+        std::vector<unsigned char> sig = step_signature(S, parity);
+        StepGraph *gr = nullptr;
+        for (StepGraph &x : graphs_)
+            if (x.steps == S && x.parity == parity) gr = &x;
+        if (gr && gr->sig != sig) {  // launch arguments changed since the capture
+            (void)hipGraphExecDestroy(gr->exec);
+            gr->exec = nullptr;
+        }
+        if (!gr) {
+            graphs_.push_back(StepGraph{});
+            gr = &graphs_.back();
+            gr->steps = S;
+            gr->parity = parity;
+        }
+        if (!gr->exec) {
+            hipGraph_t graph = nullptr;
+            FL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            bool c0 = cur_is_c0_;
+            for (int j = 0; j < S; j++) {
+                launch_step(c0 ? d_c0_ : d_c1_, c0 ? d_c1_ : d_c0_, 0.0, 0.0, d_plan_ + 2 * j);
+                c0 = !c0;
+            }
+            FL_HIP(hipStreamEndCapture(stream_, &graph));
+            const hipError_t e = hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                gr->exec = nullptr;
+                throw Error(-2, std::string("transport step graph: ") + hipGetErrorString(e));
+            }
+            gr->sig = sig;
+        }
+        FL_HIP(hipMemcpyAsync(d_plan_, h_plan_ + 2 * k, 2 * S * sizeof(double), hipMemcpyHostToDevice, stream_));
+        FL_HIP(hipGraphLaunch(gr->exec, stream_));
+        if (S % 2) cur_is_c0_ = !cur_is_c0_;
+        k += S;
+    }
+    tg_ = g;
+    last_time_ = last;
+    stats_.steps += n;
+    acc_dirty_ = true;
+    return n;
 }
 
 int TransportGPU::update_solution_step() {
