@@ -433,6 +433,7 @@ def main():
             "compare_batch_forms": compare,
             "iterations": {"mean": float(it.mean()), "min": int(it.min()), "max": int(it.max())},
             "converged": int(info["n_converged"]), "handovers": int(info["handovers"]),
+            "operator_tile_rows": int(info["tile_rows"]),
             "phases_s": {k: round(info[m] * 1e-3, 3) for k, m in (
                 ("queue_empty", "ms_queue_empty"), ("narrowed_16", "ms_narrow16"), ("narrowed_4", "ms_narrow4"),
                 ("batch_loop_end", "ms_loop_end"), ("handover_end", "ms_handover_end"), ("total", "ms_total"))},

@@ -373,6 +373,7 @@ typedef struct {
      * two-level kernels narrowed to 16 / 4 lanes per row (0: never), the batch
      * loop ended (hand-over issued), the hand-over path finished */
     double ms_queue_empty, ms_narrow16, ms_narrow4, ms_loop_end, ms_handover_end;
+    int tile_rows; /* two-level FP32 operator: rows per tile of the tiling the tuner kept */
 } fl_batch_info;
 
 /* Solve every realization to ||b - S x|| <= max(r_tol ||b||, a_tol) (opts as

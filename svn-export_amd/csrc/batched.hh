@@ -50,6 +50,7 @@ struct BatchStats {
     double ms_op_avg = 0, ms_update_avg = 0;  // live event samples of the two main kernels
     int op_samples = 0;
     int grid = 0, xcd_map = 0, chunk = 0;     // launch shape used (autotuned)
+    int tile_rows = 0;                        // two-level FP32 operator: rows per tile (autotuned)
     std::vector<int> iterations, converged;  // per realization
     std::vector<double> residual;             // ||b - S x|| / ||b|| (FP64, true residual at the end)
 };
@@ -167,6 +168,17 @@ class DarcyBatchGPU {
     bool narrow_ = true;        // two-level path: narrow lanes for the tail instead of the hand-over
     // tiled FP32 operator: general rows (vb-coupled elements) and row tiles
     void build_op_tiles(const std::vector<int> &rec);
+    // candidate row tilings of the FP32 operator; the one in use is copied to
+    // d_thptr_ .. nseg_max_ below
+    struct OplTiling {
+        const int *hptr = nullptr, *halo = nullptr, *eptr = nullptr, *els = nullptr;
+        const unsigned short *lrec = nullptr;
+        int T = 0, ntiles = 0, nseg_max = 0;
+    };
+    std::vector<OplTiling> tilings_;
+    int tiling_ = 0;
+    bool build_tiling(const std::vector<int> &rec, int tmax);
+    void use_tiling(int i);
     const int *d_gen_ = nullptr, *d_grec_ = nullptr, *d_thptr_ = nullptr, *d_thalo_ = nullptr, *d_teptr_ = nullptr, *d_tels_ = nullptr;
     const unsigned short *d_lrec_ = nullptr;
     int ngen_ = 0, gen_blocks_ = 0, ntiles_ = 0, tile_T_ = 128, nseg_max_ = 0;

@@ -31,7 +31,7 @@ namespace {
 constexpr int kWaves = 4;            // waves per block of the row kernels
 constexpr int kAggWaves = 8;         // waves per block of the aggregate-blocked kernels (one aggregate per block)
 constexpr int kFoldThreads = 1024;   // 16 waves per fold block
-constexpr int kMaxGrid = 8192;       // partial arrays are sized for this many blocks
+constexpr int kMaxGrid = 16384;      // partial arrays are sized for this many blocks
 constexpr int kMaxLaunchGrid = 2048; // blocks of the row-strided kernels
 constexpr int kMaxConn = 16;         // element index packing e * 16 + local index
 
@@ -2070,17 +2070,15 @@ void DarcyBatchGPU::free_graphs() {
 // its halo rows, its elements and the local-index records of its fast rows.
 // T = 128 rows, halved while a tile's staged segments would not fit the LDS
 // budget; a multiple of 8 tiles.
-void DarcyBatchGPU::build_op_tiles(const std::vector<int> &rec) {
+// The row tiles of k_c_opl for at most tmax rows per tile (fewer while a
+// tile's neighbourhood exceeds the staging budget), uploaded as a candidate
+// tiling; returns false when the tiling equals an existing candidate.
+bool DarcyBatchGPU::build_tiling(const std::vector<int> &rec, int tmax) {
     const int F = F_;
-    std::vector<int> gen;
-    for (int f = 0; f < F; f++)
-        if (!op_fast_row(&rec[(size_t)f * kRecInts])) gen.push_back(f);
-    ngen_ = (int)gen.size();
-    if (gen.empty()) gen.push_back(0);
     constexpr int kSegBudget = kStageMax * kOplThreads / 8;  // 56 KiB of staged rows per 32-slot pass
     std::vector<int> hptr, halo, eptr, els;
     std::vector<unsigned short> lrec;
-    int T = 128, nseg_max = 0, ntiles = 0;
+    int T = tmax, nseg_max = 0, ntiles = 0;
     for (;; T /= 2) {
         ntiles = ((F + T - 1) / T + 7) / 8 * 8;
         hptr.assign(1, 0);
@@ -2137,25 +2135,64 @@ void DarcyBatchGPU::build_op_tiles(const std::vector<int> &rec) {
     static_assert(kSegBudget <= 512, "staging registers cover 512 segments");
     if (nseg_max > 65535 || opl_lds_bytes(nseg_max, T) > 160 * 1024)
         throw Error(-5, "batched operator: a row tile's neighbourhood does not fit the LDS");
+    for (const OplTiling &c : tilings_)
+        if (c.T == T) return false;
     if (halo.empty()) halo.push_back(0);
     if (els.empty()) els.push_back(0);
-    tile_T_ = T;
-    ntiles_ = ntiles;
-    nseg_max_ = nseg_max;
+    hipStream_t s = stream_;
+    OplTiling c;
+    c.T = T;
+    c.ntiles = ntiles;
+    c.nseg_max = nseg_max;
+    c.hptr = b_up(owned_, hptr, s);
+    c.halo = b_up(owned_, halo, s);
+    c.eptr = b_up(owned_, eptr, s);
+    c.els = b_up(owned_, els, s);
+    c.lrec = b_up(owned_, lrec, s);
+    tilings_.push_back(c);
+    return true;
+}
+
+// Candidate tilings of the FP32 operator (at most 128 and at most 64 rows per
+// tile; tune() times them on the filled slots and keeps the faster), and the
+// general rows (elements with vb neighbours) of k_c_opg.
+void DarcyBatchGPU::build_op_tiles(const std::vector<int> &rec) {
+    const int F = F_;
+    std::vector<int> gen;
+    for (int f = 0; f < F; f++)
+        if (!op_fast_row(&rec[(size_t)f * kRecInts])) gen.push_back(f);
+    ngen_ = (int)gen.size();
+    if (gen.empty()) gen.push_back(0);
+    tilings_.clear();
+    for (int tmax : {128, 64}) build_tiling(rec, tmax);
+    int ntiles_max = 0;
+    size_t lds_max = 0;
+    for (const OplTiling &c : tilings_) {
+        ntiles_max = std::max(ntiles_max, c.ntiles);
+        lds_max = std::max(lds_max, opl_lds_bytes(c.nseg_max, c.T));
+    }
     // general rows are long dependent chains: about 3 per wave, as many
-    // blocks as the partial arrays leave
-    gen_blocks_ = ngen_ ? std::min(kMaxGrid - ntiles_, ((ngen_ + 11) / 12 + 7) / 8 * 8) / 8 * 8 : 0;
+    // blocks as the partial arrays leave beside the largest tiling
+    gen_blocks_ = ngen_ ? std::min(kMaxGrid - ntiles_max, ((ngen_ + 11) / 12 + 7) / 8 * 8) / 8 * 8 : 0;
     if (gen_blocks_ < 0 || (ngen_ && gen_blocks_ == 0))
         throw Error(-5, "batched operator: too many row tiles for the partial arrays");
-    hipStream_t s = stream_;
-    d_gen_ = b_up(owned_, gen, s);
-    d_thptr_ = b_up(owned_, hptr, s);
-    d_thalo_ = b_up(owned_, halo, s);
-    d_teptr_ = b_up(owned_, eptr, s);
-    d_tels_ = b_up(owned_, els, s);
-    d_lrec_ = b_up(owned_, lrec, s);
-    FL_HIP(hipFuncSetAttribute((const void *)k_c_opl, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)opl_lds_bytes(nseg_max_, tile_T_)));
+    d_gen_ = b_up(owned_, gen, stream_);
+    FL_HIP(hipFuncSetAttribute((const void *)k_c_opl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    use_tiling(0);
+}
+
+void DarcyBatchGPU::use_tiling(int i) {
+    const OplTiling &c = tilings_.at(i);
+    d_thptr_ = c.hptr;
+    d_thalo_ = c.halo;
+    d_teptr_ = c.eptr;
+    d_tels_ = c.els;
+    d_lrec_ = c.lrec;
+    tile_T_ = c.T;
+    ntiles_ = c.ntiles;
+    nseg_max_ = c.nseg_max;
+    tiling_ = i;
+    free_graphs();  // captured launches hold the tiling
 }
 
 void DarcyBatchGPU::build() {
@@ -3464,6 +3501,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     st.grid = grid_;
     st.xcd_map = xcd_map_;
     st.chunk = CH;
+    st.tile_rows = ncol_ && precision == 1 ? tile_T_ : 0;
     return st;
 }
 
@@ -3479,9 +3517,56 @@ void DarcyBatchGPU::autotune() {
 void DarcyBatchGPU::tune() {
     alloc32();
     if (ncol_) {
-        // two-level path: the FP32 operator's grids are fixed by its tiles;
-        // the launch grid serves the direction kernel and the FP64 operator
+        // two-level path: the FP32 operator's grid follows its tiles, so the
+        // tiling is what is tuned (each candidate times three operator
+        // launches on the filled slots; q and the operator partials are
+        // rewritten by the next iteration); the launch grid serves the
+        // direction kernel and the FP64 operator
         set_launch(1024, 0, chunk_);
+        if (tilings_.size() > 1 && lw_ == 64) {
+            hipEvent_t t0, t1;
+            FL_HIP(hipEventCreate(&t0));
+            FL_HIP(hipEventCreate(&t1));
+            float best = 1e30f;
+            int bi = 0;
+            for (int i = 0; i < (int)tilings_.size(); i++) {
+                use_tiling(i);
+                BArgs<float> a = make_args<float>(F_, Rp_, grid_, xcd_map_, d_rec_, d_row_ptr_, d_row_c_, d_meta_,
+                                                  d_op_off_, d_conn_off_, d_conn_, d_tab32_, d_K32_, d_tab_, d_K_,
+                                                  d_dinv32_, d_x32_, d_r32_, d_z32_, d_p32a_, d_p32b_, d_q32_,
+                                                  d_part_, sc_);
+                a.frec = d_frec_;
+                const OpTiles ot{d_gen_, d_grec_, d_thptr_, d_thalo_, d_teptr_, d_tels_, d_lrec_, ngen_, gen_blocks_,
+                                 ntiles_, tile_T_, nseg_max_};
+                auto op = [&]() {
+                    if (ngen_) {
+                        if (nch_ == 1)
+                            hipLaunchKernelGGL(k_c_opg<1>, dim3(gen_blocks_), dim3(256), 0, stream_, a, ot, ot.gen,
+                                               ot.grec, a.opT, a.opT64);
+                        else
+                            hipLaunchKernelGGL(k_c_opg<2>, dim3(gen_blocks_), dim3(256), 0, stream_, a, ot, ot.gen,
+                                               ot.grec, a.opT, a.opT64);
+                    }
+                    hipLaunchKernelGGL(k_c_opl, dim3(ntiles_), dim3(kOplThreads), opl_lds_bytes(nseg_max_, tile_T_),
+                                       stream_, a, ot, nch_, 2 * nch_);
+                };
+                op();  // warm
+                FL_HIP(hipEventRecord(t0, stream_));
+                for (int k = 0; k < 3; k++) op();
+                FL_HIP(hipEventRecord(t1, stream_));
+                FL_HIP(hipEventSynchronize(t1));
+                FL_HIP(hipGetLastError());
+                float ms = 0;
+                FL_HIP(hipEventElapsedTime(&ms, t0, t1));
+                if (ms < best) {
+                    best = ms;
+                    bi = i;
+                }
+            }
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            use_tiling(bi);
+        }
         tuned_ = true;
         return;
     }

@@ -621,6 +621,7 @@ int fl_darcy_batch_update_solution(fl_darcy_batch *b, const fl_solver_opts *o, f
             info->ms_narrow4 = st.ms_narrow4;
             info->ms_loop_end = st.ms_loop_end;
             info->ms_handover_end = st.ms_handover_end;
+            info->tile_rows = st.tile_rows;
         }
         if (nconv == R) return 0;
         int ndiv = 0;

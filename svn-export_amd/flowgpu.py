@@ -49,7 +49,7 @@ class BatchInfo(C.Structure):
                 ("handovers", C.c_int), ("handover_iterations", C.c_longlong), ("ms_handover", C.c_double),
                 ("ms_coarse_avg", C.c_double), ("ms_coarse_setup", C.c_double), ("ms_queue_empty", C.c_double),
                 ("ms_narrow16", C.c_double), ("ms_narrow4", C.c_double), ("ms_loop_end", C.c_double),
-                ("ms_handover_end", C.c_double)]
+                ("ms_handover_end", C.c_double), ("tile_rows", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
