@@ -496,13 +496,16 @@ def c2_section(torch, args):
     info = darcy.update_solution(r_tol=1e-12, a_tol=0.0, max_it=100000)
     torch.cuda.synchronize()
     t = time.perf_counter() - t
-    spmv_bytes = 12.0 * nnz + 4.0 * (F + 1) + 32.0 * F
+    # algorithmic bytes of the sampled pair: the direction pass (z and the old
+    # direction read, the new one written) and the SpMV (entries 12 B, slice
+    # row offsets, the direction read once, q written)
+    spmv_bytes = 12.0 * nnz + 4.0 * (F + 1) + 40.0 * F
     out = {"elements": E, "edges": F, "edge_nnz": nnz, "setup_s": round(time.time() - t0, 1),
            "plain_fp64_solve_ms": round(1e3 * t, 2), "cg_iterations": info["iterations"],
-           "spmv_us": round(1e3 * info["ms_spmv_avg"], 2),
-           "spmv_GBs": round(spmv_bytes / (info["ms_spmv_avg"] * 1e-3) / 1e9, 1),
+           "dir_spmv_us": round(1e3 * info["ms_spmv_avg"], 2),
+           "dir_spmv_GBs": round(spmv_bytes / (info["ms_spmv_avg"] * 1e-3) / 1e9, 1),
            "update_us": round(1e3 * info["ms_update_avg"], 2)}
-    log("c2: %d elements, plain FP64 solve %.1f ms (%d iterations), spmv %.1f us"
+    log("c2: %d elements, plain FP64 solve %.1f ms (%d iterations), direction + spmv %.1f us"
         % (E, 1e3 * t, info["iterations"], 1e3 * info["ms_spmv_avg"]))
     if args.transport_steps > 0:
         nsub = 4

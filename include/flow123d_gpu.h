@@ -226,8 +226,9 @@ typedef struct {
 
 /* Result of a solve: CG iterations, final residual 2-norm, rhs 2-norm,
  * converged flag, device times of the phases in ms, and the average device
- * durations of the two CG kernels (SpMV + direction update, and the vector
- * update), sampled with HIP events on the solver stream every 8th iteration. */
+ * durations of the CG iteration's two halves (direction update + SpMV, and
+ * the vector update), sampled with HIP events on the solver stream every 8th
+ * iteration. */
 typedef struct {
     int iterations;
     int converged;

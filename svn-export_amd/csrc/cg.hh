@@ -8,21 +8,25 @@
 // contiguous 512 B (FP64) segment.  Padding entries have value 0 and point
 // at their own row.
 //
-// Two kernels per iteration, no atomics and no device-scope fences: every
+// Three kernels per iteration, no atomics and no device-scope fences: every
 // block writes its partial dot products to a per-block array, and every block
-// of the NEXT kernel folds the whole (small) partial array itself in a fixed
+// of a later kernel folds the whole (small) partial array itself in a fixed
 // order.  All blocks therefore see bitwise identical scalars (alpha, beta, the
 // stop test), and the result does not depend on block scheduling.
-//   K_A(k): rz_k, rr_k, rz_{k-1} folded; stop test; beta; p = z + beta p_old;
-//           q = A p; partial p.q
+//   K_D(k): rz_k, rr_k, rz_{k-1} folded; stop test; beta; p = z + beta p_old
+//   K_A(k): q = A p; partial p.q
 //   K_B(k): alpha = rz_k / p.q; x += alpha p; r -= alpha q; z = D^-1 r;
 //           partial r.z, r.r into the (k+1) parity buffers
+// The direction is formed by its own streaming pass, so the operator gathers
+// one vector per entry (forming z + beta p_old at every gathered column
+// instead gathers two); the matrix is read with non-temporal loads, so the
+// 17 MB-per-million-rows vectors stay in the caches between the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace fl {
 
-constexpr int kCgBlock = 256;
+constexpr int kCgBlock = 512;
 
 // done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
 // norm: 0 the stop test is on ||r||_2 / ||b||_2; 1 on the natural norm of the
@@ -89,6 +93,26 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *const (&
         }
 }
 
+// Slices walked by one wavefront.  Workgroups are dispatched round-robin over
+// the 8 XCDs (workgroup b runs on XCD b % 8), each XCD with its own L2.  When
+// the grid is a multiple of 8, XCD x walks the x-th eighth of the slices: the
+// rows a slice gathers from (its mesh neighbours, numbered a few layers away
+// at most) then sit in the same L2 instead of being fetched by all eight.
+struct SliceWalk {
+    int first, end, step;
+};
+
+__device__ __forceinline__ SliceWalk slice_walk(int nslices) {
+    const int wid = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    if ((gridDim.x & 7) == 0) {
+        const int per = (nslices + 7) >> 3;
+        const int s0 = min((int)(blockIdx.x & 7) * per, nslices);
+        return SliceWalk{s0 + (int)(blockIdx.x >> 3) * wpb + wid, min(s0 + per, nslices),
+                         (int)(gridDim.x >> 3) * wpb};
+    }
+    return SliceWalk{(int)blockIdx.x * wpb + wid, nslices, (int)gridDim.x * wpb};
+}
+
 template <class V>
 __global__ void __launch_bounds__(kCgBlock) k_cg_init(CGDev<V> a, V *p0) {
     double v[2] = {0.0, 0.0};
@@ -132,8 +156,8 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double
 // delta2 > 0: pause (done = 2) once the recursive residual has dropped by
 // sqrt(delta2) since the last true residual (reliable-update criterion).
 template <class V>
-__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
-                                                    V *__restrict__ p_out, int is_ru, double delta2) {
+__global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
+                                                   V *__restrict__ p_out, int is_ru, double delta2) {
     CGState *st = a.st;
     if (st->done) return;
     const double *cur_rz = (k & 1) ? a.part_rz1 : a.part_rz0;
@@ -162,11 +186,18 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
         return;
     }
     const V beta = (V)(k == 0 ? 0.0 : rz / s[2]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+        p_out[i] = a.z[i] + beta * p_in[i];
+}
+
+// q = A p and the partial p.q; skipped once K_D has stopped the iteration
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, const V *__restrict__ p) {
+    if (a.st->done) return;
     double v[1] = {0.0};
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+    const SliceWalk sw = slice_walk(a.nslices);
+    for (int sl = sw.first; sl < sw.end; sl += sw.step) {
         const int row = sl * 64 + lane;
         const int e0 = a.slice_ptr[sl] + lane;
         const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
@@ -175,10 +206,11 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
         // 4 entries per step: all index / value loads issue before the gathers
         for (; k + 4 <= w; k += 4) {
             const int i0 = e0 + k * 64;
-            int c0 = a.col[i0], c1 = a.col[i0 + 64], c2 = a.col[i0 + 128], c3 = a.col[i0 + 192];
-            V v0 = a.val[i0], v1 = a.val[i0 + 64], v2 = a.val[i0 + 128], v3 = a.val[i0 + 192];
-            V g0 = a.z[c0] + beta * p_in[c0], g1 = a.z[c1] + beta * p_in[c1];
-            V g2 = a.z[c2] + beta * p_in[c2], g3 = a.z[c3] + beta * p_in[c3];
+            const int c0 = __builtin_nontemporal_load(a.col + i0), c1 = __builtin_nontemporal_load(a.col + i0 + 64);
+            const int c2 = __builtin_nontemporal_load(a.col + i0 + 128), c3 = __builtin_nontemporal_load(a.col + i0 + 192);
+            const V v0 = __builtin_nontemporal_load(a.val + i0), v1 = __builtin_nontemporal_load(a.val + i0 + 64);
+            const V v2 = __builtin_nontemporal_load(a.val + i0 + 128), v3 = __builtin_nontemporal_load(a.val + i0 + 192);
+            const V g0 = p[c0], g1 = p[c1], g2 = p[c2], g3 = p[c3];
             acc += v0 * g0;
             acc += v1 * g1;
             acc += v2 * g2;
@@ -186,14 +218,11 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, int k, int G, 
         }
         for (; k < w; k++) {
             const int i0 = e0 + k * 64;
-            int c = a.col[i0];
-            acc += a.val[i0] * (a.z[c] + beta * p_in[c]);
+            acc += __builtin_nontemporal_load(a.val + i0) * p[__builtin_nontemporal_load(a.col + i0)];
         }
         if (row < a.n) {
-            V p = a.z[row] + beta * p_in[row];
-            p_out[row] = p;
             a.q[row] = acc;
-            v[0] += (double)p * (double)acc;
+            v[0] += (double)p[row] * (double)acc;
         }
     }
     double *const dst[1] = {a.part_pq};
@@ -250,9 +279,8 @@ static __global__ void __launch_bounds__(kCgBlock) k_ru_resid(CGDev<float> a, co
                                                      double *part_rr) {
     double v[2] = {0.0, 0.0};
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+    const SliceWalk sw = slice_walk(a.nslices);
+    for (int sl = sw.first; sl < sw.end; sl += sw.step) {
         const int row = sl * 64 + lane;
         const int e0 = a.slice_ptr[sl] + lane;
         const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
@@ -281,9 +309,8 @@ static __global__ void k_cg_clear_done(CGState *st) { st->done = 0; }
 static __global__ void __launch_bounds__(kCgBlock) k_cg_init_warm(CGDev<double> a, double *p0) {
     double v[2] = {0.0, 0.0};
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int sl = wave; sl < a.nslices; sl += nwaves) {
+    const SliceWalk sw = slice_walk(a.nslices);
+    for (int sl = sw.first; sl < sw.end; sl += sw.step) {
         const int row = sl * 64 + lane;
         const int e0 = a.slice_ptr[sl] + lane;
         const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;

@@ -678,7 +678,7 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
     CGState h{};
     int launched = 0;
     int chunk = 32;
-    // every kSampleEvery-th iteration is bracketed by events (spmv, update)
+    // every kSampleEvery-th iteration is bracketed by events (direction + spmv, update)
     const int kSampleEvery = 8;
     struct Sample { int ev, it; };
     std::vector<Sample> samples;
@@ -700,8 +700,9 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
                 e0 = next_ev(); next_ev(); next_ev();
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
-            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
+            hipLaunchKernelGGL(k_cg_dir<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
                                (const double *)pin, pout, itx == 0 ? 1 : 0, 0.0);
+            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, (const double *)pout);
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
                                (const double *)pout);
@@ -801,8 +802,9 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
                 e0 = next_ev(); next_ev(); next_ev();
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
-            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pin,
+            hipLaunchKernelGGL(k_cg_dir<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pin,
                                pout, k == last_ru ? 1 : 0, delta2);
+            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, (const float *)pout);
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pout);
             if (samp) {
