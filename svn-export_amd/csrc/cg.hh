@@ -20,13 +20,18 @@
 // The direction is formed by its own streaming pass, so the operator gathers
 // one vector per entry (forming z + beta p_old at every gathered column
 // instead gathers two); the matrix is read with non-temporal loads, so the
-// 17 MB-per-million-rows vectors stay in the caches between the kernels.
+// vectors (17 MB per million rows each) stay in the caches between the
+// kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace fl {
 
 constexpr int kCgBlock = 512;
+// grid of the iteration kernels: kCgGridLarge blocks from kCgLargeRows rows
+// on, kCgGridSmall below (one wave of resident blocks on 256 CUs, and half
+// the partial sums to fold)
+constexpr int kCgGridSmall = 512, kCgGridLarge = 1024, kCgLargeRows = 1 << 20;
 
 // done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
 // norm: 0 the stop test is on ||r||_2 / ||b||_2; 1 on the natural norm of the
@@ -152,14 +157,14 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_setup(CGDev<V> a, int G, double
     }
 }
 
+// Folds rz_k, rr_k and rz_{k-1}; false (and the state set by block 0) when the
+// iteration stops or pauses at step k, else true with beta_k.
 // is_ru: the residual of step k is a freshly recomputed true residual;
 // delta2 > 0: pause (done = 2) once the recursive residual has dropped by
 // sqrt(delta2) since the last true residual (reliable-update criterion).
 template <class V>
-__global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
-                                                   V *__restrict__ p_out, int is_ru, double delta2) {
+__device__ __forceinline__ bool cg_beta(const CGDev<V> &a, int k, int G, int is_ru, double delta2, V &beta) {
     CGState *st = a.st;
-    if (st->done) return;
     const double *cur_rz = (k & 1) ? a.part_rz1 : a.part_rz0;
     const double *cur_rr = (k & 1) ? a.part_rr1 : a.part_rr0;
     const double *old_rz = (k & 1) ? a.part_rz0 : a.part_rz1;
@@ -173,7 +178,7 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, c
             st->it = k;
             st->rr = rr;
         }
-        return;
+        return false;
     }
     if (is_ru) {
         if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_ru = rr;
@@ -183,9 +188,18 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, c
             st->it = k;
             st->rr = rr;
         }
-        return;
+        return false;
     }
-    const V beta = (V)(k == 0 ? 0.0 : rz / s[2]);
+    beta = (V)(k == 0 ? 0.0 : rz / s[2]);
+    return true;
+}
+
+template <class V>
+__global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
+                                                   V *__restrict__ p_out, int is_ru, double delta2) {
+    if (a.st->done) return;
+    V beta;
+    if (!cg_beta(a, k, G, is_ru, delta2, beta)) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
         p_out[i] = a.z[i] + beta * p_in[i];
 }

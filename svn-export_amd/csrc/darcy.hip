@@ -387,6 +387,7 @@ void DarcyGPU::build_pattern() {
         for (int i = 0; i < F; i++) rinv[rperm[i]] = i;
     }
     nslices_ = (F + 63) / 64;
+    cg_grid_ = F >= kCgLargeRows ? kCgGridLarge : kCgGridSmall;
     std::vector<int> slice_ptr(nslices_ + 1, 0);
     for (int sl = 0; sl < nslices_; sl++) {
         int w = 0;
