@@ -394,8 +394,9 @@ int fl_darcy_batch_get_solution(fl_darcy_batch *b, int r, double *out);
 const double *fl_darcy_batch_device_solution(fl_darcy_batch *b, int *n_padded);
 /* Launch shape (grid blocks per 64 realizations, multiple of 8 up to 2048;
  * row mapping 0 strided, 1 XCD-contiguous, 2 XCD- and block-contiguous;
- * iterations per captured graph chunk) and the
- * autotuner that picks grid and mapping by timing the operator kernel. */
+ * iterations per captured graph chunk, 0 automatic: 64 with the two-level
+ * preconditioner, 16 without) and the autotuner that picks grid and mapping
+ * by timing the operator kernel. */
 int fl_darcy_batch_set_launch(fl_darcy_batch *b, int grid, int xcd_map, int chunk);
 int fl_darcy_batch_autotune(fl_darcy_batch *b);
 /* Once the queue is empty and at most n realizations are still iterating,

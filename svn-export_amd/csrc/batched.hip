@@ -757,12 +757,31 @@ __global__ void __launch_bounds__(kFoldThreads) k_b_fold(BatchScal sc, const dou
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int r = blockIdx.x * 64 + lane;
     double s[3] = {0.0, 0.0, 0.0};
-    for (int g = wid; g < G; g += kFoldThreads / 64) {
-        const size_t o = (size_t)g * Rp + r;
-        if (MODE == kFoldAlpha) s[0] += part0[o];
-        else {
-            s[1] += part1[o];
-            s[2] += part2[o];
+    // kFoldChunk rows of the wave's share loaded together (rows past the end
+    // re-read row wid and are dropped), then summed in row order
+    constexpr int kW = kFoldThreads / 64, kFoldChunk = 8;
+    for (int g0 = wid; g0 < G; g0 += kFoldChunk * kW) {
+        double v1[kFoldChunk], v2[kFoldChunk];
+#pragma unroll
+        for (int u = 0; u < kFoldChunk; u++) {
+            const int g = g0 + u * kW;
+            const size_t o = (size_t)(g < G ? g : wid) * Rp + r;
+            if (MODE == kFoldAlpha) {
+                v1[u] = part0[o];
+                v2[u] = 0.0;
+            } else {
+                v1[u] = part1[o];
+                v2[u] = part2[o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldChunk; u++) {
+            if (g0 + u * kW >= G) break;
+            if (MODE == kFoldAlpha) s[0] += v1[u];
+            else {
+                s[1] += v1[u];
+                s[2] += v2[u];
+            }
         }
     }
     for (int k = 0; k < 3; k++) sh[k][wid][lane] = s[k];
@@ -1772,7 +1791,7 @@ __global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
 
 // Coarse solve y = (Z^T S Z)^-1 Z^T r of the slots the fold will step, from
 // the packed lower tiles of the FP32 inverse (FP64 accumulation), in two
-// kernels.  k_c_solve_t: block (tile (I, J), slot) streams its 64 x 64 tile
+// kernels.  k_c_solve_t: block (tile (I, J), kSolveSlots slots) streams its 64 x 64 tiles
 // once and forms the tile's row product X_IJ r_J for y_I and, off the
 // diagonal, its transposed product X_IJ^T r_I for y_J, into the per-slot
 // table py[slot][block row][block column][64].  k_c_solve_y: block (I, slot)
@@ -1789,57 +1808,82 @@ __device__ __forceinline__ bool coarse_act(const BatchScal &sc, int slot, const 
     return ru_pick(sc, slot, st0);
 }
 
+// slots per k_c_solve_t block, their tiles loaded together (2 and 4 ran
+// slower: more registers per thread, fewer blocks in flight)
+constexpr int kSolveSlots = 1;
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_c_solve_t(BatchScal sc, CArgs c, size_t Rp, const int *__restrict__ lsel,
                                                    double *__restrict__ py) {
-    const int t = blockIdx.x, slot = blockIdx.y;
-    if (!coarse_act<MODE>(sc, slot, lsel)) return;  // block-uniform
+    const int t = blockIdx.x, s0 = blockIdx.y * kSolveSlots;
+    bool act[kSolveSlots], any = false;
+#pragma unroll
+    for (int u = 0; u < kSolveSlots; u++) {
+        act[u] = coarse_act<MODE>(sc, s0 + u, lsel);
+        any |= act[u];
+    }
+    if (!any) return;  // block-uniform
     int I, J;
     coarse::tile_of(t, I, J);
-    __shared__ double xc[64], xr[64], red[4][64];
+    __shared__ double xc[kSolveSlots][64], xr[kSolveSlots][64], red[4][64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int nt = c.nt;
-    // tile rows (tid >> 4) + 16 u, columns 4 (tid & 15) .. + 3
-    const float4 *T = reinterpret_cast<const float4 *>(c.ainv + ((size_t)slot * coarse::ntiles_lower(nt) + t) * 4096);
-    float4 v[4];
+    const size_t ntl = coarse::ntiles_lower(nt);
+    // tile rows (tid >> 4) + 16 q, columns 4 (tid & 15) .. + 3
+    float4 v[kSolveSlots][4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) v[u] = T[tid + 256 * u];
-    if (tid < 64) xc[tid] = c.rc[(size_t)(J * 64 + tid) * Rp + slot];
-    else if (tid < 128) xr[tid - 64] = c.rc[(size_t)(I * 64 + tid - 64) * Rp + slot];
+    for (int u = 0; u < kSolveSlots; u++) {
+        const float4 *T = reinterpret_cast<const float4 *>(c.ainv + ((size_t)(s0 + u) * ntl + t) * 4096);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[u][q] = act[u] ? T[tid + 256 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // the slots' tile column (J) and row (I) entries of Z^T r, staged in LDS
+    for (int k = tid; k < 128 * kSolveSlots; k += 256) {
+        const int u = (k >> 6) % kSolveSlots, l = k & 63;
+        if (k < 64 * kSolveSlots) xc[u][l] = c.rc[(size_t)(J * 64 + l) * Rp + s0 + u];
+        else xr[u][l] = c.rc[(size_t)(I * 64 + l) * Rp + s0 + u];
+    }
     __syncthreads();
     const int c0 = 4 * (tid & 15), rb = tid >> 4;
-    double cp[4] = {0.0, 0.0, 0.0, 0.0};
-    double *row_out = py + (((size_t)slot * nt + I) * nt + J) * 64;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int r = rb + 16 * u;
-        double s = (double)v[u].x * xc[c0];
-        s += (double)v[u].y * xc[c0 + 1];
-        s += (double)v[u].z * xc[c0 + 2];
-        s += (double)v[u].w * xc[c0 + 3];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        s += __shfl_xor(s, 4, 64);
-        s += __shfl_xor(s, 8, 64);
-        if ((tid & 15) == 0) row_out[r] = s;
-        const double xi = xr[r];
-        cp[0] += (double)v[u].x * xi;
-        cp[1] += (double)v[u].y * xi;
-        cp[2] += (double)v[u].z * xi;
-        cp[3] += (double)v[u].w * xi;
+    for (int u = 0; u < kSolveSlots; u++) {
+        if (!act[u]) continue;  // block-uniform
+        const int slot = s0 + u;
+        double cp[4] = {0.0, 0.0, 0.0, 0.0};
+        double *row_out = py + (((size_t)slot * nt + I) * nt + J) * 64;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = rb + 16 * q;
+            double s = (double)v[u][q].x * xc[u][c0];
+            s += (double)v[u][q].y * xc[u][c0 + 1];
+            s += (double)v[u][q].z * xc[u][c0 + 2];
+            s += (double)v[u][q].w * xc[u][c0 + 3];
+            s += __shfl_xor(s, 1, 64);
+            s += __shfl_xor(s, 2, 64);
+            s += __shfl_xor(s, 4, 64);
+            s += __shfl_xor(s, 8, 64);
+            if ((tid & 15) == 0) row_out[r] = s;
+            const double xi = xr[u][r];
+            cp[0] += (double)v[u][q].x * xi;
+            cp[1] += (double)v[u][q].y * xi;
+            cp[2] += (double)v[u][q].z * xi;
+            cp[3] += (double)v[u][q].w * xi;
+        }
+        if (I == J) continue;  // block-uniform
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            cp[q] += __shfl_xor(cp[q], 16, 64);
+            cp[q] += __shfl_xor(cp[q], 32, 64);
+        }
+        if (lane < 16)
+#pragma unroll
+            for (int q = 0; q < 4; q++) red[wid][c0 + q] = cp[q];
+        __syncthreads();
+        if (tid < 64)
+            py[(((size_t)slot * nt + J) * nt + I) * 64 + tid] =
+                ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        __syncthreads();  // red is reused by the next slot
     }
-    if (I == J) return;  // block-uniform
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        cp[q] += __shfl_xor(cp[q], 16, 64);
-        cp[q] += __shfl_xor(cp[q], 32, 64);
-    }
-    if (lane < 16)
-#pragma unroll
-        for (int q = 0; q < 4; q++) red[wid][c0 + q] = cp[q];
-    __syncthreads();
-    if (tid < 64)
-        py[(((size_t)slot * nt + J) * nt + I) * 64 + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
 }
 
 template <int MODE>
@@ -1852,7 +1896,15 @@ __global__ void __launch_bounds__(64) k_c_solve_y(BatchScal sc, CArgs c, const d
     const int nt = c.nt;
     const double *p = py + ((size_t)slot * nt + I) * nt * 64 + r;
     double y = 0.0;
-    for (int J = 0; J < nt; J++) y += p[(size_t)J * 64];
+    constexpr int kChunk = 16;  // loads in flight together; summed in J order
+    for (int J0 = 0; J0 < nt; J0 += kChunk) {
+        double v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) v[u] = p[(size_t)min(J0 + u, nt - 1) * 64];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++)
+            if (J0 + u < nt) y += v[u];
+    }
     double s[3] = {0.0, 0.0, 0.0};
     const int row = I * 64 + r;
     if (row < c.nc) {
@@ -1886,11 +1938,20 @@ __global__ void k_c_fin(BatchScal sc, const double *__restrict__ cpart, int npar
         sc.zp[slot] = 0;
         return;
     }
-    double t[3];
-    for (int k = 0; k < 3; k++) {
-        double x = 0.0;
-        for (int p = 0; p < nparts; p++) x += cpart[((size_t)k * nparts + p) * Rp + slot];
-        t[k] = x;
+    double t[3] = {0.0, 0.0, 0.0};
+    constexpr int kChunk = 16;  // loads in flight together; each sum in part order
+    for (int p0 = 0; p0 < nparts; p0 += kChunk) {
+        double v[3][kChunk];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int u = 0; u < kChunk; u++)
+                v[k][u] = cpart[((size_t)k * nparts + min(p0 + u, nparts - 1)) * Rp + slot];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int u = 0; u < kChunk; u++)
+                if (p0 + u < nparts) t[k] += v[k][u];
     }
     // t[0] = r^T D^-1 r (Jacobi part), t[1] = r.r, t[2] = y . Z^T r (coarse part);
     // rr: square of the stop-test norm
@@ -2814,8 +2875,11 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     CArgs ca{};
     int G = grid_, X = xcd_map_;
     const int NC = ncol_;  // two-level preconditioner columns (0: Jacobi)
-    // iterations per captured chunk (automatic: 32 two-level, 16 one-level)
-    const int CH = chunk_ > 0 ? chunk_ : (NC ? 32 : 16);
+    // iterations per captured chunk (automatic: 64 two-level, 16 one-level).
+    // Every chunk ends with the true residual of its running realizations;
+    // on the two-level path 64 beat 32 (the residual pass costs about one
+    // operator's worth of FP64 gathers) and 96 gained nothing more.
+    const int CH = chunk_ > 0 ? chunk_ : (NC ? 64 : 16);
     auto rebuild = [&]() {
         G = grid_;
         X = xcd_map_;
@@ -2860,7 +2924,7 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const dim3 gt(coarse::ntiles_lower(nt_), 64 * nch_), gy(nt_, 64 * nch_), gfin(nch_), b64(64);
+        const dim3 gt(coarse::ntiles_lower(nt_), 64 * nch_ / kSolveSlots), gy(nt_, 64 * nch_), gfin(nch_), b64(64);
         const double *p1 = a64.part1, *p2 = a64.part2;
 #define FL_CFOLD(M)                                                                                                  \
     hipLaunchKernelGGL(k_c_solve_t<M>, gt, blk, 0, stream_, sc_, ca, (size_t)Rp_, lsel, d_py_);                      \

@@ -141,6 +141,7 @@ class DarcyGPU {
     double *d_sol_ = nullptr;
     double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
+    CGState *h_state_ = nullptr;  // pinned host copy of the CG state, read after every chunk of iterations
     int *d_flag_ = nullptr;
     int cg_grid_ = kCgGridLarge;  // set from the system size with the pattern
     int stop_norm_ = 0;

@@ -280,12 +280,16 @@ DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
     FL_HIP(hipGetDevice(&device_));
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     build_pattern();
+    // pinned, so the per-chunk state read is a plain copy (a pageable target
+    // is staged through a copy kernel of its own)
+    FL_HIP(hipHostMalloc((void **)&h_state_, sizeof(CGState), hipHostMallocDefault));
 }
 
 DarcyGPU::~DarcyGPU() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     for (void *p : owned_) dev_free(p);
+    if (h_state_) (void)hipHostFree(h_state_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -715,8 +719,9 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
         }
         launched += chunk;
         FL_HIP(hipGetLastError());
-        FL_HIP(hipMemcpyAsync(&h, state, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(h_state_, state, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
+        h = *h_state_;
         if (h.done || launched > max_it + chunk) break;
         if (chunk < 256) chunk *= 2;
     }
@@ -814,8 +819,9 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
             }
         }
         FL_HIP(hipGetLastError());
-        FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(h_state_, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
+        h = *h_state_;
         if (h.done) {
             // converged on a true FP64 residual, or out of iterations
             if ((h.done == 1 && h.it == last_ru) || h.it >= max_it || n_ru >= kMaxUpdates) break;

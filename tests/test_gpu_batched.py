@@ -305,9 +305,11 @@ def test_batch_two_level_narrow_tail(cube6, precision):
 
 @pytest.mark.parametrize("coarse", [0, 1])
 def test_batch_handover8_fp32(cube6, coarse):
-    """Hand-over of the last 8 live realizations in the FP32 batch (one-level,
-    and two-level with its narrowed tail): the batch terminates with every
-    realization converged, equal to the batch run without hand-over."""
+    """Hand-over of the live realizations once at most 8 are left in the FP32
+    batch (one-level, and two-level with its narrowed tail; the count is read
+    at chunk ends, so fewer than 8 may be left by then): the batch terminates
+    with every realization converged, equal to the batch run without
+    hand-over."""
     om, gm, d, g = cube6
     R = 96
     K = cases.mc_conductivity(om, R, seed=21)
@@ -323,7 +325,7 @@ def test_batch_handover8_fp32(cube6, coarse):
     print("coarse %d: handovers %d (%d iterations), chunks %d / %d" % (
         coarse, i1["handovers"], i1["handover_iterations"], i1["chunks"], i0["chunks"]), flush=True)
     assert i0["converged"].all() and i1["converged"].all()
-    assert i1["handovers"] >= 8
+    assert 1 <= i1["handovers"] and i0["handovers"] == 0
     S = om.n_sides
     worst = max(rel(a[:S], c[:S]) for a, c in zip(x1, x0))
     print("  worst flux difference %.2e" % worst, flush=True)
