@@ -1791,7 +1791,7 @@ __global__ void __launch_bounds__(256) k_c_zp(BArgs<V> a, CArgs c) {
 
 // Coarse solve y = (Z^T S Z)^-1 Z^T r of the slots the fold will step, from
 // the packed lower tiles of the FP32 inverse (FP64 accumulation), in two
-// kernels.  k_c_solve_t: block (tile (I, J), kSolveSlots slots) streams its 64 x 64 tiles
+// kernels.  k_c_solve_t: block (kSolveTiles tiles (I, J), slot) streams its 64 x 64 tiles
 // once and forms the tile's row product X_IJ r_J for y_I and, off the
 // diagonal, its transposed product X_IJ^T r_I for y_J, into the per-slot
 // table py[slot][block row][block column][64].  k_c_solve_y: block (I, slot)
@@ -1808,49 +1808,42 @@ __device__ __forceinline__ bool coarse_act(const BatchScal &sc, int slot, const 
     return ru_pick(sc, slot, st0);
 }
 
-// slots per k_c_solve_t block, their tiles loaded together (2 and 4 ran
-// slower: more registers per thread, fewer blocks in flight)
-constexpr int kSolveSlots = 1;
+// consecutive tiles of one slot per k_c_solve_t block, loaded together (2 and
+// 4 ran slower, as did 4 slots' tiles per block: the blocks are short and
+// many, and more work per block lengthens each one's serial part)
+constexpr int kSolveTiles = 1;
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_c_solve_t(BatchScal sc, CArgs c, size_t Rp, const int *__restrict__ lsel,
                                                    double *__restrict__ py) {
-    const int t = blockIdx.x, s0 = blockIdx.y * kSolveSlots;
-    bool act[kSolveSlots], any = false;
-#pragma unroll
-    for (int u = 0; u < kSolveSlots; u++) {
-        act[u] = coarse_act<MODE>(sc, s0 + u, lsel);
-        any |= act[u];
-    }
-    if (!any) return;  // block-uniform
-    int I, J;
-    coarse::tile_of(t, I, J);
-    __shared__ double xc[kSolveSlots][64], xr[kSolveSlots][64], red[4][64];
+    const int slot = blockIdx.y;
+    if (!coarse_act<MODE>(sc, slot, lsel)) return;  // block-uniform
+    const int nt = c.nt, ntl = coarse::ntiles_lower(nt), t0 = blockIdx.x * kSolveTiles;
+    __shared__ double xc[kSolveTiles][64], xr[kSolveTiles][64], red[4][64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int nt = c.nt;
-    const size_t ntl = coarse::ntiles_lower(nt);
+    int I[kSolveTiles], J[kSolveTiles];
+#pragma unroll
+    for (int u = 0; u < kSolveTiles; u++) coarse::tile_of(min(t0 + u, ntl - 1), I[u], J[u]);
     // tile rows (tid >> 4) + 16 q, columns 4 (tid & 15) .. + 3
-    float4 v[kSolveSlots][4];
+    const float4 *T = reinterpret_cast<const float4 *>(c.ainv + ((size_t)slot * ntl + t0) * 4096);
+    float4 v[kSolveTiles][4];
 #pragma unroll
-    for (int u = 0; u < kSolveSlots; u++) {
-        const float4 *T = reinterpret_cast<const float4 *>(c.ainv + ((size_t)(s0 + u) * ntl + t) * 4096);
+    for (int u = 0; u < kSolveTiles; u++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[u][q] = act[u] ? T[tid + 256 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    // the slots' tile column (J) and row (I) entries of Z^T r, staged in LDS
-    for (int k = tid; k < 128 * kSolveSlots; k += 256) {
-        const int u = (k >> 6) % kSolveSlots, l = k & 63;
-        if (k < 64 * kSolveSlots) xc[u][l] = c.rc[(size_t)(J * 64 + l) * Rp + s0 + u];
-        else xr[u][l] = c.rc[(size_t)(I * 64 + l) * Rp + s0 + u];
+        for (int q = 0; q < 4; q++)
+            v[u][q] = t0 + u < ntl ? T[u * 1024 + tid + 256 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kSolveTiles; u++) {
+        if (tid < 64) xc[u][tid] = c.rc[(size_t)(J[u] * 64 + tid) * Rp + slot];
+        else if (tid < 128) xr[u][tid - 64] = c.rc[(size_t)(I[u] * 64 + tid - 64) * Rp + slot];
     }
     __syncthreads();
     const int c0 = 4 * (tid & 15), rb = tid >> 4;
 #pragma unroll
-    for (int u = 0; u < kSolveSlots; u++) {
-        if (!act[u]) continue;  // block-uniform
-        const int slot = s0 + u;
+    for (int u = 0; u < kSolveTiles; u++) {
+        if (t0 + u >= ntl) break;  // block-uniform
         double cp[4] = {0.0, 0.0, 0.0, 0.0};
-        double *row_out = py + (((size_t)slot * nt + I) * nt + J) * 64;
+        double *row_out = py + (((size_t)slot * nt + I[u]) * nt + J[u]) * 64;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int r = rb + 16 * q;
@@ -1869,7 +1862,7 @@ __global__ void __launch_bounds__(256) k_c_solve_t(BatchScal sc, CArgs c, size_t
             cp[2] += (double)v[u][q].z * xi;
             cp[3] += (double)v[u][q].w * xi;
         }
-        if (I == J) continue;  // block-uniform
+        if (I[u] == J[u]) continue;  // block-uniform
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             cp[q] += __shfl_xor(cp[q], 16, 64);
@@ -1880,9 +1873,9 @@ __global__ void __launch_bounds__(256) k_c_solve_t(BatchScal sc, CArgs c, size_t
             for (int q = 0; q < 4; q++) red[wid][c0 + q] = cp[q];
         __syncthreads();
         if (tid < 64)
-            py[(((size_t)slot * nt + J) * nt + I) * 64 + tid] =
+            py[(((size_t)slot * nt + J[u]) * nt + I[u]) * 64 + tid] =
                 ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-        __syncthreads();  // red is reused by the next slot
+        __syncthreads();  // red is reused by the next tile
     }
 }
 
@@ -2924,7 +2917,8 @@ BatchStats DarcyBatchGPU::solve(double rtol, double atol, int max_it, int precis
     const int *nosel = nullptr;
     auto gagg = [&]() { return dim3(nagg_, nch_); };
     auto cfold = [&](int mode, int mixed, const int *lsel) {
-        const dim3 gt(coarse::ntiles_lower(nt_), 64 * nch_ / kSolveSlots), gy(nt_, 64 * nch_), gfin(nch_), b64(64);
+        const dim3 gt((coarse::ntiles_lower(nt_) + kSolveTiles - 1) / kSolveTiles, 64 * nch_), gy(nt_, 64 * nch_),
+            gfin(nch_), b64(64);
         const double *p1 = a64.part1, *p2 = a64.part2;
 #define FL_CFOLD(M)                                                                                                  \
     hipLaunchKernelGGL(k_c_solve_t<M>, gt, blk, 0, stream_, sc_, ca, (size_t)Rp_, lsel, d_py_);                      \

@@ -17,11 +17,14 @@
 //   K_A(k): q = A p; partial p.q
 //   K_B(k): alpha = rz_k / p.q; x += alpha p; r -= alpha q; z = D^-1 r;
 //           partial r.z, r.r into the (k+1) parity buffers
-// The direction is formed by its own streaming pass, so the operator gathers
-// one vector per entry (forming z + beta p_old at every gathered column
-// instead gathers two); the matrix is read with non-temporal loads, so the
-// vectors (17 MB per million rows each) stay in the caches between the
-// kernels.
+// Systems of kCgLargeRows rows and more: the direction is formed by its own
+// streaming pass, so the operator gathers one vector per entry (forming
+// z + beta p_old at every gathered column instead gathers two), and the
+// matrix is read with non-temporal loads, so the vectors (17 MB per million
+// rows each) stay in the caches between the kernels.  Smaller systems, whose
+// matrix and vectors stay in the last-level cache and where a launch costs
+// more than the second gather, run K_D and K_A as one kernel (k_cg_dir_spmv)
+// with ordinary loads.  Both forms give bitwise the same iterates.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,8 +33,9 @@ namespace fl {
 constexpr int kCgBlock = 512;
 // grid of the iteration kernels: kCgGridLarge blocks from kCgLargeRows rows
 // on, kCgGridSmall below (one wave of resident blocks on 256 CUs, and half
-// the partial sums to fold)
+// the partial sums to fold), kCgGridTiny below kCgTinyRows
 constexpr int kCgGridSmall = 512, kCgGridLarge = 1024, kCgLargeRows = 1 << 20;
+constexpr int kCgGridTiny = 128, kCgTinyRows = 1 << 17;
 
 // done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
 // norm: 0 the stop test is on ||r||_2 / ||b||_2; 1 on the natural norm of the
@@ -116,6 +120,15 @@ __device__ __forceinline__ SliceWalk slice_walk(int nslices) {
                          (int)(gridDim.x >> 3) * wpb};
     }
     return SliceWalk{(int)blockIdx.x * wpb + wid, nslices, (int)gridDim.x * wpb};
+}
+
+// Matrix entries: non-temporal loads (NT) where the matrix and vectors exceed
+// the last-level cache, so the streamed matrix does not evict the vectors;
+// ordinary loads where everything stays cached between iterations.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_mat(const T *p) {
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
 }
 
 template <class V>
@@ -205,7 +218,7 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_dir(CGDev<V> a, int k, int G, c
 }
 
 // q = A p and the partial p.q; skipped once K_D has stopped the iteration
-template <class V>
+template <class V, bool NT>
 __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, const V *__restrict__ p) {
     if (a.st->done) return;
     double v[1] = {0.0};
@@ -220,10 +233,10 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, const V *__res
         // 4 entries per step: all index / value loads issue before the gathers
         for (; k + 4 <= w; k += 4) {
             const int i0 = e0 + k * 64;
-            const int c0 = __builtin_nontemporal_load(a.col + i0), c1 = __builtin_nontemporal_load(a.col + i0 + 64);
-            const int c2 = __builtin_nontemporal_load(a.col + i0 + 128), c3 = __builtin_nontemporal_load(a.col + i0 + 192);
-            const V v0 = __builtin_nontemporal_load(a.val + i0), v1 = __builtin_nontemporal_load(a.val + i0 + 64);
-            const V v2 = __builtin_nontemporal_load(a.val + i0 + 128), v3 = __builtin_nontemporal_load(a.val + i0 + 192);
+            const int c0 = ld_mat<NT>(a.col + i0), c1 = ld_mat<NT>(a.col + i0 + 64);
+            const int c2 = ld_mat<NT>(a.col + i0 + 128), c3 = ld_mat<NT>(a.col + i0 + 192);
+            const V v0 = ld_mat<NT>(a.val + i0), v1 = ld_mat<NT>(a.val + i0 + 64);
+            const V v2 = ld_mat<NT>(a.val + i0 + 128), v3 = ld_mat<NT>(a.val + i0 + 192);
             const V g0 = p[c0], g1 = p[c1], g2 = p[c2], g3 = p[c3];
             acc += v0 * g0;
             acc += v1 * g1;
@@ -232,11 +245,57 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(CGDev<V> a, const V *__res
         }
         for (; k < w; k++) {
             const int i0 = e0 + k * 64;
-            acc += __builtin_nontemporal_load(a.val + i0) * p[__builtin_nontemporal_load(a.col + i0)];
+            acc += ld_mat<NT>(a.val + i0) * p[ld_mat<NT>(a.col + i0)];
         }
         if (row < a.n) {
             a.q[row] = acc;
             v[0] += (double)p[row] * (double)acc;
+        }
+    }
+    double *const dst[1] = {a.part_pq};
+    block_partials<1>(v, dst);
+}
+
+// K_D and K_A in one kernel: the direction z + beta p_old is formed at every
+// gathered column (two gathers per entry) and written for the own rows.
+template <class V, bool NT>
+__global__ void __launch_bounds__(kCgBlock) k_cg_dir_spmv(CGDev<V> a, int k, int G, const V *__restrict__ p_in,
+                                                        V *__restrict__ p_out, int is_ru, double delta2) {
+    if (a.st->done) return;
+    V beta;
+    if (!cg_beta(a, k, G, is_ru, delta2, beta)) return;
+    double v[1] = {0.0};
+    const int lane = threadIdx.x & 63;
+    const SliceWalk sw = slice_walk(a.nslices);
+    for (int sl = sw.first; sl < sw.end; sl += sw.step) {
+        const int row = sl * 64 + lane;
+        const int e0 = a.slice_ptr[sl] + lane;
+        const int w = (a.slice_ptr[sl + 1] - a.slice_ptr[sl]) >> 6;
+        V acc = V(0);
+        int j = 0;
+        for (; j + 4 <= w; j += 4) {
+            const int i0 = e0 + j * 64;
+            const int c0 = ld_mat<NT>(a.col + i0), c1 = ld_mat<NT>(a.col + i0 + 64);
+            const int c2 = ld_mat<NT>(a.col + i0 + 128), c3 = ld_mat<NT>(a.col + i0 + 192);
+            const V v0 = ld_mat<NT>(a.val + i0), v1 = ld_mat<NT>(a.val + i0 + 64);
+            const V v2 = ld_mat<NT>(a.val + i0 + 128), v3 = ld_mat<NT>(a.val + i0 + 192);
+            const V g0 = a.z[c0] + beta * p_in[c0], g1 = a.z[c1] + beta * p_in[c1];
+            const V g2 = a.z[c2] + beta * p_in[c2], g3 = a.z[c3] + beta * p_in[c3];
+            acc += v0 * g0;
+            acc += v1 * g1;
+            acc += v2 * g2;
+            acc += v3 * g3;
+        }
+        for (; j < w; j++) {
+            const int i0 = e0 + j * 64;
+            const int c = ld_mat<NT>(a.col + i0);
+            acc += ld_mat<NT>(a.val + i0) * (a.z[c] + beta * p_in[c]);
+        }
+        if (row < a.n) {
+            const V p = a.z[row] + beta * p_in[row];
+            p_out[row] = p;
+            a.q[row] = acc;
+            v[0] += (double)p * (double)acc;
         }
     }
     double *const dst[1] = {a.part_pq};

@@ -71,6 +71,9 @@ class DarcyGPU {
    public:
     explicit DarcyGPU(const HostMesh &m);
     ~DarcyGPU();
+    // owns a stream, events and device buffers
+    DarcyGPU(const DarcyGPU &) = delete;
+    DarcyGPU &operator=(const DarcyGPU &) = delete;
     // kErrBadField unless every value of v[E] (host or device) is finite and > 0
     void check_field(const char *what, const double *v, bool device) const;
     // kErrSingularElement if an element's local matrix was singular (resets the flag)
@@ -141,9 +144,9 @@ class DarcyGPU {
     double *d_sol_ = nullptr;
     double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
-    CGState *h_state_ = nullptr;  // pinned host copy of the CG state, read after every chunk of iterations
     int *d_flag_ = nullptr;
     int cg_grid_ = kCgGridLarge;  // set from the system size with the pattern
+    bool cg_split_ = true;        // direction pass + SpMV with non-temporal matrix loads (large systems)
     int stop_norm_ = 0;
     int device_ = 0;
     // CG partial sums and state of the current solve (null: the handle's own)

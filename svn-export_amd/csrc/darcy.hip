@@ -280,16 +280,12 @@ DarcyGPU::DarcyGPU(const HostMesh &m) : m_(m) {
     FL_HIP(hipGetDevice(&device_));
     FL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     build_pattern();
-    // pinned, so the per-chunk state read is a plain copy (a pageable target
-    // is staged through a copy kernel of its own)
-    FL_HIP(hipHostMalloc((void **)&h_state_, sizeof(CGState), hipHostMallocDefault));
 }
 
 DarcyGPU::~DarcyGPU() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     for (void *p : owned_) dev_free(p);
-    if (h_state_) (void)hipHostFree(h_state_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -391,7 +387,8 @@ void DarcyGPU::build_pattern() {
         for (int i = 0; i < F; i++) rinv[rperm[i]] = i;
     }
     nslices_ = (F + 63) / 64;
-    cg_grid_ = F >= kCgLargeRows ? kCgGridLarge : kCgGridSmall;
+    cg_grid_ = F >= kCgLargeRows ? kCgGridLarge : (F >= kCgTinyRows ? kCgGridSmall : kCgGridTiny);
+    cg_split_ = F >= kCgLargeRows;
     std::vector<int> slice_ptr(nslices_ + 1, 0);
     for (int sl = 0; sl < nslices_; sl++) {
         int w = 0;
@@ -705,9 +702,16 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
                 e0 = next_ev(); next_ev(); next_ev();
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
-            hipLaunchKernelGGL(k_cg_dir<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
-                               (const double *)pin, pout, itx == 0 ? 1 : 0, 0.0);
-            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, (const double *)pout);
+            const int ru = itx == 0 ? 1 : 0;
+            if (cg_split_) {
+                hipLaunchKernelGGL(k_cg_dir<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
+                                   (const double *)pin, pout, ru, 0.0);
+                hipLaunchKernelGGL((k_cg_spmv<double, true>), dim3(G), dim3(kCgBlock), 0, stream_, a,
+                                   (const double *)pout);
+            } else {
+                hipLaunchKernelGGL((k_cg_dir_spmv<double, false>), dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
+                                   (const double *)pin, pout, ru, 0.0);
+            }
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<double>, dim3(G), dim3(kCgBlock), 0, stream_, a, itx, G,
                                (const double *)pout);
@@ -719,9 +723,8 @@ int DarcyGPU::cg(double rtol, double atol, int max_it, SolveStats &st, bool warm
         }
         launched += chunk;
         FL_HIP(hipGetLastError());
-        FL_HIP(hipMemcpyAsync(h_state_, state, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(&h, state, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
-        h = *h_state_;
         if (h.done || launched > max_it + chunk) break;
         if (chunk < 256) chunk *= 2;
     }
@@ -808,9 +811,16 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
                 e0 = next_ev(); next_ev(); next_ev();
                 FL_HIP(hipEventRecord(ev_pool_[e0], stream_));
             }
-            hipLaunchKernelGGL(k_cg_dir<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pin,
-                               pout, k == last_ru ? 1 : 0, delta2);
-            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, (const float *)pout);
+            const int ru = k == last_ru ? 1 : 0;
+            if (cg_split_) {
+                hipLaunchKernelGGL(k_cg_dir<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G,
+                                   (const float *)pin, pout, ru, delta2);
+                hipLaunchKernelGGL((k_cg_spmv<float, true>), dim3(G), dim3(kCgBlock), 0, stream_, a,
+                                   (const float *)pout);
+            } else {
+                hipLaunchKernelGGL((k_cg_dir_spmv<float, false>), dim3(G), dim3(kCgBlock), 0, stream_, a, k, G,
+                                   (const float *)pin, pout, ru, delta2);
+            }
             if (samp) FL_HIP(hipEventRecord(ev_pool_[e0 + 1], stream_));
             hipLaunchKernelGGL(k_cg_update<float>, dim3(G), dim3(kCgBlock), 0, stream_, a, k, G, (const float *)pout);
             if (samp) {
@@ -819,9 +829,8 @@ int DarcyGPU::cg_mixed(double rtol, double atol, int max_it, int ru_every, Solve
             }
         }
         FL_HIP(hipGetLastError());
-        FL_HIP(hipMemcpyAsync(h_state_, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        FL_HIP(hipMemcpyAsync(&h, d_state_, sizeof(CGState), hipMemcpyDeviceToHost, stream_));
         FL_HIP(hipStreamSynchronize(stream_));
-        h = *h_state_;
         if (h.done) {
             // converged on a true FP64 residual, or out of iterations
             if ((h.done == 1 && h.it == last_ru) || h.it >= max_it || n_ru >= kMaxUpdates) break;
