@@ -8,7 +8,8 @@
 // contiguous 512 B (FP64) segment.  Padding entries have value 0 and point
 // at their own row.
 //
-// Three kernels per iteration, no atomics and no device-scope fences: every
+// Three kernels per iteration (two below kCgLargeRows rows, see below), no
+// atomics and no device-scope fences: every
 // block writes its partial dot products to a per-block array, and every block
 // of a later kernel folds the whole (small) partial array itself in a fixed
 // order.  All blocks therefore see bitwise identical scalars (alpha, beta, the
