@@ -191,6 +191,39 @@ def test_cpp_adapter_example(tmp_path):
         assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-9 * np.max(np.abs(ref))), name
 
 
+def test_darcy_split_cg_million_rows():
+    """Edge systems of a million rows and more run the CG iteration in its split
+    form (a direction pass, then the SpMV with non-temporal matrix loads over
+    XCD-contiguous slices), smaller ones the fused kernel that every other test
+    here runs: a 45^3 x 6 cube with two fractures (1.13 M edge unknowns)
+    against the oracle in FP64 and in FP32 + FP64 reliable updates, and the
+    FP64 solve repeated bit for bit."""
+    import meshgen
+    mesh = meshgen.cube_mesh(45, n_fractures=2, n_channels=0, seed=1)
+    om, gm = O.Mesh(arrays=mesh), G.Mesh(arrays=mesh)
+    assert gm.n_edges >= 1 << 20  # the split form's threshold (kCgLargeRows)
+    d = cases.darcy_case("cube", om, seed=7)
+    O.set_threads(min(16, len(os.sched_getaffinity(0))))  # the oracle's CG, at most 16 host threads
+    try:
+        D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"],
+                     d["bc_sigma"])
+        D0.solve(0, rtol=1e-14, max_it=200000)
+    finally:
+        O.set_threads(1)
+    g = G.DarcyFlowMH_Steady(gm)
+    g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    i64 = g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=200000)
+    print("F", gm.n_edges, "iterations gpu", i64["iterations"], "oracle", D0.stats["iters"], flush=True)
+    assert i64["converged"] == 1
+    x1 = check_solution(D0, g, tol=1e-9)
+    g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=200000)
+    assert np.array_equal(g.get_solution_vector(), x1)
+    i32 = g.update_solution(r_tol=1e-12, a_tol=0.0, max_it=200000, precision=1)
+    print("mixed iterations", i32["iterations"], "reliable updates", i32["reliable_updates"], flush=True)
+    assert i32["converged"] == 1
+    check_solution(D0, g, tol=1e-8)
+
+
 @pytest.mark.parametrize("case,mesh", REF_CASES)
 def test_darcy_mixed_precision_reference_meshes(case, mesh):
     """FP32 CG with FP64 reliable updates: the stop test is on the true FP64
