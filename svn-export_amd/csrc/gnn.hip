@@ -84,11 +84,24 @@ __global__ void __launch_bounds__(256) k_g_fwd_mean(int n, const bf16 *__restric
                                                     const int *__restrict__ ptr, const int *__restrict__ idx,
                                                     const bf16 *__restrict__ S, const bf16 *__restrict__ W2t,
                                                     const float *__restrict__ bias, bf16 *__restrict__ Y) {
-    __shared__ __attribute__((aligned(16))) bf16 tile[4][16][H + 8];
+    // both weight matrices staged once per block (rows padded against bank
+    // conflicts); the output tile reuses the same LDS after the products
+    constexpr int WS = H + 8;
+    __shared__ __attribute__((aligned(16))) bf16 lds[2 * H * WS];
+    bf16 (*wl)[H][WS] = reinterpret_cast<bf16 (*)[H][WS]>(lds);
+    bf16 (*tile)[16][WS] = reinterpret_cast<bf16 (*)[16][WS]>(lds);
+    static_assert(4 * 16 <= 2 * H, "output tile fits the weight staging");
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = l & 15, kg = l >> 4;
     const size_t rb = (size_t)blockIdx.x * 64 + w * 16;
     const int row = (int)rb + r;
+    // weight loads first: 2 x 64 rows x 8 segments of 16 bytes, 4 per thread
+    uint4 wv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = threadIdx.x + 256 * j, m = q >> 9, wr = (q >> 3) & 63, seg = q & 7;
+        wv[j] = *(const uint4 *)((m ? W2t : W1t) + (size_t)wr * H + seg * 8);
+    }
     int a0 = 0, a1 = 0;
     if (row < n) {
         a0 = ptr[row];
@@ -106,6 +119,12 @@ __global__ void __launch_bounds__(256) k_g_fwd_mean(int n, const bf16 *__restric
             for (int c = 0; c < 4; c++) m[4 * q + c] += bf2f((bf16)v[c]);
         }
     }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = threadIdx.x + 256 * j, mm = q >> 9, wr = (q >> 3) & 63, seg = q & 7;
+        *(uint4 *)&wl[mm][wr][seg * 8] = wv[j];
+    }
+    __syncthreads();
     f4 acc[4];
     for (int t = 0; t < 4; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -113,7 +132,7 @@ __global__ void __launch_bounds__(256) k_g_fwd_mean(int n, const bf16 *__restric
         const s4 a = *(const s4 *)(X1 + (rb + r) * H + k0 + kg * 4);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const s4 b = *(const s4 *)(W1t + (size_t)(t * 16 + r) * H + k0 + kg * 4);
+            const s4 b = *(const s4 *)&wl[0][t * 16 + r][k0 + kg * 4];
             acc[t] = mfma(a, b, acc[t]);
         }
     }
@@ -125,10 +144,11 @@ __global__ void __launch_bounds__(256) k_g_fwd_mean(int n, const bf16 *__restric
         for (int c = 0; c < 4; c++) a[c] = (short)f2bf(cnt > 0 ? m[4 * q + c] / (float)cnt : 0.f);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const s4 b = *(const s4 *)(W2t + (size_t)(t * 16 + r) * H + 16 * q + kg * 4);
+            const s4 b = *(const s4 *)&wl[1][t * 16 + r][16 * q + kg * 4];
             acc[t] = mfma(a, b, acc[t]);
         }
     }
+    __syncthreads();  // every wave is done with the weights: the LDS becomes the output tile
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int col = t * 16 + r;
