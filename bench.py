@@ -433,6 +433,7 @@ def main():
             "compare_batch_forms": compare,
             "iterations": {"mean": float(it.mean()), "min": int(it.min()), "max": int(it.max())},
             "converged": int(info["n_converged"]), "handovers": int(info["handovers"]),
+            "coarse_setup_s": round(info["ms_coarse_setup"] / 1e3, 4),
             "operator_tile_rows": int(info["tile_rows"]),
             "phases_s": {k: round(info[m] * 1e-3, 3) for k, m in (
                 ("queue_empty", "ms_queue_empty"), ("narrowed_16", "ms_narrow16"), ("narrowed_4", "ms_narrow4"),

@@ -199,6 +199,10 @@ class DarcyBatchGPU {
     std::vector<int> h_nbptr_, h_nbv_, h_lbmap_;  // neighbour aggregates (host copies)
     int *d_tptr_ = nullptr, *d_tagg_ = nullptr;    // coarse-block term lists: offsets per block, block aggregate
     int2 *d_terms_ = nullptr;                      // (row, e << 8 | i << 4 | j) per term
+    int *d_cbeg_ = nullptr, *d_cblk_ = nullptr;    // term-list chunks: first term, block
+    int *d_cfirst_ = nullptr;                      // first chunk of each block (+ end)
+    int nasm_chunks_ = 0;
+    double *d_Apart_ = nullptr;                    // chunk partial sums [chunk][ncol^2][Rp]
     int *d_agg_ = nullptr, *d_aptr_ = nullptr, *d_nbptr_ = nullptr, *d_nb_ = nullptr, *d_lbmap_ = nullptr;
     float *d_phiAll_ = nullptr, *d_ainv_ = nullptr, *d_spainv_ = nullptr;
     unsigned short *d_phi_ = nullptr, *d_spphi_ = nullptr;  // phi per slot, bfloat16 [F][Rp]

@@ -1036,26 +1036,32 @@ __device__ __forceinline__ double elem_entry(const double *__restrict__ T, int n
 
 // Coarse operator Z^T S Z of the selected slots, as per-aggregate blocks
 // Asp[((a * kNbMax + lb) * ncol^2 + s * ncol + t) * Rp + slot] (lb: local
-// index of the neighbour aggregate b).  One wave per (block (a, b), 64
-// slots): the block is the sum, over the precomputed terms (row of a, element
-// contribution (e, i) of the row, local column j of e whose edge lies in b),
-// of z_s(row) S_e(K)_ij z_t(edge j), taken in row / contribution / column
-// order, accumulated in registers.
+// index of the neighbour aggregate b).  Each block is the sum, over the
+// precomputed terms (row of a, element contribution (e, i) of the row, local
+// column j of e whose edge lies in b), of z_s(row) S_e(K)_ij z_t(edge j), in
+// row / contribution / column order.  The term lists are cut into chunks of
+// at most kAsmChunk terms, one wave per (chunk, 64 slots) accumulating its
+// chunk in registers (a block of the diagonal holds thousands of terms, and
+// one wave walking them all set the kernel's length); k_c_assemble_sum adds
+// the chunks of each block in chunk order.
+constexpr int kAsmChunk = 256;
+
 template <int NCOL>
 __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, const int *__restrict__ lsel,
+                                                   const int *__restrict__ cbeg, const int *__restrict__ cblk,
                                                    const int *__restrict__ tptr, const int2 *__restrict__ terms,
-                                                   const int *__restrict__ tagg, double *__restrict__ Asp) {
+                                                   double *__restrict__ Apart) {
     const int lane = threadIdx.x;
     const int slot = blockIdx.y * 64 + lane;
-    const int tg = blockIdx.x;  // global neighbour slot: aggregate tagg[tg], local index tg - nbptr[a]
+    const int ch = blockIdx.x;
     const bool act = lsel[slot] != 0;
     if (!__any(act)) return;
-    const int ag = tagg[tg], lb = tg - c.nbptr[ag];
+    const int k0 = cbeg[ch], k1 = min(k0 + kAsmChunk, tptr[cblk[ch] + 1]);
     double acc[NCOL * NCOL];
 #pragma unroll
     for (int q = 0; q < NCOL * NCOL; q++) acc[q] = 0.0;
 #pragma unroll 4
-    for (int k = tptr[tg]; k < tptr[tg + 1]; k++) {
+    for (int k = k0; k < k1; k++) {
         const int2 u = terms[k];
         const int row = u.x, e = u.y >> 8, i = (u.y >> 4) & 15, j = u.y & 15;
         const int mt = a.meta[e];
@@ -1077,7 +1083,25 @@ __global__ void __launch_bounds__(64) k_c_assemble(BArgs<double> a, CArgs c, con
     }
     if (!act) return;
 #pragma unroll
-    for (int q = 0; q < NCOL * NCOL; q++) Asp[((size_t)(ag * kNbMax + lb) * (NCOL * NCOL) + q) * a.Rp + slot] = acc[q];
+    for (int q = 0; q < NCOL * NCOL; q++) Apart[((size_t)ch * (NCOL * NCOL) + q) * a.Rp + slot] = acc[q];
+}
+
+// Asp of block tg = the sum of its chunks cfirst[tg] .. cfirst[tg + 1] - 1 in order
+template <int NCOL>
+__global__ void __launch_bounds__(64) k_c_assemble_sum(CArgs c, size_t Rp, const int *__restrict__ lsel,
+                                                       const int *__restrict__ cfirst, const int *__restrict__ tagg,
+                                                       const double *__restrict__ Apart, double *__restrict__ Asp) {
+    const int slot = blockIdx.y * 64 + threadIdx.x;
+    if (lsel[slot] == 0) return;
+    const int tg = blockIdx.x, ag = tagg[tg], lb = tg - c.nbptr[ag];
+    double acc[NCOL * NCOL];
+#pragma unroll
+    for (int q = 0; q < NCOL * NCOL; q++) acc[q] = 0.0;
+    for (int ch = cfirst[tg]; ch < cfirst[tg + 1]; ch++)
+#pragma unroll
+        for (int q = 0; q < NCOL * NCOL; q++) acc[q] += Apart[((size_t)ch * (NCOL * NCOL) + q) * Rp + slot];
+#pragma unroll
+    for (int q = 0; q < NCOL * NCOL; q++) Asp[((size_t)(ag * kNbMax + lb) * (NCOL * NCOL) + q) * Rp + slot] = acc[q];
 }
 
 // Dense coarse matrices of the listed slots, packed [np][ncp][ncp] (ncp = 64
@@ -2424,6 +2448,20 @@ void DarcyBatchGPU::build() {
                     }
                 }
             if (pass == 1) {
+                // chunks of at most kAsmChunk terms, in block order
+                std::vector<int> cbeg, cblk, cfirst(ntg + 1);
+                for (int t = 0; t < ntg; t++) {
+                    cfirst[t] = (int)cbeg.size();
+                    for (int b = cnt[t]; b < cnt[t + 1]; b += kAsmChunk) {
+                        cbeg.push_back(b);
+                        cblk.push_back(t);
+                    }
+                }
+                cfirst[ntg] = (int)cbeg.size();
+                nasm_chunks_ = (int)cbeg.size();
+                d_cbeg_ = b_up(owned_, cbeg, stream_);
+                d_cblk_ = b_up(owned_, cblk, stream_);
+                d_cfirst_ = b_up(owned_, cfirst, stream_);
                 d_tptr_ = b_up(owned_, cnt, stream_);
                 d_tagg_ = b_up(owned_, tagg, stream_);
                 int2 *dt = dev_alloc<int2>(terms.size());
@@ -2675,6 +2713,7 @@ void DarcyBatchGPU::set_coarse(int ncol) {
         d_y_ = ad((size_t)ncpmax * Rp_);
         d_spy_ = ad((size_t)ncpmax * Rp_);
         d_Asp_ = ad((size_t)nagg_ * kNbMax * 4 * Rp_);
+        d_Apart_ = ad((size_t)std::max(nasm_chunks_, 1) * 4 * Rp_);
         d_ainv_ = af((size_t)Rp_ * tiles);
         d_spainv_ = af((size_t)Rp_ * tiles);
         d_Apack_ = ad((size_t)Rp_ * ncpmax * ncpmax);
@@ -2725,14 +2764,22 @@ void DarcyBatchGPU::coarse_setup(const std::vector<int> &pairs) {
                                         d_conn_off_, d_conn_, d_tab_, d_K_, d_tab_, d_K_, d_dinv_, d_x_, d_r_, d_z_,
                                         d_p0_, d_p1_, d_q_, d_part_, sc_);
     if (ncol_ == 1) {
-        hipLaunchKernelGGL(k_c_assemble<1>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
-                           (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
+        if (nasm_chunks_ > 0)
+            hipLaunchKernelGGL(k_c_assemble<1>, dim3(nasm_chunks_, nch_), dim3(64), 0, stream_, a, c,
+                               (const int *)d_lsel_, (const int *)d_cbeg_, (const int *)d_cblk_, (const int *)d_tptr_,
+                               (const int2 *)d_terms_, d_Apart_);
+        hipLaunchKernelGGL(k_c_assemble_sum<1>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, c, (size_t)Rp_,
+                           (const int *)d_lsel_, (const int *)d_cfirst_, (const int *)d_tagg_, (const double *)d_Apart_,
                            d_Asp_);
         hipLaunchKernelGGL(k_c_pack<1>, dim3(nt_ * coarse::kT, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
     } else {
-        hipLaunchKernelGGL(k_c_assemble<2>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, a, c,
-                           (const int *)d_lsel_, (const int *)d_tptr_, (const int2 *)d_terms_, (const int *)d_tagg_,
+        if (nasm_chunks_ > 0)
+            hipLaunchKernelGGL(k_c_assemble<2>, dim3(nasm_chunks_, nch_), dim3(64), 0, stream_, a, c,
+                               (const int *)d_lsel_, (const int *)d_cbeg_, (const int *)d_cblk_, (const int *)d_tptr_,
+                               (const int2 *)d_terms_, d_Apart_);
+        hipLaunchKernelGGL(k_c_assemble_sum<2>, dim3((int)h_nbv_.size(), nch_), dim3(64), 0, stream_, c, (size_t)Rp_,
+                           (const int *)d_lsel_, (const int *)d_cfirst_, (const int *)d_tagg_, (const double *)d_Apart_,
                            d_Asp_);
         hipLaunchKernelGGL(k_c_pack<2>, dim3(nt_ * coarse::kT, np), blk, 0, stream_, c, (const double *)d_Asp_, (size_t)Rp_,
                            (const int *)d_pairs_, d_Apack_);
