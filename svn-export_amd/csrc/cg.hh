@@ -8,8 +8,8 @@
 // contiguous 512 B (FP64) segment.  Padding entries have value 0 and point
 // at their own row.
 //
-// Three kernels per iteration (two below kCgLargeRows rows, see below), no
-// atomics and no device-scope fences: every
+// Three kernels per iteration (two for systems that stay in the last-level
+// cache, see below), no atomics and no device-scope fences: every
 // block writes its partial dot products to a per-block array, and every block
 // of a later kernel folds the whole (small) partial array itself in a fixed
 // order.  All blocks therefore see bitwise identical scalars (alpha, beta, the
@@ -18,25 +18,28 @@
 //   K_A(k): q = A p; partial p.q
 //   K_B(k): alpha = rz_k / p.q; x += alpha p; r -= alpha q; z = D^-1 r;
 //           partial r.z, r.r into the (k+1) parity buffers
-// Systems of kCgLargeRows rows and more: the direction is formed by its own
-// streaming pass, so the operator gathers one vector per entry (forming
+// Systems whose matrix and vectors exceed the last-level cache (kCgSplitBytes
+// of 12 B per stored entry and 56 B per row): the direction is formed by its
+// own streaming pass, so the operator gathers one vector per entry (forming
 // z + beta p_old at every gathered column instead gathers two), and the
-// matrix is read with non-temporal loads, so the vectors (17 MB per million
-// rows each) stay in the caches between the kernels.  Smaller systems, whose
-// matrix and vectors stay in the last-level cache and where a launch costs
-// more than the second gather, run K_D and K_A as one kernel (k_cg_dir_spmv)
-// with ordinary loads.  Both forms give bitwise the same iterates.
+// matrix is read with non-temporal loads, so the vectors stay in the caches
+// between the kernels.  Smaller systems stay in the cache from one iteration
+// to the next; a launch costs more there than the second gather, so they run
+// K_D and K_A as one kernel (k_cg_dir_spmv) with ordinary loads.  Both forms
+// give bitwise the same iterates.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace fl {
 
 constexpr int kCgBlock = 512;
-// grid of the iteration kernels: kCgGridLarge blocks from kCgLargeRows rows
-// on, kCgGridSmall below (one wave of resident blocks on 256 CUs, and half
-// the partial sums to fold), kCgGridTiny below kCgTinyRows
-constexpr int kCgGridSmall = 512, kCgGridLarge = 1024, kCgLargeRows = 1 << 20;
+// grid of the iteration kernels: kCgGridLarge blocks in the split form,
+// kCgGridSmall in the fused one (one wave of resident blocks on 256 CUs, and
+// half the partial sums to fold), kCgGridTiny below kCgTinyRows rows
+constexpr int kCgGridSmall = 512, kCgGridLarge = 1024;
 constexpr int kCgGridTiny = 128, kCgTinyRows = 1 << 17;
+// working set above which the split form runs (MI355X: 256 MB Infinity Cache)
+constexpr double kCgSplitBytes = 224.0 * (1 << 20);
 
 // done: 0 running, 1 stop test met, 2 paused for a reliable update (mixed precision)
 // norm: 0 the stop test is on ||r||_2 / ||b||_2; 1 on the natural norm of the

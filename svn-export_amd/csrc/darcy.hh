@@ -145,8 +145,8 @@ class DarcyGPU {
     double *d_part_ = nullptr;  // 5 partial arrays of cg_grid_ doubles
     CGState *d_state_ = nullptr;
     int *d_flag_ = nullptr;
-    int cg_grid_ = kCgGridLarge;  // set from the system size with the pattern
-    bool cg_split_ = true;        // direction pass + SpMV with non-temporal matrix loads (large systems)
+    int cg_grid_ = kCgGridLarge;  // set from the system's working set with the pattern
+    bool cg_split_ = true;        // direction pass + SpMV with non-temporal matrix loads (beyond the cache)
     int stop_norm_ = 0;
     int device_ = 0;
     // CG partial sums and state of the current solve (null: the handle's own)

@@ -387,8 +387,6 @@ void DarcyGPU::build_pattern() {
         for (int i = 0; i < F; i++) rinv[rperm[i]] = i;
     }
     nslices_ = (F + 63) / 64;
-    cg_grid_ = F >= kCgLargeRows ? kCgGridLarge : (F >= kCgTinyRows ? kCgGridSmall : kCgGridTiny);
-    cg_split_ = F >= kCgLargeRows;
     std::vector<int> slice_ptr(nslices_ + 1, 0);
     for (int sl = 0; sl < nslices_; sl++) {
         int w = 0;
@@ -399,6 +397,10 @@ void DarcyGPU::build_pattern() {
         slice_ptr[sl + 1] = slice_ptr[sl] + 64 * w;
     }
     const int nslots = slice_ptr[nslices_];
+    // an iteration's working set: the stored matrix entries (12 B) and the CG
+    // vectors (56 B per row) against the last-level cache
+    cg_split_ = 12.0 * nslots + 56.0 * F > kCgSplitBytes;
+    cg_grid_ = cg_split_ ? kCgGridLarge : (F >= kCgTinyRows ? kCgGridSmall : kCgGridTiny);
     std::vector<int> sell_col(nslots), csr2slot(nnz_), slot_csr(nslots, -1);
     for (int sl = 0; sl < nslices_; sl++) {
         int w = (slice_ptr[sl + 1] - slice_ptr[sl]) / 64;

@@ -191,29 +191,54 @@ def test_cpp_adapter_example(tmp_path):
         assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-9 * np.max(np.abs(ref))), name
 
 
-def test_darcy_split_cg_million_rows():
-    """Edge systems of a million rows and more run the CG iteration in its split
-    form (a direction pass, then the SpMV with non-temporal matrix loads over
-    XCD-contiguous slices), smaller ones the fused kernel that every other test
-    here runs: a 45^3 x 6 cube with two fractures (1.13 M edge unknowns)
-    against the oracle in FP64 and in FP32 + FP64 reliable updates, and the
-    FP64 solve repeated bit for bit."""
+
+class _Solved:
+    """The oracle solution of a solve made elsewhere, as check_solution reads it."""
+
+    def __init__(self, solution, iters, S, E):
+        self.solution, self.S, self.E, self.stats = solution, S, E, {"iters": iters}
+
+
+def _oracle_cube54(out):
     import meshgen
-    mesh = meshgen.cube_mesh(45, n_fractures=2, n_channels=0, seed=1)
-    om, gm = O.Mesh(arrays=mesh), G.Mesh(arrays=mesh)
-    assert gm.n_edges >= 1 << 20  # the split form's threshold (kCgLargeRows)
+    mesh = meshgen.cube_mesh(54, n_fractures=2, n_channels=0, seed=1)
+    om = O.Mesh(arrays=mesh)
     d = cases.darcy_case("cube", om, seed=7)
     O.set_threads(min(16, len(os.sched_getaffinity(0))))  # the oracle's CG, at most 16 host threads
-    try:
-        D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"],
-                     d["bc_sigma"])
-        D0.solve(0, rtol=1e-14, max_it=200000)
-    finally:
-        O.set_threads(1)
+    D0 = O.Darcy(om, d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    D0.solve(0, rtol=1e-14, max_it=200000)
+    np.savez(out, solution=D0.solution, iters=D0.stats["iters"])
+
+
+def test_darcy_split_cg_beyond_cache():
+    """Edge systems whose matrix and vectors exceed the last-level cache run the
+    CG iteration in its split form (a direction pass, then the SpMV with
+    non-temporal matrix loads over XCD-contiguous slices), smaller ones the
+    fused kernel that every other test here runs: a 54^3 x 6 cube with two
+    fractures (1.9 M edge unknowns) against the oracle in FP64 and in FP32 +
+    FP64 reliable updates, and the FP64 solve repeated bit for bit."""
+    import meshgen
+    mesh = meshgen.cube_mesh(54, n_fractures=2, n_channels=0, seed=1)
+    om, gm = O.Mesh(arrays=mesh), G.Mesh(arrays=mesh)
+    d = cases.darcy_case("cube", om, seed=7)
+    # the oracle's 1.9 M-unknown solve (multi-threaded) runs in a process of its
+    # own, so nothing of it stays in this one for the tests that follow
+    import multiprocessing as mp
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(), "oracle.npz")
+    p = mp.get_context("spawn").Process(target=_oracle_cube54, args=(out,))
+    p.start()
+    p.join(600)
+    assert p.exitcode == 0, "oracle process failed"
+    ref = np.load(out)
+    D0 = _Solved(ref["solution"], int(ref["iters"]), om.n_sides, om.n_elements)
     g = G.DarcyFlowMH_Steady(gm)
     g.set_data(d["K"], d["cs"], d["sigma"], d["f"], d["bc_type"], d["bc_p"], d["bc_flux"], d["bc_sigma"])
+    nnz = G.lib().fl_darcy_edge_nnz(g.h)
+    # the split form's condition (stored entries >= nnz): 12 B per entry + 56 B per row > 224 MiB
+    assert 12.0 * nnz + 56.0 * gm.n_edges > 224.0 * (1 << 20)
     i64 = g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=200000)
-    print("F", gm.n_edges, "iterations gpu", i64["iterations"], "oracle", D0.stats["iters"], flush=True)
+    print("F", gm.n_edges, "nnz", nnz, "iterations gpu", i64["iterations"], "oracle", D0.stats["iters"], flush=True)
     assert i64["converged"] == 1
     x1 = check_solution(D0, g, tol=1e-9)
     g.update_solution(r_tol=1e-14, a_tol=0.0, max_it=200000)
